@@ -1,0 +1,1106 @@
+/*
+ * gf_oracle.c -- plain-C restatement of the reference GeoClaw Fortran step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see gf_oracle.h).  Every routine follows the
+ * reference Fortran statement by statement, in the same floating-point
+ * operation order, so that a build with -ffp-contract=off reproduces the
+ * reference bit for bit.  Fortran intrinsics are restated with the semantics
+ * LLVM Flang gives them (checked against the reference build):
+ *   MAX(a,b)  -> a > b ? a : b        MIN(a,b) -> a < b ? a : b
+ *   DMAX1/DMIN1 with three arguments fold left to right
+ *   SIGN(a,b) -> copysign(|a|, b)     x**2 -> x*x      x**(7/3) -> pow()
+ * Unary minus in Fortran applies to a whole product (-a*b = -(a*b)); IEEE
+ * negation commutes with rounding, so C's (-a)*b is bitwise identical.
+ */
+#include "gf_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MEQN 3
+#define MWAVES 3
+
+static inline double FMAX(double a, double b) { return a > b ? a : b; }
+static inline double FMIN(double a, double b) { return a < b ? a : b; }
+
+/* 1-D slice helpers: Fortran index i in [1-mbc, n+mbc] -> 0-based k = i+mbc-1 */
+#define K(i) ((i) + mbc - 1)
+#define Q1(m, i) q1d[K(i) * MEQN + (m) - 1]
+#define AX(a, i) (a)[K(i) * maux]
+#define FW(m, mw, i) fwave[(K(i) * MWAVES + (mw) - 1) * MEQN + (m) - 1]
+#define SS(mw, i) s[K(i) * MWAVES + (mw) - 1]
+#define V3(a, m, i) (a)[K(i) * MEQN + (m) - 1]
+
+/* ------------------------------------------------------------------------- */
+/* riemanntype: geoclaw_riemann_utils.f:566-677                              */
+/* ------------------------------------------------------------------------- */
+void gfo_riemanntype(double hL, double hR, double uL, double uR, int maxiter,
+                     double drytol, double g, double out[3], int rare[2])
+{
+    double hm, s1m, s2m, u1m, u2m, um, delu;
+    double h_max, h_min, h0, F_max, F_min, dfdh, F0, slope, gL, gR;
+    int rare1, rare2;
+
+    h_min = FMIN(hR, hL);
+    h_max = FMAX(hR, hL);
+    delu = uR - uL;
+
+    if (h_min <= drytol) {
+        hm = 0.0;
+        um = 0.0;
+        s1m = uR + uL - 2.0 * sqrt(g * hR) + 2.0 * sqrt(g * hL);
+        s2m = uR + uL - 2.0 * sqrt(g * hR) + 2.0 * sqrt(g * hL);
+        if (hL <= 0.0) {
+            rare2 = 1;
+            rare1 = 0;
+        } else {
+            rare1 = 1;
+            rare2 = 0;
+        }
+    } else {
+        F_min = delu + 2.0 * (sqrt(g * h_min) - sqrt(g * h_max));
+        F_max = delu + (h_max - h_min) * (sqrt(.5 * g * (h_max + h_min) / (h_max * h_min)));
+
+        if (F_min > 0.0) { /* 2-rarefactions */
+            double t = FMAX(0.0, -delu + 2.0 * (sqrt(g * hL) + sqrt(g * hR)));
+            hm = (1.0 / (16.0 * g)) * (t * t);
+            um = copysign(1.0, hm) * (uL + 2.0 * (sqrt(g * hL) - sqrt(g * hm)));
+            s1m = uL + 2.0 * sqrt(g * hL) - 3.0 * sqrt(g * hm);
+            s2m = uR - 2.0 * sqrt(g * hR) + 3.0 * sqrt(g * hm);
+            rare1 = 1;
+            rare2 = 1;
+        } else if (F_max <= 0.0) { /* 2 shocks */
+            h0 = h_max;
+            for (int iter = 1; iter <= maxiter; iter++) {
+                gL = sqrt(.5 * g * (1 / h0 + 1 / hL));
+                gR = sqrt(.5 * g * (1 / h0 + 1 / hR));
+                F0 = delu + (h0 - hL) * gL + (h0 - hR) * gR;
+                dfdh = gL - g * (h0 - hL) / (4.0 * (h0 * h0) * gL) + gR -
+                       g * (h0 - hR) / (4.0 * (h0 * h0) * gR);
+                slope = 2.0 * sqrt(h0) * dfdh;
+                double t = sqrt(h0) - F0 / slope;
+                h0 = t * t;
+            }
+            hm = h0;
+            u1m = uL - (hm - hL) * sqrt((.5 * g) * (1 / hm + 1 / hL));
+            u2m = uR + (hm - hR) * sqrt((.5 * g) * (1 / hm + 1 / hR));
+            um = .5 * (u1m + u2m);
+            s1m = u1m - sqrt(g * hm);
+            s2m = u2m + sqrt(g * hm);
+            rare1 = 0;
+            rare2 = 0;
+        } else { /* one shock one rarefaction */
+            h0 = h_min;
+            for (int iter = 1; iter <= maxiter; iter++) {
+                F0 = delu + 2.0 * (sqrt(g * h0) - sqrt(g * h_max)) +
+                     (h0 - h_min) * sqrt(.5 * g * (1 / h0 + 1 / h_min));
+                slope = (F_max - F0) / (h_max - h_min);
+                h0 = h0 - F0 / slope;
+            }
+            hm = h0;
+            if (hL > hR) {
+                um = uL + 2.0 * sqrt(g * hL) - 2.0 * sqrt(g * hm);
+                s1m = uL + 2.0 * sqrt(g * hL) - 3.0 * sqrt(g * hm);
+                s2m = uL + 2.0 * sqrt(g * hL) - sqrt(g * hm);
+                rare1 = 1;
+                rare2 = 0;
+            } else {
+                s2m = uR - 2.0 * sqrt(g * hR) + 3.0 * sqrt(g * hm);
+                s1m = uR - 2.0 * sqrt(g * hR) + sqrt(g * hm);
+                um = uR - 2.0 * sqrt(g * hR) + 2.0 * sqrt(g * hm);
+                rare2 = 1;
+                rare1 = 0;
+            }
+        }
+    }
+    (void)um;
+    out[0] = hm;
+    out[1] = s1m;
+    out[2] = s2m;
+    rare[0] = rare1;
+    rare[1] = rare2;
+}
+
+/* ------------------------------------------------------------------------- */
+/* riemann_aug_JCP: geoclaw_riemann_utils.f:2-311 (rarecorrectortest is      */
+/* .false. in the reference, so the middle wave is always the steady-state   */
+/* "jump" wave with eigenvector (0,0,1)).                                    */
+/* ------------------------------------------------------------------------- */
+void gfo_riemann_aug_jcp(int maxiter, const double in[18], double drytol, double g,
+                         double sw[3], double fw[9])
+{
+    double hL = in[0], hR = in[1], huL = in[2], huR = in[3];
+    double bL = in[6], bR = in[7], uL = in[8], uR = in[9];
+    double vL = in[10], vR = in[11], phiL = in[12], phiR = in[13];
+    double pL = in[14], pR = in[15], sE1 = in[16], sE2 = in[17];
+    double A[3][3], r[3][3], lambda[3], del[3], beta[3];
+    double delh, delhu, delphi, delb, delP, delnorm;
+    double criticaltol, criticaltol_2, hustar_interface;
+    double s1s2bar, s1s2tilde, hbar, hLstar, hRstar, huRstar, huLstar, uRstar, uLstar, hstarHLL;
+    double deldelh, deldelphi, s1m, s2m, hm;
+    double det1, det2, det3, determinant;
+    int rare[2], sonic;
+    double rt[3];
+
+    (void)in[4];
+    (void)in[5];
+    delh = hR - hL;
+    delhu = huR - huL;
+    delphi = phiR - phiL;
+    delb = bR - bL;
+    delP = pR - pL;
+    delnorm = delh * delh + delphi * delphi;
+
+    gfo_riemanntype(hL, hR, uL, uR, 1, drytol, g, rt, rare);
+    hm = rt[0];
+    s1m = rt[1];
+    s2m = rt[2];
+    (void)hm;
+
+    lambda[0] = FMIN(sE1, s2m);
+    lambda[2] = FMAX(sE2, s1m);
+    sE1 = lambda[0];
+    sE2 = lambda[2];
+    lambda[1] = 0.0;
+
+    hstarHLL = FMAX((huL - huR + sE2 * hR - sE1 * hL) / (sE2 - sE1), 0.0);
+
+    /* r(m,mw) stored as r[m][mw] */
+    for (int mw = 0; mw < 3; mw++) {
+        r[0][mw] = 1.0;
+        r[1][mw] = lambda[mw];
+        r[2][mw] = lambda[mw] * lambda[mw];
+    }
+    /* .not.rarecorrector */
+    lambda[1] = 0.5 * (lambda[0] + lambda[2]);
+    r[0][1] = 0.0;
+    r[1][1] = 0.0;
+    r[2][1] = 1.0;
+
+    criticaltol = FMAX(drytol * g, 1e-6);
+    criticaltol_2 = sqrt(criticaltol);
+    deldelh = -delb;
+    deldelphi = -0.5 * (hR + hL) * (g * delb);
+
+    hLstar = hL;
+    hRstar = hR;
+    uLstar = uL;
+    uRstar = uR;
+    huLstar = uLstar * hLstar;
+    huRstar = uRstar * hRstar;
+
+    for (int iter = 1; iter <= maxiter; iter++) {
+        double t;
+        hbar = FMAX(0.5 * (hLstar + hRstar), 0.0);
+        t = uLstar + uRstar;
+        s1s2bar = 0.25 * (t * t) - g * hbar;
+        s1s2tilde = FMAX(0.0, uLstar * uRstar) - g * hbar;
+
+        sonic = 0;
+        if (fabs(s1s2bar) <= criticaltol) {
+            sonic = 1;
+        } else if (s1s2bar * s1s2tilde <= criticaltol * criticaltol) {
+            sonic = 1;
+        } else if (s1s2bar * sE1 * sE2 <= criticaltol * criticaltol) {
+            sonic = 1;
+        } else if (FMIN(fabs(sE1), fabs(sE2)) < criticaltol_2) {
+            sonic = 1;
+        } else if (sE1 < criticaltol_2 && s1m > -criticaltol_2) {
+            sonic = 1;
+        } else if (sE2 > -criticaltol_2 && s2m < criticaltol_2) {
+            sonic = 1;
+        } else if ((uL + sqrt(g * hL)) * (uR + sqrt(g * hR)) < 0.0) {
+            sonic = 1;
+        } else if ((uL - sqrt(g * hL)) * (uR - sqrt(g * hR)) < 0.0) {
+            sonic = 1;
+        }
+
+        if (sonic)
+            deldelh = -delb;
+        else
+            deldelh = delb * g * hbar / s1s2bar;
+
+        if (sE1 < -criticaltol && sE2 > criticaltol) {
+            deldelh = FMIN(deldelh, hstarHLL * (sE2 - sE1) / sE2);
+            deldelh = FMAX(deldelh, hstarHLL * (sE2 - sE1) / sE1);
+        } else if (sE1 >= criticaltol) {
+            deldelh = FMIN(deldelh, hstarHLL * (sE2 - sE1) / sE1);
+            deldelh = FMAX(deldelh, -hL);
+        } else if (sE2 <= -criticaltol) {
+            deldelh = FMIN(deldelh, hR);
+            deldelh = FMAX(deldelh, hstarHLL * (sE2 - sE1) / sE2);
+        }
+
+        if (sonic)
+            deldelphi = -g * hbar * delb;
+        else
+            deldelphi = -delb * g * hbar * s1s2tilde / s1s2bar;
+
+        deldelphi = FMIN(deldelphi, g * FMAX(-hLstar * delb, -hRstar * delb));
+        deldelphi = FMAX(deldelphi, g * FMIN(-hLstar * delb, -hRstar * delb));
+
+        del[0] = delh - deldelh;
+        del[1] = delhu;
+        del[2] = delphi - deldelphi;
+
+        det1 = r[0][0] * (r[1][1] * r[2][2] - r[1][2] * r[2][1]);
+        det2 = r[0][1] * (r[1][0] * r[2][2] - r[1][2] * r[2][0]);
+        det3 = r[0][2] * (r[1][0] * r[2][1] - r[1][1] * r[2][0]);
+        determinant = det1 - det2 + det3;
+
+        for (int k = 0; k < 3; k++) {
+            for (int mw = 0; mw < 3; mw++) {
+                A[0][mw] = r[0][mw];
+                A[1][mw] = r[1][mw];
+                A[2][mw] = r[2][mw];
+            }
+            A[0][k] = del[0];
+            A[1][k] = del[1];
+            A[2][k] = del[2];
+            det1 = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]);
+            det2 = A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]);
+            det3 = A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+            beta[k] = (det1 - det2 + det3) / determinant;
+        }
+
+        /* exit if things aren't changing */
+        if (fabs(del[0] * del[0] + del[2] * del[2] - delnorm) < 1e-6)
+            break;
+        delnorm = del[0] * del[0] + del[2] * del[2];
+        hLstar = hL;
+        hRstar = hR;
+        uLstar = uL;
+        uRstar = uR;
+        huLstar = uLstar * hLstar;
+        huRstar = uRstar * hRstar;
+        for (int mw = 0; mw < 3; mw++) {
+            if (lambda[mw] < 0.0) {
+                hLstar = hLstar + beta[mw] * r[0][mw];
+                huLstar = huLstar + beta[mw] * r[1][mw];
+            }
+        }
+        for (int mw = 2; mw >= 0; mw--) {
+            if (lambda[mw] > 0.0) {
+                hRstar = hRstar - beta[mw] * r[0][mw];
+                huRstar = huRstar - beta[mw] * r[1][mw];
+            }
+        }
+        if (hLstar > drytol) {
+            uLstar = huLstar / hLstar;
+        } else {
+            hLstar = FMAX(hLstar, 0.0);
+            uLstar = 0.0;
+        }
+        if (hRstar > drytol) {
+            uRstar = huRstar / hRstar;
+        } else {
+            hRstar = FMAX(hRstar, 0.0);
+            uRstar = 0.0;
+        }
+    }
+    (void)delP;
+    (void)huLstar;
+    (void)huRstar;
+
+    /* fw(m,mw) -> fw[mw*3 + m] */
+    for (int mw = 0; mw < 3; mw++) {
+        sw[mw] = lambda[mw];
+        fw[mw * 3 + 0] = beta[mw] * r[1][mw];
+        fw[mw * 3 + 1] = beta[mw] * r[2][mw];
+        fw[mw * 3 + 2] = beta[mw] * r[1][mw];
+    }
+    fw[0 * 3 + 2] = fw[0 * 3 + 2] * vL;
+    fw[2 * 3 + 2] = fw[2 * 3 + 2] * vR;
+    fw[1 * 3 + 2] = 0.0;
+
+    hustar_interface = huL + fw[0 * 3 + 0];
+    if (hustar_interface <= 0.0)
+        fw[0 * 3 + 2] = fw[0 * 3 + 2] + (hR * uR * vR - hL * uL * vL - fw[0 * 3 + 2] - fw[2 * 3 + 2]);
+    else
+        fw[2 * 3 + 2] = fw[2 * 3 + 2] + (hR * uR * vR - hL * uL * vL - fw[0 * 3 + 2] - fw[2 * 3 + 2]);
+}
+
+/* ------------------------------------------------------------------------- */
+/* rpn2: rpn2_geoclaw.f:2-386                                                */
+/* ------------------------------------------------------------------------- */
+void gfo_rpn2(const gfo_params *p, int ixy, int mx, int mbc, int maux,
+              double *q1d, const double *aux2,
+              double *fwave, double *s, double *amdq, double *apdq)
+{
+    const double g = p->grav, drytol = p->dry_tolerance;
+    const int mu = (ixy == 1) ? 2 : 3;
+    const int nv = (ixy == 1) ? 3 : 2;
+
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double hR, hL, huR, huL, uR, uL, hvR, hvL, vR, vL, phiR, phiL;
+        double bR, bL, sL, sR, sRoe1, sRoe2, sE1, sE2, uhat, chat;
+        double hstartest, wall[3], sw[3], fw[9], rt[3], in[18];
+        int rare[2];
+
+        for (int mw = 1; mw <= MWAVES; mw++) {
+            SS(mw, i) = 0.0;
+            FW(1, mw, i) = 0.0;
+            FW(2, mw, i) = 0.0;
+            FW(3, mw, i) = 0.0;
+        }
+
+        /* zero (small) negative values (ql and qr alias q1d) */
+        if (Q1(1, i - 1) < 0.0) {
+            Q1(1, i - 1) = 0.0;
+            Q1(2, i - 1) = 0.0;
+            Q1(3, i - 1) = 0.0;
+        }
+        if (Q1(1, i) < 0.0) {
+            Q1(1, i) = 0.0;
+            Q1(2, i) = 0.0;
+            Q1(3, i) = 0.0;
+        }
+
+        if (Q1(1, i - 1) <= drytol && Q1(1, i) <= drytol)
+            continue;
+
+        hL = Q1(1, i - 1);
+        hR = Q1(1, i);
+        huL = Q1(mu, i - 1);
+        huR = Q1(mu, i);
+        bL = AX(aux2, i - 1);
+        bR = AX(aux2, i);
+        hvL = Q1(nv, i - 1);
+        hvR = Q1(nv, i);
+
+        if (hR > drytol) {
+            uR = huR / hR;
+            vR = hvR / hR;
+            phiR = 0.5 * g * (hR * hR) + (huR * huR) / hR;
+        } else {
+            hR = 0.0;
+            huR = 0.0;
+            hvR = 0.0;
+            uR = 0.0;
+            vR = 0.0;
+            phiR = 0.0;
+        }
+        if (hL > drytol) {
+            uL = huL / hL;
+            vL = hvL / hL;
+            phiL = 0.5 * g * (hL * hL) + (huL * huL) / hL;
+        } else {
+            hL = 0.0;
+            huL = 0.0;
+            hvL = 0.0;
+            uL = 0.0;
+            vL = 0.0;
+            phiL = 0.0;
+        }
+
+        wall[0] = 1.0;
+        wall[1] = 1.0;
+        wall[2] = 1.0;
+        if (hR <= drytol) {
+            gfo_riemanntype(hL, hL, uL, -uL, 1, drytol, g, rt, rare);
+            hstartest = FMAX(hL, rt[0]);
+            if (hstartest + bL < bR) {
+                wall[1] = 0.0;
+                wall[2] = 0.0;
+                hR = hL;
+                huR = -huL;
+                bR = bL;
+                phiR = phiL;
+                uR = -uL;
+                vR = vL;
+            } else if (hL + bL < bR) {
+                bR = hL + bL;
+            }
+        } else if (hL <= drytol) {
+            gfo_riemanntype(hR, hR, -uR, uR, 1, drytol, g, rt, rare);
+            hstartest = FMAX(hR, rt[0]);
+            if (hstartest + bR < bL) {
+                wall[0] = 0.0;
+                wall[1] = 0.0;
+                hL = hR;
+                huL = -huR;
+                bL = bR;
+                phiL = phiR;
+                uL = -uR;
+                vL = vR;
+            } else if (hR + bR < bL) {
+                bL = hR + bR;
+            }
+        }
+
+        sL = uL - sqrt(g * hL);
+        sR = uR + sqrt(g * hR);
+        uhat = (sqrt(g * hL) * uL + sqrt(g * hR) * uR) / (sqrt(g * hR) + sqrt(g * hL));
+        chat = sqrt(g * 0.5 * (hR + hL));
+        sRoe1 = uhat - chat;
+        sRoe2 = uhat + chat;
+        sE1 = FMIN(sL, sRoe1);
+        sE2 = FMAX(sR, sRoe2);
+
+        in[0] = hL; in[1] = hR; in[2] = huL; in[3] = huR; in[4] = hvL; in[5] = hvR;
+        in[6] = bL; in[7] = bR; in[8] = uL; in[9] = uR; in[10] = vL; in[11] = vR;
+        in[12] = phiL; in[13] = phiR; in[14] = 0.0; in[15] = 0.0; in[16] = sE1; in[17] = sE2;
+        gfo_riemann_aug_jcp(1, in, drytol, g, sw, fw);
+
+        for (int mw = 0; mw < 3; mw++) {
+            sw[mw] = sw[mw] * wall[mw];
+            fw[mw * 3 + 0] = fw[mw * 3 + 0] * wall[mw];
+            fw[mw * 3 + 1] = fw[mw * 3 + 1] * wall[mw];
+            fw[mw * 3 + 2] = fw[mw * 3 + 2] * wall[mw];
+        }
+        for (int mw = 1; mw <= MWAVES; mw++) {
+            SS(mw, i) = sw[mw - 1];
+            FW(1, mw, i) = fw[(mw - 1) * 3 + 0];
+            FW(mu, mw, i) = fw[(mw - 1) * 3 + 1];
+            FW(nv, mw, i) = fw[(mw - 1) * 3 + 2];
+        }
+    }
+
+    /* fluctuations: amdq(1:3,:) = 0, apdq(1:3,:) = 0 over the whole slice */
+    for (int i = 1 - mbc; i <= mx + mbc; i++)
+        for (int m = 1; m <= MEQN; m++) {
+            V3(amdq, m, i) = 0.0;
+            V3(apdq, m, i) = 0.0;
+        }
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        for (int mw = 1; mw <= MWAVES; mw++) {
+            if (SS(mw, i) < 0.0) {
+                for (int m = 1; m <= 3; m++)
+                    V3(amdq, m, i) = V3(amdq, m, i) + FW(m, mw, i);
+            } else if (SS(mw, i) > 0.0) {
+                for (int m = 1; m <= 3; m++)
+                    V3(apdq, m, i) = V3(apdq, m, i) + FW(m, mw, i);
+            } else {
+                for (int m = 1; m <= 3; m++) {
+                    V3(amdq, m, i) = V3(amdq, m, i) + 0.5 * FW(m, mw, i);
+                    V3(apdq, m, i) = V3(apdq, m, i) + 0.5 * FW(m, mw, i);
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* rpt2: rpt2_geoclaw.f:2-190 (Cartesian coordinates)                        */
+/* ------------------------------------------------------------------------- */
+void gfo_rpt2(const gfo_params *p, int ixy, int imp, int mx, int mbc, int maux,
+              const double *q1d, const double *aux1, const double *aux2,
+              const double *aux3, const double *asdq,
+              double *bmasdq, double *bpasdq)
+{
+    const double g = p->grav, tol = p->dry_tolerance;
+    const int mu = (ixy == 1) ? 2 : 3;
+    const int mv = (ixy == 1) ? 3 : 2;
+
+    for (int i = 1 - mbc; i <= mx + mbc; i++)
+        for (int m = 1; m <= MEQN; m++) {
+            V3(bmasdq, m, i) = 0.0;
+            V3(bpasdq, m, i) = 0.0;
+        }
+
+    for (int i = 2 - mbc; i <= mx + mbc; i++) {
+        double h, u, v, eta, topo1, topo3, s[3], beta[3], r[3][3];
+        double delf1, delf2, delf3, dxdcm, dxdcp;
+        const int ic = (imp == 1) ? i - 1 : i;
+
+        h = Q1(1, ic);
+        if (h <= tol)
+            continue;
+        u = Q1(mu, ic) / h;
+        v = Q1(mv, ic) / h;
+        eta = h + AX(aux2, ic);
+        topo1 = AX(aux1, ic);
+        topo3 = AX(aux3, ic);
+
+        if (eta < FMIN(topo1, topo3))
+            continue;
+
+        dxdcp = 1.0;
+        dxdcm = 1.0;
+
+        s[0] = v - sqrt(g * h);
+        s[1] = v;
+        s[2] = v + sqrt(g * h);
+
+        delf1 = V3(asdq, 1, i);
+        delf2 = V3(asdq, mu, i);
+        delf3 = V3(asdq, mv, i);
+
+        beta[0] = (s[2] * delf1 - delf3) / (s[2] - s[0]);
+        beta[1] = -u * delf1 + delf2;
+        beta[2] = (delf3 - s[0] * delf1) / (s[2] - s[0]);
+
+        r[0][0] = 1.0; r[1][0] = u;   r[2][0] = s[0];
+        r[0][1] = 0.0; r[1][1] = 1.0; r[2][1] = 0.0;
+        r[0][2] = 1.0; r[1][2] = u;   r[2][2] = s[2];
+
+        for (int mw = 0; mw < 3; mw++) {
+            if ((s[mw] < 0.0) && (eta >= topo1)) {
+                V3(bmasdq, 1, i) = V3(bmasdq, 1, i) + dxdcm * s[mw] * beta[mw] * r[0][mw];
+                V3(bmasdq, mu, i) = V3(bmasdq, mu, i) + dxdcm * s[mw] * beta[mw] * r[1][mw];
+                V3(bmasdq, mv, i) = V3(bmasdq, mv, i) + dxdcm * s[mw] * beta[mw] * r[2][mw];
+            } else if ((s[mw] > 0.0) && (eta >= topo3)) {
+                V3(bpasdq, 1, i) = V3(bpasdq, 1, i) + dxdcp * s[mw] * beta[mw] * r[0][mw];
+                V3(bpasdq, mu, i) = V3(bpasdq, mu, i) + dxdcp * s[mw] * beta[mw] * r[1][mw];
+                V3(bpasdq, mv, i) = V3(bpasdq, mv, i) + dxdcp * s[mw] * beta[mw] * r[2][mw];
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* philim.f:65-126 and geoclaw_limiter_fort.f:4-61                           */
+/* ------------------------------------------------------------------------- */
+double gfo_philim(double a, double b, int meth)
+{
+    double r, c;
+    if (a == 0.0)
+        return 0.0;
+    r = b / a;
+    switch (meth) {
+    case 1:
+        return FMAX(0.0, FMIN(1.0, r));
+    case 2:
+        return FMAX(FMAX(0.0, FMIN(1.0, 2.0 * r)), FMIN(2.0, r));
+    case 3:
+        return (r + fabs(r)) / (1.0 + fabs(r));
+    case 4:
+        c = (1.0 + r) / 2.0;
+        return FMAX(0.0, FMIN(FMIN(c, 2.0), 2.0 * r));
+    case 5:
+        return r;
+    default:
+        /* a computed GO TO whose index is out of range falls through to the
+         * next statement, which is the minmod branch */
+        return FMAX(0.0, FMIN(1.0, r));
+    }
+}
+
+void gfo_limiter(int mx, int mbc, double *wave, const double *s, const int mthlim[3])
+{
+    double *fwave = wave;
+    for (int mw = 1; mw <= MWAVES; mw++) {
+        double dotr, dotl, wnorm2, wlimitr;
+        if (mthlim[mw - 1] == 0)
+            continue;
+        dotr = 0.0;
+        for (int i = 0; i <= mx + 1; i++) {
+            wnorm2 = 0.0;
+            dotl = dotr;
+            dotr = 0.0;
+            for (int m = 1; m <= MEQN; m++) {
+                wnorm2 = wnorm2 + FW(m, mw, i) * FW(m, mw, i);
+                dotr = dotr + FW(m, mw, i) * FW(m, mw, i + 1);
+            }
+            if (i == 0)
+                continue;
+            if (wnorm2 == 0.0)
+                continue;
+            if (SS(mw, i) > 0.0)
+                wlimitr = gfo_philim(wnorm2, dotl, mthlim[mw - 1]);
+            else
+                wlimitr = gfo_philim(wnorm2, dotr, mthlim[mw - 1]);
+            for (int m = 1; m <= MEQN; m++)
+                FW(m, mw, i) = wlimitr * FW(m, mw, i);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* fc2d_geoclaw_flux2: geoclaw_flux2fw_fort.f:4-269                          */
+/* ------------------------------------------------------------------------- */
+void gfo_flux2(const gfo_params *p, int ixy, int mx, int mbc, int maux,
+               double *q1d, double *dtdx1d,
+               const double *aux1, const double *aux2, const double *aux3,
+               double *faddm, double *faddp, double *gaddm, double *gaddp,
+               double *cfl1d_out)
+{
+    const int n = mx + 2 * mbc;
+    double *fwave = (double *)malloc(sizeof(double) * (size_t)n * MEQN * MWAVES);
+    double *s = (double *)malloc(sizeof(double) * (size_t)n * MWAVES);
+    double *amdq = (double *)malloc(sizeof(double) * (size_t)n * MEQN);
+    double *apdq = (double *)malloc(sizeof(double) * (size_t)n * MEQN);
+    double *cqxx = (double *)calloc((size_t)n * MEQN, sizeof(double));
+    double *bmasdq = (double *)malloc(sizeof(double) * (size_t)n * MEQN);
+    double *bpasdq = (double *)malloc(sizeof(double) * (size_t)n * MEQN);
+    int limit = 0;
+    double cfl1d;
+#define DT(i) dtdx1d[K(i)]
+#define G2(a, m, i, side) (a)[((side) - 1) * n * MEQN + K(i) * MEQN + (m) - 1]
+
+    for (int mw = 0; mw < MWAVES; mw++)
+        if (p->mthlim[mw] > 0)
+            limit = 1;
+
+    for (int jside = 1; jside <= 2; jside++)
+        for (int i = 1 - mbc; i <= mx + mbc; i++)
+            for (int m = 1; m <= MEQN; m++) {
+                V3(faddm, m, i) = 0.0;
+                V3(faddp, m, i) = 0.0;
+                G2(gaddm, m, i, jside) = 0.0;
+                G2(gaddp, m, i, jside) = 0.0;
+            }
+
+    gfo_rpn2(p, ixy, mx, mbc, maux, q1d, aux2, fwave, s, amdq, apdq);
+
+    for (int i = 2 - mbc; i <= mx + mbc - 1; i++)
+        for (int m = 1; m <= MEQN; m++) {
+            V3(faddp, m, i) = V3(faddp, m, i) - V3(apdq, m, i);
+            V3(faddm, m, i) = V3(faddm, m, i) + V3(amdq, m, i);
+        }
+
+    cfl1d = 0.0;
+    for (int mw = 1; mw <= MWAVES; mw++)
+        for (int i = 2 - mbc; i <= mx + mbc; i++) {
+            cfl1d = FMAX(cfl1d, DT(i) * SS(mw, i));
+            cfl1d = FMAX(cfl1d, -DT(i - 1) * SS(mw, i));
+        }
+
+    if (p->order != 1) {
+        if (limit)
+            gfo_limiter(mx, mbc, fwave, s, p->mthlim);
+        for (int i = 2 - mbc; i <= mx + mbc; i++) {
+            DT(i - 1) = 0.5 * (DT(i - 1) + DT(i));
+            for (int m = 1; m <= MEQN; m++) {
+                V3(cqxx, m, i) = 0.0;
+                for (int mw = 1; mw <= MWAVES; mw++)
+                    V3(cqxx, m, i) = V3(cqxx, m, i) + copysign(1.0, SS(mw, i)) *
+                                     (1.0 - fabs(SS(mw, i)) * DT(i - 1)) * FW(m, mw, i);
+                V3(faddm, m, i) = V3(faddm, m, i) + 0.5 * V3(cqxx, m, i);
+                V3(faddp, m, i) = V3(faddp, m, i) + 0.5 * V3(cqxx, m, i);
+            }
+        }
+    }
+
+    if (p->transverse != 0) {
+        if (p->order > 1 && p->transverse == 2) {
+            for (int i = 2 - mbc; i <= mx + mbc - 1; i++)
+                for (int m = 1; m <= MEQN; m++) {
+                    V3(amdq, m, i) = V3(amdq, m, i) + V3(cqxx, m, i);
+                    V3(apdq, m, i) = V3(apdq, m, i) - V3(cqxx, m, i);
+                }
+        }
+        gfo_rpt2(p, ixy, 1, mx, mbc, maux, q1d, aux1, aux2, aux3, amdq, bmasdq, bpasdq);
+        for (int i = 2 - mbc; i <= mx + mbc; i++)
+            for (int m = 1; m <= MEQN; m++) {
+                double gupdate = 0.5 * DT(i - 1) * V3(bmasdq, m, i);
+                G2(gaddm, m, i - 1, 1) = G2(gaddm, m, i - 1, 1) - gupdate;
+                G2(gaddp, m, i - 1, 1) = G2(gaddp, m, i - 1, 1) - gupdate;
+                gupdate = 0.5 * DT(i - 1) * V3(bpasdq, m, i);
+                G2(gaddm, m, i - 1, 2) = G2(gaddm, m, i - 1, 2) - gupdate;
+                G2(gaddp, m, i - 1, 2) = G2(gaddp, m, i - 1, 2) - gupdate;
+            }
+        gfo_rpt2(p, ixy, 2, mx, mbc, maux, q1d, aux1, aux2, aux3, apdq, bmasdq, bpasdq);
+        for (int i = 2 - mbc; i <= mx + mbc; i++)
+            for (int m = 1; m <= MEQN; m++) {
+                double gupdate = 0.5 * DT(i - 1) * V3(bmasdq, m, i);
+                G2(gaddm, m, i, 1) = G2(gaddm, m, i, 1) - gupdate;
+                G2(gaddp, m, i, 1) = G2(gaddp, m, i, 1) - gupdate;
+                gupdate = 0.5 * DT(i - 1) * V3(bpasdq, m, i);
+                G2(gaddm, m, i, 2) = G2(gaddm, m, i, 2) - gupdate;
+                G2(gaddp, m, i, 2) = G2(gaddp, m, i, 2) - gupdate;
+            }
+    }
+#undef DT
+#undef G2
+    *cfl1d_out = cfl1d;
+    free(fwave);
+    free(s);
+    free(amdq);
+    free(apdq);
+    free(cqxx);
+    free(bmasdq);
+    free(bpasdq);
+}
+
+/* ------------------------------------------------------------------------- */
+/* fc2d_geoclaw_step2 (geoclaw_step2_fort.f90:1-294) + the update of         */
+/* fc2d_geoclaw_step2_wrap (geoclaw_step2_wrap_fort.f:94-115), mcapa = 0.     */
+/* ------------------------------------------------------------------------- */
+#define P2(a, nc, m, i, j) (a)[((size_t)((j) + mbc - 1) * (mx + 2 * mbc) + (size_t)((i) + mbc - 1)) * (nc) + (m) - 1]
+
+double gfo_step2_wrap(const gfo_params *p, int mx, int my, int mbc, int maux,
+                      double *qold, double *aux, double dx, double dy, double dt)
+{
+    const int maxm = mx > my ? mx : my;
+    const int n = maxm + 2 * mbc;
+    const size_t ncell = (size_t)(mx + 2 * mbc) * (my + 2 * mbc);
+    double *fm = (double *)calloc(ncell * MEQN, sizeof(double));
+    double *fp = (double *)calloc(ncell * MEQN, sizeof(double));
+    double *gm = (double *)calloc(ncell * MEQN, sizeof(double));
+    double *gp = (double *)calloc(ncell * MEQN, sizeof(double));
+    double *faddm = (double *)malloc(sizeof(double) * n * MEQN);
+    double *faddp = (double *)malloc(sizeof(double) * n * MEQN);
+    double *gaddm = (double *)malloc(sizeof(double) * n * MEQN * 2);
+    double *gaddp = (double *)malloc(sizeof(double) * n * MEQN * 2);
+    double *q1d = (double *)malloc(sizeof(double) * n * MEQN);
+    double *aux1 = (double *)malloc(sizeof(double) * n * maux);
+    double *aux2 = (double *)malloc(sizeof(double) * n * maux);
+    double *aux3 = (double *)malloc(sizeof(double) * n * maux);
+    double *dtdx1d = (double *)malloc(sizeof(double) * n);
+    double cflgrid, cfl1d, dtdx, dtdy;
+
+    cflgrid = 0.0;
+    dtdx = dt / dx;
+    dtdy = dt / dy;
+
+    /* x-sweeps */
+    for (int j = 2 - mbc; j <= my + mbc - 1; j++) {
+        for (int i = 1 - mbc; i <= mx + mbc; i++)
+            for (int m = 1; m <= MEQN; m++)
+                V3(q1d, m, i) = P2(qold, MEQN, m, i, j);
+        for (int k = 0; k < n; k++)
+            dtdx1d[k] = dtdx;
+        for (int i = 1 - mbc; i <= mx + mbc; i++)
+            for (int ma = 0; ma < maux; ma++) {
+                aux1[K(i) * maux + ma] = P2(aux, maux, ma + 1, i, j - 1);
+                aux2[K(i) * maux + ma] = P2(aux, maux, ma + 1, i, j);
+                aux3[K(i) * maux + ma] = P2(aux, maux, ma + 1, i, j + 1);
+            }
+        gfo_flux2(p, 1, mx, mbc, maux, q1d, dtdx1d, aux1, aux2, aux3,
+                  faddm, faddp, gaddm, gaddp, &cfl1d);
+        cflgrid = FMAX(cflgrid, cfl1d);
+        const int nx = mx + 2 * mbc;
+        for (int i = 2 - mbc; i <= mx + mbc; i++)
+            for (int m = 1; m <= MEQN; m++) {
+                P2(fm, MEQN, m, i, j) = P2(fm, MEQN, m, i, j) + V3(faddm, m, i);
+                P2(fp, MEQN, m, i, j) = P2(fp, MEQN, m, i, j) + V3(faddp, m, i);
+                P2(gm, MEQN, m, i, j) = P2(gm, MEQN, m, i, j) + gaddm[K(i) * MEQN + m - 1];
+                P2(gp, MEQN, m, i, j) = P2(gp, MEQN, m, i, j) + gaddp[K(i) * MEQN + m - 1];
+                P2(gm, MEQN, m, i, j + 1) = P2(gm, MEQN, m, i, j + 1) + gaddm[nx * MEQN + K(i) * MEQN + m - 1];
+                P2(gp, MEQN, m, i, j + 1) = P2(gp, MEQN, m, i, j + 1) + gaddp[nx * MEQN + K(i) * MEQN + m - 1];
+            }
+    }
+
+    /* y-sweeps */
+    for (int i = 2 - mbc; i <= mx + mbc - 1; i++) {
+        for (int j = 1 - mbc; j <= my + mbc; j++)
+            for (int m = 1; m <= MEQN; m++)
+                V3(q1d, m, j) = P2(qold, MEQN, m, i, j);
+        for (int k = 0; k < n; k++)
+            dtdx1d[k] = dtdy;
+        for (int j = 1 - mbc; j <= my + mbc; j++)
+            for (int ma = 0; ma < maux; ma++) {
+                aux1[K(j) * maux + ma] = P2(aux, maux, ma + 1, i - 1, j);
+                aux2[K(j) * maux + ma] = P2(aux, maux, ma + 1, i, j);
+                aux3[K(j) * maux + ma] = P2(aux, maux, ma + 1, i + 1, j);
+            }
+        gfo_flux2(p, 2, my, mbc, maux, q1d, dtdx1d, aux1, aux2, aux3,
+                  faddm, faddp, gaddm, gaddp, &cfl1d);
+        cflgrid = FMAX(cflgrid, cfl1d);
+        const int ny = my + 2 * mbc;
+        for (int j = 2 - mbc; j <= my + mbc; j++)
+            for (int m = 1; m <= MEQN; m++) {
+                P2(gm, MEQN, m, i, j) = P2(gm, MEQN, m, i, j) + V3(faddm, m, j);
+                P2(gp, MEQN, m, i, j) = P2(gp, MEQN, m, i, j) + V3(faddp, m, j);
+                P2(fm, MEQN, m, i, j) = P2(fm, MEQN, m, i, j) + gaddm[K(j) * MEQN + m - 1];
+                P2(fp, MEQN, m, i, j) = P2(fp, MEQN, m, i, j) + gaddp[K(j) * MEQN + m - 1];
+                P2(fm, MEQN, m, i + 1, j) = P2(fm, MEQN, m, i + 1, j) + gaddm[ny * MEQN + K(j) * MEQN + m - 1];
+                P2(fp, MEQN, m, i + 1, j) = P2(fp, MEQN, m, i + 1, j) + gaddp[ny * MEQN + K(j) * MEQN + m - 1];
+            }
+    }
+
+    /* update q (step2_wrap), ghost cells included */
+    for (int m = 1; m <= MEQN; m++)
+        for (int i = 1 - mbc; i <= mx + mbc - 1; i++)
+            for (int j = 1 - mbc; j <= my + mbc - 1; j++)
+                P2(qold, MEQN, m, i, j) = P2(qold, MEQN, m, i, j)
+                    - dtdx * (P2(fm, MEQN, m, i + 1, j) - P2(fp, MEQN, m, i, j))
+                    - dtdy * (P2(gm, MEQN, m, i, j + 1) - P2(gp, MEQN, m, i, j));
+
+    free(fm); free(fp); free(gm); free(gp);
+    free(faddm); free(faddp); free(gaddm); free(gaddp);
+    free(q1d); free(aux1); free(aux2); free(aux3); free(dtdx1d);
+    return cflgrid;
+}
+
+/* ------------------------------------------------------------------------- */
+void gfo_b4step2(const gfo_params *p, int mx, int my, int mbc, double *q)
+{
+    for (int j = 1 - mbc; j <= my + mbc; j++)
+        for (int i = 1 - mbc; i <= mx + mbc; i++)
+            if (P2(q, MEQN, 1, i, j) < p->dry_tolerance) {
+                P2(q, MEQN, 1, i, j) = FMAX(P2(q, MEQN, 1, i, j), 0.0);
+                P2(q, MEQN, 2, i, j) = 0.0;
+                P2(q, MEQN, 3, i, j) = 0.0;
+            }
+}
+
+void gfo_src2(const gfo_params *p, int mx, int my, int mbc, int maux,
+              double *q, const double *aux, double dt)
+{
+    const double g = p->grav;
+    /* manning_break(num_manning) = rinfinity (geoclaw_module.f90:110),
+     * rinfinity = 10.e32, a default-real literal (amr_module.f90:171) */
+    const double manning_break = (double)10.e32f;
+    (void)aux;
+    (void)maux;
+    if (!p->friction_forcing)
+        return;
+    for (int j = 1; j <= my; j++)
+        for (int i = 1; i <= mx; i++) {
+            if (P2(q, MEQN, 1, i, j) < 1.0e-30) {
+                P2(q, MEQN, 2, i, j) = 0.0;
+                P2(q, MEQN, 3, i, j) = 0.0;
+            } else if (P2(q, MEQN, 1, i, j) <= p->friction_depth) {
+                double coeff = 0.0, gamma, dgamma, h = P2(q, MEQN, 1, i, j);
+                double hu = P2(q, MEQN, 2, i, j), hv = P2(q, MEQN, 3, i, j);
+                if (P2(aux, maux, 1, i, j) < manning_break)
+                    coeff = p->manning_n;
+                gamma = sqrt(hu * hu + hv * hv) * g * (coeff * coeff) / pow(h, 7.0 / 3.0);
+                dgamma = 1.0 + dt * gamma;
+                P2(q, MEQN, 2, i, j) = hu / dgamma;
+                P2(q, MEQN, 3, i, j) = P2(q, MEQN, 3, i, j) / dgamma;
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------- */
+/* fc2d_geoclaw_bc2 (geoflood_bc2_fort.f90): extrapolation and solid walls.  */
+/* ------------------------------------------------------------------------- */
+void gfo_bc2(int mx, int my, int mbc, int maux, double *q, double *aux, const int mthbc[4])
+{
+    /* left */
+    if (mthbc[0] == 1) {
+        for (int j = 1 - mbc; j <= my + mbc; j++)
+            for (int ibc = 1; ibc <= mbc; ibc++) {
+                P2(aux, maux, 1, 1 - ibc, j) = P2(aux, maux, 1, 1, j);
+                for (int m = 1; m <= MEQN; m++)
+                    P2(q, MEQN, m, 1 - ibc, j) = P2(q, MEQN, m, 1, j);
+            }
+    } else if (mthbc[0] == 3) {
+        for (int j = 1 - mbc; j <= my + mbc; j++)
+            for (int ibc = 1; ibc <= mbc; ibc++) {
+                P2(aux, maux, 1, 1 - ibc, j) = P2(aux, maux, 1, ibc, j);
+                for (int m = 1; m <= MEQN; m++)
+                    P2(q, MEQN, m, 1 - ibc, j) = P2(q, MEQN, m, ibc, j);
+            }
+        for (int j = 1 - mbc; j <= my + mbc; j++)
+            for (int ibc = 1; ibc <= mbc; ibc++)
+                P2(q, MEQN, 2, 1 - ibc, j) = -P2(q, MEQN, 2, ibc, j);
+    }
+    /* right */
+    if (mthbc[1] == 1) {
+        for (int j = 1 - mbc; j <= my + mbc; j++)
+            for (int ibc = 1; ibc <= mbc; ibc++) {
+                P2(aux, maux, 1, mx + ibc, j) = P2(aux, maux, 1, mx, j);
+                for (int m = 1; m <= MEQN; m++)
+                    P2(q, MEQN, m, mx + ibc, j) = P2(q, MEQN, m, mx, j);
+            }
+    } else if (mthbc[1] == 3) {
+        for (int j = 1 - mbc; j <= my + mbc; j++)
+            for (int ibc = 1; ibc <= mbc; ibc++) {
+                P2(aux, maux, 1, mx + ibc, j) = P2(aux, maux, 1, mx + 1 - ibc, j);
+                for (int m = 1; m <= MEQN; m++)
+                    P2(q, MEQN, m, mx + ibc, j) = P2(q, MEQN, m, mx + 1 - ibc, j);
+            }
+        for (int j = 1 - mbc; j <= my + mbc; j++)
+            for (int ibc = 1; ibc <= mbc; ibc++)
+                P2(q, MEQN, 2, mx + ibc, j) = -P2(q, MEQN, 2, mx + 1 - ibc, j);
+    }
+    /* bottom */
+    if (mthbc[2] == 1) {
+        for (int jbc = 1; jbc <= mbc; jbc++)
+            for (int i = 1 - mbc; i <= mx + mbc; i++) {
+                P2(aux, maux, 1, i, 1 - jbc) = P2(aux, maux, 1, i, 1);
+                for (int m = 1; m <= MEQN; m++)
+                    P2(q, MEQN, m, i, 1 - jbc) = P2(q, MEQN, m, i, 1);
+            }
+    } else if (mthbc[2] == 3) {
+        for (int jbc = 1; jbc <= mbc; jbc++)
+            for (int i = 1 - mbc; i <= mx + mbc; i++) {
+                P2(aux, maux, 1, i, 1 - jbc) = P2(aux, maux, 1, i, jbc);
+                for (int m = 1; m <= MEQN; m++)
+                    P2(q, MEQN, m, i, 1 - jbc) = P2(q, MEQN, m, i, jbc);
+            }
+        for (int jbc = 1; jbc <= mbc; jbc++)
+            for (int i = 1 - mbc; i <= mx + mbc; i++)
+                P2(q, MEQN, 3, i, 1 - jbc) = -P2(q, MEQN, 3, i, jbc);
+    }
+    /* top */
+    if (mthbc[3] == 1) {
+        for (int jbc = 1; jbc <= mbc; jbc++)
+            for (int i = 1 - mbc; i <= mx + mbc; i++) {
+                P2(aux, maux, 1, i, my + jbc) = P2(aux, maux, 1, i, my);
+                for (int m = 1; m <= MEQN; m++)
+                    P2(q, MEQN, m, i, my + jbc) = P2(q, MEQN, m, i, my);
+            }
+    } else if (mthbc[3] == 3) {
+        for (int jbc = 1; jbc <= mbc; jbc++)
+            for (int i = 1 - mbc; i <= mx + mbc; i++) {
+                P2(aux, maux, 1, i, my + jbc) = P2(aux, maux, 1, i, my + 1 - jbc);
+                for (int m = 1; m <= MEQN; m++)
+                    P2(q, MEQN, m, i, my + jbc) = P2(q, MEQN, m, i, my + 1 - jbc);
+            }
+        for (int jbc = 1; jbc <= mbc; jbc++)
+            for (int i = 1 - mbc; i <= mx + mbc; i++)
+                P2(q, MEQN, 3, i, my + jbc) = -P2(q, MEQN, 3, i, my + 1 - jbc);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+double gfo_patch_update(const gfo_params *p, int mx, int my, int mbc, int maux,
+                        double *q, double *aux, double dx, double dy,
+                        double t, double dt)
+{
+    double maxcfl;
+    (void)t;
+    gfo_b4step2(p, mx, my, mbc, q);
+    maxcfl = gfo_step2_wrap(p, mx, my, mbc, maux, q, aux, dx, dy, dt);
+    if (maxcfl > 1000.0) /* MAXCFL_CAP, fc2d_cpucuda.h:43 */
+        maxcfl = 0.0;
+    if (p->src_term > 0)
+        gfo_src2(p, mx, my, mbc, maux, q, aux, dt);
+    return maxcfl;
+}
+
+static double builtin_update(const void *ctx, int mx, int my, int mbc, int maux,
+                             double *q, double *aux, double dx, double dy,
+                             double t, double dt)
+{
+    return gfo_patch_update((const gfo_params *)ctx, mx, my, mbc, maux, q, aux, dx, dy, t, dt);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Uniform brick driver                                                      */
+/* ------------------------------------------------------------------------- */
+void gfo_brick_ghost_fill(int mi, int mj, int mx, int my, int mbc, int maux,
+                          double **q, double **aux, const int mthbc[4])
+{
+    (void)maux;
+    /* same-level face copies (q only; aux ghosts are set at creation) */
+    for (int pj = 0; pj < mj; pj++)
+        for (int pi = 0; pi < mi; pi++) {
+            double *d = q[pj * mi + pi];
+            if (pi > 0) {
+                const double *s = q[pj * mi + pi - 1];
+                for (int j = 1; j <= my; j++)
+                    for (int i = 1 - mbc; i <= 0; i++)
+                        for (int m = 1; m <= MEQN; m++)
+                            P2(d, MEQN, m, i, j) = P2(s, MEQN, m, mx + i, j);
+            }
+            if (pi < mi - 1) {
+                const double *s = q[pj * mi + pi + 1];
+                for (int j = 1; j <= my; j++)
+                    for (int i = mx + 1; i <= mx + mbc; i++)
+                        for (int m = 1; m <= MEQN; m++)
+                            P2(d, MEQN, m, i, j) = P2(s, MEQN, m, i - mx, j);
+            }
+            if (pj > 0) {
+                const double *s = q[(pj - 1) * mi + pi];
+                for (int j = 1 - mbc; j <= 0; j++)
+                    for (int i = 1; i <= mx; i++)
+                        for (int m = 1; m <= MEQN; m++)
+                            P2(d, MEQN, m, i, j) = P2(s, MEQN, m, i, my + j);
+            }
+            if (pj < mj - 1) {
+                const double *s = q[(pj + 1) * mi + pi];
+                for (int j = my + 1; j <= my + mbc; j++)
+                    for (int i = 1; i <= mx; i++)
+                        for (int m = 1; m <= MEQN; m++)
+                            P2(d, MEQN, m, i, j) = P2(s, MEQN, m, i, j - my);
+            }
+        }
+    /* same-level corner copies */
+    for (int pj = 0; pj < mj; pj++)
+        for (int pi = 0; pi < mi; pi++) {
+            double *d = q[pj * mi + pi];
+            for (int cy = 0; cy < 2; cy++)
+                for (int cx = 0; cx < 2; cx++) {
+                    int ni = pi + (cx ? 1 : -1), nj = pj + (cy ? 1 : -1);
+                    if (ni < 0 || ni >= mi || nj < 0 || nj >= mj)
+                        continue;
+                    const double *s = q[nj * mi + ni];
+                    for (int jb = 1; jb <= mbc; jb++)
+                        for (int ib = 1; ib <= mbc; ib++) {
+                            int i = cx ? mx + ib : 1 - ib, j = cy ? my + jb : 1 - jb;
+                            int si = cx ? i - mx : i + mx, sj = cy ? j - my : j + my;
+                            for (int m = 1; m <= MEQN; m++)
+                                P2(d, MEQN, m, i, j) = P2(s, MEQN, m, si, sj);
+                        }
+                }
+        }
+    /* physical boundaries */
+    for (int pj = 0; pj < mj; pj++)
+        for (int pi = 0; pi < mi; pi++) {
+            int bc[4];
+            bc[0] = (pi == 0) ? mthbc[0] : -1;
+            bc[1] = (pi == mi - 1) ? mthbc[1] : -1;
+            bc[2] = (pj == 0) ? mthbc[2] : -1;
+            bc[3] = (pj == mj - 1) ? mthbc[3] : -1;
+            gfo_bc2(mx, my, mbc, maux, q[pj * mi + pi], aux[pj * mi + pi], bc);
+        }
+}
+
+double gfo_brick_advance(const gfo_params *p, int mi, int mj, int mx, int my,
+                         int mbc, int maux, double **q, double **aux,
+                         const int mthbc[4], double dx, double dy, double t,
+                         double dt, gfo_update_fn fn, const void *ctx,
+                         int nthreads)
+{
+    const int np = mi * mj;
+    double maxcfl = 0.0;
+    double *cfl = (double *)malloc(sizeof(double) * np);
+    if (!fn) {
+        fn = builtin_update;
+        ctx = p;
+    }
+    gfo_brick_ghost_fill(mi, mj, mx, my, mbc, maux, q, aux, mthbc);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int k = 0; k < np; k++)
+        cfl[k] = fn(ctx, mx, my, mbc, maux, q[k], aux[k], dx, dy, t, dt);
+    for (int k = 0; k < np; k++)
+        maxcfl = FMAX(maxcfl, cfl[k]);
+    free(cfl);
+    return maxcfl;
+}
+
+int gfo_brick_run(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
+                  int maux, double **q, double **aux, const int mthbc[4],
+                  double dx, double dy, double t0, double initial_dt,
+                  double desired_cfl, double max_cfl, int nsteps,
+                  gfo_update_fn fn, const void *ctx, int nthreads,
+                  double *dt_out, double *cfl_out, double *mass_out)
+{
+    const int np = mi * mj;
+    const size_t nq = (size_t)MEQN * (mx + 2 * mbc) * (my + 2 * mbc);
+    double *save = (double *)malloc(sizeof(double) * nq * np);
+    double t = t0, dt = initial_dt;
+    int n = 0, retakes = 0;
+
+    while (n < nsteps) {
+        double maxcfl;
+        for (int k = 0; k < np; k++)
+            memcpy(save + nq * k, q[k], sizeof(double) * nq);
+        maxcfl = gfo_brick_advance(p, mi, mj, mx, my, mbc, maux, q, aux, mthbc,
+                                   dx, dy, t, dt, fn, ctx, nthreads);
+        if (maxcfl > max_cfl) {
+            for (int k = 0; k < np; k++)
+                memcpy(q[k], save + nq * k, sizeof(double) * nq);
+            dt = dt * desired_cfl / maxcfl;
+            retakes++;
+            continue;
+        }
+        if (dt_out)
+            dt_out[n] = dt;
+        if (cfl_out)
+            cfl_out[n] = maxcfl;
+        if (mass_out) {
+            double mass = 0.0;
+            for (int k = 0; k < np; k++) {
+                double sum = 0.0;
+                for (int j = 1; j <= my; j++)
+                    for (int i = 1; i <= mx; i++)
+                        sum += P2(q[k], MEQN, 1, i, j);
+                mass += sum * dx * dy;
+            }
+            mass_out[n] = mass;
+        }
+        t = t + dt;
+        dt = dt * desired_cfl / maxcfl;
+        n++;
+    }
+    free(save);
+    return retakes;
+}

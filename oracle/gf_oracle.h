@@ -1,0 +1,133 @@
+/*
+ * gf_oracle.h -- CPU oracle for the GeoFlood per-patch finite-volume step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * GeoClaw Fortran step (the CPU route of the fc2d_cpu_cuda plugin).  It is
+ * used by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
+ * leg, and only as the checker: the product library (geoflood_amd) never
+ * links, loads or calls it.
+ *
+ * Array layouts are the reference's host layouts (Fortran column-major):
+ *   q(meqn, 1-mbc:mx+mbc, 1-mbc:my+mbc)   -> q[((j+mbc-1)*(mx+2mbc) + (i+mbc-1))*meqn + (m-1)]
+ *   aux(maux, ...)                        -> same with maux
+ *   1-D slices q1d(meqn, 1-mbc:n+mbc), fwave(meqn, mwaves, 1-mbc:n+mbc),
+ *   s(mwaves, 1-mbc:n+mbc), gadd(meqn, 1-mbc:n+mbc, 2).
+ * meqn = 3, mwaves = 3 (shallow water with topography); aux(1,.,.) is the
+ * bathymetry b.
+ */
+#ifndef GF_ORACLE_H
+#define GF_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gfo_params {
+    double grav;            /* geoclaw_module::grav */
+    double dry_tolerance;   /* geoclaw_module::dry_tolerance */
+    double manning_n;       /* manning_coefficient(1) (num_manning = 1, manning_break = huge) */
+    double friction_depth;  /* geoclaw_module::friction_depth */
+    int friction_forcing;   /* geoclaw_module::friction_forcing (src2 Manning on/off) */
+    int src_term;           /* fc2d_cpucuda_options::src_term (call src2 at all) */
+    int order;              /* method(2): 1 = Godunov, 2 = + limited corrections */
+    int transverse;         /* method(3): 0 none, 1 increment waves, 2 + correction waves */
+    int mthlim[3];          /* limiter per wave family (4 = MC) */
+} gfo_params;
+
+/* riemanntype (geoclaw_riemann_utils.f:566-677).  out = {hm, s1m, s2m}, rare = {rare1, rare2} */
+void gfo_riemanntype(double hL, double hR, double uL, double uR, int maxiter,
+                     double drytol, double g, double out[3], int rare[2]);
+
+/* riemann_aug_JCP (geoclaw_riemann_utils.f:2-311) with meqn = mwaves = 3.
+ * in = {hL,hR,huL,huR,hvL,hvR,bL,bR,uL,uR,vL,vR,phiL,phiR,pL,pR,sE1,sE2}
+ * sw[3], fw[3*3] (fw(m,mw) at fw[(mw-1)*3 + m-1]). */
+void gfo_riemann_aug_jcp(int maxiter, const double in[18], double drytol, double g,
+                         double sw[3], double fw[9]);
+
+/* rpn2 (rpn2_geoclaw.f:2-386).  q1d is modified in place exactly as the
+ * reference modifies its ql/qr alias (negative depths zeroed). */
+void gfo_rpn2(const gfo_params *p, int ixy, int mx, int mbc, int maux,
+              double *q1d, const double *aux2,
+              double *fwave, double *s, double *amdq, double *apdq);
+
+/* rpt2 (rpt2_geoclaw.f:2-190). */
+void gfo_rpt2(const gfo_params *p, int ixy, int imp, int mx, int mbc, int maux,
+              const double *q1d, const double *aux1, const double *aux2,
+              const double *aux3, const double *asdq,
+              double *bmasdq, double *bpasdq);
+
+/* philim (philim.f:65-126) and fc2d_geoclaw_limiter (geoclaw_limiter_fort.f:4-61). */
+double gfo_philim(double a, double b, int meth);
+void gfo_limiter(int mx, int mbc, double *wave, const double *s, const int mthlim[3]);
+
+/* fc2d_geoclaw_flux2 (geoclaw_flux2fw_fort.f:4-269).  dtdx1d is modified in
+ * place as in the reference. */
+void gfo_flux2(const gfo_params *p, int ixy, int mx, int mbc, int maux,
+               double *q1d, double *dtdx1d,
+               const double *aux1, const double *aux2, const double *aux3,
+               double *faddm, double *faddp, double *gaddm, double *gaddp,
+               double *cfl1d);
+
+/* fc2d_geoclaw_step2 (geoclaw_step2_fort.f90:1-294) followed by the update in
+ * fc2d_geoclaw_step2_wrap (geoclaw_step2_wrap_fort.f:1-117).  qold is updated
+ * in place (ghost cells included, as the reference does); returns cflgrid. */
+double gfo_step2_wrap(const gfo_params *p, int mx, int my, int mbc, int maux,
+                      double *qold, double *aux, double dx, double dy, double dt);
+
+/* fc2d_geoclaw_b4step2 (geoclaw_b4step2_fort.f90:153-156), all cells. */
+void gfo_b4step2(const gfo_params *p, int mx, int my, int mbc, double *q);
+
+/* fc2d_geoclaw_src2 friction part (geoclaw_src2_fort.f90:41-71), interior. */
+void gfo_src2(const gfo_params *p, int mx, int my, int mbc, int maux,
+              double *q, const double *aux, double dt);
+
+/* fc2d_geoclaw_bc2 (geoflood_bc2_fort.f90) for mthbc in {-1 (none), 1
+ * (extrapolation), 3 (solid wall)}; sides left, right, bottom, top. */
+void gfo_bc2(int mx, int my, int mbc, int maux, double *q, double *aux,
+             const int mthbc[4]);
+
+/* geoclaw_update (fc2d_cpucuda.cpp:362-406): b4step2, step2 (+update), the
+ * per-patch MAXCFL_CAP (1000 -> 0), then src2 when src_term > 0. */
+double gfo_patch_update(const gfo_params *p, int mx, int my, int mbc, int maux,
+                        double *q, double *aux, double dx, double dy,
+                        double t, double dt);
+
+/* ---------------------------------------------------------------------------
+ * Uniform single-level brick of mi x mj patches (the forest of the uniform
+ * configurations).  Patch (pi, pj) has index pj*mi + pi and its own AoS q and
+ * aux arrays with ghosts.  The host-side ghost update follows ForestClaw's
+ * order: same-level face copies, same-level corner copies, then bc2 on the
+ * patch sides that lie on the physical boundary (mthbc = -1 elsewhere).
+ * ------------------------------------------------------------------------- */
+typedef double (*gfo_update_fn)(const void *ctx, int mx, int my, int mbc, int maux,
+                                double *q, double *aux, double dx, double dy,
+                                double t, double dt);
+
+void gfo_brick_ghost_fill(int mi, int mj, int mx, int my, int mbc, int maux,
+                          double **q, double **aux, const int mthbc[4]);
+
+/* One step attempt over all patches (ghost fill, then every patch's update
+ * through fn(ctx, ...) or the C restatement when fn is NULL).  Returns the
+ * maximum of the per-patch capped CFL numbers.  nthreads > 1 uses OpenMP. */
+double gfo_brick_advance(const gfo_params *p, int mi, int mj, int mx, int my,
+                         int mbc, int maux, double **q, double **aux,
+                         const int mthbc[4], double dx, double dy, double t,
+                         double dt, gfo_update_fn fn, const void *ctx,
+                         int nthreads);
+
+/* outstyle_3 time loop (fc2d_geoclaw_run.c:417-590) over nsteps accepted
+ * steps with variable dt: retake when maxcfl > max_cfl, next dt =
+ * dt*desired_cfl/maxcfl.  Records per accepted step: dt used, maxcfl, the
+ * interior water volume sum(h)*dx*dy after the step.  Returns the number of
+ * retakes.  Arrays may be NULL. */
+int gfo_brick_run(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
+                  int maux, double **q, double **aux, const int mthbc[4],
+                  double dx, double dy, double t0, double initial_dt,
+                  double desired_cfl, double max_cfl, int nsteps,
+                  gfo_update_fn fn, const void *ctx, int nthreads,
+                  double *dt_out, double *cfl_out, double *mass_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,396 @@
+"""ctypes bindings for the CPU oracle.  TEST INFRASTRUCTURE ONLY.
+
+Two implementations with the same entry points:
+  * ``C``   -- oracle/liboracle.so, the C restatement (gf_oracle.c);
+  * ``REF`` -- oracle/_ref/libgfref.so, the reference GeoClaw Fortran step
+    compiled from its own sources (present only where oracle/Makefile's
+    ``ref`` target was built).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker.  Arrays follow the reference's host
+layout (Fortran order): q has shape (my+2mbc, mx+2mbc, 3) in C order, i.e.
+q[j, i, m], with Fortran index (m+1, i-mbc+1, j-mbc+1).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+C_LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_LIB_PATH = os.path.join(HERE, "_ref", "libgfref.so")
+
+_dp = ct.POINTER(ct.c_double)
+_ip = ct.POINTER(ct.c_int)
+UPDATE_FN = ct.CFUNCTYPE(ct.c_double, ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+                         _dp, _dp, ct.c_double, ct.c_double, ct.c_double, ct.c_double)
+
+
+class GfoParams(ct.Structure):
+    _fields_ = [("grav", ct.c_double), ("dry_tolerance", ct.c_double),
+                ("manning_n", ct.c_double), ("friction_depth", ct.c_double),
+                ("friction_forcing", ct.c_int), ("src_term", ct.c_int),
+                ("order", ct.c_int), ("transverse", ct.c_int),
+                ("mthlim", ct.c_int * 3)]
+
+
+@dataclass
+class Params:
+    """Physics/method parameters shared by the oracle and the product."""
+    grav: float = 9.81
+    dry_tolerance: float = 1e-3
+    manning_n: float = 0.025
+    friction_depth: float = 1.0e6
+    friction_forcing: int = 1
+    src_term: int = 1
+    order: int = 2
+    transverse: int = 2
+    mthlim: tuple = field(default=(4, 4, 4))
+
+    def to_c(self) -> GfoParams:
+        p = GfoParams()
+        p.grav, p.dry_tolerance = self.grav, self.dry_tolerance
+        p.manning_n, p.friction_depth = self.manning_n, self.friction_depth
+        p.friction_forcing, p.src_term = self.friction_forcing, self.src_term
+        p.order, p.transverse = self.order, self.transverse
+        for k in range(3):
+            p.mthlim[k] = self.mthlim[k]
+        return p
+
+
+def ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)
+
+
+class _Lib:
+    """Common wrappers over either implementation."""
+
+    def __init__(self, path: str, prefix: str):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = ct.CDLL(path)
+        self.prefix = prefix
+        self.params = Params()
+
+    def f(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    # --- point functions -------------------------------------------------
+    def riemanntype(self, hL, hR, uL, uR, drytol, g, maxiter=1):
+        out = np.zeros(3)
+        rare = np.zeros(2, dtype=np.int32)
+        fn = self.f("riemanntype")
+        fn.argtypes = [ct.c_double] * 4 + [ct.c_int, ct.c_double, ct.c_double, _dp, _ip]
+        fn(hL, hR, uL, uR, maxiter, drytol, g, ptr(out), iptr(rare))
+        return out, rare
+
+    def riemann_aug_jcp(self, vin, drytol, g, maxiter=1):
+        vin = np.ascontiguousarray(vin, dtype=np.float64)
+        sw = np.zeros(3)
+        fw = np.zeros(9)
+        fn = self.f("riemann_aug_jcp")
+        fn.argtypes = [ct.c_int, _dp, ct.c_double, ct.c_double, _dp, _dp]
+        fn(maxiter, ptr(vin), drytol, g, ptr(sw), ptr(fw))
+        return sw, fw.reshape(3, 3)  # fw[mw, m]
+
+    def philim(self, a, b, meth):
+        fn = self.f("philim")
+        fn.argtypes = [ct.c_double, ct.c_double, ct.c_int]
+        fn.restype = ct.c_double
+        return fn(a, b, meth)
+
+
+class CLib(_Lib):
+    def __init__(self, path: str = C_LIB_PATH):
+        super().__init__(path, "gfo_")
+        self._cp = self.params.to_c()
+
+    def set_params(self, params: Params):
+        self.params = params
+        self._cp = params.to_c()
+
+    def rpn2(self, ixy, mx, mbc, q1d, aux2, maux=1):
+        n = mx + 2 * mbc
+        fw = np.zeros((n, 3, 3)); s = np.zeros((n, 3))
+        amdq = np.zeros((n, 3)); apdq = np.zeros((n, 3))
+        fn = self.lib.gfo_rpn2
+        fn.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        fn(ct.byref(self._cp), ixy, mx, mbc, maux, ptr(q1d), ptr(aux2), ptr(fw), ptr(s), ptr(amdq), ptr(apdq))
+        return fw, s, amdq, apdq
+
+    def rpt2(self, ixy, imp, mx, mbc, q1d, aux1, aux2, aux3, asdq, maux=1):
+        n = mx + 2 * mbc
+        bm = np.zeros((n, 3)); bp = np.zeros((n, 3))
+        fn = self.lib.gfo_rpt2
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 5 + [_dp] * 7
+        fn(ct.byref(self._cp), ixy, imp, mx, mbc, maux, ptr(q1d), ptr(aux1), ptr(aux2), ptr(aux3),
+           ptr(asdq), ptr(bm), ptr(bp))
+        return bm, bp
+
+    def limiter(self, mx, mbc, wave, s, mthlim):
+        ml = np.ascontiguousarray(mthlim, dtype=np.int32)
+        fn = self.lib.gfo_limiter
+        fn.argtypes = [ct.c_int, ct.c_int, _dp, _dp, _ip]
+        fn(mx, mbc, ptr(wave), ptr(s), iptr(ml))
+
+    def flux2(self, ixy, mx, mbc, q1d, dtdx1d, aux1, aux2, aux3, maux=1):
+        n = mx + 2 * mbc
+        fm = np.zeros((n, 3)); fp = np.zeros((n, 3))
+        gm = np.zeros((2, n, 3)); gp = np.zeros((2, n, 3))
+        cfl = ct.c_double(0)
+        fn = self.lib.gfo_flux2
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 4 + [_dp] * 9 + [_dp]
+        fn(ct.byref(self._cp), ixy, mx, mbc, maux, ptr(q1d), ptr(dtdx1d), ptr(aux1), ptr(aux2),
+           ptr(aux3), ptr(fm), ptr(fp), ptr(gm), ptr(gp), ct.byref(cfl))
+        return fm, fp, gm, gp, cfl.value
+
+    def step2_wrap(self, mx, my, mbc, q, aux, dx, dy, dt, maux=1):
+        fn = self.lib.gfo_step2_wrap
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 4 + [_dp, _dp] + [ct.c_double] * 3
+        fn.restype = ct.c_double
+        return fn(ct.byref(self._cp), mx, my, mbc, maux, ptr(q), ptr(aux), dx, dy, dt)
+
+    def b4step2(self, mx, my, mbc, q, aux=None, dt=0.0):
+        fn = self.lib.gfo_b4step2
+        fn.argtypes = [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, _dp]
+        fn(ct.byref(self._cp), mx, my, mbc, ptr(q))
+
+    def src2(self, mx, my, mbc, q, aux, dt, maux=1):
+        fn = self.lib.gfo_src2
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 4 + [_dp, _dp, ct.c_double]
+        fn(ct.byref(self._cp), mx, my, mbc, maux, ptr(q), ptr(aux), dt)
+
+    def bc2(self, mx, my, mbc, q, aux, mthbc, maux=1):
+        mb = np.ascontiguousarray(mthbc, dtype=np.int32)
+        fn = self.lib.gfo_bc2
+        fn.argtypes = [ct.c_int] * 4 + [_dp, _dp, _ip]
+        fn(mx, my, mbc, maux, ptr(q), ptr(aux), iptr(mb))
+
+    def patch_update(self, mx, my, mbc, q, aux, dx, dy, t, dt, maux=1):
+        fn = self.lib.gfo_patch_update
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 4 + [_dp, _dp] + [ct.c_double] * 4
+        fn.restype = ct.c_double
+        return fn(ct.byref(self._cp), mx, my, mbc, maux, ptr(q), ptr(aux), dx, dy, t, dt)
+
+    # --- brick driver -------------------------------------------------------
+    def brick_run(self, brick: "Brick", nsteps, initial_dt, desired_cfl=0.9, max_cfl=1.0,
+                  t0=0.0, update=None, nthreads=1):
+        """Run the outstyle_3 loop on a Brick in place.  ``update`` is None
+        (C restatement) or a RefLib (reference Fortran patch update)."""
+        np_ = brick.mi * brick.mj
+        qp = (_dp * np_)(*[ptr(a) for a in brick.q])
+        ap = (_dp * np_)(*[ptr(a) for a in brick.aux])
+        mb = np.ascontiguousarray(brick.mthbc, dtype=np.int32)
+        dt_out = np.zeros(nsteps); cfl_out = np.zeros(nsteps); mass_out = np.zeros(nsteps)
+        fnp = None
+        if update is not None:
+            fnp = ct.cast(update.lib.gfref_patch_update, ct.c_void_p)
+        fn = self.lib.gfo_brick_run
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 6 + [ct.POINTER(_dp), ct.POINTER(_dp), _ip] + \
+            [ct.c_double] * 6 + [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int, _dp, _dp, _dp]
+        fn.restype = ct.c_int
+        retakes = fn(ct.byref(self._cp), brick.mi, brick.mj, brick.mx, brick.my, brick.mbc, 1,
+                     qp, ap, iptr(mb), brick.dx, brick.dy, t0, initial_dt, desired_cfl, max_cfl,
+                     nsteps, fnp, None, nthreads, ptr(dt_out), ptr(cfl_out), ptr(mass_out))
+        return dict(dt=dt_out, cfl=cfl_out, mass=mass_out, retakes=retakes)
+
+    def brick_advance(self, brick: "Brick", t, dt, update=None, nthreads=1):
+        np_ = brick.mi * brick.mj
+        qp = (_dp * np_)(*[ptr(a) for a in brick.q])
+        ap = (_dp * np_)(*[ptr(a) for a in brick.aux])
+        mb = np.ascontiguousarray(brick.mthbc, dtype=np.int32)
+        fnp = None if update is None else ct.cast(update.lib.gfref_patch_update, ct.c_void_p)
+        fn = self.lib.gfo_brick_advance
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 6 + [ct.POINTER(_dp), ct.POINTER(_dp), _ip] + \
+            [ct.c_double] * 4 + [ct.c_void_p, ct.c_void_p, ct.c_int]
+        fn.restype = ct.c_double
+        return fn(ct.byref(self._cp), brick.mi, brick.mj, brick.mx, brick.my, brick.mbc, 1,
+                  qp, ap, iptr(mb), brick.dx, brick.dy, t, dt, fnp, None, nthreads)
+
+    def brick_ghost_fill(self, brick: "Brick"):
+        np_ = brick.mi * brick.mj
+        qp = (_dp * np_)(*[ptr(a) for a in brick.q])
+        ap = (_dp * np_)(*[ptr(a) for a in brick.aux])
+        mb = np.ascontiguousarray(brick.mthbc, dtype=np.int32)
+        fn = self.lib.gfo_brick_ghost_fill
+        fn.argtypes = [ct.c_int] * 6 + [ct.POINTER(_dp), ct.POINTER(_dp), _ip]
+        fn(brick.mi, brick.mj, brick.mx, brick.my, brick.mbc, 1, qp, ap, iptr(mb))
+
+
+class RefLib(_Lib):
+    """The reference Fortran step (oracle/_ref/libgfref.so)."""
+
+    def __init__(self, path: str = REF_LIB_PATH):
+        super().__init__(path, "gfref_")
+
+    def set_params(self, params: Params):
+        self.params = params
+        ml = np.ascontiguousarray(params.mthlim, dtype=np.int32)
+        fn = self.lib.gfref_set_params
+        fn.argtypes = [ct.c_double] * 4 + [ct.c_int] * 4 + [_ip]
+        fn(params.grav, params.dry_tolerance, params.manning_n, params.friction_depth,
+           params.friction_forcing, params.src_term, params.order, params.transverse, iptr(ml))
+
+    def rpn2(self, ixy, mx, mbc, q1d, aux2, maux=1):
+        n = mx + 2 * mbc
+        fw = np.zeros((n, 3, 3)); s = np.zeros((n, 3))
+        amdq = np.zeros((n, 3)); apdq = np.zeros((n, 3))
+        fn = self.lib.gfref_rpn2
+        fn.argtypes = [ct.c_int] * 4 + [_dp] * 6
+        fn(ixy, mx, mbc, maux, ptr(q1d), ptr(aux2), ptr(fw), ptr(s), ptr(amdq), ptr(apdq))
+        return fw, s, amdq, apdq
+
+    def rpt2(self, ixy, imp, mx, mbc, q1d, aux1, aux2, aux3, asdq, maux=1):
+        n = mx + 2 * mbc
+        bm = np.zeros((n, 3)); bp = np.zeros((n, 3))
+        fn = self.lib.gfref_rpt2
+        fn.argtypes = [ct.c_int] * 5 + [_dp] * 7
+        fn(ixy, imp, mx, mbc, maux, ptr(q1d), ptr(aux1), ptr(aux2), ptr(aux3), ptr(asdq), ptr(bm), ptr(bp))
+        return bm, bp
+
+    def limiter(self, mx, mbc, wave, s, mthlim):
+        ml = np.ascontiguousarray(mthlim, dtype=np.int32)
+        fn = self.lib.gfref_limiter
+        fn.argtypes = [ct.c_int, ct.c_int, _dp, _dp, _ip]
+        fn(mx, mbc, ptr(wave), ptr(s), iptr(ml))
+
+    def flux2(self, ixy, mx, mbc, q1d, dtdx1d, aux1, aux2, aux3, maux=1):
+        n = mx + 2 * mbc
+        fm = np.zeros((n, 3)); fp = np.zeros((n, 3))
+        gm = np.zeros((2, n, 3)); gp = np.zeros((2, n, 3))
+        cfl = ct.c_double(0)
+        fn = self.lib.gfref_flux2
+        fn.argtypes = [ct.c_int] * 4 + [_dp] * 9 + [_dp]
+        fn(ixy, mx, mbc, maux, ptr(q1d), ptr(dtdx1d), ptr(aux1), ptr(aux2), ptr(aux3),
+           ptr(fm), ptr(fp), ptr(gm), ptr(gp), ct.byref(cfl))
+        return fm, fp, gm, gp, cfl.value
+
+    def step2_wrap(self, mx, my, mbc, q, aux, dx, dy, dt, maux=1):
+        fn = self.lib.gfref_step2_wrap
+        fn.argtypes = [ct.c_int] * 4 + [_dp, _dp] + [ct.c_double] * 3
+        fn.restype = ct.c_double
+        return fn(mx, my, mbc, maux, ptr(q), ptr(aux), dx, dy, dt)
+
+    def b4step2(self, mx, my, mbc, q, aux, dt=0.0, maux=1):
+        fn = self.lib.gfref_b4step2
+        fn.argtypes = [ct.c_int] * 4 + [_dp, _dp, ct.c_double]
+        fn(mx, my, mbc, maux, ptr(q), ptr(aux), dt)
+
+    def src2(self, mx, my, mbc, q, aux, dt, maux=1):
+        fn = self.lib.gfref_src2
+        fn.argtypes = [ct.c_int] * 4 + [_dp, _dp, ct.c_double]
+        fn(mx, my, mbc, maux, ptr(q), ptr(aux), dt)
+
+    def bc2(self, mx, my, mbc, q, aux, mthbc, maux=1):
+        mb = np.ascontiguousarray(mthbc, dtype=np.int32)
+        fn = self.lib.gfref_bc2
+        fn.argtypes = [ct.c_int] * 4 + [_dp, _dp, _ip]
+        fn(mx, my, mbc, maux, ptr(q), ptr(aux), iptr(mb))
+
+    def patch_update(self, mx, my, mbc, q, aux, dx, dy, t, dt, maux=1):
+        fn = self.lib.gfref_patch_update
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 4 + [_dp, _dp] + [ct.c_double] * 4
+        fn.restype = ct.c_double
+        return fn(None, mx, my, mbc, maux, ptr(q), ptr(aux), dx, dy, t, dt)
+
+
+@dataclass
+class Brick:
+    """A uniform single-level brick of mi x mj patches in the reference's host
+    layout: q[k] has shape (my+2mbc, mx+2mbc, 3), aux[k] (my+2mbc, mx+2mbc, 1),
+    for patch k = pj*mi + pi."""
+    mi: int
+    mj: int
+    mx: int
+    my: int
+    mbc: int
+    dx: float
+    dy: float
+    q: list
+    aux: list
+    mthbc: tuple = (3, 3, 3, 3)
+
+    @classmethod
+    def from_global(cls, h, hu, hv, b, mi, mj, mbc, dx, dy, mthbc=(3, 3, 3, 3)):
+        """Split global interior fields (shape (ny, nx)) into patches.  Ghost
+        values of aux are taken from a globally padded b with the wall/extrap
+        rule applied (bc2 then re-applies it on the boundary every step)."""
+        ny, nx = h.shape
+        mx, my = nx // mi, ny // mj
+        bpad = pad_aux(b, mbc, mthbc)
+        q, aux = [], []
+        for pj in range(mj):
+            for pi in range(mi):
+                qq = np.zeros((my + 2 * mbc, mx + 2 * mbc, 3))
+                sl = (slice(pj * my, (pj + 1) * my), slice(pi * mx, (pi + 1) * mx))
+                qq[mbc:mbc + my, mbc:mbc + mx, 0] = h[sl]
+                qq[mbc:mbc + my, mbc:mbc + mx, 1] = hu[sl]
+                qq[mbc:mbc + my, mbc:mbc + mx, 2] = hv[sl]
+                aa = np.ascontiguousarray(
+                    bpad[pj * my:pj * my + my + 2 * mbc, pi * mx:pi * mx + mx + 2 * mbc, None])
+                q.append(qq)
+                aux.append(aa)
+        return cls(mi, mj, mx, my, mbc, dx, dy, q, aux, tuple(mthbc))
+
+    def to_global(self):
+        """Interior fields (h, hu, hv) as global arrays of shape (ny, nx)."""
+        mx, my, mbc = self.mx, self.my, self.mbc
+        out = np.zeros((3, self.mj * my, self.mi * mx))
+        for pj in range(self.mj):
+            for pi in range(self.mi):
+                qq = self.q[pj * self.mi + pi]
+                out[:, pj * my:(pj + 1) * my, pi * mx:(pi + 1) * mx] = \
+                    np.moveaxis(qq[mbc:mbc + my, mbc:mbc + mx, :], 2, 0)
+        return out
+
+    def copy(self):
+        return Brick(self.mi, self.mj, self.mx, self.my, self.mbc, self.dx, self.dy,
+                     [a.copy() for a in self.q], [a.copy() for a in self.aux], self.mthbc)
+
+
+def pad_aux(b, mbc, mthbc=(3, 3, 3, 3)):
+    """Globally padded bathymetry: walls (3) mirror, extrapolation (1) copies
+    the edge cell, applied left, right, bottom, top as bc2 does."""
+    ny, nx = b.shape
+    out = np.zeros((ny + 2 * mbc, nx + 2 * mbc))
+    out[mbc:mbc + ny, mbc:mbc + nx] = b
+    for side in range(4):
+        kind = mthbc[side]
+        for k in range(1, mbc + 1):
+            if side == 0:
+                out[:, mbc - k] = out[:, mbc + k - 1] if kind == 3 else out[:, mbc]
+            elif side == 1:
+                out[:, mbc + nx - 1 + k] = out[:, mbc + nx - k] if kind == 3 else out[:, mbc + nx - 1]
+            elif side == 2:
+                out[mbc - k, :] = out[mbc + k - 1, :] if kind == 3 else out[mbc, :]
+            else:
+                out[mbc + ny - 1 + k, :] = out[mbc + ny - k, :] if kind == 3 else out[mbc + ny - 1, :]
+    return out
+
+
+_C = None
+_REF = None
+
+
+def c_lib() -> CLib:
+    global _C
+    if _C is None:
+        _C = CLib()
+    return _C
+
+
+def ref_lib():
+    """The reference Fortran build, or None where it was not built."""
+    global _REF
+    if _REF is None and os.path.exists(REF_LIB_PATH):
+        _REF = RefLib()
+    return _REF
