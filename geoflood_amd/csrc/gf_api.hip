@@ -1,0 +1,756 @@
+// gf_api.hip -- host runtime and C ABI of libgeoflood_amd.so.
+//
+// Owns the HIP stream, the device-resident patch pools, the pinned/device
+// staging used by the drop-in batch entry point, the hipGraph of one step
+// attempt, and the HIP-event timing of the fused step launches.  No HIP
+// allocation happens per step: pools and staging are sized at creation (the
+// drop-in staging grows only when a larger batch arrives, as buffer-len does
+// in the reference).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/geoflood_amd.h"
+#include "gf_launch.h"
+
+using gf::PatchDesc;
+using gf::ScenCtrl;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                       \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess) return fail(GF_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+struct Lib {
+    bool init = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    gf::Params P{};
+    gf_params user{};
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    std::vector<hipEvent_t> ev_step;  // pairs around step launches
+    double last_total_ms = 0.0, last_step_ms = 0.0;
+    int last_launches = 0;
+    // drop-in batch staging
+    gf_pool* batch_pool = nullptr;
+    double* h_stage = nullptr;  // pinned
+    size_t h_stage_bytes = 0;
+    double* d_max = nullptr;
+    double* h_max = nullptr;  // pinned
+};
+Lib g;
+
+constexpr size_t kChunk = 1024;  // patches per upload/download chunk
+
+}  // namespace
+
+struct gf_pool {
+    int mx = 0, my = 0, mbc = 2, capacity = 0, n = 0;
+    size_t np2 = 0;
+    double* q[2] = {nullptr, nullptr};
+    double* aux = nullptr;
+    PatchDesc* desc = nullptr;
+    double* cfl = nullptr;
+    double* hmax = nullptr;
+    ScenCtrl* ctrl = nullptr;
+    double* d_stage = nullptr;  // device staging (kChunk patches of q and aux)
+    double* h_stage = nullptr;  // pinned host staging
+    double* h_cfl = nullptr;    // pinned
+    std::vector<PatchDesc> h_desc;
+    std::vector<ScenCtrl> h_ctrl;
+    int nscen = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_dirty = true;
+};
+
+static void default_params()
+{
+    gf_params p{};
+    p.grav = 9.81;
+    p.dry_tolerance = 1e-3;
+    p.manning_n = 0.025;
+    p.friction_depth = 1.0e6;
+    p.friction_forcing = 1;
+    p.src_term = 1;
+    p.order = 2;
+    p.transverse = 2;
+    p.mthlim[0] = p.mthlim[1] = p.mthlim[2] = 4;
+    p.mcapa = 0;
+    g.user = p;
+    g.P.grav = p.grav;
+    g.P.dry_tolerance = p.dry_tolerance;
+    g.P.manning_n = p.manning_n;
+    g.P.friction_depth = p.friction_depth;
+    g.P.friction_forcing = p.friction_forcing;
+    g.P.src_term = p.src_term;
+    g.P.order = p.order;
+    g.P.transverse = p.transverse;
+    for (int k = 0; k < 3; k++) g.P.mthlim[k] = p.mthlim[k];
+}
+
+static int need_init()
+{
+    if (!g.init) return fail(GF_ESTATE, "gf_init has not been called");
+    return GF_OK;
+}
+
+static int ensure_step_events(size_t pairs)
+{
+    while (g.ev_step.size() < 2 * pairs) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        g.ev_step.push_back(e);
+    }
+    return GF_OK;
+}
+
+static gf::StepArgs step_args(gf_pool* p, int first, int count, double* q0, double* q1,
+                              ScenCtrl* ctrl, double dt, int cur)
+{
+    gf::StepArgs a{};
+    a.desc = p->desc;
+    a.list = nullptr;
+    a.first = first;
+    a.count = count;
+    a.q0 = q0;
+    a.q1 = q1;
+    a.aux = p->aux;
+    a.cfl_out = p->cfl;
+    a.hmax_out = p->hmax;
+    a.ctrl = ctrl;
+    a.dt = dt;
+    a.cur = cur;
+    a.P = g.P;
+    return a;
+}
+
+extern "C" {
+
+int gf_init(int device)
+{
+    int count = 0;
+    HIPCHK(hipGetDeviceCount(&count));
+    if (count <= 0) return fail(GF_EHIP, "no HIP device");
+    if (device < 0 || device >= count) return fail(GF_EINVAL, "device %d out of range [0,%d)", device, count);
+    if (g.init) {
+        if (device == g.device) return GF_OK;
+        return fail(GF_ESTATE, "already initialised on device %d", g.device);
+    }
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&g.ev_begin));
+    HIPCHK(hipEventCreate(&g.ev_end));
+    HIPCHK(hipMalloc(&g.d_max, sizeof(double)));
+    HIPCHK(hipHostMalloc(&g.h_max, sizeof(double), hipHostMallocDefault));
+    g.device = device;
+    default_params();
+    g.init = true;
+    return GF_OK;
+}
+
+int gf_finalize(void)
+{
+    if (!g.init) return GF_OK;
+    hipStreamSynchronize(g.stream);
+    if (g.batch_pool) gf_pool_destroy(g.batch_pool);
+    g.batch_pool = nullptr;
+    if (g.h_stage) hipHostFree(g.h_stage);
+    g.h_stage = nullptr;
+    g.h_stage_bytes = 0;
+    for (auto e : g.ev_step) hipEventDestroy(e);
+    g.ev_step.clear();
+    hipEventDestroy(g.ev_begin);
+    hipEventDestroy(g.ev_end);
+    hipFree(g.d_max);
+    hipHostFree(g.h_max);
+    hipStreamDestroy(g.stream);
+    g = Lib();
+    return GF_OK;
+}
+
+int gf_check_parameters(int mwaves)
+{
+    if (mwaves != 3) return fail(GF_EUNSUPPORTED, "mwaves = %d; the shallow-water solver has 3 waves", mwaves);
+    return GF_OK;
+}
+
+int gf_set_params(const gf_params* p)
+{
+    if (int r = need_init()) return r;
+    if (!p) return fail(GF_EINVAL, "params is NULL");
+    if (!(p->grav > 0.0)) return fail(GF_EINVAL, "grav must be > 0");
+    if (!(p->dry_tolerance >= 0.0)) return fail(GF_EINVAL, "dry_tolerance must be >= 0");
+    if (p->order != 1 && p->order != 2) return fail(GF_EINVAL, "order must be 1 or 2");
+    if (p->transverse < 0 || p->transverse > 2) return fail(GF_EINVAL, "transverse must be 0, 1 or 2");
+    if (p->mcapa != 0) return fail(GF_EUNSUPPORTED, "mcapa = %d; only mcapa = 0 is supported", p->mcapa);
+    for (int k = 0; k < 3; k++)
+        if (p->mthlim[k] < 0 || p->mthlim[k] > 5) return fail(GF_EINVAL, "mthlim[%d] = %d out of [0,5]", k, p->mthlim[k]);
+    g.user = *p;
+    g.P.grav = p->grav;
+    g.P.dry_tolerance = p->dry_tolerance;
+    g.P.manning_n = p->manning_n;
+    g.P.friction_depth = p->friction_depth;
+    g.P.friction_forcing = p->friction_forcing;
+    g.P.src_term = p->src_term;
+    g.P.order = p->order;
+    g.P.transverse = p->transverse;
+    for (int k = 0; k < 3; k++) g.P.mthlim[k] = p->mthlim[k];
+    return GF_OK;
+}
+
+int gf_get_params(gf_params* p)
+{
+    if (int r = need_init()) return r;
+    if (!p) return fail(GF_EINVAL, "params is NULL");
+    *p = g.user;
+    return GF_OK;
+}
+
+const char* gf_last_error(void) { return g_err.c_str(); }
+
+long long gf_pool_bytes(int mx, int my, int mbc, int capacity)
+{
+    const long long np2 = (long long)(mx + 2 * mbc) * (my + 2 * mbc);
+    const long long per = (2 * 3 * np2 + np2) * 8 + (long long)sizeof(PatchDesc) + 16 + sizeof(ScenCtrl);
+    const long long staging = (long long)kChunk * 4 * np2 * 8;
+    return per * capacity + staging;
+}
+
+gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
+{
+    if (need_init()) return nullptr;
+    if (mbc != gf::MBC) {
+        fail(GF_EUNSUPPORTED, "mbc = %d; only mbc = 2 is supported", mbc);
+        return nullptr;
+    }
+    if (!gf::supported_shape(mx, my)) {
+        fail(GF_EUNSUPPORTED, "patch shape %dx%d not supported (8x8, 16x16, 32x32)", mx, my);
+        return nullptr;
+    }
+    if (capacity <= 0) {
+        fail(GF_EINVAL, "capacity must be > 0");
+        return nullptr;
+    }
+    gf_pool* p = new gf_pool();
+    p->mx = mx;
+    p->my = my;
+    p->mbc = mbc;
+    p->capacity = capacity;
+    p->np2 = (size_t)(mx + 2 * mbc) * (my + 2 * mbc);
+    const size_t qbytes = (size_t)capacity * 3 * p->np2 * sizeof(double);
+    const size_t chunk = std::min<size_t>(kChunk, capacity);
+    bool ok = hipMalloc(&p->q[0], qbytes) == hipSuccess && hipMalloc(&p->q[1], qbytes) == hipSuccess &&
+              hipMalloc(&p->aux, (size_t)capacity * p->np2 * sizeof(double)) == hipSuccess &&
+              hipMalloc(&p->desc, (size_t)capacity * sizeof(PatchDesc)) == hipSuccess &&
+              hipMalloc(&p->cfl, (size_t)capacity * sizeof(double)) == hipSuccess &&
+              hipMalloc(&p->hmax, (size_t)capacity * sizeof(double)) == hipSuccess &&
+              hipMalloc(&p->ctrl, (size_t)capacity * sizeof(ScenCtrl)) == hipSuccess &&
+              hipMalloc(&p->d_stage, chunk * 4 * p->np2 * sizeof(double)) == hipSuccess &&
+              hipHostMalloc(&p->h_stage, chunk * 4 * p->np2 * sizeof(double), hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc(&p->h_cfl, (size_t)capacity * sizeof(double), hipHostMallocDefault) == hipSuccess;
+    if (ok)
+        ok = hipMemsetAsync(p->q[0], 0, qbytes, g.stream) == hipSuccess &&
+             hipMemsetAsync(p->q[1], 0, qbytes, g.stream) == hipSuccess &&
+             hipMemsetAsync(p->aux, 0, (size_t)capacity * p->np2 * sizeof(double), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->desc, 0, (size_t)capacity * sizeof(PatchDesc), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->cfl, 0, (size_t)capacity * sizeof(double), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->hmax, 0, (size_t)capacity * sizeof(double), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->ctrl, 0, (size_t)capacity * sizeof(ScenCtrl), g.stream) == hipSuccess &&
+             hipStreamSynchronize(g.stream) == hipSuccess;
+    if (!ok) {
+        fail(GF_ENOMEM, "pool allocation of %d patches (%lld bytes) failed", capacity,
+             gf_pool_bytes(mx, my, mbc, capacity));
+        gf_pool_destroy(p);
+        return nullptr;
+    }
+    p->h_desc.resize(capacity);
+    std::memset(p->h_desc.data(), 0, sizeof(PatchDesc) * capacity);
+    return p;
+}
+
+int gf_pool_destroy(gf_pool* p)
+{
+    if (!p) return GF_OK;
+    if (g.stream) hipStreamSynchronize(g.stream);
+    if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    hipFree(p->q[0]);
+    hipFree(p->q[1]);
+    hipFree(p->aux);
+    hipFree(p->desc);
+    hipFree(p->cfl);
+    hipFree(p->hmax);
+    hipFree(p->ctrl);
+    hipFree(p->d_stage);
+    if (p->h_stage) hipHostFree(p->h_stage);
+    if (p->h_cfl) hipHostFree(p->h_cfl);
+    delete p;
+    return GF_OK;
+}
+
+int gf_pool_set_patches(gf_pool* p, int n, const double* dx, const double* dy, const int* scen,
+                        const int* nbr)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (n < 0 || n > p->capacity) return fail(GF_EINVAL, "n = %d outside [0, %d]", n, p->capacity);
+    if (n > 0 && (!dx || !dy)) return fail(GF_EINVAL, "dx/dy are NULL");
+    for (int k = 0; k < n; k++) {
+        PatchDesc& d = p->h_desc[k];
+        if (!(dx[k] > 0.0) || !(dy[k] > 0.0)) return fail(GF_EINVAL, "patch %d: dx, dy must be > 0", k);
+        d.dx = dx[k];
+        d.dy = dy[k];
+        d.scen = scen ? scen[k] : -1;
+        if (d.scen < -1) return fail(GF_EINVAL, "patch %d: scenario id %d invalid", k, d.scen);
+        for (int s = 0; s < 8; s++) {
+            const int v = nbr ? nbr[8 * k + s] : GF_NB_STORED;
+            if (v >= n || v < GF_NB_STORED)
+                return fail(GF_EINVAL, "patch %d: neighbour entry %d = %d invalid", k, s, v);
+            d.nbr[s] = v;
+        }
+        d.pad = 0;
+    }
+    p->n = n;
+    if (n > 0) HIPCHK(hipMemcpyAsync(p->desc, p->h_desc.data(), sizeof(PatchDesc) * n, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    p->graph_dirty = true;
+    return GF_OK;
+}
+
+int gf_pool_upload(gf_pool* p, int first, int n, int buf, const double* q_aos, const double* aux_aos, int maux)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (first < 0 || n < 0 || first + n > p->capacity) return fail(GF_EINVAL, "slots [%d,%d) outside the pool", first, first + n);
+    if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
+    if (n > 0 && !q_aos) return fail(GF_EINVAL, "q is NULL");
+    if (aux_aos && maux < 1) return fail(GF_EINVAL, "maux must be >= 1");
+    const size_t np2 = p->np2;
+    const size_t chunk = std::min<size_t>(kChunk, p->capacity);
+    for (int off = 0; off < n; off += (int)chunk) {
+        const int m = std::min<int>((int)chunk, n - off);
+        const size_t qd = (size_t)m * 3 * np2;
+        const size_t ad = aux_aos ? (size_t)m * maux * np2 : 0;
+        std::memcpy(p->h_stage, q_aos + (size_t)off * 3 * np2, qd * sizeof(double));
+        if (aux_aos) std::memcpy(p->h_stage + qd, aux_aos + (size_t)off * maux * np2, ad * sizeof(double));
+        HIPCHK(hipMemcpyAsync(p->d_stage, p->h_stage, (qd + ad) * sizeof(double), hipMemcpyHostToDevice, g.stream));
+        HIPCHK(gf::launch_pack(p->d_stage, aux_aos ? p->d_stage + qd : nullptr, p->q[buf], p->aux,
+                               first + off, m, (int)np2, maux, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+    }
+    return GF_OK;
+}
+
+int gf_pool_download(gf_pool* p, int first, int n, int buf, double* q_aos)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (first < 0 || n < 0 || first + n > p->capacity) return fail(GF_EINVAL, "slots [%d,%d) outside the pool", first, first + n);
+    if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
+    if (n > 0 && !q_aos) return fail(GF_EINVAL, "q is NULL");
+    const size_t np2 = p->np2;
+    const size_t chunk = std::min<size_t>(kChunk, p->capacity);
+    for (int off = 0; off < n; off += (int)chunk) {
+        const int m = std::min<int>((int)chunk, n - off);
+        const size_t qd = (size_t)m * 3 * np2;
+        HIPCHK(gf::launch_unpack(p->d_stage, p->q[buf], first + off, m, (int)np2, g.stream));
+        HIPCHK(hipMemcpyAsync(p->h_stage, p->d_stage, qd * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        std::memcpy(q_aos + (size_t)off * 3 * np2, p->h_stage, qd * sizeof(double));
+    }
+    return GF_OK;
+}
+
+int gf_pool_ghost_fill(gf_pool* p, int first, int n, int buf)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (first < 0 || n < 0 || first + n > p->n) return fail(GF_EINVAL, "patches [%d,%d) outside [0,%d)", first, first + n, p->n);
+    if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
+    gf::GhostArgs a{p->desc, first, n, p->q[0], p->q[1], nullptr, buf};
+    HIPCHK(gf::launch_ghost(p->mx, p->my, a, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+}
+
+static int pool_step_impl(gf_pool* p, int first, int n, int src, int dst, double dt, double* cfl,
+                          double* maxcfl, bool plain)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (first < 0 || n < 0 || first + n > p->n) return fail(GF_EINVAL, "patches [%d,%d) outside [0,%d)", first, first + n, p->n);
+    if ((src != 0 && src != 1) || (dst != 0 && dst != 1)) return fail(GF_EINVAL, "buffers must be 0 or 1");
+    if (!(dt > 0.0)) return fail(GF_EINVAL, "dt must be > 0");
+    double* q0 = (src == dst) ? p->q[src] : p->q[0];
+    double* q1 = (src == dst) ? nullptr : p->q[1];
+    const int cur = (src == dst) ? 0 : src;
+    const int launches = plain ? n : (n > 0 ? 1 : 0);
+    if (int r = ensure_step_events(std::max(1, launches))) return r;
+    HIPCHK(hipEventRecord(g.ev_begin, g.stream));
+    if (plain) {
+        for (int k = 0; k < n; k++) {
+            HIPCHK(hipEventRecord(g.ev_step[2 * k], g.stream));
+            HIPCHK(gf::launch_step(p->mx, p->my, step_args(p, first + k, 1, q0, q1, nullptr, dt, cur), g.stream));
+            HIPCHK(hipEventRecord(g.ev_step[2 * k + 1], g.stream));
+        }
+    } else if (n > 0) {
+        HIPCHK(hipEventRecord(g.ev_step[0], g.stream));
+        HIPCHK(gf::launch_step(p->mx, p->my, step_args(p, first, n, q0, q1, nullptr, dt, cur), g.stream));
+        HIPCHK(hipEventRecord(g.ev_step[1], g.stream));
+    }
+    if (n > 0) HIPCHK(hipMemcpyAsync(p->h_cfl, p->cfl + first, sizeof(double) * n, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipEventRecord(g.ev_end, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, g.ev_begin, g.ev_end));
+    g.last_total_ms = ms;
+    g.last_step_ms = 0.0;
+    for (int k = 0; k < launches; k++) {
+        HIPCHK(hipEventElapsedTime(&ms, g.ev_step[2 * k], g.ev_step[2 * k + 1]));
+        g.last_step_ms += ms;
+    }
+    g.last_launches = launches;
+    double m = 0.0;
+    for (int k = 0; k < n; k++) {
+        if (cfl) cfl[k] = p->h_cfl[k];
+        m = p->h_cfl[k] > m ? p->h_cfl[k] : m;
+    }
+    if (maxcfl) *maxcfl = m;
+    return GF_OK;
+}
+
+int gf_pool_step(gf_pool* p, int first, int n, int src, int dst, double dt, double* cfl, double* maxcfl)
+{
+    return pool_step_impl(p, first, n, src, dst, dt, cfl, maxcfl, false);
+}
+
+int gf_pool_step_plain(gf_pool* p, int first, int n, int src, int dst, double dt, double* cfl, double* maxcfl)
+{
+    return pool_step_impl(p, first, n, src, dst, dt, cfl, maxcfl, true);
+}
+
+int gf_pool_add_scenario(gf_pool* p, int first_patch, int npatch, double t0, double dt0,
+                         double desired_cfl, double max_cfl, int target_steps, double t_end)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (first_patch < 0 || npatch <= 0 || first_patch + npatch > p->n)
+        return fail(GF_EINVAL, "scenario patches [%d,%d) outside [0,%d)", first_patch, first_patch + npatch, p->n);
+    if (!(dt0 > 0.0) || !(desired_cfl > 0.0) || !(max_cfl > 0.0)) return fail(GF_EINVAL, "dt0, desired_cfl, max_cfl must be > 0");
+    if (p->nscen >= p->capacity) return fail(GF_EINVAL, "too many scenarios");
+    const int id = p->nscen;
+    ScenCtrl c{};
+    c.t = t0;
+    c.dt = dt0;
+    c.maxcfl = 0.0;
+    c.desired_cfl = desired_cfl;
+    c.max_cfl = max_cfl;
+    c.t_end = t_end;
+    c.last_dt = 0.0;
+    c.cur = 0;
+    c.nsteps = 0;
+    c.target_steps = target_steps;
+    c.retakes = 0;
+    c.active = 1;
+    c.first_patch = first_patch;
+    c.npatch = npatch;
+    c.attempts = 0;
+    p->h_ctrl.resize(id + 1);
+    p->h_ctrl[id] = c;
+    for (int k = first_patch; k < first_patch + npatch; k++) p->h_desc[k].scen = id;
+    HIPCHK(hipMemcpyAsync(p->desc + first_patch, p->h_desc.data() + first_patch, sizeof(PatchDesc) * npatch,
+                          hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(p->ctrl + id, &p->h_ctrl[id], sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    p->nscen = id + 1;
+    p->graph_dirty = true;
+    return id;
+}
+
+int gf_pool_clear_scenarios(gf_pool* p)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    p->nscen = 0;
+    p->h_ctrl.clear();
+    HIPCHK(hipMemsetAsync(p->ctrl, 0, sizeof(ScenCtrl) * p->capacity, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    p->graph_dirty = true;
+    return GF_OK;
+}
+
+int gf_pool_set_scenario_active(gf_pool* p, int s, int active)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (s < 0 || s >= p->nscen) return fail(GF_EINVAL, "scenario %d unknown", s);
+    HIPCHK(hipMemcpyAsync(&p->h_ctrl[s], p->ctrl + s, sizeof(ScenCtrl), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    p->h_ctrl[s].active = active ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(p->ctrl + s, &p->h_ctrl[s], sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+}
+
+int gf_pool_get_scenario(gf_pool* p, int s, gf_scenario_info* info)
+{
+    if (!p || !info) return fail(GF_EINVAL, "pool or info is NULL");
+    if (s < 0 || s >= p->nscen) return fail(GF_EINVAL, "scenario %d unknown", s);
+    ScenCtrl c;
+    HIPCHK(hipMemcpyAsync(&c, p->ctrl + s, sizeof(ScenCtrl), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    info->t = c.t;
+    info->dt = c.dt;
+    info->maxcfl = c.maxcfl;
+    info->last_dt = c.last_dt;
+    info->nsteps = c.nsteps;
+    info->retakes = c.retakes;
+    info->attempts = c.attempts;
+    info->active = c.active;
+    info->cur = c.cur;
+    return GF_OK;
+}
+
+// One attempt of every active scenario, enqueued on the stream.
+static int enqueue_attempt(gf_pool* p, int ev_pair)
+{
+    gf::GhostArgs ga{p->desc, 0, p->n, p->q[0], p->q[1], p->ctrl, 0};
+    HIPCHK(gf::launch_ghost(p->mx, p->my, ga, g.stream));
+    if (ev_pair >= 0) HIPCHK(hipEventRecord(g.ev_step[2 * ev_pair], g.stream));
+    HIPCHK(gf::launch_step(p->mx, p->my, step_args(p, 0, p->n, p->q[0], p->q[1], p->ctrl, 0.0, 0), g.stream));
+    if (ev_pair >= 0) HIPCHK(hipEventRecord(g.ev_step[2 * ev_pair + 1], g.stream));
+    HIPCHK(gf::launch_control(p->ctrl, p->nscen, p->cfl, g.stream));
+    return GF_OK;
+}
+
+static int any_active_flag(gf_pool* p, int* any_active)
+{
+    if (!any_active) return GF_OK;
+    p->h_ctrl.resize(p->nscen);
+    if (p->nscen > 0)
+        HIPCHK(hipMemcpyAsync(p->h_ctrl.data(), p->ctrl, sizeof(ScenCtrl) * p->nscen, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    int a = 0;
+    for (int s = 0; s < p->nscen; s++) a |= p->h_ctrl[s].active;
+    *any_active = a;
+    return GF_OK;
+}
+
+int gf_pool_run(gf_pool* p, int substeps, int use_graph, int* any_active)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (substeps < 0) return fail(GF_EINVAL, "substeps must be >= 0");
+    if (p->nscen == 0 || p->n == 0) return fail(GF_ESTATE, "no scenario registered");
+    if (use_graph) {
+        if (p->graph_dirty) {
+            if (p->graph_exec) HIPCHK(hipGraphExecDestroy(p->graph_exec));
+            if (p->graph) HIPCHK(hipGraphDestroy(p->graph));
+            p->graph_exec = nullptr;
+            p->graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal));
+            int r = enqueue_attempt(p, -1);
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamEndCapture(g.stream, &graph);
+            if (r != GF_OK) return r;
+            HIPCHK(e);
+            p->graph = graph;
+            HIPCHK(hipGraphInstantiate(&p->graph_exec, p->graph, nullptr, nullptr, 0));
+            p->graph_dirty = false;
+        }
+        HIPCHK(hipEventRecord(g.ev_begin, g.stream));
+        for (int k = 0; k < substeps; k++) HIPCHK(hipGraphLaunch(p->graph_exec, g.stream));
+        HIPCHK(hipEventRecord(g.ev_end, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, g.ev_begin, g.ev_end));
+        g.last_total_ms = ms;
+        g.last_step_ms = 0.0;
+        g.last_launches = 0;
+    } else {
+        if (int r = ensure_step_events(std::max(1, substeps))) return r;
+        HIPCHK(hipEventRecord(g.ev_begin, g.stream));
+        for (int k = 0; k < substeps; k++)
+            if (int r = enqueue_attempt(p, k)) return r;
+        HIPCHK(hipEventRecord(g.ev_end, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, g.ev_begin, g.ev_end));
+        g.last_total_ms = ms;
+        g.last_step_ms = 0.0;
+        for (int k = 0; k < substeps; k++) {
+            HIPCHK(hipEventElapsedTime(&ms, g.ev_step[2 * k], g.ev_step[2 * k + 1]));
+            g.last_step_ms += ms;
+        }
+        g.last_launches = substeps;
+    }
+    return any_active_flag(p, any_active);
+}
+
+int gf_pool_run_plain(gf_pool* p, int substeps, int* any_active)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (substeps < 0) return fail(GF_EINVAL, "substeps must be >= 0");
+    if (p->nscen == 0 || p->n == 0) return fail(GF_ESTATE, "no scenario registered");
+    std::vector<ScenCtrl>& hc = p->h_ctrl;
+    hc.resize(p->nscen);
+    HIPCHK(hipMemcpyAsync(hc.data(), p->ctrl, sizeof(ScenCtrl) * p->nscen, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    double total_ms = 0.0, step_ms = 0.0;
+    int launches = 0;
+    for (int it = 0; it < substeps; it++) {
+        int nact = 0;
+        for (int s = 0; s < p->nscen; s++) nact += hc[s].active;
+        if (nact == 0) break;
+        // patches of active scenarios, one launch each
+        int cnt = 0;
+        for (int s = 0; s < p->nscen; s++)
+            if (hc[s].active) cnt += hc[s].npatch;
+        if (int r = ensure_step_events(cnt)) return r;
+        HIPCHK(hipEventRecord(g.ev_begin, g.stream));
+        int e = 0;
+        for (int s = 0; s < p->nscen; s++) {
+            if (!hc[s].active) continue;
+            gf::GhostArgs ga{p->desc, hc[s].first_patch, hc[s].npatch, p->q[0], p->q[1], nullptr, hc[s].cur};
+            HIPCHK(gf::launch_ghost(p->mx, p->my, ga, g.stream));
+            for (int k = hc[s].first_patch; k < hc[s].first_patch + hc[s].npatch; k++, e++) {
+                HIPCHK(hipEventRecord(g.ev_step[2 * e], g.stream));
+                HIPCHK(gf::launch_step(p->mx, p->my,
+                                       step_args(p, k, 1, p->q[0], p->q[1], nullptr, hc[s].dt, hc[s].cur), g.stream));
+                HIPCHK(hipEventRecord(g.ev_step[2 * e + 1], g.stream));
+            }
+        }
+        HIPCHK(hipMemcpyAsync(p->h_cfl, p->cfl, sizeof(double) * p->n, hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipEventRecord(g.ev_end, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, g.ev_begin, g.ev_end));
+        total_ms += ms;
+        for (int k = 0; k < e; k++) {
+            HIPCHK(hipEventElapsedTime(&ms, g.ev_step[2 * k], g.ev_step[2 * k + 1]));
+            step_ms += ms;
+        }
+        launches += e;
+        // dt rule on the host (fc2d_geoclaw_run.c:529-565)
+        for (int s = 0; s < p->nscen; s++) {
+            ScenCtrl& c = hc[s];
+            if (!c.active) continue;
+            double m = 0.0;
+            for (int k = c.first_patch; k < c.first_patch + c.npatch; k++) m = p->h_cfl[k] > m ? p->h_cfl[k] : m;
+            c.maxcfl = m;
+            c.attempts += 1;
+            if (m > c.max_cfl) {
+                c.retakes += 1;
+                c.dt = c.dt * c.desired_cfl / m;
+            } else {
+                c.t = c.t + c.dt;
+                c.last_dt = c.dt;
+                c.cur ^= 1;
+                c.nsteps += 1;
+                c.dt = c.dt * c.desired_cfl / m;
+                if ((c.target_steps > 0 && c.nsteps >= c.target_steps) || (c.t_end > 0.0 && c.t >= c.t_end))
+                    c.active = 0;
+            }
+        }
+    }
+    HIPCHK(hipMemcpyAsync(p->ctrl, hc.data(), sizeof(ScenCtrl) * p->nscen, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    g.last_total_ms = total_ms;
+    g.last_step_ms = step_ms;
+    g.last_launches = launches;
+    if (any_active) {
+        int a = 0;
+        for (int s = 0; s < p->nscen; s++) a |= hc[s].active;
+        *any_active = a;
+    }
+    return GF_OK;
+}
+
+int gf_step2_batch(int mx, int my, int mbc, int maux, int n, double* const* q, double* const* aux,
+                   const double* dx, const double* dy, double t, double dt, double* maxcfl)
+{
+    (void)t;
+    if (int r = need_init()) return r;
+    if (n <= 0) return fail(GF_EINVAL, "batch size must be > 0");
+    if (!q || !aux || !dx || !dy) return fail(GF_EINVAL, "NULL argument");
+    if (maux < 1) return fail(GF_EINVAL, "maux must be >= 1");
+    if (!(dt > 0.0)) return fail(GF_EINVAL, "dt must be > 0");
+    gf_pool* p = g.batch_pool;
+    if (!p || p->mx != mx || p->my != my || p->capacity < n) {
+        if (p) gf_pool_destroy(p);
+        g.batch_pool = nullptr;
+        p = gf_pool_create(mx, my, mbc, n);
+        if (!p) return GF_ENOMEM;
+        g.batch_pool = p;
+    }
+    const size_t np2 = p->np2;
+    // patch descriptors: only dx, dy matter (ghost cells come from the host)
+    for (int k = 0; k < n; k++) {
+        PatchDesc& d = p->h_desc[k];
+        if (!(dx[k] > 0.0) || !(dy[k] > 0.0)) return fail(GF_EINVAL, "patch %d: dx, dy must be > 0", k);
+        d.dx = dx[k];
+        d.dy = dy[k];
+        d.scen = 0;
+        for (int s = 0; s < 8; s++) d.nbr[s] = GF_NB_STORED;
+    }
+    p->n = n;
+    HIPCHK(hipMemcpyAsync(p->desc, p->h_desc.data(), sizeof(PatchDesc) * n, hipMemcpyHostToDevice, g.stream));
+    // pack: host patches -> contiguous pinned staging -> device SoA (chunked)
+    const size_t chunk = std::min<size_t>(kChunk, p->capacity);
+    for (int off = 0; off < n; off += (int)chunk) {
+        const int m = std::min<int>((int)chunk, n - off);
+        const size_t qd = (size_t)m * 3 * np2;
+        HIPCHK(hipStreamSynchronize(g.stream));
+        for (int k = 0; k < m; k++) {
+            std::memcpy(p->h_stage + (size_t)k * 3 * np2, q[off + k], 3 * np2 * sizeof(double));
+            std::memcpy(p->h_stage + qd + (size_t)k * maux * np2, aux[off + k], maux * np2 * sizeof(double));
+        }
+        HIPCHK(hipMemcpyAsync(p->d_stage, p->h_stage, (qd + (size_t)m * maux * np2) * sizeof(double),
+                              hipMemcpyHostToDevice, g.stream));
+        HIPCHK(gf::launch_pack(p->d_stage, p->d_stage + qd, p->q[0], p->aux, off, m, (int)np2, maux, g.stream));
+    }
+    HIPCHK(gf::launch_step(mx, my, step_args(p, 0, n, p->q[0], nullptr, nullptr, dt, 0), g.stream));
+    HIPCHK(gf::launch_max(p->cfl, n, g.d_max, g.stream));
+    HIPCHK(hipMemcpyAsync(g.h_max, g.d_max, sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    for (int off = 0; off < n; off += (int)chunk) {
+        const int m = std::min<int>((int)chunk, n - off);
+        const size_t qd = (size_t)m * 3 * np2;
+        HIPCHK(gf::launch_unpack(p->d_stage, p->q[0], off, m, (int)np2, g.stream));
+        HIPCHK(hipMemcpyAsync(p->h_stage, p->d_stage, qd * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        for (int k = 0; k < m; k++)
+            std::memcpy(q[off + k], p->h_stage + (size_t)k * 3 * np2, 3 * np2 * sizeof(double));
+    }
+    if (maxcfl) *maxcfl = *g.h_max;
+    return GF_OK;
+}
+
+int gf_synchronize(void)
+{
+    if (int r = need_init()) return r;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+}
+
+int gf_last_timing(double* total_ms, double* step_ms, int* step_launches)
+{
+    if (total_ms) *total_ms = g.last_total_ms;
+    if (step_ms) *step_ms = g.last_step_ms;
+    if (step_launches) *step_launches = g.last_launches;
+    return GF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,757 @@
+// gf_step.hip -- fused FP64 patch step for gfx950 (MI355X).
+//
+// One workgroup advances one patch by one step:
+//   tile load (+ b4step2)  ->  x-faces: rpn2, MC limiter, flux corrections
+//   -> x-transverse rpt2   ->  y-faces: rpn2, limiter, corrections
+//   -> y-transverse rpt2   ->  conservative update + src2 (Manning)
+// together with the patch's CFL number (max |s| dt/dx over the reference's
+// face set, including the faces in the ghost rows/columns).
+//
+// Work decomposition.  The patch has (mx+3) x (my+3) "positions" (i, j),
+// Fortran indices i, j in [0, mx+2] x [0, my+2].  Position (i, j) owns x-face
+// (i, j) (between cells (i-1, j) and (i, j)), y-face (i, j) (between (i, j-1)
+// and (i, j)) and cell (i, j).  Thread t owns positions t, t+NT, ...; all
+// per-face and per-cell intermediates stay in the owning thread's registers,
+// and only what a neighbouring position needs crosses through LDS:
+//   unlimited f-waves (limiter), amdq* (transverse split of the face to the
+//   right/top), the G/F transverse increments of the cell below/left, and the
+//   final fm/gm of the face to the right/top.
+// Every floating-point value is formed with the same operations in the same
+// order as the reference Fortran (geoclaw_step2_fort.f90, geoclaw_flux2fw_
+// fort.f, rpn2_geoclaw.f, rpt2_geoclaw.f, geoclaw_src2_fort.f90), so the
+// results equal the reference's up to the sign of exact zeros and the last
+// bit of pow() in src2.
+#include <hip/hip_runtime.h>
+
+#include "gf_launch.h"
+
+namespace gf {
+
+template <int MX, int MY>
+struct Geo {
+    static constexpr int NXP = MX + 2 * MBC;
+    static constexpr int NYP = MY + 2 * MBC;
+    static constexpr int NP2 = NXP * NYP;
+    static constexpr int PX = MX + 3;
+    static constexpr int PY = MY + 3;
+    static constexpr int NPOS = PX * PY;
+    // LDS index of Fortran cell (fi, fj), fi in [-1, MX+2]
+    __device__ static constexpr int ti(int fi, int fj) { return (fj + 1) * NXP + (fi + 1); }
+};
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = fmaxf64(v, o);
+    }
+    return v;
+}
+
+// Compressed f-waves of one face: wave 1 (3), normal momentum of wave 2 (1),
+// wave 3 (3).  The other two components of wave 2 are exact zeros
+// (geoclaw_riemann_utils.f:289-299: fw(1,2) = beta(2)*r(2,2) = 0, fw(3,2) = 0).
+__device__ __forceinline__ void compress(const double w[3][3], double c[7])
+{
+    c[0] = w[0][0]; c[1] = w[0][1]; c[2] = w[0][2];
+    c[3] = w[1][1];
+    c[4] = w[2][0]; c[5] = w[2][1]; c[6] = w[2][2];
+}
+
+// Dot product of two 3-vectors given in direction-local order (h, n, t),
+// summed in the reference's physical component order (h, hu, hv):
+// x-direction (h, n, t) = (h, hu, hv); y-direction (h, n, t) = (h, hv, hu).
+template <int DIR>
+__device__ __forceinline__ double pdot(double a0, double a1, double a2, double b0, double b1, double b2)
+{
+    if (DIR == 0) return a0 * b0 + a1 * b1 + a2 * b2;
+    return a0 * b0 + a2 * b2 + a1 * b1;
+}
+
+// Limiter, second-order correction and fluctuation split of one face
+// (geoclaw_flux2fw_fort.f:143-224 with geoclaw_limiter_fort.f:34-58 and
+// philim.f).  wc: own compressed unlimited waves; wl / wr: those of the faces
+// before / after along the sweep.  Outputs in direction-local order: faddm,
+// faddp (flux increments) and amdq*, apdq* (fluctuations handed to rpt2).
+template <int DIR>
+__device__ __forceinline__ void face_second_order(const double wc[7], const double s[3],
+                                                  const double wl[7], const double wr[7],
+                                                  double dtdx, const Params& P,
+                                                  double faddm[3], double faddp[3],
+                                                  double amq[3], double apq[3])
+{
+    double w[3][3] = {{wc[0], wc[1], wc[2]}, {0.0, wc[3], 0.0}, {wc[4], wc[5], wc[6]}};
+    double amdq[3], apdq[3];
+    fluctuations(s, w, amdq, apdq);
+    if (P.order == 1) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            faddm[k] = amdq[k];
+            faddp[k] = -apdq[k];
+            amq[k] = amdq[k];
+            apq[k] = apdq[k];
+        }
+        return;
+    }
+    // limiter on the three wave families with the unlimited neighbours
+    {
+        // wave 1
+        if (P.mthlim[0] != 0) {
+            const double wn2 = pdot<DIR>(wc[0], wc[1], wc[2], wc[0], wc[1], wc[2]);
+            if (wn2 != 0.0) {
+                const double dotl = pdot<DIR>(wl[0], wl[1], wl[2], wc[0], wc[1], wc[2]);
+                const double dotr = pdot<DIR>(wc[0], wc[1], wc[2], wr[0], wr[1], wr[2]);
+                const double lim = philim(wn2, s[0] > 0.0 ? dotl : dotr, P.mthlim[0]);
+                w[0][0] = lim * w[0][0]; w[0][1] = lim * w[0][1]; w[0][2] = lim * w[0][2];
+            }
+        }
+        // wave 2 (only its normal-momentum component is non-zero)
+        if (P.mthlim[1] != 0) {
+            const double wn2 = wc[3] * wc[3];
+            if (wn2 != 0.0) {
+                const double dotl = wl[3] * wc[3];
+                const double dotr = wc[3] * wr[3];
+                const double lim = philim(wn2, s[1] > 0.0 ? dotl : dotr, P.mthlim[1]);
+                w[1][1] = lim * w[1][1];
+            }
+        }
+        // wave 3
+        if (P.mthlim[2] != 0) {
+            const double wn2 = pdot<DIR>(wc[4], wc[5], wc[6], wc[4], wc[5], wc[6]);
+            if (wn2 != 0.0) {
+                const double dotl = pdot<DIR>(wl[4], wl[5], wl[6], wc[4], wc[5], wc[6]);
+                const double dotr = pdot<DIR>(wc[4], wc[5], wc[6], wr[4], wr[5], wr[6]);
+                const double lim = philim(wn2, s[2] > 0.0 ? dotl : dotr, P.mthlim[2]);
+                w[2][0] = lim * w[2][0]; w[2][1] = lim * w[2][1]; w[2][2] = lim * w[2][2];
+            }
+        }
+    }
+    double c[3];
+#pragma unroll
+    for (int mw = 0; mw < 3; mw++) c[mw] = copysign(1.0, s[mw]) * (1.0 - fabs(s[mw]) * dtdx);
+    double cq[3];
+    cq[0] = c[0] * w[0][0] + c[2] * w[2][0];
+    cq[1] = c[0] * w[0][1] + c[1] * w[1][1] + c[2] * w[2][1];
+    cq[2] = c[0] * w[0][2] + c[2] * w[2][2];
+    const bool fold = (P.transverse == 2);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        faddm[k] = amdq[k] + 0.5 * cq[k];
+        faddp[k] = -apdq[k] + 0.5 * cq[k];
+        amq[k] = fold ? amdq[k] + cq[k] : amdq[k];
+        apq[k] = fold ? apdq[k] - cq[k] : apdq[k];
+    }
+}
+
+// Transverse increments of one cell from the two normal fluctuations that
+// its state splits (geoclaw_flux2fw_fort.f:232-265): G1 (below/left) and G2
+// (above/right), direction-local order, hd = dt/dx / 2.
+__device__ __forceinline__ void cell_transverse(const Rpt2Cell& C, const double am_next[3],
+                                                const double ap_own[3], double hd,
+                                                double G1[3], double G2[3])
+{
+    double bm1[3], bp1[3], bm2[3], bp2[3];
+    rpt2_apply(C, am_next, bm1, bp1);
+    rpt2_apply(C, ap_own, bm2, bp2);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        G1[k] = -(hd * bm1[k]) - hd * bm2[k];
+        G2[k] = -(hd * bp1[k]) - hd * bp2[k];
+    }
+}
+
+template <int MX, int MY, int NT>
+__global__ void __launch_bounds__(NT) k_step(StepArgs a)
+{
+    using G = Geo<MX, MY>;
+    constexpr int NXP = G::NXP, NP2 = G::NP2, PX = G::PX, NPOS = G::NPOS;
+    constexpr int SLOTS = (NPOS + NT - 1) / NT;
+    constexpr int NWAVE = NT / 64;
+
+    __shared__ double T[4 * NP2];  // h, hu, hv, b of the padded tile
+    __shared__ double E[7 * NPOS]; // exchange buffer, component-major
+    __shared__ double red[NWAVE];
+
+    const int tid = threadIdx.x;
+    const int bid = blockIdx.x;
+    if (bid >= a.count) return;
+    const int patch = a.list ? a.list[bid] : a.first + bid;
+    const PatchDesc d = a.desc[patch];
+    const Params& P = a.P;
+
+    double dt;
+    int cur;
+    if (a.ctrl) {
+        const ScenCtrl* c = a.ctrl + (d.scen < 0 ? 0 : d.scen);
+        if (d.scen < 0 || !c->active) {
+            if (tid == 0) {
+                a.cfl_out[patch] = 0.0;
+                if (a.hmax_out) a.hmax_out[patch] = 0.0;
+            }
+            return;
+        }
+        dt = c->dt;
+        cur = c->cur;
+    } else {
+        dt = a.dt;
+        cur = a.cur;
+    }
+    const double* src = (cur ? a.q1 : a.q0) + (size_t)patch * 3 * NP2;
+    double* dst = (a.q1 ? (cur ? a.q0 : a.q1) : a.q0) + (size_t)patch * 3 * NP2;
+    const double* auxp = a.aux + (size_t)patch * NP2;
+    const double g = P.grav, drytol = P.dry_tolerance;
+    const double dtdx = dt / d.dx;
+    const double dtdy = dt / d.dy;
+
+    // ---- tile load + b4step2 (geoclaw_b4step2_fort.f90:153-156) ----
+    int wet = 0;
+    for (int c = tid; c < NP2; c += NT) {
+        double h = src[c], hu = src[NP2 + c], hv = src[2 * NP2 + c];
+        if (h < drytol) {
+            h = fmaxf64(h, 0.0);
+            hu = 0.0;
+            hv = 0.0;
+        }
+        T[c] = h;
+        T[NP2 + c] = hu;
+        T[2 * NP2 + c] = hv;
+        T[3 * NP2 + c] = auxp[c];
+        wet |= (h > drytol);
+    }
+    const int anywet = __syncthreads_or(wet);
+
+    if (!anywet) {
+        // Every face of the tile is dry on both sides: all fluctuations, waves
+        // and transverse splits are zero and the update leaves q unchanged, so
+        // the step reduces to b4step2 + src2 on the interior and CFL = 0.
+        double hm = 0.0;
+        for (int c = tid; c < MX * MY; c += NT) {
+            const int i = c % MX + 1, j = c / MX + 1;
+            const int k = G::ti(i, j);
+            double h = T[k], hu = T[NP2 + k], hv = T[2 * NP2 + k];
+            if (P.src_term > 0) src2_cell(h, hu, hv, T[3 * NP2 + k], dt, P);
+            dst[k] = h;
+            dst[NP2 + k] = hu;
+            dst[2 * NP2 + k] = hv;
+            hm = fmaxf64(hm, h);
+        }
+        if (a.hmax_out) {
+            hm = wave_max(hm);
+            if ((tid & 63) == 0) red[tid >> 6] = hm;
+            __syncthreads();
+            if (tid == 0) {
+                double v = red[0];
+                for (int k = 1; k < NWAVE; k++) v = fmaxf64(v, red[k]);
+                a.hmax_out[patch] = v;
+            }
+        }
+        if (tid == 0) a.cfl_out[patch] = 0.0;
+        return;
+    }
+
+    int pi[SLOTS], pj[SLOTS];
+    bool in[SLOTS];
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int p = tid + s * NT;
+        in[s] = p < NPOS;
+        pi[s] = p % PX;
+        pj[s] = p / PX;
+    }
+
+    double cfl = 0.0;
+    double Wr[SLOTS][7], Sr[SLOTS][3];
+    double FXm[SLOTS][3], FXp[SLOTS][3], AP[SLOTS][3], AM[SLOTS][3], GA[SLOTS][3], GB[SLOTS][3];
+
+    // ================= x-faces: normal Riemann problems =================
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int i = pi[s], j = pj[s];
+        double sw[3] = {0.0, 0.0, 0.0};
+        double w[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+        if (in[s] && j <= MY + 1) {
+            const int kL = G::ti(i - 1, j), kR = G::ti(i, j);
+            rpn2_face(T[kL], T[NP2 + kL], T[2 * NP2 + kL], T[3 * NP2 + kL],
+                      T[kR], T[NP2 + kR], T[2 * NP2 + kR], T[3 * NP2 + kR], g, drytol, sw, w);
+#pragma unroll
+            for (int mw = 0; mw < 3; mw++) {
+                const double v = dtdx * sw[mw];
+                cfl = fmaxf64(cfl, v);
+                cfl = fmaxf64(cfl, -v);
+            }
+        }
+        compress(w, Wr[s]);
+        Sr[s][0] = sw[0]; Sr[s][1] = sw[1]; Sr[s][2] = sw[2];
+        if (in[s]) {
+            const int p = tid + s * NT;
+#pragma unroll
+            for (int k = 0; k < 7; k++) E[k * NPOS + p] = Wr[s][k];
+        }
+    }
+    __syncthreads();
+
+    // ================= x-faces: limiter + corrections =================
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int i = pi[s], j = pj[s];
+        const int p = tid + s * NT;
+        if (in[s] && i >= 1 && i <= MX + 1 && j <= MY + 1) {
+            double wl[7], wr[7];
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                wl[k] = E[k * NPOS + p - 1];
+                wr[k] = E[k * NPOS + p + 1];
+            }
+            face_second_order<0>(Wr[s], Sr[s], wl, wr, dtdx, P, FXm[s], FXp[s], AM[s], AP[s]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; k++) { FXm[s][k] = 0.0; FXp[s][k] = 0.0; AM[s][k] = 0.0; AP[s][k] = 0.0; }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++)
+        if (in[s]) {
+            const int p = tid + s * NT;
+#pragma unroll
+            for (int k = 0; k < 3; k++) E[k * NPOS + p] = AM[s][k];
+        }
+    __syncthreads();
+
+    // ================= x-transverse (cell-centred rpt2) =================
+    const bool transverse = P.transverse != 0;
+    const double hdx = 0.5 * dtdx;
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int i = pi[s], j = pj[s];
+        const int p = tid + s * NT;
+#pragma unroll
+        for (int k = 0; k < 3; k++) { GA[s][k] = 0.0; GB[s][k] = 0.0; }
+        if (transverse && in[s] && i >= 1 && i <= MX && j <= MY + 1) {
+            const int kc = G::ti(i, j);
+            const Rpt2Cell C = rpt2_setup(T[kc], T[NP2 + kc], T[2 * NP2 + kc], T[3 * NP2 + kc],
+                                          T[3 * NP2 + G::ti(i, j - 1)], T[3 * NP2 + G::ti(i, j + 1)],
+                                          g, drytol);
+            const double amn[3] = {E[p + 1], E[NPOS + p + 1], E[2 * NPOS + p + 1]};
+            cell_transverse(C, amn, AP[s], hdx, GA[s], GB[s]);  // GA = G1 (below), GB = G2 (above)
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++)
+        if (in[s]) {
+            const int p = tid + s * NT;
+#pragma unroll
+            for (int k = 0; k < 3; k++) E[k * NPOS + p] = GB[s][k];
+        }
+    __syncthreads();
+    // GT(i, J) = GX2(i, J-1) + GX1(i, J): the x-sweep part of gm and gp at y-face (i, J)
+    double GT[SLOTS][3];
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int i = pi[s], j = pj[s];
+        const int p = tid + s * NT;
+        if (in[s] && i >= 1 && i <= MX && j >= 1 && j <= MY + 1) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) GT[s][k] = E[k * NPOS + p - PX] + GA[s][k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; k++) GT[s][k] = 0.0;
+        }
+    }
+    __syncthreads();
+
+    // ================= y-faces: normal Riemann problems =================
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int i = pi[s], j = pj[s];
+        double sw[3] = {0.0, 0.0, 0.0};
+        double w[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+        if (in[s] && i <= MX + 1) {
+            const int kL = G::ti(i, j - 1), kR = G::ti(i, j);
+            // direction-local order for y: (h, hv, hu)
+            rpn2_face(T[kL], T[2 * NP2 + kL], T[NP2 + kL], T[3 * NP2 + kL],
+                      T[kR], T[2 * NP2 + kR], T[NP2 + kR], T[3 * NP2 + kR], g, drytol, sw, w);
+#pragma unroll
+            for (int mw = 0; mw < 3; mw++) {
+                const double v = dtdy * sw[mw];
+                cfl = fmaxf64(cfl, v);
+                cfl = fmaxf64(cfl, -v);
+            }
+        }
+        compress(w, Wr[s]);
+        Sr[s][0] = sw[0]; Sr[s][1] = sw[1]; Sr[s][2] = sw[2];
+        if (in[s]) {
+            const int p = tid + s * NT;
+#pragma unroll
+            for (int k = 0; k < 7; k++) E[k * NPOS + p] = Wr[s][k];
+        }
+    }
+    __syncthreads();
+
+    // ================= y-faces: limiter + corrections =================
+    double GMr[SLOTS][3], GPr[SLOTS][3];
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int i = pi[s], j = pj[s];
+        const int p = tid + s * NT;
+#pragma unroll
+        for (int k = 0; k < 3; k++) { GMr[s][k] = 0.0; GPr[s][k] = 0.0; AM[s][k] = 0.0; AP[s][k] = 0.0; }
+        if (in[s] && i <= MX + 1 && j >= 1 && j <= MY + 1) {
+            double wl[7], wr[7], fm[3], fp[3];
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                wl[k] = E[k * NPOS + p - PX];
+                wr[k] = E[k * NPOS + p + PX];
+            }
+            face_second_order<1>(Wr[s], Sr[s], wl, wr, dtdy, P, fm, fp, AM[s], AP[s]);
+            // gm = GT + faddm in physical order (h, hu, hv) = local (0, 2, 1)
+            GMr[s][0] = GT[s][0] + fm[0];
+            GMr[s][1] = GT[s][1] + fm[2];
+            GMr[s][2] = GT[s][2] + fm[1];
+            GPr[s][0] = GT[s][0] + fp[0];
+            GPr[s][1] = GT[s][1] + fp[2];
+            GPr[s][2] = GT[s][2] + fp[1];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++)
+        if (in[s]) {
+            const int p = tid + s * NT;
+#pragma unroll
+            for (int k = 0; k < 3; k++) E[k * NPOS + p] = AM[s][k];
+        }
+    __syncthreads();
+
+    // ================= y-transverse (cell-centred rpt2) =================
+    const double hdy = 0.5 * dtdy;
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int i = pi[s], j = pj[s];
+        const int p = tid + s * NT;
+#pragma unroll
+        for (int k = 0; k < 3; k++) { GA[s][k] = 0.0; GB[s][k] = 0.0; }
+        if (transverse && in[s] && i <= MX + 1 && j >= 1 && j <= MY) {
+            const int kc = G::ti(i, j);
+            const Rpt2Cell C = rpt2_setup(T[kc], T[2 * NP2 + kc], T[NP2 + kc], T[3 * NP2 + kc],
+                                          T[3 * NP2 + G::ti(i - 1, j)], T[3 * NP2 + G::ti(i + 1, j)],
+                                          g, drytol);
+            const double amn[3] = {E[p + PX], E[NPOS + p + PX], E[2 * NPOS + p + PX]};
+            cell_transverse(C, amn, AP[s], hdy, GA[s], GB[s]);  // GA = left (side 1), GB = right (side 2)
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++)
+        if (in[s]) {
+            const int p = tid + s * NT;
+#pragma unroll
+            for (int k = 0; k < 3; k++) E[k * NPOS + p] = GB[s][k];
+        }
+    __syncthreads();
+    // fm, fp at x-face (I, j) = (faddm_x + GY2(I-1, j)) + GY1(I, j), physical order
+    double FMr[SLOTS][3], FPr[SLOTS][3];
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int i = pi[s], j = pj[s];
+        const int p = tid + s * NT;
+#pragma unroll
+        for (int k = 0; k < 3; k++) { FMr[s][k] = 0.0; FPr[s][k] = 0.0; }
+        if (in[s] && i >= 1 && i <= MX + 1 && j >= 1 && j <= MY) {
+            const double g2[3] = {E[p - 1], E[2 * NPOS + p - 1], E[NPOS + p - 1]};  // local -> physical
+            const double g1[3] = {GA[s][0], GA[s][2], GA[s][1]};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                FMr[s][k] = FXm[s][k] + g2[k] + g1[k];
+                FPr[s][k] = FXp[s][k] + g2[k] + g1[k];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++)
+        if (in[s]) {
+            const int p = tid + s * NT;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                E[k * NPOS + p] = FMr[s][k];
+                E[(3 + k) * NPOS + p] = GMr[s][k];
+            }
+        }
+    __syncthreads();
+
+    // ================= conservative update + src2 =================
+    double hm = 0.0;
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        const int i = pi[s], j = pj[s];
+        const int p = tid + s * NT;
+        if (in[s] && i >= 1 && i <= MX && j >= 1 && j <= MY) {
+            const int k = G::ti(i, j);
+            double qn[3];
+#pragma unroll
+            for (int m = 0; m < 3; m++) {
+                const double fmr = E[m * NPOS + p + 1];
+                const double gmt = E[(3 + m) * NPOS + p + PX];
+                qn[m] = T[m * NP2 + k] - dtdx * (fmr - FPr[s][m]) - dtdy * (gmt - GPr[s][m]);
+            }
+            if (P.src_term > 0) src2_cell(qn[0], qn[1], qn[2], T[3 * NP2 + k], dt, P);
+            dst[k] = qn[0];
+            dst[NP2 + k] = qn[1];
+            dst[2 * NP2 + k] = qn[2];
+            hm = fmaxf64(hm, qn[0]);
+        }
+    }
+
+    // ================= CFL (and depth) reductions =================
+    cfl = wave_max(cfl);
+    if ((tid & 63) == 0) red[tid >> 6] = cfl;
+    __syncthreads();
+    if (tid == 0) {
+        double v = red[0];
+        for (int k = 1; k < NWAVE; k++) v = fmaxf64(v, red[k]);
+        if (v > 1000.0) v = 0.0;  // MAXCFL_CAP per patch (fc2d_cpucuda.cpp:392)
+        a.cfl_out[patch] = v;
+    }
+    if (a.hmax_out) {
+        __syncthreads();
+        hm = wave_max(hm);
+        if ((tid & 63) == 0) red[tid >> 6] = hm;
+        __syncthreads();
+        if (tid == 0) {
+            double v = red[0];
+            for (int k = 1; k < NWAVE; k++) v = fmaxf64(v, red[k]);
+            a.hmax_out[patch] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ghost fill on the device for same-level patches: each ghost cell takes the
+// value of the cell that ForestClaw's face copy, corner copy and bc2 pass
+// (left, right, bottom, top over full extents) would leave there.
+// ---------------------------------------------------------------------------
+
+template <int MX, int MY>
+__global__ void __launch_bounds__(256) k_ghost(GhostArgs a)
+{
+    using G = Geo<MX, MY>;
+    constexpr int NXP = G::NXP, NP2 = G::NP2;
+    const int bid = blockIdx.x;
+    if (bid >= a.count) return;
+    const int patch = a.first + bid;
+    const PatchDesc d = a.desc[patch];
+    int cur = a.cur;
+    if (a.ctrl) {
+        if (d.scen < 0) return;
+        const ScenCtrl* c = a.ctrl + d.scen;
+        if (!c->active) return;
+        cur = c->cur;
+    }
+    double* base = cur ? a.q1 : a.q0;
+    double* q = base + (size_t)patch * 3 * NP2;
+
+    for (int c = threadIdx.x; c < NP2; c += blockDim.x) {
+        const int fi = c % NXP - 1, fj = c / NXP - 1;  // Fortran indices
+        const int ox = fi < 1 ? -1 : (fi > MX ? 1 : 0);
+        const int oy = fj < 1 ? -1 : (fj > MY ? 1 : 0);
+        if (ox == 0 && oy == 0) continue;
+        const int nx = ox ? d.nbr[ox < 0 ? 0 : 1] : 0;
+        const int ny = oy ? d.nbr[oy < 0 ? 2 : 3] : 0;
+        // physical transforms along x / y
+        auto tx = [&](int v, int kind) {
+            if (kind == NB_WALL) return ox < 0 ? 1 - v : 2 * MX + 1 - v;
+            return ox < 0 ? 1 : MX;  // extrapolation
+        };
+        auto ty = [&](int v, int kind) {
+            if (kind == NB_WALL) return oy < 0 ? 1 - v : 2 * MY + 1 - v;
+            return oy < 0 ? 1 : MY;
+        };
+        int sp = -1, si = 0, sj = 0;
+        bool neg_u = false, neg_v = false;
+        if (ox != 0 && oy != 0) {
+            const int nc = d.nbr[4 + (ox > 0 ? 1 : 0) + (oy > 0 ? 2 : 0)];
+            const bool px = (nx == NB_WALL || nx == NB_EXTRAP);
+            const bool py = (ny == NB_WALL || ny == NB_EXTRAP);
+            if (nc >= 0) {
+                sp = nc; si = fi - ox * MX; sj = fj - oy * MY;
+            } else if (px && py) {
+                sp = patch; si = tx(fi, nx); sj = ty(fj, ny);
+                neg_u = (nx == NB_WALL); neg_v = (ny == NB_WALL);
+            } else if (px && ny >= 0) {
+                sp = ny; si = tx(fi, nx); sj = fj - oy * MY;
+                neg_u = (nx == NB_WALL);
+            } else if (py && nx >= 0) {
+                sp = nx; si = fi - ox * MX; sj = ty(fj, ny);
+                neg_v = (ny == NB_WALL);
+            }
+        } else if (ox != 0) {
+            if (nx >= 0) { sp = nx; si = fi - ox * MX; sj = fj; }
+            else if (nx == NB_WALL || nx == NB_EXTRAP) { sp = patch; si = tx(fi, nx); sj = fj; neg_u = (nx == NB_WALL); }
+        } else {
+            if (ny >= 0) { sp = ny; si = fi; sj = fj - oy * MY; }
+            else if (ny == NB_WALL || ny == NB_EXTRAP) { sp = patch; si = fi; sj = ty(fj, ny); neg_v = (ny == NB_WALL); }
+        }
+        if (sp < 0) continue;  // ghost supplied by the host (NB_STORED)
+        const double* s = base + (size_t)sp * 3 * NP2;
+        const int k = G::ti(si, sj);
+        const double h = s[k], hu = s[NP2 + k], hv = s[2 * NP2 + k];
+        q[c] = h;
+        q[NP2 + c] = neg_u ? -hu : hu;
+        q[2 * NP2 + c] = neg_v ? -hv : hv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Device-side dt controller (fc2d_geoclaw_run.c:504-565): one workgroup per
+// scenario reduces the patches' capped CFL numbers, then either accepts the
+// step (flip the q buffer, t += dt) or rejects it (retake), and sets the next
+// dt = dt * desired_cfl / maxcfl.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_control(ScenCtrl* ctrl, int nscen, const double* cfl)
+{
+    const int s = blockIdx.x;
+    if (s >= nscen) return;
+    ScenCtrl* c = ctrl + s;
+    if (!c->active) return;
+    __shared__ double red[4];
+    double v = 0.0;
+    for (int k = threadIdx.x; k < c->npatch; k += blockDim.x) v = fmaxf64(v, cfl[c->first_patch + k]);
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) m = fmaxf64(m, red[k]);
+        c->maxcfl = m;
+        c->attempts += 1;
+        if (m > c->max_cfl) {
+            c->retakes += 1;
+            c->dt = c->dt * c->desired_cfl / m;
+        } else {
+            c->t = c->t + c->dt;
+            c->last_dt = c->dt;
+            c->cur ^= 1;
+            c->nsteps += 1;
+            c->dt = c->dt * c->desired_cfl / m;
+            if ((c->target_steps > 0 && c->nsteps >= c->target_steps) ||
+                (c->t_end > 0.0 && c->t >= c->t_end))
+                c->active = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pack / unpack between the reference's host layout (AoS q(m,i,j) with ghost
+// cells, data_swap.f90:7-54) and the pool's SoA layout, one workgroup per
+// patch.  ``stage`` holds n consecutive AoS patches.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_pack(const double* stage_q, const double* stage_aux,
+                                              double* q, double* aux, int first, int n,
+                                              int np2, int maux)
+{
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    const double* sq = stage_q + (size_t)b * 3 * np2;
+    double* dq = q + (size_t)(first + b) * 3 * np2;
+    for (int e = threadIdx.x; e < 3 * np2; e += blockDim.x) {
+        const int c = e / 3, m = e - 3 * c;
+        dq[(size_t)m * np2 + c] = sq[e];
+    }
+    if (stage_aux) {
+        const double* sa = stage_aux + (size_t)b * maux * np2;
+        double* da = aux + (size_t)(first + b) * np2;
+        for (int c = threadIdx.x; c < np2; c += blockDim.x) da[c] = sa[(size_t)c * maux];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_unpack(double* stage_q, const double* q, int first,
+                                                int n, int np2)
+{
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    double* sq = stage_q + (size_t)b * 3 * np2;
+    const double* dq = q + (size_t)(first + b) * 3 * np2;
+    for (int e = threadIdx.x; e < 3 * np2; e += blockDim.x) {
+        const int c = e / 3, m = e - 3 * c;
+        sq[e] = dq[(size_t)m * np2 + c];
+    }
+}
+
+// Maximum of n values into out[0] (one workgroup; batch CFL of the drop-in
+// entry point).
+__global__ void __launch_bounds__(256) k_max(const double* v, int n, double* out)
+{
+    __shared__ double red[4];
+    double m = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) m = fmaxf64(m, v[k]);
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int k = 1; k < 4; k++) r = fmaxf64(r, red[k]);
+        out[0] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (declared in gf_launch.h)
+// ---------------------------------------------------------------------------
+bool supported_shape(int mx, int my) { return mx == my && (mx == 8 || mx == 16 || mx == 32); }
+
+hipError_t launch_step(int mx, int my, const StepArgs& a, hipStream_t st)
+{
+    if (a.count <= 0) return hipSuccess;
+    switch (mx * 1000 + my) {
+    case 32032: k_step<32, 32, 640><<<a.count, 640, 0, st>>>(a); break;
+    case 16016: k_step<16, 16, 384><<<a.count, 384, 0, st>>>(a); break;
+    case 8008: k_step<8, 8, 128><<<a.count, 128, 0, st>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ghost(int mx, int my, const GhostArgs& a, hipStream_t st)
+{
+    if (a.count <= 0) return hipSuccess;
+    switch (mx * 1000 + my) {
+    case 32032: k_ghost<32, 32><<<a.count, 256, 0, st>>>(a); break;
+    case 16016: k_ghost<16, 16><<<a.count, 256, 0, st>>>(a); break;
+    case 8008: k_ghost<8, 8><<<a.count, 256, 0, st>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_control(ScenCtrl* ctrl, int nscen, const double* cfl, hipStream_t st)
+{
+    if (nscen <= 0) return hipSuccess;
+    k_control<<<nscen, 256, 0, st>>>(ctrl, nscen, cfl);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const double* stage_q, const double* stage_aux, double* q, double* aux,
+                       int first, int n, int np2, int maux, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_pack<<<n, 256, 0, st>>>(stage_q, stage_aux, q, aux, first, n, np2, maux);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack(double* stage_q, const double* q, int first, int n, int np2, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_unpack<<<n, 256, 0, st>>>(stage_q, q, first, n, np2);
+    return hipGetLastError();
+}
+
+hipError_t launch_max(const double* v, int n, double* out, hipStream_t st)
+{
+    k_max<<<1, 256, 0, st>>>(v, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace gf

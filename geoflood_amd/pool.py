@@ -1,0 +1,238 @@
+"""Device-resident patch pools and the drop-in batch step (Python side of
+the C ABI in include/geoflood_amd.h).
+
+A *brick* is a uniform single-level block of mi x mj patches of mx x my cells
+(the forest of the uniform configurations and of each ensemble scenario).
+Global interior fields are arrays of shape (mj*my, mi*mx); patch (pi, pj) has
+pool slot first + pj*mi + pi.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import (GF_NB_EXTRAP, GF_NB_WALL, GfScenarioInfo, check, dptr, iptr, load)
+
+MBC = 2
+_KIND = {3: GF_NB_WALL, 1: GF_NB_EXTRAP}
+
+
+@dataclass
+class BrickLayout:
+    mi: int
+    mj: int
+    mx: int
+    my: int
+    dx: float
+    dy: float
+    mthbc: tuple = (3, 3, 3, 3)  # left, right, bottom, top (bc2 codes: 3 wall, 1 extrapolation)
+    first: int = 0
+
+    @property
+    def npatch(self) -> int:
+        return self.mi * self.mj
+
+    def neighbours(self) -> np.ndarray:
+        """(npatch, 8) neighbour table: left, right, bottom, top, left-bottom,
+        right-bottom, left-top, right-top; pool indices or GF_NB_* codes."""
+        mi, mj, f = self.mi, self.mj, self.first
+        pi, pj = np.meshgrid(np.arange(mi), np.arange(mj))
+        pi, pj = pi.ravel(), pj.ravel()
+        nb = np.empty((mi * mj, 8), dtype=np.int32)
+        kind = [_KIND[self.mthbc[s]] for s in range(4)]
+
+        def idx(a, b):
+            return f + b * mi + a
+
+        nb[:, 0] = np.where(pi > 0, idx(pi - 1, pj), kind[0])
+        nb[:, 1] = np.where(pi < mi - 1, idx(pi + 1, pj), kind[1])
+        nb[:, 2] = np.where(pj > 0, idx(pi, pj - 1), kind[2])
+        nb[:, 3] = np.where(pj < mj - 1, idx(pi, pj + 1), kind[3])
+        for c, (dxs, dys) in enumerate([(-1, -1), (1, -1), (-1, 1), (1, 1)]):
+            ai, bj = pi + dxs, pj + dys
+            ok = (ai >= 0) & (ai < mi) & (bj >= 0) & (bj < mj)
+            nb[:, 4 + c] = np.where(ok, idx(ai, bj), GF_NB_WALL)
+        return nb
+
+
+def pad_bathymetry(b: np.ndarray, mthbc=(3, 3, 3, 3), mbc: int = MBC) -> np.ndarray:
+    """Globally padded bathymetry with the ghost values bc2 leaves at the
+    domain boundary (walls mirror, extrapolation copies the edge cell;
+    sides applied left, right, bottom, top)."""
+    ny, nx = b.shape
+    out = np.zeros((ny + 2 * mbc, nx + 2 * mbc))
+    out[mbc:mbc + ny, mbc:mbc + nx] = b
+    for side in range(4):
+        wall = mthbc[side] == 3
+        for k in range(1, mbc + 1):
+            if side == 0:
+                out[:, mbc - k] = out[:, mbc + k - 1] if wall else out[:, mbc]
+            elif side == 1:
+                out[:, mbc + nx - 1 + k] = out[:, mbc + nx - k] if wall else out[:, mbc + nx - 1]
+            elif side == 2:
+                out[mbc - k, :] = out[mbc + k - 1, :] if wall else out[mbc, :]
+            else:
+                out[mbc + ny - 1 + k, :] = out[mbc + ny - k, :] if wall else out[mbc + ny - 1, :]
+    return out
+
+
+def to_patches(padded: np.ndarray, mi: int, mj: int, mx: int, my: int) -> np.ndarray:
+    """Split a globally padded field (ny+4, nx+4) into overlapping patch
+    tiles (mj*mi, my+4, mx+4)."""
+    w = np.lib.stride_tricks.sliding_window_view(padded, (my + 2 * MBC, mx + 2 * MBC))
+    return np.ascontiguousarray(w[::my, ::mx][:mj, :mi]).reshape(mi * mj, my + 2 * MBC, mx + 2 * MBC)
+
+
+def global_to_aos(h, hu, hv, b, layout: BrickLayout):
+    """Host AoS q (npatch, my+4, mx+4, 3) and aux (npatch, my+4, mx+4, 1) in
+    the reference's layout.  q ghost cells are left zero (the ghost fill sets
+    them); aux ghosts hold the padded bathymetry."""
+    mi, mj, mx, my = layout.mi, layout.mj, layout.mx, layout.my
+    assert h.shape == (mj * my, mi * mx)
+    q = np.zeros((mi * mj, my + 2 * MBC, mx + 2 * MBC, 3))
+    for m, f in enumerate((h, hu, hv)):
+        q[:, MBC:MBC + my, MBC:MBC + mx, m] = f.reshape(mj, my, mi, mx).transpose(0, 2, 1, 3).reshape(
+            mi * mj, my, mx)
+    aux = to_patches(pad_bathymetry(b, layout.mthbc), mi, mj, mx, my)[..., None]
+    return q, np.ascontiguousarray(aux)
+
+
+def aos_to_global(q: np.ndarray, layout: BrickLayout):
+    mi, mj, mx, my = layout.mi, layout.mj, layout.mx, layout.my
+    inner = q[:, MBC:MBC + my, MBC:MBC + mx, :]
+    out = inner.reshape(mj, mi, my, mx, 3).transpose(4, 0, 2, 1, 3).reshape(3, mj * my, mi * mx)
+    return out[0].copy(), out[1].copy(), out[2].copy()
+
+
+class Pool:
+    """A device-resident patch pool (gf_pool_* of the C ABI)."""
+
+    def __init__(self, mx: int, my: int, capacity: int, mbc: int = MBC):
+        self.lib = load()
+        self.mx, self.my, self.mbc, self.capacity = mx, my, mbc, capacity
+        h = self.lib.gf_pool_create(mx, my, mbc, capacity)
+        if not h:
+            raise MemoryError(self.lib.gf_last_error().decode())
+        self.h = ct.c_void_p(h)
+        self.layouts: list[BrickLayout] = []
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            self.lib.gf_pool_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def bytes_for(mx, my, capacity, mbc=MBC) -> int:
+        return int(load().gf_pool_bytes(mx, my, mbc, capacity))
+
+    # -- layout ---------------------------------------------------------------
+    def set_layout(self, layouts: list[BrickLayout]):
+        """Place bricks one after another in the pool and describe them."""
+        first = 0
+        dx, dy, nb = [], [], []
+        for L in layouts:
+            assert (L.mx, L.my) == (self.mx, self.my)
+            L.first = first
+            dx.append(np.full(L.npatch, L.dx))
+            dy.append(np.full(L.npatch, L.dy))
+            nb.append(L.neighbours())
+            first += L.npatch
+        if first > self.capacity:
+            raise ValueError(f"{first} patches exceed the pool capacity {self.capacity}")
+        self.layouts = list(layouts)
+        self.n = first
+        dxa = np.ascontiguousarray(np.concatenate(dx)) if dx else np.zeros(0)
+        dya = np.ascontiguousarray(np.concatenate(dy)) if dy else np.zeros(0)
+        nba = np.ascontiguousarray(np.concatenate(nb)) if nb else np.zeros((0, 8), np.int32)
+        sc = np.full(first, -1, dtype=np.int32)
+        check(self.lib.gf_pool_set_patches(self.h, first, dptr(dxa), dptr(dya), iptr(sc), iptr(nba)),
+              "gf_pool_set_patches")
+
+    # -- data movement --------------------------------------------------------
+    def upload_aos(self, first: int, q: np.ndarray, aux: np.ndarray | None, buf: int = 0):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        n = q.shape[0]
+        a = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+        check(self.lib.gf_pool_upload(self.h, first, n, buf, dptr(q), None if a is None else dptr(a), 1),
+              "gf_pool_upload")
+
+    def download_aos(self, first: int, n: int, buf: int = 0) -> np.ndarray:
+        q = np.zeros((n, self.my + 2 * MBC, self.mx + 2 * MBC, 3))
+        check(self.lib.gf_pool_download(self.h, first, n, buf, dptr(q)), "gf_pool_download")
+        return q
+
+    def upload_global(self, layout: BrickLayout, h, hu, hv, b, buf: int = 0):
+        q, aux = global_to_aos(h, hu, hv, b, layout)
+        self.upload_aos(layout.first, q, aux, buf)
+
+    def download_global(self, layout: BrickLayout, buf: int = 0):
+        return aos_to_global(self.download_aos(layout.first, layout.npatch, buf), layout)
+
+    # -- stepping ---------------------------------------------------------------
+    def ghost_fill(self, first: int = 0, n: int | None = None, buf: int = 0):
+        n = self.n - first if n is None else n
+        check(self.lib.gf_pool_ghost_fill(self.h, first, n, buf), "gf_pool_ghost_fill")
+
+    def step(self, dt: float, first: int = 0, n: int | None = None, src: int = 0, dst: int = 1,
+             plain: bool = False):
+        n = self.n - first if n is None else n
+        cfl = np.zeros(n)
+        m = ct.c_double(0)
+        fn = self.lib.gf_pool_step_plain if plain else self.lib.gf_pool_step
+        check(fn(self.h, first, n, src, dst, dt, dptr(cfl), ct.byref(m)), "gf_pool_step")
+        return m.value, cfl
+
+    def add_scenario(self, layout: BrickLayout, dt0: float, desired_cfl: float = 0.9,
+                     max_cfl: float = 1.0, target_steps: int = 0, t_end: float = 0.0,
+                     t0: float = 0.0) -> int:
+        sid = self.lib.gf_pool_add_scenario(self.h, layout.first, layout.npatch, t0, dt0,
+                                            desired_cfl, max_cfl, target_steps, t_end)
+        if sid < 0:
+            check(sid, "gf_pool_add_scenario")
+        return sid
+
+    def clear_scenarios(self):
+        check(self.lib.gf_pool_clear_scenarios(self.h), "gf_pool_clear_scenarios")
+
+    def set_active(self, sid: int, active: bool):
+        check(self.lib.gf_pool_set_scenario_active(self.h, sid, int(active)), "gf_pool_set_scenario_active")
+
+    def scenario(self, sid: int) -> dict:
+        info = GfScenarioInfo()
+        check(self.lib.gf_pool_get_scenario(self.h, sid, ct.byref(info)), "gf_pool_get_scenario")
+        return {k: getattr(info, k) for k, _ in GfScenarioInfo._fields_}
+
+    def run(self, substeps: int, graph: bool = True) -> bool:
+        act = ct.c_int(0)
+        check(self.lib.gf_pool_run(self.h, substeps, int(graph), ct.byref(act)), "gf_pool_run")
+        return bool(act.value)
+
+    def run_plain(self, substeps: int) -> bool:
+        act = ct.c_int(0)
+        check(self.lib.gf_pool_run_plain(self.h, substeps, ct.byref(act)), "gf_pool_run_plain")
+        return bool(act.value)
+
+
+def step2_batch(q_list, aux_list, dx, dy, t, dt):
+    """Drop-in batch step over host patches in the reference layout
+    (gf_step2_batch).  Arrays are updated in place; returns maxcfl."""
+    lib = load()
+    n = len(q_list)
+    my, mx = q_list[0].shape[0] - 2 * MBC, q_list[0].shape[1] - 2 * MBC
+    qp = (ct.POINTER(ct.c_double) * n)(*[dptr(a) for a in q_list])
+    ap = (ct.POINTER(ct.c_double) * n)(*[dptr(a) for a in aux_list])
+    dxa = np.ascontiguousarray(np.broadcast_to(np.asarray(dx, dtype=np.float64), (n,)))
+    dya = np.ascontiguousarray(np.broadcast_to(np.asarray(dy, dtype=np.float64), (n,)))
+    m = ct.c_double(0)
+    check(lib.gf_step2_batch(mx, my, MBC, aux_list[0].shape[-1], n, qp, ap, dptr(dxa), dptr(dya),
+                             t, dt, ct.byref(m)), "gf_step2_batch")
+    return m.value

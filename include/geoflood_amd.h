@@ -1,0 +1,213 @@
+/*
+ * geoflood_amd.h -- C ABI of the MI355X (gfx950) GeoFlood patch-step library
+ * (libgeoflood_amd.so).
+ *
+ * The library replaces the GPU route of GeoFlood's fc2d_cpu_cuda solver
+ * plugin: the per-patch finite-volume step (b4step2, rpn2 + rpt2, MC-limited
+ * flux2 with the conservative update, src2 Manning friction) and the CFL
+ * reduction, in FP64, behind plain C entry points.  Each declaration names
+ * the reference interface it replaces (paths relative to the GeoFlood tree).
+ *
+ * Conventions
+ *   - Every entry point returns an int status (GF_OK = 0, negative on error)
+ *     instead of calling exit() as the reference's CHECK macro does
+ *     (src/fc2d_cpu_cuda/fc2d_cudaclaw_check.h:3-11); gf_last_error() gives
+ *     the message of the last failure on the calling thread.
+ *   - Host patch arrays use the reference's layout: q(meqn=3, 1-mbc:mx+mbc,
+ *     1-mbc:my+mbc) and aux(maux, ...) in Fortran order, ghost cells included.
+ *   - Only mbc = 2, meqn = 3, mwaves = 3, mcapa = 0 and Cartesian coordinates
+ *     are supported (the configurations of the GeoFlood GPU route), with
+ *     (mx, my) in {(8,8), (16,16), (32,32)}.
+ */
+#ifndef GEOFLOOD_AMD_H
+#define GEOFLOOD_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GF_OK = 0,
+    GF_EINVAL = -1,       /* invalid argument */
+    GF_EHIP = -2,         /* HIP runtime error */
+    GF_ENOMEM = -3,       /* device or pinned host allocation failed */
+    GF_EUNSUPPORTED = -4, /* configuration outside the supported set */
+    GF_ESTATE = -5        /* call out of order (e.g. no gf_init) */
+};
+
+/* Neighbour codes for gf_pool_set_patches (values >= 0 are patch indices). */
+enum { GF_NB_WALL = -1, GF_NB_EXTRAP = -2, GF_NB_STORED = -3 };
+
+/* Physics and method parameters.  Replaces the __constant__ GeofloodVars of
+ * src/fc2d_cpu_cuda/riemann_source/variables.h:20-37 and the method symbols
+ * order/mthlim/use_fwaves of cuda_source/cudaclaw_flux2.cu:40-59. */
+typedef struct gf_params {
+    double grav;           /* geoclaw_module::grav */
+    double dry_tolerance;  /* geoclaw_module::dry_tolerance */
+    double manning_n;      /* manning_coefficient(1) */
+    double friction_depth; /* geoclaw_module::friction_depth */
+    int friction_forcing;  /* apply Manning friction in src2 */
+    int src_term;          /* call src2 (fc2d_cpucuda_options::src_term) */
+    int order;             /* method(2): 1 or 2 */
+    int transverse;        /* method(3): 0, 1 or 2 */
+    int mthlim[3];         /* limiter per wave family: 1 minmod .. 4 MC, 5 BW */
+    int mcapa;             /* capacity index; must be 0 */
+} gf_params;
+
+/* Snapshot of a scenario's device-side time-step control record. */
+typedef struct gf_scenario_info {
+    double t;        /* simulated time of the accepted state */
+    double dt;       /* dt of the next attempt */
+    double maxcfl;   /* max CFL number of the last attempt */
+    double last_dt;  /* dt of the last accepted step */
+    int nsteps;      /* accepted steps */
+    int retakes;     /* rejected attempts (maxcfl > max_cfl) */
+    int attempts;    /* all attempts */
+    int active;      /* 1 while stepping */
+    int cur;         /* q buffer holding the accepted state */
+} gf_scenario_info;
+
+typedef struct gf_pool gf_pool;
+
+/* ---- device and parameters ---------------------------------------------- */
+
+/* Select the device and create the library's stream.  Replaces
+ * fc2d_cudaclaw_initialize_GPUs (cuda_source/cudaclaw_initialize.cu:37-74). */
+int gf_init(int device);
+
+/* Release everything gf_init created. */
+int gf_finalize(void);
+
+/* Validate mwaves against the supported solver.  Replaces
+ * cudaclaw_check_parameters (fc2d_cpucuda_options.h:93,
+ * cuda_source/cudaclaw_flux2.cu:48-51). */
+int gf_check_parameters(int mwaves);
+
+/* Set physics/method parameters for subsequent steps.  Replaces setprob_cuda
+ * (riemann_source/variables.cu:72-113) and cudaclaw_set_method_parameters
+ * (fc2d_cpucuda_options.h:94, cuda_source/cudaclaw_flux2.cu:53-59). */
+int gf_set_params(const gf_params *params);
+
+/* Read back the parameters.  Replaces cudaclaw_get_method_parameters
+ * (fc2d_cudaclaw_cuda.h:130). */
+int gf_get_params(gf_params *params);
+
+/* Message of the last error on this thread ("" if none). */
+const char *gf_last_error(void);
+
+/* Number of bytes of HBM a pool of `capacity` patches of mx x my occupies
+ * (both q buffers, aux, descriptors, per-patch scratch). */
+long long gf_pool_bytes(int mx, int my, int mbc, int capacity);
+
+/* ---- drop-in batch step (reference architecture) ------------------------ */
+
+/* Advance n host-resident patches by one step: pack AoS -> SoA, copy to the
+ * device, run the fused step, reduce the CFL number, copy back and unpack
+ * (interior cells are updated; ghost cells are left as given).  q[k], aux[k]
+ * point to patch k's arrays in the reference layout with ghost cells filled
+ * by the caller; dx[k], dy[k] are its cell sizes.  *maxcfl receives the
+ * maximum over the patches of the per-patch CFL numbers, each capped to 0
+ * above 1000 as geoclaw_update does (fc2d_cpucuda.cpp:392).  Replaces
+ * cudaclaw_step2_batch (cuda_source/cudaclaw_step2.cu:57-324). */
+int gf_step2_batch(int mx, int my, int mbc, int maux, int n,
+                   double *const *q, double *const *aux,
+                   const double *dx, const double *dy,
+                   double t, double dt, double *maxcfl);
+
+/* ---- device-resident patch pool ----------------------------------------- */
+
+/* Allocate a pool of `capacity` patches: two SoA q buffers with ghost cells,
+ * aux, descriptors, per-patch CFL/depth and scenario control records.
+ * Replaces fc2d_cudaclaw_allocate_buffers (cuda_source/cudaclaw_allocate.cu:
+ * 118-138) and the per-patch cudaclaw_allocate_fluxes (:21-85), which
+ * allocated 12 scratch arrays per patch; the fused kernel needs none. */
+gf_pool *gf_pool_create(int mx, int my, int mbc, int capacity);
+
+/* Free a pool.  Replaces fc2d_cudaclaw_deallocate_buffers and
+ * cudaclaw_deallocate_fluxes (cuda_source/cudaclaw_allocate.cu:88-145). */
+int gf_pool_destroy(gf_pool *pool);
+
+/* Describe patches [0, n): cell sizes, owning scenario (-1: none; patches
+ * of no active scenario are skipped by gf_pool_run), and 8 neighbour
+ * entries per patch (left, right, bottom, top, left-bottom, right-bottom,
+ * left-top, right-top), each a patch index or GF_NB_*.  This is the
+ * information ForestClaw's ghost exchange uses (fc2d_geoclaw_copy_fort.f:
+ * 11-107) together with the per-side mthbc of geoclaw_bc2
+ * (fc2d_cpucuda.cpp:131-186). */
+int gf_pool_set_patches(gf_pool *pool, int n, const double *dx, const double *dy,
+                        const int *scenario, const int *nbr);
+
+/* Copy n patches from host AoS arrays (contiguous, one patch after another,
+ * reference layout with ghosts) into pool slots [first, first+n), q into
+ * buffer `buf` (0 or 1).  aux_aos may be NULL to keep the pool's aux.
+ * Replaces CUDACLAW_SWAP_DATA with geoclaw2cudaclaw = 1 (cuda_source/
+ * data_swap.f90:22-40) plus the H2D copy (cudaclaw_step2.cu:158). */
+int gf_pool_upload(gf_pool *pool, int first, int n, int buf, const double *q_aos,
+                   const double *aux_aos, int maux);
+
+/* Copy pool slots [first, first+n) of buffer `buf` back to host AoS arrays.
+ * Replaces the D2H copy and CUDACLAW_SWAP_DATA with geoclaw2cudaclaw = 0
+ * (cudaclaw_step2.cu:298-315; data_swap.f90:41-53). */
+int gf_pool_download(gf_pool *pool, int first, int n, int buf, double *q_aos);
+
+/* Fill the ghost cells of buffer `buf` on the device for patches [first,
+ * first+n): same-level face and corner copies, then solid-wall /
+ * extrapolation sides in the order left, right, bottom, top
+ * (geoflood_bc2_fort.f90:118-131 and twins).  GF_NB_STORED sides keep the
+ * values already stored (e.g. coarse/fine ghosts filled by the host). */
+int gf_pool_ghost_fill(gf_pool *pool, int first, int n, int buf);
+
+/* One step attempt of patches [first, first+n) from buffer `src` to buffer
+ * `dst` (== src: in place) with time step dt, all patches in one launch.
+ * cfl (optional, host, length n) receives the per-patch capped CFL numbers;
+ * *maxcfl (optional) their maximum.  Ghost cells must already be filled. */
+int gf_pool_step(gf_pool *pool, int first, int n, int src, int dst, double dt,
+                 double *cfl, double *maxcfl);
+
+/* The same step issued as one kernel launch per patch from a host loop (the
+ * "plain" per-patch step). */
+int gf_pool_step_plain(gf_pool *pool, int first, int n, int src, int dst, double dt,
+                       double *cfl, double *maxcfl);
+
+/* Register a scenario over pool slots [first_patch, first_patch+npatch) with
+ * its own device-side time-step control (fc2d_geoclaw_run.c:417-590): start
+ * time, initial dt, desired/max CFL, and the stop condition (target_steps
+ * accepted steps and/or simulated time t_end; <= 0 disables each).  The
+ * accepted state starts in q buffer 0.  Returns the scenario id (>= 0). */
+int gf_pool_add_scenario(gf_pool *pool, int first_patch, int npatch, double t0,
+                         double dt0, double desired_cfl, double max_cfl,
+                         int target_steps, double t_end);
+
+/* Deactivate every scenario and forget them (slots keep their data). */
+int gf_pool_clear_scenarios(gf_pool *pool);
+
+/* Stop or restart a scenario without removing it. */
+int gf_pool_set_scenario_active(gf_pool *pool, int scenario, int active);
+
+/* Copy a scenario's control record to the host. */
+int gf_pool_get_scenario(gf_pool *pool, int scenario, gf_scenario_info *info);
+
+/* Advance all active scenarios by up to `substeps` attempts: each attempt is
+ * ghost fill -> fused step of every patch -> per-scenario CFL reduction and
+ * accept/retake decision on the device.  With use_graph != 0 the attempt is
+ * captured once in a hipGraph and replayed.  Returns after the device has
+ * finished; *any_active (optional) tells whether a scenario is still active. */
+int gf_pool_run(gf_pool *pool, int substeps, int use_graph, int *any_active);
+
+/* The plain loop for the same scenarios: per attempt, a host-issued launch
+ * per patch, a host read of the CFL numbers and the dt rule on the host.
+ * Semantics equal gf_pool_run. */
+int gf_pool_run_plain(gf_pool *pool, int substeps, int *any_active);
+
+/* Wait for all work queued on the library stream. */
+int gf_synchronize(void);
+
+/* Device time (ms) of the last gf_pool_run / gf_pool_run_plain /
+ * gf_pool_step* call and of its fused-step launches alone, from HIP events
+ * on the library stream. */
+int gf_last_timing(double *total_ms, double *step_ms, int *step_launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GEOFLOOD_AMD_H */
