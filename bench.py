@@ -1,0 +1,239 @@
+"""Benchmark: FP64 cell-updates/s of the GeoFlood patch step on one MI355X.
+
+Workload (uniform large batch): 4096 x 4096 cells as 16,384 patches of
+32 x 32 (mbc = 2), single level, seeded fractal DEM (spectral slope -2,
+RMS 3 m, seed 0) plus a 1-in-500 slope descending in +x, dx = 1 m; dam-break
+reservoir h = 10 m over the upper-left eighth, dry elsewhere; Manning
+n = 0.03; walls; initial dt 0.01 s, desired CFL 0.9, max CFL 1.0.
+
+A "step" is one step attempt over all 16,384 patches: device ghost fill,
+fused step kernel, device dt controller.  One cell-update is one interior
+cell advanced by one accepted step (retaken attempts do not count).
+
+Runs, all from the same initial state, inputs resident in HBM:
+  batched  -- K attempts, each one hipGraph launch (the reported value);
+  events   -- the same K attempts launched without a graph, with HIP events
+              around every fused-step launch (per-kernel time -> roofline);
+  plain    -- a few attempts with one kernel launch per patch from a host
+              loop and the dt rule on the host;
+  cpu      -- the reference GeoClaw Fortran step (oracle/_ref) on the host
+              cores over the same first steps, when it is present.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 cell-updates/s at 4096² (batched vs plain) + ensemble p50/p99 latency"
+N = 4096
+MX = 32
+BYTES_PER_CELL_UPDATE = 56  # read h, hu, hv, b; write h, hu, hv (FP64)
+HBM_PEAK_GBS = 8000.0
+PARAMS = dict(grav=9.81, dry_tolerance=1e-3, manning_n=0.03, friction_depth=1.0e6,
+              friction_forcing=1, src_term=1, order=2, transverse=2, mthlim=(4, 4, 4))
+DT0, DESIRED_CFL, MAX_CFL = 0.01, 0.9, 1.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_state():
+    from geoflood_amd.scenarios import uniform_dam_break
+    return uniform_dam_break(N, dx=1.0, h0=10.0, rms=3.0, regional_slope=1.0 / 500.0, seed=0)
+
+
+def reset(pool, layout, state, target_steps):
+    h, hu, hv, b, dx = state
+    pool.clear_scenarios()
+    pool.upload_global(layout, h, hu, hv, b, buf=0)
+    sid = pool.add_scenario(layout, dt0=DT0, desired_cfl=DESIRED_CFL, max_cfl=MAX_CFL,
+                            target_steps=target_steps)
+    return sid
+
+
+def cpu_baseline(state, nsteps, nthreads):
+    """The reference Fortran step (or, where it was not built, the C
+    restatement) over the first nsteps accepted steps of the same workload."""
+    from oracle.oracle import Brick, CLib, Params, ref_lib
+    h, hu, hv, b, dx = state
+    p = Params(manning_n=PARAMS["manning_n"])
+    c = CLib()
+    c.set_params(p)
+    ref = ref_lib()
+    kind = "port"
+    if ref is not None:
+        ref.set_params(p)
+        kind = "reference"
+    br = Brick.from_global(h, hu, hv, b, N // MX, N // MX, 2, dx, dx)
+    t0 = time.perf_counter()
+    rec = c.brick_run(br, nsteps, DT0, DESIRED_CFL, MAX_CFL, update=ref, nthreads=nthreads)
+    el = time.perf_counter() - t0
+    return dict(value=nsteps * N * N / el, seconds=el, kind=kind, rec=rec, state=br.to_global())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-events", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy kernel (torch) three times; "
+                         "used to scale the FETCH_SIZE / WRITE_SIZE counters")
+    args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 8, "copy_bytes_written": y.numel() * 8}))
+        return
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding of this path is a design, not a measured run)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import geoflood_amd
+    from geoflood_amd.pool import BrickLayout, Pool
+
+    geoflood_amd.init(0)
+    geoflood_amd.set_params(**PARAMS)
+    log("building workload ...")
+    state = build_state()
+    mi = N // MX
+    layout = BrickLayout(mi, mi, MX, MX, state[4], state[4], (3, 3, 3, 3))
+    pool = Pool(MX, MX, mi * mi)
+    pool.set_layout([layout])
+    cells = N * N
+
+    # ---- warm-up (includes graph capture), then the timed batched run ----
+    reset(pool, layout, state, 0)
+    pool.run(args.warmup, graph=True)
+    sid = reset(pool, layout, state, 0)
+    pool.run(0, graph=True)  # capture outside the timed region
+    geoflood_amd.load().gf_synchronize()
+    log(f"timed batched run: {args.steps} attempts ...")
+    t0 = time.perf_counter()
+    pool.run(args.steps, graph=True)
+    wall = time.perf_counter() - t0
+    total_ms, _, _ = geoflood_amd.last_timing()
+    info_b = pool.scenario(sid)
+    accepted = info_b["nsteps"]
+    value = accepted * cells / wall
+    ms_per_step = 1e3 * wall / args.steps
+    log(f"batched: {accepted} accepted / {args.steps} attempts, {wall:.3f} s wall, "
+        f"{total_ms:.1f} ms device, t = {info_b['t']:.4f} s, {value / 1e9:.2f} G cell-updates/s")
+
+    # ---- same attempts without a graph, HIP events around each step launch ----
+    roofline = None
+    kernel_ms = None
+    if not args.no_events:
+        sid = reset(pool, layout, state, 0)
+        pool.run(args.steps, graph=False)
+        ev_total, step_ms, launches = geoflood_amd.last_timing()
+        info_e = pool.scenario(sid)
+        kernel_ms = step_ms / max(launches, 1)
+        bytes_per_launch = BYTES_PER_CELL_UPDATE * cells
+        achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+        traffic = None
+        tfile = os.path.join(ROOT, "profiles", "k_step_32x32_uniform4096_traffic.txt")
+        if os.path.exists(tfile):
+            kv = dict(line.split("=", 1) for line in open(tfile).read().split("\n") if "=" in line)
+            traffic = float(kv.get("hbm_bytes_per_launch", "nan"))
+        roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                        frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                        kernel="k_step<32,32,640>", kernel_avg_ms=round(kernel_ms, 5),
+                        bytes_per_launch=bytes_per_launch)
+        same = (info_e["nsteps"] == info_b["nsteps"] and info_e["t"] == info_b["t"])
+        log(f"events run: step kernel avg {kernel_ms:.4f} ms over {launches} launches, "
+            f"{ev_total:.1f} ms device total; same trajectory as graph run: {same}")
+
+    # ---- plain per-patch loop ----
+    plain = None
+    gpu_first = None
+    if not args.no_plain:
+        sid = reset(pool, layout, state, args.plain_steps)
+        t0 = time.perf_counter()
+        while pool.run_plain(args.plain_steps):
+            pass
+        pw = time.perf_counter() - t0
+        info_p = pool.scenario(sid)
+        pv = info_p["nsteps"] * cells / pw
+        plain = dict(value=pv, unit="cell-updates/s", steps=info_p["nsteps"],
+                     attempts=info_p["attempts"], ms_per_step=1e3 * pw / max(info_p["attempts"], 1))
+        gpu_first = np.stack(pool.download_global(layout, buf=info_p["cur"]))
+        log(f"plain: {info_p['nsteps']} steps in {pw:.3f} s, {pv / 1e9:.4f} G cell-updates/s")
+
+    # ---- CPU baseline (reference Fortran step on the host cores) ----
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        nthreads = min(16, len(os.sched_getaffinity(0)))
+        log(f"cpu baseline: {args.cpu_steps} steps on {nthreads} threads ...")
+        cb = cpu_baseline(state, args.cpu_steps, nthreads)
+        cpu = dict(value=cb["value"], unit="cell-updates/s", cores=nthreads, kind=cb["kind"],
+                   sample=f"first {args.cpu_steps} accepted steps of the full 4096^2 workload "
+                          f"({args.cpu_steps * cells} cell-updates, {cb['seconds']:.1f} s)")
+        if gpu_first is not None and args.cpu_steps == args.plain_steps:
+            ref = cb["state"]
+            scale = np.maximum(np.abs(ref), np.abs(gpu_first))
+            d = np.abs(ref - gpu_first)
+            parity = float(np.max(np.where(scale > 0, d / np.where(scale > 0, scale, 1), 0)))
+        log(f"cpu: {cb['value'] / 1e6:.2f} M cell-updates/s ({cb['kind']}); parity max rel {parity}")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "uniform 4096x4096 dam break over a seeded fractal DEM "
+                        "(16384 patches of 32x32, mbc=2, single level)",
+            "dem": "spectral slope -2, RMS 3 m, seed 0, plus 1/500 slope down +x; dx = 1 m",
+            "initial_state": "h = 10 m for x < L/2, y >= 3L/4; dry elsewhere; u = v = 0",
+            "numerics": "g 9.81, drytol 1e-3, Manning n 0.03, MC limiter, transverse 2, walls",
+            "dt_control": f"initial dt {DT0} s, desired CFL {DESIRED_CFL}, max CFL {MAX_CFL}, on device",
+            "mode": "batched: one hipGraph launch (ghost fill + fused step + dt control) per attempt",
+            "accepted_steps": accepted,
+            "simulated_time_s": info_b["t"],
+        },
+        "batched_vs_plain": (value / plain["value"]) if plain else None,
+        "plain": plain,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "parity_vs_cpu_max_rel": parity,
+        "ensemble": None,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

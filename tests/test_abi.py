@@ -1,0 +1,64 @@
+"""The C-ABI library: it loads, exports every entry point declared in
+include/*.h, the Python prototypes cover exactly those, and argument/state
+errors come back as status codes with a message (no GPU needed)."""
+import ctypes as ct
+import glob
+import os
+import re
+
+import pytest
+
+import geoflood_amd as gf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(gf_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = gf.load()
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert names == set(gf.PROTOTYPES), names ^ set(gf.PROTOTYPES)
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    saved = gf._LIB
+    gf._LIB = None
+    try:
+        with pytest.raises(gf.GeoFloodError):
+            gf.load(str(tmp_path / "libgeoflood_amd.so"))
+    finally:
+        gf._LIB = saved
+
+
+def test_errors_are_status_codes():
+    lib = gf.load()
+    assert lib.gf_check_parameters(3) == 0
+    assert lib.gf_check_parameters(4) == -4
+    assert b"mwaves" in lib.gf_last_error()
+    # no gf_init in this process: state errors, not crashes
+    p = gf.make_params()
+    assert lib.gf_set_params(ct.byref(p)) == -5
+    assert lib.gf_pool_create(32, 32, 2, 4) is None
+    assert b"gf_init" in lib.gf_last_error()
+    assert lib.gf_pool_destroy(None) == 0
+    assert lib.gf_pool_set_patches(None, 0, None, None, None, None) == -1
+
+
+def test_pool_bytes_accounting():
+    lib = gf.load()
+    b1 = lib.gf_pool_bytes(32, 32, 2, 1000)
+    b2 = lib.gf_pool_bytes(32, 32, 2, 2000)
+    per = b2 - b1
+    # two q buffers (3 fields) + aux, 36x36 FP64, plus descriptors and control
+    assert 7 * 36 * 36 * 8 * 1000 <= per <= 7 * 36 * 36 * 8 * 1000 + 300 * 1000
