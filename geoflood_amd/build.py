@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgeoflood_amd.so")
-SOURCES = ["gf_step.hip", "gf_api.hip"]
+SOURCES = ["gf_step.hip", "gf_api.hip", "gf_plugin.cpp"]
 HEADERS = ["gf_physics.h", "gf_kernels.h", "gf_launch.h"]
 
 # -ffp-contract=off: no fused multiply-add anywhere, so every FP64 operation
@@ -22,6 +22,9 @@ HIPCC_FLAGS = [
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
+    # keep loop-invariant materialisations inside the persistent step loop
+    # (hoisting them costs ~25 spilled VGPRs in the 32x32 kernel)
+    "-mllvm", "-disable-machine-licm",
 ]
 
 

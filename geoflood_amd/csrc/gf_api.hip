@@ -48,8 +48,14 @@ struct Lib {
     hipStream_t stream = nullptr;
     gf::Params P{};
     gf_params user{};
+    hipStream_t stream2 = nullptr;    // dry-patch branch of the batched step
+    int num_cu = 256;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // timing-free fork/join of the two streams
     std::vector<hipEvent_t> ev_step;  // pairs around step launches
+    std::vector<hipEvent_t> ev_wet, ev_dry;  // pairs around wet / dry launches (non-graph runs)
+    double last_wet_ms = 0.0, last_dry_ms = 0.0;
+    unsigned params_gen = 1;  // bumped by gf_set_params: pools re-sync their batch arguments
     double last_total_ms = 0.0, last_step_ms = 0.0;
     int last_launches = 0;
     // drop-in batch staging
@@ -77,6 +83,16 @@ struct gf_pool {
     double* d_stage = nullptr;  // device staging (kChunk patches of q and aux)
     double* h_stage = nullptr;  // pinned host staging
     double* h_cfl = nullptr;    // pinned
+    // batched path
+    double* hmax2 = nullptr;               // [2][capacity]
+    unsigned long long* cflbits = nullptr; // [capacity] per scenario
+    int* list_wet = nullptr;
+    int* list_dry = nullptr;
+    int* counters = nullptr;               // gf::NCOUNTERS
+    long long* stats = nullptr;            // [3]
+    gf::BatchArgs* d_args = nullptr;       // device copy read by the persistent kernel
+    bool args_dirty = true;
+    unsigned args_gen = 0;                 // params generation of d_args and of the graph
     std::vector<PatchDesc> h_desc;
     std::vector<ScenCtrl> h_ctrl;
     int nscen = 0;
@@ -138,7 +154,7 @@ static gf::StepArgs step_args(gf_pool* p, int first, int count, double* q0, doub
     a.q1 = q1;
     a.aux = p->aux;
     a.cfl_out = p->cfl;
-    a.hmax_out = p->hmax;
+    a.hmax_out = p->hmax2 ? p->hmax2 + (size_t)(q1 ? (cur ^ 1) : cur) * p->capacity : nullptr;
     a.ctrl = ctrl;
     a.dt = dt;
     a.cur = cur;
@@ -160,8 +176,12 @@ int gf_init(int device)
     }
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&g.stream2, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&g.ev_begin));
     HIPCHK(hipEventCreate(&g.ev_end));
+    HIPCHK(hipEventCreateWithFlags(&g.ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&g.ev_join, hipEventDisableTiming));
+    HIPCHK(hipDeviceGetAttribute(&g.num_cu, hipDeviceAttributeMultiprocessorCount, device));
     HIPCHK(hipMalloc(&g.d_max, sizeof(double)));
     HIPCHK(hipHostMalloc(&g.h_max, sizeof(double), hipHostMallocDefault));
     g.device = device;
@@ -180,9 +200,14 @@ int gf_finalize(void)
     g.h_stage = nullptr;
     g.h_stage_bytes = 0;
     for (auto e : g.ev_step) hipEventDestroy(e);
+    for (auto e : g.ev_wet) hipEventDestroy(e);
+    for (auto e : g.ev_dry) hipEventDestroy(e);
     g.ev_step.clear();
     hipEventDestroy(g.ev_begin);
     hipEventDestroy(g.ev_end);
+    hipEventDestroy(g.ev_fork);
+    hipEventDestroy(g.ev_join);
+    hipStreamDestroy(g.stream2);
     hipFree(g.d_max);
     hipHostFree(g.h_max);
     hipStreamDestroy(g.stream);
@@ -217,6 +242,7 @@ int gf_set_params(const gf_params* p)
     g.P.order = p->order;
     g.P.transverse = p->transverse;
     for (int k = 0; k < 3; k++) g.P.mthlim[k] = p->mthlim[k];
+    g.params_gen += 1;
     return GF_OK;
 }
 
@@ -233,7 +259,8 @@ const char* gf_last_error(void) { return g_err.c_str(); }
 long long gf_pool_bytes(int mx, int my, int mbc, int capacity)
 {
     const long long np2 = (long long)(mx + 2 * mbc) * (my + 2 * mbc);
-    const long long per = (2 * 3 * np2 + np2) * 8 + (long long)sizeof(PatchDesc) + 16 + sizeof(ScenCtrl);
+    const long long per = (2 * 3 * np2 + np2) * 8 + (long long)sizeof(PatchDesc) + 8 /*cfl*/ + 16 /*hmax*/ +
+                          8 /*cflbits*/ + 8 /*lists*/ + 8 /*legacy hmax*/ + sizeof(ScenCtrl);
     const long long staging = (long long)kChunk * 4 * np2 * 8;
     return per * capacity + staging;
 }
@@ -269,7 +296,14 @@ gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
               hipMalloc(&p->ctrl, (size_t)capacity * sizeof(ScenCtrl)) == hipSuccess &&
               hipMalloc(&p->d_stage, chunk * 4 * p->np2 * sizeof(double)) == hipSuccess &&
               hipHostMalloc(&p->h_stage, chunk * 4 * p->np2 * sizeof(double), hipHostMallocDefault) == hipSuccess &&
-              hipHostMalloc(&p->h_cfl, (size_t)capacity * sizeof(double), hipHostMallocDefault) == hipSuccess;
+              hipHostMalloc(&p->h_cfl, (size_t)capacity * sizeof(double), hipHostMallocDefault) == hipSuccess &&
+              hipMalloc(&p->hmax2, (size_t)2 * capacity * sizeof(double)) == hipSuccess &&
+              hipMalloc(&p->cflbits, (size_t)capacity * sizeof(unsigned long long)) == hipSuccess &&
+              hipMalloc(&p->list_wet, (size_t)capacity * sizeof(int)) == hipSuccess &&
+              hipMalloc(&p->list_dry, (size_t)capacity * sizeof(int)) == hipSuccess &&
+              hipMalloc(&p->counters, gf::NCOUNTERS * sizeof(int)) == hipSuccess &&
+              hipMalloc(&p->stats, 3 * sizeof(long long)) == hipSuccess &&
+              hipMalloc(&p->d_args, sizeof(gf::BatchArgs)) == hipSuccess;
     if (ok)
         ok = hipMemsetAsync(p->q[0], 0, qbytes, g.stream) == hipSuccess &&
              hipMemsetAsync(p->q[1], 0, qbytes, g.stream) == hipSuccess &&
@@ -278,6 +312,10 @@ gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
              hipMemsetAsync(p->cfl, 0, (size_t)capacity * sizeof(double), g.stream) == hipSuccess &&
              hipMemsetAsync(p->hmax, 0, (size_t)capacity * sizeof(double), g.stream) == hipSuccess &&
              hipMemsetAsync(p->ctrl, 0, (size_t)capacity * sizeof(ScenCtrl), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->hmax2, 0, (size_t)2 * capacity * sizeof(double), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->cflbits, 0, (size_t)capacity * sizeof(unsigned long long), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->counters, 0, gf::NCOUNTERS * sizeof(int), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->stats, 0, 3 * sizeof(long long), g.stream) == hipSuccess &&
              hipStreamSynchronize(g.stream) == hipSuccess;
     if (!ok) {
         fail(GF_ENOMEM, "pool allocation of %d patches (%lld bytes) failed", capacity,
@@ -304,6 +342,13 @@ int gf_pool_destroy(gf_pool* p)
     hipFree(p->hmax);
     hipFree(p->ctrl);
     hipFree(p->d_stage);
+    hipFree(p->hmax2);
+    hipFree(p->cflbits);
+    hipFree(p->list_wet);
+    hipFree(p->list_dry);
+    hipFree(p->counters);
+    hipFree(p->stats);
+    hipFree(p->d_args);
     if (p->h_stage) hipHostFree(p->h_stage);
     if (p->h_cfl) hipHostFree(p->h_cfl);
     delete p;
@@ -335,6 +380,7 @@ int gf_pool_set_patches(gf_pool* p, int n, const double* dx, const double* dy, c
     if (n > 0) HIPCHK(hipMemcpyAsync(p->desc, p->h_desc.data(), sizeof(PatchDesc) * n, hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     p->graph_dirty = true;
+    p->args_dirty = true;
     return GF_OK;
 }
 
@@ -358,6 +404,9 @@ int gf_pool_upload(gf_pool* p, int first, int n, int buf, const double* q_aos, c
                                first + off, m, (int)np2, maux, g.stream));
         HIPCHK(hipStreamSynchronize(g.stream));
     }
+    // interior depth maxima of the uploaded state (classification of the first attempt)
+    HIPCHK(gf::launch_hmax(p->mx, p->my, p->q[buf], p->hmax2 + (size_t)buf * p->capacity, first, n, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
 }
 
@@ -480,6 +529,7 @@ int gf_pool_add_scenario(gf_pool* p, int first_patch, int npatch, double t0, dou
     HIPCHK(hipStreamSynchronize(g.stream));
     p->nscen = id + 1;
     p->graph_dirty = true;
+    p->args_dirty = true;
     return id;
 }
 
@@ -489,8 +539,10 @@ int gf_pool_clear_scenarios(gf_pool* p)
     p->nscen = 0;
     p->h_ctrl.clear();
     HIPCHK(hipMemsetAsync(p->ctrl, 0, sizeof(ScenCtrl) * p->capacity, g.stream));
+    HIPCHK(hipMemsetAsync(p->cflbits, 0, sizeof(unsigned long long) * p->capacity, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     p->graph_dirty = true;
+    p->args_dirty = true;
     return GF_OK;
 }
 
@@ -525,15 +577,87 @@ int gf_pool_get_scenario(gf_pool* p, int s, gf_scenario_info* info)
     return GF_OK;
 }
 
-// One attempt of every active scenario, enqueued on the stream.
-static int enqueue_attempt(gf_pool* p, int ev_pair)
+static int ensure_pair_events(std::vector<hipEvent_t>& v, size_t pairs)
 {
-    gf::GhostArgs ga{p->desc, 0, p->n, p->q[0], p->q[1], p->ctrl, 0};
-    HIPCHK(gf::launch_ghost(p->mx, p->my, ga, g.stream));
-    if (ev_pair >= 0) HIPCHK(hipEventRecord(g.ev_step[2 * ev_pair], g.stream));
-    HIPCHK(gf::launch_step(p->mx, p->my, step_args(p, 0, p->n, p->q[0], p->q[1], p->ctrl, 0.0, 0), g.stream));
-    if (ev_pair >= 0) HIPCHK(hipEventRecord(g.ev_step[2 * ev_pair + 1], g.stream));
-    HIPCHK(gf::launch_control(p->ctrl, p->nscen, p->cfl, g.stream));
+    while (v.size() < 2 * pairs) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        v.push_back(e);
+    }
+    return GF_OK;
+}
+
+static int sync_batch_args(gf_pool* p)
+{
+    if (p->args_gen != g.params_gen) {
+        p->args_dirty = true;
+        p->graph_dirty = true;
+    }
+    if (!p->args_dirty) return GF_OK;
+    gf::BatchArgs a{};
+    a.desc = p->desc;
+    a.n = p->n;
+    a.capacity = p->capacity;
+    a.q0 = p->q[0];
+    a.q1 = p->q[1];
+    a.aux = p->aux;
+    a.hmax = p->hmax2;
+    a.cflbits = p->cflbits;
+    a.ctrl = p->ctrl;
+    a.nscen = p->nscen;
+    a.list_wet = p->list_wet;
+    a.list_dry = p->list_dry;
+    a.counters = p->counters;
+    a.stats = p->stats;
+    a.P = g.P;
+    HIPCHK(hipMemcpyAsync(p->d_args, &a, sizeof(a), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    p->args_dirty = false;
+    p->args_gen = g.params_gen;
+    return GF_OK;
+}
+
+static gf::BatchArgs host_batch_args(gf_pool* p)
+{
+    gf::BatchArgs a{};
+    a.desc = p->desc;
+    a.n = p->n;
+    a.capacity = p->capacity;
+    a.q0 = p->q[0];
+    a.q1 = p->q[1];
+    a.aux = p->aux;
+    a.hmax = p->hmax2;
+    a.cflbits = p->cflbits;
+    a.ctrl = p->ctrl;
+    a.nscen = p->nscen;
+    a.list_wet = p->list_wet;
+    a.list_dry = p->list_dry;
+    a.counters = p->counters;
+    a.stats = p->stats;
+    a.P = g.P;
+    return a;
+}
+
+// One attempt of every active scenario: classify -> (active patches on the
+// main stream || dry patches on the second stream) -> controller.  With
+// k >= 0, timing events bracket the step phase and each branch.
+static int enqueue_attempt(gf_pool* p, int k)
+{
+    const gf::BatchArgs a = host_batch_args(p);
+    HIPCHK(gf::launch_classify(a, g.stream));
+    if (k >= 0) HIPCHK(hipEventRecord(g.ev_step[2 * k], g.stream));
+    HIPCHK(hipEventRecord(g.ev_fork, g.stream));
+    HIPCHK(hipStreamWaitEvent(g.stream2, g.ev_fork, 0));
+    if (k >= 0) HIPCHK(hipEventRecord(g.ev_dry[2 * k], g.stream2));
+    HIPCHK(gf::launch_dry(p->mx, p->my, a, 4 * g.num_cu, g.stream2));
+    if (k >= 0) HIPCHK(hipEventRecord(g.ev_dry[2 * k + 1], g.stream2));
+    if (k >= 0) HIPCHK(hipEventRecord(g.ev_wet[2 * k], g.stream));
+    HIPCHK(gf::launch_wet(p->mx, p->my, p->d_args, g.num_cu, g.stream));
+    if (k >= 0) HIPCHK(hipEventRecord(g.ev_wet[2 * k + 1], g.stream));
+    HIPCHK(hipEventRecord(g.ev_join, g.stream2));
+    HIPCHK(hipStreamWaitEvent(g.stream, g.ev_join, 0));
+    if (k >= 0) HIPCHK(hipEventRecord(g.ev_step[2 * k + 1], g.stream));
+    HIPCHK(gf::launch_batch_control(a, g.stream));
     return GF_OK;
 }
 
@@ -555,6 +679,8 @@ int gf_pool_run(gf_pool* p, int substeps, int use_graph, int* any_active)
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (substeps < 0) return fail(GF_EINVAL, "substeps must be >= 0");
     if (p->nscen == 0 || p->n == 0) return fail(GF_ESTATE, "no scenario registered");
+    if (int r = sync_batch_args(p)) return r;
+    g.last_wet_ms = g.last_dry_ms = 0.0;
     if (use_graph) {
         if (p->graph_dirty) {
             if (p->graph_exec) HIPCHK(hipGraphExecDestroy(p->graph_exec));
@@ -582,6 +708,8 @@ int gf_pool_run(gf_pool* p, int substeps, int use_graph, int* any_active)
         g.last_launches = 0;
     } else {
         if (int r = ensure_step_events(std::max(1, substeps))) return r;
+        if (int r = ensure_pair_events(g.ev_wet, std::max(1, substeps))) return r;
+        if (int r = ensure_pair_events(g.ev_dry, std::max(1, substeps))) return r;
         HIPCHK(hipEventRecord(g.ev_begin, g.stream));
         for (int k = 0; k < substeps; k++)
             if (int r = enqueue_attempt(p, k)) return r;
@@ -594,6 +722,10 @@ int gf_pool_run(gf_pool* p, int substeps, int use_graph, int* any_active)
         for (int k = 0; k < substeps; k++) {
             HIPCHK(hipEventElapsedTime(&ms, g.ev_step[2 * k], g.ev_step[2 * k + 1]));
             g.last_step_ms += ms;
+            HIPCHK(hipEventElapsedTime(&ms, g.ev_wet[2 * k], g.ev_wet[2 * k + 1]));
+            g.last_wet_ms += ms;
+            HIPCHK(hipEventElapsedTime(&ms, g.ev_dry[2 * k], g.ev_dry[2 * k + 1]));
+            g.last_dry_ms += ms;
         }
         g.last_launches = substeps;
     }
@@ -742,6 +874,25 @@ int gf_synchronize(void)
 {
     if (int r = need_init()) return r;
     HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+}
+
+int gf_last_timing_branches(double* wet_ms, double* dry_ms)
+{
+    if (wet_ms) *wet_ms = g.last_wet_ms;
+    if (dry_ms) *dry_ms = g.last_dry_ms;
+    return GF_OK;
+}
+
+int gf_pool_stats(gf_pool* p, long long* active_patches, long long* dry_patches, long long* attempts)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    long long s[3];
+    HIPCHK(hipMemcpyAsync(s, p->stats, sizeof(s), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (active_patches) *active_patches = s[0];
+    if (dry_patches) *dry_patches = s[1];
+    if (attempts) *attempts = s[2];
     return GF_OK;
 }
 

@@ -33,7 +33,35 @@ struct GhostArgs {
     int cur;         // otherwise this buffer
 };
 
+// Device counters of the batched path (BatchArgs::counters).
+enum { CNT_WET = 0, CNT_DRY = 1, NEXT_WET = 2, NCOUNTERS = 4 };
+
+// Arguments of the batched (work-list) path: classify -> wet || dry -> control.
+struct BatchArgs {
+    const PatchDesc* desc;
+    int n;                        // patches in the pool
+    int capacity;
+    double* q0;
+    double* q1;
+    const double* aux;
+    double* hmax;                 // [2][capacity] max interior depth of each buffer's state
+    unsigned long long* cflbits;  // per scenario: max CFL of the attempt (bit pattern, CFL >= 0)
+    ScenCtrl* ctrl;
+    int nscen;
+    int* list_wet;
+    int* list_dry;
+    int* counters;                // NCOUNTERS ints
+    long long* stats;             // accumulated active patches, dry patches, attempts
+    Params P;
+};
+
 bool supported_shape(int mx, int my);
+hipError_t launch_classify(const BatchArgs& a, hipStream_t st);
+// da: the BatchArgs in device memory (read by the persistent kernel on every iteration)
+hipError_t launch_wet(int mx, int my, const BatchArgs* da, int grid, hipStream_t st);
+hipError_t launch_dry(int mx, int my, const BatchArgs& a, int grid, hipStream_t st);
+hipError_t launch_batch_control(const BatchArgs& a, hipStream_t st);
+hipError_t launch_hmax(int mx, int my, const double* q, double* hmax, int first, int n, hipStream_t st);
 hipError_t launch_step(int mx, int my, const StepArgs& a, hipStream_t st);
 hipError_t launch_ghost(int mx, int my, const GhostArgs& a, hipStream_t st);
 hipError_t launch_control(ScenCtrl* ctrl, int nscen, const double* cfl, hipStream_t st);

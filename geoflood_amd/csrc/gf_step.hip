@@ -35,6 +35,10 @@ struct Geo {
     static constexpr int PX = MX + 3;
     static constexpr int PY = MY + 3;
     static constexpr int NPOS = PX * PY;
+    // x-face flux increments kept in LDS between the x and y phases:
+    // faces I in [1, MX+1] of rows j in [1, MY]
+    static constexpr int NF = (MX + 1) * MY;
+    __device__ static constexpr int fi_(int I, int j) { return (j - 1) * (MX + 1) + (I - 1); }
     // LDS index of Fortran cell (fi, fj), fi in [-1, MX+2]
     __device__ static constexpr int ti(int fi, int fj) { return (fj + 1) * NXP + (fi + 1); }
 };
@@ -161,53 +165,117 @@ __device__ __forceinline__ void cell_transverse(const Rpt2Cell& C, const double 
     }
 }
 
-template <int MX, int MY, int NT>
-__global__ void __launch_bounds__(NT) k_step(StepArgs a)
+// Source of ghost cell (fi, fj) of a patch (Fortran indices, outside the
+// interior): the pool index and interior cell whose value ForestClaw's
+// same-level face copy, corner copy and bc2 passes (left, right, bottom, top
+// over full extents; geoflood_bc2_fort.f90:118-131 and twins) leave there,
+// and whether hu / hv change sign.  Returns false when the ghost is supplied
+// by the host (GF_NB_STORED or an unresolvable corner).
+template <int MX, int MY>
+__device__ __forceinline__ bool ghost_source(const int* __restrict__ nbr, int patch, int fi, int fj,
+                                             int& sp, int& si, int& sj, bool& neg_u, bool& neg_v)
+{
+    const int ox = fi < 1 ? -1 : (fi > MX ? 1 : 0);
+    const int oy = fj < 1 ? -1 : (fj > MY ? 1 : 0);
+    const int nx = ox < 0 ? nbr[0] : (ox > 0 ? nbr[1] : 0);
+    const int ny = oy < 0 ? nbr[2] : (oy > 0 ? nbr[3] : 0);
+    const bool px = (nx == NB_WALL || nx == NB_EXTRAP);
+    const bool py = (ny == NB_WALL || ny == NB_EXTRAP);
+    const int txv = (nx == NB_WALL) ? (ox < 0 ? 1 - fi : 2 * MX + 1 - fi) : (ox < 0 ? 1 : MX);
+    const int tyv = (ny == NB_WALL) ? (oy < 0 ? 1 - fj : 2 * MY + 1 - fj) : (oy < 0 ? 1 : MY);
+    neg_u = false;
+    neg_v = false;
+    if (ox != 0 && oy != 0) {
+        const int nc = oy < 0 ? (ox < 0 ? nbr[4] : nbr[5]) : (ox < 0 ? nbr[6] : nbr[7]);
+        if (nc >= 0) {
+            sp = nc; si = fi - ox * MX; sj = fj - oy * MY;
+        } else if (px && py) {
+            sp = patch; si = txv; sj = tyv;
+            neg_u = (nx == NB_WALL); neg_v = (ny == NB_WALL);
+        } else if (px && ny >= 0) {
+            sp = ny; si = txv; sj = fj - oy * MY;
+            neg_u = (nx == NB_WALL);
+        } else if (py && nx >= 0) {
+            sp = nx; si = fi - ox * MX; sj = tyv;
+            neg_v = (ny == NB_WALL);
+        } else {
+            return false;
+        }
+    } else if (ox != 0) {
+        if (nx >= 0) { sp = nx; si = fi - ox * MX; sj = fj; }
+        else if (px) { sp = patch; si = txv; sj = fj; neg_u = (nx == NB_WALL); }
+        else return false;
+    } else {
+        if (ny >= 0) { sp = ny; si = fi; sj = fj - oy * MY; }
+        else if (py) { sp = patch; si = fi; sj = tyv; neg_v = (ny == NB_WALL); }
+        else return false;
+    }
+    return true;
+}
+
+// Block-wide maximum (result valid in thread 0).
+template <int NT>
+__device__ __forceinline__ double block_max(double v, double* red)
+{
+    constexpr int NWAVE = NT / 64;
+    v = wave_max(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = red[0];
+#pragma unroll
+    for (int k = 1; k < NWAVE; k++) r = fmaxf64(r, red[k]);
+    return r;
+}
+
+// The whole step of one patch by one workgroup of NT threads.  q_src / q_dst
+// are the pool's source and destination buffers (the patch's slot is
+// selected here; q_dst == q_src updates in place).  With PULL the tile's
+// ghost cells are gathered from the neighbours' interiors in q_src
+// (same-level exchange and physical sides fused into the load); otherwise
+// the stored ghost cells are used.  Thread 0 receives the patch's capped CFL
+// number and the maximum interior depth after the step.
+template <int MX, int MY, int NT, bool PULL>
+__device__ __forceinline__ void step_patch(double* T, double* E, double* F, double* red, const PatchDesc* dptr, int patch,
+                           const double* q_src, double* q_dst, const double* aux,
+                           double dt, const Params& P, double& cfl_res, double& hmax_res)
 {
     using G = Geo<MX, MY>;
     constexpr int NXP = G::NXP, NP2 = G::NP2, PX = G::PX, NPOS = G::NPOS;
     constexpr int SLOTS = (NPOS + NT - 1) / NT;
-    constexpr int NWAVE = NT / 64;
-
-    __shared__ double T[4 * NP2];  // h, hu, hv, b of the padded tile
-    __shared__ double E[7 * NPOS]; // exchange buffer, component-major
-    __shared__ double red[NWAVE];
-
-    const int tid = threadIdx.x;
-    const int bid = blockIdx.x;
-    if (bid >= a.count) return;
-    const int patch = a.list ? a.list[bid] : a.first + bid;
-    const PatchDesc d = a.desc[patch];
-    const Params& P = a.P;
-
-    double dt;
-    int cur;
-    if (a.ctrl) {
-        const ScenCtrl* c = a.ctrl + (d.scen < 0 ? 0 : d.scen);
-        if (d.scen < 0 || !c->active) {
-            if (tid == 0) {
-                a.cfl_out[patch] = 0.0;
-                if (a.hmax_out) a.hmax_out[patch] = 0.0;
-            }
-            return;
-        }
-        dt = c->dt;
-        cur = c->cur;
-    } else {
-        dt = a.dt;
-        cur = a.cur;
-    }
-    const double* src = (cur ? a.q1 : a.q0) + (size_t)patch * 3 * NP2;
-    double* dst = (a.q1 ? (cur ? a.q0 : a.q1) : a.q0) + (size_t)patch * 3 * NP2;
-    const double* auxp = a.aux + (size_t)patch * NP2;
+    // Opaque copy of the thread index: inside the persistent loop this stops
+    // the compiler from hoisting every index computation of the body out of
+    // the loop, which would keep them live in registers across iterations.
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const double* src = q_src + (size_t)patch * 3 * NP2;
+    double* dst = q_dst + (size_t)patch * 3 * NP2;
+    const double* auxp = aux + (size_t)patch * NP2;
     const double g = P.grav, drytol = P.dry_tolerance;
-    const double dtdx = dt / d.dx;
-    const double dtdy = dt / d.dy;
+    const double dtdx = dt / dptr->dx;
+    const double dtdy = dt / dptr->dy;
 
-    // ---- tile load + b4step2 (geoclaw_b4step2_fort.f90:153-156) ----
+    // ---- tile load (+ halo gather) + b4step2 (geoclaw_b4step2_fort.f90:153-156) ----
     int wet = 0;
     for (int c = tid; c < NP2; c += NT) {
-        double h = src[c], hu = src[NP2 + c], hv = src[2 * NP2 + c];
+        double h, hu, hv;
+        if (PULL) {
+            const int fi = c % NXP - 1, fj = c / NXP - 1;
+            int sp = patch, si = fi, sj = fj;
+            bool nu = false, nv = false;
+            const bool interior = (fi >= 1 && fi <= MX && fj >= 1 && fj <= MY);
+            if (interior || !ghost_source<MX, MY>(dptr->nbr, patch, fi, fj, sp, si, sj, nu, nv)) {
+                h = src[c]; hu = src[NP2 + c]; hv = src[2 * NP2 + c];
+            } else {
+                const double* s2 = q_src + (size_t)sp * 3 * NP2;
+                const int k = G::ti(si, sj);
+                h = s2[k]; hu = s2[NP2 + k]; hv = s2[2 * NP2 + k];
+                if (nu) hu = -hu;
+                if (nv) hv = -hv;
+            }
+        } else {
+            h = src[c]; hu = src[NP2 + c]; hv = src[2 * NP2 + c];
+        }
         if (h < drytol) {
             h = fmaxf64(h, 0.0);
             hu = 0.0;
@@ -236,17 +304,11 @@ __global__ void __launch_bounds__(NT) k_step(StepArgs a)
             dst[2 * NP2 + k] = hv;
             hm = fmaxf64(hm, h);
         }
-        if (a.hmax_out) {
-            hm = wave_max(hm);
-            if ((tid & 63) == 0) red[tid >> 6] = hm;
-            __syncthreads();
-            if (tid == 0) {
-                double v = red[0];
-                for (int k = 1; k < NWAVE; k++) v = fmaxf64(v, red[k]);
-                a.hmax_out[patch] = v;
-            }
+        hm = block_max<NT>(hm, red);
+        if (tid == 0) {
+            cfl_res = 0.0;
+            hmax_res = hm;
         }
-        if (tid == 0) a.cfl_out[patch] = 0.0;
         return;
     }
 
@@ -262,7 +324,7 @@ __global__ void __launch_bounds__(NT) k_step(StepArgs a)
 
     double cfl = 0.0;
     double Wr[SLOTS][7], Sr[SLOTS][3];
-    double FXm[SLOTS][3], FXp[SLOTS][3], AP[SLOTS][3], AM[SLOTS][3], GA[SLOTS][3], GB[SLOTS][3];
+    double AP[SLOTS][3], AM[SLOTS][3], GA[SLOTS][3], GB[SLOTS][3];
 
     // ================= x-faces: normal Riemann problems =================
 #pragma unroll
@@ -303,10 +365,19 @@ __global__ void __launch_bounds__(NT) k_step(StepArgs a)
                 wl[k] = E[k * NPOS + p - 1];
                 wr[k] = E[k * NPOS + p + 1];
             }
-            face_second_order<0>(Wr[s], Sr[s], wl, wr, dtdx, P, FXm[s], FXp[s], AM[s], AP[s]);
+            double fxm[3], fxp[3];
+            face_second_order<0>(Wr[s], Sr[s], wl, wr, dtdx, P, fxm, fxp, AM[s], AP[s]);
+            if (j >= 1 && j <= MY) {
+                const int f = G::fi_(i, j);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    F[k * G::NF + f] = fxm[k];
+                    F[(3 + k) * G::NF + f] = fxp[k];
+                }
+            }
         } else {
 #pragma unroll
-            for (int k = 0; k < 3; k++) { FXm[s][k] = 0.0; FXp[s][k] = 0.0; AM[s][k] = 0.0; AP[s][k] = 0.0; }
+            for (int k = 0; k < 3; k++) { AM[s][k] = 0.0; AP[s][k] = 0.0; }
         }
     }
     __syncthreads();
@@ -462,10 +533,11 @@ __global__ void __launch_bounds__(NT) k_step(StepArgs a)
         if (in[s] && i >= 1 && i <= MX + 1 && j >= 1 && j <= MY) {
             const double g2[3] = {E[p - 1], E[2 * NPOS + p - 1], E[NPOS + p - 1]};  // local -> physical
             const double g1[3] = {GA[s][0], GA[s][2], GA[s][1]};
+            const int f = G::fi_(i, j);
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                FMr[s][k] = FXm[s][k] + g2[k] + g1[k];
-                FPr[s][k] = FXp[s][k] + g2[k] + g1[k];
+                FMr[s][k] = F[k * G::NF + f] + g2[k] + g1[k];
+                FPr[s][k] = F[(3 + k) * G::NF + f] + g2[k] + g1[k];
             }
         }
     }
@@ -505,26 +577,81 @@ __global__ void __launch_bounds__(NT) k_step(StepArgs a)
         }
     }
 
-    // ================= CFL (and depth) reductions =================
-    cfl = wave_max(cfl);
-    if ((tid & 63) == 0) red[tid >> 6] = cfl;
-    __syncthreads();
+    // ================= CFL and depth reductions =================
+    double v = block_max<NT>(cfl, red);
+    if (v > 1000.0) v = 0.0;  // MAXCFL_CAP per patch (fc2d_cpucuda.cpp:392)
+    const double hmx = block_max<NT>(hm, red);
     if (tid == 0) {
-        double v = red[0];
-        for (int k = 1; k < NWAVE; k++) v = fmaxf64(v, red[k]);
-        if (v > 1000.0) v = 0.0;  // MAXCFL_CAP per patch (fc2d_cpucuda.cpp:392)
-        a.cfl_out[patch] = v;
+        cfl_res = v;
+        hmax_res = hmx;
     }
-    if (a.hmax_out) {
+}
+
+// Plain step kernel: one workgroup per patch, stored ghost cells.
+template <int MX, int MY, int NT>
+__global__ void __launch_bounds__(NT) k_step(StepArgs a)
+{
+    using G = Geo<MX, MY>;
+    __shared__ double T[4 * G::NP2];
+    __shared__ double E[7 * G::NPOS];
+    __shared__ double F[6 * G::NF];
+    __shared__ double red[NT / 64];
+    const int bid = blockIdx.x;
+    if (bid >= a.count) return;
+    const int patch = a.first + bid;
+    const double* src = a.cur ? a.q1 : a.q0;
+    double* dst = a.q1 ? (a.cur ? a.q0 : a.q1) : a.q0;
+    double cfl = 0.0, hm = 0.0;
+    step_patch<MX, MY, NT, false>(T, E, F, red, a.desc + patch, patch, src, dst, a.aux, a.dt, a.P, cfl, hm);
+    if (threadIdx.x == 0) {
+        a.cfl_out[patch] = cfl;
+        if (a.hmax_out) a.hmax_out[patch] = hm;
+    }
+}
+
+// Persistent step kernel over the device work list of active patches:
+// each workgroup takes the next list entry from an atomic counter, pulls the
+// patch's halo from its neighbours, steps it, and folds its CFL number into
+// the scenario's maximum (atomicMax on the bit pattern; CFL >= 0).
+// One list entry of the persistent kernel.  (The library is built with
+// machine-level loop-invariant code motion off: hoisting the body's constant
+// and index materialisations out of the persistent loop would keep them live
+// across the whole step and spill.)
+template <int MX, int MY, int NT>
+__device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx)
+{
+    using G = Geo<MX, MY>;
+    __shared__ double T[4 * G::NP2];
+    __shared__ double E[7 * G::NPOS];
+    __shared__ double F[6 * G::NF];
+    __shared__ double red[NT / 64];
+    const BatchArgs& a = *pa;
+    const int patch = a.list_wet[idx];
+    const int scen = a.desc[patch].scen;
+    const int cur = a.ctrl[scen].cur;
+    const double dt = a.ctrl[scen].dt;
+    const double* src = cur ? a.q1 : a.q0;
+    double* dst = cur ? a.q0 : a.q1;
+    double cfl = 0.0, hm = 0.0;
+    step_patch<MX, MY, NT, true>(T, E, F, red, a.desc + patch, patch, src, dst, a.aux, dt, a.P, cfl, hm);
+    if (threadIdx.x == 0) {
+        a.hmax[(size_t)(cur ^ 1) * a.capacity + patch] = hm;
+        atomicMax(a.cflbits + scen, (unsigned long long)__double_as_longlong(cfl));
+    }
+    __syncthreads();
+}
+
+template <int MX, int MY, int NT>
+__global__ void __launch_bounds__(NT) k_wet(const BatchArgs* pa)
+{
+    __shared__ int item;
+    for (;;) {
+        if (threadIdx.x == 0) item = atomicAdd(&pa->counters[NEXT_WET], 1);
         __syncthreads();
-        hm = wave_max(hm);
-        if ((tid & 63) == 0) red[tid >> 6] = hm;
+        const int idx = item;
         __syncthreads();
-        if (tid == 0) {
-            double v = red[0];
-            for (int k = 1; k < NWAVE; k++) v = fmaxf64(v, red[k]);
-            a.hmax_out[patch] = v;
-        }
+        if (idx >= pa->counters[CNT_WET]) break;
+        wet_item<MX, MY, NT>(pa, idx);
     }
 }
 
@@ -542,11 +669,12 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs a)
     const int bid = blockIdx.x;
     if (bid >= a.count) return;
     const int patch = a.first + bid;
-    const PatchDesc d = a.desc[patch];
+    const PatchDesc* dp = a.desc + patch;
     int cur = a.cur;
     if (a.ctrl) {
-        if (d.scen < 0) return;
-        const ScenCtrl* c = a.ctrl + d.scen;
+        const int scen = dp->scen;
+        if (scen < 0) return;
+        const ScenCtrl* c = a.ctrl + scen;
         if (!c->active) return;
         cur = c->cur;
     }
@@ -555,46 +683,10 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs a)
 
     for (int c = threadIdx.x; c < NP2; c += blockDim.x) {
         const int fi = c % NXP - 1, fj = c / NXP - 1;  // Fortran indices
-        const int ox = fi < 1 ? -1 : (fi > MX ? 1 : 0);
-        const int oy = fj < 1 ? -1 : (fj > MY ? 1 : 0);
-        if (ox == 0 && oy == 0) continue;
-        const int nx = ox ? d.nbr[ox < 0 ? 0 : 1] : 0;
-        const int ny = oy ? d.nbr[oy < 0 ? 2 : 3] : 0;
-        // physical transforms along x / y
-        auto tx = [&](int v, int kind) {
-            if (kind == NB_WALL) return ox < 0 ? 1 - v : 2 * MX + 1 - v;
-            return ox < 0 ? 1 : MX;  // extrapolation
-        };
-        auto ty = [&](int v, int kind) {
-            if (kind == NB_WALL) return oy < 0 ? 1 - v : 2 * MY + 1 - v;
-            return oy < 0 ? 1 : MY;
-        };
-        int sp = -1, si = 0, sj = 0;
-        bool neg_u = false, neg_v = false;
-        if (ox != 0 && oy != 0) {
-            const int nc = d.nbr[4 + (ox > 0 ? 1 : 0) + (oy > 0 ? 2 : 0)];
-            const bool px = (nx == NB_WALL || nx == NB_EXTRAP);
-            const bool py = (ny == NB_WALL || ny == NB_EXTRAP);
-            if (nc >= 0) {
-                sp = nc; si = fi - ox * MX; sj = fj - oy * MY;
-            } else if (px && py) {
-                sp = patch; si = tx(fi, nx); sj = ty(fj, ny);
-                neg_u = (nx == NB_WALL); neg_v = (ny == NB_WALL);
-            } else if (px && ny >= 0) {
-                sp = ny; si = tx(fi, nx); sj = fj - oy * MY;
-                neg_u = (nx == NB_WALL);
-            } else if (py && nx >= 0) {
-                sp = nx; si = fi - ox * MX; sj = ty(fj, ny);
-                neg_v = (ny == NB_WALL);
-            }
-        } else if (ox != 0) {
-            if (nx >= 0) { sp = nx; si = fi - ox * MX; sj = fj; }
-            else if (nx == NB_WALL || nx == NB_EXTRAP) { sp = patch; si = tx(fi, nx); sj = fj; neg_u = (nx == NB_WALL); }
-        } else {
-            if (ny >= 0) { sp = ny; si = fi; sj = fj - oy * MY; }
-            else if (ny == NB_WALL || ny == NB_EXTRAP) { sp = patch; si = fi; sj = ty(fj, ny); neg_v = (ny == NB_WALL); }
-        }
-        if (sp < 0) continue;  // ghost supplied by the host (NB_STORED)
+        if (fi >= 1 && fi <= MX && fj >= 1 && fj <= MY) continue;
+        int sp, si, sj;
+        bool neg_u, neg_v;
+        if (!ghost_source<MX, MY>(dp->nbr, patch, fi, fj, sp, si, sj, neg_u, neg_v)) continue;
         const double* s = base + (size_t)sp * 3 * NP2;
         const int k = G::ti(si, sj);
         const double h = s[k], hu = s[NP2 + k], hv = s[2 * NP2 + k];
@@ -641,6 +733,150 @@ __global__ void __launch_bounds__(256) k_control(ScenCtrl* ctrl, int nscen, cons
                 c->active = 0;
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batched path: classification, dry-patch stream, controller.
+// ---------------------------------------------------------------------------
+
+// A patch is active for the next attempt when its tile (interior + 2-cell
+// halo from the 8 neighbours) may hold a cell with h > dry_tolerance: its own
+// or a neighbour's maximum interior depth exceeds the tolerance, or a side's
+// ghost cells come from the host.  Active patches go to the wet list (full
+// step), the others to the dry list.  Appends are aggregated per wavefront.
+__global__ void __launch_bounds__(256) k_classify(BatchArgs a)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int kind = -1;  // -1: not stepping, 0: dry, 1: active
+    if (p < a.n) {
+        const PatchDesc d = a.desc[p];
+        if (d.scen >= 0) {
+            const ScenCtrl* c = a.ctrl + d.scen;
+            if (c->active) {
+                const double* hm = a.hmax + (size_t)c->cur * a.capacity;
+                const double tol = a.P.dry_tolerance;
+                bool w = hm[p] > tol;
+#pragma unroll
+                for (int s = 0; s < 8; s++) {
+                    const int nb = d.nbr[s];
+                    if (nb >= 0) w = w || (hm[nb] > tol);
+                    else if (nb == NB_STORED) w = true;
+                }
+                kind = w ? 1 : 0;
+            }
+        }
+    }
+    const unsigned long long mw = __ballot(kind == 1);
+    const unsigned long long md = __ballot(kind == 0);
+    const int lane = threadIdx.x & 63;
+    int bw = 0, bd = 0;
+    if (lane == 0) {
+        if (mw) bw = atomicAdd(&a.counters[CNT_WET], __popcll(mw));
+        if (md) bd = atomicAdd(&a.counters[CNT_DRY], __popcll(md));
+    }
+    bw = __shfl(bw, 0, 64);
+    bd = __shfl(bd, 0, 64);
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    if (kind == 1) a.list_wet[bw + __popcll(mw & below)] = p;
+    if (kind == 0) a.list_dry[bd + __popcll(md & below)] = p;
+}
+
+// Dry patches: every cell of the tile has h <= dry_tolerance, so all waves,
+// fluctuations and transverse splits vanish and the step is b4step2 + src2
+// on the interior.  A cell with h < dry_tolerance becomes (max(h,0), 0, 0)
+// whatever its momentum (src2 then leaves zero momentum at zero), so only h
+// is read for it; hu, hv and b are read only for h == dry_tolerance.
+template <int MX, int MY>
+__global__ void __launch_bounds__(256) k_dry(BatchArgs a)
+{
+    using G = Geo<MX, MY>;
+    constexpr int NP2 = G::NP2;
+    constexpr int PER = (MX * MY + 255) / 256;
+    const int count = a.counters[CNT_DRY];
+    const double tol = a.P.dry_tolerance;
+    for (int k = blockIdx.x; k < count; k += gridDim.x) {
+        const int patch = a.list_dry[k];
+        const PatchDesc d = a.desc[patch];
+        const ScenCtrl* c = a.ctrl + d.scen;
+        const int cur = c->cur;
+        const double dt = c->dt;
+        const double* src = (cur ? a.q1 : a.q0) + (size_t)patch * 3 * NP2;
+        double* dst = (cur ? a.q0 : a.q1) + (size_t)patch * 3 * NP2;
+        const double* auxp = a.aux + (size_t)patch * NP2;
+        double hv_[PER];
+        int kk[PER];
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            const int e = threadIdx.x + 256 * r;
+            kk[r] = (e < MX * MY) ? G::ti(e % MX + 1, e / MX + 1) : -1;
+            hv_[r] = (kk[r] >= 0) ? src[kk[r]] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+            if (kk[r] < 0) continue;
+            double h = hv_[r], hu = 0.0, hv = 0.0;
+            if (h < tol) {
+                h = fmaxf64(h, 0.0);
+            } else {
+                hu = src[NP2 + kk[r]];
+                hv = src[2 * NP2 + kk[r]];
+                if (a.P.src_term > 0) src2_cell(h, hu, hv, auxp[kk[r]], dt, a.P);
+            }
+            dst[kk[r]] = h;
+            dst[NP2 + kk[r]] = hu;
+            dst[2 * NP2 + kk[r]] = hv;
+        }
+        if (threadIdx.x == 0) a.hmax[(size_t)(cur ^ 1) * a.capacity + patch] = 0.0;
+    }
+}
+
+// Controller of the batched path: the per-scenario CFL maximum was folded by
+// atomicMax in the step kernels; apply the dt rule (fc2d_geoclaw_run.c:
+// 529-565), clear the maxima and the list counters for the next attempt.
+__global__ void __launch_bounds__(64) k_batch_control(BatchArgs a)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) {
+        a.stats[0] += a.counters[CNT_WET];
+        a.stats[1] += a.counters[CNT_DRY];
+        a.stats[2] += 1;
+        for (int k = 0; k < NCOUNTERS; k++) a.counters[k] = 0;
+    }
+    if (s >= a.nscen) return;
+    ScenCtrl* c = a.ctrl + s;
+    if (!c->active) return;
+    const double m = __longlong_as_double((long long)a.cflbits[s]);
+    a.cflbits[s] = 0ull;
+    c->maxcfl = m;
+    c->attempts += 1;
+    if (m > c->max_cfl) {
+        c->retakes += 1;
+        c->dt = c->dt * c->desired_cfl / m;
+    } else {
+        c->t = c->t + c->dt;
+        c->last_dt = c->dt;
+        c->cur ^= 1;
+        c->nsteps += 1;
+        c->dt = c->dt * c->desired_cfl / m;
+        if ((c->target_steps > 0 && c->nsteps >= c->target_steps) ||
+            (c->t_end > 0.0 && c->t >= c->t_end))
+            c->active = 0;
+    }
+}
+
+// Maximum interior depth of patches [first, first+n) of a q buffer.
+template <int MX, int MY>
+__global__ void __launch_bounds__(256) k_hmax(const double* q, double* hmax, int first, int n)
+{
+    using G = Geo<MX, MY>;
+    __shared__ double red[4];
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    const double* src = q + (size_t)(first + b) * 3 * G::NP2;
+    double m = 0.0;
+    for (int e = threadIdx.x; e < MX * MY; e += 256) m = fmaxf64(m, src[G::ti(e % MX + 1, e / MX + 1)]);
+    m = block_max<256>(m, red);
+    if (threadIdx.x == 0) hmax[first + b] = m;
 }
 
 // ---------------------------------------------------------------------------
@@ -721,6 +957,53 @@ hipError_t launch_ghost(int mx, int my, const GhostArgs& a, hipStream_t st)
     case 32032: k_ghost<32, 32><<<a.count, 256, 0, st>>>(a); break;
     case 16016: k_ghost<16, 16><<<a.count, 256, 0, st>>>(a); break;
     case 8008: k_ghost<8, 8><<<a.count, 256, 0, st>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_classify(const BatchArgs& a, hipStream_t st)
+{
+    if (a.n <= 0) return hipSuccess;
+    k_classify<<<(a.n + 255) / 256, 256, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_wet(int mx, int my, const BatchArgs* da, int grid, hipStream_t st)
+{
+    switch (mx * 1000 + my) {
+    case 32032: k_wet<32, 32, 640><<<grid, 640, 0, st>>>(da); break;
+    case 16016: k_wet<16, 16, 384><<<grid, 384, 0, st>>>(da); break;
+    case 8008: k_wet<8, 8, 128><<<grid, 128, 0, st>>>(da); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_dry(int mx, int my, const BatchArgs& a, int grid, hipStream_t st)
+{
+    switch (mx * 1000 + my) {
+    case 32032: k_dry<32, 32><<<grid, 256, 0, st>>>(a); break;
+    case 16016: k_dry<16, 16><<<grid, 256, 0, st>>>(a); break;
+    case 8008: k_dry<8, 8><<<grid, 256, 0, st>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_control(const BatchArgs& a, hipStream_t st)
+{
+    k_batch_control<<<(a.nscen + 63) / 64 > 0 ? (a.nscen + 63) / 64 : 1, 64, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_hmax(int mx, int my, const double* q, double* hmax, int first, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    switch (mx * 1000 + my) {
+    case 32032: k_hmax<32, 32><<<n, 256, 0, st>>>(q, hmax, first, n); break;
+    case 16016: k_hmax<16, 16><<<n, 256, 0, st>>>(q, hmax, first, n); break;
+    case 8008: k_hmax<8, 8><<<n, 256, 0, st>>>(q, hmax, first, n); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -216,6 +216,11 @@ class Pool:
         check(self.lib.gf_pool_run(self.h, substeps, int(graph), ct.byref(act)), "gf_pool_run")
         return bool(act.value)
 
+    def stats(self) -> dict:
+        a, d, n = ct.c_longlong(0), ct.c_longlong(0), ct.c_longlong(0)
+        check(self.lib.gf_pool_stats(self.h, ct.byref(a), ct.byref(d), ct.byref(n)), "gf_pool_stats")
+        return dict(active_patches=a.value, dry_patches=d.value, attempts=n.value)
+
     def run_plain(self, substeps: int) -> bool:
         act = ct.c_int(0)
         check(self.lib.gf_pool_run_plain(self.h, substeps, ct.byref(act)), "gf_pool_run_plain")

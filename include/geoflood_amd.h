@@ -114,6 +114,33 @@ int gf_step2_batch(int mx, int my, int mbc, int maux, int n,
                    const double *dx, const double *dy,
                    double t, double dt, double *maxcfl);
 
+/* ---- plugin slot: per-patch callback with batching ---------------------- */
+
+typedef struct gf_plugin gf_plugin;
+
+/* Batching state of the single_step_update slot for patches of mx x my with
+ * maux aux fields, flushing every buffer_len patches.  Replaces the
+ * flux_array allocated by cudaclaw_update at iter == 0 (fc2d_cpucuda.cpp:
+ * 448-458) and the buffer-len option of the plugin (fc2d_cpucuda_options.c).
+ * Returns NULL on invalid arguments. */
+gf_plugin *gf_plugin_create(int mx, int my, int mbc, int maux, int buffer_len);
+
+/* Free the batching state. */
+int gf_plugin_destroy(gf_plugin *plugin);
+
+/* The single_step_update callback for patch number iter of total_count in
+ * the current sweep (ForestClaw's buffer_data): queues the host patch (q,
+ * aux with ghost cells filled, cell sizes dx, dy) and, when buffer_len
+ * patches are queued or iter is the last, advances the queued patches with
+ * gf_step2_batch.  *maxcfl is the batch's maximum per-patch CFL number
+ * (capped per patch) on flushing calls and 0 otherwise, so the caller's
+ * running maximum over all calls is the level's CFL number.  Replaces
+ * cudaclaw_update (fc2d_cpucuda.cpp:409-495) with cudaclaw_store_buffer
+ * (cuda_source/cudaclaw_store_patches.cu:13-42). */
+int gf_plugin_single_step_update(gf_plugin *plugin, double *q, double *aux, double dx,
+                                 double dy, int iter, int total_count, double t,
+                                 double dt, double *maxcfl);
+
 /* ---- device-resident patch pool ----------------------------------------- */
 
 /* Allocate a pool of `capacity` patches: two SoA q buffers with ghost cells,
@@ -203,9 +230,21 @@ int gf_pool_run_plain(gf_pool *pool, int substeps, int *any_active);
 int gf_synchronize(void);
 
 /* Device time (ms) of the last gf_pool_run / gf_pool_run_plain /
- * gf_pool_step* call and of its fused-step launches alone, from HIP events
- * on the library stream. */
+ * gf_pool_step* call and of its step launches alone (for gf_pool_run: from
+ * the end of classification to the end of both step kernels, per attempt,
+ * summed), from HIP events on the library streams; step_launches counts the
+ * timed step phases.  Graph runs report the total only. */
 int gf_last_timing(double *total_ms, double *step_ms, int *step_launches);
+
+/* Device time (ms) of the last non-graph gf_pool_run, summed over its
+ * attempts, of the active-patch kernel and of the dry-patch kernel; the two
+ * run concurrently, and gf_last_timing's step_ms spans both. */
+int gf_last_timing_branches(double *wet_ms, double *dry_ms);
+
+/* Patches stepped by the full kernel, patches handled as dry, and attempts,
+ * accumulated over all gf_pool_run calls on this pool. */
+int gf_pool_stats(gf_pool *pool, long long *active_patches, long long *dry_patches,
+                  long long *attempts);
 
 #ifdef __cplusplus
 }

@@ -176,3 +176,75 @@ def test_plain_equals_batched(gf):
     (i1, s1), (i2, s2) = out
     assert i1["nsteps"] == i2["nsteps"] == 12 and i1["t"] == i2["t"]
     assert np.array_equal(s1, s2)
+
+
+def test_dry_bed_dam_break_batched_split(gf, oracle_c):
+    """Dam break onto a dry fractal DEM (the 4096^2 workload at 512^2): the
+    batched path classifies patches into active and dry lists every attempt;
+    its states, dt sequence and wet/dry mask must match the oracle."""
+    from geoflood_amd.scenarios import uniform_dam_break
+    n, steps = 512, 60
+    h, hu, hv, b, dx = uniform_dam_break(n, dx=1.0, h0=10.0, rms=3.0, seed=1)
+    oracle_c.set_params(Params(manning_n=0.03))
+    gf.set_params(manning_n=0.03)
+    try:
+        print(f"case dry-bed dam break {n}^2 {steps} steps", flush=True)
+        br = Brick.from_global(h, hu, hv, b, n // 32, n // 32, 2, dx, dx)
+        rec = oracle_c.brick_run(br, steps, 0.01, 0.9, 1.0, nthreads=8)
+        layout = BrickLayout(n // 32, n // 32, 32, 32, dx, dx)
+        pool = Pool(32, 32, layout.npatch)
+        pool.set_layout([layout])
+        pool.upload_global(layout, h, hu, hv, b)
+        sid = pool.add_scenario(layout, dt0=0.01, target_steps=steps)
+        dts, prev = [], 0
+        while True:
+            active = pool.run(1, graph=True)
+            info = pool.scenario(sid)
+            if info["nsteps"] > prev:
+                dts.append(info["last_dt"]); prev = info["nsteps"]
+            if not active:
+                break
+        st = pool.stats()
+        assert st["dry_patches"] > 0 and st["active_patches"] > 0
+        np.testing.assert_allclose(dts, rec["dt"], rtol=1e-12, atol=0)
+        got = np.stack(pool.download_global(layout, buf=info["cur"]))
+        ref = br.to_global()
+        assert rel_err(got, ref) <= RTOL_RUN
+        assert np.array_equal(got[0] > 1e-3, ref[0] > 1e-3)
+    finally:
+        oracle_c.set_params(Params())
+        gf.set_params()
+
+
+def test_plugin_single_step_update_protocol(gf, oracle_c):
+    """The single_step_update slot over a host brick whose ghosts the host
+    fills in ForestClaw order: flushes every buffer_len patches and at the
+    last one, non-flushing calls return 0, and the running maximum equals
+    the oracle's level CFL; the patches equal the oracle's geoclaw_update."""
+    import ctypes as ct
+    rng = np.random.default_rng(21)
+    h = 1 + rng.random((96, 128)); hu = rng.normal(0, 0.5, h.shape) * h
+    hv = rng.normal(0, 0.5, h.shape) * h; b = rng.normal(0, 0.2, h.shape)
+    br = Brick.from_global(h, hu, hv, b, 4, 3, 2, 0.1, 0.1)
+    ref = br.copy()
+    cref = oracle_c.brick_advance(ref, 0.0, 0.004)
+    oracle_c.brick_ghost_fill(br)
+    lib = gf.load()
+    plug = lib.gf_plugin_create(32, 32, 2, 1, 5)
+    assert plug
+    total, m, flushes = 12, 0.0, 0
+    for it in range(total):
+        c = ct.c_double(-1)
+        rc = lib.gf_plugin_single_step_update(plug, gf.dptr(br.q[it]), gf.dptr(br.aux[it]), 0.1, 0.1,
+                                              it, total, 0.0, 0.004, ct.byref(c))
+        assert rc == 0
+        if (it + 1) % 5 == 0 or it + 1 == total:
+            flushes += 1
+            assert c.value > 0
+        else:
+            assert c.value == 0.0
+        m = max(m, c.value)
+    lib.gf_plugin_destroy(plug)
+    assert flushes == 3
+    assert m == pytest.approx(cref, rel=1e-15)
+    assert rel_err(br.to_global(), ref.to_global()) <= RTOL_STEP
