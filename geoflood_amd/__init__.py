@@ -56,6 +56,7 @@ PROTOTYPES = {
     "gf_pool_set_patches": (ct.c_int, [ct.c_void_p, ct.c_int, _dp, _dp, _ip, _ip]),
     "gf_pool_upload": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, _dp, _dp, ct.c_int]),
     "gf_pool_download": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, _dp]),
+    "gf_pool_describe": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _dp, _dp, _ip]),
     "gf_pool_ghost_fill": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int]),
     "gf_pool_step": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
                                 ct.c_double, _dp, _dp]),
@@ -65,6 +66,8 @@ PROTOTYPES = {
                                         ct.c_double, ct.c_double, ct.c_double, ct.c_int,
                                         ct.c_double]),
     "gf_pool_clear_scenarios": (ct.c_int, [ct.c_void_p]),
+    "gf_pool_remove_scenario": (ct.c_int, [ct.c_void_p, ct.c_int]),
+    "gf_pool_set_scenario_friction": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_double]),
     "gf_pool_set_scenario_active": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int]),
     "gf_pool_get_scenario": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(GfScenarioInfo)]),
     "gf_pool_run": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip]),

@@ -325,6 +325,18 @@ gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
     }
     p->h_desc.resize(capacity);
     std::memset(p->h_desc.data(), 0, sizeof(PatchDesc) * capacity);
+    for (auto& d : p->h_desc) {
+        d.scen = -1;
+        for (int k = 0; k < 8; k++) d.nbr[k] = GF_NB_STORED;
+    }
+    if (hipMemcpyAsync(p->desc, p->h_desc.data(), sizeof(PatchDesc) * capacity, hipMemcpyHostToDevice,
+                       g.stream) != hipSuccess ||
+        hipStreamSynchronize(g.stream) != hipSuccess) {
+        fail(GF_EHIP, "descriptor upload failed");
+        gf_pool_destroy(p);
+        return nullptr;
+    }
+    p->h_ctrl.assign(capacity, ScenCtrl{});
     return p;
 }
 
@@ -366,8 +378,9 @@ int gf_pool_set_patches(gf_pool* p, int n, const double* dx, const double* dy, c
         if (!(dx[k] > 0.0) || !(dy[k] > 0.0)) return fail(GF_EINVAL, "patch %d: dx, dy must be > 0", k);
         d.dx = dx[k];
         d.dy = dy[k];
-        d.scen = scen ? scen[k] : -1;
-        if (d.scen < -1) return fail(GF_EINVAL, "patch %d: scenario id %d invalid", k, d.scen);
+        if (scen && scen[k] != -1)
+            return fail(GF_EINVAL, "patch %d: scenario ids are assigned by gf_pool_add_scenario (pass -1)", k);
+        d.scen = -1;
         for (int s = 0; s < 8; s++) {
             const int v = nbr ? nbr[8 * k + s] : GF_NB_STORED;
             if (v >= n || v < GF_NB_STORED)
@@ -376,11 +389,42 @@ int gf_pool_set_patches(gf_pool* p, int n, const double* dx, const double* dy, c
         }
         d.pad = 0;
     }
+    for (int k = n; k < p->capacity; k++) p->h_desc[k].scen = -1;
     p->n = n;
-    if (n > 0) HIPCHK(hipMemcpyAsync(p->desc, p->h_desc.data(), sizeof(PatchDesc) * n, hipMemcpyHostToDevice, g.stream));
+    p->nscen = 0;
+    p->h_ctrl.assign(p->capacity, ScenCtrl{});
+    HIPCHK(hipMemcpyAsync(p->desc, p->h_desc.data(), sizeof(PatchDesc) * p->capacity, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemsetAsync(p->ctrl, 0, sizeof(ScenCtrl) * p->capacity, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     p->graph_dirty = true;
     p->args_dirty = true;
+    return GF_OK;
+}
+
+int gf_pool_describe(gf_pool* p, int first, int n, const double* dx, const double* dy, const int* nbr)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (first < 0 || n <= 0 || first + n > p->capacity) return fail(GF_EINVAL, "slots [%d,%d) outside the pool", first, first + n);
+    if (!dx || !dy || !nbr) return fail(GF_EINVAL, "NULL argument");
+    for (int k = 0; k < n; k++) {
+        PatchDesc& d = p->h_desc[first + k];
+        if (!(dx[k] > 0.0) || !(dy[k] > 0.0)) return fail(GF_EINVAL, "patch %d: dx, dy must be > 0", first + k);
+        for (int s = 0; s < 8; s++) {
+            const int v = nbr[8 * k + s];
+            if (v >= p->capacity || v < GF_NB_STORED)
+                return fail(GF_EINVAL, "patch %d: neighbour entry %d = %d invalid", first + k, s, v);
+        }
+    }
+    for (int k = 0; k < n; k++) {
+        PatchDesc& d = p->h_desc[first + k];
+        d.dx = dx[k];
+        d.dy = dy[k];
+        d.scen = -1;
+        for (int s = 0; s < 8; s++) d.nbr[s] = nbr[8 * k + s];
+    }
+    if (first + n > p->n) p->n = first + n;
+    HIPCHK(hipMemcpyAsync(p->desc + first, p->h_desc.data() + first, sizeof(PatchDesc) * n, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
 }
 
@@ -500,10 +544,18 @@ int gf_pool_add_scenario(gf_pool* p, int first_patch, int npatch, double t0, dou
 {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (first_patch < 0 || npatch <= 0 || first_patch + npatch > p->n)
-        return fail(GF_EINVAL, "scenario patches [%d,%d) outside [0,%d)", first_patch, first_patch + npatch, p->n);
+        return fail(GF_EINVAL, "scenario patches [%d,%d) outside the described slots [0,%d)", first_patch,
+                    first_patch + npatch, p->n);
     if (!(dt0 > 0.0) || !(desired_cfl > 0.0) || !(max_cfl > 0.0)) return fail(GF_EINVAL, "dt0, desired_cfl, max_cfl must be > 0");
-    if (p->nscen >= p->capacity) return fail(GF_EINVAL, "too many scenarios");
-    const int id = p->nscen;
+    for (int k = first_patch; k < first_patch + npatch; k++)
+        if (p->h_desc[k].scen >= 0) return fail(GF_EINVAL, "patch %d already belongs to scenario %d", k, p->h_desc[k].scen);
+    int id = -1;
+    for (int s = 0; s < p->capacity; s++)
+        if (p->h_ctrl[s].npatch == 0) {
+            id = s;
+            break;
+        }
+    if (id < 0) return fail(GF_EINVAL, "too many scenarios");
     ScenCtrl c{};
     c.t = t0;
     c.dt = dt0;
@@ -512,6 +564,7 @@ int gf_pool_add_scenario(gf_pool* p, int first_patch, int npatch, double t0, dou
     c.max_cfl = max_cfl;
     c.t_end = t_end;
     c.last_dt = 0.0;
+    c.manning_n = g.P.manning_n;
     c.cur = 0;
     c.nsteps = 0;
     c.target_steps = target_steps;
@@ -520,24 +573,52 @@ int gf_pool_add_scenario(gf_pool* p, int first_patch, int npatch, double t0, dou
     c.first_patch = first_patch;
     c.npatch = npatch;
     c.attempts = 0;
-    p->h_ctrl.resize(id + 1);
     p->h_ctrl[id] = c;
     for (int k = first_patch; k < first_patch + npatch; k++) p->h_desc[k].scen = id;
     HIPCHK(hipMemcpyAsync(p->desc + first_patch, p->h_desc.data() + first_patch, sizeof(PatchDesc) * npatch,
                           hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipMemcpyAsync(p->ctrl + id, &p->h_ctrl[id], sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
-    p->nscen = id + 1;
-    p->graph_dirty = true;
-    p->args_dirty = true;
+    if (id + 1 > p->nscen) p->nscen = id + 1;
     return id;
+}
+
+int gf_pool_remove_scenario(gf_pool* p, int s)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
+    const int first = p->h_ctrl[s].first_patch, n = p->h_ctrl[s].npatch;
+    for (int k = first; k < first + n; k++) p->h_desc[k].scen = -1;
+    p->h_ctrl[s] = ScenCtrl{};
+    HIPCHK(hipMemcpyAsync(p->desc + first, p->h_desc.data() + first, sizeof(PatchDesc) * n, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(p->ctrl + s, &p->h_ctrl[s], sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+}
+
+int gf_pool_set_scenario_friction(gf_pool* p, int s, double manning_n)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
+    if (!(manning_n >= 0.0)) return fail(GF_EINVAL, "manning_n must be >= 0");
+    HIPCHK(hipMemcpyAsync(&p->h_ctrl[s], p->ctrl + s, sizeof(ScenCtrl), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    p->h_ctrl[s].manning_n = manning_n;
+    HIPCHK(hipMemcpyAsync(p->ctrl + s, &p->h_ctrl[s], sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
 }
 
 int gf_pool_clear_scenarios(gf_pool* p)
 {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
+    for (int s = 0; s < p->capacity; s++)
+        if (p->h_ctrl[s].npatch > 0)
+            for (int k = p->h_ctrl[s].first_patch; k < p->h_ctrl[s].first_patch + p->h_ctrl[s].npatch; k++)
+                p->h_desc[k].scen = -1;
     p->nscen = 0;
-    p->h_ctrl.clear();
+    p->h_ctrl.assign(p->capacity, ScenCtrl{});
+    HIPCHK(hipMemcpyAsync(p->desc, p->h_desc.data(), sizeof(PatchDesc) * p->capacity, hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipMemsetAsync(p->ctrl, 0, sizeof(ScenCtrl) * p->capacity, g.stream));
     HIPCHK(hipMemsetAsync(p->cflbits, 0, sizeof(unsigned long long) * p->capacity, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
@@ -549,7 +630,7 @@ int gf_pool_clear_scenarios(gf_pool* p)
 int gf_pool_set_scenario_active(gf_pool* p, int s, int active)
 {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
-    if (s < 0 || s >= p->nscen) return fail(GF_EINVAL, "scenario %d unknown", s);
+    if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
     HIPCHK(hipMemcpyAsync(&p->h_ctrl[s], p->ctrl + s, sizeof(ScenCtrl), hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     p->h_ctrl[s].active = active ? 1 : 0;
@@ -561,7 +642,7 @@ int gf_pool_set_scenario_active(gf_pool* p, int s, int active)
 int gf_pool_get_scenario(gf_pool* p, int s, gf_scenario_info* info)
 {
     if (!p || !info) return fail(GF_EINVAL, "pool or info is NULL");
-    if (s < 0 || s >= p->nscen) return fail(GF_EINVAL, "scenario %d unknown", s);
+    if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
     ScenCtrl c;
     HIPCHK(hipMemcpyAsync(&c, p->ctrl + s, sizeof(ScenCtrl), hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
@@ -596,7 +677,7 @@ static int sync_batch_args(gf_pool* p)
     if (!p->args_dirty) return GF_OK;
     gf::BatchArgs a{};
     a.desc = p->desc;
-    a.n = p->n;
+    a.n = p->capacity;  // every slot; undescribed and idle slots carry scenario -1
     a.capacity = p->capacity;
     a.q0 = p->q[0];
     a.q1 = p->q[1];
@@ -604,7 +685,7 @@ static int sync_batch_args(gf_pool* p)
     a.hmax = p->hmax2;
     a.cflbits = p->cflbits;
     a.ctrl = p->ctrl;
-    a.nscen = p->nscen;
+    a.nscen = p->capacity;  // every scenario slot; idle ones are inactive
     a.list_wet = p->list_wet;
     a.list_dry = p->list_dry;
     a.counters = p->counters;
@@ -621,7 +702,7 @@ static gf::BatchArgs host_batch_args(gf_pool* p)
 {
     gf::BatchArgs a{};
     a.desc = p->desc;
-    a.n = p->n;
+    a.n = p->capacity;  // every slot; undescribed and idle slots carry scenario -1
     a.capacity = p->capacity;
     a.q0 = p->q[0];
     a.q1 = p->q[1];
@@ -629,7 +710,7 @@ static gf::BatchArgs host_batch_args(gf_pool* p)
     a.hmax = p->hmax2;
     a.cflbits = p->cflbits;
     a.ctrl = p->ctrl;
-    a.nscen = p->nscen;
+    a.nscen = p->capacity;  // every scenario slot; idle ones are inactive
     a.list_wet = p->list_wet;
     a.list_dry = p->list_dry;
     a.counters = p->counters;
@@ -664,7 +745,6 @@ static int enqueue_attempt(gf_pool* p, int k)
 static int any_active_flag(gf_pool* p, int* any_active)
 {
     if (!any_active) return GF_OK;
-    p->h_ctrl.resize(p->nscen);
     if (p->nscen > 0)
         HIPCHK(hipMemcpyAsync(p->h_ctrl.data(), p->ctrl, sizeof(ScenCtrl) * p->nscen, hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
@@ -738,7 +818,6 @@ int gf_pool_run_plain(gf_pool* p, int substeps, int* any_active)
     if (substeps < 0) return fail(GF_EINVAL, "substeps must be >= 0");
     if (p->nscen == 0 || p->n == 0) return fail(GF_ESTATE, "no scenario registered");
     std::vector<ScenCtrl>& hc = p->h_ctrl;
-    hc.resize(p->nscen);
     HIPCHK(hipMemcpyAsync(hc.data(), p->ctrl, sizeof(ScenCtrl) * p->nscen, hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     double total_ms = 0.0, step_ms = 0.0;

@@ -40,7 +40,7 @@ struct ScenCtrl {
     double max_cfl;
     double t_end;        // stop once t >= t_end (<= 0: no limit)
     double last_dt;      // dt of the last accepted step
-    double pad0;
+    double manning_n;    // Manning coefficient of this scenario's src2
     int cur;             // index of the q buffer holding the accepted state
     int nsteps;          // accepted steps
     int target_steps;    // stop after this many accepted steps (<= 0: no limit)

@@ -632,8 +632,10 @@ __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx)
     const double dt = a.ctrl[scen].dt;
     const double* src = cur ? a.q1 : a.q0;
     double* dst = cur ? a.q0 : a.q1;
+    Params P = a.P;
+    P.manning_n = a.ctrl[scen].manning_n;
     double cfl = 0.0, hm = 0.0;
-    step_patch<MX, MY, NT, true>(T, E, F, red, a.desc + patch, patch, src, dst, a.aux, dt, a.P, cfl, hm);
+    step_patch<MX, MY, NT, true>(T, E, F, red, a.desc + patch, patch, src, dst, a.aux, dt, P, cfl, hm);
     if (threadIdx.x == 0) {
         a.hmax[(size_t)(cur ^ 1) * a.capacity + patch] = hm;
         atomicMax(a.cflbits + scen, (unsigned long long)__double_as_longlong(cfl));
@@ -800,6 +802,8 @@ __global__ void __launch_bounds__(256) k_dry(BatchArgs a)
         const ScenCtrl* c = a.ctrl + d.scen;
         const int cur = c->cur;
         const double dt = c->dt;
+        Params P = a.P;
+        P.manning_n = c->manning_n;
         const double* src = (cur ? a.q1 : a.q0) + (size_t)patch * 3 * NP2;
         double* dst = (cur ? a.q0 : a.q1) + (size_t)patch * 3 * NP2;
         const double* auxp = a.aux + (size_t)patch * NP2;
@@ -820,7 +824,7 @@ __global__ void __launch_bounds__(256) k_dry(BatchArgs a)
             } else {
                 hu = src[NP2 + kk[r]];
                 hv = src[2 * NP2 + kk[r]];
-                if (a.P.src_term > 0) src2_cell(h, hu, hv, auxp[kk[r]], dt, a.P);
+                if (P.src_term > 0) src2_cell(h, hu, hv, auxp[kk[r]], dt, P);
             }
             dst[kk[r]] = h;
             dst[NP2 + kk[r]] = hu;

@@ -154,8 +154,9 @@ gf_pool *gf_pool_create(int mx, int my, int mbc, int capacity);
  * cudaclaw_deallocate_fluxes (cuda_source/cudaclaw_allocate.cu:88-145). */
 int gf_pool_destroy(gf_pool *pool);
 
-/* Describe patches [0, n): cell sizes, owning scenario (-1: none; patches
- * of no active scenario are skipped by gf_pool_run), and 8 neighbour
+/* Describe patches [0, n) and forget every scenario: cell sizes, scenario
+ * entries (NULL or all -1: membership is assigned by gf_pool_add_scenario;
+ * patches of no active scenario are skipped by gf_pool_run), and 8 neighbour
  * entries per patch (left, right, bottom, top, left-bottom, right-bottom,
  * left-top, right-top), each a patch index or GF_NB_*.  This is the
  * information ForestClaw's ghost exchange uses (fc2d_geoclaw_copy_fort.f:
@@ -163,6 +164,15 @@ int gf_pool_destroy(gf_pool *pool);
  * (fc2d_cpucuda.cpp:131-186). */
 int gf_pool_set_patches(gf_pool *pool, int n, const double *dx, const double *dy,
                         const int *scenario, const int *nbr);
+
+/* Describe pool slots [first, first+n) without touching the others: the
+ * admission path of a serving engine, which places a new scenario's forest in
+ * free slots while other scenarios keep stepping.  nbr holds 8 entries per
+ * patch as in gf_pool_set_patches (indices are pool slots).  The slots belong
+ * to no scenario until gf_pool_add_scenario.  Same reference interface as
+ * gf_pool_set_patches, per forest instead of per pool. */
+int gf_pool_describe(gf_pool *pool, int first, int n, const double *dx, const double *dy,
+                     const int *nbr);
 
 /* Copy n patches from host AoS arrays (contiguous, one patch after another,
  * reference layout with ghosts) into pool slots [first, first+n), q into
@@ -207,6 +217,16 @@ int gf_pool_add_scenario(gf_pool *pool, int first_patch, int npatch, double t0,
 
 /* Deactivate every scenario and forget them (slots keep their data). */
 int gf_pool_clear_scenarios(gf_pool *pool);
+
+/* Forget one scenario: its slots return to no scenario and its id may be
+ * reused by the next gf_pool_add_scenario.  Other scenarios are unaffected.
+ * Ends one outstyle_3 run (fc2d_geoclaw_run.c:417-590) of an ensemble. */
+int gf_pool_remove_scenario(gf_pool *pool, int scenario);
+
+/* Per-scenario Manning coefficient (the scenario starts with gf_params'
+ * manning_n).  Replaces manning_coefficient(1) of friction_module
+ * (fortran_source/geoclaw_src2_fort.f90:53) for one member of an ensemble. */
+int gf_pool_set_scenario_friction(gf_pool *pool, int scenario, double manning_n);
 
 /* Stop or restart a scenario without removing it. */
 int gf_pool_set_scenario_active(gf_pool *pool, int scenario, int active);
