@@ -940,6 +940,139 @@ void gfo_bc2(int mx, int my, int mbc, int maux, double *q, double *aux, const in
 }
 
 /* ------------------------------------------------------------------------- */
+/* Hydrograph inflow                                                          */
+/* ------------------------------------------------------------------------- */
+double gfo_hydro_interpolate(const gfo_hydrograph *h, double t)
+{
+    /* hydrograh_module.f90:186-196: clamp outside the table, else the first
+     * interval [time(i), time(i+1)] holding t, linear in t. */
+    const int n = h->nrows;
+    if (t < h->time[0])
+        return h->hu[0];
+    if (t > h->time[n - 1])
+        return h->hu[n - 1];
+    for (int i = 0; i < n - 1; i++)
+        if (t >= h->time[i] && t <= h->time[i + 1])
+            return h->hu[i] + (h->hu[i + 1] - h->hu[i]) / (h->time[i + 1] - h->time[i]) * (t - h->time[i]);
+    return 0.0;
+}
+
+/* tol = 1.0e-6 is a default-real literal in the reference: its double value. */
+#define HYDRO_TOL ((double)1.0e-6f)
+#define HYDRO_MAXITER 100
+
+void gfo_riemann_invariants(double q0[2], const double q1[2], double grav)
+{
+    /* hydrograh_module.f90:213-242 (discharge branch), Fr = 0.5 */
+    const double u1 = q1[1] / q1[0];
+    const double Fr = 0.5;
+    q0[0] = pow(q0[1] / sqrt(grav) * Fr, 2.0 / 3.0);
+    for (int it = 0; it < HYDRO_MAXITER; it++) {
+        const double fxn = q0[1] / q0[0] - 2.0 * sqrt(grav * q0[0]) - u1 + 2.0 * sqrt(grav * q1[0]);
+        if (fabs(fxn) < HYDRO_TOL)
+            return;
+        const double dfxn = -(q0[1] / (q0[0] * q0[0])) - sqrt(grav / q0[0]);
+        q0[0] = q0[0] - fxn / dfxn;
+    }
+}
+
+void gfo_two_shock(double q0[2], const double q1[2], double grav)
+{
+    /* hydrograh_module.f90:274-303 (discharge branch), Fr = 1 */
+    const double u1 = q1[1] / q1[0];
+    const double Fr = 1.0;
+    q0[0] = pow(q0[1] / sqrt(grav) * Fr, 2.0 / 3.0);
+    for (int it = 0; it < HYDRO_MAXITER; it++) {
+        const double fxn = q0[1] / q0[0] - u1 -
+                           (q0[0] - q1[0]) * sqrt((grav / 2.0) * (1.0 / q0[0] + 1.0 / q1[0]));
+        if (fabs(fxn) < HYDRO_TOL)
+            return;
+        const double num = sqrt(grav * (1.0 / q0[0] + 1.0 / q1[0]));
+        const double deno = 2.0 * sqrt(2.0) * (q0[0] * q0[0]) * num;
+        const double dfxn = -(q0[1] / (q0[0] * q0[0])) - num / sqrt(2.0) + grav * (q0[0] - q1[0]) / deno;
+        q0[0] = q0[0] - fxn / dfxn;
+    }
+}
+
+/* One side of bc2 with mthbc = 0 (geoflood_bc2_fort.f90:52-102 left,
+ * :141-183 right, :229-272 bottom, :318-361 top).  For the boundary cell of
+ * each row (column) the interpolated discharge sets the ghost state inside
+ * the channel strip: dry boundary cell -> critical depth with the discharge,
+ * wet with non-zero discharge -> depth from the Riemann invariant (two-shock
+ * when that depth exceeds the cell's), zero discharge -> wall.  Outside the
+ * strip the side is a wall.  Wall branches also mirror aux; the inflow
+ * branches leave aux alone.  The inflow state puts the discharge into q(2)
+ * on every side, as the reference does. */
+static void bc2_inflow_side(const gfo_params *p, int side, int mx, int my, int mbc, int maux,
+                            double *q, double *aux, double xlower, double ylower,
+                            double dx, double dy, double t, const gfo_hydrograph *hg)
+{
+    const double g = p->grav;
+    const int xside = side < 2;
+    const int n = xside ? my : mx;
+    const int nrm = xside ? 2 : 3;
+    for (int r = 1 - mbc; r <= n + mbc; r++) {
+        /* boundary cell (bi, bj) of this row */
+        const int bi = xside ? (side == 0 ? 1 : mx) : r;
+        const int bj = xside ? r : (side == 2 ? 1 : my);
+        const double c = xside ? ylower + ((double)r - 0.5) * dy : xlower + ((double)r - 0.5) * dx;
+        const double c0 = xside ? hg->y0 : hg->x0;
+        const double q1[2] = {P2(q, MEQN, 1, bi, bj), P2(q, MEQN, 2, bi, bj)};
+        double q0[2];
+        q0[1] = gfo_hydro_interpolate(hg, t);
+        const int in_strip = fabs(c - c0) <= hg->b / 2.0;
+        for (int k = 1; k <= mbc; k++) {
+            int gi, gj, mi_, mj_;
+            if (side == 0) { gi = 1 - k; gj = r; mi_ = k; mj_ = r; }
+            else if (side == 1) { gi = mx + k; gj = r; mi_ = mx + 1 - k; mj_ = r; }
+            else if (side == 2) { gi = r; gj = 1 - k; mi_ = r; mj_ = k; }
+            else { gi = r; gj = my + k; mi_ = r; mj_ = my + 1 - k; }
+            int mirror = !in_strip;
+            if (in_strip) {
+                if (q1[0] < p->dry_tolerance) {
+                    P2(q, MEQN, 1, gi, gj) = FMAX(pow(q0[1] / sqrt(g), 2.0 / 3.0), 0.001);
+                    P2(q, MEQN, 2, gi, gj) = q0[1];
+                    P2(q, MEQN, 3, gi, gj) = 0.0;
+                } else if (q0[1] != 0.0) {
+                    gfo_riemann_invariants(q0, q1, g);
+                    if (q0[0] > q1[0])
+                        gfo_two_shock(q0, q1, g);
+                    P2(q, MEQN, 1, gi, gj) = q0[0];
+                    P2(q, MEQN, 2, gi, gj) = q0[1];
+                    P2(q, MEQN, 3, gi, gj) = 0.0;
+                } else {
+                    mirror = 1;
+                }
+            }
+            if (mirror) {
+                P2(aux, maux, 1, gi, gj) = P2(aux, maux, 1, mi_, mj_);
+                for (int m = 1; m <= MEQN; m++)
+                    P2(q, MEQN, m, gi, gj) = P2(q, MEQN, m, mi_, mj_);
+                P2(q, MEQN, nrm, gi, gj) = -P2(q, MEQN, nrm, mi_, mj_);
+            }
+        }
+    }
+}
+
+int gfo_bc2_ex(const gfo_params *p, int mx, int my, int mbc, int maux, double *q,
+               double *aux, const int mthbc[4], double xlower, double ylower,
+               double dx, double dy, double t, const gfo_hydrograph *h)
+{
+    for (int side = 0; side < 4; side++) {
+        if (mthbc[side] == 0) {
+            if (!h || !h->side[side] || h->nrows < 1)
+                return -1;
+            bc2_inflow_side(p, side, mx, my, mbc, maux, q, aux, xlower, ylower, dx, dy, t, h);
+        } else {
+            int one[4] = {-1, -1, -1, -1};
+            one[side] = mthbc[side];
+            gfo_bc2(mx, my, mbc, maux, q, aux, one);
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
 double gfo_patch_update(const gfo_params *p, int mx, int my, int mbc, int maux,
                         double *q, double *aux, double dx, double dy,
                         double t, double dt)
@@ -965,10 +1098,12 @@ static double builtin_update(const void *ctx, int mx, int my, int mbc, int maux,
 /* ------------------------------------------------------------------------- */
 /* Uniform brick driver                                                      */
 /* ------------------------------------------------------------------------- */
-void gfo_brick_ghost_fill(int mi, int mj, int mx, int my, int mbc, int maux,
-                          double **q, double **aux, const int mthbc[4])
+int gfo_brick_ghost_fill_ex(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
+                            int maux, double **q, double **aux, const int mthbc[4],
+                            double xlo, double ylo, double dx, double dy, double t,
+                            const gfo_hydrograph *h)
 {
-    (void)maux;
+    int rc = 0;
     /* same-level face copies (q only; aux ghosts are set at creation) */
     for (int pj = 0; pj < mj; pj++)
         for (int pi = 0; pi < mi; pi++) {
@@ -1029,15 +1164,26 @@ void gfo_brick_ghost_fill(int mi, int mj, int mx, int my, int mbc, int maux,
             bc[1] = (pi == mi - 1) ? mthbc[1] : -1;
             bc[2] = (pj == 0) ? mthbc[2] : -1;
             bc[3] = (pj == mj - 1) ? mthbc[3] : -1;
-            gfo_bc2(mx, my, mbc, maux, q[pj * mi + pi], aux[pj * mi + pi], bc);
+            if (gfo_bc2_ex(p, mx, my, mbc, maux, q[pj * mi + pi], aux[pj * mi + pi], bc,
+                           xlo + pi * mx * dx, ylo + pj * my * dy, dx, dy, t, h) != 0)
+                rc = -1;
         }
+    return rc;
 }
 
-double gfo_brick_advance(const gfo_params *p, int mi, int mj, int mx, int my,
-                         int mbc, int maux, double **q, double **aux,
-                         const int mthbc[4], double dx, double dy, double t,
-                         double dt, gfo_update_fn fn, const void *ctx,
-                         int nthreads)
+void gfo_brick_ghost_fill(int mi, int mj, int mx, int my, int mbc, int maux,
+                          double **q, double **aux, const int mthbc[4])
+{
+    gfo_params p;
+    memset(&p, 0, sizeof(p));
+    gfo_brick_ghost_fill_ex(&p, mi, mj, mx, my, mbc, maux, q, aux, mthbc, 0.0, 0.0, 1.0, 1.0, 0.0, NULL);
+}
+
+static double brick_advance(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
+                            int maux, double **q, double **aux, const int mthbc[4],
+                            double xlo, double ylo, double dx, double dy,
+                            const gfo_hydrograph *h, double t, double dt, gfo_update_fn fn,
+                            const void *ctx, int nthreads)
 {
     const int np = mi * mj;
     double maxcfl = 0.0;
@@ -1046,7 +1192,7 @@ double gfo_brick_advance(const gfo_params *p, int mi, int mj, int mx, int my,
         fn = builtin_update;
         ctx = p;
     }
-    gfo_brick_ghost_fill(mi, mj, mx, my, mbc, maux, q, aux, mthbc);
+    gfo_brick_ghost_fill_ex(p, mi, mj, mx, my, mbc, maux, q, aux, mthbc, xlo, ylo, dx, dy, t, h);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
     for (int k = 0; k < np; k++)
         cfl[k] = fn(ctx, mx, my, mbc, maux, q[k], aux[k], dx, dy, t, dt);
@@ -1056,12 +1202,23 @@ double gfo_brick_advance(const gfo_params *p, int mi, int mj, int mx, int my,
     return maxcfl;
 }
 
-int gfo_brick_run(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
-                  int maux, double **q, double **aux, const int mthbc[4],
-                  double dx, double dy, double t0, double initial_dt,
-                  double desired_cfl, double max_cfl, int nsteps,
-                  gfo_update_fn fn, const void *ctx, int nthreads,
-                  double *dt_out, double *cfl_out, double *mass_out)
+double gfo_brick_advance(const gfo_params *p, int mi, int mj, int mx, int my,
+                         int mbc, int maux, double **q, double **aux,
+                         const int mthbc[4], double dx, double dy, double t,
+                         double dt, gfo_update_fn fn, const void *ctx,
+                         int nthreads)
+{
+    return brick_advance(p, mi, mj, mx, my, mbc, maux, q, aux, mthbc, 0.0, 0.0, dx, dy, NULL, t, dt,
+                         fn, ctx, nthreads);
+}
+
+int gfo_brick_run_ex(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
+                     int maux, double **q, double **aux, const int mthbc[4],
+                     double xlo, double ylo, double dx, double dy,
+                     const gfo_hydrograph *h, double t0, double initial_dt,
+                     double desired_cfl, double max_cfl, int nsteps,
+                     gfo_update_fn fn, const void *ctx, int nthreads,
+                     double *dt_out, double *cfl_out, double *mass_out)
 {
     const int np = mi * mj;
     const size_t nq = (size_t)MEQN * (mx + 2 * mbc) * (my + 2 * mbc);
@@ -1073,8 +1230,8 @@ int gfo_brick_run(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
         double maxcfl;
         for (int k = 0; k < np; k++)
             memcpy(save + nq * k, q[k], sizeof(double) * nq);
-        maxcfl = gfo_brick_advance(p, mi, mj, mx, my, mbc, maux, q, aux, mthbc,
-                                   dx, dy, t, dt, fn, ctx, nthreads);
+        maxcfl = brick_advance(p, mi, mj, mx, my, mbc, maux, q, aux, mthbc, xlo, ylo, dx, dy, h,
+                               t, dt, fn, ctx, nthreads);
         if (maxcfl > max_cfl) {
             for (int k = 0; k < np; k++)
                 memcpy(q[k], save + nq * k, sizeof(double) * nq);
@@ -1103,4 +1260,16 @@ int gfo_brick_run(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
     }
     free(save);
     return retakes;
+}
+
+int gfo_brick_run(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
+                  int maux, double **q, double **aux, const int mthbc[4],
+                  double dx, double dy, double t0, double initial_dt,
+                  double desired_cfl, double max_cfl, int nsteps,
+                  gfo_update_fn fn, const void *ctx, int nthreads,
+                  double *dt_out, double *cfl_out, double *mass_out)
+{
+    return gfo_brick_run_ex(p, mi, mj, mx, my, mbc, maux, q, aux, mthbc, 0.0, 0.0, dx, dy, NULL, t0,
+                            initial_dt, desired_cfl, max_cfl, nsteps, fn, ctx, nthreads, dt_out,
+                            cfl_out, mass_out);
 }

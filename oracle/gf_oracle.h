@@ -86,6 +86,37 @@ void gfo_src2(const gfo_params *p, int mx, int my, int mbc, int maux,
 void gfo_bc2(int mx, int my, int mbc, int maux, double *q, double *aux,
              const int mthbc[4]);
 
+/* Hydrograph inflow (hydrograph/hydrograh_module.f90) as used by bc2's
+ * mthbc = 0 branch.  Only hydrograph_type 'discharge' is restated: the 'eta'
+ * type leaves q0(2), which bc2 then reads, undefined in the reference.
+ * hu[k] is the discharge per unit width at time[k]; b, x0, y0 are the
+ * channel width and centre; side[k] != 0 enables the inflow on bc2 side k
+ * (left, right, bottom, top), i.e. boundary_location(1) == 'left',
+ * (2) == 'right', (4) == 'bottom', (3) == 'top' in the reference. */
+typedef struct gfo_hydrograph {
+    int nrows;
+    const double *time;
+    const double *hu;
+    double b, x0, y0;
+    int side[4];
+} gfo_hydrograph;
+
+/* interpolation (hydrograh_module.f90:170-210): the discharge at time t. */
+double gfo_hydro_interpolate(const gfo_hydrograph *h, double t);
+
+/* Riemann_invariants and two_shock (hydrograh_module.f90:213-331),
+ * discharge form: q0 = {h0, hu0} (h0 is solved for), q1 = {h1, hu1}. */
+void gfo_riemann_invariants(double q0[2], const double q1[2], double grav);
+void gfo_two_shock(double q0[2], const double q1[2], double grav);
+
+/* fc2d_geoclaw_bc2 (geoflood_bc2_fort.f90:7-499) for mthbc in {-1, 0
+ * (hydrograph inflow), 1, 3}; sides processed left, right, bottom, top.
+ * Returns -1 (the reference stops) when a side has mthbc = 0 but h is NULL
+ * or does not enable that side, else 0. */
+int gfo_bc2_ex(const gfo_params *p, int mx, int my, int mbc, int maux, double *q,
+               double *aux, const int mthbc[4], double xlower, double ylower,
+               double dx, double dy, double t, const gfo_hydrograph *h);
+
 /* geoclaw_update (fc2d_cpucuda.cpp:362-406): b4step2, step2 (+update), the
  * per-patch MAXCFL_CAP (1000 -> 0), then src2 when src_term > 0. */
 double gfo_patch_update(const gfo_params *p, int mx, int my, int mbc, int maux,
@@ -126,6 +157,22 @@ int gfo_brick_run(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
                   double desired_cfl, double max_cfl, int nsteps,
                   gfo_update_fn fn, const void *ctx, int nthreads,
                   double *dt_out, double *cfl_out, double *mass_out);
+
+/* The same driver with the brick's lower-left corner (xlo, ylo), so that bc2
+ * sees each patch's xlower/ylower, and an optional hydrograph for sides with
+ * mthbc = 0.  Return -1 on a bc2 error (see gfo_bc2_ex). */
+int gfo_brick_ghost_fill_ex(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
+                            int maux, double **q, double **aux, const int mthbc[4],
+                            double xlo, double ylo, double dx, double dy, double t,
+                            const gfo_hydrograph *h);
+
+int gfo_brick_run_ex(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
+                     int maux, double **q, double **aux, const int mthbc[4],
+                     double xlo, double ylo, double dx, double dy,
+                     const gfo_hydrograph *h, double t0, double initial_dt,
+                     double desired_cfl, double max_cfl, int nsteps,
+                     gfo_update_fn fn, const void *ctx, int nthreads,
+                     double *dt_out, double *cfl_out, double *mass_out);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,39 @@ def iptr(a: np.ndarray):
     return a.ctypes.data_as(_ip)
 
 
+@dataclass
+class Hydrograph:
+    """Discharge hydrograph of the bc2 inflow (hydrograh_module.f90): hu[k]
+    is the discharge per unit width at time[k]; the channel strip has width
+    b about x0 (bottom/top sides) or y0 (left/right sides); sides are the
+    bc2 sides (left, right, bottom, top) whose mthbc = 0 uses it."""
+    time: np.ndarray
+    hu: np.ndarray
+    b: float
+    x0: float
+    y0: float
+    sides: tuple = (1, 0, 0, 0)
+
+    def __post_init__(self):
+        self.time = np.ascontiguousarray(self.time, dtype=np.float64)
+        self.hu = np.ascontiguousarray(self.hu, dtype=np.float64)
+
+    def to_c(self):
+        h = GfoHydrograph()
+        h.nrows = len(self.time)
+        h.time = self.time.ctypes.data_as(_dp)
+        h.hu = self.hu.ctypes.data_as(_dp)
+        h.b, h.x0, h.y0 = self.b, self.x0, self.y0
+        for k in range(4):
+            h.side[k] = int(self.sides[k])
+        return h
+
+
+class GfoHydrograph(ct.Structure):
+    _fields_ = [("nrows", ct.c_int), ("time", _dp), ("hu", _dp), ("b", ct.c_double),
+                ("x0", ct.c_double), ("y0", ct.c_double), ("side", ct.c_int * 4)]
+
+
 class _Lib:
     """Common wrappers over either implementation."""
 
@@ -181,6 +214,40 @@ class CLib(_Lib):
         fn.restype = ct.c_double
         return fn(ct.byref(self._cp), mx, my, mbc, maux, ptr(q), ptr(aux), dx, dy, t, dt)
 
+    # --- hydrograph inflow -------------------------------------------------
+    def hydro_interpolate(self, hyd: Hydrograph, t):
+        fn = self.lib.gfo_hydro_interpolate
+        fn.argtypes = [ct.POINTER(GfoHydrograph), ct.c_double]
+        fn.restype = ct.c_double
+        h = hyd.to_c()
+        return fn(ct.byref(h), t)
+
+    def riemann_invariants(self, q0, q1, grav=9.81):
+        a = np.ascontiguousarray(q0, dtype=np.float64).copy()
+        c = np.ascontiguousarray(q1, dtype=np.float64)
+        fn = self.lib.gfo_riemann_invariants
+        fn.argtypes = [_dp, _dp, ct.c_double]
+        fn(ptr(a), ptr(c), grav)
+        return a
+
+    def two_shock(self, q0, q1, grav=9.81):
+        a = np.ascontiguousarray(q0, dtype=np.float64).copy()
+        c = np.ascontiguousarray(q1, dtype=np.float64)
+        fn = self.lib.gfo_two_shock
+        fn.argtypes = [_dp, _dp, ct.c_double]
+        fn(ptr(a), ptr(c), grav)
+        return a
+
+    def bc2_ex(self, mx, my, mbc, q, aux, mthbc, xlower, ylower, dx, dy, t, hyd=None, maux=1):
+        mb = np.ascontiguousarray(mthbc, dtype=np.int32)
+        fn = self.lib.gfo_bc2_ex
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 4 + [_dp, _dp, _ip] + [ct.c_double] * 5 + \
+            [ct.c_void_p]
+        fn.restype = ct.c_int
+        h = None if hyd is None else hyd.to_c()
+        return fn(ct.byref(self._cp), mx, my, mbc, maux, ptr(q), ptr(aux), iptr(mb), xlower, ylower,
+                  dx, dy, t, None if h is None else ct.cast(ct.byref(h), ct.c_void_p))
+
     # --- brick driver -------------------------------------------------------
     def brick_run(self, brick: "Brick", nsteps, initial_dt, desired_cfl=0.9, max_cfl=1.0,
                   t0=0.0, update=None, nthreads=1):
@@ -194,12 +261,16 @@ class CLib(_Lib):
         fnp = None
         if update is not None:
             fnp = ct.cast(update.lib.gfref_patch_update, ct.c_void_p)
-        fn = self.lib.gfo_brick_run
+        h = None if brick.hydro is None else brick.hydro.to_c()
+        fn = self.lib.gfo_brick_run_ex
         fn.argtypes = [ct.c_void_p] + [ct.c_int] * 6 + [ct.POINTER(_dp), ct.POINTER(_dp), _ip] + \
-            [ct.c_double] * 6 + [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int, _dp, _dp, _dp]
+            [ct.c_double] * 4 + [ct.c_void_p] + [ct.c_double] * 4 + \
+            [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int, _dp, _dp, _dp]
         fn.restype = ct.c_int
         retakes = fn(ct.byref(self._cp), brick.mi, brick.mj, brick.mx, brick.my, brick.mbc, 1,
-                     qp, ap, iptr(mb), brick.dx, brick.dy, t0, initial_dt, desired_cfl, max_cfl,
+                     qp, ap, iptr(mb), brick.xlo, brick.ylo, brick.dx, brick.dy,
+                     None if h is None else ct.cast(ct.byref(h), ct.c_void_p),
+                     t0, initial_dt, desired_cfl, max_cfl,
                      nsteps, fnp, None, nthreads, ptr(dt_out), ptr(cfl_out), ptr(mass_out))
         return dict(dt=dt_out, cfl=cfl_out, mass=mass_out, retakes=retakes)
 
@@ -216,14 +287,19 @@ class CLib(_Lib):
         return fn(ct.byref(self._cp), brick.mi, brick.mj, brick.mx, brick.my, brick.mbc, 1,
                   qp, ap, iptr(mb), brick.dx, brick.dy, t, dt, fnp, None, nthreads)
 
-    def brick_ghost_fill(self, brick: "Brick"):
+    def brick_ghost_fill(self, brick: "Brick", t=0.0):
         np_ = brick.mi * brick.mj
         qp = (_dp * np_)(*[ptr(a) for a in brick.q])
         ap = (_dp * np_)(*[ptr(a) for a in brick.aux])
         mb = np.ascontiguousarray(brick.mthbc, dtype=np.int32)
-        fn = self.lib.gfo_brick_ghost_fill
-        fn.argtypes = [ct.c_int] * 6 + [ct.POINTER(_dp), ct.POINTER(_dp), _ip]
-        fn(brick.mi, brick.mj, brick.mx, brick.my, brick.mbc, 1, qp, ap, iptr(mb))
+        h = None if brick.hydro is None else brick.hydro.to_c()
+        fn = self.lib.gfo_brick_ghost_fill_ex
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 6 + [ct.POINTER(_dp), ct.POINTER(_dp), _ip] + \
+            [ct.c_double] * 5 + [ct.c_void_p]
+        fn.restype = ct.c_int
+        return fn(ct.byref(self._cp), brick.mi, brick.mj, brick.mx, brick.my, brick.mbc, 1, qp, ap,
+                  iptr(mb), brick.xlo, brick.ylo, brick.dx, brick.dy, t,
+                  None if h is None else ct.cast(ct.byref(h), ct.c_void_p))
 
 
 class RefLib(_Lib):
@@ -296,6 +372,40 @@ class RefLib(_Lib):
         fn.argtypes = [ct.c_int] * 4 + [_dp, _dp, _ip]
         fn(mx, my, mbc, maux, ptr(q), ptr(aux), iptr(mb))
 
+    def set_hydrograph(self, hyd: Hydrograph):
+        fn = self.lib.gfref_set_hydrograph
+        fn.argtypes = [ct.c_int, _dp, _dp, ct.c_double, ct.c_double, ct.c_double, _ip]
+        sides = np.ascontiguousarray(hyd.sides, dtype=np.int32)
+        fn(len(hyd.time), ptr(hyd.time), ptr(hyd.hu), hyd.b, hyd.x0, hyd.y0, iptr(sides))
+
+    def hydro_interpolate(self, t):
+        fn = self.lib.gfref_hydro_interpolate
+        fn.argtypes = [ct.c_double]
+        fn.restype = ct.c_double
+        return fn(t)
+
+    def riemann_invariants(self, q0, q1):
+        a = np.ascontiguousarray(q0, dtype=np.float64).copy()
+        c = np.ascontiguousarray(q1, dtype=np.float64)
+        fn = self.lib.gfref_riemann_invariants
+        fn.argtypes = [_dp, _dp]
+        fn(ptr(a), ptr(c))
+        return a
+
+    def two_shock(self, q0, q1):
+        a = np.ascontiguousarray(q0, dtype=np.float64).copy()
+        c = np.ascontiguousarray(q1, dtype=np.float64)
+        fn = self.lib.gfref_two_shock
+        fn.argtypes = [_dp, _dp]
+        fn(ptr(a), ptr(c))
+        return a
+
+    def bc2_ex(self, mx, my, mbc, q, aux, mthbc, xlower, ylower, dx, dy, t, maux=1):
+        mb = np.ascontiguousarray(mthbc, dtype=np.int32)
+        fn = self.lib.gfref_bc2_ex
+        fn.argtypes = [ct.c_int] * 4 + [_dp, _dp, _ip] + [ct.c_double] * 5
+        fn(mx, my, mbc, maux, ptr(q), ptr(aux), iptr(mb), xlower, ylower, dx, dy, t)
+
     def patch_update(self, mx, my, mbc, q, aux, dx, dy, t, dt, maux=1):
         fn = self.lib.gfref_patch_update
         fn.argtypes = [ct.c_void_p] + [ct.c_int] * 4 + [_dp, _dp] + [ct.c_double] * 4
@@ -318,9 +428,13 @@ class Brick:
     q: list
     aux: list
     mthbc: tuple = (3, 3, 3, 3)
+    xlo: float = 0.0
+    ylo: float = 0.0
+    hydro: Hydrograph | None = None
 
     @classmethod
-    def from_global(cls, h, hu, hv, b, mi, mj, mbc, dx, dy, mthbc=(3, 3, 3, 3)):
+    def from_global(cls, h, hu, hv, b, mi, mj, mbc, dx, dy, mthbc=(3, 3, 3, 3), xlo=0.0, ylo=0.0,
+                    hydro=None):
         """Split global interior fields (shape (ny, nx)) into patches.  Ghost
         values of aux are taken from a globally padded b with the wall/extrap
         rule applied (bc2 then re-applies it on the boundary every step)."""
@@ -339,7 +453,7 @@ class Brick:
                     bpad[pj * my:pj * my + my + 2 * mbc, pi * mx:pi * mx + mx + 2 * mbc, None])
                 q.append(qq)
                 aux.append(aa)
-        return cls(mi, mj, mx, my, mbc, dx, dy, q, aux, tuple(mthbc))
+        return cls(mi, mj, mx, my, mbc, dx, dy, q, aux, tuple(mthbc), xlo, ylo, hydro)
 
     def to_global(self):
         """Interior fields (h, hu, hv) as global arrays of shape (ny, nx)."""
@@ -354,12 +468,14 @@ class Brick:
 
     def copy(self):
         return Brick(self.mi, self.mj, self.mx, self.my, self.mbc, self.dx, self.dy,
-                     [a.copy() for a in self.q], [a.copy() for a in self.aux], self.mthbc)
+                     [a.copy() for a in self.q], [a.copy() for a in self.aux], self.mthbc,
+                     self.xlo, self.ylo, self.hydro)
 
 
 def pad_aux(b, mbc, mthbc=(3, 3, 3, 3)):
-    """Globally padded bathymetry: walls (3) mirror, extrapolation (1) copies
-    the edge cell, applied left, right, bottom, top as bc2 does."""
+    """Globally padded bathymetry: walls (3) and hydrograph sides (0, a wall
+    outside the channel strip) mirror, extrapolation (1) copies the edge
+    cell, applied left, right, bottom, top as bc2 does."""
     ny, nx = b.shape
     out = np.zeros((ny + 2 * mbc, nx + 2 * mbc))
     out[mbc:mbc + ny, mbc:mbc + nx] = b
@@ -367,13 +483,13 @@ def pad_aux(b, mbc, mthbc=(3, 3, 3, 3)):
         kind = mthbc[side]
         for k in range(1, mbc + 1):
             if side == 0:
-                out[:, mbc - k] = out[:, mbc + k - 1] if kind == 3 else out[:, mbc]
+                out[:, mbc - k] = out[:, mbc + k - 1] if kind in (0, 3) else out[:, mbc]
             elif side == 1:
-                out[:, mbc + nx - 1 + k] = out[:, mbc + nx - k] if kind == 3 else out[:, mbc + nx - 1]
+                out[:, mbc + nx - 1 + k] = out[:, mbc + nx - k] if kind in (0, 3) else out[:, mbc + nx - 1]
             elif side == 2:
-                out[mbc - k, :] = out[mbc + k - 1, :] if kind == 3 else out[mbc, :]
+                out[mbc - k, :] = out[mbc + k - 1, :] if kind in (0, 3) else out[mbc, :]
             else:
-                out[mbc + ny - 1 + k, :] = out[mbc + ny - k, :] if kind == 3 else out[mbc + ny - 1, :]
+                out[mbc + ny - 1 + k, :] = out[mbc + ny - k, :] if kind in (0, 3) else out[mbc + ny - 1, :]
     return out
 
 

@@ -198,6 +198,89 @@ contains
             0.d0, 0.d0, mb)
     end subroutine gfref_bc2
 
+    ! Hydrograph module state that read_hydrograph_data would read from
+    ! hydrograph.data (hydrograh_module.f90:35-144): discharge type, the
+    ! table, the channel strip and the enabled sides (left, right, bottom,
+    ! top -> boundary_location(1), (2), (4), (3)).
+    subroutine gfref_set_hydrograph(nrows, t_in, hu_in, b_in, x0_in, y0_in, sides) &
+            bind(C, name="gfref_set_hydrograph")
+        use hydrograph_module, only: use_hydrograph, hydrograph_type, boundary_location, &
+            hb => b, hx0 => x0, hy0 => y0, htime => time, heta => eta, hhu => hu, num_rows
+        integer(c_int), value :: nrows
+        real(c_double), intent(in) :: t_in(nrows), hu_in(nrows)
+        real(c_double), value :: b_in, x0_in, y0_in
+        integer(c_int), intent(in) :: sides(4)
+        use_hydrograph = 'True'
+        hydrograph_type = 'discharge'
+        boundary_location(:) = 'none'
+        if (sides(1) /= 0) boundary_location(1) = 'left'
+        if (sides(2) /= 0) boundary_location(2) = 'right'
+        if (sides(3) /= 0) boundary_location(4) = 'bottom'
+        if (sides(4) /= 0) boundary_location(3) = 'top'
+        hb = b_in
+        hx0 = x0_in
+        hy0 = y0_in
+        num_rows = nrows
+        if (allocated(htime)) deallocate(htime)
+        if (allocated(heta)) deallocate(heta)
+        if (allocated(hhu)) deallocate(hhu)
+        allocate(htime(nrows), heta(nrows), hhu(nrows))
+        htime = t_in
+        hhu = hu_in
+        heta = 0.d0
+    end subroutine gfref_set_hydrograph
+
+    ! inflow_interpolate (hydrograh_module.f90:155-168): returns q0(2).
+    real(c_double) function gfref_hydro_interpolate(t) bind(C, name="gfref_hydro_interpolate")
+        use hydrograph_module, only: inflow_interpolate
+        real(c_double), value :: t
+        double precision :: q0(4), q1(4)
+        q0 = 0.d0
+        q1 = 0.d0
+        call inflow_interpolate(t, q0, q1)
+        gfref_hydro_interpolate = q0(2)
+    end function gfref_hydro_interpolate
+
+    subroutine gfref_riemann_invariants(q0, q1) bind(C, name="gfref_riemann_invariants")
+        use hydrograph_module, only: Riemann_invariants
+        real(c_double), intent(inout) :: q0(2)
+        real(c_double), intent(in) :: q1(2)
+        double precision :: a(4), c(4)
+        a = 0.d0
+        c = 0.d0
+        a(1:2) = q0
+        c(1:2) = q1
+        call Riemann_invariants(a, c)
+        q0 = a(1:2)
+    end subroutine gfref_riemann_invariants
+
+    subroutine gfref_two_shock(q0, q1) bind(C, name="gfref_two_shock")
+        use hydrograph_module, only: two_shock
+        real(c_double), intent(inout) :: q0(2)
+        real(c_double), intent(in) :: q1(2)
+        double precision :: a(4), c(4)
+        a = 0.d0
+        c = 0.d0
+        a(1:2) = q0
+        c(1:2) = q1
+        call two_shock(a, c)
+        q0 = a(1:2)
+    end subroutine gfref_two_shock
+
+    ! fc2d_geoclaw_bc2 with the patch geometry and time (mthbc = 0 uses the
+    ! hydrograph set above).
+    subroutine gfref_bc2_ex(mx, my, mbc, maux, q, aux, mthbc, xlower, ylower, dx, dy, t) &
+            bind(C, name="gfref_bc2_ex")
+        integer(c_int), value :: mx, my, mbc, maux
+        real(c_double) :: q(*), aux(*)
+        integer(c_int), intent(in) :: mthbc(4)
+        real(c_double), value :: xlower, ylower, dx, dy, t
+        integer :: mb(4)
+        mb = mthbc
+        call fc2d_geoclaw_bc2(3, mbc, mx, my, xlower, ylower, dx, dy, q, maux, aux, &
+            t, 0.d0, mb)
+    end subroutine gfref_bc2_ex
+
     ! geoclaw_update (fc2d_cpucuda.cpp:362-406): b4step2, step2, MAXCFL_CAP,
     ! src2.  Signature matches gfo_update_fn so the C brick driver can run it.
     real(c_double) function gfref_patch_update(ctx, mx, my, mbc, maux, q, aux, &
