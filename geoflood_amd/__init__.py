@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgeoflood_amd.so")
 
 GF_OK = 0
-GF_NB_WALL, GF_NB_EXTRAP, GF_NB_STORED = -1, -2, -3
+GF_NB_WALL, GF_NB_EXTRAP, GF_NB_STORED, GF_NB_INFLOW = -1, -2, -3, -4
 
 _dp = ct.POINTER(ct.c_double)
 _ip = ct.POINTER(ct.c_int)
@@ -53,11 +53,12 @@ PROTOTYPES = {
                                   ct.c_double, ct.c_double, _dp]),
     "gf_pool_create": (ct.c_void_p, [ct.c_int, ct.c_int, ct.c_int, ct.c_int]),
     "gf_pool_destroy": (ct.c_int, [ct.c_void_p]),
-    "gf_pool_set_patches": (ct.c_int, [ct.c_void_p, ct.c_int, _dp, _dp, _ip, _ip]),
+    "gf_pool_set_patches": (ct.c_int, [ct.c_void_p, ct.c_int, _dp, _dp, _dp, _dp, _ip, _ip]),
     "gf_pool_upload": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, _dp, _dp, ct.c_int]),
     "gf_pool_download": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, _dp]),
-    "gf_pool_describe": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _dp, _dp, _ip]),
-    "gf_pool_ghost_fill": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int]),
+    "gf_pool_describe": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _dp, _dp, _dp, _dp, _ip]),
+    "gf_pool_download_aux": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _dp]),
+    "gf_pool_ghost_fill": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_double]),
     "gf_pool_step": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
                                 ct.c_double, _dp, _dp]),
     "gf_pool_step_plain": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
@@ -67,6 +68,8 @@ PROTOTYPES = {
                                         ct.c_double]),
     "gf_pool_clear_scenarios": (ct.c_int, [ct.c_void_p]),
     "gf_pool_remove_scenario": (ct.c_int, [ct.c_void_p, ct.c_int]),
+    "gf_pool_set_scenario_inflow": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _dp, _dp, ct.c_double,
+                                               ct.c_double, ct.c_double]),
     "gf_pool_set_scenario_friction": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_double]),
     "gf_pool_set_scenario_active": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int]),
     "gf_pool_get_scenario": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(GfScenarioInfo)]),

@@ -91,6 +91,10 @@ struct gf_pool {
     int* counters = nullptr;               // gf::NCOUNTERS
     long long* stats = nullptr;            // [3]
     gf::BatchArgs* d_args = nullptr;       // device copy read by the persistent kernel
+    int* inflow_list = nullptr;            // [capacity] patches with a hydrograph side
+    int* inflow_n = nullptr;               // [1] entries of inflow_list
+    std::vector<double*> hyd_mem;          // per scenario slot: device hydrograph table
+    std::string inflow_error;              // set when a scenario's inflow cannot run
     bool args_dirty = true;
     unsigned args_gen = 0;                 // params generation of d_args and of the graph
     std::vector<PatchDesc> h_desc;
@@ -265,6 +269,45 @@ long long gf_pool_bytes(int mx, int my, int mbc, int capacity)
     return per * capacity + staging;
 }
 
+static void free_hydrographs(gf_pool* p)
+{
+    for (double*& m : p->hyd_mem) {
+        if (m) hipFree(m);
+        m = nullptr;
+    }
+}
+
+// Rebuild the device list of patches with a hydrograph side, and check that
+// every scenario with such patches has a table and uses one side only.
+static int rebuild_inflow(gf_pool* p)
+{
+    std::vector<int> list;
+    p->inflow_error.clear();
+    for (int s = 0; s < p->capacity; s++) {
+        const ScenCtrl& c = p->h_ctrl[s];
+        if (c.npatch == 0) continue;
+        int sides = 0;
+        for (int k = c.first_patch; k < c.first_patch + c.npatch; k++) {
+            bool in = false;
+            for (int e = 0; e < 4; e++)
+                if (p->h_desc[k].nbr[e] == GF_NB_INFLOW) {
+                    sides |= 1 << e;
+                    in = true;
+                }
+            if (in) list.push_back(k);
+        }
+        if (sides && (sides & (sides - 1)) && p->inflow_error.empty())
+            p->inflow_error = "scenario " + std::to_string(s) + " has hydrograph inflow on more than one side";
+        if (sides && c.hyd_rows == 0 && p->inflow_error.empty())
+            p->inflow_error = "scenario " + std::to_string(s) + " has inflow sides but no hydrograph";
+    }
+    const int n = (int)list.size();
+    if (n) HIPCHK(hipMemcpyAsync(p->inflow_list, list.data(), sizeof(int) * n, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(p->inflow_n, &n, sizeof(int), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+}
+
 gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
 {
     if (need_init()) return nullptr;
@@ -303,7 +346,9 @@ gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
               hipMalloc(&p->list_dry, (size_t)capacity * sizeof(int)) == hipSuccess &&
               hipMalloc(&p->counters, gf::NCOUNTERS * sizeof(int)) == hipSuccess &&
               hipMalloc(&p->stats, 3 * sizeof(long long)) == hipSuccess &&
-              hipMalloc(&p->d_args, sizeof(gf::BatchArgs)) == hipSuccess;
+              hipMalloc(&p->d_args, sizeof(gf::BatchArgs)) == hipSuccess &&
+              hipMalloc(&p->inflow_list, (size_t)capacity * sizeof(int)) == hipSuccess &&
+              hipMalloc(&p->inflow_n, sizeof(int)) == hipSuccess;
     if (ok)
         ok = hipMemsetAsync(p->q[0], 0, qbytes, g.stream) == hipSuccess &&
              hipMemsetAsync(p->q[1], 0, qbytes, g.stream) == hipSuccess &&
@@ -316,6 +361,8 @@ gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
              hipMemsetAsync(p->cflbits, 0, (size_t)capacity * sizeof(unsigned long long), g.stream) == hipSuccess &&
              hipMemsetAsync(p->counters, 0, gf::NCOUNTERS * sizeof(int), g.stream) == hipSuccess &&
              hipMemsetAsync(p->stats, 0, 3 * sizeof(long long), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->inflow_list, 0, (size_t)capacity * sizeof(int), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->inflow_n, 0, sizeof(int), g.stream) == hipSuccess &&
              hipStreamSynchronize(g.stream) == hipSuccess;
     if (!ok) {
         fail(GF_ENOMEM, "pool allocation of %d patches (%lld bytes) failed", capacity,
@@ -337,6 +384,7 @@ gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
         return nullptr;
     }
     p->h_ctrl.assign(capacity, ScenCtrl{});
+    p->hyd_mem.assign(capacity, nullptr);
     return p;
 }
 
@@ -361,14 +409,17 @@ int gf_pool_destroy(gf_pool* p)
     hipFree(p->counters);
     hipFree(p->stats);
     hipFree(p->d_args);
+    hipFree(p->inflow_list);
+    hipFree(p->inflow_n);
+    for (double* m : p->hyd_mem) hipFree(m);
     if (p->h_stage) hipHostFree(p->h_stage);
     if (p->h_cfl) hipHostFree(p->h_cfl);
     delete p;
     return GF_OK;
 }
 
-int gf_pool_set_patches(gf_pool* p, int n, const double* dx, const double* dy, const int* scen,
-                        const int* nbr)
+int gf_pool_set_patches(gf_pool* p, int n, const double* dx, const double* dy, const double* xlo,
+                        const double* ylo, const int* scen, const int* nbr)
 {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (n < 0 || n > p->capacity) return fail(GF_EINVAL, "n = %d outside [0, %d]", n, p->capacity);
@@ -378,12 +429,14 @@ int gf_pool_set_patches(gf_pool* p, int n, const double* dx, const double* dy, c
         if (!(dx[k] > 0.0) || !(dy[k] > 0.0)) return fail(GF_EINVAL, "patch %d: dx, dy must be > 0", k);
         d.dx = dx[k];
         d.dy = dy[k];
+        d.xlo = xlo ? xlo[k] : 0.0;
+        d.ylo = ylo ? ylo[k] : 0.0;
         if (scen && scen[k] != -1)
             return fail(GF_EINVAL, "patch %d: scenario ids are assigned by gf_pool_add_scenario (pass -1)", k);
         d.scen = -1;
         for (int s = 0; s < 8; s++) {
             const int v = nbr ? nbr[8 * k + s] : GF_NB_STORED;
-            if (v >= n || v < GF_NB_STORED)
+            if (v >= n || v < GF_NB_INFLOW || (s >= 4 && v == GF_NB_INFLOW))
                 return fail(GF_EINVAL, "patch %d: neighbour entry %d = %d invalid", k, s, v);
             d.nbr[s] = v;
         }
@@ -393,15 +446,17 @@ int gf_pool_set_patches(gf_pool* p, int n, const double* dx, const double* dy, c
     p->n = n;
     p->nscen = 0;
     p->h_ctrl.assign(p->capacity, ScenCtrl{});
+    free_hydrographs(p);
     HIPCHK(hipMemcpyAsync(p->desc, p->h_desc.data(), sizeof(PatchDesc) * p->capacity, hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipMemsetAsync(p->ctrl, 0, sizeof(ScenCtrl) * p->capacity, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     p->graph_dirty = true;
     p->args_dirty = true;
-    return GF_OK;
+    return rebuild_inflow(p);
 }
 
-int gf_pool_describe(gf_pool* p, int first, int n, const double* dx, const double* dy, const int* nbr)
+int gf_pool_describe(gf_pool* p, int first, int n, const double* dx, const double* dy, const double* xlo,
+                     const double* ylo, const int* nbr)
 {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (first < 0 || n <= 0 || first + n > p->capacity) return fail(GF_EINVAL, "slots [%d,%d) outside the pool", first, first + n);
@@ -411,14 +466,17 @@ int gf_pool_describe(gf_pool* p, int first, int n, const double* dx, const doubl
         if (!(dx[k] > 0.0) || !(dy[k] > 0.0)) return fail(GF_EINVAL, "patch %d: dx, dy must be > 0", first + k);
         for (int s = 0; s < 8; s++) {
             const int v = nbr[8 * k + s];
-            if (v >= p->capacity || v < GF_NB_STORED)
+            if (v >= p->capacity || v < GF_NB_INFLOW || (s >= 4 && v == GF_NB_INFLOW))
                 return fail(GF_EINVAL, "patch %d: neighbour entry %d = %d invalid", first + k, s, v);
         }
+        if (d.scen >= 0) return fail(GF_EINVAL, "patch %d belongs to scenario %d", first + k, d.scen);
     }
     for (int k = 0; k < n; k++) {
         PatchDesc& d = p->h_desc[first + k];
         d.dx = dx[k];
         d.dy = dy[k];
+        d.xlo = xlo ? xlo[k] : 0.0;
+        d.ylo = ylo ? ylo[k] : 0.0;
         d.scen = -1;
         for (int s = 0; s < 8; s++) d.nbr[s] = nbr[8 * k + s];
     }
@@ -473,13 +531,47 @@ int gf_pool_download(gf_pool* p, int first, int n, int buf, double* q_aos)
     return GF_OK;
 }
 
-int gf_pool_ghost_fill(gf_pool* p, int first, int n, int buf)
+int gf_pool_download_aux(gf_pool* p, int first, int n, double* aux_aos)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (first < 0 || n < 0 || first + n > p->capacity) return fail(GF_EINVAL, "slots [%d,%d) outside the pool", first, first + n);
+    if (n > 0 && !aux_aos) return fail(GF_EINVAL, "aux is NULL");
+    if (n == 0) return GF_OK;
+    HIPCHK(hipMemcpyAsync(aux_aos, p->aux + (size_t)first * p->np2, sizeof(double) * (size_t)n * p->np2,
+                          hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+}
+
+static gf::InflowArgs inflow_args(gf_pool* p, int first, int last, int buf, double t)
+{
+    gf::InflowArgs a{};
+    a.desc = p->desc;
+    a.ctrl = p->ctrl;
+    a.q0 = p->q[0];
+    a.q1 = p->q[1];
+    a.aux = p->aux;
+    a.list = p->inflow_list;
+    a.list_n = p->inflow_n;
+    a.first = first;
+    a.last = last;
+    a.buf = buf;
+    a.t = t;
+    a.P = g.P;
+    return a;
+}
+
+constexpr int kInflowGrid = 128;  // workgroups of k_inflow (grid-stride over the inflow list)
+
+int gf_pool_ghost_fill(gf_pool* p, int first, int n, int buf, double t)
 {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (first < 0 || n < 0 || first + n > p->n) return fail(GF_EINVAL, "patches [%d,%d) outside [0,%d)", first, first + n, p->n);
     if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
+    if (!p->inflow_error.empty()) return fail(GF_EINVAL, "%s", p->inflow_error.c_str());
     gf::GhostArgs a{p->desc, first, n, p->q[0], p->q[1], nullptr, buf};
     HIPCHK(gf::launch_ghost(p->mx, p->my, a, g.stream));
+    HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, first, first + n, buf, t), kInflowGrid, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
 }
@@ -580,6 +672,7 @@ int gf_pool_add_scenario(gf_pool* p, int first_patch, int npatch, double t0, dou
     HIPCHK(hipMemcpyAsync(p->ctrl + id, &p->h_ctrl[id], sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     if (id + 1 > p->nscen) p->nscen = id + 1;
+    if (int r = rebuild_inflow(p)) return r;
     return id;
 }
 
@@ -593,7 +686,38 @@ int gf_pool_remove_scenario(gf_pool* p, int s)
     HIPCHK(hipMemcpyAsync(p->desc + first, p->h_desc.data() + first, sizeof(PatchDesc) * n, hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipMemcpyAsync(p->ctrl + s, &p->h_ctrl[s], sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
-    return GF_OK;
+    if (p->hyd_mem[s]) HIPCHK(hipFree(p->hyd_mem[s]));
+    p->hyd_mem[s] = nullptr;
+    return rebuild_inflow(p);
+}
+
+int gf_pool_set_scenario_inflow(gf_pool* p, int s, int nrows, const double* time, const double* hu,
+                                double b, double x0, double y0)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
+    if (nrows < 1 || !time || !hu) return fail(GF_EINVAL, "hydrograph needs >= 1 rows of time and discharge");
+    if (!(b >= 0.0)) return fail(GF_EINVAL, "channel width must be >= 0");
+    double* d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(double) * 2 * (size_t)nrows));
+    if (hipMemcpy(d, time, sizeof(double) * nrows, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d + nrows, hu, sizeof(double) * nrows, hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(d);
+        return fail(GF_EHIP, "hydrograph upload failed");
+    }
+    HIPCHK(hipMemcpyAsync(&p->h_ctrl[s], p->ctrl + s, sizeof(ScenCtrl), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    ScenCtrl& c = p->h_ctrl[s];
+    c.hyd = d;
+    c.hyd_rows = nrows;
+    c.hyd_b = b;
+    c.hyd_x0 = x0;
+    c.hyd_y0 = y0;
+    HIPCHK(hipMemcpyAsync(p->ctrl + s, &c, sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (p->hyd_mem[s]) HIPCHK(hipFree(p->hyd_mem[s]));
+    p->hyd_mem[s] = d;
+    return rebuild_inflow(p);
 }
 
 int gf_pool_set_scenario_friction(gf_pool* p, int s, double manning_n)
@@ -620,6 +744,9 @@ int gf_pool_clear_scenarios(gf_pool* p)
     p->h_ctrl.assign(p->capacity, ScenCtrl{});
     HIPCHK(hipMemcpyAsync(p->desc, p->h_desc.data(), sizeof(PatchDesc) * p->capacity, hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipMemsetAsync(p->ctrl, 0, sizeof(ScenCtrl) * p->capacity, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    free_hydrographs(p);
+    if (int r = rebuild_inflow(p)) return r;
     HIPCHK(hipMemsetAsync(p->cflbits, 0, sizeof(unsigned long long) * p->capacity, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     p->graph_dirty = true;
@@ -725,6 +852,7 @@ static gf::BatchArgs host_batch_args(gf_pool* p)
 static int enqueue_attempt(gf_pool* p, int k)
 {
     const gf::BatchArgs a = host_batch_args(p);
+    HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, 0, p->capacity, -1, 0.0), kInflowGrid, g.stream));
     HIPCHK(gf::launch_classify(a, g.stream));
     if (k >= 0) HIPCHK(hipEventRecord(g.ev_step[2 * k], g.stream));
     HIPCHK(hipEventRecord(g.ev_fork, g.stream));
@@ -759,6 +887,7 @@ int gf_pool_run(gf_pool* p, int substeps, int use_graph, int* any_active)
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (substeps < 0) return fail(GF_EINVAL, "substeps must be >= 0");
     if (p->nscen == 0 || p->n == 0) return fail(GF_ESTATE, "no scenario registered");
+    if (!p->inflow_error.empty()) return fail(GF_EINVAL, "%s", p->inflow_error.c_str());
     if (int r = sync_batch_args(p)) return r;
     g.last_wet_ms = g.last_dry_ms = 0.0;
     if (use_graph) {
@@ -817,6 +946,7 @@ int gf_pool_run_plain(gf_pool* p, int substeps, int* any_active)
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (substeps < 0) return fail(GF_EINVAL, "substeps must be >= 0");
     if (p->nscen == 0 || p->n == 0) return fail(GF_ESTATE, "no scenario registered");
+    if (!p->inflow_error.empty()) return fail(GF_EINVAL, "%s", p->inflow_error.c_str());
     std::vector<ScenCtrl>& hc = p->h_ctrl;
     HIPCHK(hipMemcpyAsync(hc.data(), p->ctrl, sizeof(ScenCtrl) * p->nscen, hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
@@ -837,6 +967,9 @@ int gf_pool_run_plain(gf_pool* p, int substeps, int* any_active)
             if (!hc[s].active) continue;
             gf::GhostArgs ga{p->desc, hc[s].first_patch, hc[s].npatch, p->q[0], p->q[1], nullptr, hc[s].cur};
             HIPCHK(gf::launch_ghost(p->mx, p->my, ga, g.stream));
+            HIPCHK(gf::launch_inflow(p->mx, p->my,
+                                     inflow_args(p, hc[s].first_patch, hc[s].first_patch + hc[s].npatch, hc[s].cur, hc[s].t),
+                                     kInflowGrid, g.stream));
             for (int k = hc[s].first_patch; k < hc[s].first_patch + hc[s].npatch; k++, e++) {
                 HIPCHK(hipEventRecord(g.ev_step[2 * e], g.stream));
                 HIPCHK(gf::launch_step(p->mx, p->my,

@@ -21,10 +21,12 @@ constexpr int MBC = 2;
 constexpr int NB_WALL = -1;    // solid wall (bc2 mthbc = 3)
 constexpr int NB_EXTRAP = -2;  // zero-order extrapolation (bc2 mthbc = 1)
 constexpr int NB_STORED = -3;  // ghost cells are supplied by the host
+constexpr int NB_INFLOW = -4;  // hydrograph inflow (bc2 mthbc = 0); ghosts written by k_inflow
 
 // nbr[] order: left, right, bottom, top, left-bottom, right-bottom, left-top, right-top
 struct PatchDesc {
     double dx, dy;
+    double xlo, ylo;  // lower-left corner of the interior (bc2's xlower, ylower)
     int scen;
     int nbr[8];
     int pad;
@@ -41,6 +43,10 @@ struct ScenCtrl {
     double t_end;        // stop once t >= t_end (<= 0: no limit)
     double last_dt;      // dt of the last accepted step
     double manning_n;    // Manning coefficient of this scenario's src2
+    const double* hyd;   // hydrograph table: hyd[0..rows) times, hyd[rows..2 rows) discharge per unit width
+    double hyd_b, hyd_x0, hyd_y0;  // channel width and centre of the inflow strip
+    int hyd_rows;        // 0: no hydrograph
+    int hyd_pad;
     int cur;             // index of the q buffer holding the accepted state
     int nsteps;          // accepted steps
     int target_steps;    // stop after this many accepted steps (<= 0: no limit)

@@ -33,6 +33,25 @@ struct GhostArgs {
     int cur;         // otherwise this buffer
 };
 
+// Hydrograph inflow ghost cells of the patches in list[0 .. *list_n) that lie
+// in [first, last): each patch's ghost ring is rebuilt in its q buffer (and
+// aux) by ForestClaw's sequence -- same-level copies, then bc2 sides left,
+// right, bottom, top.  buf < 0: each patch uses its scenario's current buffer
+// and time and skips inactive scenarios; otherwise buffer buf at time t.
+struct InflowArgs {
+    const PatchDesc* desc;
+    const ScenCtrl* ctrl;
+    double* q0;
+    double* q1;
+    double* aux;
+    const int* list;
+    const int* list_n;
+    int first, last;
+    int buf;
+    double t;
+    Params P;
+};
+
 // Device counters of the batched path (BatchArgs::counters).
 enum { CNT_WET = 0, CNT_DRY = 1, NEXT_WET = 2, NCOUNTERS = 4 };
 
@@ -65,6 +84,7 @@ hipError_t launch_hmax(int mx, int my, const double* q, double* hmax, int first,
 hipError_t launch_step(int mx, int my, const StepArgs& a, hipStream_t st);
 hipError_t launch_ghost(int mx, int my, const GhostArgs& a, hipStream_t st);
 hipError_t launch_control(ScenCtrl* ctrl, int nscen, const double* cfl, hipStream_t st);
+hipError_t launch_inflow(int mx, int my, const InflowArgs& a, int grid, hipStream_t st);
 hipError_t launch_pack(const double* stage_q, const double* stage_aux, double* q, double* aux,
                        int first, int n, int np2, int maux, hipStream_t st);
 hipError_t launch_unpack(double* stage_q, const double* q, int first, int n, int np2, hipStream_t st);

@@ -23,6 +23,7 @@
 // bit of pow() in src2.
 #include <hip/hip_runtime.h>
 
+#include "gf_inflow.h"
 #include "gf_launch.h"
 
 namespace gf {
@@ -169,8 +170,10 @@ __device__ __forceinline__ void cell_transverse(const Rpt2Cell& C, const double 
 // interior): the pool index and interior cell whose value ForestClaw's
 // same-level face copy, corner copy and bc2 passes (left, right, bottom, top
 // over full extents; geoflood_bc2_fort.f90:118-131 and twins) leave there,
-// and whether hu / hv change sign.  Returns false when the ghost is supplied
-// by the host (GF_NB_STORED or an unresolvable corner).
+// and whether hu / hv change sign.  Returns false when the ghost is read from
+// the patch's own ghost cells: supplied by the host (NB_STORED), written by
+// k_inflow (an NB_INFLOW side or a corner next to one), or an unresolvable
+// corner.
 template <int MX, int MY>
 __device__ __forceinline__ bool ghost_source(const int* __restrict__ nbr, int patch, int fi, int fj,
                                              int& sp, int& si, int& sj, bool& neg_u, bool& neg_v)
@@ -187,7 +190,9 @@ __device__ __forceinline__ bool ghost_source(const int* __restrict__ nbr, int pa
     neg_v = false;
     if (ox != 0 && oy != 0) {
         const int nc = oy < 0 ? (ox < 0 ? nbr[4] : nbr[5]) : (ox < 0 ? nbr[6] : nbr[7]);
-        if (nc >= 0) {
+        if (nx == NB_INFLOW || ny == NB_INFLOW) {
+            return false;  // bc2's inflow side pass overwrites the corner; k_inflow stores it
+        } else if (nc >= 0) {
             sp = nc; si = fi - ox * MX; sj = fj - oy * MY;
         } else if (px && py) {
             sp = patch; si = txv; sj = tyv;
@@ -699,6 +704,146 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Hydrograph inflow patches.  The ghost ring of each listed patch is rebuilt
+// in LDS in ForestClaw's order -- same-level face/corner copies (and the
+// closed-form wall/extrapolation states) first, then the bc2 passes left,
+// right, bottom, top over full extents (geoflood_bc2_fort.f90:7-499) -- and
+// written back to the patch's own ghost cells and aux, where the step reads
+// the inflow sides as stored ghosts.  The passes run in sequence because a
+// later side's corner cells read what an earlier side wrote.
+// ---------------------------------------------------------------------------
+
+// One bc2 side on the LDS tile L (h, hu, hv, b planes): wall (mthbc 3),
+// extrapolation (mthbc 1) or hydrograph inflow (mthbc 0).  Reads interior
+// rows/columns and, for the y sides, the ghost columns the x sides wrote;
+// writes only this side's ghost cells.
+template <int MX, int MY>
+__device__ void bc2_side(double* L, int side, int code, const PatchDesc* __restrict__ dp,
+                         const ScenCtrl* __restrict__ c, double t, const Params& P)
+{
+    using G = Geo<MX, MY>;
+    constexpr int NP2 = G::NP2;
+    const bool xs = side < 2;
+    const int n = xs ? MY : MX;
+    const int items = (n + 2 * MBC) * MBC;
+    for (int it = threadIdx.x; it < items; it += blockDim.x) {
+        const int r = it / MBC + 1 - MBC;  // row (x sides) or column (y sides)
+        const int kb = it % MBC + 1;       // ghost layer
+        int gi, gj, mi, mj, bi, bj;        // ghost, mirror source, boundary cell
+        if (side == 0) { gi = 1 - kb; gj = r; mi = kb; mj = r; bi = 1; bj = r; }
+        else if (side == 1) { gi = MX + kb; gj = r; mi = MX + 1 - kb; mj = r; bi = MX; bj = r; }
+        else if (side == 2) { gi = r; gj = 1 - kb; mi = r; mj = kb; bi = r; bj = 1; }
+        else { gi = r; gj = MY + kb; mi = r; mj = MY + 1 - kb; bi = r; bj = MY; }
+        const int gk = G::ti(gi, gj), mk = G::ti(mi, mj), bk = G::ti(bi, bj);
+        const int nrm = xs ? 1 : 2;  // plane of the normal momentum
+        if (code == NB_EXTRAP) {
+            L[gk] = L[bk];
+            L[NP2 + gk] = L[NP2 + bk];
+            L[2 * NP2 + gk] = L[2 * NP2 + bk];
+            L[3 * NP2 + gk] = L[3 * NP2 + bk];
+            continue;
+        }
+        bool mirror = true;
+        if (code == NB_INFLOW) {
+            const double lo = xs ? dp->ylo : dp->xlo;
+            const double del = xs ? dp->dy : dp->dx;
+            const double c0 = xs ? c->hyd_y0 : c->hyd_x0;
+            const double coord = lo + ((double)r - 0.5) * del;
+            if (fabs(coord - c0) <= c->hyd_b / 2.0) {
+                const double h1 = L[bk], hu1 = L[NP2 + bk];
+                const double hu0 = hydro_interpolate(c->hyd, c->hyd + c->hyd_rows, c->hyd_rows, t);
+                mirror = false;
+                double h0;
+                if (h1 < P.dry_tolerance) {
+                    h0 = fmaxf64(pow(hu0 / sqrt(P.grav), 2.0 / 3.0), 0.001);
+                } else if (hu0 != 0.0) {
+                    h0 = riemann_invariant_depth(hu0, h1, hu1, P.grav);
+                    if (h0 > h1) h0 = two_shock_depth(hu0, h1, hu1, P.grav);
+                } else {
+                    mirror = true;
+                    h0 = 0.0;
+                }
+                if (!mirror) {
+                    L[gk] = h0;
+                    L[NP2 + gk] = hu0;  // the discharge goes to q(2) on every side
+                    L[2 * NP2 + gk] = 0.0;
+                }
+            }
+        }
+        if (mirror) {
+            L[gk] = L[mk];
+            L[NP2 + gk] = L[NP2 + mk];
+            L[2 * NP2 + gk] = L[2 * NP2 + mk];
+            L[nrm * NP2 + gk] = -L[nrm * NP2 + mk];
+            L[3 * NP2 + gk] = L[3 * NP2 + mk];
+        }
+    }
+}
+
+template <int MX, int MY>
+__global__ void __launch_bounds__(256) k_inflow(InflowArgs a)
+{
+    using G = Geo<MX, MY>;
+    constexpr int NXP = G::NXP, NP2 = G::NP2;
+    __shared__ double L[4 * NP2];
+    const int n = *a.list_n;
+    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+        const int patch = a.list[k];
+        if (patch < a.first || patch >= a.last) continue;
+        const PatchDesc* dp = a.desc + patch;
+        const int scen = dp->scen;
+        if (scen < 0) continue;
+        const ScenCtrl* c = a.ctrl + scen;
+        int cur;
+        double t;
+        if (a.buf < 0) {
+            if (!c->active) continue;
+            cur = c->cur;
+            t = c->t;
+        } else {
+            cur = a.buf;
+            t = a.t;
+        }
+        double* base = cur ? a.q1 : a.q0;
+        double* q = base + (size_t)patch * 3 * NP2;
+        double* ax = a.aux + (size_t)patch * NP2;
+        __syncthreads();  // the previous patch's write-back has read L
+        for (int e = threadIdx.x; e < NP2; e += blockDim.x) {
+            const int fi = e % NXP - 1, fj = e / NXP - 1;
+            const double* src = q;
+            int kk = e;
+            bool nu = false, nv = false;
+            if (!(fi >= 1 && fi <= MX && fj >= 1 && fj <= MY)) {
+                int sp, si, sj;
+                if (ghost_source<MX, MY>(dp->nbr, patch, fi, fj, sp, si, sj, nu, nv)) {
+                    src = base + (size_t)sp * 3 * NP2;
+                    kk = G::ti(si, sj);
+                }
+            }
+            const double h = src[kk], hu = src[NP2 + kk], hv = src[2 * NP2 + kk];
+            L[e] = h;
+            L[NP2 + e] = nu ? -hu : hu;
+            L[2 * NP2 + e] = nv ? -hv : hv;
+            L[3 * NP2 + e] = ax[e];
+        }
+        __syncthreads();
+        for (int side = 0; side < 4; side++) {
+            const int code = dp->nbr[side];
+            if (code == NB_WALL || code == NB_EXTRAP || code == NB_INFLOW) bc2_side<MX, MY>(L, side, code, dp, c, t, a.P);
+            __syncthreads();
+        }
+        for (int e = threadIdx.x; e < NP2; e += blockDim.x) {
+            const int fi = e % NXP - 1, fj = e / NXP - 1;
+            if (fi >= 1 && fi <= MX && fj >= 1 && fj <= MY) continue;
+            q[e] = L[e];
+            q[NP2 + e] = L[NP2 + e];
+            q[2 * NP2 + e] = L[2 * NP2 + e];
+            ax[e] = L[3 * NP2 + e];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Device-side dt controller (fc2d_geoclaw_run.c:504-565): one workgroup per
 // scenario reduces the patches' capped CFL numbers, then either accepts the
 // step (flip the q buffer, t += dt) or rejects it (retake), and sets the next
@@ -762,7 +907,7 @@ __global__ void __launch_bounds__(256) k_classify(BatchArgs a)
                 for (int s = 0; s < 8; s++) {
                     const int nb = d.nbr[s];
                     if (nb >= 0) w = w || (hm[nb] > tol);
-                    else if (nb == NB_STORED) w = true;
+                    else if (nb == NB_STORED || nb == NB_INFLOW) w = true;
                 }
                 kind = w ? 1 : 0;
             }
@@ -961,6 +1106,18 @@ hipError_t launch_ghost(int mx, int my, const GhostArgs& a, hipStream_t st)
     case 32032: k_ghost<32, 32><<<a.count, 256, 0, st>>>(a); break;
     case 16016: k_ghost<16, 16><<<a.count, 256, 0, st>>>(a); break;
     case 8008: k_ghost<8, 8><<<a.count, 256, 0, st>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_inflow(int mx, int my, const InflowArgs& a, int grid, hipStream_t st)
+{
+    if (grid <= 0) return hipSuccess;
+    switch (mx * 1000 + my) {
+    case 32032: k_inflow<32, 32><<<grid, 256, 0, st>>>(a); break;
+    case 16016: k_inflow<16, 16><<<grid, 256, 0, st>>>(a); break;
+    case 8008: k_inflow<8, 8><<<grid, 256, 0, st>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -13,10 +13,10 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (GF_NB_EXTRAP, GF_NB_WALL, GfScenarioInfo, check, dptr, iptr, load)
+from . import (GF_NB_EXTRAP, GF_NB_INFLOW, GF_NB_WALL, GfScenarioInfo, check, dptr, iptr, load)
 
 MBC = 2
-_KIND = {3: GF_NB_WALL, 1: GF_NB_EXTRAP}
+_KIND = {3: GF_NB_WALL, 1: GF_NB_EXTRAP, 0: GF_NB_INFLOW}
 
 
 @dataclass
@@ -27,12 +27,20 @@ class BrickLayout:
     my: int
     dx: float
     dy: float
-    mthbc: tuple = (3, 3, 3, 3)  # left, right, bottom, top (bc2 codes: 3 wall, 1 extrapolation)
+    mthbc: tuple = (3, 3, 3, 3)  # left, right, bottom, top (bc2: 3 wall, 1 extrapolation, 0 inflow)
     first: int = 0
+    xlo: float = 0.0  # lower-left corner of the brick
+    ylo: float = 0.0
 
     @property
     def npatch(self) -> int:
         return self.mi * self.mj
+
+    def origins(self):
+        """Per-patch lower-left corners (xlower, ylower)."""
+        pi, pj = np.meshgrid(np.arange(self.mi), np.arange(self.mj))
+        return (np.ascontiguousarray(self.xlo + pi.ravel() * self.mx * self.dx, dtype=np.float64),
+                np.ascontiguousarray(self.ylo + pj.ravel() * self.my * self.dy, dtype=np.float64))
 
     def neighbours(self) -> np.ndarray:
         """(npatch, 8) neighbour table: left, right, bottom, top, left-bottom,
@@ -59,13 +67,13 @@ class BrickLayout:
 
 def pad_bathymetry(b: np.ndarray, mthbc=(3, 3, 3, 3), mbc: int = MBC) -> np.ndarray:
     """Globally padded bathymetry with the ghost values bc2 leaves at the
-    domain boundary (walls mirror, extrapolation copies the edge cell;
-    sides applied left, right, bottom, top)."""
+    domain boundary (walls and hydrograph sides mirror, extrapolation copies
+    the edge cell; sides applied left, right, bottom, top)."""
     ny, nx = b.shape
     out = np.zeros((ny + 2 * mbc, nx + 2 * mbc))
     out[mbc:mbc + ny, mbc:mbc + nx] = b
     for side in range(4):
-        wall = mthbc[side] == 3
+        wall = mthbc[side] in (0, 3)
         for k in range(1, mbc + 1):
             if side == 0:
                 out[:, mbc - k] = out[:, mbc + k - 1] if wall else out[:, mbc]
@@ -138,12 +146,15 @@ class Pool:
     def set_layout(self, layouts: list[BrickLayout]):
         """Place bricks one after another in the pool and describe them."""
         first = 0
-        dx, dy, nb = [], [], []
+        dx, dy, xl, yl, nb = [], [], [], [], []
         for L in layouts:
             assert (L.mx, L.my) == (self.mx, self.my)
             L.first = first
             dx.append(np.full(L.npatch, L.dx))
             dy.append(np.full(L.npatch, L.dy))
+            ox, oy = L.origins()
+            xl.append(ox)
+            yl.append(oy)
             nb.append(L.neighbours())
             first += L.npatch
         if first > self.capacity:
@@ -152,10 +163,25 @@ class Pool:
         self.n = first
         dxa = np.ascontiguousarray(np.concatenate(dx)) if dx else np.zeros(0)
         dya = np.ascontiguousarray(np.concatenate(dy)) if dy else np.zeros(0)
+        xla = np.ascontiguousarray(np.concatenate(xl)) if xl else np.zeros(0)
+        yla = np.ascontiguousarray(np.concatenate(yl)) if yl else np.zeros(0)
         nba = np.ascontiguousarray(np.concatenate(nb)) if nb else np.zeros((0, 8), np.int32)
         sc = np.full(first, -1, dtype=np.int32)
-        check(self.lib.gf_pool_set_patches(self.h, first, dptr(dxa), dptr(dya), iptr(sc), iptr(nba)),
-              "gf_pool_set_patches")
+        check(self.lib.gf_pool_set_patches(self.h, first, dptr(dxa), dptr(dya), dptr(xla), dptr(yla),
+                                           iptr(sc), iptr(nba)), "gf_pool_set_patches")
+
+    def describe(self, layout: BrickLayout, first: int):
+        """Describe one brick at pool slots [first, first+npatch) without
+        touching the other slots (gf_pool_describe)."""
+        assert (layout.mx, layout.my) == (self.mx, self.my)
+        layout.first = first
+        n = layout.npatch
+        ox, oy = layout.origins()
+        nb = np.ascontiguousarray(layout.neighbours())
+        check(self.lib.gf_pool_describe(self.h, first, n, dptr(np.full(n, float(layout.dx))),
+                                        dptr(np.full(n, float(layout.dy))), dptr(ox), dptr(oy),
+                                        iptr(nb)), "gf_pool_describe")
+        self.n = max(self.n, first + n)
 
     # -- data movement --------------------------------------------------------
     def upload_aos(self, first: int, q: np.ndarray, aux: np.ndarray | None, buf: int = 0):
@@ -170,6 +196,11 @@ class Pool:
         check(self.lib.gf_pool_download(self.h, first, n, buf, dptr(q)), "gf_pool_download")
         return q
 
+    def download_aux(self, first: int, n: int) -> np.ndarray:
+        a = np.zeros((n, self.my + 2 * MBC, self.mx + 2 * MBC, 1))
+        check(self.lib.gf_pool_download_aux(self.h, first, n, dptr(a)), "gf_pool_download_aux")
+        return a
+
     def upload_global(self, layout: BrickLayout, h, hu, hv, b, buf: int = 0):
         q, aux = global_to_aos(h, hu, hv, b, layout)
         self.upload_aos(layout.first, q, aux, buf)
@@ -178,9 +209,9 @@ class Pool:
         return aos_to_global(self.download_aos(layout.first, layout.npatch, buf), layout)
 
     # -- stepping ---------------------------------------------------------------
-    def ghost_fill(self, first: int = 0, n: int | None = None, buf: int = 0):
+    def ghost_fill(self, first: int = 0, n: int | None = None, buf: int = 0, t: float = 0.0):
         n = self.n - first if n is None else n
-        check(self.lib.gf_pool_ghost_fill(self.h, first, n, buf), "gf_pool_ghost_fill")
+        check(self.lib.gf_pool_ghost_fill(self.h, first, n, buf, t), "gf_pool_ghost_fill")
 
     def step(self, dt: float, first: int = 0, n: int | None = None, src: int = 0, dst: int = 1,
              plain: bool = False):
@@ -199,6 +230,21 @@ class Pool:
         if sid < 0:
             check(sid, "gf_pool_add_scenario")
         return sid
+
+    def remove_scenario(self, sid: int):
+        check(self.lib.gf_pool_remove_scenario(self.h, sid), "gf_pool_remove_scenario")
+
+    def set_friction(self, sid: int, manning_n: float):
+        check(self.lib.gf_pool_set_scenario_friction(self.h, sid, manning_n),
+              "gf_pool_set_scenario_friction")
+
+    def set_inflow(self, sid: int, time, hu, b: float, x0: float, y0: float):
+        """Hydrograph of the scenario's inflow sides (gf_pool_set_scenario_inflow)."""
+        t = np.ascontiguousarray(time, dtype=np.float64)
+        q = np.ascontiguousarray(hu, dtype=np.float64)
+        assert t.shape == q.shape and t.ndim == 1
+        check(self.lib.gf_pool_set_scenario_inflow(self.h, sid, len(t), dptr(t), dptr(q), b, x0, y0),
+              "gf_pool_set_scenario_inflow")
 
     def clear_scenarios(self):
         check(self.lib.gf_pool_clear_scenarios(self.h), "gf_pool_clear_scenarios")

@@ -51,3 +51,45 @@ def uniform_dam_break(n: int = 4096, dx: float = 1.0, h0: float = 10.0, rms: flo
     h = np.zeros((n, n))
     h[int(0.75 * n):, : n // 2] = h0
     return h, np.zeros_like(h), np.zeros_like(h), b, dx
+
+
+def gaussian_sum_dem(n: int, length: float = 2000.0, valley_slope: float = 0.002, ngauss: int = 64,
+                     amp=(0.0, 5.0), width=(20.0, 200.0), seed: int = 0) -> np.ndarray:
+    """Overland-flood terrain on an n x n grid over [0, length]^2: a valley
+    floor descending in +x with ``valley_slope`` plus ``ngauss`` Gaussian
+    hills (amplitude U[amp], width U[width], centres uniform in the domain,
+    drawn from ``seed`` in that order)."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(amp[0], amp[1], ngauss)
+    w = rng.uniform(width[0], width[1], ngauss)
+    cx = rng.uniform(0.0, length, ngauss)
+    cy = rng.uniform(0.0, length, ngauss)
+    dx = length / n
+    xc = (np.arange(n) + 0.5) * dx
+    b = np.broadcast_to(valley_slope * (length - xc)[None, :], (n, n)).copy()
+    for k in range(ngauss):
+        gx = np.exp(-((xc - cx[k]) ** 2) / (2.0 * w[k] ** 2))
+        gy = np.exp(-((xc - cy[k]) ** 2) / (2.0 * w[k] ** 2))
+        b += a[k] * np.outer(gy, gx)
+    return b
+
+
+def sine_hydrograph(peak_q: float, strip: float = 100.0, period: float = 3600.0, rows: int = 61):
+    """Tabulated Q(t) = peak_q sin(pi t / period) m^3/s over [0, period],
+    as discharge per unit width of a ``strip``-metre channel."""
+    t = np.linspace(0.0, period, rows)
+    return t, peak_q * np.sin(np.pi * t / period) / strip
+
+
+def overland_flood(n: int, length: float = 2000.0, peak_q: float = 200.0, strip: float = 100.0,
+                   seed: int = 0):
+    """Initially dry overland flood: Gaussian-sum DEM (``seed``), hydrograph
+    inflow through a ``strip``-metre channel centred on the west edge
+    (bc2 sides: west inflow, east extrapolation, south and north walls).
+    Returns a dict with the global fields h, hu, hv, b, the cell size dx,
+    mthbc and the hydrograph (time, hu per unit width, b, x0, y0)."""
+    b = gaussian_sum_dem(n, length, seed=seed)
+    z = np.zeros((n, n))
+    t, hu = sine_hydrograph(peak_q, strip)
+    return dict(h=z.copy(), hu=z.copy(), hv=z.copy(), b=b, dx=length / n, mthbc=(0, 1, 3, 3),
+                hydro=dict(time=t, hu=hu, b=strip, x0=0.0, y0=0.5 * length))

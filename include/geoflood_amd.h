@@ -35,8 +35,11 @@ enum {
     GF_ESTATE = -5        /* call out of order (e.g. no gf_init) */
 };
 
-/* Neighbour codes for gf_pool_set_patches (values >= 0 are patch indices). */
-enum { GF_NB_WALL = -1, GF_NB_EXTRAP = -2, GF_NB_STORED = -3 };
+/* Neighbour codes for gf_pool_set_patches (values >= 0 are patch indices):
+ * the bc2 side kinds mthbc = 3 (wall), 1 (extrapolation) and 0 (hydrograph
+ * inflow, geoflood_bc2_fort.f90:52-102; face entries only), or ghosts that
+ * the host stores. */
+enum { GF_NB_WALL = -1, GF_NB_EXTRAP = -2, GF_NB_STORED = -3, GF_NB_INFLOW = -4 };
 
 /* Physics and method parameters.  Replaces the __constant__ GeofloodVars of
  * src/fc2d_cpu_cuda/riemann_source/variables.h:20-37 and the method symbols
@@ -163,16 +166,21 @@ int gf_pool_destroy(gf_pool *pool);
  * 11-107) together with the per-side mthbc of geoclaw_bc2
  * (fc2d_cpucuda.cpp:131-186). */
 int gf_pool_set_patches(gf_pool *pool, int n, const double *dx, const double *dy,
-                        const int *scenario, const int *nbr);
+                        const double *xlower, const double *ylower, const int *scenario,
+                        const int *nbr);
 
 /* Describe pool slots [first, first+n) without touching the others: the
  * admission path of a serving engine, which places a new scenario's forest in
  * free slots while other scenarios keep stepping.  nbr holds 8 entries per
- * patch as in gf_pool_set_patches (indices are pool slots).  The slots belong
+ * patch as in gf_pool_set_patches (indices are pool slots); xlower/ylower
+ * (nullable) are the patches' lower-left corners, used by hydrograph sides
+ * (cudaclaw_patch_data_t's xlower/ylower, cudaclaw_allocate.h:18-64).  The
+ * slots must belong to no scenario and stay so until gf_pool_add_scenario.
+ * The slots belong
  * to no scenario until gf_pool_add_scenario.  Same reference interface as
  * gf_pool_set_patches, per forest instead of per pool. */
 int gf_pool_describe(gf_pool *pool, int first, int n, const double *dx, const double *dy,
-                     const int *nbr);
+                     const double *xlower, const double *ylower, const int *nbr);
 
 /* Copy n patches from host AoS arrays (contiguous, one patch after another,
  * reference layout with ghosts) into pool slots [first, first+n), q into
@@ -187,12 +195,20 @@ int gf_pool_upload(gf_pool *pool, int first, int n, int buf, const double *q_aos
  * (cudaclaw_step2.cu:298-315; data_swap.f90:41-53). */
 int gf_pool_download(gf_pool *pool, int first, int n, int buf, double *q_aos);
 
+/* Copy the aux (bathymetry, maux = 1, ghosts included) of pool slots
+ * [first, first+n) back to host AoS arrays: bc2's wall branches of a
+ * hydrograph side rewrite ghost aux on the device (geoflood_bc2_fort.f90:
+ * 82-90), so the device copy is the current one.  The reference keeps aux on
+ * the host (CUDACLAW_SWAP_DATA copies it forward only, data_swap.f90:22-40). */
+int gf_pool_download_aux(gf_pool *pool, int first, int n, double *aux_aos);
+
 /* Fill the ghost cells of buffer `buf` on the device for patches [first,
  * first+n): same-level face and corner copies, then solid-wall /
- * extrapolation sides in the order left, right, bottom, top
- * (geoflood_bc2_fort.f90:118-131 and twins).  GF_NB_STORED sides keep the
- * values already stored (e.g. coarse/fine ghosts filled by the host). */
-int gf_pool_ghost_fill(gf_pool *pool, int first, int n, int buf);
+ * extrapolation / hydrograph sides in the order left, right, bottom, top
+ * (geoflood_bc2_fort.f90:7-499), the hydrograph at time t.  GF_NB_STORED
+ * sides keep the values already stored (e.g. coarse/fine ghosts filled by
+ * the host).  Hydrograph sides are filled for patches of a scenario only. */
+int gf_pool_ghost_fill(gf_pool *pool, int first, int n, int buf, double t);
 
 /* One step attempt of patches [first, first+n) from buffer `src` to buffer
  * `dst` (== src: in place) with time step dt, all patches in one launch.
@@ -222,6 +238,19 @@ int gf_pool_clear_scenarios(gf_pool *pool);
  * reused by the next gf_pool_add_scenario.  Other scenarios are unaffected.
  * Ends one outstyle_3 run (fc2d_geoclaw_run.c:417-590) of an ensemble. */
 int gf_pool_remove_scenario(gf_pool *pool, int scenario);
+
+/* Hydrograph inflow of a scenario's GF_NB_INFLOW sides: discharge per unit
+ * width hu[k] at time[k] (nrows rows, linear in between, clamped outside),
+ * channel strip of width b centred at y0 (left/right sides) or x0
+ * (bottom/top).  Inside the strip the ghost state is the reference's
+ * inflow state (critical depth for a dry boundary cell, Riemann-invariant /
+ * two-shock depth for a wet one, wall for zero discharge); outside it is a
+ * wall.  A scenario may use one inflow side.  Replaces read_hydrograph_data
+ * and the mthbc = 0 branch of fc2d_geoclaw_bc2 (hydrograph/
+ * hydrograh_module.f90:35-331, fortran_source/geoflood_bc2_fort.f90:52-102
+ * and the right/bottom/top twins) with hydrograph_type 'discharge'. */
+int gf_pool_set_scenario_inflow(gf_pool *pool, int scenario, int nrows, const double *time,
+                                const double *hu, double b, double x0, double y0);
 
 /* Per-scenario Manning coefficient (the scenario starts with gf_params'
  * manning_n).  Replaces manning_coefficient(1) of friction_module

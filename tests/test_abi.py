@@ -52,7 +52,7 @@ def test_errors_are_status_codes():
     assert lib.gf_pool_create(32, 32, 2, 4) is None
     assert b"gf_init" in lib.gf_last_error()
     assert lib.gf_pool_destroy(None) == 0
-    assert lib.gf_pool_set_patches(None, 0, None, None, None, None) == -1
+    assert lib.gf_pool_set_patches(None, 0, None, None, None, None, None, None) == -1
 
 
 def test_pool_bytes_accounting():
