@@ -6,8 +6,11 @@ RMS 3 m, seed 0) plus a 1-in-500 slope descending in +x, dx = 1 m; dam-break
 reservoir h = 10 m over the upper-left eighth, dry elsewhere; Manning
 n = 0.03; walls; initial dt 0.01 s, desired CFL 0.9, max CFL 1.0.
 
-A "step" is one step attempt over all 16,384 patches: device ghost fill,
-fused step kernel, device dt controller.  One cell-update is one interior
+A "step" is one step attempt over all 16,384 patches: device work lists
+(patches whose 2-cell halo holds water are active, the rest dry), the
+persistent fused step over the active patches (halo pulled from the
+neighbours) concurrently with the dry-patch stream, and the device dt
+controller.  One cell-update is one interior
 cell advanced by one accepted step (retaken attempts do not count).
 
 Runs, all from the same initial state, inputs resident in HBM:
@@ -144,29 +147,47 @@ def main():
     log(f"batched: {accepted} accepted / {args.steps} attempts, {wall:.3f} s wall, "
         f"{total_ms:.1f} ms device, t = {info_b['t']:.4f} s, {value / 1e9:.2f} G cell-updates/s")
 
-    # ---- same attempts without a graph, HIP events around each step launch ----
+    # ---- same attempts without a graph, HIP events around each launch ----
+    # The dominant kernel is k_wet (the persistent step over the active
+    # patches): its algorithmic bytes per launch are 56 B x 1024 cells x the
+    # patches on that attempt's active list; k_dry streams the dry patches
+    # concurrently on a second stream.
     roofline = None
-    kernel_ms = None
+    composite = None
     if not args.no_events:
         sid = reset(pool, layout, state, 0)
+        s0 = pool.stats()
         pool.run(args.steps, graph=False)
         ev_total, step_ms, launches = geoflood_amd.last_timing()
+        wet_ms, dry_ms = geoflood_amd.last_branch_timing()
+        s1 = pool.stats()
         info_e = pool.scenario(sid)
-        kernel_ms = step_ms / max(launches, 1)
-        bytes_per_launch = BYTES_PER_CELL_UPDATE * cells
-        achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+        act = s1["active_patches"] - s0["active_patches"]
+        dry = s1["dry_patches"] - s0["dry_patches"]
+        launches = max(launches, 1)
+        wet_avg_ms = wet_ms / launches
+        wet_bytes = BYTES_PER_CELL_UPDATE * MX * MX * act / launches
+        achieved = wet_bytes / (wet_avg_ms * 1e-3) / 1e9
         traffic = None
-        tfile = os.path.join(ROOT, "profiles", "k_step_32x32_uniform4096_traffic.txt")
+        tfile = os.path.join(ROOT, "profiles", "k_wet_32x32_uniform4096_traffic.txt")
         if os.path.exists(tfile):
             kv = dict(line.split("=", 1) for line in open(tfile).read().split("\n") if "=" in line)
             traffic = float(kv.get("hbm_bytes_per_launch", "nan"))
         roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                        kernel="k_step<32,32,640>", kernel_avg_ms=round(kernel_ms, 5),
-                        bytes_per_launch=bytes_per_launch)
+                        kernel="k_wet<32,32,640>", kernel_avg_ms=round(wet_avg_ms, 5),
+                        bytes_per_launch=round(wet_bytes), active_patches_per_launch=round(act / launches, 1),
+                        launches=launches)
+        step_avg_ms = step_ms / launches
+        composite = dict(step_phase_avg_ms=round(step_avg_ms, 5),
+                         achieved_GBs=round(BYTES_PER_CELL_UPDATE * cells / (step_avg_ms * 1e-3) / 1e9, 2),
+                         dry_kernel_avg_ms=round(dry_ms / launches, 5),
+                         dry_patches_per_launch=round(dry / launches, 1),
+                         note="classify -> (k_wet || k_dry) -> join, per attempt, all 16,384 patches")
         same = (info_e["nsteps"] == info_b["nsteps"] and info_e["t"] == info_b["t"])
-        log(f"events run: step kernel avg {kernel_ms:.4f} ms over {launches} launches, "
-            f"{ev_total:.1f} ms device total; same trajectory as graph run: {same}")
+        log(f"events run: k_wet avg {wet_avg_ms:.4f} ms ({act / launches:.0f} active patches), "
+            f"k_dry avg {dry_ms / launches:.4f} ms ({dry / launches:.0f} dry patches), step phase avg "
+            f"{step_avg_ms:.4f} ms over {launches} attempts; same trajectory as graph run: {same}")
 
     # ---- plain per-patch loop ----
     plain = None
@@ -228,6 +249,7 @@ def main():
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "plain": plain,
         "roofline": roofline,
+        "step_composite": composite,
         "cpu_baseline": cpu,
         "parity_vs_cpu_max_rel": parity,
         "ensemble": None,

@@ -617,7 +617,7 @@ __global__ void __launch_bounds__(NT) k_step(StepArgs a)
 // Persistent step kernel over the device work list of active patches:
 // each workgroup takes the next list entry from an atomic counter, pulls the
 // patch's halo from its neighbours, steps it, and folds its CFL number into
-// the scenario's maximum (atomicMax on the bit pattern; CFL >= 0).
+// the scenario's maximum (atomicMax on the bit pattern of a CFL > 0).
 // One list entry of the persistent kernel.  (The library is built with
 // machine-level loop-invariant code motion off: hoisting the body's constant
 // and index materialisations out of the persistent loop would keep them live
@@ -643,7 +643,11 @@ __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx)
     step_patch<MX, MY, NT, true>(T, E, F, red, a.desc + patch, patch, src, dst, a.aux, dt, P, cfl, hm);
     if (threadIdx.x == 0) {
         a.hmax[(size_t)(cur ^ 1) * a.capacity + patch] = hm;
-        atomicMax(a.cflbits + scen, (unsigned long long)__double_as_longlong(cfl));
+        // The unsigned order of bit patterns is the order of non-negative
+        // doubles, but -0.0 (which the CFL maximum can return for a patch
+        // with no waves) sorts above them all: zeros are left out (the
+        // scenario's maximum starts at +0.0).  NaN still propagates.
+        if (cfl != 0.0) atomicMax(a.cflbits + scen, (unsigned long long)__double_as_longlong(cfl));
     }
     __syncthreads();
 }

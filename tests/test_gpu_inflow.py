@@ -2,9 +2,9 @@
 CPU oracle's bc2 (mthbc = 0, pinned bitwise to the reference build) and the
 oracle's brick driver.
 
-Tolerances as for the step: ghost states agree to rounding of pow/sqrt
-(1e-13 relative); runs are held to the per-cell 1e-10 bar, the dt sequence
-to 1e-12 and an identical wet/dry mask.
+Tolerances as for the step: ghost states agree to rounding of pow
+(1e-13 relative); runs are held to the per-cell 1e-10 bar of
+tests/_parity.py, the dt sequence to 1e-12 and an identical wet/dry mask.
 """
 import numpy as np
 import pytest
@@ -12,6 +12,7 @@ import pytest
 from oracle.oracle import Brick, Hydrograph, Params
 from geoflood_amd.pool import BrickLayout, Pool
 from geoflood_amd.scenarios import overland_flood
+from _parity import assert_state_close
 
 pytestmark = pytest.mark.gpu
 
@@ -109,8 +110,7 @@ def test_overland_flood_run(gf, oracle_c, flood_params, mode, mx, peak, y0):
     got = np.stack(pool.download_global(layout, buf=info["cur"]))
     ref = br.to_global()
     assert (ref[0] > 1e-3).sum() > 100  # the inflow has wetted part of the domain
-    assert rel_err(got, ref) <= RTOL_RUN
-    assert np.array_equal(got[0] > 1e-3, ref[0] > 1e-3)
+    assert_state_close(got, ref, RTOL_RUN)
     pool.close()
 
 
