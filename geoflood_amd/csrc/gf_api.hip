@@ -785,6 +785,29 @@ int gf_pool_get_scenario(gf_pool* p, int s, gf_scenario_info* info)
     return GF_OK;
 }
 
+int gf_pool_get_scenarios(gf_pool* p, int first, int n, gf_scenario_info* info)
+{
+    if (!p || (n > 0 && !info)) return fail(GF_EINVAL, "pool or info is NULL");
+    if (first < 0 || n < 0 || first + n > p->capacity) return fail(GF_EINVAL, "scenarios [%d,%d) outside [0,%d)", first, first + n, p->capacity);
+    if (n == 0) return GF_OK;
+    std::vector<ScenCtrl> c(n);
+    HIPCHK(hipMemcpyAsync(c.data(), p->ctrl + first, sizeof(ScenCtrl) * n, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    for (int k = 0; k < n; k++) {
+        gf_scenario_info& o = info[k];
+        o.t = c[k].t;
+        o.dt = c[k].dt;
+        o.maxcfl = c[k].maxcfl;
+        o.last_dt = c[k].last_dt;
+        o.nsteps = c[k].nsteps;
+        o.retakes = c[k].retakes;
+        o.attempts = c[k].attempts;
+        o.active = p->h_ctrl[first + k].npatch > 0 ? c[k].active : 0;
+        o.cur = c[k].cur;
+    }
+    return GF_OK;
+}
+
 static int ensure_pair_events(std::vector<hipEvent_t>& v, size_t pairs)
 {
     while (v.size() < 2 * pairs) {

@@ -257,6 +257,15 @@ class Pool:
         check(self.lib.gf_pool_get_scenario(self.h, sid, ct.byref(info)), "gf_pool_get_scenario")
         return {k: getattr(info, k) for k, _ in GfScenarioInfo._fields_}
 
+    def scenarios(self, first: int, n: int) -> np.ndarray:
+        """Control records of scenario ids [first, first+n) as a structured
+        array with the fields of gf_scenario_info (gf_pool_get_scenarios)."""
+        arr = (GfScenarioInfo * max(n, 1))()
+        check(self.lib.gf_pool_get_scenarios(self.h, first, n, arr), "gf_pool_get_scenarios")
+        dt = np.dtype([(k, np.float64 if t is ct.c_double else np.int32) for k, t in GfScenarioInfo._fields_],
+                      align=True)
+        return np.frombuffer(bytes(arr), dtype=dt, count=n).copy()
+
     def run(self, substeps: int, graph: bool = True) -> bool:
         act = ct.c_int(0)
         check(self.lib.gf_pool_run(self.h, substeps, int(graph), ct.byref(act)), "gf_pool_run")

@@ -263,6 +263,12 @@ int gf_pool_set_scenario_active(gf_pool *pool, int scenario, int active);
 /* Copy a scenario's control record to the host. */
 int gf_pool_get_scenario(gf_pool *pool, int scenario, gf_scenario_info *info);
 
+/* Control records of scenario ids [first, first+n) in one device-to-host
+ * copy (ids with no scenario report active = 0): the poll of a serving loop
+ * that runs many scenarios (the per-step host read of maxcfl in
+ * fc2d_geoclaw_run.c:504-515, for an ensemble). */
+int gf_pool_get_scenarios(gf_pool *pool, int first, int n, gf_scenario_info *info);
+
 /* Advance all active scenarios by up to `substeps` attempts: each attempt is
  * ghost fill -> fused step of every patch -> per-scenario CFL reduction and
  * accept/retake decision on the device.  With use_graph != 0 the attempt is
