@@ -22,6 +22,7 @@ from __future__ import annotations
 import argparse
 import json
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -142,9 +143,27 @@ class Engine:
         self.live: dict[int, _Live] = {}
         self.attempts = 0
         self.cell_updates = 0
+        self.prep = ThreadPoolExecutor(max_workers=2)  # host-side DEM/layout of queued requests
+        self.ready: dict[int, object] = {}
 
     def close(self):
+        self.prep.shutdown(wait=True)
         self.pool.close()
+
+    @staticmethod
+    def prepare(req: Request):
+        """Host arrays of a request's initial state (dry, Gaussian-sum DEM) in
+        the pool's upload layout."""
+        m = req.n // MX
+        dx = LENGTH / req.n
+        layout = BrickLayout(m, m, MX, MX, dx, dx, mthbc=(0, 1, 3, 3))
+        b = gaussian_sum_dem(req.n, LENGTH, seed=req.seed)
+        z = np.zeros((req.n, req.n))
+        q, aux = global_to_aos(z, z, z, b, layout)
+        return layout, q, aux
+
+    def enqueue(self, req: Request):
+        self.ready[req.rid] = self.prep.submit(self.prepare, req)
 
     @staticmethod
     def footprint_bytes(req: Request) -> int:
@@ -152,16 +171,12 @@ class Engine:
 
     def admit(self, req: Request, now: float) -> bool:
         """Place the request in free slots if its footprint fits."""
-        first = self.alloc.alloc(req.npatch)
-        if first is None:
+        if req.npatch > self.alloc.largest():
             return False
-        m = req.n // MX
-        dx = LENGTH / req.n
-        layout = BrickLayout(m, m, MX, MX, dx, dx, mthbc=(0, 1, 3, 3))
+        fut = self.ready.pop(req.rid, None)
+        layout, q, aux = fut.result() if fut is not None else self.prepare(req)
+        first = self.alloc.alloc(req.npatch)
         self.pool.describe(layout, first)
-        b = gaussian_sum_dem(req.n, LENGTH, seed=req.seed)
-        z = np.zeros((req.n, req.n))
-        q, aux = global_to_aos(z, z, z, b, layout)
         self.pool.upload_aos(first, q, aux, buf=0)
         sid = self.pool.add_scenario(layout, dt0=DT0, desired_cfl=0.9, max_cfl=1.0, t_end=req.t_end,
                                      t0=T_START)
@@ -212,6 +227,7 @@ class Engine:
             tnow = now()
             while pending and pending[0].arrival <= tnow:
                 queue.append(pending.pop(0))
+                self.enqueue(queue[-1])
             # admit in arrival order; a request that does not fit waits, later
             # smaller ones may pass it (backfill)
             still = []
