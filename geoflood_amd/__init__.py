@@ -76,6 +76,9 @@ PROTOTYPES = {
     "gf_pool_get_scenarios": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.POINTER(GfScenarioInfo)]),
     "gf_pool_run": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip]),
     "gf_pool_run_plain": (ct.c_int, [ct.c_void_p, ct.c_int, _ip]),
+    "gf_pool_variants": (ct.c_int, [ct.c_void_p, _ip, _ip, _ip]),
+    "gf_pool_set_variant": (ct.c_int, [ct.c_void_p, ct.c_int]),
+    "gf_pool_autotune": (ct.c_int, [ct.c_void_p, ct.c_int, _ip, _dp]),
     "gf_synchronize": (ct.c_int, []),
     "gf_last_timing": (ct.c_int, [_dp, _dp, _ip]),
     "gf_last_timing_branches": (ct.c_int, [_dp, _dp]),

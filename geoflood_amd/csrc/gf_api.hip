@@ -91,6 +91,11 @@ struct gf_pool {
     int* counters = nullptr;               // gf::NCOUNTERS
     long long* stats = nullptr;            // [3]
     gf::BatchArgs* d_args = nullptr;       // device copy read by the persistent kernel
+    unsigned long long* tile_cfl = nullptr; // [capacity] tile fold slots of tiled step variants
+    unsigned long long* tile_h = nullptr;
+    int* tile_cnt = nullptr;
+    int variant = 0;                       // step variant (tile shape, threads); see gf_pool_autotune
+    int wet_grid = 0;                      // workgroups of the persistent step kernel (0: not computed)
     int* inflow_list = nullptr;            // [capacity] patches with a hydrograph side
     int* inflow_n = nullptr;               // [1] entries of inflow_list
     std::vector<double*> hyd_mem;          // per scenario slot: device hydrograph table
@@ -163,6 +168,9 @@ static gf::StepArgs step_args(gf_pool* p, int first, int count, double* q0, doub
     a.dt = dt;
     a.cur = cur;
     a.P = g.P;
+    a.tile_cfl = p->tile_cfl;
+    a.tile_h = p->tile_h;
+    a.tile_cnt = p->tile_cnt;
     return a;
 }
 
@@ -347,6 +355,9 @@ gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
               hipMalloc(&p->counters, gf::NCOUNTERS * sizeof(int)) == hipSuccess &&
               hipMalloc(&p->stats, 3 * sizeof(long long)) == hipSuccess &&
               hipMalloc(&p->d_args, sizeof(gf::BatchArgs)) == hipSuccess &&
+              hipMalloc(&p->tile_cfl, (size_t)capacity * sizeof(unsigned long long)) == hipSuccess &&
+              hipMalloc(&p->tile_h, (size_t)capacity * sizeof(unsigned long long)) == hipSuccess &&
+              hipMalloc(&p->tile_cnt, (size_t)capacity * sizeof(int)) == hipSuccess &&
               hipMalloc(&p->inflow_list, (size_t)capacity * sizeof(int)) == hipSuccess &&
               hipMalloc(&p->inflow_n, sizeof(int)) == hipSuccess;
     if (ok)
@@ -361,6 +372,9 @@ gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
              hipMemsetAsync(p->cflbits, 0, (size_t)capacity * sizeof(unsigned long long), g.stream) == hipSuccess &&
              hipMemsetAsync(p->counters, 0, gf::NCOUNTERS * sizeof(int), g.stream) == hipSuccess &&
              hipMemsetAsync(p->stats, 0, 3 * sizeof(long long), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->tile_cfl, 0, (size_t)capacity * sizeof(unsigned long long), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->tile_h, 0, (size_t)capacity * sizeof(unsigned long long), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->tile_cnt, 0, (size_t)capacity * sizeof(int), g.stream) == hipSuccess &&
              hipMemsetAsync(p->inflow_list, 0, (size_t)capacity * sizeof(int), g.stream) == hipSuccess &&
              hipMemsetAsync(p->inflow_n, 0, sizeof(int), g.stream) == hipSuccess &&
              hipStreamSynchronize(g.stream) == hipSuccess;
@@ -409,6 +423,9 @@ int gf_pool_destroy(gf_pool* p)
     hipFree(p->counters);
     hipFree(p->stats);
     hipFree(p->d_args);
+    hipFree(p->tile_cfl);
+    hipFree(p->tile_h);
+    hipFree(p->tile_cnt);
     hipFree(p->inflow_list);
     hipFree(p->inflow_n);
     for (double* m : p->hyd_mem) hipFree(m);
@@ -592,12 +609,12 @@ static int pool_step_impl(gf_pool* p, int first, int n, int src, int dst, double
     if (plain) {
         for (int k = 0; k < n; k++) {
             HIPCHK(hipEventRecord(g.ev_step[2 * k], g.stream));
-            HIPCHK(gf::launch_step(p->mx, p->my, step_args(p, first + k, 1, q0, q1, nullptr, dt, cur), g.stream));
+            HIPCHK(gf::launch_step(p->mx, p->my, p->variant, step_args(p, first + k, 1, q0, q1, nullptr, dt, cur), g.stream));
             HIPCHK(hipEventRecord(g.ev_step[2 * k + 1], g.stream));
         }
     } else if (n > 0) {
         HIPCHK(hipEventRecord(g.ev_step[0], g.stream));
-        HIPCHK(gf::launch_step(p->mx, p->my, step_args(p, first, n, q0, q1, nullptr, dt, cur), g.stream));
+        HIPCHK(gf::launch_step(p->mx, p->my, p->variant, step_args(p, first, n, q0, q1, nullptr, dt, cur), g.stream));
         HIPCHK(hipEventRecord(g.ev_step[1], g.stream));
     }
     if (n > 0) HIPCHK(hipMemcpyAsync(p->h_cfl, p->cfl + first, sizeof(double) * n, hipMemcpyDeviceToHost, g.stream));
@@ -841,6 +858,9 @@ static int sync_batch_args(gf_pool* p)
     a.counters = p->counters;
     a.stats = p->stats;
     a.P = g.P;
+    a.tile_cfl = p->tile_cfl;
+    a.tile_h = p->tile_h;
+    a.tile_cnt = p->tile_cnt;
     HIPCHK(hipMemcpyAsync(p->d_args, &a, sizeof(a), hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     p->args_dirty = false;
@@ -866,7 +886,18 @@ static gf::BatchArgs host_batch_args(gf_pool* p)
     a.counters = p->counters;
     a.stats = p->stats;
     a.P = g.P;
+    a.tile_cfl = p->tile_cfl;
+    a.tile_h = p->tile_h;
+    a.tile_cnt = p->tile_cnt;
     return a;
+}
+
+// Workgroups of the persistent step kernel: every CU filled to the
+// variant's occupancy.
+static int wet_grid(gf_pool* p)
+{
+    if (p->wet_grid == 0) p->wet_grid = g.num_cu * gf::wet_occupancy(p->mx, p->my, p->variant);
+    return p->wet_grid;
 }
 
 // One attempt of every active scenario: classify -> (active patches on the
@@ -884,7 +915,7 @@ static int enqueue_attempt(gf_pool* p, int k)
     HIPCHK(gf::launch_dry(p->mx, p->my, a, 4 * g.num_cu, g.stream2));
     if (k >= 0) HIPCHK(hipEventRecord(g.ev_dry[2 * k + 1], g.stream2));
     if (k >= 0) HIPCHK(hipEventRecord(g.ev_wet[2 * k], g.stream));
-    HIPCHK(gf::launch_wet(p->mx, p->my, p->d_args, g.num_cu, g.stream));
+    HIPCHK(gf::launch_wet(p->mx, p->my, p->variant, p->d_args, wet_grid(p), g.stream));
     if (k >= 0) HIPCHK(hipEventRecord(g.ev_wet[2 * k + 1], g.stream));
     HIPCHK(hipEventRecord(g.ev_join, g.stream2));
     HIPCHK(hipStreamWaitEvent(g.stream, g.ev_join, 0));
@@ -995,7 +1026,7 @@ int gf_pool_run_plain(gf_pool* p, int substeps, int* any_active)
                                      kInflowGrid, g.stream));
             for (int k = hc[s].first_patch; k < hc[s].first_patch + hc[s].npatch; k++, e++) {
                 HIPCHK(hipEventRecord(g.ev_step[2 * e], g.stream));
-                HIPCHK(gf::launch_step(p->mx, p->my,
+                HIPCHK(gf::launch_step(p->mx, p->my, p->variant,
                                        step_args(p, k, 1, p->q[0], p->q[1], nullptr, hc[s].dt, hc[s].cur), g.stream));
                 HIPCHK(hipEventRecord(g.ev_step[2 * e + 1], g.stream));
             }
@@ -1089,7 +1120,7 @@ int gf_step2_batch(int mx, int my, int mbc, int maux, int n, double* const* q, d
                               hipMemcpyHostToDevice, g.stream));
         HIPCHK(gf::launch_pack(p->d_stage, p->d_stage + qd, p->q[0], p->aux, off, m, (int)np2, maux, g.stream));
     }
-    HIPCHK(gf::launch_step(mx, my, step_args(p, 0, n, p->q[0], nullptr, nullptr, dt, 0), g.stream));
+    HIPCHK(gf::launch_step(mx, my, p->variant, step_args(p, 0, n, p->q[0], nullptr, nullptr, dt, 0), g.stream));
     HIPCHK(gf::launch_max(p->cfl, n, g.d_max, g.stream));
     HIPCHK(hipMemcpyAsync(g.h_max, g.d_max, sizeof(double), hipMemcpyDeviceToHost, g.stream));
     for (int off = 0; off < n; off += (int)chunk) {
@@ -1129,6 +1160,85 @@ int gf_pool_stats(gf_pool* p, long long* active_patches, long long* dry_patches,
     if (dry_patches) *dry_patches = s[1];
     if (attempts) *attempts = s[2];
     return GF_OK;
+}
+
+int gf_pool_variants(gf_pool* p, int* current, int* count, int* desc)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    const int n = gf::num_variants(p->mx, p->my);
+    if (current) *current = p->variant;
+    if (count) *count = n;
+    if (desc)
+        for (int k = 0; k < n; k++) {
+            gf::StepVariant v;
+            gf::get_variant(p->mx, p->my, k, &v);
+            desc[3 * k] = v.tx;
+            desc[3 * k + 1] = v.ty;
+            desc[3 * k + 2] = v.nt;
+        }
+    return GF_OK;
+}
+
+int gf_pool_set_variant(gf_pool* p, int variant)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (variant < 0 || variant >= gf::num_variants(p->mx, p->my))
+        return fail(GF_EINVAL, "variant %d outside [0,%d)", variant, gf::num_variants(p->mx, p->my));
+    if (variant != p->variant) {
+        p->variant = variant;
+        p->wet_grid = 0;
+        p->graph_dirty = true;
+    }
+    return GF_OK;
+}
+
+int gf_pool_autotune(gf_pool* p, int reps, int* best, double* ms)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (reps < 1) return fail(GF_EINVAL, "reps must be >= 1");
+    if (p->nscen == 0 || p->n == 0) return fail(GF_ESTATE, "no scenario registered");
+    if (!p->inflow_error.empty()) return fail(GF_EINVAL, "%s", p->inflow_error.c_str());
+    if (int r = sync_batch_args(p)) return r;
+    const int nv = gf::num_variants(p->mx, p->my);
+    const gf::BatchArgs a = host_batch_args(p);
+    // The active list of the current states (what the next attempt steps).
+    // Each candidate steps it from the accepted buffer into the other one,
+    // which the next attempt overwrites: no scenario advances.
+    HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, 0, p->capacity, -1, 0.0), kInflowGrid, g.stream));
+    HIPCHK(gf::launch_classify(a, g.stream));
+    if (int r = ensure_pair_events(g.ev_wet, 1)) return r;
+    const int saved = p->variant;
+    double best_ms = 0.0;
+    int best_v = saved;
+    for (int v = 0; v < nv; v++) {
+        p->variant = v;
+        p->wet_grid = 0;
+        double tot = 0.0;
+        for (int r = 0; r <= reps; r++) {  // the first run warms up
+            HIPCHK(hipMemsetAsync(p->counters + gf::NEXT_WET, 0, sizeof(int), g.stream));
+            HIPCHK(hipEventRecord(g.ev_wet[0], g.stream));
+            HIPCHK(gf::launch_wet(p->mx, p->my, v, p->d_args, wet_grid(p), g.stream));
+            HIPCHK(hipEventRecord(g.ev_wet[1], g.stream));
+            HIPCHK(hipEventSynchronize(g.ev_wet[1]));
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, g.ev_wet[0], g.ev_wet[1]));
+            if (r > 0) tot += t;
+        }
+        const double avg = tot / reps;
+        if (ms) ms[v] = avg;
+        if (v == 0 || avg < best_ms) {
+            best_ms = avg;
+            best_v = v;
+        }
+    }
+    // undo the candidates' side effects on the attempt's counters and maxima
+    HIPCHK(hipMemsetAsync(p->counters, 0, gf::NCOUNTERS * sizeof(int), g.stream));
+    HIPCHK(hipMemsetAsync(p->cflbits, 0, (size_t)p->capacity * sizeof(unsigned long long), g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    p->variant = saved;
+    p->wet_grid = 0;
+    if (best) *best = best_v;
+    return gf_pool_set_variant(p, best_v);
 }
 
 int gf_last_timing(double* total_ms, double* step_ms, int* step_launches)

@@ -22,6 +22,9 @@ struct StepArgs {
     double dt;            // when ctrl == nullptr
     int cur;              // when ctrl == nullptr: read buffer cur, write buffer cur^1
     Params P;
+    unsigned long long* tile_cfl;  // per patch: tile fold slots (tiled variants)
+    unsigned long long* tile_h;
+    int* tile_cnt;
 };
 
 struct GhostArgs {
@@ -72,16 +75,28 @@ struct BatchArgs {
     int* counters;                // NCOUNTERS ints
     long long* stats;             // accumulated active patches, dry patches, attempts
     Params P;
+    unsigned long long* tile_cfl; // per patch: tile fold slots (tiled variants)
+    unsigned long long* tile_h;
+    int* tile_cnt;
 };
+
+// A step variant: patch shape, tile shape, threads per workgroup.
+struct StepVariant {
+    int mx, my, tx, ty, nt;
+};
+int num_variants(int mx, int my);
+bool get_variant(int mx, int my, int k, StepVariant* out);
+// Resident workgroups per CU of the persistent step kernel of a variant.
+int wet_occupancy(int mx, int my, int variant);
 
 bool supported_shape(int mx, int my);
 hipError_t launch_classify(const BatchArgs& a, hipStream_t st);
 // da: the BatchArgs in device memory (read by the persistent kernel on every iteration)
-hipError_t launch_wet(int mx, int my, const BatchArgs* da, int grid, hipStream_t st);
+hipError_t launch_wet(int mx, int my, int variant, const BatchArgs* da, int grid, hipStream_t st);
 hipError_t launch_dry(int mx, int my, const BatchArgs& a, int grid, hipStream_t st);
 hipError_t launch_batch_control(const BatchArgs& a, hipStream_t st);
 hipError_t launch_hmax(int mx, int my, const double* q, double* hmax, int first, int n, hipStream_t st);
-hipError_t launch_step(int mx, int my, const StepArgs& a, hipStream_t st);
+hipError_t launch_step(int mx, int my, int variant, const StepArgs& a, hipStream_t st);
 hipError_t launch_ghost(int mx, int my, const GhostArgs& a, hipStream_t st);
 hipError_t launch_control(ScenCtrl* ctrl, int nscen, const double* cfl, hipStream_t st);
 hipError_t launch_inflow(int mx, int my, const InflowArgs& a, int grid, hipStream_t st);

@@ -233,29 +233,36 @@ __device__ __forceinline__ double block_max(double v, double* red)
     return r;
 }
 
-// The whole step of one patch by one workgroup of NT threads.  q_src / q_dst
-// are the pool's source and destination buffers (the patch's slot is
-// selected here; q_dst == q_src updates in place).  With PULL the tile's
-// ghost cells are gathered from the neighbours' interiors in q_src
-// (same-level exchange and physical sides fused into the load); otherwise
-// the stored ghost cells are used.  Thread 0 receives the patch's capped CFL
-// number and the maximum interior depth after the step.
-template <int MX, int MY, int NT, bool PULL>
+// The whole step of one tile of a patch by one workgroup of NT threads.  A
+// tile is the TX x TY block of interior cells at offset (ox, oy) of an
+// MX x MY patch (TX = MX, TY = MY: the whole patch) with its own 2-cell
+// halo: the update of an interior cell reaches two cells in each direction
+// (limiter and transverse corrections), so a tile computes exactly what the
+// whole-patch step computes for its cells.  q_src / q_dst are the pool's
+// source and destination buffers (q_dst == q_src updates in place).  With
+// PULL the halo cells outside the patch are gathered from the neighbours'
+// interiors in q_src (same-level exchange and physical sides fused into the
+// load); otherwise the stored ghost cells are used; halo cells inside the
+// patch are its own interior.  Thread 0 receives the tile's maximum CFL
+// number (uncapped) and the maximum interior depth after the step.
+template <int MX, int MY, int TX, int TY, int NT, bool PULL>
 __device__ __forceinline__ void step_patch(double* T, double* E, double* F, double* red, const PatchDesc* dptr, int patch,
-                           const double* q_src, double* q_dst, const double* aux,
+                           int ox, int oy, const double* q_src, double* q_dst, const double* aux,
                            double dt, const Params& P, double& cfl_res, double& hmax_res)
 {
-    using G = Geo<MX, MY>;
+    using G = Geo<TX, TY>;   // tile geometry (LDS)
+    using GP = Geo<MX, MY>;  // patch geometry (HBM)
     constexpr int NXP = G::NXP, NP2 = G::NP2, PX = G::PX, NPOS = G::NPOS;
+    constexpr int PNP2 = GP::NP2;
     constexpr int SLOTS = (NPOS + NT - 1) / NT;
     // Opaque copy of the thread index: inside the persistent loop this stops
     // the compiler from hoisting every index computation of the body out of
     // the loop, which would keep them live in registers across iterations.
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
-    const double* src = q_src + (size_t)patch * 3 * NP2;
-    double* dst = q_dst + (size_t)patch * 3 * NP2;
-    const double* auxp = aux + (size_t)patch * NP2;
+    const double* src = q_src + (size_t)patch * 3 * PNP2;
+    double* dst = q_dst + (size_t)patch * 3 * PNP2;
+    const double* auxp = aux + (size_t)patch * PNP2;
     const double g = P.grav, drytol = P.dry_tolerance;
     const double dtdx = dt / dptr->dx;
     const double dtdy = dt / dptr->dy;
@@ -263,23 +270,24 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
     // ---- tile load (+ halo gather) + b4step2 (geoclaw_b4step2_fort.f90:153-156) ----
     int wet = 0;
     for (int c = tid; c < NP2; c += NT) {
+        const int fi = ox + c % NXP - 1, fj = oy + c / NXP - 1;  // patch Fortran indices
+        const int pc = GP::ti(fi, fj);
         double h, hu, hv;
         if (PULL) {
-            const int fi = c % NXP - 1, fj = c / NXP - 1;
             int sp = patch, si = fi, sj = fj;
             bool nu = false, nv = false;
             const bool interior = (fi >= 1 && fi <= MX && fj >= 1 && fj <= MY);
             if (interior || !ghost_source<MX, MY>(dptr->nbr, patch, fi, fj, sp, si, sj, nu, nv)) {
-                h = src[c]; hu = src[NP2 + c]; hv = src[2 * NP2 + c];
+                h = src[pc]; hu = src[PNP2 + pc]; hv = src[2 * PNP2 + pc];
             } else {
-                const double* s2 = q_src + (size_t)sp * 3 * NP2;
-                const int k = G::ti(si, sj);
-                h = s2[k]; hu = s2[NP2 + k]; hv = s2[2 * NP2 + k];
+                const double* s2 = q_src + (size_t)sp * 3 * PNP2;
+                const int k = GP::ti(si, sj);
+                h = s2[k]; hu = s2[PNP2 + k]; hv = s2[2 * PNP2 + k];
                 if (nu) hu = -hu;
                 if (nv) hv = -hv;
             }
         } else {
-            h = src[c]; hu = src[NP2 + c]; hv = src[2 * NP2 + c];
+            h = src[pc]; hu = src[PNP2 + pc]; hv = src[2 * PNP2 + pc];
         }
         if (h < drytol) {
             h = fmaxf64(h, 0.0);
@@ -289,7 +297,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         T[c] = h;
         T[NP2 + c] = hu;
         T[2 * NP2 + c] = hv;
-        T[3 * NP2 + c] = auxp[c];
+        T[3 * NP2 + c] = auxp[pc];
         wet |= (h > drytol);
     }
     const int anywet = __syncthreads_or(wet);
@@ -299,14 +307,15 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         // and transverse splits are zero and the update leaves q unchanged, so
         // the step reduces to b4step2 + src2 on the interior and CFL = 0.
         double hm = 0.0;
-        for (int c = tid; c < MX * MY; c += NT) {
-            const int i = c % MX + 1, j = c / MX + 1;
+        for (int c = tid; c < TX * TY; c += NT) {
+            const int i = c % TX + 1, j = c / TX + 1;
             const int k = G::ti(i, j);
+            const int pk = GP::ti(ox + i, oy + j);
             double h = T[k], hu = T[NP2 + k], hv = T[2 * NP2 + k];
             if (P.src_term > 0) src2_cell(h, hu, hv, T[3 * NP2 + k], dt, P);
-            dst[k] = h;
-            dst[NP2 + k] = hu;
-            dst[2 * NP2 + k] = hv;
+            dst[pk] = h;
+            dst[PNP2 + pk] = hu;
+            dst[2 * PNP2 + pk] = hv;
             hm = fmaxf64(hm, h);
         }
         hm = block_max<NT>(hm, red);
@@ -337,7 +346,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         const int i = pi[s], j = pj[s];
         double sw[3] = {0.0, 0.0, 0.0};
         double w[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-        if (in[s] && j <= MY + 1) {
+        if (in[s] && j <= TY + 1) {
             const int kL = G::ti(i - 1, j), kR = G::ti(i, j);
             rpn2_face(T[kL], T[NP2 + kL], T[2 * NP2 + kL], T[3 * NP2 + kL],
                       T[kR], T[NP2 + kR], T[2 * NP2 + kR], T[3 * NP2 + kR], g, drytol, sw, w);
@@ -363,7 +372,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
     for (int s = 0; s < SLOTS; s++) {
         const int i = pi[s], j = pj[s];
         const int p = tid + s * NT;
-        if (in[s] && i >= 1 && i <= MX + 1 && j <= MY + 1) {
+        if (in[s] && i >= 1 && i <= TX + 1 && j <= TY + 1) {
             double wl[7], wr[7];
 #pragma unroll
             for (int k = 0; k < 7; k++) {
@@ -372,7 +381,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
             }
             double fxm[3], fxp[3];
             face_second_order<0>(Wr[s], Sr[s], wl, wr, dtdx, P, fxm, fxp, AM[s], AP[s]);
-            if (j >= 1 && j <= MY) {
+            if (j >= 1 && j <= TY) {
                 const int f = G::fi_(i, j);
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
@@ -404,7 +413,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         const int p = tid + s * NT;
 #pragma unroll
         for (int k = 0; k < 3; k++) { GA[s][k] = 0.0; GB[s][k] = 0.0; }
-        if (transverse && in[s] && i >= 1 && i <= MX && j <= MY + 1) {
+        if (transverse && in[s] && i >= 1 && i <= TX && j <= TY + 1) {
             const int kc = G::ti(i, j);
             const Rpt2Cell C = rpt2_setup(T[kc], T[NP2 + kc], T[2 * NP2 + kc], T[3 * NP2 + kc],
                                           T[3 * NP2 + G::ti(i, j - 1)], T[3 * NP2 + G::ti(i, j + 1)],
@@ -428,7 +437,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
     for (int s = 0; s < SLOTS; s++) {
         const int i = pi[s], j = pj[s];
         const int p = tid + s * NT;
-        if (in[s] && i >= 1 && i <= MX && j >= 1 && j <= MY + 1) {
+        if (in[s] && i >= 1 && i <= TX && j >= 1 && j <= TY + 1) {
 #pragma unroll
             for (int k = 0; k < 3; k++) GT[s][k] = E[k * NPOS + p - PX] + GA[s][k];
         } else {
@@ -444,7 +453,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         const int i = pi[s], j = pj[s];
         double sw[3] = {0.0, 0.0, 0.0};
         double w[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-        if (in[s] && i <= MX + 1) {
+        if (in[s] && i <= TX + 1) {
             const int kL = G::ti(i, j - 1), kR = G::ti(i, j);
             // direction-local order for y: (h, hv, hu)
             rpn2_face(T[kL], T[2 * NP2 + kL], T[NP2 + kL], T[3 * NP2 + kL],
@@ -474,7 +483,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         const int p = tid + s * NT;
 #pragma unroll
         for (int k = 0; k < 3; k++) { GMr[s][k] = 0.0; GPr[s][k] = 0.0; AM[s][k] = 0.0; AP[s][k] = 0.0; }
-        if (in[s] && i <= MX + 1 && j >= 1 && j <= MY + 1) {
+        if (in[s] && i <= TX + 1 && j >= 1 && j <= TY + 1) {
             double wl[7], wr[7], fm[3], fp[3];
 #pragma unroll
             for (int k = 0; k < 7; k++) {
@@ -509,7 +518,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         const int p = tid + s * NT;
 #pragma unroll
         for (int k = 0; k < 3; k++) { GA[s][k] = 0.0; GB[s][k] = 0.0; }
-        if (transverse && in[s] && i <= MX + 1 && j >= 1 && j <= MY) {
+        if (transverse && in[s] && i <= TX + 1 && j >= 1 && j <= TY) {
             const int kc = G::ti(i, j);
             const Rpt2Cell C = rpt2_setup(T[kc], T[2 * NP2 + kc], T[NP2 + kc], T[3 * NP2 + kc],
                                           T[3 * NP2 + G::ti(i - 1, j)], T[3 * NP2 + G::ti(i + 1, j)],
@@ -535,7 +544,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         const int p = tid + s * NT;
 #pragma unroll
         for (int k = 0; k < 3; k++) { FMr[s][k] = 0.0; FPr[s][k] = 0.0; }
-        if (in[s] && i >= 1 && i <= MX + 1 && j >= 1 && j <= MY) {
+        if (in[s] && i >= 1 && i <= TX + 1 && j >= 1 && j <= TY) {
             const double g2[3] = {E[p - 1], E[2 * NPOS + p - 1], E[NPOS + p - 1]};  // local -> physical
             const double g1[3] = {GA[s][0], GA[s][2], GA[s][1]};
             const int f = G::fi_(i, j);
@@ -565,8 +574,9 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
     for (int s = 0; s < SLOTS; s++) {
         const int i = pi[s], j = pj[s];
         const int p = tid + s * NT;
-        if (in[s] && i >= 1 && i <= MX && j >= 1 && j <= MY) {
+        if (in[s] && i >= 1 && i <= TX && j >= 1 && j <= TY) {
             const int k = G::ti(i, j);
+            const int pk = GP::ti(ox + i, oy + j);
             double qn[3];
 #pragma unroll
             for (int m = 0; m < 3; m++) {
@@ -575,16 +585,16 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
                 qn[m] = T[m * NP2 + k] - dtdx * (fmr - FPr[s][m]) - dtdy * (gmt - GPr[s][m]);
             }
             if (P.src_term > 0) src2_cell(qn[0], qn[1], qn[2], T[3 * NP2 + k], dt, P);
-            dst[k] = qn[0];
-            dst[NP2 + k] = qn[1];
-            dst[2 * NP2 + k] = qn[2];
+            dst[pk] = qn[0];
+            dst[PNP2 + pk] = qn[1];
+            dst[2 * PNP2 + pk] = qn[2];
             hm = fmaxf64(hm, qn[0]);
         }
     }
 
     // ================= CFL and depth reductions =================
-    double v = block_max<NT>(cfl, red);
-    if (v > 1000.0) v = 0.0;  // MAXCFL_CAP per patch (fc2d_cpucuda.cpp:392)
+    // (the per-patch MAXCFL_CAP is applied by the caller, once per patch)
+    const double v = block_max<NT>(cfl, red);
     const double hmx = block_max<NT>(hm, red);
     if (tid == 0) {
         cfl_res = v;
@@ -592,46 +602,93 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
     }
 }
 
-// Plain step kernel: one workgroup per patch, stored ghost cells.
-template <int MX, int MY, int NT>
+// Tile decomposition of an MX x MY patch into TX x TY tiles.
+template <int MX, int MY, int TX, int TY>
+struct Tiling {
+    static_assert(MX % TX == 0 && MY % TY == 0, "tiles must divide the patch");
+    static constexpr int NTX = MX / TX;
+    static constexpr int NTILES = NTX * (MY / TY);
+    __device__ static void origin(int tile, int& ox, int& oy)
+    {
+        ox = (tile % NTX) * TX;
+        oy = (tile / NTX) * TY;
+    }
+};
+
+// Per-patch result of a tiled step: every tile folds its CFL maximum and
+// depth maximum into the patch's slots (atomicMax on the bit patterns of
+// non-negative values; zeros are left out because -0.0 sorts above every
+// positive pattern), then counts itself in; the tile that completes the
+// count takes the patch's totals and clears the slots for the next use.
+// Returns true in thread 0 of that last tile, with the totals in cfl / hm.
+__device__ __forceinline__ bool tile_fold(unsigned long long* tcfl, unsigned long long* th, int* tcnt,
+                                          int patch, int ntiles, double& cfl, double& hm)
+{
+    if (cfl != 0.0) atomicMax(tcfl + patch, (unsigned long long)__double_as_longlong(cfl));
+    if (hm != 0.0) atomicMax(th + patch, (unsigned long long)__double_as_longlong(hm));
+    __threadfence();
+    const int done = atomicAdd(tcnt + patch, 1);
+    if (done != ntiles - 1) return false;
+    __threadfence();
+    cfl = __longlong_as_double((long long)atomicExch(tcfl + patch, 0ull));
+    hm = __longlong_as_double((long long)atomicExch(th + patch, 0ull));
+    atomicExch(tcnt + patch, 0);
+    return true;
+}
+
+// Plain step kernel: one workgroup per tile, stored ghost cells.
+template <int MX, int MY, int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT) k_step(StepArgs a)
 {
-    using G = Geo<MX, MY>;
+    using G = Geo<TX, TY>;
+    using TL = Tiling<MX, MY, TX, TY>;
     __shared__ double T[4 * G::NP2];
     __shared__ double E[7 * G::NPOS];
     __shared__ double F[6 * G::NF];
     __shared__ double red[NT / 64];
     const int bid = blockIdx.x;
-    if (bid >= a.count) return;
-    const int patch = a.first + bid;
+    if (bid >= a.count * TL::NTILES) return;
+    const int patch = a.first + bid / TL::NTILES;
+    int ox, oy;
+    TL::origin(bid % TL::NTILES, ox, oy);
     const double* src = a.cur ? a.q1 : a.q0;
     double* dst = a.q1 ? (a.cur ? a.q0 : a.q1) : a.q0;
     double cfl = 0.0, hm = 0.0;
-    step_patch<MX, MY, NT, false>(T, E, F, red, a.desc + patch, patch, src, dst, a.aux, a.dt, a.P, cfl, hm);
+    step_patch<MX, MY, TX, TY, NT, false>(T, E, F, red, a.desc + patch, patch, ox, oy, src, dst, a.aux, a.dt,
+                                          a.P, cfl, hm);
     if (threadIdx.x == 0) {
-        a.cfl_out[patch] = cfl;
-        if (a.hmax_out) a.hmax_out[patch] = hm;
+        bool last = true;
+        if (TL::NTILES > 1) last = tile_fold(a.tile_cfl, a.tile_h, a.tile_cnt, patch, TL::NTILES, cfl, hm);
+        if (last) {
+            if (cfl > 1000.0) cfl = 0.0;  // MAXCFL_CAP per patch (fc2d_cpucuda.cpp:392)
+            a.cfl_out[patch] = cfl;
+            if (a.hmax_out) a.hmax_out[patch] = hm;
+        }
     }
 }
 
 // Persistent step kernel over the device work list of active patches:
-// each workgroup takes the next list entry from an atomic counter, pulls the
-// patch's halo from its neighbours, steps it, and folds its CFL number into
-// the scenario's maximum (atomicMax on the bit pattern of a CFL > 0).
-// One list entry of the persistent kernel.  (The library is built with
-// machine-level loop-invariant code motion off: hoisting the body's constant
-// and index materialisations out of the persistent loop would keep them live
-// across the whole step and spill.)
-template <int MX, int MY, int NT>
+// each workgroup takes the next (patch, tile) entry from an atomic counter,
+// pulls the tile's halo from the patch and its neighbours, steps it, and
+// folds the patch's capped CFL number into the scenario's maximum
+// (atomicMax on the bit pattern of a CFL > 0).  One entry of the persistent
+// kernel.  (The library is built with machine-level loop-invariant code
+// motion off: hoisting the body's constant and index materialisations out
+// of the persistent loop would keep them live across the whole step and
+// spill.)
+template <int MX, int MY, int TX, int TY, int NT>
 __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx)
 {
-    using G = Geo<MX, MY>;
+    using G = Geo<TX, TY>;
+    using TL = Tiling<MX, MY, TX, TY>;
     __shared__ double T[4 * G::NP2];
     __shared__ double E[7 * G::NPOS];
     __shared__ double F[6 * G::NF];
     __shared__ double red[NT / 64];
     const BatchArgs& a = *pa;
-    const int patch = a.list_wet[idx];
+    const int patch = a.list_wet[idx / TL::NTILES];
+    int ox, oy;
+    TL::origin(idx % TL::NTILES, ox, oy);
     const int scen = a.desc[patch].scen;
     const int cur = a.ctrl[scen].cur;
     const double dt = a.ctrl[scen].dt;
@@ -640,29 +697,36 @@ __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx)
     Params P = a.P;
     P.manning_n = a.ctrl[scen].manning_n;
     double cfl = 0.0, hm = 0.0;
-    step_patch<MX, MY, NT, true>(T, E, F, red, a.desc + patch, patch, src, dst, a.aux, dt, P, cfl, hm);
+    step_patch<MX, MY, TX, TY, NT, true>(T, E, F, red, a.desc + patch, patch, ox, oy, src, dst, a.aux, dt, P,
+                                         cfl, hm);
     if (threadIdx.x == 0) {
-        a.hmax[(size_t)(cur ^ 1) * a.capacity + patch] = hm;
-        // The unsigned order of bit patterns is the order of non-negative
-        // doubles, but -0.0 (which the CFL maximum can return for a patch
-        // with no waves) sorts above them all: zeros are left out (the
-        // scenario's maximum starts at +0.0).  NaN still propagates.
-        if (cfl != 0.0) atomicMax(a.cflbits + scen, (unsigned long long)__double_as_longlong(cfl));
+        bool last = true;
+        if (TL::NTILES > 1) last = tile_fold(a.tile_cfl, a.tile_h, a.tile_cnt, patch, TL::NTILES, cfl, hm);
+        if (last) {
+            if (cfl > 1000.0) cfl = 0.0;  // MAXCFL_CAP per patch (fc2d_cpucuda.cpp:392)
+            a.hmax[(size_t)(cur ^ 1) * a.capacity + patch] = hm;
+            // The unsigned order of bit patterns is the order of non-negative
+            // doubles, but -0.0 (which the CFL maximum can return for a patch
+            // with no waves) sorts above them all: zeros are left out (the
+            // scenario's maximum starts at +0.0).  NaN still propagates.
+            if (cfl != 0.0) atomicMax(a.cflbits + scen, (unsigned long long)__double_as_longlong(cfl));
+        }
     }
     __syncthreads();
 }
 
-template <int MX, int MY, int NT>
+template <int MX, int MY, int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT) k_wet(const BatchArgs* pa)
 {
+    using TL = Tiling<MX, MY, TX, TY>;
     __shared__ int item;
     for (;;) {
         if (threadIdx.x == 0) item = atomicAdd(&pa->counters[NEXT_WET], 1);
         __syncthreads();
         const int idx = item;
         __syncthreads();
-        if (idx >= pa->counters[CNT_WET]) break;
-        wet_item<MX, MY, NT>(pa, idx);
+        if (idx >= pa->counters[CNT_WET] * TL::NTILES) break;
+        wet_item<MX, MY, TX, TY, NT>(pa, idx);
     }
 }
 
@@ -1089,15 +1153,82 @@ __global__ void __launch_bounds__(256) k_max(const double* v, int n, double* out
 // ---------------------------------------------------------------------------
 // Host-side launchers (declared in gf_launch.h)
 // ---------------------------------------------------------------------------
-bool supported_shape(int mx, int my) { return mx == my && (mx == 8 || mx == 16 || mx == 32); }
+bool supported_shape(int mx, int my) { return mx == my && (mx == 8 || mx == 16 || mx == 32 || mx == 64); }
 
-hipError_t launch_step(int mx, int my, const StepArgs& a, hipStream_t st)
+// Step variants per patch shape: tile shape and threads per workgroup.
+// Variant 0 is the default; the autotuner picks among them by timing.
+static const StepVariant kVariants[] = {
+    {8, 8, 8, 8, 128},
+    {16, 16, 16, 16, 384},
+    {16, 16, 16, 16, 192},
+    {32, 32, 32, 32, 640},
+    {32, 32, 16, 16, 384},
+    {64, 64, 32, 32, 640},
+    {64, 64, 16, 16, 384},
+};
+
+int num_variants(int mx, int my)
+{
+    int n = 0;
+    for (const StepVariant& v : kVariants) n += (v.mx == mx && v.my == my);
+    return n;
+}
+
+bool get_variant(int mx, int my, int k, StepVariant* out)
+{
+    for (const StepVariant& v : kVariants)
+        if (v.mx == mx && v.my == my && k-- == 0) {
+            *out = v;
+            return true;
+        }
+    return false;
+}
+
+static int variant_key(int mx, int variant)
+{
+    StepVariant v;
+    if (!get_variant(mx, mx, variant, &v)) return -1;
+    return (v.mx * 100 + v.tx) * 10000 + v.nt;
+}
+
+template <typename K>
+static int occupancy_of(K kernel, int nt)
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, nt, 0) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
+}
+
+int wet_occupancy(int mx, int my, int variant)
+{
+    if (mx != my) return 1;
+    switch (variant_key(mx, variant)) {
+    case 8080128: return occupancy_of(k_wet<8, 8, 8, 8, 128>, 128);
+    case 16160384: return occupancy_of(k_wet<16, 16, 16, 16, 384>, 384);
+    case 16160192: return occupancy_of(k_wet<16, 16, 16, 16, 192>, 192);
+    case 32320640: return occupancy_of(k_wet<32, 32, 32, 32, 640>, 640);
+    case 32160384: return occupancy_of(k_wet<32, 32, 16, 16, 384>, 384);
+    case 64320640: return occupancy_of(k_wet<64, 64, 32, 32, 640>, 640);
+    case 64160384: return occupancy_of(k_wet<64, 64, 16, 16, 384>, 384);
+    default: return 1;
+    }
+}
+
+hipError_t launch_step(int mx, int my, int variant, const StepArgs& a, hipStream_t st)
 {
     if (a.count <= 0) return hipSuccess;
-    switch (mx * 1000 + my) {
-    case 32032: k_step<32, 32, 640><<<a.count, 640, 0, st>>>(a); break;
-    case 16016: k_step<16, 16, 384><<<a.count, 384, 0, st>>>(a); break;
-    case 8008: k_step<8, 8, 128><<<a.count, 128, 0, st>>>(a); break;
+    if (mx != my) return hipErrorInvalidValue;
+    StepVariant v;
+    if (!get_variant(mx, my, variant, &v)) return hipErrorInvalidValue;
+    const int grid = a.count * (mx / v.tx) * (my / v.ty);
+    switch (variant_key(mx, variant)) {
+    case 8080128: k_step<8, 8, 8, 8, 128><<<grid, 128, 0, st>>>(a); break;
+    case 16160384: k_step<16, 16, 16, 16, 384><<<grid, 384, 0, st>>>(a); break;
+    case 16160192: k_step<16, 16, 16, 16, 192><<<grid, 192, 0, st>>>(a); break;
+    case 32320640: k_step<32, 32, 32, 32, 640><<<grid, 640, 0, st>>>(a); break;
+    case 32160384: k_step<32, 32, 16, 16, 384><<<grid, 384, 0, st>>>(a); break;
+    case 64320640: k_step<64, 64, 32, 32, 640><<<grid, 640, 0, st>>>(a); break;
+    case 64160384: k_step<64, 64, 16, 16, 384><<<grid, 384, 0, st>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1110,6 +1241,7 @@ hipError_t launch_ghost(int mx, int my, const GhostArgs& a, hipStream_t st)
     case 32032: k_ghost<32, 32><<<a.count, 256, 0, st>>>(a); break;
     case 16016: k_ghost<16, 16><<<a.count, 256, 0, st>>>(a); break;
     case 8008: k_ghost<8, 8><<<a.count, 256, 0, st>>>(a); break;
+    case 64064: k_ghost<64, 64><<<a.count, 256, 0, st>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1122,6 +1254,7 @@ hipError_t launch_inflow(int mx, int my, const InflowArgs& a, int grid, hipStrea
     case 32032: k_inflow<32, 32><<<grid, 256, 0, st>>>(a); break;
     case 16016: k_inflow<16, 16><<<grid, 256, 0, st>>>(a); break;
     case 8008: k_inflow<8, 8><<<grid, 256, 0, st>>>(a); break;
+    case 64064: k_inflow<64, 64><<<grid, 256, 0, st>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1134,12 +1267,17 @@ hipError_t launch_classify(const BatchArgs& a, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_wet(int mx, int my, const BatchArgs* da, int grid, hipStream_t st)
+hipError_t launch_wet(int mx, int my, int variant, const BatchArgs* da, int grid, hipStream_t st)
 {
-    switch (mx * 1000 + my) {
-    case 32032: k_wet<32, 32, 640><<<grid, 640, 0, st>>>(da); break;
-    case 16016: k_wet<16, 16, 384><<<grid, 384, 0, st>>>(da); break;
-    case 8008: k_wet<8, 8, 128><<<grid, 128, 0, st>>>(da); break;
+    if (mx != my) return hipErrorInvalidValue;
+    switch (variant_key(mx, variant)) {
+    case 8080128: k_wet<8, 8, 8, 8, 128><<<grid, 128, 0, st>>>(da); break;
+    case 16160384: k_wet<16, 16, 16, 16, 384><<<grid, 384, 0, st>>>(da); break;
+    case 16160192: k_wet<16, 16, 16, 16, 192><<<grid, 192, 0, st>>>(da); break;
+    case 32320640: k_wet<32, 32, 32, 32, 640><<<grid, 640, 0, st>>>(da); break;
+    case 32160384: k_wet<32, 32, 16, 16, 384><<<grid, 384, 0, st>>>(da); break;
+    case 64320640: k_wet<64, 64, 32, 32, 640><<<grid, 640, 0, st>>>(da); break;
+    case 64160384: k_wet<64, 64, 16, 16, 384><<<grid, 384, 0, st>>>(da); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1151,6 +1289,7 @@ hipError_t launch_dry(int mx, int my, const BatchArgs& a, int grid, hipStream_t 
     case 32032: k_dry<32, 32><<<grid, 256, 0, st>>>(a); break;
     case 16016: k_dry<16, 16><<<grid, 256, 0, st>>>(a); break;
     case 8008: k_dry<8, 8><<<grid, 256, 0, st>>>(a); break;
+    case 64064: k_dry<64, 64><<<grid, 256, 0, st>>>(a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1169,6 +1308,7 @@ hipError_t launch_hmax(int mx, int my, const double* q, double* hmax, int first,
     case 32032: k_hmax<32, 32><<<n, 256, 0, st>>>(q, hmax, first, n); break;
     case 16016: k_hmax<16, 16><<<n, 256, 0, st>>>(q, hmax, first, n); break;
     case 8008: k_hmax<8, 8><<<n, 256, 0, st>>>(q, hmax, first, n); break;
+    case 64064: k_hmax<64, 64><<<n, 256, 0, st>>>(q, hmax, first, n); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

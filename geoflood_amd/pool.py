@@ -271,6 +271,27 @@ class Pool:
         check(self.lib.gf_pool_run(self.h, substeps, int(graph), ct.byref(act)), "gf_pool_run")
         return bool(act.value)
 
+    # -- step variants ------------------------------------------------------------
+    def variants(self):
+        """(current variant, [(tile_x, tile_y, threads), ...])."""
+        cur, n = ct.c_int(0), ct.c_int(0)
+        check(self.lib.gf_pool_variants(self.h, ct.byref(cur), ct.byref(n), None), "gf_pool_variants")
+        d = np.zeros(3 * n.value, dtype=np.int32)
+        check(self.lib.gf_pool_variants(self.h, None, None, iptr(d)), "gf_pool_variants")
+        return cur.value, [tuple(int(x) for x in d[3 * k:3 * k + 3]) for k in range(n.value)]
+
+    def set_variant(self, v: int):
+        check(self.lib.gf_pool_set_variant(self.h, v), "gf_pool_set_variant")
+
+    def autotune(self, reps: int = 3):
+        """Time each variant on the current active patches and select the
+        fastest; returns (best, [ms per variant])."""
+        _, vs = self.variants()
+        ms = np.zeros(len(vs))
+        best = ct.c_int(0)
+        check(self.lib.gf_pool_autotune(self.h, reps, ct.byref(best), dptr(ms)), "gf_pool_autotune")
+        return best.value, ms.tolist()
+
     def stats(self) -> dict:
         a, d, n = ct.c_longlong(0), ct.c_longlong(0), ct.c_longlong(0)
         check(self.lib.gf_pool_stats(self.h, ct.byref(a), ct.byref(d), ct.byref(n)), "gf_pool_stats")

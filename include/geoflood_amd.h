@@ -157,6 +157,8 @@ gf_pool *gf_pool_create(int mx, int my, int mbc, int capacity);
  * cudaclaw_deallocate_fluxes (cuda_source/cudaclaw_allocate.cu:88-145). */
 int gf_pool_destroy(gf_pool *pool);
 
+/* Patch shapes: mx = my in {8, 16, 32, 64}, mbc = 2. */
+
 /* Describe patches [0, n) and forget every scenario: cell sizes, scenario
  * entries (NULL or all -1: membership is assigned by gf_pool_add_scenario;
  * patches of no active scenario are skipped by gf_pool_run), and 8 neighbour
@@ -259,6 +261,26 @@ int gf_pool_set_scenario_friction(gf_pool *pool, int scenario, double manning_n)
 
 /* Stop or restart a scenario without removing it. */
 int gf_pool_set_scenario_active(gf_pool *pool, int scenario, int active);
+
+/* ---- step variants and the autotuner ------------------------------------ */
+
+/* Step variants of the pool's patch shape: *count variants, desc[3k..3k+2] =
+ * tile width, tile height, threads per workgroup of variant k (a tile
+ * smaller than the patch splits each patch into tiles with their own
+ * 2-cell halo; the results do not depend on the variant).  *current is the
+ * variant in use.  Any pointer may be NULL.  The reference fixes one shape
+ * (cudaclaw_flux2.cu:68-120: one block of 128 threads per patch). */
+int gf_pool_variants(gf_pool *pool, int *current, int *count, int *desc);
+
+/* Select a step variant (the next gf_pool_run recaptures its graph). */
+int gf_pool_set_variant(gf_pool *pool, int variant);
+
+/* Time every variant's persistent step kernel `reps` times on the active
+ * patches of the pool's current states (no scenario advances) and select
+ * the fastest; *best (nullable) receives it, ms[k] (nullable, one entry per
+ * variant) the average time of variant k in milliseconds.  Re-run when the
+ * active population changes (after regrid, or as a flood front grows). */
+int gf_pool_autotune(gf_pool *pool, int reps, int *best, double *ms);
 
 /* Copy a scenario's control record to the host. */
 int gf_pool_get_scenario(gf_pool *pool, int scenario, gf_scenario_info *info);
