@@ -45,6 +45,7 @@ HBM_PEAK_GBS = 8000.0
 PARAMS = dict(grav=9.81, dry_tolerance=1e-3, manning_n=0.03, friction_depth=1.0e6,
               friction_forcing=1, src_term=1, order=2, transverse=2, mthlim=(4, 4, 4))
 DT0, DESIRED_CFL, MAX_CFL = 0.01, 0.9, 1.0
+RETUNE_FACTOR = 1.5
 
 
 def log(*a):
@@ -91,6 +92,8 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--retune-every", type=int, default=100,
+                    help="attempts between checks of the active population (autotune on a change)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -130,14 +133,34 @@ def main():
     cells = N * N
 
     # ---- warm-up (includes graph capture), then the timed batched run ----
+    # The step variant (tile shape, threads per workgroup) is chosen by the
+    # autotuner on the active population, and chosen again inside the timed
+    # run whenever that population has changed by more than RETUNE_FACTOR.
     reset(pool, layout, state, 0)
     pool.run(args.warmup, graph=True)
     sid = reset(pool, layout, state, 0)
+    best, tune_ms = pool.autotune(reps=3)
+    _, variants = pool.variants()
+    log(f"autotune: variants {variants}, k_wet ms {['%.4f' % m for m in tune_ms]}, chose {best}")
     pool.run(0, graph=True)  # capture outside the timed region
     geoflood_amd.load().gf_synchronize()
     log(f"timed batched run: {args.steps} attempts ...")
+    retunes = []
     t0 = time.perf_counter()
-    pool.run(args.steps, graph=True)
+    done, last_act, st_prev = 0, None, pool.stats()
+    while done < args.steps:
+        k = min(args.retune_every, args.steps - done)
+        pool.run(k, graph=True)
+        done += k
+        st = pool.stats()
+        act = (st["active_patches"] - st_prev["active_patches"]) / max(st["attempts"] - st_prev["attempts"], 1)
+        st_prev = st
+        if last_act is None:
+            last_act = act
+        elif done < args.steps and (act > RETUNE_FACTOR * last_act or act * RETUNE_FACTOR < last_act):
+            v, _ = pool.autotune(reps=2)
+            retunes.append(dict(after_attempts=done, active_patches=round(act), variant=v))
+            last_act = act
     wall = time.perf_counter() - t0
     total_ms, _, _ = geoflood_amd.last_timing()
     info_b = pool.scenario(sid)
@@ -245,6 +268,9 @@ def main():
             "mode": "batched: one hipGraph launch (ghost fill + fused step + dt control) per attempt",
             "accepted_steps": accepted,
             "simulated_time_s": info_b["t"],
+            "step_variants": [dict(tile=[v[0], v[1]], threads=v[2]) for v in variants],
+            "autotune_initial": dict(variant=best, k_wet_ms=[round(m, 5) for m in tune_ms]),
+            "autotune_retunes": retunes,
         },
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "plain": plain,
