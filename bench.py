@@ -148,6 +148,7 @@ def main():
     retunes = []
     t0 = time.perf_counter()
     done, last_act, st_prev = 0, None, pool.stats()
+    st_start = st_prev
     while done < args.steps:
         k = min(args.retune_every, args.steps - done)
         pool.run(k, graph=True)
@@ -164,9 +165,15 @@ def main():
     wall = time.perf_counter() - t0
     total_ms, _, _ = geoflood_amd.last_timing()
     info_b = pool.scenario(sid)
+    st_end = pool.stats()
+    active_per_attempt = (st_end["active_patches"] - st_start["active_patches"]) / max(
+        st_end["attempts"] - st_start["attempts"], 1)
+    dry_per_attempt = (st_end["dry_patches"] - st_start["dry_patches"]) / max(
+        st_end["attempts"] - st_start["attempts"], 1)
     accepted = info_b["nsteps"]
     value = accepted * cells / wall
     ms_per_step = 1e3 * wall / args.steps
+    log(f"batched: {active_per_attempt:.1f} active + {dry_per_attempt:.1f} dry patches per attempt")
     log(f"batched: {accepted} accepted / {args.steps} attempts, {wall:.3f} s wall, "
         f"{total_ms:.1f} ms device, t = {info_b['t']:.4f} s, {value / 1e9:.2f} G cell-updates/s")
 
@@ -271,6 +278,8 @@ def main():
             "step_variants": [dict(tile=[v[0], v[1]], threads=v[2]) for v in variants],
             "autotune_initial": dict(variant=best, k_wet_ms=[round(m, 5) for m in tune_ms]),
             "autotune_retunes": retunes,
+            "active_patches_per_attempt": round(active_per_attempt, 1),
+            "dry_patches_per_attempt": round(dry_per_attempt, 1),
         },
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "plain": plain,

@@ -1073,6 +1073,104 @@ int gfo_bc2_ex(const gfo_params *p, int mx, int my, int mbc, int maux, double *q
 }
 
 /* ------------------------------------------------------------------------- */
+/* Refinement tagging                                                         */
+/* ------------------------------------------------------------------------- */
+int gfo_flag2refine(const gfo_params *p, const gfo_refine *r, int mx, int my, int mbc, int maux,
+                    const double *q, const double *aux, double x1, double y1, double x2,
+                    double y2, double t, int level, int maxlevel, int init_flag,
+                    int is_coarsening, int i, int j)
+{
+    (void)my;
+    /* no coarsening while the initial grids are built (:27-31) */
+    if (init_flag != 0 && is_coarsening)
+        return 0;
+    /* allowflag (geolib_source/allowflag.f) is always true. */
+    /* forced refinement in a region (:42-50) */
+    for (int m = 0; m < r->mregions; m++) {
+        const double *g = r->region + 8 * m;
+        if (level < (int)g[6] && t >= g[4] && t <= g[5])
+            if (x2 > g[0] && x1 < g[2] && y2 > g[1] && y1 < g[3])
+                return 1;
+    }
+    const int max_num_speeds = r->num_speeds < maxlevel ? r->num_speeds : maxlevel;
+    const double hu = P2(q, MEQN, 2, i, j), hv = P2(q, MEQN, 3, i, j);
+    const double momentum = sqrt(hu * hu + hv * hv);
+    const double depth = P2(q, MEQN, 1, i, j);
+    const double eta = depth + P2(aux, maux, 1, i, j);
+    /* th_factor is an INTEGER in the reference: 1.0 -> 1, 0.5 -> 0 (:59-61) */
+    const int th_factor = is_coarsening ? 0 : 1;
+    for (int m = 0; m < r->num_flowgrades; m++) {
+        int needed = 0;
+        switch (r->iflowgradevariable[m]) {
+        case 1: needed = depth > r->flowgradevalue[m]; break;
+        case 2: needed = momentum > r->flowgradevalue[m]; break;
+        case 3:
+            if (depth > p->dry_tolerance)
+                needed = fabs(eta) > r->flowgradevalue[m];
+            break;
+        default: break;
+        }
+        if (needed && level < r->iflowgrademinlevel[m])
+            return 1;
+    }
+    double speed = 0.0;
+    if (depth > p->dry_tolerance) {
+        speed = momentum / depth;
+        for (int n = 1; n <= max_num_speeds; n++)
+            if (speed > th_factor * r->speed_tolerance[n - 1] && level <= n)
+                return 1;
+    }
+    if (r->num_flowgrades == 0 && depth > p->dry_tolerance) {
+        if (fabs(eta - r->sea_level) > th_factor * r->wave_tolerance) {
+            if (level < r->max_level_deep)
+                return 1;
+            if (fabs(P2(aux, maux, 1, i, j)) < r->deep_depth)
+                return 1;
+        }
+        if (fabs(P2(aux, maux, 1, i, j)) > r->deep_depth && speed < r->speed_tolerance[0])
+            return 0;
+        if (speed / depth > r->max_velocity_depth_product)
+            return 1;
+    }
+    return 0;
+}
+
+int gfo_tag4refinement(const gfo_params *p, const gfo_refine *r, int mx, int my, int mbc,
+                       int maux, double xlower, double ylower, double dx, double dy, double t,
+                       const double *q, const double *aux, int level, int maxlevel,
+                       int init_flag)
+{
+    for (int j = 1; j <= my; j++) {
+        const double y1 = ylower + (j - 1) * dy, y2 = ylower + j * dy;
+        for (int i = 1; i <= mx; i++) {
+            const double x1 = xlower + (i - 1) * dx, x2 = xlower + i * dx;
+            if (gfo_flag2refine(p, r, mx, my, mbc, maux, q, aux, x1, y1, x2, y2, t, level, maxlevel,
+                                init_flag, 0, i, j) > 0)
+                return 1;
+        }
+    }
+    return 0;
+}
+
+int gfo_tag4coarsening(const gfo_params *p, const gfo_refine *r, int mx, int my, int mbc,
+                       int maux, const double xlower[4], const double ylower[4], double dx,
+                       double dy, double t, const double *const q[4],
+                       const double *const aux[4], int level, int maxlevel, int init_flag)
+{
+    for (int k = 0; k < 4; k++)
+        for (int j = 1; j <= my; j++) {
+            const double y1 = ylower[k] + (j - 1) * dy, y2 = ylower[k] + j * dy;
+            for (int i = 1; i <= mx; i++) {
+                const double x1 = xlower[k] + (i - 1) * dx, x2 = xlower[k] + i * dx;
+                if (gfo_flag2refine(p, r, mx, my, mbc, maux, q[k], aux[k], x1, y1, x2, y2, t, level,
+                                    maxlevel, init_flag, 1, i, j) > 0)
+                    return 0;
+            }
+        }
+    return 1;
+}
+
+/* ------------------------------------------------------------------------- */
 double gfo_patch_update(const gfo_params *p, int mx, int my, int mbc, int maux,
                         double *q, double *aux, double dx, double dy,
                         double t, double dt)

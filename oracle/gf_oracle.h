@@ -117,6 +117,48 @@ int gfo_bc2_ex(const gfo_params *p, int mx, int my, int mbc, int maux, double *q
                double *aux, const int mthbc[4], double xlower, double ylower,
                double dx, double dy, double t, const gfo_hydrograph *h);
 
+/* Refinement criteria (geolib_source/refinement_data_module.f90, the
+ * geoclaw_module sea level and the regions common block of
+ * fclaw2d_source/regions.i).  region[8*m .. 8*m+7] = xlow, ylow, xhi, yhi,
+ * tlow, thi, minlevel, maxlevel of region m. */
+typedef struct gfo_refine {
+    double wave_tolerance;
+    double deep_depth;
+    int max_level_deep;
+    double max_velocity_depth_product;
+    int num_speeds;
+    const double *speed_tolerance;
+    int num_flowgrades;
+    const double *flowgradevalue;
+    const int *iflowgradevariable; /* 1 depth, 2 momentum, 3 surface elevation */
+    const int *iflowgrademinlevel;
+    double sea_level;
+    int mregions;
+    const double *region;
+} gfo_refine;
+
+/* fc2d_geoclaw_flag2refine (fclaw2d_source/fc2d_geoflood_flag2refine.f90:
+ * 7-130) at cell (i, j): 1 refine, 0 not, -1 inconclusive. */
+int gfo_flag2refine(const gfo_params *p, const gfo_refine *r, int mx, int my, int mbc, int maux,
+                    const double *q, const double *aux, double x1, double y1, double x2,
+                    double y2, double t, int level, int maxlevel, int init_flag,
+                    int is_coarsening, int i, int j);
+
+/* fc2d_geoclaw_fort_tag4refinement (fc2d_geoclaw_tag4refinement_fort.f90:
+ * 1-59): 1 when any interior cell flags the patch. */
+int gfo_tag4refinement(const gfo_params *p, const gfo_refine *r, int mx, int my, int mbc,
+                       int maux, double xlower, double ylower, double dx, double dy, double t,
+                       const double *q, const double *aux, int level, int maxlevel,
+                       int init_flag);
+
+/* fc2d_geoclaw_fort_tag4coarsening (fc2d_geoclaw_tag4coarsening_fort.f90:
+ * 1-110): 1 when no cell of the four siblings q[0..3] flags (coarsening
+ * thresholds), else 0. */
+int gfo_tag4coarsening(const gfo_params *p, const gfo_refine *r, int mx, int my, int mbc,
+                       int maux, const double xlower[4], const double ylower[4], double dx,
+                       double dy, double t, const double *const q[4],
+                       const double *const aux[4], int level, int maxlevel, int init_flag);
+
 /* geoclaw_update (fc2d_cpucuda.cpp:362-406): b4step2, step2 (+update), the
  * per-patch MAXCFL_CAP (1000 -> 0), then src2 when src_term > 0. */
 double gfo_patch_update(const gfo_params *p, int mx, int my, int mbc, int maux,

@@ -104,6 +104,55 @@ class GfoHydrograph(ct.Structure):
                 ("x0", ct.c_double), ("y0", ct.c_double), ("side", ct.c_int * 4)]
 
 
+@dataclass
+class Refine:
+    """Refinement criteria (refinement_data_module, geoclaw sea_level and the
+    regions common block).  regions: rows of (xlow, ylow, xhi, yhi, tlow,
+    thi, minlevel, maxlevel); flowgrades: rows of (value, variable 1-3,
+    minlevel)."""
+    wave_tolerance: float = 0.1
+    deep_depth: float = 100.0
+    max_level_deep: int = 3
+    max_velocity_depth_product: float = 1.0e10
+    speed_tolerance: tuple = (1.0e10,)
+    flowgrades: tuple = ()
+    sea_level: float = 0.0
+    regions: tuple = ()
+
+    def arrays(self):
+        self._sp = np.ascontiguousarray(self.speed_tolerance, dtype=np.float64)
+        fg = np.asarray(self.flowgrades, dtype=np.float64).reshape(-1, 3)
+        self._fgv = np.ascontiguousarray(fg[:, 0]) if len(fg) else np.zeros(1)
+        self._fgvar = np.ascontiguousarray(fg[:, 1], dtype=np.int32) if len(fg) else np.zeros(1, np.int32)
+        self._fgmin = np.ascontiguousarray(fg[:, 2], dtype=np.int32) if len(fg) else np.zeros(1, np.int32)
+        self._reg = np.ascontiguousarray(np.asarray(self.regions, dtype=np.float64).reshape(-1, 8)) \
+            if len(self.regions) else np.zeros((1, 8))
+        return len(fg)
+
+    def to_c(self):
+        nfg = self.arrays()
+        r = GfoRefine()
+        r.wave_tolerance, r.deep_depth = self.wave_tolerance, self.deep_depth
+        r.max_level_deep, r.max_velocity_depth_product = self.max_level_deep, self.max_velocity_depth_product
+        r.num_speeds, r.speed_tolerance = len(self._sp), self._sp.ctypes.data_as(_dp)
+        r.num_flowgrades = nfg
+        r.flowgradevalue = self._fgv.ctypes.data_as(_dp)
+        r.iflowgradevariable = self._fgvar.ctypes.data_as(_ip)
+        r.iflowgrademinlevel = self._fgmin.ctypes.data_as(_ip)
+        r.sea_level = self.sea_level
+        r.mregions = len(self.regions)
+        r.region = self._reg.ctypes.data_as(_dp)
+        return r
+
+
+class GfoRefine(ct.Structure):
+    _fields_ = [("wave_tolerance", ct.c_double), ("deep_depth", ct.c_double),
+                ("max_level_deep", ct.c_int), ("max_velocity_depth_product", ct.c_double),
+                ("num_speeds", ct.c_int), ("speed_tolerance", _dp), ("num_flowgrades", ct.c_int),
+                ("flowgradevalue", _dp), ("iflowgradevariable", _ip), ("iflowgrademinlevel", _ip),
+                ("sea_level", ct.c_double), ("mregions", ct.c_int), ("region", _dp)]
+
+
 class _Lib:
     """Common wrappers over either implementation."""
 
@@ -247,6 +296,31 @@ class CLib(_Lib):
         h = None if hyd is None else hyd.to_c()
         return fn(ct.byref(self._cp), mx, my, mbc, maux, ptr(q), ptr(aux), iptr(mb), xlower, ylower,
                   dx, dy, t, None if h is None else ct.cast(ct.byref(h), ct.c_void_p))
+
+    # --- refinement tagging -----------------------------------------------
+    def tag4refinement(self, rf: Refine, mx, my, mbc, q, aux, xlower, ylower, dx, dy, t, level,
+                       maxlevel, init_flag=0, maux=1):
+        fn = self.lib.gfo_tag4refinement
+        fn.argtypes = [ct.c_void_p, ct.POINTER(GfoRefine)] + [ct.c_int] * 4 + [ct.c_double] * 5 + \
+            [_dp, _dp] + [ct.c_int] * 3
+        fn.restype = ct.c_int
+        r = rf.to_c()
+        return fn(ct.byref(self._cp), ct.byref(r), mx, my, mbc, maux, xlower, ylower, dx, dy, t,
+                  ptr(q), ptr(aux), level, maxlevel, init_flag)
+
+    def tag4coarsening(self, rf: Refine, mx, my, mbc, qs, auxs, xlower, ylower, dx, dy, t, level,
+                       maxlevel, init_flag=0, maux=1):
+        fn = self.lib.gfo_tag4coarsening
+        fn.argtypes = [ct.c_void_p, ct.POINTER(GfoRefine)] + [ct.c_int] * 4 + [_dp, _dp] + \
+            [ct.c_double] * 3 + [ct.POINTER(_dp), ct.POINTER(_dp)] + [ct.c_int] * 3
+        fn.restype = ct.c_int
+        r = rf.to_c()
+        xl = np.ascontiguousarray(xlower, dtype=np.float64)
+        yl = np.ascontiguousarray(ylower, dtype=np.float64)
+        qp = (_dp * 4)(*[ptr(a) for a in qs])
+        ap = (_dp * 4)(*[ptr(a) for a in auxs])
+        return fn(ct.byref(self._cp), ct.byref(r), mx, my, mbc, maux, ptr(xl), ptr(yl), dx, dy, t, qp, ap,
+                  level, maxlevel, init_flag)
 
     # --- brick driver -------------------------------------------------------
     def brick_run(self, brick: "Brick", nsteps, initial_dt, desired_cfl=0.9, max_cfl=1.0,
@@ -405,6 +479,32 @@ class RefLib(_Lib):
         fn = self.lib.gfref_bc2_ex
         fn.argtypes = [ct.c_int] * 4 + [_dp, _dp, _ip] + [ct.c_double] * 5
         fn(mx, my, mbc, maux, ptr(q), ptr(aux), iptr(mb), xlower, ylower, dx, dy, t)
+
+    def set_refinement(self, rf: Refine):
+        nfg = rf.arrays()
+        fn = self.lib.gfref_set_refinement
+        fn.argtypes = [ct.c_double, ct.c_double, ct.c_int, ct.c_double, ct.c_int, _dp, ct.c_int, _dp, _ip,
+                       _ip, ct.c_double, ct.c_int, _dp]
+        fn(rf.wave_tolerance, rf.deep_depth, rf.max_level_deep, rf.max_velocity_depth_product,
+           len(rf._sp), ptr(rf._sp), nfg, ptr(rf._fgv), iptr(rf._fgvar), iptr(rf._fgmin), rf.sea_level,
+           len(rf.regions), ptr(rf._reg))
+
+    def tag4refinement(self, mx, my, mbc, q, aux, xlower, ylower, dx, dy, t, level, maxlevel,
+                       init_flag=0, maux=1):
+        fn = self.lib.gfref_tag4refinement
+        fn.argtypes = [ct.c_int] * 4 + [ct.c_double] * 5 + [_dp, _dp] + [ct.c_int] * 3
+        fn.restype = ct.c_int
+        return fn(mx, my, mbc, maux, xlower, ylower, dx, dy, t, ptr(q), ptr(aux), level, maxlevel, init_flag)
+
+    def tag4coarsening(self, mx, my, mbc, qs, auxs, xlower, ylower, dx, dy, t, level, maxlevel,
+                       init_flag=0, maux=1):
+        fn = self.lib.gfref_tag4coarsening
+        fn.argtypes = [ct.c_int] * 4 + [_dp, _dp] + [ct.c_double] * 3 + [_dp] * 8 + [ct.c_int] * 3
+        fn.restype = ct.c_int
+        xl = np.ascontiguousarray(xlower, dtype=np.float64)
+        yl = np.ascontiguousarray(ylower, dtype=np.float64)
+        return fn(mx, my, mbc, maux, ptr(xl), ptr(yl), dx, dy, t, *[ptr(a) for a in qs],
+                  *[ptr(a) for a in auxs], level, maxlevel, init_flag)
 
     def patch_update(self, mx, my, mbc, q, aux, dx, dy, t, dt, maux=1):
         fn = self.lib.gfref_patch_update

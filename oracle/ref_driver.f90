@@ -281,6 +281,82 @@ contains
             t, 0.d0, mb)
     end subroutine gfref_bc2_ex
 
+    ! Refinement criteria that read_refinement_data / set_flow_grades read
+    ! from refinement.data / flowgrades.data (refinement_data_module.f90),
+    ! the sea level, and the regions common block (fclaw2d_source/regions.i).
+    subroutine gfref_set_refinement(wave_tol, deep, max_lev_deep, max_vdp, nspeeds, speeds, &
+            nfg, fgvalue, fgvar, fgminlevel, sealevel, nreg, reg) bind(C, name="gfref_set_refinement")
+        use refinement_data_module, only: wave_tolerance, deep_depth, max_level_deep, &
+            max_velocity_depth_product, speed_tolerance, num_flowgrades, flowgradevalue, &
+            iflowgradevariable, iflowgradetype, iflowgrademinlevel
+        use geoclaw_module, only: sea_level
+        real(c_double), value :: wave_tol, deep, max_vdp, sealevel
+        integer(c_int), value :: max_lev_deep, nspeeds, nfg, nreg
+        real(c_double), intent(in) :: speeds(nspeeds), fgvalue(*), reg(8, *)
+        integer(c_int), intent(in) :: fgvar(*), fgminlevel(*)
+        integer :: m
+        include 'regions.i'
+        wave_tolerance = wave_tol
+        deep_depth = deep
+        max_level_deep = max_lev_deep
+        max_velocity_depth_product = max_vdp
+        if (allocated(speed_tolerance)) deallocate(speed_tolerance)
+        allocate(speed_tolerance(nspeeds))
+        speed_tolerance = speeds
+        num_flowgrades = nfg
+        if (allocated(flowgradevalue)) deallocate(flowgradevalue)
+        if (allocated(iflowgradevariable)) deallocate(iflowgradevariable)
+        if (allocated(iflowgradetype)) deallocate(iflowgradetype)
+        if (allocated(iflowgrademinlevel)) deallocate(iflowgrademinlevel)
+        allocate(flowgradevalue(max(nfg, 1)), iflowgradevariable(max(nfg, 1)), &
+            iflowgradetype(max(nfg, 1)), iflowgrademinlevel(max(nfg, 1)))
+        do m = 1, nfg
+            flowgradevalue(m) = fgvalue(m)
+            iflowgradevariable(m) = fgvar(m)
+            iflowgradetype(m) = 1
+            iflowgrademinlevel(m) = fgminlevel(m)
+        end do
+        sea_level = sealevel
+        mregions = nreg
+        do m = 1, nreg
+            xlowregion(m) = reg(1, m)
+            ylowregion(m) = reg(2, m)
+            xhiregion(m) = reg(3, m)
+            yhiregion(m) = reg(4, m)
+            tlowregion(m) = reg(5, m)
+            thiregion(m) = reg(6, m)
+            minlevelregion(m) = int(reg(7, m))
+            maxlevelregion(m) = int(reg(8, m))
+        end do
+    end subroutine gfref_set_refinement
+
+    integer(c_int) function gfref_tag4refinement(mx, my, mbc, maux, xlower, ylower, dx, dy, t, &
+            q, aux, level, maxlevel, init_flag) bind(C, name="gfref_tag4refinement")
+        integer(c_int), value :: mx, my, mbc, maux, level, maxlevel, init_flag
+        real(c_double), value :: xlower, ylower, dx, dy, t
+        real(c_double) :: q(*), aux(*)
+        integer :: tag
+        call fc2d_geoclaw_fort_tag4refinement(mx, my, mbc, 3, maux, xlower, ylower, dx, dy, t, &
+            0, q, aux, level, maxlevel, init_flag, tag)
+        gfref_tag4refinement = tag
+    end function gfref_tag4refinement
+
+    integer(c_int) function gfref_tag4coarsening(mx, my, mbc, maux, xlower, ylower, dx, dy, t, &
+            q0, q1, q2, q3, a0, a1, a2, a3, level, maxlevel, init_flag) &
+            bind(C, name="gfref_tag4coarsening")
+        integer(c_int), value :: mx, my, mbc, maux, level, maxlevel, init_flag
+        real(c_double), intent(in) :: xlower(4), ylower(4)
+        real(c_double), value :: dx, dy, t
+        real(c_double) :: q0(*), q1(*), q2(*), q3(*), a0(*), a1(*), a2(*), a3(*)
+        integer :: tag
+        double precision :: xl(0:3), yl(0:3)
+        xl = xlower
+        yl = ylower
+        call fc2d_geoclaw_fort_tag4coarsening(0, mx, my, mbc, 3, maux, xl, yl, dx, dy, t, &
+            q0, q1, q2, q3, a0, a1, a2, a3, level, maxlevel, init_flag, tag)
+        gfref_tag4coarsening = tag
+    end function gfref_tag4coarsening
+
     ! geoclaw_update (fc2d_cpucuda.cpp:362-406): b4step2, step2, MAXCFL_CAP,
     ! src2.  Signature matches gfo_update_fn so the C brick driver can run it.
     real(c_double) function gfref_patch_update(ctx, mx, my, mbc, maux, q, aux, &
