@@ -35,6 +35,15 @@ class GfScenarioInfo(ct.Structure):
                 ("attempts", ct.c_int), ("active", ct.c_int), ("cur", ct.c_int)]
 
 
+class GfRefine(ct.Structure):
+    _fields_ = [("wave_tolerance", ct.c_double), ("deep_depth", ct.c_double),
+                ("max_level_deep", ct.c_int), ("max_velocity_depth_product", ct.c_double),
+                ("num_speeds", ct.c_int), ("speed_tolerance", ct.POINTER(ct.c_double)),
+                ("num_flowgrades", ct.c_int), ("flowgradevalue", ct.POINTER(ct.c_double)),
+                ("iflowgradevariable", ct.POINTER(ct.c_int)), ("iflowgrademinlevel", ct.POINTER(ct.c_int)),
+                ("sea_level", ct.c_double), ("mregions", ct.c_int), ("region", ct.POINTER(ct.c_double))]
+
+
 class GeoFloodError(RuntimeError):
     pass
 
@@ -76,6 +85,10 @@ PROTOTYPES = {
     "gf_pool_get_scenarios": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.POINTER(GfScenarioInfo)]),
     "gf_pool_run": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip]),
     "gf_pool_run_plain": (ct.c_int, [ct.c_void_p, ct.c_int, _ip]),
+    "gf_pool_tag_refine": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_double, ct.POINTER(GfRefine), ct.c_int,
+                                      ct.c_int, ct.c_int, _ip, ct.c_int, _ip]),
+    "gf_pool_tag_coarsen": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_double, ct.POINTER(GfRefine), ct.c_int,
+                                       ct.c_int, ct.c_int, _ip, ct.c_int, _ip]),
     "gf_pool_variants": (ct.c_int, [ct.c_void_p, _ip, _ip, _ip]),
     "gf_pool_set_variant": (ct.c_int, [ct.c_void_p, ct.c_int]),
     "gf_pool_autotune": (ct.c_int, [ct.c_void_p, ct.c_int, _ip, _dp]),

@@ -100,6 +100,8 @@ struct gf_pool {
     int* inflow_n = nullptr;               // [1] entries of inflow_list
     std::vector<double*> hyd_mem;          // per scenario slot: device hydrograph table
     std::string inflow_error;              // set when a scenario's inflow cannot run
+    void* tag_buf = nullptr;               // criteria, patch list and tags of the tagging calls
+    size_t tag_bytes = 0;
     bool args_dirty = true;
     unsigned args_gen = 0;                 // params generation of d_args and of the graph
     std::vector<PatchDesc> h_desc;
@@ -426,6 +428,7 @@ int gf_pool_destroy(gf_pool* p)
     hipFree(p->tile_cfl);
     hipFree(p->tile_h);
     hipFree(p->tile_cnt);
+    hipFree(p->tag_buf);
     hipFree(p->inflow_list);
     hipFree(p->inflow_n);
     for (double* m : p->hyd_mem) hipFree(m);
@@ -1160,6 +1163,96 @@ int gf_pool_stats(gf_pool* p, long long* active_patches, long long* dry_patches,
     if (dry_patches) *dry_patches = s[1];
     if (attempts) *attempts = s[2];
     return GF_OK;
+}
+
+static int tag_impl(gf_pool* p, bool coarsen, int buf, double t, const gf_refine* c, int level, int maxlevel,
+                    int init_flag, const int* patches, int n, int* tags)
+{
+    if (!p || !c) return fail(GF_EINVAL, "pool or criteria is NULL");
+    if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
+    if (n < 0 || (n > 0 && (!patches || !tags))) return fail(GF_EINVAL, "bad patch list");
+    if (c->num_speeds < 1 || !c->speed_tolerance) return fail(GF_EINVAL, "speed_tolerance needs >= 1 entry");
+    if (c->num_flowgrades < 0 || (c->num_flowgrades > 0 && (!c->flowgradevalue || !c->iflowgradevariable ||
+                                                            !c->iflowgrademinlevel)))
+        return fail(GF_EINVAL, "flowgrade arrays missing");
+    if (c->mregions < 0 || (c->mregions > 0 && !c->region)) return fail(GF_EINVAL, "region array missing");
+    if (n == 0) return GF_OK;
+    const int np = coarsen ? 4 * n : n;
+    for (int k = 0; k < np; k++)
+        if (patches[k] < 0 || patches[k] >= p->capacity) return fail(GF_EINVAL, "patch %d outside the pool", patches[k]);
+    // one allocation: criteria struct, its arrays, the patch list, the tags
+    const int nfg = c->num_flowgrades, nreg = c->mregions;
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t o_sp = al(sizeof(gf::TagCriteria));
+    const size_t o_fv = al(o_sp + sizeof(double) * c->num_speeds);
+    const size_t o_rg = al(o_fv + sizeof(double) * nfg);
+    const size_t o_var = al(o_rg + sizeof(double) * 8 * nreg);
+    const size_t o_min = al(o_var + sizeof(int) * nfg);
+    const size_t o_pat = al(o_min + sizeof(int) * nfg);
+    const size_t o_out = al(o_pat + sizeof(int) * np);
+    const size_t bytes = al(o_out + sizeof(int) * n);
+    if (bytes > p->tag_bytes) {
+        if (p->tag_buf) HIPCHK(hipFree(p->tag_buf));
+        p->tag_buf = nullptr;
+        p->tag_bytes = 0;
+        HIPCHK(hipMalloc(&p->tag_buf, bytes));
+        p->tag_bytes = bytes;
+    }
+    char* d = (char*)p->tag_buf;
+    std::vector<char> h(bytes, 0);
+    gf::TagCriteria tc{};
+    tc.wave_tolerance = c->wave_tolerance;
+    tc.deep_depth = c->deep_depth;
+    tc.max_velocity_depth_product = c->max_velocity_depth_product;
+    tc.sea_level = c->sea_level;
+    tc.max_level_deep = c->max_level_deep;
+    tc.num_speeds = c->num_speeds;
+    tc.num_flowgrades = nfg;
+    tc.mregions = nreg;
+    tc.speed_tolerance = (const double*)(d + o_sp);
+    tc.fg_value = (const double*)(d + o_fv);
+    tc.region = (const double*)(d + o_rg);
+    tc.fg_var = (const int*)(d + o_var);
+    tc.fg_minlevel = (const int*)(d + o_min);
+    std::memcpy(h.data(), &tc, sizeof(tc));
+    std::memcpy(h.data() + o_sp, c->speed_tolerance, sizeof(double) * c->num_speeds);
+    if (nfg) {
+        std::memcpy(h.data() + o_fv, c->flowgradevalue, sizeof(double) * nfg);
+        std::memcpy(h.data() + o_var, c->iflowgradevariable, sizeof(int) * nfg);
+        std::memcpy(h.data() + o_min, c->iflowgrademinlevel, sizeof(int) * nfg);
+    }
+    if (nreg) std::memcpy(h.data() + o_rg, c->region, sizeof(double) * 8 * nreg);
+    std::memcpy(h.data() + o_pat, patches, sizeof(int) * np);
+    HIPCHK(hipMemcpyAsync(d, h.data(), o_out, hipMemcpyHostToDevice, g.stream));
+    gf::TagArgs a{};
+    a.desc = p->desc;
+    a.q = p->q[buf];
+    a.aux = p->aux;
+    a.crit = (const gf::TagCriteria*)d;
+    a.patches = (const int*)(d + o_pat);
+    a.n = n;
+    a.out = (int*)(d + o_out);
+    a.t = t;
+    a.drytol = g.P.dry_tolerance;
+    a.level = level;
+    a.maxlevel = maxlevel;
+    a.init_flag = init_flag;
+    HIPCHK(gf::launch_tag(p->mx, p->my, coarsen, a, g.stream));
+    HIPCHK(hipMemcpyAsync(tags, d + o_out, sizeof(int) * n, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+}
+
+int gf_pool_tag_refine(gf_pool* p, int buf, double t, const gf_refine* crit, int level, int maxlevel,
+                       int init_flag, const int* patches, int n, int* tags)
+{
+    return tag_impl(p, false, buf, t, crit, level, maxlevel, init_flag, patches, n, tags);
+}
+
+int gf_pool_tag_coarsen(gf_pool* p, int buf, double t, const gf_refine* crit, int level, int maxlevel,
+                        int init_flag, const int* families, int n, int* tags)
+{
+    return tag_impl(p, true, buf, t, crit, level, maxlevel, init_flag, families, n, tags);
 }
 
 int gf_pool_variants(gf_pool* p, int* current, int* count, int* desc)

@@ -80,6 +80,31 @@ struct BatchArgs {
     int* tile_cnt;
 };
 
+// Refinement criteria in device memory (the arrays follow the struct in
+// the same allocation).
+struct TagCriteria {
+    double wave_tolerance, deep_depth, max_velocity_depth_product, sea_level;
+    int max_level_deep, num_speeds, num_flowgrades, mregions;
+    const double* speed_tolerance;
+    const double* fg_value;
+    const int* fg_var;
+    const int* fg_minlevel;
+    const double* region;  // 8 per region: xlow, ylow, xhi, yhi, tlow, thi, minlevel, maxlevel
+};
+
+struct TagArgs {
+    const PatchDesc* desc;
+    const double* q;        // the q buffer to examine
+    const double* aux;
+    const TagCriteria* crit;
+    const int* patches;     // n patches (refine) or 4n (coarsen: four siblings per family)
+    int n;
+    int* out;
+    double t, drytol;
+    int level, maxlevel, init_flag;
+};
+hipError_t launch_tag(int mx, int my, bool coarsen, const TagArgs& a, hipStream_t st);
+
 // A step variant: patch shape, tile shape, threads per workgroup.
 struct StepVariant {
     int mx, my, tx, ty, nt;

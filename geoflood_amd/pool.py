@@ -13,7 +13,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (GF_NB_EXTRAP, GF_NB_INFLOW, GF_NB_WALL, GfScenarioInfo, check, dptr, iptr, load)
+from . import (GF_NB_EXTRAP, GF_NB_INFLOW, GF_NB_WALL, GfRefine, GfScenarioInfo, check, dptr, iptr,
+               load)
 
 MBC = 2
 _KIND = {3: GF_NB_WALL, 1: GF_NB_EXTRAP, 0: GF_NB_INFLOW}
@@ -270,6 +271,46 @@ class Pool:
         act = ct.c_int(0)
         check(self.lib.gf_pool_run(self.h, substeps, int(graph), ct.byref(act)), "gf_pool_run")
         return bool(act.value)
+
+    # -- refinement tagging ------------------------------------------------------
+    @staticmethod
+    def refine_criteria(wave_tolerance=0.1, deep_depth=100.0, max_level_deep=3,
+                        max_velocity_depth_product=1e10, speed_tolerance=(1e10,), flowgrades=(),
+                        sea_level=0.0, regions=()):
+        """gf_refine from Python values; flowgrades rows (value, variable,
+        minlevel); regions rows (xlow, ylow, xhi, yhi, tlow, thi, minlevel,
+        maxlevel).  Returns (struct, arrays kept alive)."""
+        sp = np.ascontiguousarray(speed_tolerance, dtype=np.float64)
+        fg = np.asarray(flowgrades, dtype=np.float64).reshape(-1, 3)
+        fv = np.ascontiguousarray(fg[:, 0]) if len(fg) else np.zeros(1)
+        fvar = np.ascontiguousarray(fg[:, 1], dtype=np.int32) if len(fg) else np.zeros(1, np.int32)
+        fmin = np.ascontiguousarray(fg[:, 2], dtype=np.int32) if len(fg) else np.zeros(1, np.int32)
+        rg = np.ascontiguousarray(np.asarray(regions, dtype=np.float64).reshape(-1, 8)) if len(regions) \
+            else np.zeros((1, 8))
+        c = GfRefine()
+        c.wave_tolerance, c.deep_depth, c.max_level_deep = wave_tolerance, deep_depth, max_level_deep
+        c.max_velocity_depth_product = max_velocity_depth_product
+        c.num_speeds, c.speed_tolerance = len(sp), dptr(sp)
+        c.num_flowgrades, c.flowgradevalue = len(fg), dptr(fv)
+        c.iflowgradevariable, c.iflowgrademinlevel = iptr(fvar), iptr(fmin)
+        c.sea_level, c.mregions, c.region = sea_level, len(regions), dptr(rg)
+        return c, (sp, fv, fvar, fmin, rg)
+
+    def tag_refine(self, patches, t, level, maxlevel, init_flag=0, buf=0, **criteria):
+        pt = np.ascontiguousarray(patches, dtype=np.int32)
+        out = np.zeros(len(pt), dtype=np.int32)
+        c, keep = self.refine_criteria(**criteria)
+        check(self.lib.gf_pool_tag_refine(self.h, buf, t, ct.byref(c), level, maxlevel, init_flag, iptr(pt),
+                                          len(pt), iptr(out)), "gf_pool_tag_refine")
+        return out
+
+    def tag_coarsen(self, families, t, level, maxlevel, init_flag=0, buf=0, **criteria):
+        fm = np.ascontiguousarray(np.asarray(families, dtype=np.int32).reshape(-1, 4))
+        out = np.zeros(len(fm), dtype=np.int32)
+        c, keep = self.refine_criteria(**criteria)
+        check(self.lib.gf_pool_tag_coarsen(self.h, buf, t, ct.byref(c), level, maxlevel, init_flag, iptr(fm),
+                                           len(fm), iptr(out)), "gf_pool_tag_coarsen")
+        return out
 
     # -- step variants ------------------------------------------------------------
     def variants(self):

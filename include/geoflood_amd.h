@@ -70,6 +70,27 @@ typedef struct gf_scenario_info {
     int cur;         /* q buffer holding the accepted state */
 } gf_scenario_info;
 
+/* Refinement criteria of the tagging entry points (refinement_data_module,
+ * geoclaw_module::sea_level and the regions common block of the reference:
+ * geolib_source/refinement_data_module.f90, fclaw2d_source/regions.i).
+ * region[8*m .. 8*m+7] = xlow, ylow, xhi, yhi, tlow, thi, minlevel,
+ * maxlevel of region m. */
+typedef struct gf_refine {
+    double wave_tolerance;
+    double deep_depth;
+    int max_level_deep;
+    double max_velocity_depth_product;
+    int num_speeds;                 /* >= 1 */
+    const double *speed_tolerance;
+    int num_flowgrades;
+    const double *flowgradevalue;
+    const int *iflowgradevariable;  /* 1 depth, 2 momentum, 3 surface elevation */
+    const int *iflowgrademinlevel;
+    double sea_level;
+    int mregions;
+    const double *region;
+} gf_refine;
+
 typedef struct gf_pool gf_pool;
 
 /* ---- device and parameters ---------------------------------------------- */
@@ -261,6 +282,24 @@ int gf_pool_set_scenario_friction(gf_pool *pool, int scenario, double manning_n)
 
 /* Stop or restart a scenario without removing it. */
 int gf_pool_set_scenario_active(gf_pool *pool, int scenario, int active);
+
+/* ---- refinement tagging on resident patches ------------------------------ */
+
+/* tags[k] = 1 when any interior cell of pool slot patches[k] (buffer buf,
+ * time t, refinement level `level`) flags for refinement.  Replaces
+ * fc2d_geoclaw_fort_tag4refinement + fc2d_geoclaw_flag2refine
+ * (fclaw2d_source/fc2d_geoclaw_tag4refinement_fort.f90:1-59,
+ * fc2d_geoflood_flag2refine.f90:7-130) behind the plugin's tag4refinement
+ * hook (fc2d_cpucuda.cpp:512-560), without moving the patch to the host. */
+int gf_pool_tag_refine(gf_pool *pool, int buf, double t, const gf_refine *crit, int level,
+                       int maxlevel, int init_flag, const int *patches, int n, int *tags);
+
+/* tags[k] = 1 when the four siblings families[4k .. 4k+3] may coarsen (no
+ * cell of any of them flags under the coarsening thresholds).  Replaces
+ * fc2d_geoclaw_fort_tag4coarsening (fc2d_geoclaw_tag4coarsening_fort.f90:
+ * 1-110) behind the tag4coarsening hook (fc2d_cpucuda.cpp:562-614). */
+int gf_pool_tag_coarsen(gf_pool *pool, int buf, double t, const gf_refine *crit, int level,
+                        int maxlevel, int init_flag, const int *families, int n, int *tags);
 
 /* ---- step variants and the autotuner ------------------------------------ */
 
