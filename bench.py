@@ -42,10 +42,21 @@ N = 4096
 MX = 32
 BYTES_PER_CELL_UPDATE = 56  # read h, hu, hv, b; write h, hu, hv (FP64)
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 (FMA = 2 FLOPs)
+VALU_LANE_PEAK_T = 39.3          # 256 CUs x 64 lanes x 2.4 GHz, lane-instructions per second (T)
 PARAMS = dict(grav=9.81, dry_tolerance=1e-3, manning_n=0.03, friction_depth=1.0e6,
               friction_forcing=1, src_term=1, order=2, transverse=2, mthlim=(4, 4, 4))
 DT0, DESIRED_CFL, MAX_CFL = 0.01, 0.9, 1.0
 RETUNE_FACTOR = 1.5
+
+
+def profile_values(name):
+    """key=value lines of a summary under profiles/ (empty when absent)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return {}
+    return dict(line.split("=", 1) for line in open(path).read().split("\n")
+                if "=" in line and " " not in line.split("=", 1)[0])
 
 
 def log(*a):
@@ -96,6 +107,9 @@ def main():
                     help="attempts between checks of the active population (autotune on a change)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble-requests", type=int, default=64)
+    ap.add_argument("--ensemble-rate", type=float, default=32.0)
+    ap.add_argument("--ensemble-capacity", type=int, default=65536)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-events", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -198,16 +212,25 @@ def main():
         wet_avg_ms = wet_ms / launches
         wet_bytes = BYTES_PER_CELL_UPDATE * MX * MX * act / launches
         achieved = wet_bytes / (wet_avg_ms * 1e-3) / 1e9
-        traffic = None
-        tfile = os.path.join(ROOT, "profiles", "k_wet_32x32_uniform4096_traffic.txt")
-        if os.path.exists(tfile):
-            kv = dict(line.split("=", 1) for line in open(tfile).read().split("\n") if "=" in line)
-            traffic = float(kv.get("hbm_bytes_per_launch", "nan"))
+        # measured HBM bytes and FP64 work per active patch (rocprofv3 --pmc,
+        # profiles/), scaled by this run's active patches per launch
+        traffic = valu = None
+        kv = profile_values("k_wet_32x32_uniform4096_traffic.txt")
+        if "hbm_bytes_per_active_patch" in kv:
+            traffic = round(float(kv["hbm_bytes_per_active_patch"]) * act / launches)
+        kv = profile_values("k_wet_32x32_uniform4096_valu.txt")
+        if "fp64_flops_per_active_patch" in kv:
+            fl = float(kv["fp64_flops_per_active_patch"]) * act / launches / (wet_avg_ms * 1e-3) / 1e12
+            li = float(kv["valu_lane_instr_per_active_patch"]) * act / launches / (wet_avg_ms * 1e-3) / 1e12
+            valu = dict(fp64_tflops=round(fl, 3), fp64_peak_tflops=FP64_PEAK_TFLOPS,
+                        fp64_frac=round(fl / FP64_PEAK_TFLOPS, 4), valu_issue_frac=round(li / VALU_LANE_PEAK_T, 4),
+                        note="FP64 FLOPs (FMA = 2) and VALU lane-instructions per active patch from the "
+                             "profiles/k_wet_32x32_uniform4096_valu.txt counters, over this run's k_wet time")
         roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                        kernel="k_wet<32,32,640>", kernel_avg_ms=round(wet_avg_ms, 5),
+                        kernel="k_wet<32,32,32,32,640>", kernel_avg_ms=round(wet_avg_ms, 5),
                         bytes_per_launch=round(wet_bytes), active_patches_per_launch=round(act / launches, 1),
-                        launches=launches)
+                        launches=launches, valu=valu)
         step_avg_ms = step_ms / launches
         composite = dict(step_phase_avg_ms=round(step_avg_ms, 5),
                          achieved_GBs=round(BYTES_PER_CELL_UPDATE * cells / (step_avg_ms * 1e-3) / 1e9, 2),
@@ -218,6 +241,27 @@ def main():
         log(f"events run: k_wet avg {wet_avg_ms:.4f} ms ({act / launches:.0f} active patches), "
             f"k_dry avg {dry_ms / launches:.4f} ms ({dry / launches:.0f} dry patches), step phase avg "
             f"{step_avg_ms:.4f} ms over {launches} attempts; same trajectory as graph run: {same}")
+
+    # ---- ensemble serving sample (continuous batching of flood scenarios) ----
+    ensemble = None
+    if args.ensemble_requests > 0:
+        from geoflood_amd.pool import Pool as _P
+        from geoflood_amd.serving import Engine, make_requests, summarize
+        geoflood_amd.set_params(**dict(PARAMS, manning_n=0.035))
+        reqs = make_requests(args.ensemble_requests, args.ensemble_rate, seed=0)
+        eng = Engine(args.ensemble_capacity, substeps=8)
+        log(f"ensemble: {len(reqs)} requests at {args.ensemble_rate}/s ...")
+        res = eng.serve(reqs)
+        ensemble = summarize(res, eng.wall, eng.cell_updates)
+        ensemble.update(rate_per_s=args.ensemble_rate, t_end_s=1800.0, capacity_patches=args.ensemble_capacity,
+                        hbm_budget_bytes=_P.bytes_for(32, 32, args.ensemble_capacity),
+                        sample=f"first {len(reqs)} requests of the Poisson stream (seed 0); domains 128^2-1024^2 "
+                               "of 32x32 patches over 2 km, Gaussian-sum DEM (seed = request id), Manning n "
+                               "U[0.02,0.06], inflow peak log-U[20,2000] m^3/s, simulated 10 s -> 1800 s")
+        eng.close()
+        geoflood_amd.set_params(**PARAMS)
+        log(f"ensemble: p50 {ensemble['p50_latency_s']} s, p99 {ensemble['p99_latency_s']} s, "
+            f"{ensemble['cell_updates_per_s'] / 1e9:.2f} G cell-updates/s")
 
     # ---- plain per-patch loop ----
     plain = None
@@ -287,7 +331,7 @@ def main():
         "step_composite": composite,
         "cpu_baseline": cpu,
         "parity_vs_cpu_max_rel": parity,
-        "ensemble": None,
+        "ensemble": ensemble,
     }
     print(json.dumps(out))
 
