@@ -233,6 +233,29 @@ __device__ __forceinline__ double block_max(double v, double* red)
     return r;
 }
 
+// Block-wide maxima of two values with one barrier (results valid in every
+// thread).  red holds 2 * NT / 64 doubles; the caller guarantees that no
+// thread still reads red from an earlier use (a barrier separates them).
+template <int NT>
+__device__ __forceinline__ void block_max2(double& a, double& b, double* red)
+{
+    constexpr int NWAVE = NT / 64;
+    a = wave_max(a);
+    b = wave_max(b);
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = a;
+        red[NWAVE + (threadIdx.x >> 6)] = b;
+    }
+    __syncthreads();
+    a = red[0];
+    b = red[NWAVE];
+#pragma unroll
+    for (int k = 1; k < NWAVE; k++) {
+        a = fmaxf64(a, red[k]);
+        b = fmaxf64(b, red[NWAVE + k]);
+    }
+}
+
 // The whole step of one tile of a patch by one workgroup of NT threads.  A
 // tile is the TX x TY block of interior cells at offset (ox, oy) of an
 // MX x MY patch (TX = MX, TY = MY: the whole patch) with its own 2-cell
@@ -268,37 +291,64 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
     const double dtdy = dt / dptr->dy;
 
     // ---- tile load (+ halo gather) + b4step2 (geoclaw_b4step2_fort.f90:153-156) ----
-    int wet = 0;
-    for (int c = tid; c < NP2; c += NT) {
-        const int fi = ox + c % NXP - 1, fj = oy + c / NXP - 1;  // patch Fortran indices
+    // Addresses of every cell this thread loads first, then all loads, then
+    // the LDS stores: one memory round trip per tile instead of one per cell
+    // group.
+    constexpr int LS = (NP2 + NT - 1) / NT;
+    const double* lbase[LS];
+    int lk[LS], lpc[LS];
+    bool lok[LS], lnu[LS], lnv[LS];
+#pragma unroll
+    for (int r = 0; r < LS; r++) {
+        const int c = tid + r * NT;
+        lok[r] = c < NP2;
+        const int cc = lok[r] ? c : 0;
+        const int fi = ox + cc % NXP - 1, fj = oy + cc / NXP - 1;  // patch Fortran indices
         const int pc = GP::ti(fi, fj);
-        double h, hu, hv;
+        const double* base = src;
+        int k = pc;
+        bool nu = false, nv = false;
         if (PULL) {
-            int sp = patch, si = fi, sj = fj;
-            bool nu = false, nv = false;
             const bool interior = (fi >= 1 && fi <= MX && fj >= 1 && fj <= MY);
-            if (interior || !ghost_source<MX, MY>(dptr->nbr, patch, fi, fj, sp, si, sj, nu, nv)) {
-                h = src[pc]; hu = src[PNP2 + pc]; hv = src[2 * PNP2 + pc];
-            } else {
-                const double* s2 = q_src + (size_t)sp * 3 * PNP2;
-                const int k = GP::ti(si, sj);
-                h = s2[k]; hu = s2[PNP2 + k]; hv = s2[2 * PNP2 + k];
-                if (nu) hu = -hu;
-                if (nv) hv = -hv;
+            int sp = patch, si = fi, sj = fj;
+            if (!interior && ghost_source<MX, MY>(dptr->nbr, patch, fi, fj, sp, si, sj, nu, nv)) {
+                base = q_src + (size_t)sp * 3 * PNP2;
+                k = GP::ti(si, sj);
             }
-        } else {
-            h = src[pc]; hu = src[PNP2 + pc]; hv = src[2 * PNP2 + pc];
         }
-        if (h < drytol) {
-            h = fmaxf64(h, 0.0);
-            hu = 0.0;
-            hv = 0.0;
+        lbase[r] = base;
+        lk[r] = k;
+        lpc[r] = pc;
+        lnu[r] = nu;
+        lnv[r] = nv;
+    }
+    double lh[LS], lhu[LS], lhv[LS], lb[LS];
+#pragma unroll
+    for (int r = 0; r < LS; r++) {
+        if (lok[r]) {
+            lh[r] = lbase[r][lk[r]];
+            lhu[r] = lbase[r][PNP2 + lk[r]];
+            lhv[r] = lbase[r][2 * PNP2 + lk[r]];
+            lb[r] = auxp[lpc[r]];
         }
-        T[c] = h;
-        T[NP2 + c] = hu;
-        T[2 * NP2 + c] = hv;
-        T[3 * NP2 + c] = auxp[pc];
-        wet |= (h > drytol);
+    }
+    int wet = 0;
+#pragma unroll
+    for (int r = 0; r < LS; r++) {
+        if (lok[r]) {
+            const int c = tid + r * NT;
+            double h = lh[r], hu = lnu[r] ? -lhu[r] : lhu[r], hv = lnv[r] ? -lhv[r] : lhv[r];
+            if (h < drytol) {
+                h = fmaxf64(h, 0.0);
+                hu = 0.0;
+                hv = 0.0;
+            }
+            T[c] = h;
+            T[NP2 + c] = hu;
+            T[2 * NP2 + c] = hv;
+            T[3 * NP2 + c] = lb[r];
+            wet |= (h > drytol);
+        }
     }
     const int anywet = __syncthreads_or(wet);
 
@@ -594,8 +644,8 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
 
     // ================= CFL and depth reductions =================
     // (the per-patch MAXCFL_CAP is applied by the caller, once per patch)
-    const double v = block_max<NT>(cfl, red);
-    const double hmx = block_max<NT>(hm, red);
+    double v = cfl, hmx = hm;
+    block_max2<NT>(v, hmx, red);
     if (tid == 0) {
         cfl_res = v;
         hmax_res = hmx;
@@ -645,7 +695,7 @@ __global__ void __launch_bounds__(NT) k_step(StepArgs a)
     __shared__ double T[4 * G::NP2];
     __shared__ double E[7 * G::NPOS];
     __shared__ double F[6 * G::NF];
-    __shared__ double red[NT / 64];
+    __shared__ double red[2 * (NT / 64)];
     const int bid = blockIdx.x;
     if (bid >= a.count * TL::NTILES) return;
     const int patch = a.first + bid / TL::NTILES;
@@ -677,14 +727,14 @@ __global__ void __launch_bounds__(NT) k_step(StepArgs a)
 // of the persistent loop would keep them live across the whole step and
 // spill.)
 template <int MX, int MY, int TX, int TY, int NT>
-__device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx)
+__device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, int next, int* next_slot)
 {
     using G = Geo<TX, TY>;
     using TL = Tiling<MX, MY, TX, TY>;
     __shared__ double T[4 * G::NP2];
     __shared__ double E[7 * G::NPOS];
     __shared__ double F[6 * G::NF];
-    __shared__ double red[NT / 64];
+    __shared__ double red[2 * (NT / 64)];
     const BatchArgs& a = *pa;
     const int patch = a.list_wet[idx / TL::NTILES];
     int ox, oy;
@@ -711,6 +761,7 @@ __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx)
             // scenario's maximum starts at +0.0).  NaN still propagates.
             if (cfl != 0.0) atomicMax(a.cflbits + scen, (unsigned long long)__double_as_longlong(cfl));
         }
+        *next_slot = next;
     }
     __syncthreads();
 }
@@ -719,14 +770,20 @@ template <int MX, int MY, int TX, int TY, int NT>
 __global__ void __launch_bounds__(NT) k_wet(const BatchArgs* pa)
 {
     using TL = Tiling<MX, MY, TX, TY>;
-    __shared__ int item;
-    for (;;) {
-        if (threadIdx.x == 0) item = atomicAdd(&pa->counters[NEXT_WET], 1);
-        __syncthreads();
-        const int idx = item;
-        __syncthreads();
-        if (idx >= pa->counters[CNT_WET] * TL::NTILES) break;
-        wet_item<MX, MY, TX, TY, NT>(pa, idx);
+    // The next entry is taken from the counter while the current one is
+    // stepped (thread 0 publishes it just before the entry's last barrier);
+    // the two slots alternate, so a slot is rewritten only after every thread
+    // has read it (the barriers inside wet_item separate the two).
+    __shared__ int item[2];
+    if (threadIdx.x == 0) item[0] = atomicAdd(&pa->counters[NEXT_WET], 1);
+    __syncthreads();
+    const int limit = pa->counters[CNT_WET] * TL::NTILES;
+    for (int it = 0;; it++) {
+        const int idx = item[it & 1];
+        if (idx >= limit) break;
+        int next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(&pa->counters[NEXT_WET], 1);
+        wet_item<MX, MY, TX, TY, NT>(pa, idx, next, &item[(it + 1) & 1]);
     }
 }
 
