@@ -918,7 +918,7 @@ static int enqueue_attempt(gf_pool* p, int k)
     HIPCHK(gf::launch_dry(p->mx, p->my, a, 4 * g.num_cu, g.stream2));
     if (k >= 0) HIPCHK(hipEventRecord(g.ev_dry[2 * k + 1], g.stream2));
     if (k >= 0) HIPCHK(hipEventRecord(g.ev_wet[2 * k], g.stream));
-    HIPCHK(gf::launch_wet(p->mx, p->my, p->variant, p->d_args, wet_grid(p), g.stream));
+    HIPCHK(gf::launch_wet(p->mx, p->my, p->variant, gf::default_method(g.P), p->d_args, wet_grid(p), g.stream));
     if (k >= 0) HIPCHK(hipEventRecord(g.ev_wet[2 * k + 1], g.stream));
     HIPCHK(hipEventRecord(g.ev_join, g.stream2));
     HIPCHK(hipStreamWaitEvent(g.stream, g.ev_join, 0));
@@ -1310,7 +1310,7 @@ int gf_pool_autotune(gf_pool* p, int reps, int* best, double* ms)
         for (int r = 0; r <= reps; r++) {  // the first run warms up
             HIPCHK(hipMemsetAsync(p->counters + gf::NEXT_WET, 0, sizeof(int), g.stream));
             HIPCHK(hipEventRecord(g.ev_wet[0], g.stream));
-            HIPCHK(gf::launch_wet(p->mx, p->my, v, p->d_args, wet_grid(p), g.stream));
+            HIPCHK(gf::launch_wet(p->mx, p->my, v, gf::default_method(g.P), p->d_args, wet_grid(p), g.stream));
             HIPCHK(hipEventRecord(g.ev_wet[1], g.stream));
             HIPCHK(hipEventSynchronize(g.ev_wet[1]));
             float t = 0.f;

@@ -79,7 +79,9 @@ __device__ __forceinline__ double pdot(double a0, double a1, double a2, double b
 // philim.f).  wc: own compressed unlimited waves; wl / wr: those of the faces
 // before / after along the sweep.  Outputs in direction-local order: faddm,
 // faddp (flux increments) and amdq*, apdq* (fluctuations handed to rpt2).
-template <int DIR>
+// DEF: the default method (order 2, transverse 2, MC limiter on all three
+// families) fixed at compile time; otherwise the method is read from P.
+template <int DIR, bool DEF>
 __device__ __forceinline__ void face_second_order(const double wc[7], const double s[3],
                                                   const double wl[7], const double wr[7],
                                                   double dtdx, const Params& P,
@@ -89,7 +91,9 @@ __device__ __forceinline__ void face_second_order(const double wc[7], const doub
     double w[3][3] = {{wc[0], wc[1], wc[2]}, {0.0, wc[3], 0.0}, {wc[4], wc[5], wc[6]}};
     double amdq[3], apdq[3];
     fluctuations(s, w, amdq, apdq);
-    if (P.order == 1) {
+    const int order = DEF ? 2 : P.order;
+    const int lim0 = DEF ? 4 : P.mthlim[0], lim1 = DEF ? 4 : P.mthlim[1], lim2 = DEF ? 4 : P.mthlim[2];
+    if (order == 1) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             faddm[k] = amdq[k];
@@ -102,32 +106,32 @@ __device__ __forceinline__ void face_second_order(const double wc[7], const doub
     // limiter on the three wave families with the unlimited neighbours
     {
         // wave 1
-        if (P.mthlim[0] != 0) {
+        if (lim0 != 0) {
             const double wn2 = pdot<DIR>(wc[0], wc[1], wc[2], wc[0], wc[1], wc[2]);
             if (wn2 != 0.0) {
                 const double dotl = pdot<DIR>(wl[0], wl[1], wl[2], wc[0], wc[1], wc[2]);
                 const double dotr = pdot<DIR>(wc[0], wc[1], wc[2], wr[0], wr[1], wr[2]);
-                const double lim = philim(wn2, s[0] > 0.0 ? dotl : dotr, P.mthlim[0]);
+                const double lim = philim(wn2, s[0] > 0.0 ? dotl : dotr, lim0);
                 w[0][0] = lim * w[0][0]; w[0][1] = lim * w[0][1]; w[0][2] = lim * w[0][2];
             }
         }
         // wave 2 (only its normal-momentum component is non-zero)
-        if (P.mthlim[1] != 0) {
+        if (lim1 != 0) {
             const double wn2 = wc[3] * wc[3];
             if (wn2 != 0.0) {
                 const double dotl = wl[3] * wc[3];
                 const double dotr = wc[3] * wr[3];
-                const double lim = philim(wn2, s[1] > 0.0 ? dotl : dotr, P.mthlim[1]);
+                const double lim = philim(wn2, s[1] > 0.0 ? dotl : dotr, lim1);
                 w[1][1] = lim * w[1][1];
             }
         }
         // wave 3
-        if (P.mthlim[2] != 0) {
+        if (lim2 != 0) {
             const double wn2 = pdot<DIR>(wc[4], wc[5], wc[6], wc[4], wc[5], wc[6]);
             if (wn2 != 0.0) {
                 const double dotl = pdot<DIR>(wl[4], wl[5], wl[6], wc[4], wc[5], wc[6]);
                 const double dotr = pdot<DIR>(wc[4], wc[5], wc[6], wr[4], wr[5], wr[6]);
-                const double lim = philim(wn2, s[2] > 0.0 ? dotl : dotr, P.mthlim[2]);
+                const double lim = philim(wn2, s[2] > 0.0 ? dotl : dotr, lim2);
                 w[2][0] = lim * w[2][0]; w[2][1] = lim * w[2][1]; w[2][2] = lim * w[2][2];
             }
         }
@@ -139,7 +143,7 @@ __device__ __forceinline__ void face_second_order(const double wc[7], const doub
     cq[0] = c[0] * w[0][0] + c[2] * w[2][0];
     cq[1] = c[0] * w[0][1] + c[1] * w[1][1] + c[2] * w[2][1];
     cq[2] = c[0] * w[0][2] + c[2] * w[2][2];
-    const bool fold = (P.transverse == 2);
+    const bool fold = DEF ? true : (P.transverse == 2);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         faddm[k] = amdq[k] + 0.5 * cq[k];
@@ -268,7 +272,7 @@ __device__ __forceinline__ void block_max2(double& a, double& b, double* red)
 // load); otherwise the stored ghost cells are used; halo cells inside the
 // patch are its own interior.  Thread 0 receives the tile's maximum CFL
 // number (uncapped) and the maximum interior depth after the step.
-template <int MX, int MY, int TX, int TY, int NT, bool PULL>
+template <int MX, int MY, int TX, int TY, int NT, bool PULL, bool DEF>
 __device__ __forceinline__ void step_patch(double* T, double* E, double* F, double* red, const PatchDesc* dptr, int patch,
                            int ox, int oy, const double* q_src, double* q_dst, const double* aux,
                            double dt, const Params& P, double& cfl_res, double& hmax_res)
@@ -430,7 +434,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
                 wr[k] = E[k * NPOS + p + 1];
             }
             double fxm[3], fxp[3];
-            face_second_order<0>(Wr[s], Sr[s], wl, wr, dtdx, P, fxm, fxp, AM[s], AP[s]);
+            face_second_order<0, DEF>(Wr[s], Sr[s], wl, wr, dtdx, P, fxm, fxp, AM[s], AP[s]);
             if (j >= 1 && j <= TY) {
                 const int f = G::fi_(i, j);
 #pragma unroll
@@ -455,7 +459,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
     __syncthreads();
 
     // ================= x-transverse (cell-centred rpt2) =================
-    const bool transverse = P.transverse != 0;
+    const bool transverse = DEF ? true : P.transverse != 0;
     const double hdx = 0.5 * dtdx;
 #pragma unroll
     for (int s = 0; s < SLOTS; s++) {
@@ -540,7 +544,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
                 wl[k] = E[k * NPOS + p - PX];
                 wr[k] = E[k * NPOS + p + PX];
             }
-            face_second_order<1>(Wr[s], Sr[s], wl, wr, dtdy, P, fm, fp, AM[s], AP[s]);
+            face_second_order<1, DEF>(Wr[s], Sr[s], wl, wr, dtdy, P, fm, fp, AM[s], AP[s]);
             // gm = GT + faddm in physical order (h, hu, hv) = local (0, 2, 1)
             GMr[s][0] = GT[s][0] + fm[0];
             GMr[s][1] = GT[s][1] + fm[2];
@@ -687,7 +691,7 @@ __device__ __forceinline__ bool tile_fold(unsigned long long* tcfl, unsigned lon
 }
 
 // Plain step kernel: one workgroup per tile, stored ghost cells.
-template <int MX, int MY, int TX, int TY, int NT>
+template <int MX, int MY, int TX, int TY, int NT, bool DEF>
 __global__ void __launch_bounds__(NT) k_step(StepArgs a)
 {
     using G = Geo<TX, TY>;
@@ -704,7 +708,7 @@ __global__ void __launch_bounds__(NT) k_step(StepArgs a)
     const double* src = a.cur ? a.q1 : a.q0;
     double* dst = a.q1 ? (a.cur ? a.q0 : a.q1) : a.q0;
     double cfl = 0.0, hm = 0.0;
-    step_patch<MX, MY, TX, TY, NT, false>(T, E, F, red, a.desc + patch, patch, ox, oy, src, dst, a.aux, a.dt,
+    step_patch<MX, MY, TX, TY, NT, false, DEF>(T, E, F, red, a.desc + patch, patch, ox, oy, src, dst, a.aux, a.dt,
                                           a.P, cfl, hm);
     if (threadIdx.x == 0) {
         bool last = true;
@@ -726,7 +730,7 @@ __global__ void __launch_bounds__(NT) k_step(StepArgs a)
 // motion off: hoisting the body's constant and index materialisations out
 // of the persistent loop would keep them live across the whole step and
 // spill.)
-template <int MX, int MY, int TX, int TY, int NT>
+template <int MX, int MY, int TX, int TY, int NT, bool DEF>
 __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, int next, int* next_slot)
 {
     using G = Geo<TX, TY>;
@@ -747,7 +751,7 @@ __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, int next,
     Params P = a.P;
     P.manning_n = a.ctrl[scen].manning_n;
     double cfl = 0.0, hm = 0.0;
-    step_patch<MX, MY, TX, TY, NT, true>(T, E, F, red, a.desc + patch, patch, ox, oy, src, dst, a.aux, dt, P,
+    step_patch<MX, MY, TX, TY, NT, true, DEF>(T, E, F, red, a.desc + patch, patch, ox, oy, src, dst, a.aux, dt, P,
                                          cfl, hm);
     if (threadIdx.x == 0) {
         bool last = true;
@@ -766,7 +770,7 @@ __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, int next,
     __syncthreads();
 }
 
-template <int MX, int MY, int TX, int TY, int NT>
+template <int MX, int MY, int TX, int TY, int NT, bool DEF>
 __global__ void __launch_bounds__(NT) k_wet(const BatchArgs* pa)
 {
     using TL = Tiling<MX, MY, TX, TY>;
@@ -783,7 +787,7 @@ __global__ void __launch_bounds__(NT) k_wet(const BatchArgs* pa)
         if (idx >= limit) break;
         int next = 0;
         if (threadIdx.x == 0) next = atomicAdd(&pa->counters[NEXT_WET], 1);
-        wet_item<MX, MY, TX, TY, NT>(pa, idx, next, &item[(it + 1) & 1]);
+        wet_item<MX, MY, TX, TY, NT, DEF>(pa, idx, next, &item[(it + 1) & 1]);
     }
 }
 
@@ -1212,6 +1216,11 @@ __global__ void __launch_bounds__(256) k_max(const double* v, int n, double* out
 // ---------------------------------------------------------------------------
 bool supported_shape(int mx, int my) { return mx == my && (mx == 8 || mx == 16 || mx == 32 || mx == 64); }
 
+bool default_method(const Params& P)
+{
+    return P.order == 2 && P.transverse == 2 && P.mthlim[0] == 4 && P.mthlim[1] == 4 && P.mthlim[2] == 4;
+}
+
 // Step variants per patch shape: tile shape and threads per workgroup.
 // Variant 0 is the default; the autotuner picks among them by timing.
 static const StepVariant kVariants[] = {
@@ -1260,13 +1269,13 @@ int wet_occupancy(int mx, int my, int variant)
 {
     if (mx != my) return 1;
     switch (variant_key(mx, variant)) {
-    case 8080128: return occupancy_of(k_wet<8, 8, 8, 8, 128>, 128);
-    case 16160384: return occupancy_of(k_wet<16, 16, 16, 16, 384>, 384);
-    case 16160192: return occupancy_of(k_wet<16, 16, 16, 16, 192>, 192);
-    case 32320640: return occupancy_of(k_wet<32, 32, 32, 32, 640>, 640);
-    case 32160384: return occupancy_of(k_wet<32, 32, 16, 16, 384>, 384);
-    case 64320640: return occupancy_of(k_wet<64, 64, 32, 32, 640>, 640);
-    case 64160384: return occupancy_of(k_wet<64, 64, 16, 16, 384>, 384);
+    case 8080128: return occupancy_of(k_wet<8, 8, 8, 8, 128, true>, 128);
+    case 16160384: return occupancy_of(k_wet<16, 16, 16, 16, 384, true>, 384);
+    case 16160192: return occupancy_of(k_wet<16, 16, 16, 16, 192, true>, 192);
+    case 32320640: return occupancy_of(k_wet<32, 32, 32, 32, 640, true>, 640);
+    case 32160384: return occupancy_of(k_wet<32, 32, 16, 16, 384, true>, 384);
+    case 64320640: return occupancy_of(k_wet<64, 64, 32, 32, 640, true>, 640);
+    case 64160384: return occupancy_of(k_wet<64, 64, 16, 16, 384, true>, 384);
     default: return 1;
     }
 }
@@ -1278,16 +1287,21 @@ hipError_t launch_step(int mx, int my, int variant, const StepArgs& a, hipStream
     StepVariant v;
     if (!get_variant(mx, my, variant, &v)) return hipErrorInvalidValue;
     const int grid = a.count * (mx / v.tx) * (my / v.ty);
+    const bool def = default_method(a.P);
+#define GF_STEP(MX_, TX_, NT_)                                                                  \
+    (def ? (k_step<MX_, MX_, TX_, TX_, NT_, true><<<grid, NT_, 0, st>>>(a), 0)                 \
+         : (k_step<MX_, MX_, TX_, TX_, NT_, false><<<grid, NT_, 0, st>>>(a), 0))
     switch (variant_key(mx, variant)) {
-    case 8080128: k_step<8, 8, 8, 8, 128><<<grid, 128, 0, st>>>(a); break;
-    case 16160384: k_step<16, 16, 16, 16, 384><<<grid, 384, 0, st>>>(a); break;
-    case 16160192: k_step<16, 16, 16, 16, 192><<<grid, 192, 0, st>>>(a); break;
-    case 32320640: k_step<32, 32, 32, 32, 640><<<grid, 640, 0, st>>>(a); break;
-    case 32160384: k_step<32, 32, 16, 16, 384><<<grid, 384, 0, st>>>(a); break;
-    case 64320640: k_step<64, 64, 32, 32, 640><<<grid, 640, 0, st>>>(a); break;
-    case 64160384: k_step<64, 64, 16, 16, 384><<<grid, 384, 0, st>>>(a); break;
+    case 8080128: GF_STEP(8, 8, 128); break;
+    case 16160384: GF_STEP(16, 16, 384); break;
+    case 16160192: GF_STEP(16, 16, 192); break;
+    case 32320640: GF_STEP(32, 32, 640); break;
+    case 32160384: GF_STEP(32, 16, 384); break;
+    case 64320640: GF_STEP(64, 32, 640); break;
+    case 64160384: GF_STEP(64, 16, 384); break;
     default: return hipErrorInvalidValue;
     }
+#undef GF_STEP
     return hipGetLastError();
 }
 
@@ -1324,19 +1338,23 @@ hipError_t launch_classify(const BatchArgs& a, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_wet(int mx, int my, int variant, const BatchArgs* da, int grid, hipStream_t st)
+hipError_t launch_wet(int mx, int my, int variant, bool def, const BatchArgs* da, int grid, hipStream_t st)
 {
     if (mx != my) return hipErrorInvalidValue;
+#define GF_WET(MX_, TX_, NT_)                                                                   \
+    (def ? (k_wet<MX_, MX_, TX_, TX_, NT_, true><<<grid, NT_, 0, st>>>(da), 0)                 \
+         : (k_wet<MX_, MX_, TX_, TX_, NT_, false><<<grid, NT_, 0, st>>>(da), 0))
     switch (variant_key(mx, variant)) {
-    case 8080128: k_wet<8, 8, 8, 8, 128><<<grid, 128, 0, st>>>(da); break;
-    case 16160384: k_wet<16, 16, 16, 16, 384><<<grid, 384, 0, st>>>(da); break;
-    case 16160192: k_wet<16, 16, 16, 16, 192><<<grid, 192, 0, st>>>(da); break;
-    case 32320640: k_wet<32, 32, 32, 32, 640><<<grid, 640, 0, st>>>(da); break;
-    case 32160384: k_wet<32, 32, 16, 16, 384><<<grid, 384, 0, st>>>(da); break;
-    case 64320640: k_wet<64, 64, 32, 32, 640><<<grid, 640, 0, st>>>(da); break;
-    case 64160384: k_wet<64, 64, 16, 16, 384><<<grid, 384, 0, st>>>(da); break;
+    case 8080128: GF_WET(8, 8, 128); break;
+    case 16160384: GF_WET(16, 16, 384); break;
+    case 16160192: GF_WET(16, 16, 192); break;
+    case 32320640: GF_WET(32, 32, 640); break;
+    case 32160384: GF_WET(32, 16, 384); break;
+    case 64320640: GF_WET(64, 32, 640); break;
+    case 64160384: GF_WET(64, 16, 384); break;
     default: return hipErrorInvalidValue;
     }
+#undef GF_WET
     return hipGetLastError();
 }
 
