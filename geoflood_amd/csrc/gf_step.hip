@@ -1229,8 +1229,12 @@ static const StepVariant kVariants[] = {
     {16, 16, 16, 16, 192},
     {32, 32, 32, 32, 640},
     {32, 32, 16, 16, 384},
+    {32, 32, 32, 32, 768},
+    {32, 32, 32, 32, 704},
+    {32, 32, 32, 32, 1024},
     {64, 64, 32, 32, 640},
     {64, 64, 16, 16, 384},
+    {64, 64, 32, 32, 768},
 };
 
 int num_variants(int mx, int my)
@@ -1274,6 +1278,10 @@ int wet_occupancy(int mx, int my, int variant)
     case 16160192: return occupancy_of(k_wet<16, 16, 16, 16, 192, true>, 192);
     case 32320640: return occupancy_of(k_wet<32, 32, 32, 32, 640, true>, 640);
     case 32160384: return occupancy_of(k_wet<32, 32, 16, 16, 384, true>, 384);
+    case 32320768: return occupancy_of(k_wet<32, 32, 32, 32, 768, true>, 768);
+    case 32320704: return occupancy_of(k_wet<32, 32, 32, 32, 704, true>, 704);
+    case 32321024: return occupancy_of(k_wet<32, 32, 32, 32, 1024, true>, 1024);
+    case 64320768: return occupancy_of(k_wet<64, 64, 32, 32, 768, true>, 768);
     case 64320640: return occupancy_of(k_wet<64, 64, 32, 32, 640, true>, 640);
     case 64160384: return occupancy_of(k_wet<64, 64, 16, 16, 384, true>, 384);
     default: return 1;
@@ -1297,6 +1305,10 @@ hipError_t launch_step(int mx, int my, int variant, const StepArgs& a, hipStream
     case 16160192: GF_STEP(16, 16, 192); break;
     case 32320640: GF_STEP(32, 32, 640); break;
     case 32160384: GF_STEP(32, 16, 384); break;
+    case 32320768: GF_STEP(32, 32, 768); break;
+    case 32320704: GF_STEP(32, 32, 704); break;
+    case 32321024: GF_STEP(32, 32, 1024); break;
+    case 64320768: GF_STEP(64, 32, 768); break;
     case 64320640: GF_STEP(64, 32, 640); break;
     case 64160384: GF_STEP(64, 16, 384); break;
     default: return hipErrorInvalidValue;
@@ -1350,6 +1362,10 @@ hipError_t launch_wet(int mx, int my, int variant, bool def, const BatchArgs* da
     case 16160192: GF_WET(16, 16, 192); break;
     case 32320640: GF_WET(32, 32, 640); break;
     case 32160384: GF_WET(32, 16, 384); break;
+    case 32320768: GF_WET(32, 32, 768); break;
+    case 32320704: GF_WET(32, 32, 704); break;
+    case 32321024: GF_WET(32, 32, 1024); break;
+    case 64320768: GF_WET(64, 32, 768); break;
     case 64320640: GF_WET(64, 32, 640); break;
     case 64160384: GF_WET(64, 16, 384); break;
     default: return hipErrorInvalidValue;
