@@ -48,12 +48,10 @@ struct Lib {
     hipStream_t stream = nullptr;
     gf::Params P{};
     gf_params user{};
-    hipStream_t stream2 = nullptr;    // dry-patch branch of the batched step
     int num_cu = 256;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // timing-free fork/join of the two streams
     std::vector<hipEvent_t> ev_step;  // pairs around step launches
-    std::vector<hipEvent_t> ev_wet, ev_dry;  // pairs around wet / dry launches (non-graph runs)
+    std::vector<hipEvent_t> ev_tune;  // pair around the autotuner's candidate launches
     double last_wet_ms = 0.0, last_dry_ms = 0.0;
     unsigned params_gen = 1;  // bumped by gf_set_params: pools re-sync their batch arguments
     double last_total_ms = 0.0, last_step_ms = 0.0;
@@ -98,6 +96,7 @@ struct gf_pool {
     int wet_grid = 0;                      // workgroups of the persistent step kernel (0: not computed)
     int* inflow_list = nullptr;            // [capacity] patches with a hydrograph side
     int* inflow_n = nullptr;               // [1] entries of inflow_list
+    bool has_inflow = false;               // inflow_list is not empty (the graph launches k_inflow)
     std::vector<double*> hyd_mem;          // per scenario slot: device hydrograph table
     std::string inflow_error;              // set when a scenario's inflow cannot run
     void* tag_buf = nullptr;               // criteria, patch list and tags of the tagging calls
@@ -190,11 +189,8 @@ int gf_init(int device)
     }
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
-    HIPCHK(hipStreamCreateWithFlags(&g.stream2, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&g.ev_begin));
     HIPCHK(hipEventCreate(&g.ev_end));
-    HIPCHK(hipEventCreateWithFlags(&g.ev_fork, hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&g.ev_join, hipEventDisableTiming));
     HIPCHK(hipDeviceGetAttribute(&g.num_cu, hipDeviceAttributeMultiprocessorCount, device));
     HIPCHK(hipMalloc(&g.d_max, sizeof(double)));
     HIPCHK(hipHostMalloc(&g.h_max, sizeof(double), hipHostMallocDefault));
@@ -214,14 +210,10 @@ int gf_finalize(void)
     g.h_stage = nullptr;
     g.h_stage_bytes = 0;
     for (auto e : g.ev_step) hipEventDestroy(e);
-    for (auto e : g.ev_wet) hipEventDestroy(e);
-    for (auto e : g.ev_dry) hipEventDestroy(e);
+    for (auto e : g.ev_tune) hipEventDestroy(e);
     g.ev_step.clear();
     hipEventDestroy(g.ev_begin);
     hipEventDestroy(g.ev_end);
-    hipEventDestroy(g.ev_fork);
-    hipEventDestroy(g.ev_join);
-    hipStreamDestroy(g.stream2);
     hipFree(g.d_max);
     hipHostFree(g.h_max);
     hipStreamDestroy(g.stream);
@@ -312,6 +304,10 @@ static int rebuild_inflow(gf_pool* p)
             p->inflow_error = "scenario " + std::to_string(s) + " has inflow sides but no hydrograph";
     }
     const int n = (int)list.size();
+    if ((n > 0) != p->has_inflow) {
+        p->has_inflow = n > 0;
+        p->graph_dirty = true;
+    }
     if (n) HIPCHK(hipMemcpyAsync(p->inflow_list, list.data(), sizeof(int) * n, hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipMemcpyAsync(p->inflow_n, &n, sizeof(int), hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
@@ -581,7 +577,8 @@ static gf::InflowArgs inflow_args(gf_pool* p, int first, int last, int buf, doub
     return a;
 }
 
-constexpr int kInflowGrid = 128;  // workgroups of k_inflow (grid-stride over the inflow list)
+// Workgroups of k_inflow (grid-stride over the inflow list): a few per CU.
+static int inflow_grid() { return 4 * (g.num_cu > 0 ? g.num_cu : 64); }
 
 int gf_pool_ghost_fill(gf_pool* p, int first, int n, int buf, double t)
 {
@@ -591,7 +588,7 @@ int gf_pool_ghost_fill(gf_pool* p, int first, int n, int buf, double t)
     if (!p->inflow_error.empty()) return fail(GF_EINVAL, "%s", p->inflow_error.c_str());
     gf::GhostArgs a{p->desc, first, n, p->q[0], p->q[1], nullptr, buf};
     HIPCHK(gf::launch_ghost(p->mx, p->my, a, g.stream));
-    HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, first, first + n, buf, t), kInflowGrid, g.stream));
+    HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, first, first + n, buf, t), inflow_grid(), g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
 }
@@ -903,25 +900,18 @@ static int wet_grid(gf_pool* p)
     return p->wet_grid;
 }
 
-// One attempt of every active scenario: classify -> (active patches on the
-// main stream || dry patches on the second stream) -> controller.  With
-// k >= 0, timing events bracket the step phase and each branch.
+// One attempt of every active scenario: (hydrograph ghosts) -> classify ->
+// one persistent kernel over the active tiles and then the dry patches ->
+// controller, all on one stream.  With k >= 0, timing events bracket the
+// step kernel.
 static int enqueue_attempt(gf_pool* p, int k)
 {
     const gf::BatchArgs a = host_batch_args(p);
-    HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, 0, p->capacity, -1, 0.0), kInflowGrid, g.stream));
+    if (p->has_inflow)
+        HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, 0, p->capacity, -1, 0.0), inflow_grid(), g.stream));
     HIPCHK(gf::launch_classify(a, g.stream));
     if (k >= 0) HIPCHK(hipEventRecord(g.ev_step[2 * k], g.stream));
-    HIPCHK(hipEventRecord(g.ev_fork, g.stream));
-    HIPCHK(hipStreamWaitEvent(g.stream2, g.ev_fork, 0));
-    if (k >= 0) HIPCHK(hipEventRecord(g.ev_dry[2 * k], g.stream2));
-    HIPCHK(gf::launch_dry(p->mx, p->my, a, 4 * g.num_cu, g.stream2));
-    if (k >= 0) HIPCHK(hipEventRecord(g.ev_dry[2 * k + 1], g.stream2));
-    if (k >= 0) HIPCHK(hipEventRecord(g.ev_wet[2 * k], g.stream));
-    HIPCHK(gf::launch_wet(p->mx, p->my, p->variant, gf::default_method(g.P), p->d_args, wet_grid(p), g.stream));
-    if (k >= 0) HIPCHK(hipEventRecord(g.ev_wet[2 * k + 1], g.stream));
-    HIPCHK(hipEventRecord(g.ev_join, g.stream2));
-    HIPCHK(hipStreamWaitEvent(g.stream, g.ev_join, 0));
+    HIPCHK(gf::launch_all(p->mx, p->my, p->variant, gf::default_method(g.P), p->d_args, wet_grid(p), g.stream));
     if (k >= 0) HIPCHK(hipEventRecord(g.ev_step[2 * k + 1], g.stream));
     HIPCHK(gf::launch_batch_control(a, g.stream));
     return GF_OK;
@@ -974,8 +964,6 @@ int gf_pool_run(gf_pool* p, int substeps, int use_graph, int* any_active)
         g.last_launches = 0;
     } else {
         if (int r = ensure_step_events(std::max(1, substeps))) return r;
-        if (int r = ensure_pair_events(g.ev_wet, std::max(1, substeps))) return r;
-        if (int r = ensure_pair_events(g.ev_dry, std::max(1, substeps))) return r;
         HIPCHK(hipEventRecord(g.ev_begin, g.stream));
         for (int k = 0; k < substeps; k++)
             if (int r = enqueue_attempt(p, k)) return r;
@@ -988,11 +976,9 @@ int gf_pool_run(gf_pool* p, int substeps, int use_graph, int* any_active)
         for (int k = 0; k < substeps; k++) {
             HIPCHK(hipEventElapsedTime(&ms, g.ev_step[2 * k], g.ev_step[2 * k + 1]));
             g.last_step_ms += ms;
-            HIPCHK(hipEventElapsedTime(&ms, g.ev_wet[2 * k], g.ev_wet[2 * k + 1]));
-            g.last_wet_ms += ms;
-            HIPCHK(hipEventElapsedTime(&ms, g.ev_dry[2 * k], g.ev_dry[2 * k + 1]));
-            g.last_dry_ms += ms;
         }
+        g.last_wet_ms = g.last_step_ms;  // active and dry patches share the one step kernel
+        g.last_dry_ms = 0.0;
         g.last_launches = substeps;
     }
     return any_active_flag(p, any_active);
@@ -1026,7 +1012,7 @@ int gf_pool_run_plain(gf_pool* p, int substeps, int* any_active)
             HIPCHK(gf::launch_ghost(p->mx, p->my, ga, g.stream));
             HIPCHK(gf::launch_inflow(p->mx, p->my,
                                      inflow_args(p, hc[s].first_patch, hc[s].first_patch + hc[s].npatch, hc[s].cur, hc[s].t),
-                                     kInflowGrid, g.stream));
+                                     inflow_grid(), g.stream));
             for (int k = hc[s].first_patch; k < hc[s].first_patch + hc[s].npatch; k++, e++) {
                 HIPCHK(hipEventRecord(g.ev_step[2 * e], g.stream));
                 HIPCHK(gf::launch_step(p->mx, p->my, p->variant,
@@ -1297,9 +1283,9 @@ int gf_pool_autotune(gf_pool* p, int reps, int* best, double* ms)
     // The active list of the current states (what the next attempt steps).
     // Each candidate steps it from the accepted buffer into the other one,
     // which the next attempt overwrites: no scenario advances.
-    HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, 0, p->capacity, -1, 0.0), kInflowGrid, g.stream));
+    HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, 0, p->capacity, -1, 0.0), inflow_grid(), g.stream));
     HIPCHK(gf::launch_classify(a, g.stream));
-    if (int r = ensure_pair_events(g.ev_wet, 1)) return r;
+    if (int r = ensure_pair_events(g.ev_tune, 1)) return r;
     const int saved = p->variant;
     double best_ms = 0.0;
     int best_v = saved;
@@ -1309,12 +1295,12 @@ int gf_pool_autotune(gf_pool* p, int reps, int* best, double* ms)
         double tot = 0.0;
         for (int r = 0; r <= reps; r++) {  // the first run warms up
             HIPCHK(hipMemsetAsync(p->counters + gf::NEXT_WET, 0, sizeof(int), g.stream));
-            HIPCHK(hipEventRecord(g.ev_wet[0], g.stream));
-            HIPCHK(gf::launch_wet(p->mx, p->my, v, gf::default_method(g.P), p->d_args, wet_grid(p), g.stream));
-            HIPCHK(hipEventRecord(g.ev_wet[1], g.stream));
-            HIPCHK(hipEventSynchronize(g.ev_wet[1]));
+            HIPCHK(hipEventRecord(g.ev_tune[0], g.stream));
+            HIPCHK(gf::launch_all(p->mx, p->my, v, gf::default_method(g.P), p->d_args, wet_grid(p), g.stream));
+            HIPCHK(hipEventRecord(g.ev_tune[1], g.stream));
+            HIPCHK(hipEventSynchronize(g.ev_tune[1]));
             float t = 0.f;
-            HIPCHK(hipEventElapsedTime(&t, g.ev_wet[0], g.ev_wet[1]));
+            HIPCHK(hipEventElapsedTime(&t, g.ev_tune[0], g.ev_tune[1]));
             if (r > 0) tot += t;
         }
         const double avg = tot / reps;

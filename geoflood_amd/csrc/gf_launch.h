@@ -120,8 +120,8 @@ hipError_t launch_classify(const BatchArgs& a, hipStream_t st);
 // def: the method is the default one (order 2, transverse 2, MC limiter);
 // its kernels have the method fixed at compile time.
 bool default_method(const Params& P);
-hipError_t launch_wet(int mx, int my, int variant, bool def, const BatchArgs* da, int grid, hipStream_t st);
-hipError_t launch_dry(int mx, int my, const BatchArgs& a, int grid, hipStream_t st);
+// Active tiles then dry patches from one persistent kernel (see k_all).
+hipError_t launch_all(int mx, int my, int variant, bool def, const BatchArgs* da, int grid, hipStream_t st);
 hipError_t launch_batch_control(const BatchArgs& a, hipStream_t st);
 hipError_t launch_hmax(int mx, int my, const double* q, double* hmax, int first, int n, hipStream_t st);
 hipError_t launch_step(int mx, int my, int variant, const StepArgs& a, hipStream_t st);

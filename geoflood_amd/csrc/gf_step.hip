@@ -770,24 +770,100 @@ __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, int next,
     __syncthreads();
 }
 
+// Dry patches per work item of the unified kernel.
+constexpr int DRY_PER_ITEM = 2;
+
+// One dry work item of the unified kernel: DRY_PER_ITEM consecutive entries of
+// the dry list.  A dry patch's whole tile is at most dry_tolerance, so every
+// wave and transverse split vanishes and the step is b4step2 + src2 on the
+// interior: a cell with h < dry_tolerance becomes (max(h,0), 0, 0) whatever
+// its momentum (src2 then keeps zero momentum at zero), so only h is read for
+// it; hu, hv and b are read only for h == dry_tolerance.
+template <int MX, int MY, int NT>
+__device__ __forceinline__ void dry_item(const BatchArgs* pa, int k, int next, int* next_slot)
+{
+    using G = Geo<MX, MY>;
+    constexpr int NP2 = G::NP2, CELLS = MX * MY;
+    constexpr int PER = (DRY_PER_ITEM * CELLS + NT - 1) / NT;
+    __shared__ int d_patch[DRY_PER_ITEM], d_cur[DRY_PER_ITEM];
+    __shared__ double d_dt[DRY_PER_ITEM], d_n[DRY_PER_ITEM];
+    const BatchArgs& a = *pa;
+    const int first = k * DRY_PER_ITEM;
+    if (threadIdx.x < DRY_PER_ITEM) {
+        const int j = first + threadIdx.x;
+        int pt = -1;
+        if (j < a.counters[CNT_DRY]) {
+            pt = a.list_dry[j];
+            const ScenCtrl* c = a.ctrl + a.desc[pt].scen;
+            d_cur[threadIdx.x] = c->cur;
+            d_dt[threadIdx.x] = c->dt;
+            d_n[threadIdx.x] = c->manning_n;
+            a.hmax[(size_t)(c->cur ^ 1) * a.capacity + pt] = 0.0;
+        }
+        d_patch[threadIdx.x] = pt;
+    }
+    __syncthreads();
+    const double tol = a.P.dry_tolerance;
+    int q[PER], kk[PER];
+    double h0[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int e = threadIdx.x + NT * r;
+        q[r] = (e < DRY_PER_ITEM * CELLS) ? e / CELLS : DRY_PER_ITEM;
+        if (q[r] < DRY_PER_ITEM && d_patch[q[r]] < 0) q[r] = DRY_PER_ITEM;
+        const int c = e % CELLS;
+        kk[r] = G::ti(c % MX + 1, c / MX + 1);
+    }
+#pragma unroll
+    for (int r = 0; r < PER; r++)
+        if (q[r] < DRY_PER_ITEM)
+            h0[r] = (d_cur[q[r]] ? a.q1 : a.q0)[(size_t)d_patch[q[r]] * 3 * NP2 + kk[r]];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        if (q[r] >= DRY_PER_ITEM) continue;
+        const int pt = d_patch[q[r]], cur = d_cur[q[r]];
+        const double* src = (cur ? a.q1 : a.q0) + (size_t)pt * 3 * NP2;
+        double* dst = (cur ? a.q0 : a.q1) + (size_t)pt * 3 * NP2;
+        double h = h0[r], hu = 0.0, hv = 0.0;
+        if (h < tol) {
+            h = fmaxf64(h, 0.0);
+        } else {
+            Params P = a.P;
+            P.manning_n = d_n[q[r]];
+            hu = src[NP2 + kk[r]];
+            hv = src[2 * NP2 + kk[r]];
+            if (P.src_term > 0) src2_cell(h, hu, hv, a.aux[(size_t)pt * NP2 + kk[r]], d_dt[q[r]], P);
+        }
+        dst[kk[r]] = h;
+        dst[NP2 + kk[r]] = hu;
+        dst[2 * NP2 + kk[r]] = hv;
+    }
+    if (threadIdx.x == 0) *next_slot = next;
+    __syncthreads();
+}
+
+// Unified persistent step of an attempt: the active patches' tiles first
+// (wet_item), then the dry patches in items of DRY_PER_ITEM (dry_item), from
+// one atomic cursor.  The streaming dry items fill the tail that the
+// compute-bound active items leave when the list runs out.
 template <int MX, int MY, int TX, int TY, int NT, bool DEF>
-__global__ void __launch_bounds__(NT) k_wet(const BatchArgs* pa)
+__global__ void __launch_bounds__(NT) k_all(const BatchArgs* pa)
 {
     using TL = Tiling<MX, MY, TX, TY>;
-    // The next entry is taken from the counter while the current one is
-    // stepped (thread 0 publishes it just before the entry's last barrier);
-    // the two slots alternate, so a slot is rewritten only after every thread
-    // has read it (the barriers inside wet_item separate the two).
     __shared__ int item[2];
     if (threadIdx.x == 0) item[0] = atomicAdd(&pa->counters[NEXT_WET], 1);
     __syncthreads();
-    const int limit = pa->counters[CNT_WET] * TL::NTILES;
+    const int nwet = pa->counters[CNT_WET] * TL::NTILES;
+    const int limit = nwet + (pa->counters[CNT_DRY] + DRY_PER_ITEM - 1) / DRY_PER_ITEM;
     for (int it = 0;; it++) {
         const int idx = item[it & 1];
         if (idx >= limit) break;
         int next = 0;
         if (threadIdx.x == 0) next = atomicAdd(&pa->counters[NEXT_WET], 1);
-        wet_item<MX, MY, TX, TY, NT, DEF>(pa, idx, next, &item[(it + 1) & 1]);
+        if (idx < nwet)
+            wet_item<MX, MY, TX, TY, NT, DEF>(pa, idx, next, &item[(it + 1) & 1]);
+        else
+            dry_item<MX, MY, NT>(pa, idx - nwet, next, &item[(it + 1) & 1]);
     }
 }
 
@@ -1057,57 +1133,6 @@ __global__ void __launch_bounds__(256) k_classify(BatchArgs a)
     if (kind == 0) a.list_dry[bd + __popcll(md & below)] = p;
 }
 
-// Dry patches: every cell of the tile has h <= dry_tolerance, so all waves,
-// fluctuations and transverse splits vanish and the step is b4step2 + src2
-// on the interior.  A cell with h < dry_tolerance becomes (max(h,0), 0, 0)
-// whatever its momentum (src2 then leaves zero momentum at zero), so only h
-// is read for it; hu, hv and b are read only for h == dry_tolerance.
-template <int MX, int MY>
-__global__ void __launch_bounds__(256) k_dry(BatchArgs a)
-{
-    using G = Geo<MX, MY>;
-    constexpr int NP2 = G::NP2;
-    constexpr int PER = (MX * MY + 255) / 256;
-    const int count = a.counters[CNT_DRY];
-    const double tol = a.P.dry_tolerance;
-    for (int k = blockIdx.x; k < count; k += gridDim.x) {
-        const int patch = a.list_dry[k];
-        const PatchDesc d = a.desc[patch];
-        const ScenCtrl* c = a.ctrl + d.scen;
-        const int cur = c->cur;
-        const double dt = c->dt;
-        Params P = a.P;
-        P.manning_n = c->manning_n;
-        const double* src = (cur ? a.q1 : a.q0) + (size_t)patch * 3 * NP2;
-        double* dst = (cur ? a.q0 : a.q1) + (size_t)patch * 3 * NP2;
-        const double* auxp = a.aux + (size_t)patch * NP2;
-        double hv_[PER];
-        int kk[PER];
-#pragma unroll
-        for (int r = 0; r < PER; r++) {
-            const int e = threadIdx.x + 256 * r;
-            kk[r] = (e < MX * MY) ? G::ti(e % MX + 1, e / MX + 1) : -1;
-            hv_[r] = (kk[r] >= 0) ? src[kk[r]] : 0.0;
-        }
-#pragma unroll
-        for (int r = 0; r < PER; r++) {
-            if (kk[r] < 0) continue;
-            double h = hv_[r], hu = 0.0, hv = 0.0;
-            if (h < tol) {
-                h = fmaxf64(h, 0.0);
-            } else {
-                hu = src[NP2 + kk[r]];
-                hv = src[2 * NP2 + kk[r]];
-                if (P.src_term > 0) src2_cell(h, hu, hv, auxp[kk[r]], dt, P);
-            }
-            dst[kk[r]] = h;
-            dst[NP2 + kk[r]] = hu;
-            dst[2 * NP2 + kk[r]] = hv;
-        }
-        if (threadIdx.x == 0) a.hmax[(size_t)(cur ^ 1) * a.capacity + patch] = 0.0;
-    }
-}
-
 // Controller of the batched path: the per-scenario CFL maximum was folded by
 // atomicMax in the step kernels; apply the dt rule (fc2d_geoclaw_run.c:
 // 529-565), clear the maxima and the list counters for the next attempt.
@@ -1273,17 +1298,17 @@ int wet_occupancy(int mx, int my, int variant)
 {
     if (mx != my) return 1;
     switch (variant_key(mx, variant)) {
-    case 8080128: return occupancy_of(k_wet<8, 8, 8, 8, 128, true>, 128);
-    case 16160384: return occupancy_of(k_wet<16, 16, 16, 16, 384, true>, 384);
-    case 16160192: return occupancy_of(k_wet<16, 16, 16, 16, 192, true>, 192);
-    case 32320640: return occupancy_of(k_wet<32, 32, 32, 32, 640, true>, 640);
-    case 32160384: return occupancy_of(k_wet<32, 32, 16, 16, 384, true>, 384);
-    case 32320768: return occupancy_of(k_wet<32, 32, 32, 32, 768, true>, 768);
-    case 32320704: return occupancy_of(k_wet<32, 32, 32, 32, 704, true>, 704);
-    case 32321024: return occupancy_of(k_wet<32, 32, 32, 32, 1024, true>, 1024);
-    case 64320768: return occupancy_of(k_wet<64, 64, 32, 32, 768, true>, 768);
-    case 64320640: return occupancy_of(k_wet<64, 64, 32, 32, 640, true>, 640);
-    case 64160384: return occupancy_of(k_wet<64, 64, 16, 16, 384, true>, 384);
+    case 8080128: return occupancy_of(k_all<8, 8, 8, 8, 128, true>, 128);
+    case 16160384: return occupancy_of(k_all<16, 16, 16, 16, 384, true>, 384);
+    case 16160192: return occupancy_of(k_all<16, 16, 16, 16, 192, true>, 192);
+    case 32320640: return occupancy_of(k_all<32, 32, 32, 32, 640, true>, 640);
+    case 32160384: return occupancy_of(k_all<32, 32, 16, 16, 384, true>, 384);
+    case 32320768: return occupancy_of(k_all<32, 32, 32, 32, 768, true>, 768);
+    case 32320704: return occupancy_of(k_all<32, 32, 32, 32, 704, true>, 704);
+    case 32321024: return occupancy_of(k_all<32, 32, 32, 32, 1024, true>, 1024);
+    case 64320768: return occupancy_of(k_all<64, 64, 32, 32, 768, true>, 768);
+    case 64320640: return occupancy_of(k_all<64, 64, 32, 32, 640, true>, 640);
+    case 64160384: return occupancy_of(k_all<64, 64, 16, 16, 384, true>, 384);
     default: return 1;
     }
 }
@@ -1350,41 +1375,30 @@ hipError_t launch_classify(const BatchArgs& a, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_wet(int mx, int my, int variant, bool def, const BatchArgs* da, int grid, hipStream_t st)
+hipError_t launch_all(int mx, int my, int variant, bool def, const BatchArgs* da, int grid, hipStream_t st)
 {
     if (mx != my) return hipErrorInvalidValue;
-#define GF_WET(MX_, TX_, NT_)                                                                   \
-    (def ? (k_wet<MX_, MX_, TX_, TX_, NT_, true><<<grid, NT_, 0, st>>>(da), 0)                 \
-         : (k_wet<MX_, MX_, TX_, TX_, NT_, false><<<grid, NT_, 0, st>>>(da), 0))
+#define GF_ALL(MX_, TX_, NT_)                                                                   \
+    (def ? (k_all<MX_, MX_, TX_, TX_, NT_, true><<<grid, NT_, 0, st>>>(da), 0)                 \
+         : (k_all<MX_, MX_, TX_, TX_, NT_, false><<<grid, NT_, 0, st>>>(da), 0))
     switch (variant_key(mx, variant)) {
-    case 8080128: GF_WET(8, 8, 128); break;
-    case 16160384: GF_WET(16, 16, 384); break;
-    case 16160192: GF_WET(16, 16, 192); break;
-    case 32320640: GF_WET(32, 32, 640); break;
-    case 32160384: GF_WET(32, 16, 384); break;
-    case 32320768: GF_WET(32, 32, 768); break;
-    case 32320704: GF_WET(32, 32, 704); break;
-    case 32321024: GF_WET(32, 32, 1024); break;
-    case 64320768: GF_WET(64, 32, 768); break;
-    case 64320640: GF_WET(64, 32, 640); break;
-    case 64160384: GF_WET(64, 16, 384); break;
+    case 8080128: GF_ALL(8, 8, 128); break;
+    case 16160384: GF_ALL(16, 16, 384); break;
+    case 16160192: GF_ALL(16, 16, 192); break;
+    case 32320640: GF_ALL(32, 32, 640); break;
+    case 32160384: GF_ALL(32, 16, 384); break;
+    case 32320768: GF_ALL(32, 32, 768); break;
+    case 32320704: GF_ALL(32, 32, 704); break;
+    case 32321024: GF_ALL(32, 32, 1024); break;
+    case 64320640: GF_ALL(64, 32, 640); break;
+    case 64160384: GF_ALL(64, 16, 384); break;
+    case 64320768: GF_ALL(64, 32, 768); break;
     default: return hipErrorInvalidValue;
     }
-#undef GF_WET
+#undef GF_ALL
     return hipGetLastError();
 }
 
-hipError_t launch_dry(int mx, int my, const BatchArgs& a, int grid, hipStream_t st)
-{
-    switch (mx * 1000 + my) {
-    case 32032: k_dry<32, 32><<<grid, 256, 0, st>>>(a); break;
-    case 16016: k_dry<16, 16><<<grid, 256, 0, st>>>(a); break;
-    case 8008: k_dry<8, 8><<<grid, 256, 0, st>>>(a); break;
-    case 64064: k_dry<64, 64><<<grid, 256, 0, st>>>(a); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
 
 hipError_t launch_batch_control(const BatchArgs& a, hipStream_t st)
 {

@@ -37,6 +37,7 @@ T_END = 1800.0       # simulated seconds per scenario
 T_START = 10.0       # start after the hydrograph has left zero (Q(0) = 0 gives CFL 0)
 DT0 = 1.0
 STRIP = 100.0        # inflow channel width (m), centred on the west edge
+RETUNE_FACTOR = 2.0  # re-run the autotuner when the live patches change by this factor
 
 
 @dataclass
@@ -145,6 +146,7 @@ class Engine:
         self.cell_updates = 0
         self.prep = ThreadPoolExecutor(max_workers=2)  # host-side DEM/layout of queued requests
         self.ready: dict[int, object] = {}
+        self.tuned_at = 0  # live patches when the step variant was last chosen
 
     def close(self):
         self.prep.shutdown(wait=True)
@@ -243,6 +245,10 @@ class Engine:
                 elif pending:
                     self.attempts += 1  # idle tick of the deterministic clock
                 continue
+            live = sum(lv.req.npatch for lv in self.live.values())
+            if live > RETUNE_FACTOR * self.tuned_at or RETUNE_FACTOR * live < self.tuned_at:
+                self.pool.autotune(reps=1)  # choose the step variant for this population
+                self.tuned_at = live
             self.pool.run(self.substeps, graph=True)
             self.attempts += self.substeps
             done = self.poll(now())
