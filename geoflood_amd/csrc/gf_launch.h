@@ -55,6 +55,9 @@ struct InflowArgs {
     Params P;
 };
 
+// Dry-list entries carry the patch's current buffer: patch | cur << 30.
+constexpr int DRY_CUR_SHIFT = 30;
+
 // Device counters of the batched path (BatchArgs::counters).
 enum { CNT_WET = 0, CNT_DRY = 1, NEXT_WET = 2, NCOUNTERS = 4 };
 

@@ -771,7 +771,7 @@ __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, int next,
 }
 
 // Dry patches per work item of the unified kernel.
-constexpr int DRY_PER_ITEM = 2;
+constexpr int DRY_PER_ITEM = 6;
 
 // One dry work item of the unified kernel: DRY_PER_ITEM consecutive entries of
 // the dry list.  A dry patch's whole tile is at most dry_tolerance, so every
@@ -785,58 +785,47 @@ __device__ __forceinline__ void dry_item(const BatchArgs* pa, int k, int next, i
     using G = Geo<MX, MY>;
     constexpr int NP2 = G::NP2, CELLS = MX * MY;
     constexpr int PER = (DRY_PER_ITEM * CELLS + NT - 1) / NT;
-    __shared__ int d_patch[DRY_PER_ITEM], d_cur[DRY_PER_ITEM];
-    __shared__ double d_dt[DRY_PER_ITEM], d_n[DRY_PER_ITEM];
+    constexpr int PMASK = (1 << DRY_CUR_SHIFT) - 1;
     const BatchArgs& a = *pa;
     const int first = k * DRY_PER_ITEM;
-    if (threadIdx.x < DRY_PER_ITEM) {
-        const int j = first + threadIdx.x;
-        int pt = -1;
-        if (j < a.counters[CNT_DRY]) {
-            pt = a.list_dry[j];
-            const ScenCtrl* c = a.ctrl + a.desc[pt].scen;
-            d_cur[threadIdx.x] = c->cur;
-            d_dt[threadIdx.x] = c->dt;
-            d_n[threadIdx.x] = c->manning_n;
-            a.hmax[(size_t)(c->cur ^ 1) * a.capacity + pt] = 0.0;
-        }
-        d_patch[threadIdx.x] = pt;
-    }
-    __syncthreads();
+    const int count = a.counters[CNT_DRY];
     const double tol = a.P.dry_tolerance;
-    int q[PER], kk[PER];
-    double h0[PER];
+    // entries (patch | cur << DRY_CUR_SHIFT), then h, then the stores: two
+    // dependent memory round trips per item
+    int ent[PER], kk[PER];
 #pragma unroll
     for (int r = 0; r < PER; r++) {
         const int e = threadIdx.x + NT * r;
-        q[r] = (e < DRY_PER_ITEM * CELLS) ? e / CELLS : DRY_PER_ITEM;
-        if (q[r] < DRY_PER_ITEM && d_patch[q[r]] < 0) q[r] = DRY_PER_ITEM;
+        const int j = first + e / CELLS;
+        ent[r] = (e < DRY_PER_ITEM * CELLS && j < count) ? a.list_dry[j] : -1;
         const int c = e % CELLS;
         kk[r] = G::ti(c % MX + 1, c / MX + 1);
     }
+    double h0[PER];
 #pragma unroll
     for (int r = 0; r < PER; r++)
-        if (q[r] < DRY_PER_ITEM)
-            h0[r] = (d_cur[q[r]] ? a.q1 : a.q0)[(size_t)d_patch[q[r]] * 3 * NP2 + kk[r]];
+        if (ent[r] >= 0) h0[r] = ((ent[r] >> DRY_CUR_SHIFT) ? a.q1 : a.q0)[(size_t)(ent[r] & PMASK) * 3 * NP2 + kk[r]];
 #pragma unroll
     for (int r = 0; r < PER; r++) {
-        if (q[r] >= DRY_PER_ITEM) continue;
-        const int pt = d_patch[q[r]], cur = d_cur[q[r]];
+        if (ent[r] < 0) continue;
+        const int pt = ent[r] & PMASK, cur = ent[r] >> DRY_CUR_SHIFT;
         const double* src = (cur ? a.q1 : a.q0) + (size_t)pt * 3 * NP2;
         double* dst = (cur ? a.q0 : a.q1) + (size_t)pt * 3 * NP2;
         double h = h0[r], hu = 0.0, hv = 0.0;
         if (h < tol) {
             h = fmaxf64(h, 0.0);
         } else {
+            const ScenCtrl* c = a.ctrl + a.desc[pt].scen;
             Params P = a.P;
-            P.manning_n = d_n[q[r]];
+            P.manning_n = c->manning_n;
             hu = src[NP2 + kk[r]];
             hv = src[2 * NP2 + kk[r]];
-            if (P.src_term > 0) src2_cell(h, hu, hv, a.aux[(size_t)pt * NP2 + kk[r]], d_dt[q[r]], P);
+            if (P.src_term > 0) src2_cell(h, hu, hv, a.aux[(size_t)pt * NP2 + kk[r]], c->dt, P);
         }
         dst[kk[r]] = h;
         dst[NP2 + kk[r]] = hu;
         dst[2 * NP2 + kk[r]] = hv;
+        if ((threadIdx.x + NT * r) % CELLS == 0) a.hmax[(size_t)(cur ^ 1) * a.capacity + pt] = 0.0;
     }
     if (threadIdx.x == 0) *next_slot = next;
     __syncthreads();
@@ -1100,11 +1089,13 @@ __global__ void __launch_bounds__(256) k_classify(BatchArgs a)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     int kind = -1;  // -1: not stepping, 0: dry, 1: active
+    int cur = 0;
     if (p < a.n) {
         const PatchDesc d = a.desc[p];
         if (d.scen >= 0) {
             const ScenCtrl* c = a.ctrl + d.scen;
             if (c->active) {
+                cur = c->cur;
                 const double* hm = a.hmax + (size_t)c->cur * a.capacity;
                 const double tol = a.P.dry_tolerance;
                 bool w = hm[p] > tol;
@@ -1130,7 +1121,7 @@ __global__ void __launch_bounds__(256) k_classify(BatchArgs a)
     bd = __shfl(bd, 0, 64);
     const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
     if (kind == 1) a.list_wet[bw + __popcll(mw & below)] = p;
-    if (kind == 0) a.list_dry[bd + __popcll(md & below)] = p;
+    if (kind == 0) a.list_dry[bd + __popcll(md & below)] = p | (cur << DRY_CUR_SHIFT);
 }
 
 // Controller of the batched path: the per-scenario CFL maximum was folded by
