@@ -1294,7 +1294,7 @@ int gf_pool_autotune(gf_pool* p, int reps, int* best, double* ms)
         p->wet_grid = 0;
         double tot = 0.0;
         for (int r = 0; r <= reps; r++) {  // the first run warms up
-            HIPCHK(hipMemsetAsync(p->counters + gf::NEXT_WET, 0, sizeof(int), g.stream));
+            HIPCHK(hipMemsetAsync(p->counters + gf::NEXT_WET, 0, 2 * sizeof(int), g.stream));  // NEXT_WET, NEXT_DRY
             HIPCHK(hipEventRecord(g.ev_tune[0], g.stream));
             HIPCHK(gf::launch_all(p->mx, p->my, v, gf::default_method(g.P), p->d_args, wet_grid(p), g.stream));
             HIPCHK(hipEventRecord(g.ev_tune[1], g.stream));

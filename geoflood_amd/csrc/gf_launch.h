@@ -59,7 +59,7 @@ struct InflowArgs {
 constexpr int DRY_CUR_SHIFT = 30;
 
 // Device counters of the batched path (BatchArgs::counters).
-enum { CNT_WET = 0, CNT_DRY = 1, NEXT_WET = 2, NCOUNTERS = 4 };
+enum { CNT_WET = 0, CNT_DRY = 1, NEXT_WET = 2, NEXT_DRY = 3, NCOUNTERS = 4 };
 
 // Arguments of the batched (work-list) path: classify -> wet || dry -> control.
 struct BatchArgs {
