@@ -94,9 +94,8 @@ PROTOTYPES = {
     "gf_pool_autotune": (ct.c_int, [ct.c_void_p, ct.c_int, _ip, _dp]),
     "gf_synchronize": (ct.c_int, []),
     "gf_last_timing": (ct.c_int, [_dp, _dp, _ip]),
-    "gf_last_timing_branches": (ct.c_int, [_dp, _dp]),
     "gf_pool_stats": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong),
-                                 ct.POINTER(ct.c_longlong)]),
+                                 ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong)]),
     "gf_plugin_create": (ct.c_void_p, [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int]),
     "gf_plugin_destroy": (ct.c_int, [ct.c_void_p]),
     "gf_plugin_single_step_update": (ct.c_int, [ct.c_void_p, _dp, _dp, ct.c_double, ct.c_double,
@@ -163,13 +162,6 @@ def set_params(**kw):
     p = make_params(**kw)
     check(load().gf_set_params(ct.byref(p)), "gf_set_params")
     return p
-
-
-def last_branch_timing():
-    w = ct.c_double(0)
-    d = ct.c_double(0)
-    load().gf_last_timing_branches(ct.byref(w), ct.byref(d))
-    return w.value, d.value
 
 
 def last_timing():

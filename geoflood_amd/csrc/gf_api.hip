@@ -52,7 +52,6 @@ struct Lib {
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     std::vector<hipEvent_t> ev_step;  // pairs around step launches
     std::vector<hipEvent_t> ev_tune;  // pair around the autotuner's candidate launches
-    double last_wet_ms = 0.0, last_dry_ms = 0.0;
     unsigned params_gen = 1;  // bumped by gf_set_params: pools re-sync their batch arguments
     double last_total_ms = 0.0, last_step_ms = 0.0;
     int last_launches = 0;
@@ -87,11 +86,12 @@ struct gf_pool {
     int* list_wet = nullptr;
     int* list_dry = nullptr;
     int* counters = nullptr;               // gf::NCOUNTERS
-    long long* stats = nullptr;            // [3]
+    long long* stats = nullptr;            // [4] see gf_pool_stats
     gf::BatchArgs* d_args = nullptr;       // device copy read by the persistent kernel
     unsigned long long* tile_cfl = nullptr; // [capacity] tile fold slots of tiled step variants
     unsigned long long* tile_h = nullptr;
     int* tile_cnt = nullptr;
+    int* dry_age = nullptr;                // [capacity] steady dry patches (gf::DRY_STEADY)
     int variant = 0;                       // step variant (tile shape, threads); see gf_pool_autotune
     int wet_grid = 0;                      // workgroups of the persistent step kernel (0: not computed)
     int* inflow_list = nullptr;            // [capacity] patches with a hydrograph side
@@ -266,9 +266,17 @@ long long gf_pool_bytes(int mx, int my, int mbc, int capacity)
 {
     const long long np2 = (long long)(mx + 2 * mbc) * (my + 2 * mbc);
     const long long per = (2 * 3 * np2 + np2) * 8 + (long long)sizeof(PatchDesc) + 8 /*cfl*/ + 16 /*hmax*/ +
-                          8 /*cflbits*/ + 8 /*lists*/ + 8 /*legacy hmax*/ + sizeof(ScenCtrl);
+                          8 /*cflbits*/ + 8 /*lists*/ + 8 /*legacy hmax*/ + 4 /*dry age*/ + sizeof(ScenCtrl);
     const long long staging = (long long)kChunk * 4 * np2 * 8;
     return per * capacity + staging;
+}
+
+// Forget the steady-dry history of slots [first, first + n) after a write
+// of their state, descriptors or parameters from outside the batched step.
+static int reset_dry_age(gf_pool* p, int first, int n)
+{
+    if (n > 0) HIPCHK(hipMemsetAsync(p->dry_age + first, 0, (size_t)n * sizeof(int), g.stream));
+    return GF_OK;
 }
 
 static void free_hydrographs(gf_pool* p)
@@ -351,11 +359,12 @@ gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
               hipMalloc(&p->list_wet, (size_t)capacity * sizeof(int)) == hipSuccess &&
               hipMalloc(&p->list_dry, (size_t)capacity * sizeof(int)) == hipSuccess &&
               hipMalloc(&p->counters, gf::NCOUNTERS * sizeof(int)) == hipSuccess &&
-              hipMalloc(&p->stats, 3 * sizeof(long long)) == hipSuccess &&
+              hipMalloc(&p->stats, 4 * sizeof(long long)) == hipSuccess &&
               hipMalloc(&p->d_args, sizeof(gf::BatchArgs)) == hipSuccess &&
               hipMalloc(&p->tile_cfl, (size_t)capacity * sizeof(unsigned long long)) == hipSuccess &&
               hipMalloc(&p->tile_h, (size_t)capacity * sizeof(unsigned long long)) == hipSuccess &&
               hipMalloc(&p->tile_cnt, (size_t)capacity * sizeof(int)) == hipSuccess &&
+              hipMalloc(&p->dry_age, (size_t)capacity * sizeof(int)) == hipSuccess &&
               hipMalloc(&p->inflow_list, (size_t)capacity * sizeof(int)) == hipSuccess &&
               hipMalloc(&p->inflow_n, sizeof(int)) == hipSuccess;
     if (ok)
@@ -369,10 +378,11 @@ gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
              hipMemsetAsync(p->hmax2, 0, (size_t)2 * capacity * sizeof(double), g.stream) == hipSuccess &&
              hipMemsetAsync(p->cflbits, 0, (size_t)capacity * sizeof(unsigned long long), g.stream) == hipSuccess &&
              hipMemsetAsync(p->counters, 0, gf::NCOUNTERS * sizeof(int), g.stream) == hipSuccess &&
-             hipMemsetAsync(p->stats, 0, 3 * sizeof(long long), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->stats, 0, 4 * sizeof(long long), g.stream) == hipSuccess &&
              hipMemsetAsync(p->tile_cfl, 0, (size_t)capacity * sizeof(unsigned long long), g.stream) == hipSuccess &&
              hipMemsetAsync(p->tile_h, 0, (size_t)capacity * sizeof(unsigned long long), g.stream) == hipSuccess &&
              hipMemsetAsync(p->tile_cnt, 0, (size_t)capacity * sizeof(int), g.stream) == hipSuccess &&
+             hipMemsetAsync(p->dry_age, 0, (size_t)capacity * sizeof(int), g.stream) == hipSuccess &&
              hipMemsetAsync(p->inflow_list, 0, (size_t)capacity * sizeof(int), g.stream) == hipSuccess &&
              hipMemsetAsync(p->inflow_n, 0, sizeof(int), g.stream) == hipSuccess &&
              hipStreamSynchronize(g.stream) == hipSuccess;
@@ -424,6 +434,7 @@ int gf_pool_destroy(gf_pool* p)
     hipFree(p->tile_cfl);
     hipFree(p->tile_h);
     hipFree(p->tile_cnt);
+    hipFree(p->dry_age);
     hipFree(p->tag_buf);
     hipFree(p->inflow_list);
     hipFree(p->inflow_n);
@@ -465,6 +476,7 @@ int gf_pool_set_patches(gf_pool* p, int n, const double* dx, const double* dy, c
     free_hydrographs(p);
     HIPCHK(hipMemcpyAsync(p->desc, p->h_desc.data(), sizeof(PatchDesc) * p->capacity, hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipMemsetAsync(p->ctrl, 0, sizeof(ScenCtrl) * p->capacity, g.stream));
+    if (int r = reset_dry_age(p, 0, p->capacity)) return r;
     HIPCHK(hipStreamSynchronize(g.stream));
     p->graph_dirty = true;
     p->args_dirty = true;
@@ -498,6 +510,7 @@ int gf_pool_describe(gf_pool* p, int first, int n, const double* dx, const doubl
     }
     if (first + n > p->n) p->n = first + n;
     HIPCHK(hipMemcpyAsync(p->desc + first, p->h_desc.data() + first, sizeof(PatchDesc) * n, hipMemcpyHostToDevice, g.stream));
+    if (int r = reset_dry_age(p, first, n)) return r;
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
 }
@@ -524,6 +537,7 @@ int gf_pool_upload(gf_pool* p, int first, int n, int buf, const double* q_aos, c
     }
     // interior depth maxima of the uploaded state (classification of the first attempt)
     HIPCHK(gf::launch_hmax(p->mx, p->my, p->q[buf], p->hmax2 + (size_t)buf * p->capacity, first, n, g.stream));
+    if (int r = reset_dry_age(p, first, n)) return r;
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
 }
@@ -605,6 +619,7 @@ static int pool_step_impl(gf_pool* p, int first, int n, int src, int dst, double
     const int cur = (src == dst) ? 0 : src;
     const int launches = plain ? n : (n > 0 ? 1 : 0);
     if (int r = ensure_step_events(std::max(1, launches))) return r;
+    if (int r = reset_dry_age(p, first, n)) return r;
     HIPCHK(hipEventRecord(g.ev_begin, g.stream));
     if (plain) {
         for (int k = 0; k < n; k++) {
@@ -687,6 +702,7 @@ int gf_pool_add_scenario(gf_pool* p, int first_patch, int npatch, double t0, dou
     HIPCHK(hipMemcpyAsync(p->desc + first_patch, p->h_desc.data() + first_patch, sizeof(PatchDesc) * npatch,
                           hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipMemcpyAsync(p->ctrl + id, &p->h_ctrl[id], sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
+    if (int r = reset_dry_age(p, first_patch, npatch)) return r;
     HIPCHK(hipStreamSynchronize(g.stream));
     if (id + 1 > p->nscen) p->nscen = id + 1;
     if (int r = rebuild_inflow(p)) return r;
@@ -702,6 +718,7 @@ int gf_pool_remove_scenario(gf_pool* p, int s)
     p->h_ctrl[s] = ScenCtrl{};
     HIPCHK(hipMemcpyAsync(p->desc + first, p->h_desc.data() + first, sizeof(PatchDesc) * n, hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipMemcpyAsync(p->ctrl + s, &p->h_ctrl[s], sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
+    if (int r = reset_dry_age(p, first, n)) return r;
     HIPCHK(hipStreamSynchronize(g.stream));
     if (p->hyd_mem[s]) HIPCHK(hipFree(p->hyd_mem[s]));
     p->hyd_mem[s] = nullptr;
@@ -765,6 +782,7 @@ int gf_pool_clear_scenarios(gf_pool* p)
     free_hydrographs(p);
     if (int r = rebuild_inflow(p)) return r;
     HIPCHK(hipMemsetAsync(p->cflbits, 0, sizeof(unsigned long long) * p->capacity, g.stream));
+    if (int r = reset_dry_age(p, 0, p->capacity)) return r;
     HIPCHK(hipStreamSynchronize(g.stream));
     p->graph_dirty = true;
     p->args_dirty = true;
@@ -835,39 +853,6 @@ static int ensure_pair_events(std::vector<hipEvent_t>& v, size_t pairs)
     return GF_OK;
 }
 
-static int sync_batch_args(gf_pool* p)
-{
-    if (p->args_gen != g.params_gen) {
-        p->args_dirty = true;
-        p->graph_dirty = true;
-    }
-    if (!p->args_dirty) return GF_OK;
-    gf::BatchArgs a{};
-    a.desc = p->desc;
-    a.n = p->capacity;  // every slot; undescribed and idle slots carry scenario -1
-    a.capacity = p->capacity;
-    a.q0 = p->q[0];
-    a.q1 = p->q[1];
-    a.aux = p->aux;
-    a.hmax = p->hmax2;
-    a.cflbits = p->cflbits;
-    a.ctrl = p->ctrl;
-    a.nscen = p->capacity;  // every scenario slot; idle ones are inactive
-    a.list_wet = p->list_wet;
-    a.list_dry = p->list_dry;
-    a.counters = p->counters;
-    a.stats = p->stats;
-    a.P = g.P;
-    a.tile_cfl = p->tile_cfl;
-    a.tile_h = p->tile_h;
-    a.tile_cnt = p->tile_cnt;
-    HIPCHK(hipMemcpyAsync(p->d_args, &a, sizeof(a), hipMemcpyHostToDevice, g.stream));
-    HIPCHK(hipStreamSynchronize(g.stream));
-    p->args_dirty = false;
-    p->args_gen = g.params_gen;
-    return GF_OK;
-}
-
 static gf::BatchArgs host_batch_args(gf_pool* p)
 {
     gf::BatchArgs a{};
@@ -889,7 +874,25 @@ static gf::BatchArgs host_batch_args(gf_pool* p)
     a.tile_cfl = p->tile_cfl;
     a.tile_h = p->tile_h;
     a.tile_cnt = p->tile_cnt;
+    a.dry_age = p->dry_age;
     return a;
+}
+
+static int sync_batch_args(gf_pool* p)
+{
+    if (p->args_gen != g.params_gen) {
+        // a new dry_tolerance can make a steady dry patch wet
+        if (int r = reset_dry_age(p, 0, p->capacity)) return r;
+        p->args_dirty = true;
+        p->graph_dirty = true;
+    }
+    if (!p->args_dirty) return GF_OK;
+    const gf::BatchArgs a = host_batch_args(p);
+    HIPCHK(hipMemcpyAsync(p->d_args, &a, sizeof(a), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    p->args_dirty = false;
+    p->args_gen = g.params_gen;
+    return GF_OK;
 }
 
 // Workgroups of the persistent step kernel: every CU filled to the
@@ -936,7 +939,6 @@ int gf_pool_run(gf_pool* p, int substeps, int use_graph, int* any_active)
     if (p->nscen == 0 || p->n == 0) return fail(GF_ESTATE, "no scenario registered");
     if (!p->inflow_error.empty()) return fail(GF_EINVAL, "%s", p->inflow_error.c_str());
     if (int r = sync_batch_args(p)) return r;
-    g.last_wet_ms = g.last_dry_ms = 0.0;
     if (use_graph) {
         if (p->graph_dirty) {
             if (p->graph_exec) HIPCHK(hipGraphExecDestroy(p->graph_exec));
@@ -977,8 +979,6 @@ int gf_pool_run(gf_pool* p, int substeps, int use_graph, int* any_active)
             HIPCHK(hipEventElapsedTime(&ms, g.ev_step[2 * k], g.ev_step[2 * k + 1]));
             g.last_step_ms += ms;
         }
-        g.last_wet_ms = g.last_step_ms;  // active and dry patches share the one step kernel
-        g.last_dry_ms = 0.0;
         g.last_launches = substeps;
     }
     return any_active_flag(p, any_active);
@@ -992,6 +992,7 @@ int gf_pool_run_plain(gf_pool* p, int substeps, int* any_active)
     if (!p->inflow_error.empty()) return fail(GF_EINVAL, "%s", p->inflow_error.c_str());
     std::vector<ScenCtrl>& hc = p->h_ctrl;
     HIPCHK(hipMemcpyAsync(hc.data(), p->ctrl, sizeof(ScenCtrl) * p->nscen, hipMemcpyDeviceToHost, g.stream));
+    if (int r = reset_dry_age(p, 0, p->capacity)) return r;
     HIPCHK(hipStreamSynchronize(g.stream));
     double total_ms = 0.0, step_ms = 0.0;
     int launches = 0;
@@ -1132,22 +1133,17 @@ int gf_synchronize(void)
     return GF_OK;
 }
 
-int gf_last_timing_branches(double* wet_ms, double* dry_ms)
-{
-    if (wet_ms) *wet_ms = g.last_wet_ms;
-    if (dry_ms) *dry_ms = g.last_dry_ms;
-    return GF_OK;
-}
-
-int gf_pool_stats(gf_pool* p, long long* active_patches, long long* dry_patches, long long* attempts)
+int gf_pool_stats(gf_pool* p, long long* active_patches, long long* dry_patches, long long* attempts,
+                  long long* steady_patches)
 {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
-    long long s[3];
+    long long s[4];
     HIPCHK(hipMemcpyAsync(s, p->stats, sizeof(s), hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     if (active_patches) *active_patches = s[0];
     if (dry_patches) *dry_patches = s[1];
     if (attempts) *attempts = s[2];
+    if (steady_patches) *steady_patches = s[3];
     return GF_OK;
 }
 
@@ -1294,7 +1290,7 @@ int gf_pool_autotune(gf_pool* p, int reps, int* best, double* ms)
         p->wet_grid = 0;
         double tot = 0.0;
         for (int r = 0; r <= reps; r++) {  // the first run warms up
-            HIPCHK(hipMemsetAsync(p->counters + gf::NEXT_WET, 0, 2 * sizeof(int), g.stream));  // NEXT_WET, NEXT_DRY
+            HIPCHK(hipMemsetAsync(p->counters + gf::NEXT_WET, 0, sizeof(int), g.stream));
             HIPCHK(hipEventRecord(g.ev_tune[0], g.stream));
             HIPCHK(gf::launch_all(p->mx, p->my, v, gf::default_method(g.P), p->d_args, wet_grid(p), g.stream));
             HIPCHK(hipEventRecord(g.ev_tune[1], g.stream));

@@ -59,7 +59,18 @@ struct InflowArgs {
 constexpr int DRY_CUR_SHIFT = 30;
 
 // Device counters of the batched path (BatchArgs::counters).
-enum { CNT_WET = 0, CNT_DRY = 1, NEXT_WET = 2, NEXT_DRY = 3, NCOUNTERS = 4 };
+enum { CNT_WET = 0, CNT_DRY = 1, NEXT_WET = 2, CNT_SKIP = 3, NCOUNTERS = 4 };
+
+// Steady dry patches.  A dry patch whose cells are all below dry_tolerance
+// steps to (max(h,0), 0, 0), which is idempotent: after two such steps into
+// alternating buffers both buffers hold the same interior, and every later
+// dry step leaves it unchanged.  dry_age[p] counts those steps (saturating at
+// DRY_STEADY) in its low bits and remembers the buffer last written in bit
+// DRY_LAST_BIT (a retake writes the same buffer again and does not count);
+// the classification skips patches at DRY_STEADY.  Any other write of a
+// patch's state resets its entry to 0.
+constexpr int DRY_STEADY = 2;
+constexpr int DRY_LAST_BIT = 4;
 
 // Arguments of the batched (work-list) path: classify -> wet || dry -> control.
 struct BatchArgs {
@@ -76,7 +87,8 @@ struct BatchArgs {
     int* list_wet;
     int* list_dry;
     int* counters;                // NCOUNTERS ints
-    long long* stats;             // accumulated active patches, dry patches, attempts
+    long long* stats;             // accumulated active, dry (steady included), attempts, steady
+    int* dry_age;                 // per patch, see DRY_STEADY
     Params P;
     unsigned long long* tile_cfl; // per patch: tile fold slots (tiled variants)
     unsigned long long* tile_h;

@@ -750,6 +750,7 @@ __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, int next,
     double* dst = cur ? a.q0 : a.q1;
     Params P = a.P;
     P.manning_n = a.ctrl[scen].manning_n;
+    if (threadIdx.x == 0) a.dry_age[patch] = 0;
     double cfl = 0.0, hm = 0.0;
     step_patch<MX, MY, TX, TY, NT, true, DEF>(T, E, F, red, a.desc + patch, patch, ox, oy, src, dst, a.aux, dt, P,
                                          cfl, hm);
@@ -770,88 +771,104 @@ __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, int next,
     __syncthreads();
 }
 
-// Dry patches, one wavefront per patch (taken from their own counter, with
-// no workgroup barrier, so stores never hold the next patch's loads behind a
-// barrier).  A dry patch's whole tile is at most dry_tolerance, so every wave
-// and transverse split vanishes and the step is b4step2 + src2 on the
+// Dry patches per work item of the unified kernel.
+constexpr int DRY_PER_ITEM = 6;
+
+// One dry work item of the unified kernel: DRY_PER_ITEM consecutive entries of
+// the dry list.  A dry patch's whole tile is at most dry_tolerance, so every
+// wave and transverse split vanishes and the step is b4step2 + src2 on the
 // interior: a cell with h < dry_tolerance becomes (max(h,0), 0, 0) whatever
 // its momentum (src2 then keeps zero momentum at zero), so only h is read for
-// it; hu, hv and b are read only for h == dry_tolerance.  Entries of the dry
-// list are patch | cur << DRY_CUR_SHIFT.
-template <int MX, int MY>
-__device__ __forceinline__ void dry_waves(const BatchArgs* pa)
+// it; hu, hv and b are read only for h == dry_tolerance.
+template <int MX, int MY, int NT>
+__device__ __forceinline__ void dry_item(const BatchArgs* pa, int k, int next, int* next_slot, int* unsteady)
 {
     using G = Geo<MX, MY>;
     constexpr int NP2 = G::NP2, CELLS = MX * MY;
-    constexpr int CH = CELLS / 64 < 16 ? (CELLS + 63) / 64 : (CELLS / 64 == 16 ? 16 : 4);  // cells per lane per chunk
+    constexpr int PER = (DRY_PER_ITEM * CELLS + NT - 1) / NT;
     constexpr int PMASK = (1 << DRY_CUR_SHIFT) - 1;
     const BatchArgs& a = *pa;
-    const int lane = threadIdx.x & 63;
+    const int first = k * DRY_PER_ITEM;
     const int count = a.counters[CNT_DRY];
     const double tol = a.P.dry_tolerance;
-    int j = 0;
-    if (lane == 0) j = atomicAdd(&a.counters[NEXT_DRY], 1);
-    j = __shfl(j, 0, 64);
-    while (j < count) {
-        const int ent = a.list_dry[j];
-        int jn = 0;
-        if (lane == 0) jn = atomicAdd(&a.counters[NEXT_DRY], 1);
-        const int pt = ent & PMASK, cur = ent >> DRY_CUR_SHIFT;
+    // entries (patch | cur << DRY_CUR_SHIFT), then h, then the stores: two
+    // dependent memory round trips per item
+    int ent[PER], kk[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        const int e = threadIdx.x + NT * r;
+        const int j = first + e / CELLS;
+        ent[r] = (e < DRY_PER_ITEM * CELLS && j < count) ? a.list_dry[j] : -1;
+        const int c = e % CELLS;
+        kk[r] = G::ti(c % MX + 1, c / MX + 1);
+    }
+    double h0[PER];
+#pragma unroll
+    for (int r = 0; r < PER; r++)
+        if (ent[r] >= 0) h0[r] = ((ent[r] >> DRY_CUR_SHIFT) ? a.q1 : a.q0)[(size_t)(ent[r] & PMASK) * 3 * NP2 + kk[r]];
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+        if (ent[r] < 0) continue;
+        const int pt = ent[r] & PMASK, cur = ent[r] >> DRY_CUR_SHIFT;
         const double* src = (cur ? a.q1 : a.q0) + (size_t)pt * 3 * NP2;
         double* dst = (cur ? a.q0 : a.q1) + (size_t)pt * 3 * NP2;
-        for (int c0 = 0; c0 < CELLS; c0 += 64 * CH) {
-            int kk[CH];
-            double h0[CH];
-#pragma unroll
-            for (int r = 0; r < CH; r++) {
-                const int c = c0 + lane + 64 * r;
-                kk[r] = c < CELLS ? G::ti(c % MX + 1, c / MX + 1) : -1;
-                if (kk[r] >= 0) h0[r] = src[kk[r]];
-            }
-#pragma unroll
-            for (int r = 0; r < CH; r++) {
-                if (kk[r] < 0) continue;
-                double h = h0[r], hu = 0.0, hv = 0.0;
-                if (h < tol) {
-                    h = fmaxf64(h, 0.0);
-                } else {
-                    const ScenCtrl* c = a.ctrl + a.desc[pt].scen;
-                    Params P = a.P;
-                    P.manning_n = c->manning_n;
-                    hu = src[NP2 + kk[r]];
-                    hv = src[2 * NP2 + kk[r]];
-                    if (P.src_term > 0) src2_cell(h, hu, hv, a.aux[(size_t)pt * NP2 + kk[r]], c->dt, P);
-                }
-                dst[kk[r]] = h;
-                dst[NP2 + kk[r]] = hu;
-                dst[2 * NP2 + kk[r]] = hv;
-            }
+        double h = h0[r], hu = 0.0, hv = 0.0;
+        if (h < tol) {
+            h = fmaxf64(h, 0.0);
+        } else {
+            atomicOr(&unsteady[(threadIdx.x + NT * r) / CELLS], 1);
+            const ScenCtrl* c = a.ctrl + a.desc[pt].scen;
+            Params P = a.P;
+            P.manning_n = c->manning_n;
+            hu = src[NP2 + kk[r]];
+            hv = src[2 * NP2 + kk[r]];
+            if (P.src_term > 0) src2_cell(h, hu, hv, a.aux[(size_t)pt * NP2 + kk[r]], c->dt, P);
         }
-        if (lane == 0) a.hmax[(size_t)(cur ^ 1) * a.capacity + pt] = 0.0;
-        j = __shfl(jn, 0, 64);
+        dst[kk[r]] = h;
+        dst[NP2 + kk[r]] = hu;
+        dst[2 * NP2 + kk[r]] = hv;
+        if ((threadIdx.x + NT * r) % CELLS == 0) a.hmax[(size_t)(cur ^ 1) * a.capacity + pt] = 0.0;
     }
+    __syncthreads();
+    if (threadIdx.x < DRY_PER_ITEM && first + (int)threadIdx.x < count) {
+        const int e = a.list_dry[first + threadIdx.x];
+        const int pt = e & PMASK, dstbuf = (e >> DRY_CUR_SHIFT) ^ 1;
+        const int old = a.dry_age[pt];
+        int age = old & 3;
+        if (unsteady[threadIdx.x]) age = 0;
+        else if (((old >> DRY_LAST_BIT) & 1) != dstbuf || age == 0) age = age < DRY_STEADY ? age + 1 : age;
+        a.dry_age[pt] = age | (dstbuf << DRY_LAST_BIT);
+    }
+    if (threadIdx.x < DRY_PER_ITEM) unsteady[threadIdx.x] = 0;
+    if (threadIdx.x == 0) *next_slot = next;
+    __syncthreads();
 }
 
-// Unified persistent step of an attempt: the active patches' tiles
-// (wet_item, one workgroup per tile, from one counter), then every wavefront
-// takes dry patches from a second counter (dry_waves); the streaming dry work
-// fills the tail that the compute-bound active items leave.
+// Unified persistent step of an attempt: the active patches' tiles first
+// (wet_item), then the dry patches in items of DRY_PER_ITEM (dry_item), from
+// one atomic cursor.  The streaming dry items fill the tail that the
+// compute-bound active items leave when the list runs out.
 template <int MX, int MY, int TX, int TY, int NT, bool DEF>
 __global__ void __launch_bounds__(NT) k_all(const BatchArgs* pa)
 {
     using TL = Tiling<MX, MY, TX, TY>;
     __shared__ int item[2];
+    __shared__ int unsteady[DRY_PER_ITEM];  // per dry patch of an item: a cell at dry_tolerance
     if (threadIdx.x == 0) item[0] = atomicAdd(&pa->counters[NEXT_WET], 1);
+    if (threadIdx.x < DRY_PER_ITEM) unsteady[threadIdx.x] = 0;
     __syncthreads();
     const int nwet = pa->counters[CNT_WET] * TL::NTILES;
+    const int limit = nwet + (pa->counters[CNT_DRY] + DRY_PER_ITEM - 1) / DRY_PER_ITEM;
     for (int it = 0;; it++) {
         const int idx = item[it & 1];
-        if (idx >= nwet) break;
+        if (idx >= limit) break;
         int next = 0;
         if (threadIdx.x == 0) next = atomicAdd(&pa->counters[NEXT_WET], 1);
-        wet_item<MX, MY, TX, TY, NT, DEF>(pa, idx, next, &item[(it + 1) & 1]);
+        if (idx < nwet)
+            wet_item<MX, MY, TX, TY, NT, DEF>(pa, idx, next, &item[(it + 1) & 1]);
+        else
+            dry_item<MX, MY, NT>(pa, idx - nwet, next, &item[(it + 1) & 1], unsteady);
     }
-    dry_waves<MX, MY>(pa);
 }
 
 // ---------------------------------------------------------------------------
@@ -1107,13 +1124,21 @@ __global__ void __launch_bounds__(256) k_classify(BatchArgs a)
             }
         }
     }
+    if (kind == 0 && (a.dry_age[p] & 3) >= DRY_STEADY) {
+        // steady dry: the step would rewrite the same values; only the depth
+        // maximum of the next state is recorded
+        a.hmax[(size_t)(cur ^ 1) * a.capacity + p] = 0.0;
+        kind = 2;
+    }
     const unsigned long long mw = __ballot(kind == 1);
     const unsigned long long md = __ballot(kind == 0);
+    const unsigned long long ms = __ballot(kind == 2);
     const int lane = threadIdx.x & 63;
     int bw = 0, bd = 0;
     if (lane == 0) {
         if (mw) bw = atomicAdd(&a.counters[CNT_WET], __popcll(mw));
         if (md) bd = atomicAdd(&a.counters[CNT_DRY], __popcll(md));
+        if (ms) atomicAdd(&a.counters[CNT_SKIP], __popcll(ms));
     }
     bw = __shfl(bw, 0, 64);
     bd = __shfl(bd, 0, 64);
@@ -1130,8 +1155,9 @@ __global__ void __launch_bounds__(64) k_batch_control(BatchArgs a)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s == 0) {
         a.stats[0] += a.counters[CNT_WET];
-        a.stats[1] += a.counters[CNT_DRY];
+        a.stats[1] += a.counters[CNT_DRY] + a.counters[CNT_SKIP];
         a.stats[2] += 1;
+        a.stats[3] += a.counters[CNT_SKIP];
         for (int k = 0; k < NCOUNTERS; k++) a.counters[k] = 0;
     }
     if (s >= a.nscen) return;

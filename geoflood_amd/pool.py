@@ -334,9 +334,9 @@ class Pool:
         return best.value, ms.tolist()
 
     def stats(self) -> dict:
-        a, d, n = ct.c_longlong(0), ct.c_longlong(0), ct.c_longlong(0)
-        check(self.lib.gf_pool_stats(self.h, ct.byref(a), ct.byref(d), ct.byref(n)), "gf_pool_stats")
-        return dict(active_patches=a.value, dry_patches=d.value, attempts=n.value)
+        a, d, n, s = (ct.c_longlong(0) for _ in range(4))
+        check(self.lib.gf_pool_stats(self.h, ct.byref(a), ct.byref(d), ct.byref(n), ct.byref(s)), "gf_pool_stats")
+        return dict(active_patches=a.value, dry_patches=d.value, attempts=n.value, steady_patches=s.value)
 
     def run_plain(self, substeps: int) -> bool:
         act = ct.c_int(0)

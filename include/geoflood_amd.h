@@ -346,21 +346,19 @@ int gf_pool_run_plain(gf_pool *pool, int substeps, int *any_active);
 int gf_synchronize(void);
 
 /* Device time (ms) of the last gf_pool_run / gf_pool_run_plain /
- * gf_pool_step* call and of its step launches alone (for gf_pool_run: from
- * the end of classification to the end of both step kernels, per attempt,
- * summed), from HIP events on the library streams; step_launches counts the
- * timed step phases.  Graph runs report the total only. */
+ * gf_pool_step* call and of its step launches alone (for gf_pool_run: the
+ * persistent step kernel of each attempt, summed), from HIP events on the
+ * library stream; step_launches counts the timed step launches.  Graph runs
+ * report the total only. */
 int gf_last_timing(double *total_ms, double *step_ms, int *step_launches);
 
-/* Device time (ms) of the last non-graph gf_pool_run, summed over its
- * attempts, of the active-patch kernel and of the dry-patch kernel; the two
- * run concurrently, and gf_last_timing's step_ms spans both. */
-int gf_last_timing_branches(double *wet_ms, double *dry_ms);
-
-/* Patches stepped by the full kernel, patches handled as dry, and attempts,
- * accumulated over all gf_pool_run calls on this pool. */
+/* Patches stepped by the full kernel, patches handled as dry, attempts, and
+ * of the dry patches those left untouched as steady (every interior depth
+ * below dry_tolerance for two steps in a row, so both state buffers already
+ * hold the step's result), accumulated over all gf_pool_run calls on this
+ * pool. */
 int gf_pool_stats(gf_pool *pool, long long *active_patches, long long *dry_patches,
-                  long long *attempts);
+                  long long *attempts, long long *steady_patches);
 
 #ifdef __cplusplus
 }

@@ -208,6 +208,7 @@ def test_dry_bed_dam_break_batched_split(gf, oracle_c):
                 break
         st = pool.stats()
         assert st["dry_patches"] > 0 and st["active_patches"] > 0
+        assert 0 < st["steady_patches"] < st["dry_patches"]
         np.testing.assert_allclose(dts, rec["dt"], rtol=1e-12, atol=0)
         got = np.stack(pool.download_global(layout, buf=info["cur"]))
         ref = br.to_global()
@@ -215,6 +216,50 @@ def test_dry_bed_dam_break_batched_split(gf, oracle_c):
     finally:
         oracle_c.set_params(Params())
         gf.set_params()
+
+
+def test_steady_dry_patches_equal_plain(gf):
+    """Dry patches that stay below dry_tolerance for two steps are left
+    untouched by the batched path.  A thin film (h < dry_tolerance) with
+    momentum, written into such patches between steps, must still be
+    stepped (momentum zeroed) and the run must equal the plain loop, which
+    steps every patch, bit for bit."""
+    from geoflood_amd.scenarios import uniform_dam_break
+    n = 256
+    h, hu, hv, b, dx = uniform_dam_break(n, dx=1.0, h0=10.0, rms=3.0, seed=2)
+    film = np.zeros_like(h)
+    film[: n // 2, 3 * n // 4:] = 2e-4
+    out = []
+    for mode in ("plain", "graph"):
+        print(f"case steady dry {mode}", flush=True)
+        layout = BrickLayout(n // 32, n // 32, 32, 32, dx, dx)
+        pool = Pool(32, 32, layout.npatch)
+        pool.set_layout([layout])
+        pool.upload_global(layout, h, hu, hv, b)
+        sid = pool.add_scenario(layout, dt0=0.01, target_steps=16)
+        step = (lambda: pool.run_plain(1)) if mode == "plain" else (lambda: pool.run(1, graph=True))
+        while pool.scenario(sid)["nsteps"] < 8:
+            step()
+        info = pool.scenario(sid)
+        if mode == "graph":
+            assert pool.stats()["steady_patches"] > 0
+        cur = np.stack(pool.download_global(layout, buf=info["cur"]))
+        cur[0] += film
+        cur[1] += 5.0 * film
+        cur[2] -= 3.0 * film
+        pool.upload_global(layout, cur[0], cur[1], cur[2], b, buf=info["cur"])
+        while step():
+            pass
+        info = pool.scenario(sid)
+        assert info["nsteps"] == 16
+        got = np.stack(pool.download_global(layout, buf=info["cur"]))
+        out.append((info["t"], got))
+        pool.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
+    g = out[1][1]
+    wet_front = g[0] >= 1e-3
+    assert not np.any(g[1][: n // 2, 3 * n // 4:][~wet_front[: n // 2, 3 * n // 4:]])
 
 
 def test_plugin_single_step_update_protocol(gf, oracle_c):
