@@ -7,11 +7,12 @@ reservoir h = 10 m over the upper-left eighth, dry elsewhere; Manning
 n = 0.03; walls; initial dt 0.01 s, desired CFL 0.9, max CFL 1.0.
 
 A "step" is one step attempt over all 16,384 patches: device work lists
-(patches whose 2-cell halo holds water are active, the rest dry), the
-persistent fused step over the active patches (halo pulled from the
-neighbours) concurrently with the dry-patch stream, and the device dt
-controller.  One cell-update is one interior
-cell advanced by one accepted step (retaken attempts do not count).
+(patches whose 2-cell halo holds water are active, the rest dry; dry patches
+that stayed below dry_tolerance for two steps are steady and left as they
+are), one persistent kernel that steps the active patches (halo pulled from
+the neighbours) and then the dry ones, and the device dt controller.  One
+cell-update is one interior cell advanced by one accepted step (retaken
+attempts do not count).
 
 Runs, all from the same initial state, inputs resident in HBM:
   batched  -- K attempts, each one hipGraph launch (the reported value);
@@ -111,6 +112,9 @@ def main():
     ap.add_argument("--ensemble-rate", type=float, default=32.0)
     ap.add_argument("--ensemble-capacity", type=int, default=65536)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="use this step variant instead of the autotuner's choice (counter runs, where "
+                         "the profiler's serialisation makes the autotuner's timings unreliable)")
     ap.add_argument("--no-events", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (torch) three times; "
@@ -155,7 +159,10 @@ def main():
     sid = reset(pool, layout, state, 0)
     best, tune_ms = pool.autotune(reps=3)
     _, variants = pool.variants()
-    log(f"autotune: variants {variants}, k_wet ms {['%.4f' % m for m in tune_ms]}, chose {best}")
+    if args.variant >= 0:
+        best = args.variant
+        pool.set_variant(best)
+    log(f"autotune: variants {variants}, k_all ms {['%.4f' % m for m in tune_ms]}, using {best}")
     pool.run(0, graph=True)  # capture outside the timed region
     geoflood_amd.load().gf_synchronize()
     log(f"timed batched run: {args.steps} attempts ...")
@@ -172,7 +179,8 @@ def main():
         st_prev = st
         if last_act is None:
             last_act = act
-        elif done < args.steps and (act > RETUNE_FACTOR * last_act or act * RETUNE_FACTOR < last_act):
+        elif done < args.steps and args.variant < 0 and (act > RETUNE_FACTOR * last_act
+                                                         or act * RETUNE_FACTOR < last_act):
             v, _ = pool.autotune(reps=2)
             retunes.append(dict(after_attempts=done, active_patches=round(act), variant=v))
             last_act = act
@@ -194,7 +202,9 @@ def main():
     # ---- same attempts without a graph, HIP events around each launch ----
     # The dominant kernel is k_all, the persistent step over the attempt's
     # active tiles and then its dry patches: its algorithmic bytes per launch
-    # are 56 B x 1024 cells x the patches it advances (all 16,384).
+    # are 56 B x 1024 cells x the patches it advances (all 16,384; steady dry
+    # patches, whose step leaves both buffers as they are, count as advanced
+    # but are not read or written).
     roofline = None
     composite = None
     if not args.no_events:
@@ -208,37 +218,39 @@ def main():
         info_e = pool.scenario(sid)
         act = s1["active_patches"] - s0["active_patches"]
         dry = s1["dry_patches"] - s0["dry_patches"]
+        steady = s1["steady_patches"] - s0["steady_patches"]
         launches = max(launches, 1)
         k_avg_ms = step_ms / launches
         k_bytes = BYTES_PER_CELL_UPDATE * MX * MX * (act + dry) / launches
         achieved = k_bytes / (k_avg_ms * 1e-3) / 1e9
-        # measured HBM bytes and FP64 work per active / dry patch (rocprofv3
-        # --pmc, profiles/), scaled by this run's patches per launch
+        # measured HBM bytes and VALU work per launch of this same trajectory
+        # (rocprofv3 --pmc over the 500 event-timed launches, profiles/)
         traffic = valu = None
         kv = profile_values("k_all_32x32_uniform4096_traffic.txt")
-        if "hbm_bytes_per_active_patch" in kv:
-            traffic = round((float(kv["hbm_bytes_per_active_patch"]) * act
-                             + float(kv["hbm_bytes_per_dry_patch"]) * dry) / launches)
+        if "hbm_bytes_per_launch" in kv:
+            traffic = round(float(kv["hbm_bytes_per_launch"]))
         kv = profile_values("k_all_32x32_uniform4096_valu.txt")
-        if "fp64_flops_per_active_patch" in kv:
-            fl = float(kv["fp64_flops_per_active_patch"]) * act / launches / (k_avg_ms * 1e-3) / 1e12
-            li = float(kv["valu_lane_instr_per_launch_patch"]) * (act + dry) / launches / (k_avg_ms * 1e-3) / 1e12
+        if "fp64_flops_per_launch" in kv:
+            fl = float(kv["fp64_flops_per_launch"]) / (k_avg_ms * 1e-3) / 1e12
+            li = float(kv["valu_lane_instr_per_launch"]) / (k_avg_ms * 1e-3) / 1e12
             valu = dict(fp64_tflops=round(fl, 3), fp64_peak_tflops=FP64_PEAK_TFLOPS,
                         fp64_frac=round(fl / FP64_PEAK_TFLOPS, 4), valu_issue_frac=round(li / VALU_LANE_PEAK_T, 4),
-                        note="FP64 FLOPs (FMA = 2) per active patch and VALU lane-instructions per patch from "
-                             "the profiles/k_all_32x32_uniform4096_valu.txt counters, over this run's k_all time")
+                        note="FP64 FLOPs (FMA = 2) and VALU lane-instructions per launch from the "
+                             "profiles/k_all_32x32_uniform4096_valu.txt counters, over this run's k_all time")
         roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                         kernel="k_all<32,32,32,32,%d>" % variants[pool.variants()[0]][2],
                         kernel_avg_ms=round(k_avg_ms, 5), bytes_per_launch=round(k_bytes),
                         active_patches_per_launch=round(act / launches, 1),
-                        dry_patches_per_launch=round(dry / launches, 1), launches=launches, valu=valu)
+                        dry_patches_per_launch=round(dry / launches, 1),
+                        steady_patches_per_launch=round(steady / launches, 1), launches=launches, valu=valu)
         composite = dict(attempt_avg_ms_without_graph=round(1e3 * ev_wall / launches, 5),
                          note="inflow (when present) -> classify -> k_all -> control per attempt; "
                               "the timed value uses one hipGraph replay per attempt")
         same = (info_e["nsteps"] == info_b["nsteps"] and info_e["t"] == info_b["t"])
         log(f"events run: k_all avg {k_avg_ms:.4f} ms ({act / launches:.0f} active + {dry / launches:.0f} dry "
-            f"patches), {achieved:.1f} GB/s algorithmic; same trajectory as graph run: {same}")
+            f"patches, {steady / launches:.0f} of them steady), {achieved:.1f} GB/s algorithmic; "
+            f"same trajectory as graph run: {same}")
 
     # ---- ensemble serving sample (continuous batching of flood scenarios) ----
     ensemble = None
@@ -314,11 +326,11 @@ def main():
             "initial_state": "h = 10 m for x < L/2, y >= 3L/4; dry elsewhere; u = v = 0",
             "numerics": "g 9.81, drytol 1e-3, Manning n 0.03, MC limiter, transverse 2, walls",
             "dt_control": f"initial dt {DT0} s, desired CFL {DESIRED_CFL}, max CFL {MAX_CFL}, on device",
-            "mode": "batched: one hipGraph launch (ghost fill + fused step + dt control) per attempt",
+            "mode": "batched: one hipGraph replay per attempt (classify, fused step with halo pull, dt control)",
             "accepted_steps": accepted,
             "simulated_time_s": info_b["t"],
             "step_variants": [dict(tile=[v[0], v[1]], threads=v[2]) for v in variants],
-            "autotune_initial": dict(variant=best, k_wet_ms=[round(m, 5) for m in tune_ms]),
+            "autotune_initial": dict(variant=best, k_all_ms=[round(m, 5) for m in tune_ms]),
             "autotune_retunes": retunes,
             "active_patches_per_attempt": round(active_per_attempt, 1),
             "dry_patches_per_attempt": round(dry_per_attempt, 1),
