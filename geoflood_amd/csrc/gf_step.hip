@@ -771,10 +771,16 @@ __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, int next,
     __syncthreads();
 }
 
-// Dry patches per work item of the unified kernel.
-constexpr int DRY_PER_ITEM = 6;
+// Dry patches per work item of the unified kernel: about eight cells per
+// thread (a whole patch at least), so that the item's loads are all in
+// flight at once without crowding the registers of the active items.
+template <int MX, int MY, int NT>
+constexpr int dry_per_item()
+{
+    return 8 * NT / (MX * MY) > 0 ? 8 * NT / (MX * MY) : 1;
+}
 
-// One dry work item of the unified kernel: DRY_PER_ITEM consecutive entries of
+// One dry work item of the unified kernel: dry_per_item consecutive entries of
 // the dry list.  A dry patch's whole tile is at most dry_tolerance, so every
 // wave and transverse split vanishes and the step is b4step2 + src2 on the
 // interior: a cell with h < dry_tolerance becomes (max(h,0), 0, 0) whatever
@@ -785,20 +791,26 @@ __device__ __forceinline__ void dry_item(const BatchArgs* pa, int k, int next, i
 {
     using G = Geo<MX, MY>;
     constexpr int NP2 = G::NP2, CELLS = MX * MY;
-    constexpr int PER = (DRY_PER_ITEM * CELLS + NT - 1) / NT;
+    constexpr int DPI = dry_per_item<MX, MY, NT>();
+    constexpr int PER = (DPI * CELLS + NT - 1) / NT;
+    static_assert(DPI <= NT, "one thread per dry patch of an item updates its steady state");
     constexpr int PMASK = (1 << DRY_CUR_SHIFT) - 1;
     const BatchArgs& a = *pa;
-    const int first = k * DRY_PER_ITEM;
+    const int first = k * DPI;
     const int count = a.counters[CNT_DRY];
+    // opaque thread index: keeps the item's index computations inside the
+    // persistent loop instead of live (and spilled) across the active items
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     const double tol = a.P.dry_tolerance;
     // entries (patch | cur << DRY_CUR_SHIFT), then h, then the stores: two
     // dependent memory round trips per item
     int ent[PER], kk[PER];
 #pragma unroll
     for (int r = 0; r < PER; r++) {
-        const int e = threadIdx.x + NT * r;
+        const int e = tid + NT * r;
         const int j = first + e / CELLS;
-        ent[r] = (e < DRY_PER_ITEM * CELLS && j < count) ? a.list_dry[j] : -1;
+        ent[r] = (e < DPI * CELLS && j < count) ? a.list_dry[j] : -1;
         const int c = e % CELLS;
         kk[r] = G::ti(c % MX + 1, c / MX + 1);
     }
@@ -816,7 +828,7 @@ __device__ __forceinline__ void dry_item(const BatchArgs* pa, int k, int next, i
         if (h < tol) {
             h = fmaxf64(h, 0.0);
         } else {
-            atomicOr(&unsteady[(threadIdx.x + NT * r) / CELLS], 1);
+            atomicOr(&unsteady[(tid + NT * r) / CELLS], 1);
             const ScenCtrl* c = a.ctrl + a.desc[pt].scen;
             Params P = a.P;
             P.manning_n = c->manning_n;
@@ -827,10 +839,10 @@ __device__ __forceinline__ void dry_item(const BatchArgs* pa, int k, int next, i
         dst[kk[r]] = h;
         dst[NP2 + kk[r]] = hu;
         dst[2 * NP2 + kk[r]] = hv;
-        if ((threadIdx.x + NT * r) % CELLS == 0) a.hmax[(size_t)(cur ^ 1) * a.capacity + pt] = 0.0;
+        if ((tid + NT * r) % CELLS == 0) a.hmax[(size_t)(cur ^ 1) * a.capacity + pt] = 0.0;
     }
     __syncthreads();
-    if (threadIdx.x < DRY_PER_ITEM && first + (int)threadIdx.x < count) {
+    if (threadIdx.x < DPI && first + (int)threadIdx.x < count) {
         const int e = a.list_dry[first + threadIdx.x];
         const int pt = e & PMASK, dstbuf = (e >> DRY_CUR_SHIFT) ^ 1;
         const int old = a.dry_age[pt];
@@ -839,13 +851,13 @@ __device__ __forceinline__ void dry_item(const BatchArgs* pa, int k, int next, i
         else if (((old >> DRY_LAST_BIT) & 1) != dstbuf || age == 0) age = age < DRY_STEADY ? age + 1 : age;
         a.dry_age[pt] = age | (dstbuf << DRY_LAST_BIT);
     }
-    if (threadIdx.x < DRY_PER_ITEM) unsteady[threadIdx.x] = 0;
+    if (threadIdx.x < DPI) unsteady[threadIdx.x] = 0;
     if (threadIdx.x == 0) *next_slot = next;
     __syncthreads();
 }
 
 // Unified persistent step of an attempt: the active patches' tiles first
-// (wet_item), then the dry patches in items of DRY_PER_ITEM (dry_item), from
+// (wet_item), then the dry patches in items of dry_per_item (dry_item), from
 // one atomic cursor.  The streaming dry items fill the tail that the
 // compute-bound active items leave when the list runs out.
 template <int MX, int MY, int TX, int TY, int NT, bool DEF>
@@ -853,12 +865,13 @@ __global__ void __launch_bounds__(NT) k_all(const BatchArgs* pa)
 {
     using TL = Tiling<MX, MY, TX, TY>;
     __shared__ int item[2];
-    __shared__ int unsteady[DRY_PER_ITEM];  // per dry patch of an item: a cell at dry_tolerance
+    constexpr int DPI = dry_per_item<MX, MY, NT>();
+    __shared__ int unsteady[DPI];  // per dry patch of an item: a cell at dry_tolerance
     if (threadIdx.x == 0) item[0] = atomicAdd(&pa->counters[NEXT_WET], 1);
-    if (threadIdx.x < DRY_PER_ITEM) unsteady[threadIdx.x] = 0;
+    if (threadIdx.x < DPI) unsteady[threadIdx.x] = 0;
     __syncthreads();
     const int nwet = pa->counters[CNT_WET] * TL::NTILES;
-    const int limit = nwet + (pa->counters[CNT_DRY] + DRY_PER_ITEM - 1) / DRY_PER_ITEM;
+    const int limit = nwet + (pa->counters[CNT_DRY] + DPI - 1) / DPI;
     for (int it = 0;; it++) {
         const int idx = item[it & 1];
         if (idx >= limit) break;
