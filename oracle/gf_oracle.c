@@ -15,6 +15,7 @@
 #include "gf_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1370,4 +1371,200 @@ int gfo_brick_run(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
     return gfo_brick_run_ex(p, mi, mj, mx, my, mbc, maux, q, aux, mthbc, 0.0, 0.0, dx, dy, NULL, t0,
                             initial_dt, desired_cfl, max_cfl, nsteps, fn, ctx, nthreads, dt_out,
                             cfl_out, mass_out);
+}
+
+/* ======================================================================
+ * ASCII frame output and gauges
+ * ====================================================================== */
+
+/* Fortran Ew.d as Flang writes it (probed against the reference writer
+ * built by oracle/Makefile): the value rounded to d significant digits
+ * (nearest, ties to even, as C's %e), printed as 0.<digits> with the
+ * exponent one higher than C's; "E+dd" for |exponent| <= 99, "+ddd"
+ * without the E above that; zero keeps its sign. */
+void gfo_format_E(char *out, double v, int w, int d)
+{
+    char body[64];
+    int n;
+    if (isnan(v)) {
+        n = snprintf(body, sizeof body, "NaN");
+    } else if (isinf(v)) {
+        n = snprintf(body, sizeof body, "%sInf", v < 0 ? "-" : "");
+    } else {
+        char digits[40];
+        int e10 = 0;
+        const int neg = signbit(v) ? 1 : 0;
+        if (v == 0.0) {
+            memset(digits, '0', (size_t)d);
+            e10 = 0;
+        } else {
+            char tmp[64];
+            snprintf(tmp, sizeof tmp, "%.*e", d - 1, fabs(v));  /* D.DDDDe+XX */
+            digits[0] = tmp[0];
+            memcpy(digits + 1, tmp + 2, (size_t)(d - 1));
+            e10 = atoi(strchr(tmp, 'e') + 1) + 1;
+        }
+        digits[d] = 0;
+        const int ae = e10 < 0 ? -e10 : e10;
+        if (ae <= 99)
+            n = snprintf(body, sizeof body, "%s0.%sE%c%02d", neg ? "-" : "", digits, e10 < 0 ? '-' : '+', ae);
+        else
+            n = snprintf(body, sizeof body, "%s0.%s%c%03d", neg ? "-" : "", digits, e10 < 0 ? '-' : '+', ae);
+    }
+    if (n > w) {
+        memset(out, '*', (size_t)w);
+        return;
+    }
+    memset(out, ' ', (size_t)(w - n));
+    memcpy(out + (w - n), body, (size_t)n);
+}
+
+/* Fortran I5: right-justified, "*****" when it does not fit. */
+static int put_i5(char *out, int v)
+{
+    char tmp[32];
+    const int n = snprintf(tmp, sizeof tmp, "%d", v);
+    if (n > 5) {
+        memcpy(out, "*****", 5);
+    } else {
+        memset(out, ' ', (size_t)(5 - n));
+        memcpy(out + 5 - n, tmp, (size_t)n);
+    }
+    return 5;
+}
+
+typedef struct { char *p; int len, cap, bad; } obuf;
+
+static void ob_put(obuf *b, const char *s, int n)
+{
+    if (b->bad || b->len + n > b->cap) { b->bad = 1; return; }
+    memcpy(b->p + b->len, s, (size_t)n);
+    b->len += n;
+}
+
+static void ob_i5_line(obuf *b, int v, const char *label)
+{
+    char t[8];
+    put_i5(t, v);
+    ob_put(b, t, 5);
+    ob_put(b, "                 ", 17);
+    ob_put(b, label, (int)strlen(label));
+    ob_put(b, "\n", 1);
+}
+
+static void ob_E(obuf *b, double v, int w, int d)
+{
+    char t[64];
+    gfo_format_E(t, v, w, d);
+    ob_put(b, t, w);
+}
+
+/* fc2d_geoclaw_fort_write_header (fc2d_geoclaw_output_ascii_fort.f:1-36):
+ * format 1000 = e30.20 'time', then i5 meqn+1, ngrids, maux, 2, mbc. */
+int gfo_frame_header(char *out, int cap, double time, int meqn, int maux, int ngrids, int mbc)
+{
+    obuf b = {out, 0, cap, 0};
+    ob_E(&b, time, 30, 20);
+    ob_put(&b, "    time\n", 9);
+    ob_i5_line(&b, meqn + 1, "meqn");
+    ob_i5_line(&b, ngrids, "ngrids");
+    ob_i5_line(&b, maux, "num_aux");
+    ob_i5_line(&b, 2, "num_dim");
+    ob_i5_line(&b, mbc, "num_ghost");
+    return b.bad ? -1 : b.len;
+}
+
+/* fc2d_geoclaw_fort_write_file (fc2d_geoclaw_output_ascii_fort.f:38-117):
+ * format 1001 (six I5 lines), format 1002 (four e24.16 lines and an empty
+ * record), then per row j = 1..my and cell i = 1..mx: q with |q| < 1e-99
+ * set to 0 in place, eta = q(1) + aux(1) (0 when |eta| < 1e-99), written as
+ * 5E26.16 (four values here), and after each row a list-directed ' ' (a
+ * record of two blanks). */
+int gfo_frame_patch(char *out, int cap, int mx, int my, int meqn, int maux, int mbc,
+                    double xlower, double ylower, double dx, double dy, double *q,
+                    const double *aux, int patch_num, int level, int blockno, int mpirank)
+{
+    obuf b = {out, 0, cap, 0};
+    const int nx = mx + 2 * mbc;
+    ob_i5_line(&b, patch_num, "grid_number");
+    ob_i5_line(&b, level, "AMR_level");
+    ob_i5_line(&b, blockno, "block_number");
+    ob_i5_line(&b, mpirank, "mpi_rank");
+    ob_i5_line(&b, mx, "mx");
+    ob_i5_line(&b, my, "my");
+    ob_E(&b, xlower, 24, 16); ob_put(&b, "    xlow\n", 9);
+    ob_E(&b, ylower, 24, 16); ob_put(&b, "    ylow\n", 9);
+    ob_E(&b, dx, 24, 16); ob_put(&b, "    dx\n", 7);
+    ob_E(&b, dy, 24, 16); ob_put(&b, "    dy\n", 7);
+    ob_put(&b, "\n", 1);
+    for (int j = 1; j <= my; j++) {
+        for (int i = 1; i <= mx; i++) {
+            double *qc = q + ((size_t)(j - 1 + mbc) * nx + (i - 1 + mbc)) * meqn;
+            for (int m = 0; m < meqn; m++)
+                if (fabs(qc[m]) < 1e-99) qc[m] = 0.0;
+            double eta = qc[0] + aux[((size_t)(j - 1 + mbc) * nx + (i - 1 + mbc)) * maux];
+            if (fabs(eta) < 1e-99) eta = 0.0;
+            for (int m = 0; m < meqn; m++) ob_E(&b, qc[m], 26, 16);
+            ob_E(&b, eta, 26, 16);
+            ob_put(&b, "\n", 1);
+        }
+        ob_put(&b, "  \n", 3);
+    }
+    return b.bad ? -1 : b.len;
+}
+
+/* fc2d_geoclaw_update_gauge (geoclaw_gauges_fort.f90:1-77). */
+void gfo_update_gauge(const gfo_params *p, int mx, int my, int mbc, int maux, double xlower,
+                      double ylower, double dx, double dy, const double *q, const double *aux,
+                      double xc, double yc, double qvar[3], double avar[1])
+{
+    (void)my;
+    const int nx = mx + 2 * mbc;
+#define GQ(m, i, j) q[((size_t)((j) - 1 + mbc) * nx + ((i) - 1 + mbc)) * 3 + (m)]
+#define GA(i, j) aux[((size_t)((j) - 1 + mbc) * nx + ((i) - 1 + mbc)) * maux]
+    const int ii = (int)((xc - xlower) / dx) + 1;
+    const int jj = (int)((yc - ylower) / dy) + 1;
+    const double xcent = xlower + (ii - .5) * dx;
+    const double ycent = ylower + (jj - .5) * dy;
+    const double xoff = (xc - xcent) / dx;
+    const double yoff = (yc - ycent) / dy;
+    const double drytol2 = 0.1 * p->dry_tolerance;
+    const double h1 = GQ(0, ii, jj), h2 = GQ(0, ii + 1, jj), h3 = GQ(0, ii, jj + 1), h4 = GQ(0, ii + 1, jj + 1);
+    double topo;
+    if (h1 < drytol2 || h2 < drytol2 || h3 < drytol2 || h4 < drytol2) {
+        for (int m = 0; m < 3; m++) qvar[m] = GQ(m, ii, jj);
+        topo = GA(ii, jj);
+    } else {
+        for (int m = 0; m < 3; m++)
+            qvar[m] = (1.0 - xoff) * (1.0 - yoff) * GQ(m, ii, jj) + xoff * (1.0 - yoff) * GQ(m, ii + 1, jj) +
+                      (1.0 - xoff) * yoff * GQ(m, ii, jj + 1) + xoff * yoff * GQ(m, ii + 1, jj + 1);
+        topo = (1.0 - xoff) * (1.0 - yoff) * GA(ii, jj) + xoff * (1.0 - yoff) * GA(ii + 1, jj) +
+               (1.0 - xoff) * yoff * GA(ii, jj + 1) + xoff * yoff * GA(ii + 1, jj + 1);
+    }
+    avar[0] = topo;
+    for (int m = 0; m < 3; m++)
+        if (fabs(qvar[m]) < 1e-99) qvar[m] = 0.0;
+#undef GQ
+#undef GA
+}
+
+/* geoclaw_create_gauge_files_default (fc2d_geoclaw_gauges_default.c:172-200):
+ * the two header lines of gaugeNNNNN.txt, num_eqn = 4. */
+int gfo_gauge_header(char *out, int cap, int id, double xc, double yc)
+{
+    const int n = snprintf(out, (size_t)cap,
+                           "# gauge_id= %5d location=( %17.10e %17.10e ) num_eqn= %2d\n"
+                           "# Columns: level time h    hu    hv    eta\n", id, xc, yc, 4);
+    return (n < 0 || n >= cap) ? -1 : n;
+}
+
+/* One record of geoclaw_print_gauges_default (fc2d_geoclaw_gauges_default.c:
+ * 275-305): eta = h + b, 0 when |eta| < 1e-99. */
+int gfo_gauge_line(char *out, int cap, int level, double t, const double qvar[3], const double avar[1])
+{
+    double eta = qvar[0] + avar[0];
+    eta = fabs(eta) < 1e-99 ? 0 : eta;
+    const int n = snprintf(out, (size_t)cap, "%5d %15.7e %15.7e %15.7e %15.7e %15.7e\n", level, t, qvar[0],
+                           qvar[1], qvar[2], eta);
+    return (n < 0 || n >= cap) ? -1 : n;
 }

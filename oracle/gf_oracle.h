@@ -216,6 +216,37 @@ int gfo_brick_run_ex(const gfo_params *p, int mi, int mj, int mx, int my, int mb
                      gfo_update_fn fn, const void *ctx, int nthreads,
                      double *dt_out, double *cfl_out, double *mass_out);
 
+/* ---- ASCII frame output and gauges ---------------------------------------
+ * fc2d_geoclaw_output_ascii_fort.f and fc2d_geoclaw_gauges_default.c.  The
+ * writers append bytes to out (capacity cap) and return the number written,
+ * or -1 when cap is too small. */
+
+/* Fortran Ew.d edit descriptor (Flang): [-]0.<d digits>E+xx, or +xxx for a
+ * three-digit exponent, right-justified in w characters; Inf / -Inf / NaN
+ * right-justified.  Writes exactly w characters. */
+void gfo_format_E(char *out, double v, int w, int d);
+
+/* fort.tNNNN contents (fc2d_geoclaw_fort_write_header). */
+int gfo_frame_header(char *out, int cap, double time, int meqn, int maux, int ngrids, int mbc);
+
+/* One patch's block of fort.qNNNN (fc2d_geoclaw_fort_write_file); q and aux
+ * are the AoS patch arrays with ghosts, and |q| < 1e-99 is set to 0 in q
+ * itself, as the reference does. */
+int gfo_frame_patch(char *out, int cap, int mx, int my, int meqn, int maux, int mbc,
+                    double xlower, double ylower, double dx, double dy, double *q,
+                    const double *aux, int patch_num, int level, int blockno, int mpirank);
+
+/* fc2d_geoclaw_update_gauge (geoclaw_gauges_fort.f90): q and aux(1) at
+ * (xc, yc) of the patch. */
+void gfo_update_gauge(const gfo_params *p, int mx, int my, int mbc, int maux, double xlower,
+                      double ylower, double dx, double dy, const double *q, const double *aux,
+                      double xc, double yc, double qvar[3], double avar[1]);
+
+/* Gauge file header lines and one record line
+ * (geoclaw_create_gauge_files_default, geoclaw_print_gauges_default). */
+int gfo_gauge_header(char *out, int cap, int id, double xc, double yc);
+int gfo_gauge_line(char *out, int cap, int level, double t, const double qvar[3], const double avar[1]);
+
 #ifdef __cplusplus
 }
 #endif

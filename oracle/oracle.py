@@ -322,6 +322,73 @@ class CLib(_Lib):
         return fn(ct.byref(self._cp), ct.byref(r), mx, my, mbc, maux, ptr(xl), ptr(yl), dx, dy, t, qp, ap,
                   level, maxlevel, init_flag)
 
+    # --- ASCII frame output and gauges ---------------------------------------
+    def format_E(self, v, w, d) -> str:
+        fn = self.lib.gfo_format_E
+        fn.argtypes = [ct.c_char_p, ct.c_double, ct.c_int, ct.c_int]
+        fn.restype = None
+        buf = ct.create_string_buffer(w + 1)
+        fn(buf, v, w, d)
+        return buf.raw[:w].decode()
+
+    def frame_header(self, time, meqn, maux, ngrids, mbc) -> bytes:
+        fn = self.lib.gfo_frame_header
+        fn.argtypes = [ct.c_char_p, ct.c_int, ct.c_double] + [ct.c_int] * 4
+        fn.restype = ct.c_int
+        buf = ct.create_string_buffer(1024)
+        n = fn(buf, 1024, time, meqn, maux, ngrids, mbc)
+        assert n >= 0
+        return buf.raw[:n]
+
+    def frame_patch(self, q, aux, xlower, ylower, dx, dy, patch_num, level, blockno, mpirank) -> bytes:
+        """One patch's block of fort.qNNNN; q (my+2mbc, mx+2mbc, meqn) AoS is
+        modified in place (|q| < 1e-99 -> 0) as the reference does."""
+        ny, nx, meqn = q.shape
+        maux = aux.shape[-1]
+        mbc = 2
+        mx, my = nx - 2 * mbc, ny - 2 * mbc
+        fn = self.lib.gfo_frame_patch
+        fn.argtypes = [ct.c_char_p] + [ct.c_int] * 6 + [ct.c_double] * 4 + [_dp, _dp] + [ct.c_int] * 4
+        fn.restype = ct.c_int
+        cap = 512 + mx * my * 110 + my * 4
+        buf = ct.create_string_buffer(cap)
+        n = fn(buf, cap, mx, my, meqn, maux, mbc, xlower, ylower, dx, dy, ptr(q), ptr(aux), patch_num, level,
+               blockno, mpirank)
+        assert n >= 0
+        return buf.raw[:n]
+
+    def update_gauge(self, q, aux, xlower, ylower, dx, dy, xc, yc):
+        ny, nx, _ = q.shape
+        fn = self.lib.gfo_update_gauge
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 4 + [ct.c_double] * 4 + [_dp, _dp, ct.c_double, ct.c_double,
+                                                                          _dp, _dp]
+        fn.restype = None
+        qv = np.zeros(3)
+        av = np.zeros(1)
+        fn(ct.byref(self._cp), nx - 4, ny - 4, 2, aux.shape[-1], xlower, ylower, dx, dy, ptr(q), ptr(aux), xc, yc,
+           ptr(qv), ptr(av))
+        return qv, av
+
+    def gauge_header(self, gid, xc, yc) -> bytes:
+        fn = self.lib.gfo_gauge_header
+        fn.argtypes = [ct.c_char_p, ct.c_int, ct.c_int, ct.c_double, ct.c_double]
+        fn.restype = ct.c_int
+        buf = ct.create_string_buffer(256)
+        n = fn(buf, 256, gid, xc, yc)
+        assert n >= 0
+        return buf.raw[:n]
+
+    def gauge_line(self, level, t, qvar, avar) -> bytes:
+        fn = self.lib.gfo_gauge_line
+        fn.argtypes = [ct.c_char_p, ct.c_int, ct.c_int, ct.c_double, _dp, _dp]
+        fn.restype = ct.c_int
+        buf = ct.create_string_buffer(256)
+        qv = np.ascontiguousarray(qvar, dtype=np.float64)
+        av = np.ascontiguousarray(avar, dtype=np.float64)
+        n = fn(buf, 256, level, t, ptr(qv), ptr(av))
+        assert n >= 0
+        return buf.raw[:n]
+
     # --- brick driver -------------------------------------------------------
     def brick_run(self, brick: "Brick", nsteps, initial_dt, desired_cfl=0.9, max_cfl=1.0,
                   t0=0.0, update=None, nthreads=1):
@@ -505,6 +572,43 @@ class RefLib(_Lib):
         yl = np.ascontiguousarray(ylower, dtype=np.float64)
         return fn(mx, my, mbc, maux, ptr(xl), ptr(yl), dx, dy, t, *[ptr(a) for a in qs],
                   *[ptr(a) for a in auxs], level, maxlevel, init_flag)
+
+    def write_frame(self, workdir, iframe, time, patches, meqn=3, maux=1, mbc=2, ngrids=None, mpirank=0):
+        """Run the reference writers in workdir: the header, then each patch
+        (q, aux, xlower, ylower, dx, dy, patch_num, level, blockno) in order.
+        Returns (fort.t bytes, fort.q bytes).  The reference modifies q."""
+        hdr = self.lib.gfref_write_header
+        hdr.argtypes = [ct.c_int, ct.c_double] + [ct.c_int] * 4
+        hdr.restype = None
+        wf = self.lib.gfref_write_file
+        wf.argtypes = [ct.c_int] * 5 + [ct.c_double] * 4 + [_dp, _dp] + [ct.c_int] * 5
+        wf.restype = None
+        cwd = os.getcwd()
+        os.chdir(workdir)
+        try:
+            hdr(iframe, time, meqn, maux, len(patches) if ngrids is None else ngrids, mbc)
+            for (q, aux, xl, yl, dx, dy, num, level, blockno) in patches:
+                ny, nx, _ = q.shape
+                wf(nx - 2 * mbc, ny - 2 * mbc, meqn, maux, mbc, xl, yl, dx, dy, ptr(q), ptr(aux), iframe, num,
+                   level, blockno, mpirank)
+            tag = "%04d" % (iframe % 10000)
+            with open("fort.t" + tag, "rb") as f:
+                tb = f.read()
+            with open("fort.q" + tag, "rb") as f:
+                qb = f.read()
+        finally:
+            os.chdir(cwd)
+        return tb, qb
+
+    def update_gauge(self, q, aux, xlower, ylower, dx, dy, xc, yc):
+        ny, nx, _ = q.shape
+        fn = self.lib.gfref_update_gauge
+        fn.argtypes = [ct.c_int] * 4 + [ct.c_double] * 4 + [_dp, _dp, ct.c_double, ct.c_double, _dp, _dp]
+        fn.restype = None
+        qv = np.zeros(3)
+        av = np.zeros(1)
+        fn(nx - 4, ny - 4, 2, aux.shape[-1], xlower, ylower, dx, dy, ptr(q), ptr(aux), xc, yc, ptr(qv), ptr(av))
+        return qv, av
 
     def patch_update(self, mx, my, mbc, q, aux, dx, dy, t, dt, maux=1):
         fn = self.lib.gfref_patch_update

@@ -357,6 +357,37 @@ contains
         gfref_tag4coarsening = tag
     end function gfref_tag4coarsening
 
+    ! ASCII frame output (fc2d_geoclaw_output_ascii_fort.f): the header routine
+    ! writes fort.tNNNN and empties fort.qNNNN, the patch routine appends one
+    ! patch to fort.qNNNN, both in the process's working directory.  The patch
+    ! routine zeroes |q| < 1e-99 in q itself.
+    subroutine gfref_write_header(iframe, time, meqn, maux, ngrids, mbc) &
+            bind(C, name="gfref_write_header")
+        integer(c_int), value :: iframe, meqn, maux, ngrids, mbc
+        real(c_double), value :: time
+        call fc2d_geoclaw_fort_write_header(iframe, time, meqn, maux, ngrids, mbc)
+    end subroutine gfref_write_header
+
+    subroutine gfref_write_file(mx, my, meqn, maux, mbc, xlower, ylower, dx, dy, q, aux, &
+            iframe, patch_num, level, blockno, mpirank) bind(C, name="gfref_write_file")
+        integer(c_int), value :: mx, my, meqn, maux, mbc, iframe, patch_num, level, blockno, mpirank
+        real(c_double), value :: xlower, ylower, dx, dy
+        real(c_double) :: q(*), aux(*)
+        call fc2d_geoclaw_fort_write_file(mx, my, meqn, maux, mbc, xlower, ylower, dx, dy, &
+            q, aux, iframe, patch_num, level, blockno, mpirank)
+    end subroutine gfref_write_file
+
+    ! Gauge sample of one patch (geoclaw_gauges_fort.f90): q and bathymetry at
+    ! (xc, yc), bilinear unless a neighbouring depth is below dry_tolerance/10.
+    subroutine gfref_update_gauge(mx, my, mbc, maux, xlower, ylower, dx, dy, q, aux, &
+            xc, yc, qvar, avar) bind(C, name="gfref_update_gauge")
+        integer(c_int), value :: mx, my, mbc, maux
+        real(c_double), value :: xlower, ylower, dx, dy, xc, yc
+        real(c_double) :: q(*), aux(*), qvar(3), avar(*)
+        call fc2d_geoclaw_update_gauge(mx, my, mbc, 3, xlower, ylower, dx, dy, q, maux, aux, &
+            xc, yc, qvar, avar)
+    end subroutine gfref_update_gauge
+
     ! geoclaw_update (fc2d_cpucuda.cpp:362-406): b4step2, step2, MAXCFL_CAP,
     ! src2.  Signature matches gfo_update_fn so the C brick driver can run it.
     real(c_double) function gfref_patch_update(ctx, mx, my, mbc, maux, q, aux, &
