@@ -104,6 +104,8 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--output-patches", type=int, default=1024,
+                    help="patches of the final state written as one ASCII frame (0: skip)")
     ap.add_argument("--host-steps", type=int, default=2,
                     help="steps of the host-resident drop-in batch (gf_step2_batch over all patches)")
     ap.add_argument("--retune-every", type=int, default=100,
@@ -254,6 +256,41 @@ def main():
             f"patches, {steady / launches:.0f} of them steady), {achieved:.1f} GB/s algorithmic; "
             f"same trajectory as graph run: {same}")
 
+    # ---- output: one ASCII frame of 1,024 patches and 4,096 gauge samples ----
+    # of the final state (fort.qNNNN bytes in the reference's format, written
+    # to a scratch directory that is removed afterwards)
+    output = None
+    if args.output_patches > 0:
+        import shutil
+        import tempfile
+        info_o = pool.scenario(sid)
+        m = min(args.output_patches, layout.npatch)
+        d = tempfile.mkdtemp(prefix="gf_frame_")
+        try:
+            t0o = time.perf_counter()
+            pool.write_frame(d, 1, info_o["t"], first=0, n=m, buf=info_o["cur"], ngrids=layout.npatch)
+            wo = time.perf_counter() - t0o
+            nbytes = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        rng = np.random.default_rng(0)
+        ng = 4096
+        gs = rng.integers(0, layout.npatch, ng)
+        ox, oy = layout.origins()
+        gx = ox[gs] + rng.uniform(0, MX, ng) * layout.dx
+        gy = oy[gs] + rng.uniform(0, MX, ng) * layout.dy
+        pool.sample_gauges(gs, gx, gy, buf=info_o["cur"])
+        t0g = time.perf_counter()
+        for _ in range(10):
+            pool.sample_gauges(gs, gx, gy, buf=info_o["cur"])
+        wg = (time.perf_counter() - t0g) / 10
+        output = dict(frame_patches=m, frame_bytes=nbytes, frame_seconds=round(wo, 4),
+                      frame_MB_per_s=round(nbytes / wo / 1e6, 1), gauges=ng, gauge_sample_ms=round(1e3 * wg, 4),
+                      note="gf_pool_write_frame (device pack and flush, D2H, formatting on host threads) and "
+                           "gf_pool_sample_gauges (one launch, host round trip included)")
+        log(f"output: {m} patches -> {nbytes / 1e6:.1f} MB in {wo:.3f} s ({nbytes / wo / 1e6:.0f} MB/s); "
+            f"{ng} gauges in {1e3 * wg:.3f} ms")
+
     # ---- ensemble serving sample (continuous batching of flood scenarios) ----
     ensemble = None
     if args.ensemble_requests > 0:
@@ -365,6 +402,7 @@ def main():
         },
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "host_batch": hostbatch,
+        "output": output,
         "plain": plain,
         "roofline": roofline,
         "step_composite": composite,

@@ -96,6 +96,11 @@ PROTOTYPES = {
     "gf_last_timing": (ct.c_int, [_dp, _dp, _ip]),
     "gf_pool_stats": (ct.c_int, [ct.c_void_p, ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong),
                                  ct.POINTER(ct.c_longlong), ct.POINTER(ct.c_longlong)]),
+    "gf_pool_write_frame": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_char_p, ct.c_int,
+                                       ct.c_double, ct.c_int, ct.c_int, _ip, _ip, _ip, ct.c_int]),
+    "gf_pool_sample_gauges": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip, _dp, _dp, _dp]),
+    "gf_gauge_file_create": (ct.c_int, [ct.c_char_p, ct.c_int, ct.c_double, ct.c_double]),
+    "gf_gauge_file_append": (ct.c_int, [ct.c_char_p, ct.c_int, ct.c_int, _ip, _dp, _dp]),
     "gf_plugin_create": (ct.c_void_p, [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int]),
     "gf_plugin_destroy": (ct.c_int, [ct.c_void_p]),
     "gf_plugin_single_step_update": (ct.c_int, [ct.c_void_p, _dp, _dp, ct.c_double, ct.c_double,

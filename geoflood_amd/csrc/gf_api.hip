@@ -13,10 +13,12 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/geoflood_amd.h"
 #include "gf_launch.h"
+#include "gf_output.h"
 
 using gf::PatchDesc;
 using gf::ScenCtrl;
@@ -1321,6 +1323,170 @@ int gf_last_timing(double* total_ms, double* step_ms, int* step_launches)
     if (total_ms) *total_ms = g.last_total_ms;
     if (step_ms) *step_ms = g.last_step_ms;
     if (step_launches) *step_launches = g.last_launches;
+    return GF_OK;
+}
+
+// ---- output: ASCII frames and gauges ----------------------------------------
+
+// Per-pool device scratch of the tagging and gauge calls, grown on demand.
+static int ensure_scratch(gf_pool* p, size_t bytes)
+{
+    if (bytes <= p->tag_bytes) return GF_OK;
+    if (p->tag_buf) HIPCHK(hipFree(p->tag_buf));
+    p->tag_buf = nullptr;
+    p->tag_bytes = 0;
+    HIPCHK(hipMalloc(&p->tag_buf, bytes));
+    p->tag_bytes = bytes;
+    return GF_OK;
+}
+
+static std::string join_path(const char* dir, const char* name)
+{
+    std::string s(dir);
+    if (!s.empty() && s.back() != '/') s.push_back('/');
+    return s + name;
+}
+
+int gf_pool_write_frame(gf_pool* p, int buf, int first, int n, const char* dir, int frame, double time,
+                        int ngrids, int write_header, const int* patch_num, const int* level, const int* blockno,
+                        int mpirank)
+{
+    if (!p || !dir) return fail(GF_EINVAL, "pool or dir is NULL");
+    if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
+    if (first < 0 || n < 0 || first + n > p->capacity) return fail(GF_EINVAL, "slots [%d,%d) outside the pool", first, first + n);
+    if (frame < 0) return fail(GF_EINVAL, "frame must be >= 0");
+    for (int k = 0; k < n; k++)
+        if (!(p->h_desc[first + k].dx > 0.0)) return fail(GF_EINVAL, "slot %d is not described", first + k);
+    char qname[32], tname[32];
+    std::snprintf(qname, sizeof qname, "fort.q%04d", frame % 10000);
+    std::snprintf(tname, sizeof tname, "fort.t%04d", frame % 10000);
+    const std::string qpath = join_path(dir, qname);
+    if (write_header) {
+        std::string h;
+        gf::ascii::frame_header(h, time, 3, 1, ngrids, p->mbc);
+        FILE* f = std::fopen(join_path(dir, tname).c_str(), "wb");
+        if (!f) return fail(GF_EIO, "cannot write %s", join_path(dir, tname).c_str());
+        const bool ok = std::fwrite(h.data(), 1, h.size(), f) == h.size();
+        if (std::fclose(f) != 0 || !ok) return fail(GF_EIO, "cannot write %s", tname);
+        f = std::fopen(qpath.c_str(), "wb");
+        if (!f || std::fclose(f) != 0) return fail(GF_EIO, "cannot write %s", qpath.c_str());
+    }
+    FILE* fq = std::fopen(qpath.c_str(), "ab");
+    if (!fq) return fail(GF_EIO, "cannot write %s", qpath.c_str());
+    const int cells = p->mx * p->my;
+    const size_t chunk = std::min<size_t>(kChunk, (size_t)p->capacity);  // d_stage holds >= 4 * cells per patch
+    const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::string> text(chunk);
+    int rc = GF_OK;
+    for (int off = 0; off < n && rc == GF_OK; off += (int)chunk) {
+        const int m = std::min<int>((int)chunk, n - off);
+        hipError_t e = gf::launch_frame_pack(p->mx, p->my, p->q[buf], p->aux, first + off, m, p->d_stage, g.stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(p->h_stage, p->d_stage, sizeof(double) * (size_t)m * cells * 4, hipMemcpyDeviceToHost,
+                               g.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+        if (e != hipSuccess) {
+            rc = fail(GF_EHIP, "frame pack: %s", hipGetErrorString(e));
+            break;
+        }
+        auto work = [&](int t) {
+            for (int k = t; k < m; k += nthreads) {
+                const int s = first + off + k;
+                const PatchDesc& d = p->h_desc[s];
+                std::string& out = text[k];
+                out.clear();
+                out.reserve(400 + (size_t)cells * 105 + (size_t)p->my * 3);
+                gf::ascii::frame_patch(out, p->h_stage + (size_t)k * cells * 4, p->mx, p->my, d.xlo, d.ylo, d.dx,
+                                       d.dy, patch_num ? patch_num[off + k] : s, level ? level[off + k] : 0,
+                                       blockno ? blockno[off + k] : 0, mpirank);
+            }
+        };
+        std::vector<std::thread> pool_threads;
+        for (int t = 1; t < std::min(nthreads, m); t++) pool_threads.emplace_back(work, t);
+        work(0);
+        for (auto& th : pool_threads) th.join();
+        for (int k = 0; k < m; k++)
+            if (std::fwrite(text[k].data(), 1, text[k].size(), fq) != text[k].size()) {
+                rc = fail(GF_EIO, "cannot write %s", qpath.c_str());
+                break;
+            }
+    }
+    if (std::fclose(fq) != 0 && rc == GF_OK) rc = fail(GF_EIO, "cannot write %s", qpath.c_str());
+    // the flush below 1e-99 may have changed the state
+    if (int r = reset_dry_age(p, first, n)) return r;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return rc;
+}
+
+int gf_pool_sample_gauges(gf_pool* p, int buf, int n, const int* slot, const double* xc, const double* yc,
+                          double* out)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
+    if (n < 0) return fail(GF_EINVAL, "n must be >= 0");
+    if (n == 0) return GF_OK;
+    if (!slot || !xc || !yc || !out) return fail(GF_EINVAL, "NULL argument");
+    for (int k = 0; k < n; k++) {
+        if (slot[k] < 0 || slot[k] >= p->capacity) return fail(GF_EINVAL, "gauge %d: slot %d outside the pool", k, slot[k]);
+        const PatchDesc& d = p->h_desc[slot[k]];
+        if (!(d.dx > 0.0)) return fail(GF_EINVAL, "gauge %d: slot %d is not described", k, slot[k]);
+        if (!(xc[k] >= d.xlo && xc[k] <= d.xlo + p->mx * d.dx && yc[k] >= d.ylo && yc[k] <= d.ylo + p->my * d.dy))
+            return fail(GF_EINVAL, "gauge %d at (%g, %g) is outside patch %d", k, xc[k], yc[k], slot[k]);
+    }
+    const size_t o_xy = ((sizeof(int) * (size_t)n + 255) / 256) * 256;
+    const size_t o_out = o_xy + sizeof(double) * 2 * (size_t)n;
+    const size_t bytes = o_out + sizeof(double) * 4 * (size_t)n;
+    if (int r = ensure_scratch(p, bytes)) return r;
+    std::vector<double> xy(2 * (size_t)n);
+    for (int k = 0; k < n; k++) {
+        xy[2 * k] = xc[k];
+        xy[2 * k + 1] = yc[k];
+    }
+    char* d = (char*)p->tag_buf;
+    HIPCHK(hipMemcpyAsync(d, slot, sizeof(int) * n, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(d + o_xy, xy.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, g.stream));
+    gf::GaugeArgs a{};
+    a.desc = p->desc;
+    a.q = p->q[buf];
+    a.aux = p->aux;
+    a.slot = (const int*)d;
+    a.xy = (const double*)(d + o_xy);
+    a.out = (double*)(d + o_out);
+    a.n = n;
+    a.dry_tolerance = g.P.dry_tolerance;
+    HIPCHK(gf::launch_gauges(p->mx, p->my, a, g.stream));
+    HIPCHK(hipMemcpyAsync(out, d + o_out, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+}
+
+int gf_gauge_file_create(const char* dir, int id, double xc, double yc)
+{
+    if (!dir) return fail(GF_EINVAL, "dir is NULL");
+    char name[32];
+    std::snprintf(name, sizeof name, "gauge%05d.txt", id);
+    std::string s;
+    gf::ascii::gauge_header(s, id, xc, yc);
+    FILE* f = std::fopen(join_path(dir, name).c_str(), "wb");
+    if (!f) return fail(GF_EIO, "cannot write %s", name);
+    const bool ok = std::fwrite(s.data(), 1, s.size(), f) == s.size();
+    if (std::fclose(f) != 0 || !ok) return fail(GF_EIO, "cannot write %s", name);
+    return GF_OK;
+}
+
+int gf_gauge_file_append(const char* dir, int id, int n, const int* level, const double* t, const double* rec)
+{
+    if (!dir) return fail(GF_EINVAL, "dir is NULL");
+    if (n < 0 || (n > 0 && (!level || !t || !rec))) return fail(GF_EINVAL, "invalid records");
+    char name[32];
+    std::snprintf(name, sizeof name, "gauge%05d.txt", id);
+    std::string s;
+    for (int k = 0; k < n; k++)
+        gf::ascii::gauge_line(s, level[k], t[k], rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
+    FILE* f = std::fopen(join_path(dir, name).c_str(), "ab");
+    if (!f) return fail(GF_EIO, "cannot write %s", name);
+    const bool ok = std::fwrite(s.data(), 1, s.size(), f) == s.size();
+    if (std::fclose(f) != 0 || !ok) return fail(GF_EIO, "cannot write %s", name);
     return GF_OK;
 }
 

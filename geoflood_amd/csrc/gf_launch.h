@@ -36,6 +36,19 @@ struct GhostArgs {
     int cur;         // otherwise this buffer
 };
 
+// Gauge samples: gauge g at (xy[2g], xy[2g+1]) inside patch slot[g] of
+// buffer q; out[4g..4g+3] = h, hu, hv, b there.
+struct GaugeArgs {
+    const PatchDesc* desc;
+    const double* q;    // the state buffer sampled
+    const double* aux;
+    const int* slot;
+    const double* xy;
+    double* out;
+    int n;
+    double dry_tolerance;
+};
+
 // Hydrograph inflow ghost cells of the patches in list[0 .. *list_n) that lie
 // in [first, last): each patch's ghost ring is rebuilt in its q buffer (and
 // aux) by ForestClaw's sequence -- same-level copies, then bc2 sides left,
@@ -138,6 +151,9 @@ bool default_method(const Params& P);
 // Active tiles then dry patches from one persistent kernel (see k_all).
 hipError_t launch_all(int mx, int my, int variant, bool def, const BatchArgs* da, int grid, hipStream_t st);
 hipError_t launch_batch_control(const BatchArgs& a, hipStream_t st);
+hipError_t launch_frame_pack(int mx, int my, double* q, const double* aux, int first, int n, double* out,
+                             hipStream_t st);
+hipError_t launch_gauges(int mx, int my, const GaugeArgs& a, hipStream_t st);
 hipError_t launch_hmax(int mx, int my, const double* q, double* hmax, int first, int n, hipStream_t st);
 hipError_t launch_step(int mx, int my, int variant, const StepArgs& a, hipStream_t st);
 hipError_t launch_ghost(int mx, int my, const GhostArgs& a, hipStream_t st);

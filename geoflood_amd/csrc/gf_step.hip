@@ -1211,6 +1211,104 @@ __global__ void __launch_bounds__(256) k_hmax(const double* q, double* hmax, int
 }
 
 // ---------------------------------------------------------------------------
+// ASCII frame output, device part (fc2d_geoclaw_fort_write_file,
+// fc2d_geoclaw_output_ascii_fort.f:83-108): per interior cell in row order
+// (j outer, i inner) h, hu, hv with |value| < 1e-99 set to +0 -- in the
+// state as well, as the reference does to the patch it writes -- and
+// eta = h + b, 0 when |eta| < 1e-99.  out holds n patches of MX*MY*4 values.
+// ---------------------------------------------------------------------------
+template <int MX, int MY>
+__global__ void __launch_bounds__(256) k_frame_pack(double* q, const double* aux, int first, int n, double* out)
+{
+    using G = Geo<MX, MY>;
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    double* qp = q + (size_t)(first + b) * 3 * G::NP2;
+    const double* ap = aux + (size_t)(first + b) * G::NP2;
+    double* o = out + (size_t)b * MX * MY * 4;
+    for (int c = threadIdx.x; c < MX * MY; c += 256) {
+        const int k = G::ti(c % MX + 1, c / MX + 1);
+        double v[3];
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            v[m] = qp[m * G::NP2 + k];
+            if (fabs(v[m]) < 1e-99) {
+                v[m] = 0.0;
+                qp[m * G::NP2 + k] = 0.0;
+            }
+        }
+        double eta = v[0] + ap[k];
+        if (fabs(eta) < 1e-99) eta = 0.0;
+        o[4 * c] = v[0];
+        o[4 * c + 1] = v[1];
+        o[4 * c + 2] = v[2];
+        o[4 * c + 3] = eta;
+    }
+}
+
+// Gauge sample (fc2d_geoclaw_update_gauge, geoclaw_gauges_fort.f90:1-77):
+// bilinear in the four cells around the gauge unless one of their depths is
+// below dry_tolerance/10, then the containing cell; |q| < 1e-99 -> 0.  Cells
+// outside the interior (a gauge in the last row or column) take the value a
+// ghost fill would give them: the neighbour's interior, the wall mirror or
+// the extrapolation, or the stored ghost cell.  b is the patch's aux.
+template <int MX, int MY>
+__device__ __forceinline__ double gauge_cell(const GaugeArgs& a, int patch, int fi, int fj, int m)
+{
+    using G = Geo<MX, MY>;
+    int sp = patch, si = fi, sj = fj;
+    bool nu = false, nv = false;
+    const bool interior = fi >= 1 && fi <= MX && fj >= 1 && fj <= MY;
+    if (!interior && !ghost_source<MX, MY>(a.desc[patch].nbr, patch, fi, fj, sp, si, sj, nu, nv)) {
+        sp = patch; si = fi; sj = fj;
+    }
+    const double v = a.q[(size_t)sp * 3 * G::NP2 + m * G::NP2 + G::ti(si, sj)];
+    return (m == 1 && nu) || (m == 2 && nv) ? -v : v;
+}
+
+template <int MX, int MY>
+__global__ void __launch_bounds__(64) k_gauges(GaugeArgs a)
+{
+    using G = Geo<MX, MY>;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.n) return;
+    const int patch = a.slot[g];
+    const PatchDesc& d = a.desc[patch];
+    const double xc = a.xy[2 * g], yc = a.xy[2 * g + 1];
+    const int ii = (int)((xc - d.xlo) / d.dx) + 1;
+    const int jj = (int)((yc - d.ylo) / d.dy) + 1;
+    const double xcent = d.xlo + (ii - .5) * d.dx;
+    const double ycent = d.ylo + (jj - .5) * d.dy;
+    const double xoff = (xc - xcent) / d.dx;
+    const double yoff = (yc - ycent) / d.dy;
+    const double drytol2 = 0.1 * a.dry_tolerance;
+    const double* ap = a.aux + (size_t)patch * G::NP2;
+    double qv[3], topo;
+    if (gauge_cell<MX, MY>(a, patch, ii, jj, 0) < drytol2 || gauge_cell<MX, MY>(a, patch, ii + 1, jj, 0) < drytol2 ||
+        gauge_cell<MX, MY>(a, patch, ii, jj + 1, 0) < drytol2 ||
+        gauge_cell<MX, MY>(a, patch, ii + 1, jj + 1, 0) < drytol2) {
+#pragma unroll
+        for (int m = 0; m < 3; m++) qv[m] = gauge_cell<MX, MY>(a, patch, ii, jj, m);
+        topo = ap[G::ti(ii, jj)];
+    } else {
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+            qv[m] = (1.0 - xoff) * (1.0 - yoff) * gauge_cell<MX, MY>(a, patch, ii, jj, m) +
+                    xoff * (1.0 - yoff) * gauge_cell<MX, MY>(a, patch, ii + 1, jj, m) +
+                    (1.0 - xoff) * yoff * gauge_cell<MX, MY>(a, patch, ii, jj + 1, m) +
+                    xoff * yoff * gauge_cell<MX, MY>(a, patch, ii + 1, jj + 1, m);
+        topo = (1.0 - xoff) * (1.0 - yoff) * ap[G::ti(ii, jj)] + xoff * (1.0 - yoff) * ap[G::ti(ii + 1, jj)] +
+               (1.0 - xoff) * yoff * ap[G::ti(ii, jj + 1)] + xoff * yoff * ap[G::ti(ii + 1, jj + 1)];
+    }
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        if (fabs(qv[m]) < 1e-99) qv[m] = 0.0;
+        a.out[4 * g + m] = qv[m];
+    }
+    a.out[4 * g + 3] = topo;
+}
+
+// ---------------------------------------------------------------------------
 // Pack / unpack between the reference's host layout (AoS q(m,i,j) with ghost
 // cells, data_swap.f90:7-54) and the pool's SoA layout, one workgroup per
 // patch.  ``stage`` holds n consecutive AoS patches.
@@ -1431,6 +1529,34 @@ hipError_t launch_all(int mx, int my, int variant, bool def, const BatchArgs* da
 hipError_t launch_batch_control(const BatchArgs& a, hipStream_t st)
 {
     k_batch_control<<<(a.nscen + 63) / 64 > 0 ? (a.nscen + 63) / 64 : 1, 64, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_frame_pack(int mx, int my, double* q, const double* aux, int first, int n, double* out,
+                             hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    switch (mx * 1000 + my) {
+    case 32032: k_frame_pack<32, 32><<<n, 256, 0, st>>>(q, aux, first, n, out); break;
+    case 16016: k_frame_pack<16, 16><<<n, 256, 0, st>>>(q, aux, first, n, out); break;
+    case 8008: k_frame_pack<8, 8><<<n, 256, 0, st>>>(q, aux, first, n, out); break;
+    case 64064: k_frame_pack<64, 64><<<n, 256, 0, st>>>(q, aux, first, n, out); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_gauges(int mx, int my, const GaugeArgs& a, hipStream_t st)
+{
+    if (a.n <= 0) return hipSuccess;
+    const int grid = (a.n + 63) / 64;
+    switch (mx * 1000 + my) {
+    case 32032: k_gauges<32, 32><<<grid, 64, 0, st>>>(a); break;
+    case 16016: k_gauges<16, 16><<<grid, 64, 0, st>>>(a); break;
+    case 8008: k_gauges<8, 8><<<grid, 64, 0, st>>>(a); break;
+    case 64064: k_gauges<64, 64><<<grid, 64, 0, st>>>(a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

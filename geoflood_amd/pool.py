@@ -171,6 +171,17 @@ class Pool:
         check(self.lib.gf_pool_set_patches(self.h, first, dptr(dxa), dptr(dya), dptr(xla), dptr(yla),
                                            iptr(sc), iptr(nba)), "gf_pool_set_patches")
 
+    def set_patches(self, dx, dy, xlo, ylo, nbr=None):
+        """Describe slots [0, n) one by one (gf_pool_set_patches); nbr (n, 8)
+        neighbour codes, None: every ghost cell is supplied by the host."""
+        d = [np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1)) for a in (dx, dy, xlo, ylo)]
+        n = len(d[0])
+        nb = None if nbr is None else np.ascontiguousarray(np.asarray(nbr, dtype=np.int32).reshape(n, 8))
+        check(self.lib.gf_pool_set_patches(self.h, n, *[dptr(a) for a in d], None,
+                                           None if nb is None else iptr(nb)), "gf_pool_set_patches")
+        self.layouts = []
+        self.n = n
+
     def describe(self, layout: BrickLayout, first: int):
         """Describe one brick at pool slots [first, first+npatch) without
         touching the other slots (gf_pool_describe)."""
@@ -312,6 +323,35 @@ class Pool:
                                            len(fm), iptr(out)), "gf_pool_tag_coarsen")
         return out
 
+    # -- output: ASCII frames and gauges -----------------------------------------
+    def write_frame(self, directory: str, frame: int, time: float, first: int = 0, n: int | None = None,
+                    buf: int = 0, ngrids: int | None = None, header: bool = True, patch_num=None, level=None,
+                    blockno=None, mpirank: int = 0):
+        """Append patches [first, first+n) of buffer buf to directory/fort.qNNNN
+        in the reference's format (and, with header, write fort.tNNNN and empty
+        fort.qNNNN first); gf_pool_write_frame."""
+        n = self.n - first if n is None else n
+
+        def ints(a):
+            return None if a is None else np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(n))
+        pn, lv, bn = ints(patch_num), ints(level), ints(blockno)
+        check(self.lib.gf_pool_write_frame(self.h, buf, first, n, directory.encode(), frame, time,
+                                           n if ngrids is None else ngrids, int(header),
+                                           iptr(pn) if pn is not None else None,
+                                           iptr(lv) if lv is not None else None,
+                                           iptr(bn) if bn is not None else None, mpirank),
+              "gf_pool_write_frame")
+
+    def sample_gauges(self, slots, xc, yc, buf: int = 0) -> np.ndarray:
+        """(n, 4) array of h, hu, hv, b at the gauges (gf_pool_sample_gauges)."""
+        sl = np.ascontiguousarray(np.asarray(slots, dtype=np.int32).reshape(-1))
+        x = np.ascontiguousarray(np.asarray(xc, dtype=np.float64).reshape(-1))
+        y = np.ascontiguousarray(np.asarray(yc, dtype=np.float64).reshape(-1))
+        out = np.zeros((len(sl), 4))
+        check(self.lib.gf_pool_sample_gauges(self.h, buf, len(sl), iptr(sl), dptr(x), dptr(y), dptr(out)),
+              "gf_pool_sample_gauges")
+        return out
+
     # -- step variants ------------------------------------------------------------
     def variants(self):
         """(current variant, [(tile_x, tile_y, threads), ...])."""
@@ -342,6 +382,20 @@ class Pool:
         act = ct.c_int(0)
         check(self.lib.gf_pool_run_plain(self.h, substeps, ct.byref(act)), "gf_pool_run_plain")
         return bool(act.value)
+
+
+def gauge_file_create(directory: str, gid: int, xc: float, yc: float):
+    """Write the header of directory/gaugeNNNNN.txt (gf_gauge_file_create)."""
+    check(load().gf_gauge_file_create(directory.encode(), gid, xc, yc), "gf_gauge_file_create")
+
+
+def gauge_file_append(directory: str, gid: int, level, t, rec):
+    """Append records (level, t, h hu hv b rows of rec) to directory/gaugeNNNNN.txt."""
+    lv = np.ascontiguousarray(np.asarray(level, dtype=np.int32).reshape(-1))
+    tt = np.ascontiguousarray(np.asarray(t, dtype=np.float64).reshape(-1))
+    r = np.ascontiguousarray(np.asarray(rec, dtype=np.float64).reshape(-1, 4))
+    check(load().gf_gauge_file_append(directory.encode(), gid, len(lv), iptr(lv), dptr(tt), dptr(r)),
+          "gf_gauge_file_append")
 
 
 def step2_batch(q_list, aux_list, dx, dy, t, dt):

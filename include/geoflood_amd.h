@@ -32,7 +32,8 @@ enum {
     GF_EHIP = -2,         /* HIP runtime error */
     GF_ENOMEM = -3,       /* device or pinned host allocation failed */
     GF_EUNSUPPORTED = -4, /* configuration outside the supported set */
-    GF_ESTATE = -5        /* call out of order (e.g. no gf_init) */
+    GF_ESTATE = -5,       /* call out of order (e.g. no gf_init) */
+    GF_EIO = -6           /* an output file could not be written */
 };
 
 /* Neighbour codes for gf_pool_set_patches (values >= 0 are patch indices):
@@ -359,6 +360,46 @@ int gf_last_timing(double *total_ms, double *step_ms, int *step_launches);
  * pool. */
 int gf_pool_stats(gf_pool *pool, long long *active_patches, long long *dry_patches,
                   long long *attempts, long long *steady_patches);
+
+/* ---- Output: ASCII frames and gauges --------------------------------------
+ * Replaces fc2d_geoclaw_output_ascii (fc2d_geoclaw_output_ascii.c:39-114)
+ * with its Fortran writers fc2d_geoclaw_fort_write_header / _write_file
+ * (fc2d_geoclaw_output_ascii_fort.f:1-117), for device-resident patches.
+ *
+ * Appends patches [first, first+n) of buffer buf to dir/fort.qNNNN (NNNN =
+ * frame mod 10000) in the reference's byte format; with write_header, first
+ * writes dir/fort.tNNNN (time, meqn+1 = 4, ngrids, maux = 1, 2, mbc) and
+ * empties fort.qNNNN, as the reference's header routine does.  Per patch:
+ * patch_num[k] (NULL: first + k), level[k] and blockno[k] (NULL: 0), mpirank,
+ * mx, my, xlower, ylower, dx, dy (from gf_pool_set_patches /
+ * gf_pool_describe), then h, hu, hv, eta per interior cell.  As the reference
+ * does to the patch it writes, values with |q| < 1e-99 are set to 0 in the
+ * pool's state.  GF_EIO when a file cannot be written. */
+int gf_pool_write_frame(gf_pool *pool, int buf, int first, int n, const char *dir, int frame,
+                        double time, int ngrids, int write_header, const int *patch_num,
+                        const int *level, const int *blockno, int mpirank);
+
+/* Gauge samples of buffer buf: gauge k at (xc[k], yc[k]) inside patch slot[k]
+ * (xlower <= xc <= xlower + mx*dx, likewise y, as the reference asserts,
+ * fc2d_geoclaw_gauges_default.c:250-251); out[4k..4k+3] = h, hu, hv, b by
+ * fc2d_geoclaw_update_gauge (geoclaw_gauges_fort.f90:1-77): bilinear in the
+ * four cells around the gauge, or the containing cell when one of their
+ * depths is below dry_tolerance/10.  A cell beyond the last interior row or
+ * column takes the value a ghost fill would give it (the neighbour's
+ * interior, wall mirror or extrapolation; the stored ghost for host-filled
+ * sides), where the reference reads the ghost cells its step left. */
+int gf_pool_sample_gauges(gf_pool *pool, int buf, int n, const int *slot, const double *xc,
+                          const double *yc, double *out);
+
+/* Gauge files in the reference's format (geoclaw_create_gauge_files_default
+ * and geoclaw_print_gauges_default, fc2d_geoclaw_gauges_default.c:172-200,
+ * :275-305): gf_gauge_file_create writes the two header lines of
+ * dir/gaugeNNNNN.txt (truncating it); gf_gauge_file_append appends n records
+ * "level t h hu hv eta" (eta = h + b, 0 below 1e-99) from rec[4k..4k+3] =
+ * h, hu, hv, b as gf_pool_sample_gauges returns them. */
+int gf_gauge_file_create(const char *dir, int id, double xc, double yc);
+int gf_gauge_file_append(const char *dir, int id, int n, const int *level, const double *t,
+                         const double *rec);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,7 @@ compiled by oracle/Makefile (target ``ref``):
   header and a sequence of patches of several shapes whose values include
   the format's edge cases (signed zeros, values at and below the 1e-99
   flush, three-digit exponents, rounding ties, Inf/NaN in the geometry);
-* gauge: q and bathymetry sampled at random points of random 4x4 patches
+* gauge: q and bathymetry sampled at random points of random 8x8 patches
   (interpolated, and the dry fallback).
 
     make -C oracle ref && python tests/golden/make_golden_output.py
@@ -76,8 +76,8 @@ def main():
     rec["frame_time"] = np.array([TIME])
     rec["frame_t_bytes"] = np.frombuffer(tb, dtype=np.uint8)
     rec["frame_q_bytes"] = np.frombuffer(qb, dtype=np.uint8)
-    # gauges: 4x4 patches, values on a 1/64 grid, a fifth of the depths dry
-    n, mx = 160, 4
+    # gauges: 8x8 patches, values on a 1/64 grid, some depths dry
+    n, mx = 96, 8
     gq = np.round(rng.normal(0, 1, (n, mx + 4, mx + 4, 3)) * 64) / 64
     gq[..., 0] = np.abs(gq[..., 0])
     dry = rng.random((n, mx + 4, mx + 4)) < 0.08
