@@ -31,27 +31,31 @@ __device__ __forceinline__ double fminf64(double a, double b) { return a < b ? a
 
 // riemanntype (geoclaw_riemann_utils.f:566-677) with maxiter = 1; returns
 // hm and the middle-state speeds s1m, s2m (rare1/rare2 are unused by callers).
+// sgL, sgR are sqrt(g*hL), sqrt(g*hR), which the caller has already formed:
+// every sqrt(g*h) of hL, hR, h_min or h_max below is one of them, bit for bit.
 __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, double uR,
-                                            double drytol, double g,
+                                            double sgL, double sgR, double drytol, double g,
                                             double& hm, double& s1m, double& s2m)
 {
     const double h_min = fminf64(hR, hL);
     const double h_max = fmaxf64(hR, hL);
+    const double sg_min = hR < hL ? sgR : sgL;  // sqrt(g * h_min)
+    const double sg_max = hR > hL ? sgR : sgL;  // sqrt(g * h_max)
     const double delu = uR - uL;
     if (h_min <= drytol) {
         hm = 0.0;
-        s1m = uR + uL - 2.0 * sqrt(g * hR) + 2.0 * sqrt(g * hL);
+        s1m = uR + uL - 2.0 * sgR + 2.0 * sgL;
         s2m = s1m;
         return;
     }
-    const double F_min = delu + 2.0 * (sqrt(g * h_min) - sqrt(g * h_max));
+    const double F_min = delu + 2.0 * (sg_min - sg_max);
     const double F_max = delu + (h_max - h_min) * (sqrt(.5 * g * (h_max + h_min) / (h_max * h_min)));
     if (F_min > 0.0) {  // two rarefactions
-        const double t = fmaxf64(0.0, -delu + 2.0 * (sqrt(g * hL) + sqrt(g * hR)));
+        const double t = fmaxf64(0.0, -delu + 2.0 * (sgL + sgR));
         hm = (1.0 / (16.0 * g)) * (t * t);
         const double sghm = sqrt(g * hm);
-        s1m = uL + 2.0 * sqrt(g * hL) - 3.0 * sghm;
-        s2m = uR - 2.0 * sqrt(g * hR) + 3.0 * sghm;
+        s1m = uL + 2.0 * sgL - 3.0 * sghm;
+        s2m = uR - 2.0 * sgR + 3.0 * sghm;
     } else if (F_max <= 0.0) {  // two shocks: one Newton step on sqrt(h)
         double h0 = h_max;
         const double gL = sqrt(.5 * g * (1 / h0 + 1 / hL));
@@ -69,19 +73,23 @@ __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, dou
         s1m = u1m - sghm;
         s2m = u2m + sghm;
     } else {  // one shock, one rarefaction
+        // F0 = delu + 2(sqrt(g h0) - sqrt(g h_max)) + (h0 - h_min) sqrt(g/2 (1/h0 + 1/h_min))
+        // at h0 = h_min: the last product is (+0) times a finite positive
+        // whenever g/h_min cannot overflow, so F0 is F_min + 0.0 exactly
         double h0 = h_min;
-        const double F0 = delu + 2.0 * (sqrt(g * h0) - sqrt(g * h_max)) +
-                          (h0 - h_min) * sqrt(.5 * g * (1 / h0 + 1 / h_min));
+        const double F0 = (h_min >= 1e-150 && g <= 1e150)
+                              ? F_min + 0.0
+                              : delu + 2.0 * (sg_min - sg_max) + (h0 - h_min) * sqrt(.5 * g * (1 / h0 + 1 / h_min));
         const double slope = (F_max - F0) / (h_max - h_min);
         h0 = h0 - F0 / slope;
         hm = h0;
         const double sghm = sqrt(g * hm);
         if (hL > hR) {
-            const double a = uL + 2.0 * sqrt(g * hL);
+            const double a = uL + 2.0 * sgL;
             s1m = a - 3.0 * sghm;
             s2m = a - sghm;
         } else {
-            const double a = uR - 2.0 * sqrt(g * hR);
+            const double a = uR - 2.0 * sgR;
             s2m = a + 3.0 * sghm;
             s1m = a + sghm;
         }
@@ -126,7 +134,8 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
     double wall0 = 1.0, wall1 = 1.0, wall2 = 1.0;
     if (hR <= drytol) {
         double hstar, s1m, s2m;
-        riemanntype(hL, hL, uL, -uL, drytol, g, hstar, s1m, s2m);
+        const double sg = sqrt(g * hL);
+        riemanntype(hL, hL, uL, -uL, sg, sg, drytol, g, hstar, s1m, s2m);
         const double hstartest = fmaxf64(hL, hstar);
         if (hstartest + bL < bR) {
             wall1 = 0.0; wall2 = 0.0;
@@ -136,7 +145,8 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
         }
     } else if (hL <= drytol) {
         double hstar, s1m, s2m;
-        riemanntype(hR, hR, -uR, uR, drytol, g, hstar, s1m, s2m);
+        const double sg = sqrt(g * hR);
+        riemanntype(hR, hR, -uR, uR, sg, sg, drytol, g, hstar, s1m, s2m);
         const double hstartest = fmaxf64(hR, hstar);
         if (hstartest + bR < bL) {
             wall0 = 0.0; wall1 = 0.0;
@@ -164,7 +174,7 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
     const double delb = bR - bL;
 
     double hm, s1m, s2m;
-    riemanntype(hL, hR, uL, uR, drytol, g, hm, s1m, s2m);
+    riemanntype(hL, hR, uL, uR, sghL, sghR, drytol, g, hm, s1m, s2m);
 
     const double lam0 = fminf64(sE1, s2m);
     const double lam2 = fmaxf64(sE2, s1m);
