@@ -1460,6 +1460,129 @@ int gf_pool_sample_gauges(gf_pool* p, int buf, int n, const int* slot, const dou
     return GF_OK;
 }
 
+// ---- regrid and time interpolation between levels ----------------------------
+
+// The family lists in the pool's device scratch: n coarse slots, then 4n
+// child slots.
+static int regrid_lists(gf_pool* p, int n, const int* cs, const int* fs, int** dcs, int** dfs)
+{
+    const size_t o_fs = ((sizeof(int) * (size_t)n + 255) / 256) * 256;
+    const size_t bytes = o_fs + sizeof(int) * 4 * (size_t)n;
+    if (int r = ensure_scratch(p, bytes)) return r;
+    char* d = (char*)p->tag_buf;
+    HIPCHK(hipMemcpyAsync(d, cs, sizeof(int) * n, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(d + o_fs, fs, sizeof(int) * 4 * n, hipMemcpyHostToDevice, g.stream));
+    *dcs = (int*)d;
+    *dfs = (int*)(d + o_fs);
+    return GF_OK;
+}
+
+// Slots must be described, distinct, and (for interpolation) no child may be
+// a coarse patch of the call or a same-level neighbour that a coarse patch's
+// halo reads.
+static int regrid_check(gf_pool* p, int buf, int n, const int* cs, const int* fs, bool interp)
+{
+    if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
+    if (n < 0 || (n > 0 && (!cs || !fs))) return fail(GF_EINVAL, "invalid family lists");
+    if (p->mx % 2 || p->my % 2) return fail(GF_EUNSUPPORTED, "patch size must be even");
+    std::vector<char> role(p->capacity, 0);  // 1 coarse, 2 fine, 3 read by a coarse halo
+    for (int f = 0; f < n; f++) {
+        const int c = cs[f];
+        if (c < 0 || c >= p->capacity || !(p->h_desc[c].dx > 0.0)) return fail(GF_EINVAL, "family %d: coarse slot %d invalid", f, c);
+        if (role[c]) return fail(GF_EINVAL, "slot %d appears twice", c);
+        role[c] = 1;
+    }
+    for (int f = 0; f < n && interp; f++)
+        for (int s = 0; s < 8; s++) {
+            const int nb = p->h_desc[cs[f]].nbr[s];
+            if (nb >= 0 && !role[nb]) role[nb] = 3;
+        }
+    for (int k = 0; k < 4 * n; k++) {
+        const int c = fs[k];
+        if (c < 0 || c >= p->capacity || !(p->h_desc[c].dx > 0.0)) return fail(GF_EINVAL, "child %d: slot %d invalid", k, c);
+        if (role[c] == 3) return fail(GF_EINVAL, "child slot %d is read by a coarse patch's halo", c);
+        if (role[c]) return fail(GF_EINVAL, "slot %d appears twice", c);
+        role[c] = 2;
+    }
+    return GF_OK;
+}
+
+static int regrid_impl(gf_pool* p, int buf, int n, const int* cs, const int* fs, bool interp)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (int r = regrid_check(p, buf, n, cs, fs, interp)) return r;
+    if (n == 0) return GF_OK;
+    int *dcs = nullptr, *dfs = nullptr;
+    if (int r = regrid_lists(p, n, cs, fs, &dcs, &dfs)) return r;
+    gf::RegridArgs a{};
+    a.desc = p->desc;
+    a.q = p->q[buf];
+    a.aux = p->aux;
+    a.cs = dcs;
+    a.fs = dfs;
+    a.n = n;
+    a.dry_tolerance = g.P.dry_tolerance;
+    double* hm = p->hmax2 + (size_t)buf * p->capacity;
+    if (interp) {
+        HIPCHK(gf::launch_interpolate2fine(p->mx, p->my, a, g.stream));
+        HIPCHK(gf::launch_hmax_list(p->mx, p->my, p->q[buf], hm, p->dry_age, dfs, 4 * n, g.stream));
+    } else {
+        HIPCHK(gf::launch_average2coarse(p->mx, p->my, a, g.stream));
+        HIPCHK(gf::launch_hmax_list(p->mx, p->my, p->q[buf], hm, p->dry_age, dcs, n, g.stream));
+    }
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+}
+
+int gf_pool_average2coarse(gf_pool* p, int buf, int n, const int* coarse, const int* fine)
+{
+    return regrid_impl(p, buf, n, coarse, fine, false);
+}
+
+int gf_pool_interpolate2fine(gf_pool* p, int buf, int n, const int* coarse, const int* fine)
+{
+    return regrid_impl(p, buf, n, coarse, fine, true);
+}
+
+int gf_pool_timeinterp(gf_pool* p, int n, const int* src, int buf_last, int buf_curr, double alpha,
+                       const int* dst, int dst_buf)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if ((buf_last != 0 && buf_last != 1) || (buf_curr != 0 && buf_curr != 1) || (dst_buf != 0 && dst_buf != 1))
+        return fail(GF_EINVAL, "buffers must be 0 or 1");
+    if (n < 0 || (n > 0 && (!src || !dst))) return fail(GF_EINVAL, "invalid slot lists");
+    if (p->mx < 4 || p->my < 4) return fail(GF_EUNSUPPORTED, "patch too small");
+    std::vector<char> used(p->capacity, 0);
+    for (int k = 0; k < n; k++) {
+        if (src[k] < 0 || src[k] >= p->capacity || dst[k] < 0 || dst[k] >= p->capacity)
+            return fail(GF_EINVAL, "entry %d: slot outside the pool", k);
+        if (used[dst[k]]) return fail(GF_EINVAL, "destination slot %d appears twice", dst[k]);
+        used[dst[k]] = 1;
+    }
+    // a destination in a source buffer may not be another entry's source
+    if (dst_buf == buf_last || dst_buf == buf_curr)
+        for (int k = 0; k < n; k++)
+            if (used[src[k]] && src[k] != dst[k]) return fail(GF_EINVAL, "slot %d is both a source and another entry's destination", src[k]);
+    if (n == 0) return GF_OK;
+    int *dsrc = nullptr, *ddst = nullptr;
+    std::vector<int> dst4(4 * (size_t)n, 0);
+    for (int k = 0; k < n; k++) dst4[k] = dst[k];
+    if (int r = regrid_lists(p, n, src, dst4.data(), &dsrc, &ddst)) return r;
+    gf::TimeInterpArgs a{};
+    a.qlast = p->q[buf_last];
+    a.qcurr = p->q[buf_curr];
+    a.qdst = p->q[dst_buf];
+    a.src = dsrc;
+    a.dst = ddst;
+    a.n = n;
+    a.alpha = alpha;
+    HIPCHK(gf::launch_timeinterp(p->mx, p->my, a, g.stream));
+    HIPCHK(gf::launch_hmax_list(p->mx, p->my, p->q[dst_buf], p->hmax2 + (size_t)dst_buf * p->capacity, p->dry_age,
+                                ddst, n, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+}
+
 int gf_gauge_file_create(const char* dir, int id, double xc, double yc)
 {
     if (!dir) return fail(GF_EINVAL, "dir is NULL");

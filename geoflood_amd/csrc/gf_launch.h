@@ -49,6 +49,31 @@ struct GaugeArgs {
     double dry_tolerance;
 };
 
+// Regrid between levels on resident patches (refinement ratio 2): family f
+// has the coarse patch in slot cs[f] and its children in slots fs[4f + g],
+// g = igrid = 0..3 (x fastest), all in buffer q.
+struct RegridArgs {
+    const PatchDesc* desc;
+    double* q;
+    const double* aux;
+    const int* cs;
+    const int* fs;
+    int n;
+    double dry_tolerance;
+};
+
+// Time interpolation of the two interior border layers: slot dst[k] of qdst
+// from slot src[k] of qlast and qcurr.
+struct TimeInterpArgs {
+    const double* qlast;
+    const double* qcurr;
+    double* qdst;
+    const int* src;
+    const int* dst;
+    int n;
+    double alpha;
+};
+
 // Hydrograph inflow ghost cells of the patches in list[0 .. *list_n) that lie
 // in [first, last): each patch's ghost ring is rebuilt in its q buffer (and
 // aux) by ForestClaw's sequence -- same-level copies, then bc2 sides left,
@@ -151,6 +176,12 @@ bool default_method(const Params& P);
 // Active tiles then dry patches from one persistent kernel (see k_all).
 hipError_t launch_all(int mx, int my, int variant, bool def, const BatchArgs* da, int grid, hipStream_t st);
 hipError_t launch_batch_control(const BatchArgs& a, hipStream_t st);
+hipError_t launch_average2coarse(int mx, int my, const RegridArgs& a, hipStream_t st);
+hipError_t launch_interpolate2fine(int mx, int my, const RegridArgs& a, hipStream_t st);
+hipError_t launch_timeinterp(int mx, int my, const TimeInterpArgs& a, hipStream_t st);
+// hmax[slot] of buffer q for the listed slots; clears their dry_age entries.
+hipError_t launch_hmax_list(int mx, int my, const double* q, double* hmax, int* dry_age, const int* slots, int n,
+                            hipStream_t st);
 hipError_t launch_frame_pack(int mx, int my, double* q, const double* aux, int first, int n, double* out,
                              hipStream_t st);
 hipError_t launch_gauges(int mx, int my, const GaugeArgs& a, hipStream_t st);

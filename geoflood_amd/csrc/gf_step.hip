@@ -1309,6 +1309,191 @@ __global__ void __launch_bounds__(64) k_gauges(GaugeArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Regrid between levels (refinement ratio 2), one workgroup per (family,
+// child).  Coarse cells outside the coarse patch's interior are read through
+// ghost_source, as the halo pull does.
+// ---------------------------------------------------------------------------
+
+// fc2d_geoclaw_fort_average2coarse (fc2d_geoclaw_average_fort.f:482-603),
+// mcapa = 0: quadrant g of the coarse interior from child g.
+template <int MX, int MY>
+__global__ void __launch_bounds__(256) k_average2coarse(RegridArgs a)
+{
+    using G = Geo<MX, MY>;
+    const int f = blockIdx.x >> 2, gch = blockIdx.x & 3;
+    if (f >= a.n) return;
+    const int ig = gch % 2, jg = gch / 2;
+    const double* qf = a.q + (size_t)a.fs[4 * f + gch] * 3 * G::NP2;
+    const double* af = a.aux + (size_t)a.fs[4 * f + gch] * G::NP2;
+    double* qc = a.q + (size_t)a.cs[f] * 3 * G::NP2;
+    const double* ac = a.aux + (size_t)a.cs[f] * G::NP2;
+    for (int c = threadIdx.x; c < (MX / 2) * (MY / 2); c += 256) {
+        const int i = c % (MX / 2) + 1, j = c / (MX / 2) + 1;
+        const int i1 = i + ig * MX / 2, j1 = j + jg * MY / 2;
+        double etasum = 0.0, hsum = 0.0, husum = 0.0, hvsum = 0.0;
+        int nwet = 0;
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const int k = G::ti((i - 1) * 2 + (m & 1) + 1, (j - 1) * 2 + (m >> 1) + 1);
+            const double hf = qf[k] * 1.0;
+            const double bf = af[k] * 1.0;
+            double huf = qf[G::NP2 + k] * 1.0;
+            double hvf = qf[2 * G::NP2 + k] * 1.0;
+            double etaf;
+            if (hf > a.dry_tolerance) {
+                etaf = hf + bf;
+                nwet++;
+            } else {
+                etaf = 0.0;
+                huf = 0.0;
+                hvf = 0.0;
+            }
+            hsum = hsum + hf;
+            husum = husum + huf;
+            hvsum = hvsum + hvf;
+            etasum = etasum + etaf;
+        }
+        double hc = 0.0, huc = 0.0, hvc = 0.0;
+        if (nwet > 0) {
+            const double etaav = etasum / (double)nwet;
+            const double hav = hsum / (double)nwet;
+            const double e = etaav - ac[G::ti(i1, j1)] * 1.0;
+            const double hmx = e > 0.0 ? e : 0.0;
+            hc = hav < hmx ? hav : hmx;
+            huc = (hc / hsum) * husum;
+            hvc = (hc / hsum) * hvsum;
+        }
+        const int k1 = G::ti(i1, j1);
+        qc[k1] = hc / 1.0;
+        qc[G::NP2 + k1] = huc / 1.0;
+        qc[2 * G::NP2 + k1] = hvc / 1.0;
+    }
+}
+
+// ForestClaw's fclaw2d_clawpatch_compute_slopes at mth = 5 (the slope of
+// fc2d_geoclaw_fort_interpolate2fine): the smallest of 2|sl|, 2|sr|, |sc|
+// with the sign of sc = (sl + sr)/2, 0 at an extremum.
+__device__ __forceinline__ double amr_slope(double sl, double sr)
+{
+    const double sc = (sl + sr) / 2.0;
+    const double a = 2 * fabs(sl), b = 2 * fabs(sr), c = fabs(sc);
+    double v = a < b ? a : b;
+    v = v < c ? v : c;
+    const double mn = sl < sr ? sl : sr, mxv = sl > sr ? sl : sr;
+    if (mn <= 0 && mxv >= 0) return 0.0;
+    return copysign(v, sc);
+}
+
+template <int MX, int MY>
+__device__ __forceinline__ double regrid_cell(const RegridArgs& a, int patch, int fi, int fj, int m)
+{
+    using G = Geo<MX, MY>;
+    int sp = patch, si = fi, sj = fj;
+    bool nu = false, nv = false;
+    const bool interior = fi >= 1 && fi <= MX && fj >= 1 && fj <= MY;
+    if (!interior && !ghost_source<MX, MY>(a.desc[patch].nbr, patch, fi, fj, sp, si, sj, nu, nv)) {
+        sp = patch; si = fi; sj = fj;
+    }
+    const double v = a.q[(size_t)sp * 3 * G::NP2 + m * G::NP2 + G::ti(si, sj)];
+    return (m == 1 && nu) || (m == 2 && nv) ? -v : v;
+}
+
+// fc2d_geoclaw_fort_interpolate2fine (fc2d_geoclaw_interpolate_fort.f:
+// 405-579), sea_level = 0: child g's interior and the ghost layers on the
+// family's inner sides from the coarse patch.
+template <int MX, int MY>
+__global__ void __launch_bounds__(256) k_interpolate2fine(RegridArgs a)
+{
+    using G = Geo<MX, MY>;
+    const int f = blockIdx.x >> 2, gch = blockIdx.x & 3;
+    if (f >= a.n) return;
+    const int ig = gch % 2, jg = gch / 2;
+    const int cp = a.cs[f];
+    const double* ac = a.aux + (size_t)cp * G::NP2;
+    double* qf = a.q + (size_t)a.fs[4 * f + gch] * 3 * G::NP2;
+    const double* af = a.aux + (size_t)a.fs[4 * f + gch] * G::NP2;
+    constexpr int NI = MX / 2 + 1, NJ = MY / 2 + 1;
+    for (int c = threadIdx.x; c < NI * NJ; c += 256) {
+        const int i = (1 - ig) + c % NI, j = (1 - jg) + c / NI;
+        const int ic = i + ig * MX / 2, jc = j + jg * MY / 2;
+        double eb[3][3];
+#pragma unroll
+        for (int ii = -1; ii <= 1; ii++)
+#pragma unroll
+            for (int jj = -1; jj <= 1; jj++) {
+                const double h = regrid_cell<MX, MY>(a, cp, ic + ii, jc + jj, 0);
+                const double b = ac[G::ti(ic + ii, jc + jj)];
+                eb[ii + 1][jj + 1] = h < a.dry_tolerance ? 0.0 : h + b;
+            }
+#pragma unroll
+        for (int mq = 0; mq < 3; mq++) {
+            double qc, gx, gy;
+            if (mq == 0) {
+                qc = eb[1][1];
+                gx = amr_slope(qc - eb[0][1], eb[2][1] - qc);
+                gy = amr_slope(qc - eb[1][0], eb[1][2] - qc);
+            } else {
+                qc = regrid_cell<MX, MY>(a, cp, ic, jc, mq);
+                gx = amr_slope(qc - regrid_cell<MX, MY>(a, cp, ic - 1, jc, mq),
+                               regrid_cell<MX, MY>(a, cp, ic + 1, jc, mq) - qc);
+                gy = amr_slope(qc - regrid_cell<MX, MY>(a, cp, ic, jc - 1, mq),
+                               regrid_cell<MX, MY>(a, cp, ic, jc + 1, mq) - qc);
+            }
+#pragma unroll
+            for (int ii = 1; ii <= 2; ii++)
+#pragma unroll
+                for (int jj = 1; jj <= 2; jj++) {
+                    const double shiftx = (ii - 2 / 2.0 - 0.5) / 2;
+                    const double shifty = (jj - 2 / 2.0 - 0.5) / 2;
+                    const int k = G::ti((i - 1) * 2 + ii, (j - 1) * 2 + jj);
+                    const double v = qc + shiftx * gx + shifty * gy;
+                    qf[mq * G::NP2 + k] = mq == 0 ? v - af[k] : v;
+                }
+        }
+    }
+}
+
+// fc2d_geoclaw_fort_timeinterp (fc2d_geoclaw_timeinterp_fort.f:1-69): the
+// two interior layers next to each face (every interior cell with i <= 2,
+// i >= mx-1, j <= 2 or j >= my-1).
+template <int MX, int MY>
+__global__ void __launch_bounds__(256) k_timeinterp(TimeInterpArgs a)
+{
+    using G = Geo<MX, MY>;
+    const int b = blockIdx.x;
+    if (b >= a.n) return;
+    const double* ql = a.qlast + (size_t)a.src[b] * 3 * G::NP2;
+    const double* qc = a.qcurr + (size_t)a.src[b] * 3 * G::NP2;
+    double* qd = a.qdst + (size_t)a.dst[b] * 3 * G::NP2;
+    for (int c = threadIdx.x; c < MX * MY; c += 256) {
+        const int i = c % MX + 1, j = c / MX + 1;
+        if (!(i <= 2 || i >= MX - 1 || j <= 2 || j >= MY - 1)) continue;
+        const int k = G::ti(i, j);
+#pragma unroll
+        for (int m = 0; m < 3; m++)
+            qd[m * G::NP2 + k] = ql[m * G::NP2 + k] + a.alpha * (qc[m * G::NP2 + k] - ql[m * G::NP2 + k]);
+    }
+}
+
+template <int MX, int MY>
+__global__ void __launch_bounds__(256) k_hmax_list(const double* q, double* hmax, int* dry_age, const int* slots, int n)
+{
+    using G = Geo<MX, MY>;
+    __shared__ double red[4];
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    const int s = slots[b];
+    const double* src = q + (size_t)s * 3 * G::NP2;
+    double m = 0.0;
+    for (int e = threadIdx.x; e < MX * MY; e += 256) m = fmaxf64(m, src[G::ti(e % MX + 1, e / MX + 1)]);
+    m = block_max<256>(m, red);
+    if (threadIdx.x == 0) {
+        hmax[s] = m;
+        dry_age[s] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Pack / unpack between the reference's host layout (AoS q(m,i,j) with ghost
 // cells, data_swap.f90:7-54) and the pool's SoA layout, one workgroup per
 // patch.  ``stage`` holds n consecutive AoS patches.
@@ -1531,6 +1716,53 @@ hipError_t launch_batch_control(const BatchArgs& a, hipStream_t st)
     k_batch_control<<<(a.nscen + 63) / 64 > 0 ? (a.nscen + 63) / 64 : 1, 64, 0, st>>>(a);
     return hipGetLastError();
 }
+
+#define GF_SIZES(CALL)                             \
+    switch (mx * 1000 + my) {                      \
+    case 32032: CALL(32, 32); break;               \
+    case 16016: CALL(16, 16); break;               \
+    case 8008: CALL(8, 8); break;                  \
+    case 64064: CALL(64, 64); break;               \
+    default: return hipErrorInvalidValue;          \
+    }
+
+hipError_t launch_average2coarse(int mx, int my, const RegridArgs& a, hipStream_t st)
+{
+    if (a.n <= 0) return hipSuccess;
+#define GF_AVG(M, N) k_average2coarse<M, N><<<4 * a.n, 256, 0, st>>>(a)
+    GF_SIZES(GF_AVG)
+#undef GF_AVG
+    return hipGetLastError();
+}
+
+hipError_t launch_interpolate2fine(int mx, int my, const RegridArgs& a, hipStream_t st)
+{
+    if (a.n <= 0) return hipSuccess;
+#define GF_INT(M, N) k_interpolate2fine<M, N><<<4 * a.n, 256, 0, st>>>(a)
+    GF_SIZES(GF_INT)
+#undef GF_INT
+    return hipGetLastError();
+}
+
+hipError_t launch_timeinterp(int mx, int my, const TimeInterpArgs& a, hipStream_t st)
+{
+    if (a.n <= 0) return hipSuccess;
+#define GF_TI(M, N) k_timeinterp<M, N><<<a.n, 256, 0, st>>>(a)
+    GF_SIZES(GF_TI)
+#undef GF_TI
+    return hipGetLastError();
+}
+
+hipError_t launch_hmax_list(int mx, int my, const double* q, double* hmax, int* dry_age, const int* slots, int n,
+                            hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+#define GF_HL(M, N) k_hmax_list<M, N><<<n, 256, 0, st>>>(q, hmax, dry_age, slots, n)
+    GF_SIZES(GF_HL)
+#undef GF_HL
+    return hipGetLastError();
+}
+#undef GF_SIZES
 
 hipError_t launch_frame_pack(int mx, int my, double* q, const double* aux, int first, int n, double* out,
                              hipStream_t st)

@@ -352,6 +352,27 @@ class Pool:
               "gf_pool_sample_gauges")
         return out
 
+    # -- regrid and time interpolation between levels ---------------------------
+    def average2coarse(self, coarse, fine, buf: int = 0):
+        """Coarse slots from their four children (gf_pool_average2coarse)."""
+        c = np.ascontiguousarray(np.asarray(coarse, dtype=np.int32).reshape(-1))
+        f = np.ascontiguousarray(np.asarray(fine, dtype=np.int32).reshape(-1))
+        assert len(f) == 4 * len(c)
+        check(self.lib.gf_pool_average2coarse(self.h, buf, len(c), iptr(c), iptr(f)), "gf_pool_average2coarse")
+
+    def interpolate2fine(self, coarse, fine, buf: int = 0):
+        """Children from their coarse patch (gf_pool_interpolate2fine)."""
+        c = np.ascontiguousarray(np.asarray(coarse, dtype=np.int32).reshape(-1))
+        f = np.ascontiguousarray(np.asarray(fine, dtype=np.int32).reshape(-1))
+        assert len(f) == 4 * len(c)
+        check(self.lib.gf_pool_interpolate2fine(self.h, buf, len(c), iptr(c), iptr(f)), "gf_pool_interpolate2fine")
+
+    def timeinterp(self, src, buf_last, buf_curr, alpha, dst, dst_buf):
+        s = np.ascontiguousarray(np.asarray(src, dtype=np.int32).reshape(-1))
+        d = np.ascontiguousarray(np.asarray(dst, dtype=np.int32).reshape(-1))
+        check(self.lib.gf_pool_timeinterp(self.h, len(s), iptr(s), buf_last, buf_curr, alpha, iptr(d), dst_buf),
+              "gf_pool_timeinterp")
+
     # -- step variants ------------------------------------------------------------
     def variants(self):
         """(current variant, [(tile_x, tile_y, threads), ...])."""

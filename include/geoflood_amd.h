@@ -391,6 +391,37 @@ int gf_pool_write_frame(gf_pool *pool, int buf, int first, int n, const char *di
 int gf_pool_sample_gauges(gf_pool *pool, int buf, int n, const int *slot, const double *xc,
                           const double *yc, double *out);
 
+/* ---- Regrid and time interpolation between levels --------------------------
+ * Refinement ratio 2 between slots of one pool (same patch size): family f
+ * is the coarse patch in slot coarse[f] and its four children in slots
+ * fine[4f + g], g = igrid = 0..3 (x fastest), all in buffer buf.  Slots must
+ * be described (gf_pool_set_patches / gf_pool_describe, the children with
+ * their own dx, dy, corners and bathymetry uploaded) and distinct.  Both
+ * recompute the written patches' depth maxima for the batched path.
+ *
+ * gf_pool_average2coarse replaces fc2d_geoclaw_fort_average2coarse
+ * (fc2d_geoclaw_average_fort.f:482-603, mcapa = 0): each child averaged into
+ * its quadrant of the coarse interior (wet-cell mean of eta, mean depth
+ * capped by it, momentum rescaled), as ForestClaw's coarsening calls it.
+ *
+ * gf_pool_interpolate2fine replaces fc2d_geoclaw_fort_interpolate2fine
+ * (fc2d_geoclaw_interpolate_fort.f:405-579): each child's interior (and its
+ * ghost layers on the family's inner sides) from limited linear
+ * reconstructions of eta (dry cells at sea level 0) and of hu, hv, with
+ * ForestClaw's fclaw2d_clawpatch_compute_slopes at mth = 5.  Coarse cells
+ * beyond the coarse interior are read as the halo pull reads them; a child
+ * may therefore not be a same-level neighbour of a coarse patch of the call. */
+int gf_pool_average2coarse(gf_pool *pool, int buf, int n, const int *coarse, const int *fine);
+int gf_pool_interpolate2fine(gf_pool *pool, int buf, int n, const int *coarse, const int *fine);
+
+/* Time interpolation for coarse ghost sources, replaces
+ * fc2d_geoclaw_fort_timeinterp (fc2d_geoclaw_timeinterp_fort.f:1-69): for
+ * k < n, the two interior layers next to each face of slot dst[k] of buffer
+ * dst_buf become q_last + alpha (q_curr - q_last) of slot src[k] (buffers
+ * buf_last, buf_curr). */
+int gf_pool_timeinterp(gf_pool *pool, int n, const int *src, int buf_last, int buf_curr, double alpha,
+                       const int *dst, int dst_buf);
+
 /* Gauge files in the reference's format (geoclaw_create_gauge_files_default
  * and geoclaw_print_gauges_default, fc2d_geoclaw_gauges_default.c:172-200,
  * :275-305): gf_gauge_file_create writes the two header lines of

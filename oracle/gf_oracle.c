@@ -1568,3 +1568,157 @@ int gfo_gauge_line(char *out, int cap, int level, double t, const double qvar[3]
                            qvar[1], qvar[2], eta);
     return (n < 0 || n >= cap) ? -1 : n;
 }
+
+/* ======================================================================
+ * Regrid and time interpolation between levels
+ * ====================================================================== */
+
+#define AQ(q, m, i, j) (q)[((size_t)((j) - 1 + mbc) * nx + ((i) - 1 + mbc)) * 3 + (m)]
+#define AA(a, i, j) (a)[((size_t)((j) - 1 + mbc) * nx + ((i) - 1 + mbc)) * maux]
+
+/* fc2d_geoclaw_average_fort.f:482-603 with mcapa = 0 (capa = capac = 1). */
+void gfo_average2coarse(const gfo_params *p, int mx, int my, int mbc, int maux, double *qcoarse,
+                        const double *qfine, const double *aux_coarse, const double *aux_fine, int igrid)
+{
+    const int nx = mx + 2 * mbc;
+    const int ig = igrid % 2, jg = (igrid - ig) / 2;
+    const int ic_add = ig * mx / 2, jc_add = jg * my / 2;
+    for (int j = 1; j <= my / 2; j++)
+        for (int i = 1; i <= mx / 2; i++) {
+            const int i1 = i + ic_add, j1 = j + jc_add;
+            int i2[4], j2[4], m = 0;
+            for (int jj = 1; jj <= 2; jj++)
+                for (int ii = 1; ii <= 2; ii++) {
+                    i2[m] = (i - 1) * 2 + ii;
+                    j2[m] = (j - 1) * 2 + jj;
+                    m++;
+                }
+            const double capac = 1.0;
+            double etasum = 0.0, hsum = 0.0, husum = 0.0, hvsum = 0.0;
+            int nwet = 0;
+            for (m = 0; m < 4; m++) {
+                const double capa = 1.0;
+                const double hf = AQ(qfine, 0, i2[m], j2[m]) * capa;
+                const double bf = AA(aux_fine, i2[m], j2[m]) * capa;
+                double huf = AQ(qfine, 1, i2[m], j2[m]) * capa;
+                double hvf = AQ(qfine, 2, i2[m], j2[m]) * capa;
+                double etaf;
+                if (hf > p->dry_tolerance) {
+                    etaf = hf + bf;
+                    nwet++;
+                } else {
+                    etaf = 0.0;
+                    huf = 0.0;
+                    hvf = 0.0;
+                }
+                hsum = hsum + hf;
+                husum = husum + huf;
+                hvsum = hvsum + hvf;
+                etasum = etasum + etaf;
+            }
+            double hc, huc, hvc;
+            if (nwet > 0) {
+                const double etaav = etasum / (double)nwet;
+                const double hav = hsum / (double)nwet;
+                const double e = etaav - AA(aux_coarse, i1, j1) * capac;
+                const double hmx = e > 0.0 ? e : 0.0;  /* Fortran MAX(a, b) = a > b ? a : b */
+                hc = hav < hmx ? hav : hmx;            /* MIN */
+                huc = (hc / hsum) * husum;
+                hvc = (hc / hsum) * hvsum;
+            } else {
+                hc = 0.0;
+                huc = 0.0;
+                hvc = 0.0;
+            }
+            AQ(qcoarse, 0, i1, j1) = hc / capac;
+            AQ(qcoarse, 1, i1, j1) = huc / capac;
+            AQ(qcoarse, 2, i1, j1) = hvc / capac;
+        }
+}
+
+/* ForestClaw fclaw2d_clawpatch_compute_slopes, mth = 5: the smallest of
+ * 2|sl|, 2|sr| and |sc| (sc = (sl+sr)/2) with the sign of sc, and 0 when
+ * sl and sr differ in sign or one is 0.  Restated from ForestClaw's source
+ * (not vendored in the reference); mth <= 4 (philim) is not used here. */
+double gfo_compute_slopes(double sl, double sr, int mth)
+{
+    (void)mth;
+    const double sc = (sl + sr) / 2.0;
+    double a = 2 * fabs(sl), b = 2 * fabs(sr), c = fabs(sc);
+    double v = a < b ? a : b;
+    v = v < c ? v : c;
+    const double mn = sl < sr ? sl : sr, mxv = sl > sr ? sl : sr;
+    if (mn <= 0 && mxv >= 0) return 0.0;
+    return copysign(v, sc);
+}
+
+/* fc2d_geoclaw_interpolate_fort.f:405-579 (the extremum check there is
+ * commented out and not restated). */
+void gfo_interpolate2fine(const gfo_params *p, int mx, int my, int mbc, int maux, const double *qcoarse,
+                          double *qfine, const double *aux_coarse, const double *aux_fine, int igrid)
+{
+    const int nx = mx + 2 * mbc;
+    const double sea_level = 0.0;
+    const int mth = 5;
+    const int ig = igrid % 2, jg = (igrid - ig) / 2;
+    const int i1 = 1 - ig, i2 = mx / 2 + (1 - ig), ic_add = ig * mx / 2;
+    const int j1 = 1 - jg, j2 = my / 2 + (1 - jg), jc_add = jg * my / 2;
+    for (int mq = 0; mq < 3; mq++)
+        for (int i = i1; i <= i2; i++)
+            for (int j = j1; j <= j2; j++) {
+                const int ic = i + ic_add, jc = j + jc_add;
+                double etab[3][3];
+                for (int ii = -1; ii <= 1; ii++)
+                    for (int jj = -1; jj <= 1; jj++) {
+                        const double h = AQ(qcoarse, 0, ic + ii, jc + jj);
+                        const double b = AA(aux_coarse, ic + ii, jc + jj);
+                        etab[ii + 1][jj + 1] = h < p->dry_tolerance ? sea_level : h + b;
+                    }
+                double qc, gradx, grady;
+                if (mq == 0) {
+                    qc = etab[1][1];
+                    gradx = gfo_compute_slopes(qc - etab[0][1], etab[2][1] - qc, mth);
+                    grady = gfo_compute_slopes(qc - etab[1][0], etab[1][2] - qc, mth);
+                } else {
+                    qc = AQ(qcoarse, mq, ic, jc);
+                    gradx = gfo_compute_slopes(qc - AQ(qcoarse, mq, ic - 1, jc), AQ(qcoarse, mq, ic + 1, jc) - qc, mth);
+                    grady = gfo_compute_slopes(qc - AQ(qcoarse, mq, ic, jc - 1), AQ(qcoarse, mq, ic, jc + 1) - qc, mth);
+                }
+                for (int ii = 1; ii <= 2; ii++)
+                    for (int jj = 1; jj <= 2; jj++) {
+                        const double shiftx = (ii - 2 / 2.0 - 0.5) / 2;
+                        const double shifty = (jj - 2 / 2.0 - 0.5) / 2;
+                        const int iff = (i - 1) * 2 + ii, jf = (j - 1) * 2 + jj;
+                        if (mq == 0)
+                            AQ(qfine, 0, iff, jf) = qc + shiftx * gradx + shifty * grady - AA(aux_fine, iff, jf);
+                        else
+                            AQ(qfine, mq, iff, jf) = qc + shiftx * gradx + shifty * grady;
+                    }
+            }
+}
+
+#undef AQ
+#undef AA
+
+/* fc2d_geoclaw_timeinterp_fort.f:1-69 (ng = 0, mint = 2). */
+int gfo_timeinterp(int mx, int my, int mbc, int meqn, int psize, const double *qcurr,
+                   const double *qlast, double *qinterp, double alpha)
+{
+    const int nx = mx + 2 * mbc, mint = 2, ng = 0;
+    int k = 1;
+#define IDX(m, i, j) (((size_t)((j) - 1 + mbc) * nx + ((i) - 1 + mbc)) * meqn + (m))
+#define TI(m, i, j) qinterp[IDX(m, i, j)] = qlast[IDX(m, i, j)] + alpha * (qcurr[IDX(m, i, j)] - qlast[IDX(m, i, j)]); k++
+    for (int m = 0; m < meqn; m++) {
+        for (int j = 1 - ng; j <= my - mint; j++)
+            for (int i = 1 - ng; i <= mint; i++) { TI(m, i, j); }
+        for (int j = 1 - ng; j <= mint; j++)
+            for (int i = mint + 1; i <= mx + ng; i++) { TI(m, i, j); }
+        for (int j = mint + 1; j <= my + ng; j++)
+            for (int i = mx - mint + 1; i <= mx + ng; i++) { TI(m, i, j); }
+        for (int j = my - mint + 1; j <= my + ng; j++)
+            for (int i = 1 - ng; i <= mx - mint; i++) { TI(m, i, j); }
+    }
+#undef TI
+#undef IDX
+    return (k - 1 != psize) ? 2 : 0;
+}

@@ -247,6 +247,31 @@ void gfo_update_gauge(const gfo_params *p, int mx, int my, int mbc, int maux, do
 int gfo_gauge_header(char *out, int cap, int id, double xc, double yc);
 int gfo_gauge_line(char *out, int cap, int level, double t, const double qvar[3], const double avar[1]);
 
+/* ---- Regrid and time interpolation between levels ------------------------
+ * Patches in the reference's AoS layout with ghosts, refinement ratio 2, four
+ * children numbered igrid = 0..3 (x fastest). */
+
+/* fc2d_geoclaw_fort_average2coarse (fc2d_geoclaw_average_fort.f:482-603),
+ * mcapa = 0: the quadrant igrid of qcoarse's interior from the fine patch. */
+void gfo_average2coarse(const gfo_params *p, int mx, int my, int mbc, int maux, double *qcoarse,
+                        const double *qfine, const double *aux_coarse, const double *aux_fine, int igrid);
+
+/* fc2d_geoclaw_fort_interpolate2fine (fc2d_geoclaw_interpolate_fort.f:405-579)
+ * with ForestClaw's fclaw2d_clawpatch_compute_slopes at mth = 5 (not in the
+ * reference tree; restated, parity unpinned): the fine patch igrid's interior
+ * from the coarse quadrant and its neighbours, sea_level = 0. */
+void gfo_interpolate2fine(const gfo_params *p, int mx, int my, int mbc, int maux, const double *qcoarse,
+                          double *qfine, const double *aux_coarse, const double *aux_fine, int igrid);
+
+/* ForestClaw's slope for mth = 5 (the AMRClaw slope), as used above. */
+double gfo_compute_slopes(double sl, double sr, int mth);
+
+/* fc2d_geoclaw_fort_timeinterp (fc2d_geoclaw_timeinterp_fort.f:1-69):
+ * qinterp = qlast + alpha (qcurr - qlast) on the two interior layers next to
+ * each face.  Returns ierror (2 when the count differs from psize). */
+int gfo_timeinterp(int mx, int my, int mbc, int meqn, int psize, const double *qcurr,
+                   const double *qlast, double *qinterp, double alpha);
+
 #ifdef __cplusplus
 }
 #endif

@@ -389,6 +389,38 @@ class CLib(_Lib):
         assert n >= 0
         return buf.raw[:n]
 
+    # --- regrid and time interpolation ---------------------------------------
+    def average2coarse(self, qcoarse, qfine, aux_coarse, aux_fine, igrid):
+        """Quadrant igrid of qcoarse (modified) from the fine patch qfine."""
+        ny, nx, _ = qcoarse.shape
+        fn = self.lib.gfo_average2coarse
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 4 + [_dp] * 4 + [ct.c_int]
+        fn.restype = None
+        fn(ct.byref(self._cp), nx - 4, ny - 4, 2, aux_coarse.shape[-1], ptr(qcoarse), ptr(qfine), ptr(aux_coarse),
+           ptr(aux_fine), igrid)
+
+    def interpolate2fine(self, qcoarse, qfine, aux_coarse, aux_fine, igrid):
+        """Fine patch igrid (qfine, modified) from the coarse patch."""
+        ny, nx, _ = qcoarse.shape
+        fn = self.lib.gfo_interpolate2fine
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 4 + [_dp] * 4 + [ct.c_int]
+        fn.restype = None
+        fn(ct.byref(self._cp), nx - 4, ny - 4, 2, aux_coarse.shape[-1], ptr(qcoarse), ptr(qfine), ptr(aux_coarse),
+           ptr(aux_fine), igrid)
+
+    def compute_slopes(self, sl, sr, mth=5):
+        fn = self.lib.gfo_compute_slopes
+        fn.argtypes = [ct.c_double, ct.c_double, ct.c_int]
+        fn.restype = ct.c_double
+        return fn(sl, sr, mth)
+
+    def timeinterp(self, qcurr, qlast, qinterp, alpha, psize):
+        ny, nx, meqn = qcurr.shape
+        fn = self.lib.gfo_timeinterp
+        fn.argtypes = [ct.c_int] * 5 + [_dp] * 3 + [ct.c_double]
+        fn.restype = ct.c_int
+        return fn(nx - 4, ny - 4, 2, meqn, psize, ptr(qcurr), ptr(qlast), ptr(qinterp), alpha)
+
     # --- brick driver -------------------------------------------------------
     def brick_run(self, brick: "Brick", nsteps, initial_dt, desired_cfl=0.9, max_cfl=1.0,
                   t0=0.0, update=None, nthreads=1):
@@ -706,6 +738,42 @@ def c_lib() -> CLib:
     if _C is None:
         _C = CLib()
     return _C
+
+
+class RefAmrLib:
+    """The reference's regrid averaging and time interpolation
+    (oracle/_ref/libgfref_amr.so), loaded with lazy binding: the same source
+    files' face/corner routines call ForestClaw, which is absent, and are
+    never called."""
+
+    def __init__(self, path: str = os.path.join(os.path.dirname(REF_LIB_PATH), "libgfref_amr.so")):
+        self.lib = ct.CDLL(path, mode=os.RTLD_LAZY)
+
+    def set_dry_tolerance(self, drytol):
+        fn = self.lib.gfref_amr_set_drytol
+        fn.argtypes = [ct.c_double]
+        fn.restype = None
+        fn(drytol)
+
+    def average2coarse(self, qcoarse, qfine, aux_coarse, aux_fine, igrid):
+        ny, nx, _ = qcoarse.shape
+        fn = self.lib.gfref_average2coarse
+        fn.argtypes = [ct.c_int] * 4 + [_dp] * 4 + [ct.c_int]
+        fn.restype = None
+        fn(nx - 4, ny - 4, 2, aux_coarse.shape[-1], ptr(qcoarse), ptr(qfine), ptr(aux_coarse), ptr(aux_fine), igrid)
+
+    def timeinterp(self, qcurr, qlast, qinterp, alpha, psize):
+        ny, nx, meqn = qcurr.shape
+        fn = self.lib.gfref_timeinterp
+        fn.argtypes = [ct.c_int] * 5 + [_dp] * 3 + [ct.c_double]
+        fn.restype = ct.c_int
+        return fn(nx - 4, ny - 4, 2, meqn, psize, ptr(qcurr), ptr(qlast), ptr(qinterp), alpha)
+
+
+def ref_amr_lib():
+    """The reference regrid build, or None where it was not built."""
+    path = os.path.join(os.path.dirname(REF_LIB_PATH), "libgfref_amr.so")
+    return RefAmrLib(path) if os.path.exists(path) else None
 
 
 def ref_lib():

@@ -1,0 +1,126 @@
+"""Regrid (average2coarse, interpolate2fine) and time interpolation of
+resident patches against the reference's outputs (tests/golden/
+amr_vectors.npz) and the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Brick
+from geoflood_amd.pool import BrickLayout, Pool
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "amr_vectors.npz")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(GOLD)
+
+
+def test_average2coarse_matches_reference(gf, oracle_c, gold):
+    """48 families of 8x8 patches: child g of family k holds the golden fine
+    patch of case k (the other children copies of it); every quadrant of the
+    coarse result equals the oracle's, and quadrant igrid the reference's."""
+    qc, qf, ac, af, ig, out = (gold[k] for k in ("avg_qc", "avg_qf", "avg_auxc", "avg_auxf", "avg_igrid", "avg_out"))
+    n = len(ig)
+    pool = Pool(8, 8, 5 * n)
+    try:
+        pool.set_patches(np.ones(5 * n), np.ones(5 * n), np.zeros(5 * n), np.zeros(5 * n))
+        q_all = np.concatenate([qc, np.repeat(qf, 4, axis=0)])
+        a_all = np.concatenate([ac, np.repeat(af, 4, axis=0)])
+        pool.upload_aos(0, q_all, a_all)
+        pool.average2coarse(np.arange(n), n + np.arange(4 * n))
+        got = pool.download_aos(0, n)
+        inner = (slice(2, 10), slice(2, 10))
+        for k in range(n):
+            want = qc[k].copy()
+            for g in range(4):
+                oracle_c.average2coarse(want, qf[k], ac[k], af[k], g)
+            assert np.array_equal(got[k][inner], want[inner]), k
+            g = int(ig[k])
+            sl = (slice(2 + 4 * (g // 2), 6 + 4 * (g // 2)), slice(2 + 4 * (g % 2), 6 + 4 * (g % 2)))
+            assert np.array_equal(got[k][sl], out[k][sl]), k
+    finally:
+        pool.close()
+
+
+def test_timeinterp_matches_reference(gf, gold):
+    tc, tl, al, to = gold["ti_curr"], gold["ti_last"], gold["ti_alpha"], gold["ti_out"]
+    m = len(al)
+    pool = Pool(8, 8, 2 * m)
+    try:
+        pool.set_patches(np.ones(2 * m), np.ones(2 * m), np.zeros(2 * m), np.zeros(2 * m))
+        pool.upload_aos(0, tl, None, buf=0)
+        pool.upload_aos(0, tc, None, buf=1)
+        pool.upload_aos(m, np.zeros_like(tc), None, buf=1)
+        for k in range(m):   # one alpha per call
+            pool.timeinterp([k], 0, 1, float(al[k]), [m + k], 1)
+        got = pool.download_aos(m, m, buf=1)
+        ring = np.zeros((12, 12), bool)
+        ring[2:10, 2:10] = True
+        ring[4:8, 4:8] = False
+        for k in range(m):
+            assert np.array_equal(got[k][ring], to[k][ring]), k
+            assert np.all(got[k][4:8, 4:8] == 0.0)
+    finally:
+        pool.close()
+
+
+def test_interpolate2fine_matches_oracle_on_a_brick(gf, oracle_c):
+    """Children of coarse patches of a 4x4 brick (walls outside, same-level
+    neighbours inside): the coarse halo is read through the neighbours, and
+    the children equal the oracle's interpolation from the ghost-filled
+    coarse patches, in the interior and the inner ghost layers it writes."""
+    rng = np.random.default_rng(12)
+    mx, n = 16, 64
+    b = rng.normal(0, 0.3, (n, n))
+    h = np.maximum(1.5 - b + rng.normal(0, 0.2, (n, n)), 0.0)
+    h[rng.random((n, n)) < 0.1] = 0.0
+    hu, hv = rng.normal(0, 0.5, (2, n, n)) * (h > 0)
+    layout = BrickLayout(4, 4, mx, mx, 1.0, 1.0)
+    fam = [0, 3, 5, 6, 10, 15]
+    nk = 4 * len(fam)
+    pool = Pool(mx, mx, layout.npatch + nk)
+    try:
+        pool.set_layout([layout])
+        ox, oy = layout.origins()
+        # children: their own slots after the brick, with fine bathymetry
+        cdx = np.full(nk, 0.5)
+        cxl = np.array([ox[c] + 0.5 * mx * (g % 2) for c in fam for g in range(4)])
+        cyl = np.array([oy[c] + 0.5 * mx * (g // 2) for c in fam for g in range(4)])
+        dx = np.concatenate([np.ones(layout.npatch), cdx])
+        xl = np.concatenate([ox, cxl])
+        yl = np.concatenate([oy, cyl])
+        nb = np.concatenate([layout.neighbours(), np.full((nk, 8), -3, np.int32)])
+        pool.set_patches(dx, dx, xl, yl, nb)
+        pool.upload_global(layout, h, hu, hv, b)
+        af = rng.normal(0, 0.3, (nk, mx + 4, mx + 4, 1))
+        pool.upload_aos(layout.npatch, np.zeros((nk, mx + 4, mx + 4, 3)), af)
+        pool.interpolate2fine(fam, layout.npatch + np.arange(nk))
+        got = pool.download_aos(layout.npatch, nk)
+        br = Brick.from_global(h, hu, hv, b, 4, 4, 2, 1.0, 1.0)
+        oracle_c.brick_ghost_fill(br)
+        for f, c in enumerate(fam):
+            for g in range(4):
+                want = np.zeros((mx + 4, mx + 4, 3))
+                oracle_c.interpolate2fine(br.q[c], want, br.aux[c], af[4 * f + g], g)
+                ig, jg = g % 2, g // 2
+                rows = slice(2 - 2 * jg, mx + 2 + 2 * (1 - jg))
+                cols = slice(2 - 2 * ig, mx + 2 + 2 * (1 - ig))
+                assert np.array_equal(got[4 * f + g][rows, cols], want[rows, cols]), (c, g)
+    finally:
+        pool.close()
+
+
+def test_regrid_rejects_overlapping_slots(gf):
+    pool = Pool(8, 8, 10)
+    try:
+        pool.set_patches(np.ones(10), np.ones(10), np.zeros(10), np.zeros(10))
+        with pytest.raises(RuntimeError):
+            pool.average2coarse([0], [0, 1, 2, 3])
+        with pytest.raises(RuntimeError):
+            pool.interpolate2fine([0, 1], [2, 3, 4, 5, 6, 7, 8, 2])
+    finally:
+        pool.close()
