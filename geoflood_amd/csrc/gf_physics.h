@@ -50,13 +50,16 @@ __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, dou
     }
     const double F_min = delu + 2.0 * (sg_min - sg_max);
     const double F_max = delu + (h_max - h_min) * (sqrt(.5 * g * (h_max + h_min) / (h_max * h_min)));
-    if (F_min > 0.0) {  // two rarefactions
+    // The three branches leave hm (and, for two shocks, u1m, u2m); sqrt(g*hm)
+    // and the speeds are formed once after them, so that a wave whose lanes
+    // take different branches pays for the shared part only once.
+    double u1m = 0.0, u2m = 0.0;
+    int kind;  // 0: two rarefactions, 1: two shocks, 2: one shock, one rarefaction
+    if (F_min > 0.0) {
         const double t = fmaxf64(0.0, -delu + 2.0 * (sgL + sgR));
         hm = (1.0 / (16.0 * g)) * (t * t);
-        const double sghm = sqrt(g * hm);
-        s1m = uL + 2.0 * sgL - 3.0 * sghm;
-        s2m = uR - 2.0 * sgR + 3.0 * sghm;
-    } else if (F_max <= 0.0) {  // two shocks: one Newton step on sqrt(h)
+        kind = 0;
+    } else if (F_max <= 0.0) {  // one Newton step on sqrt(h)
         double h0 = h_max;
         const double gL = sqrt(.5 * g * (1 / h0 + 1 / hL));
         const double gR = sqrt(.5 * g * (1 / h0 + 1 / hR));
@@ -67,12 +70,10 @@ __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, dou
         const double t = sqrt(h0) - F0 / slope;
         h0 = t * t;
         hm = h0;
-        const double u1m = uL - (hm - hL) * sqrt((.5 * g) * (1 / hm + 1 / hL));
-        const double u2m = uR + (hm - hR) * sqrt((.5 * g) * (1 / hm + 1 / hR));
-        const double sghm = sqrt(g * hm);
-        s1m = u1m - sghm;
-        s2m = u2m + sghm;
-    } else {  // one shock, one rarefaction
+        u1m = uL - (hm - hL) * sqrt((.5 * g) * (1 / hm + 1 / hL));
+        u2m = uR + (hm - hR) * sqrt((.5 * g) * (1 / hm + 1 / hR));
+        kind = 1;
+    } else {
         // F0 = delu + 2(sqrt(g h0) - sqrt(g h_max)) + (h0 - h_min) sqrt(g/2 (1/h0 + 1/h_min))
         // at h0 = h_min: the last product is (+0) times a finite positive
         // whenever g/h_min cannot overflow, so F0 is F_min + 0.0 exactly
@@ -83,17 +84,51 @@ __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, dou
         const double slope = (F_max - F0) / (h_max - h_min);
         h0 = h0 - F0 / slope;
         hm = h0;
-        const double sghm = sqrt(g * hm);
-        if (hL > hR) {
-            const double a = uL + 2.0 * sgL;
-            s1m = a - 3.0 * sghm;
-            s2m = a - sghm;
-        } else {
-            const double a = uR - 2.0 * sgR;
-            s2m = a + 3.0 * sghm;
-            s1m = a + sghm;
-        }
+        kind = 2;
     }
+    const double sghm = sqrt(g * hm);
+    // two rarefactions:      s1m = (uL + 2 sqrt(g hL)) - 3 sghm, s2m = (uR - 2 sqrt(g hR)) + 3 sghm
+    // shock-rarefaction:     hL > hR: a = uL + 2 sqrt(g hL), s1m = a - 3 sghm, s2m = a - sghm
+    //                        else:    a = uR - 2 sqrt(g hR), s2m = a + 3 sghm, s1m = a + sghm
+    // two shocks:            s1m = u1m - sghm, s2m = u2m + sghm
+    const double A = uL + 2.0 * sgL, B = uR - 2.0 * sgR, s3 = 3.0 * sghm;
+    if (kind == 1) {
+        s1m = u1m - sghm;
+        s2m = u2m + sghm;
+    } else if (kind == 0 || hL > hR) {
+        s1m = A - s3;
+        s2m = kind == 0 ? B + s3 : A - sghm;
+    } else {
+        s1m = B + sghm;
+        s2m = B + s3;
+    }
+}
+
+// hm of riemanntype(h, h, u, -u), the wall test of rpn2 (rpn2_geoclaw.f:
+// 185-242), whose speeds the caller does not use.  With hL = hR = h every
+// (h_max - h_min) factor is an exact +0: F_min = F_max = delu + 0.0, a
+// positive F takes the two-rarefaction branch and any other the two-shock
+// branch with gL = gR and dfdh = gL + gL.  Outside the range where those
+// zeros are exact (g/h overflowing) the general routine is used.
+__device__ __forceinline__ double wall_hstar(double h, double u, double sg, double drytol, double g)
+{
+    if (!(h >= 1e-150 && g <= 1e150) || h <= drytol) {
+        double hm, s1m, s2m;
+        riemanntype(h, h, u, -u, sg, sg, drytol, g, hm, s1m, s2m);
+        return hm;
+    }
+    const double delu = -u - u;
+    const double F = delu + 0.0;
+    if (F > 0.0) {
+        const double t = fmaxf64(0.0, -delu + 2.0 * (sg + sg));
+        return (1.0 / (16.0 * g)) * (t * t);
+    }
+    const double gL = sqrt(.5 * g * (1 / h + 1 / h));
+    const double F0 = (delu + 0.0) + 0.0;
+    const double dfdh = gL + gL;
+    const double slope = 2.0 * sqrt(h) * dfdh;
+    const double t = sqrt(h) - F0 / slope;
+    return t * t;
 }
 
 // One face of rpn2 (rpn2_geoclaw.f:84-330) followed by riemann_aug_JCP
@@ -132,10 +167,15 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
     }
 
     double wall0 = 1.0, wall1 = 1.0, wall2 = 1.0;
+    // the wall test of whichever side is dry: riemanntype(hL, hL, uL, -uL) or
+    // riemanntype(hR, hR, -uR, uR), one call for both kinds of lane
+    double hstar = 0.0;
+    if (hR <= drytol || hL <= drytol) {
+        const double hw = hR <= drytol ? hL : hR;
+        const double uw = hR <= drytol ? uL : -uR;
+        hstar = wall_hstar(hw, uw, sqrt(g * hw), drytol, g);
+    }
     if (hR <= drytol) {
-        double hstar, s1m, s2m;
-        const double sg = sqrt(g * hL);
-        riemanntype(hL, hL, uL, -uL, sg, sg, drytol, g, hstar, s1m, s2m);
         const double hstartest = fmaxf64(hL, hstar);
         if (hstartest + bL < bR) {
             wall1 = 0.0; wall2 = 0.0;
@@ -144,9 +184,6 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
             bR = hL + bL;
         }
     } else if (hL <= drytol) {
-        double hstar, s1m, s2m;
-        const double sg = sqrt(g * hR);
-        riemanntype(hR, hR, -uR, uR, sg, sg, drytol, g, hstar, s1m, s2m);
         const double hstartest = fmaxf64(hR, hstar);
         if (hstartest + bR < bL) {
             wall0 = 0.0; wall1 = 0.0;
@@ -207,15 +244,19 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
     else sonic = false;
 
     double deldelh = sonic ? -delb : delb * g * hbar / s1s2bar;
+    // the bounds' two quotients are formed once for every lane (the three
+    // cases use them in different combinations)
+    const double q2 = hstarHLL * (sE2 - sE1) / sE2;
+    const double q1 = hstarHLL * (sE2 - sE1) / sE1;
     if (sE1 < -criticaltol && sE2 > criticaltol) {
-        deldelh = fminf64(deldelh, hstarHLL * (sE2 - sE1) / sE2);
-        deldelh = fmaxf64(deldelh, hstarHLL * (sE2 - sE1) / sE1);
+        deldelh = fminf64(deldelh, q2);
+        deldelh = fmaxf64(deldelh, q1);
     } else if (sE1 >= criticaltol) {
-        deldelh = fminf64(deldelh, hstarHLL * (sE2 - sE1) / sE1);
+        deldelh = fminf64(deldelh, q1);
         deldelh = fmaxf64(deldelh, -hL);
     } else if (sE2 <= -criticaltol) {
         deldelh = fminf64(deldelh, hR);
-        deldelh = fmaxf64(deldelh, hstarHLL * (sE2 - sE1) / sE2);
+        deldelh = fmaxf64(deldelh, q2);
     }
     double deldelphi = sonic ? -g * hbar * delb : -delb * g * hbar * s1s2tilde / s1s2bar;
     deldelphi = fminf64(deldelphi, g * fmaxf64(-hL * delb, -hR * delb));
