@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--output-patches", type=int, default=1024,
                     help="patches of the final state written as one ASCII frame (0: skip)")
+    ap.add_argument("--regrid-families", type=int, default=1024,
+                    help="coarse patches refined and averaged back in the regrid measurement (0: skip)")
     ap.add_argument("--host-steps", type=int, default=2,
                     help="steps of the host-resident drop-in batch (gf_step2_batch over all patches)")
     ap.add_argument("--retune-every", type=int, default=100,
@@ -156,12 +158,15 @@ def main():
 
     # ---- warm-up (includes graph capture), then the timed batched run ----
     # The step variant (tile shape, threads per workgroup) is chosen by the
-    # autotuner on the active population, and chosen again inside the timed
-    # run whenever that population has changed by more than RETUNE_FACTOR.
+    # autotuner on the population after the warm-up attempts (the first two
+    # attempts also step every dry patch, before those become steady), and
+    # chosen again inside the timed run whenever the active population has
+    # changed by more than RETUNE_FACTOR.  The autotuner does not advance the
+    # state; the timed run starts again from the initial state.
     reset(pool, layout, state, 0)
     pool.run(args.warmup, graph=True)
-    sid = reset(pool, layout, state, 0)
     best, tune_ms = pool.autotune(reps=3)
+    sid = reset(pool, layout, state, 0)
     _, variants = pool.variants()
     if args.variant >= 0:
         best = args.variant
@@ -354,6 +359,43 @@ def main():
             f"({pcie / 1e6:.0f} MB over PCIe per step)")
         del q_aos, aux_aos, q_list, aux_list
 
+    # ---- regrid between levels on resident patches ----
+    # F coarse patches of the workload (ghost cells as stored) refined into
+    # 4F children and averaged back, in one call each
+    regrid = None
+    if args.regrid_families > 0:
+        from geoflood_amd.pool import Pool as _RP, global_to_aos
+        F = min(args.regrid_families, layout.npatch)
+        h, hu, hv, b, dx = state
+        q_aos, aux_aos = global_to_aos(h, hu, hv, b, layout)
+        ox, oy = layout.origins()
+        rp = _RP(MX, MX, 5 * F)
+        try:
+            cx = np.concatenate([ox[:F], np.repeat(ox[:F], 4) + np.tile([0, 0.5 * MX, 0, 0.5 * MX], F) * dx])
+            cy = np.concatenate([oy[:F], np.repeat(oy[:F], 4) + np.tile([0, 0, 0.5 * MX, 0.5 * MX], F) * dx])
+            cd = np.concatenate([np.full(F, dx), np.full(4 * F, 0.5 * dx)])
+            rp.set_patches(cd, cd, cx, cy)
+            rp.upload_aos(0, q_aos[:F], aux_aos[:F])
+            rp.upload_aos(F, np.zeros((4 * F,) + q_aos.shape[1:]), np.repeat(aux_aos[:F], 4, axis=0))
+            cs, fs = np.arange(F), F + np.arange(4 * F)
+            rp.interpolate2fine(cs, fs)
+            rp.average2coarse(cs, fs)
+            reps = 5
+            t0r = time.perf_counter()
+            for _ in range(reps):
+                rp.interpolate2fine(cs, fs)
+            ti = (time.perf_counter() - t0r) / reps
+            t0r = time.perf_counter()
+            for _ in range(reps):
+                rp.average2coarse(cs, fs)
+            ta = (time.perf_counter() - t0r) / reps
+        finally:
+            rp.close()
+        del q_aos, aux_aos
+        regrid = dict(families=F, interpolate2fine_ms=round(1e3 * ti, 4), average2coarse_ms=round(1e3 * ta, 4),
+                      note="one call each over all families (32x32 patches), host round trip included")
+        log(f"regrid: {F} families, interpolate2fine {1e3 * ti:.3f} ms, average2coarse {1e3 * ta:.3f} ms")
+
     # ---- CPU baseline (reference Fortran step on the host cores) ----
     cpu = None
     parity = None
@@ -403,6 +445,7 @@ def main():
         "batched_vs_plain": (value / plain["value"]) if plain else None,
         "host_batch": hostbatch,
         "output": output,
+        "regrid": regrid,
         "plain": plain,
         "roofline": roofline,
         "step_composite": composite,

@@ -23,6 +23,8 @@
 // bit of pow() in src2.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include "gf_inflow.h"
 #include "gf_launch.h"
 
@@ -272,10 +274,18 @@ __device__ __forceinline__ void block_max2(double& a, double& b, double* red)
 // load); otherwise the stored ghost cells are used; halo cells inside the
 // patch are its own interior.  Thread 0 receives the tile's maximum CFL
 // number (uncapped) and the maximum interior depth after the step.
-template <int MX, int MY, int TX, int TY, int NT, bool PULL, bool DEF>
+// Called by step_patch's last thread right after the tile is in LDS: the
+// workgroup's highest thread has no position in the last slot, so its wave
+// has slack in the phases that follow.
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+template <int MX, int MY, int TX, int TY, int NT, bool PULL, bool DEF, typename Hook = NoHook>
 __device__ __forceinline__ void step_patch(double* T, double* E, double* F, double* red, const PatchDesc* dptr, int patch,
                            int ox, int oy, const double* q_src, double* q_dst, const double* aux,
-                           double dt, const Params& P, double& cfl_res, double& hmax_res)
+                           double dt, const Params& P, double& cfl_res, double& hmax_res,
+                           const Hook& hook = Hook())
 {
     using G = Geo<TX, TY>;   // tile geometry (LDS)
     using GP = Geo<MX, MY>;  // patch geometry (HBM)
@@ -355,6 +365,9 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         }
     }
     const int anywet = __syncthreads_or(wet);
+    static_assert(NT - 1 + NT * (SLOTS - 1) >= G::NPOS || SLOTS == 1,
+                  "the hook thread must have no position in the last slot");
+    if (threadIdx.x == NT - 1) hook();
 
     if (!anywet) {
         // Every face of the tile is dry on both sides: all fluctuations, waves
@@ -721,6 +734,54 @@ __global__ void __launch_bounds__(NT) k_step(StepArgs a)
     }
 }
 
+// What an active item needs before its tile can be loaded: the patch, its
+// descriptor and its scenario's buffer, dt and Manning n.  Reading them is a
+// chain of dependent memory accesses (list -> descriptor -> control record),
+// so the item before fetches them for its successor (ItemPrefetch) and the
+// item starts with them in LDS.
+struct ItemHead {
+    PatchDesc d;
+    int patch, cur;
+    double dt, manning;
+};
+
+__device__ __forceinline__ void load_head(const BatchArgs& a, int patch, ItemHead* h)
+{
+    // the descriptor as 8-byte words, in registers (a struct copy would go
+    // through the private stack)
+    static_assert(sizeof(PatchDesc) % 8 == 0 && offsetof(PatchDesc, scen) % 8 == 0, "PatchDesc layout");
+    constexpr int W = sizeof(PatchDesc) / 8;
+    const long long* s = reinterpret_cast<const long long*>(a.desc + patch);
+    long long w[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) w[k] = s[k];
+    long long* d = reinterpret_cast<long long*>(&h->d);
+#pragma unroll
+    for (int k = 0; k < W; k++) d[k] = w[k];
+    const int scen = (int)(unsigned)(w[offsetof(PatchDesc, scen) / 8] & 0xffffffffll);
+    const ScenCtrl& c = a.ctrl[scen];
+    h->patch = patch;
+    h->cur = c.cur;
+    h->dt = c.dt;
+    h->manning = c.manning_n;
+}
+
+// Run by one thread of the current active item: takes the next item from
+// the cursor, publishes it, and when it is active fills its ItemHead.
+template <int NTILES>
+struct ItemPrefetch {
+    const BatchArgs* pa;
+    int* next_slot;
+    ItemHead* next_head;
+    int nwet;
+    __device__ void operator()() const
+    {
+        const int next = atomicAdd(&pa->counters[NEXT_WET], 1);
+        *next_slot = next;
+        if (next < nwet) load_head(*pa, pa->list_wet[next / NTILES], next_head);
+    }
+};
+
 // Persistent step kernel over the device work list of active patches:
 // each workgroup takes the next (patch, tile) entry from an atomic counter,
 // pulls the tile's halo from the patch and its neighbours, steps it, and
@@ -731,7 +792,8 @@ __global__ void __launch_bounds__(NT) k_step(StepArgs a)
 // of the persistent loop would keep them live across the whole step and
 // spill.)
 template <int MX, int MY, int TX, int TY, int NT, bool DEF>
-__device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, int next, int* next_slot)
+__device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, const ItemHead* head, ItemHead* next_head,
+                                         int* next_slot, int nwet)
 {
     using G = Geo<TX, TY>;
     using TL = Tiling<MX, MY, TX, TY>;
@@ -740,20 +802,21 @@ __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, int next,
     __shared__ double F[6 * G::NF];
     __shared__ double red[2 * (NT / 64)];
     const BatchArgs& a = *pa;
-    const int patch = a.list_wet[idx / TL::NTILES];
+    const int patch = head->patch;
     int ox, oy;
     TL::origin(idx % TL::NTILES, ox, oy);
-    const int scen = a.desc[patch].scen;
-    const int cur = a.ctrl[scen].cur;
-    const double dt = a.ctrl[scen].dt;
+    const int scen = head->d.scen;
+    const int cur = head->cur;
+    const double dt = head->dt;
     const double* src = cur ? a.q1 : a.q0;
     double* dst = cur ? a.q0 : a.q1;
     Params P = a.P;
-    P.manning_n = a.ctrl[scen].manning_n;
+    P.manning_n = head->manning;
     if (threadIdx.x == 0) a.dry_age[patch] = 0;
     double cfl = 0.0, hm = 0.0;
-    step_patch<MX, MY, TX, TY, NT, true, DEF>(T, E, F, red, a.desc + patch, patch, ox, oy, src, dst, a.aux, dt, P,
-                                         cfl, hm);
+    const ItemPrefetch<TL::NTILES> pf{pa, next_slot, next_head, nwet};
+    step_patch<MX, MY, TX, TY, NT, true, DEF>(T, E, F, red, &head->d, patch, ox, oy, src, dst, a.aux, dt, P,
+                                         cfl, hm, pf);
     if (threadIdx.x == 0) {
         bool last = true;
         if (TL::NTILES > 1) last = tile_fold(a.tile_cfl, a.tile_h, a.tile_cnt, patch, TL::NTILES, cfl, hm);
@@ -766,7 +829,6 @@ __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, int next,
             // scenario's maximum starts at +0.0).  NaN still propagates.
             if (cfl != 0.0) atomicMax(a.cflbits + scen, (unsigned long long)__double_as_longlong(cfl));
         }
-        *next_slot = next;
     }
     __syncthreads();
 }
@@ -864,23 +926,30 @@ template <int MX, int MY, int TX, int TY, int NT, bool DEF>
 __global__ void __launch_bounds__(NT) k_all(const BatchArgs* pa)
 {
     using TL = Tiling<MX, MY, TX, TY>;
-    __shared__ int item[2];
     constexpr int DPI = dry_per_item<MX, MY, NT>();
+    __shared__ int item[2];
+    __shared__ ItemHead head[2];
     __shared__ int unsteady[DPI];  // per dry patch of an item: a cell at dry_tolerance
-    if (threadIdx.x == 0) item[0] = atomicAdd(&pa->counters[NEXT_WET], 1);
+    const int nwet = pa->counters[CNT_WET] * TL::NTILES;
+    if (threadIdx.x == 0) {
+        item[0] = atomicAdd(&pa->counters[NEXT_WET], 1);
+        if (item[0] < nwet) load_head(*pa, pa->list_wet[item[0] / TL::NTILES], &head[0]);
+    }
     if (threadIdx.x < DPI) unsteady[threadIdx.x] = 0;
     __syncthreads();
-    const int nwet = pa->counters[CNT_WET] * TL::NTILES;
     const int limit = nwet + (pa->counters[CNT_DRY] + DPI - 1) / DPI;
     for (int it = 0;; it++) {
         const int idx = item[it & 1];
         if (idx >= limit) break;
-        int next = 0;
-        if (threadIdx.x == 0) next = atomicAdd(&pa->counters[NEXT_WET], 1);
-        if (idx < nwet)
-            wet_item<MX, MY, TX, TY, NT, DEF>(pa, idx, next, &item[(it + 1) & 1]);
-        else
+        if (idx < nwet) {
+            // active items come first in the cursor's order, so an active
+            // item's successor's head is always prefetched by an active item
+            wet_item<MX, MY, TX, TY, NT, DEF>(pa, idx, &head[it & 1], &head[(it + 1) & 1], &item[(it + 1) & 1], nwet);
+        } else {
+            int next = 0;
+            if (threadIdx.x == 0) next = atomicAdd(&pa->counters[NEXT_WET], 1);
             dry_item<MX, MY, NT>(pa, idx - nwet, next, &item[(it + 1) & 1], unsteady);
+        }
     }
 }
 
