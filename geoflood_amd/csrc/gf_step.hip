@@ -404,7 +404,10 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
     }
 
     double cfl = 0.0;
-    double Wr[SLOTS][7], Sr[SLOTS][3];
+    // A face's own compressed waves go to LDS for its neighbours' limiter and
+    // are read back from there by the face itself, so that they do not stay
+    // in registers while the other slots solve their Riemann problems.
+    double Sr[SLOTS][3];
     double AP[SLOTS][3], AM[SLOTS][3], GA[SLOTS][3], GB[SLOTS][3];
 
     // ================= x-faces: normal Riemann problems =================
@@ -424,12 +427,13 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
                 cfl = fmaxf64(cfl, -v);
             }
         }
-        compress(w, Wr[s]);
+        double wc[7];
+        compress(w, wc);
         Sr[s][0] = sw[0]; Sr[s][1] = sw[1]; Sr[s][2] = sw[2];
         if (in[s]) {
             const int p = tid + s * NT;
 #pragma unroll
-            for (int k = 0; k < 7; k++) E[k * NPOS + p] = Wr[s][k];
+            for (int k = 0; k < 7; k++) E[k * NPOS + p] = wc[k];
         }
     }
     __syncthreads();
@@ -440,14 +444,15 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         const int i = pi[s], j = pj[s];
         const int p = tid + s * NT;
         if (in[s] && i >= 1 && i <= TX + 1 && j <= TY + 1) {
-            double wl[7], wr[7];
+            double wl[7], wc[7], wr[7];
 #pragma unroll
             for (int k = 0; k < 7; k++) {
                 wl[k] = E[k * NPOS + p - 1];
+                wc[k] = E[k * NPOS + p];
                 wr[k] = E[k * NPOS + p + 1];
             }
             double fxm[3], fxp[3];
-            face_second_order<0, DEF>(Wr[s], Sr[s], wl, wr, dtdx, P, fxm, fxp, AM[s], AP[s]);
+            face_second_order<0, DEF>(wc, Sr[s], wl, wr, dtdx, P, fxm, fxp, AM[s], AP[s]);
             if (j >= 1 && j <= TY) {
                 const int f = G::fi_(i, j);
 #pragma unroll
@@ -532,12 +537,13 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
                 cfl = fmaxf64(cfl, -v);
             }
         }
-        compress(w, Wr[s]);
+        double wc[7];
+        compress(w, wc);
         Sr[s][0] = sw[0]; Sr[s][1] = sw[1]; Sr[s][2] = sw[2];
         if (in[s]) {
             const int p = tid + s * NT;
 #pragma unroll
-            for (int k = 0; k < 7; k++) E[k * NPOS + p] = Wr[s][k];
+            for (int k = 0; k < 7; k++) E[k * NPOS + p] = wc[k];
         }
     }
     __syncthreads();
@@ -551,13 +557,14 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
 #pragma unroll
         for (int k = 0; k < 3; k++) { GMr[s][k] = 0.0; GPr[s][k] = 0.0; AM[s][k] = 0.0; AP[s][k] = 0.0; }
         if (in[s] && i <= TX + 1 && j >= 1 && j <= TY + 1) {
-            double wl[7], wr[7], fm[3], fp[3];
+            double wl[7], wc[7], wr[7], fm[3], fp[3];
 #pragma unroll
             for (int k = 0; k < 7; k++) {
                 wl[k] = E[k * NPOS + p - PX];
+                wc[k] = E[k * NPOS + p];
                 wr[k] = E[k * NPOS + p + PX];
             }
-            face_second_order<1, DEF>(Wr[s], Sr[s], wl, wr, dtdy, P, fm, fp, AM[s], AP[s]);
+            face_second_order<1, DEF>(wc, Sr[s], wl, wr, dtdy, P, fm, fp, AM[s], AP[s]);
             // gm = GT + faddm in physical order (h, hu, hv) = local (0, 2, 1)
             GMr[s][0] = GT[s][0] + fm[0];
             GMr[s][1] = GT[s][1] + fm[2];
