@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--output-patches", type=int, default=1024,
                     help="patches of the final state written as one ASCII frame (0: skip)")
+    ap.add_argument("--patch-size", type=int, default=32, choices=(8, 16, 32, 64),
+                    help="patch size of the workload (the same 4096^2 cells)")
     ap.add_argument("--regrid-families", type=int, default=1024,
                     help="coarse patches refined and averaged back in the regrid measurement (0: skip)")
     ap.add_argument("--host-steps", type=int, default=2,
@@ -126,6 +128,8 @@ def main():
                     help="only run a 1 GiB device-to-device copy kernel (torch) three times; "
                          "used to scale the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    global MX
+    MX = args.patch_size
 
     if args.calibrate_copy:
         import torch
@@ -235,10 +239,10 @@ def main():
         # measured HBM bytes and VALU work per launch of this same trajectory
         # (rocprofv3 --pmc over the 500 event-timed launches, profiles/)
         traffic = valu = None
-        kv = profile_values("k_all_32x32_uniform4096_traffic.txt")
+        kv = profile_values("k_all_32x32_uniform4096_traffic.txt") if MX == 32 else {}
         if "hbm_bytes_per_launch" in kv:
             traffic = round(float(kv["hbm_bytes_per_launch"]))
-        kv = profile_values("k_all_32x32_uniform4096_valu.txt")
+        kv = profile_values("k_all_32x32_uniform4096_valu.txt") if MX == 32 else {}
         if "fp64_flops_per_launch" in kv:
             fl = float(kv["fp64_flops_per_launch"]) / (k_avg_ms * 1e-3) / 1e12
             li = float(kv["valu_lane_instr_per_launch"]) / (k_avg_ms * 1e-3) / 1e12
@@ -248,7 +252,7 @@ def main():
                              "profiles/k_all_32x32_uniform4096_valu.txt counters, over this run's k_all time")
         roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                        kernel="k_all<32,32,32,32,%d>" % variants[pool.variants()[0]][2],
+                        kernel="k_all<%d,%d,%d,%d,%d>" % ((MX, MX) + tuple(variants[pool.variants()[0]])),
                         kernel_avg_ms=round(k_avg_ms, 5), bytes_per_launch=round(k_bytes),
                         active_patches_per_launch=round(act / launches, 1),
                         dry_patches_per_launch=round(dry / launches, 1),
@@ -428,7 +432,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "uniform 4096x4096 dam break over a seeded fractal DEM "
-                        "(16384 patches of 32x32, mbc=2, single level)",
+                        f"({(N // MX) ** 2} patches of {MX}x{MX}, mbc=2, single level)",
             "dem": "spectral slope -2, RMS 3 m, seed 0, plus 1/500 slope down +x; dx = 1 m",
             "initial_state": "h = 10 m for x < L/2, y >= 3L/4; dry elsewhere; u = v = 0",
             "numerics": "g 9.81, drytol 1e-3, Manning n 0.03, MC limiter, transverse 2, walls",
