@@ -137,10 +137,17 @@ __device__ __forceinline__ double wall_hstar(double h, double u, double sg, doub
 // bathymetry.  Outputs: speeds s[3] and f-waves w[mw][k] with k in
 // direction-local order.  Returns false when both sides are dry (the
 // reference leaves s and the f-waves at zero).
+// With RC, rc holds the right-hand cell's huR/hR, hvR/hR and sqrt(g*hR) as the
+// caller formed them for a wet cell (the same operations on the same values).
+struct RightCell {
+    double u, v, sg;
+};
+
+template <bool RC = false>
 __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, double bL,
                                           double hR, double huR, double hvR, double bR,
                                           double g, double drytol,
-                                          double s[3], double w[3][3])
+                                          double s[3], double w[3][3], const RightCell& rc = RightCell{})
 {
     if (hL <= drytol && hR <= drytol) {
 #pragma unroll
@@ -151,9 +158,10 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
         return false;
     }
     double uR, vR, phiR, uL, vL, phiL;
+    const bool r_wet = hR > drytol;  // then hR keeps its value below
     if (hR > drytol) {
-        uR = huR / hR;
-        vR = hvR / hR;
+        uR = RC ? rc.u : huR / hR;
+        vR = RC ? rc.v : hvR / hR;
         phiR = 0.5 * g * (hR * hR) + (huR * huR) / hR;
     } else {
         hR = 0.0; huR = 0.0; hvR = 0.0; uR = 0.0; vR = 0.0; phiR = 0.0;
@@ -194,7 +202,7 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
     }
 
     const double sghL = sqrt(g * hL);
-    const double sghR = sqrt(g * hR);
+    const double sghR = (RC && r_wet) ? rc.sg : sqrt(g * hR);
     const double sL = uL - sghL;
     const double sR = uR + sghR;
     const double uhat = (sghL * uL + sghR * uR) / (sghR + sghL);
@@ -364,6 +372,26 @@ struct Rpt2Cell {
     bool active;
     double u, s0, s1, s2, eta, topo1, topo3;
 };
+
+// The same with the cell's hun/h, hvt/h and sqrt(g*h) formed by the caller
+// (used only when h > tol, where they are what rpt2_setup would form).
+__device__ __forceinline__ Rpt2Cell rpt2_setup_c(double h, double u_c, double v_c, double sg_c, double b,
+                                                 double topo1, double topo3, double tol)
+{
+    Rpt2Cell c;
+    c.active = false;
+    c.u = 0.0; c.s0 = 0.0; c.s1 = 0.0; c.s2 = 0.0; c.eta = 0.0; c.topo1 = topo1; c.topo3 = topo3;
+    if (h <= tol) return c;
+    const double eta = h + b;
+    if (eta < fminf64(topo1, topo3)) return c;
+    c.active = true;
+    c.u = u_c;
+    c.s0 = v_c - sg_c;
+    c.s1 = v_c;
+    c.s2 = v_c + sg_c;
+    c.eta = eta;
+    return c;
+}
 
 __device__ __forceinline__ Rpt2Cell rpt2_setup(double h, double hun, double hvt, double b,
                                                double topo1, double topo3, double g, double tol)

@@ -408,6 +408,27 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
     // are read back from there by the face itself, so that they do not stay
     // in registers while the other slots solve their Riemann problems.
     double Sr[SLOTS][3];
+    // each position's own cell (the right-hand cell of its x- and y-face and the
+    // cell of both transverse splits): hu/h, hv/h and sqrt(g*h) once, for a wet
+    // cell, instead of in each of those four places.  Kept for the large tiles
+    // of up to 768 threads only: there one workgroup per CU is set by LDS and
+    // the extra registers are free (up to 168 at three waves per SIMD), while
+    // the other variants' occupancy is set by registers.
+    constexpr bool OWN = TX >= 32 && TY >= 32 && NT <= 768;
+    double CU[SLOTS], CV[SLOTS], CS[SLOTS];
+#pragma unroll
+    for (int s = 0; s < SLOTS; s++) {
+        CU[s] = 0.0; CV[s] = 0.0; CS[s] = 0.0;
+        if (OWN) {
+            const int k = G::ti(pi[s], pj[s]);
+            const double h = in[s] ? T[k] : 0.0;
+            if (h > drytol) {
+                CU[s] = T[NP2 + k] / h;
+                CV[s] = T[2 * NP2 + k] / h;
+                CS[s] = sqrt(g * h);
+            }
+        }
+    }
     double AP[SLOTS][3], AM[SLOTS][3], GA[SLOTS][3], GB[SLOTS][3];
 
     // ================= x-faces: normal Riemann problems =================
@@ -418,8 +439,9 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         double w[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
         if (in[s] && j <= TY + 1) {
             const int kL = G::ti(i - 1, j), kR = G::ti(i, j);
-            rpn2_face(T[kL], T[NP2 + kL], T[2 * NP2 + kL], T[3 * NP2 + kL],
-                      T[kR], T[NP2 + kR], T[2 * NP2 + kR], T[3 * NP2 + kR], g, drytol, sw, w);
+            rpn2_face<OWN>(T[kL], T[NP2 + kL], T[2 * NP2 + kL], T[3 * NP2 + kL],
+                           T[kR], T[NP2 + kR], T[2 * NP2 + kR], T[3 * NP2 + kR], g, drytol, sw, w,
+                           RightCell{CU[s], CV[s], CS[s]});
 #pragma unroll
             for (int mw = 0; mw < 3; mw++) {
                 const double v = dtdx * sw[mw];
@@ -487,9 +509,11 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         for (int k = 0; k < 3; k++) { GA[s][k] = 0.0; GB[s][k] = 0.0; }
         if (transverse && in[s] && i >= 1 && i <= TX && j <= TY + 1) {
             const int kc = G::ti(i, j);
-            const Rpt2Cell C = rpt2_setup(T[kc], T[NP2 + kc], T[2 * NP2 + kc], T[3 * NP2 + kc],
-                                          T[3 * NP2 + G::ti(i, j - 1)], T[3 * NP2 + G::ti(i, j + 1)],
-                                          g, drytol);
+            const Rpt2Cell C =
+                OWN ? rpt2_setup_c(T[kc], CU[s], CV[s], CS[s], T[3 * NP2 + kc], T[3 * NP2 + G::ti(i, j - 1)],
+                                   T[3 * NP2 + G::ti(i, j + 1)], drytol)
+                    : rpt2_setup(T[kc], T[NP2 + kc], T[2 * NP2 + kc], T[3 * NP2 + kc], T[3 * NP2 + G::ti(i, j - 1)],
+                                 T[3 * NP2 + G::ti(i, j + 1)], g, drytol);
             const double amn[3] = {E[p + 1], E[NPOS + p + 1], E[2 * NPOS + p + 1]};
             cell_transverse(C, amn, AP[s], hdx, GA[s], GB[s]);  // GA = G1 (below), GB = G2 (above)
         }
@@ -528,8 +552,9 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         if (in[s] && i <= TX + 1) {
             const int kL = G::ti(i, j - 1), kR = G::ti(i, j);
             // direction-local order for y: (h, hv, hu)
-            rpn2_face(T[kL], T[2 * NP2 + kL], T[NP2 + kL], T[3 * NP2 + kL],
-                      T[kR], T[2 * NP2 + kR], T[NP2 + kR], T[3 * NP2 + kR], g, drytol, sw, w);
+            rpn2_face<OWN>(T[kL], T[2 * NP2 + kL], T[NP2 + kL], T[3 * NP2 + kL],
+                           T[kR], T[2 * NP2 + kR], T[NP2 + kR], T[3 * NP2 + kR], g, drytol, sw, w,
+                           RightCell{CV[s], CU[s], CS[s]});
 #pragma unroll
             for (int mw = 0; mw < 3; mw++) {
                 const double v = dtdy * sw[mw];
@@ -594,9 +619,11 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         for (int k = 0; k < 3; k++) { GA[s][k] = 0.0; GB[s][k] = 0.0; }
         if (transverse && in[s] && i <= TX + 1 && j >= 1 && j <= TY) {
             const int kc = G::ti(i, j);
-            const Rpt2Cell C = rpt2_setup(T[kc], T[2 * NP2 + kc], T[NP2 + kc], T[3 * NP2 + kc],
-                                          T[3 * NP2 + G::ti(i - 1, j)], T[3 * NP2 + G::ti(i + 1, j)],
-                                          g, drytol);
+            const Rpt2Cell C =
+                OWN ? rpt2_setup_c(T[kc], CV[s], CU[s], CS[s], T[3 * NP2 + kc], T[3 * NP2 + G::ti(i - 1, j)],
+                                   T[3 * NP2 + G::ti(i + 1, j)], drytol)
+                    : rpt2_setup(T[kc], T[2 * NP2 + kc], T[NP2 + kc], T[3 * NP2 + kc], T[3 * NP2 + G::ti(i - 1, j)],
+                                 T[3 * NP2 + G::ti(i + 1, j)], g, drytol);
             const double amn[3] = {E[p + PX], E[NPOS + p + PX], E[2 * NPOS + p + PX]};
             cell_transverse(C, amn, AP[s], hdy, GA[s], GB[s]);  // GA = left (side 1), GB = right (side 2)
         }
