@@ -181,4 +181,4 @@ def last_timing():
     return t.value, s.value, n.value
 
 
-from .pool import Pool, step2_batch  # noqa: E402,F401
+from .pool import HostBatch, Pool, step2_batch  # noqa: E402,F401

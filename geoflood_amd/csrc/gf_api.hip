@@ -9,6 +9,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -44,6 +48,71 @@ int fail(int code, const char* fmt, ...)
         if (e_ != hipSuccess) return fail(GF_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
     } while (0)
 
+// Host threads kept for the staging copies between pageable patch memory and
+// the pinned staging buffers: run(n, f) calls f(0..n-1) over the workers and
+// the calling thread and returns when every call has ended.
+class HostWorkers {
+public:
+    explicit HostWorkers(int nworkers)
+    {
+        for (int t = 0; t < nworkers; t++) th_.emplace_back([this] { loop(); });
+    }
+    ~HostWorkers()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            quit_ = true;
+        }
+        go_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    void run(int n, const std::function<void(int)>& f)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &f;
+            n_ = n;
+            done_ = 0;
+            next_.store(0);
+            gen_++;
+        }
+        go_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu_);
+        // every worker reports each generation, so none can lag into the next
+        finished_.wait(lk, [this] { return done_ == (int)th_.size(); });
+        job_ = nullptr;
+    }
+
+private:
+    void work()
+    {
+        for (int k; (k = next_.fetch_add(1)) < n_;) (*job_)(k);
+    }
+    void loop()
+    {
+        unsigned seen = 0;
+        for (;;) {
+            std::unique_lock<std::mutex> lk(mu_);
+            go_.wait(lk, [&] { return quit_ || gen_ != seen; });
+            if (quit_) return;
+            seen = gen_;
+            lk.unlock();
+            work();
+            lk.lock();
+            if (++done_ == (int)th_.size()) finished_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable go_, finished_;
+    const std::function<void(int)>* job_ = nullptr;
+    int n_ = 0, done_ = 0;
+    std::atomic<int> next_{0};
+    unsigned gen_ = 0;
+    bool quit_ = false;
+};
+
 struct Lib {
     bool init = false;
     int device = 0;
@@ -52,6 +121,8 @@ struct Lib {
     gf_params user{};
     int num_cu = 256;
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    hipEvent_t ev_half[2] = {nullptr, nullptr};  // the two halves of a pool's staging buffers
+    HostWorkers* workers = nullptr;  // created on first use, joined by gf_finalize
     std::vector<hipEvent_t> ev_step;  // pairs around step launches
     std::vector<hipEvent_t> ev_tune;  // pair around the autotuner's candidate launches
     unsigned params_gen = 1;  // bumped by gf_set_params: pools re-sync their batch arguments
@@ -193,6 +264,8 @@ int gf_init(int device)
     HIPCHK(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&g.ev_begin));
     HIPCHK(hipEventCreate(&g.ev_end));
+    HIPCHK(hipEventCreateWithFlags(&g.ev_half[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&g.ev_half[1], hipEventDisableTiming));
     HIPCHK(hipDeviceGetAttribute(&g.num_cu, hipDeviceAttributeMultiprocessorCount, device));
     HIPCHK(hipMalloc(&g.d_max, sizeof(double)));
     HIPCHK(hipHostMalloc(&g.h_max, sizeof(double), hipHostMallocDefault));
@@ -216,6 +289,11 @@ int gf_finalize(void)
     g.ev_step.clear();
     hipEventDestroy(g.ev_begin);
     hipEventDestroy(g.ev_end);
+    hipEventDestroy(g.ev_half[0]);
+    hipEventDestroy(g.ev_half[1]);
+    g.ev_half[0] = g.ev_half[1] = nullptr;
+    delete g.workers;
+    g.workers = nullptr;
     hipFree(g.d_max);
     hipHostFree(g.h_max);
     hipStreamDestroy(g.stream);
@@ -517,6 +595,98 @@ int gf_pool_describe(gf_pool* p, int first, int n, const double* dx, const doubl
     return GF_OK;
 }
 
+}  // extern "C"
+
+// Host copies of n items spread over up to 16 host threads (the caller's and
+// up to 15 kept workers).
+static void host_parallel(int n, const std::function<void(int)>& f)
+{
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (nt <= 1 || n <= 1) {
+        for (int k = 0; k < n; k++) f(k);
+        return;
+    }
+    if (!g.workers) g.workers = new HostWorkers(nt - 1);
+    g.workers->run(n, f);
+}
+
+// Split of a pool's staging buffers (min(kChunk, capacity) * 4 * np2 doubles
+// each) into nbuf parts of `part` patches of `per` doubles: two parts when two
+// patches fit, so that the DMA and (un)pack kernel of one part overlap the
+// host copies into or out of the other.
+struct StageSplit {
+    int nbuf, part;
+};
+static StageSplit stage_split(const gf_pool* p, size_t per)
+{
+    const size_t fit = std::min<size_t>(kChunk, p->capacity) * 4 * p->np2 / per;
+    return fit >= 2 ? StageSplit{2, (int)(fit / 2)} : StageSplit{1, (int)fit};
+}
+
+// Host patches -> pool slots [first, first+n) of buffer buf through the
+// pinned and device staging buffers. put(k, q, aux) copies host patch k's
+// 3*np2 values of q (and maux*np2 of aux when with_aux) into the pinned
+// staging.
+template <class Put>
+static int staged_upload(gf_pool* p, int first, int n, int buf, int maux, bool with_aux, Put&& put)
+{
+    const size_t np2 = p->np2;
+    const size_t per = (3 + (with_aux ? (size_t)maux : 0)) * np2;
+    const StageSplit sp = stage_split(p, per);
+    if (sp.part < 1) return fail(GF_EUNSUPPORTED, "maux = %d does not fit the staging buffer", maux);
+    for (int off = 0, c = 0; off < n; off += sp.part, c++) {
+        const int b = c % sp.nbuf, m = std::min(sp.part, n - off);
+        double* hs = p->h_stage + (size_t)b * sp.part * per;
+        double* ds = p->d_stage + (size_t)b * sp.part * per;
+        if (c >= sp.nbuf) HIPCHK(hipEventSynchronize(g.ev_half[b]));
+        const size_t qd = (size_t)m * 3 * np2;
+        host_parallel(m, [&](int k) {
+            put(off + k, hs + (size_t)k * 3 * np2, with_aux ? hs + qd + (size_t)k * maux * np2 : nullptr);
+        });
+        HIPCHK(hipMemcpyAsync(ds, hs, (qd + (with_aux ? (size_t)m * maux * np2 : 0)) * sizeof(double),
+                              hipMemcpyHostToDevice, g.stream));
+        HIPCHK(gf::launch_pack(ds, with_aux ? ds + qd : nullptr, p->q[buf], p->aux, first + off, m, (int)np2, maux,
+                               g.stream));
+        HIPCHK(hipEventRecord(g.ev_half[b], g.stream));
+    }
+    return GF_OK;
+}
+
+// Pool slots [first, first+n) of buffer buf -> host, the reverse pipeline:
+// get(k, q) copies patch k's 3*np2 values out of the pinned staging.
+template <class Get>
+static int staged_download(gf_pool* p, int first, int n, int buf, Get&& get)
+{
+    const size_t np2 = p->np2;
+    const size_t per = 3 * np2;
+    const StageSplit sp = stage_split(p, per);
+    const int nch = (n + sp.part - 1) / sp.part;
+    auto enqueue = [&](int c) -> int {
+        const int b = c % sp.nbuf, off = c * sp.part, m = std::min(sp.part, n - off);
+        double* hs = p->h_stage + (size_t)b * sp.part * per;
+        double* ds = p->d_stage + (size_t)b * sp.part * per;
+        HIPCHK(gf::launch_unpack(ds, p->q[buf], first + off, m, (int)np2, g.stream));
+        HIPCHK(hipMemcpyAsync(hs, ds, (size_t)m * per * sizeof(double), hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipEventRecord(g.ev_half[b], g.stream));
+        return GF_OK;
+    };
+    for (int c = 0; c < nch; c++) {
+        // with two parts, chunk c+1's unpack and DMA run while the host copies
+        // chunk c out; that part's previous chunk was copied out before
+        if (c == 0 || sp.nbuf == 1)
+            if (int r = enqueue(c)) return r;
+        if (sp.nbuf == 2 && c + 1 < nch)
+            if (int r = enqueue(c + 1)) return r;
+        const int b = c % sp.nbuf, off = c * sp.part, m = std::min(sp.part, n - off);
+        const double* hs = p->h_stage + (size_t)b * sp.part * per;
+        HIPCHK(hipEventSynchronize(g.ev_half[b]));
+        host_parallel(m, [&](int k) { get(off + k, hs + (size_t)k * per); });
+    }
+    return GF_OK;
+}
+
+extern "C" {
+
 int gf_pool_upload(gf_pool* p, int first, int n, int buf, const double* q_aos, const double* aux_aos, int maux)
 {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
@@ -525,18 +695,12 @@ int gf_pool_upload(gf_pool* p, int first, int n, int buf, const double* q_aos, c
     if (n > 0 && !q_aos) return fail(GF_EINVAL, "q is NULL");
     if (aux_aos && maux < 1) return fail(GF_EINVAL, "maux must be >= 1");
     const size_t np2 = p->np2;
-    const size_t chunk = std::min<size_t>(kChunk, p->capacity);
-    for (int off = 0; off < n; off += (int)chunk) {
-        const int m = std::min<int>((int)chunk, n - off);
-        const size_t qd = (size_t)m * 3 * np2;
-        const size_t ad = aux_aos ? (size_t)m * maux * np2 : 0;
-        std::memcpy(p->h_stage, q_aos + (size_t)off * 3 * np2, qd * sizeof(double));
-        if (aux_aos) std::memcpy(p->h_stage + qd, aux_aos + (size_t)off * maux * np2, ad * sizeof(double));
-        HIPCHK(hipMemcpyAsync(p->d_stage, p->h_stage, (qd + ad) * sizeof(double), hipMemcpyHostToDevice, g.stream));
-        HIPCHK(gf::launch_pack(p->d_stage, aux_aos ? p->d_stage + qd : nullptr, p->q[buf], p->aux,
-                               first + off, m, (int)np2, maux, g.stream));
-        HIPCHK(hipStreamSynchronize(g.stream));
-    }
+    if (int r = staged_upload(p, first, n, buf, aux_aos ? maux : 1, aux_aos != nullptr,
+                              [&](int k, double* q, double* a) {
+                                  std::memcpy(q, q_aos + (size_t)k * 3 * np2, 3 * np2 * sizeof(double));
+                                  if (a) std::memcpy(a, aux_aos + (size_t)k * maux * np2, maux * np2 * sizeof(double));
+                              }))
+        return r;
     // interior depth maxima of the uploaded state (classification of the first attempt)
     HIPCHK(gf::launch_hmax(p->mx, p->my, p->q[buf], p->hmax2 + (size_t)buf * p->capacity, first, n, g.stream));
     if (int r = reset_dry_age(p, first, n)) return r;
@@ -551,16 +715,9 @@ int gf_pool_download(gf_pool* p, int first, int n, int buf, double* q_aos)
     if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
     if (n > 0 && !q_aos) return fail(GF_EINVAL, "q is NULL");
     const size_t np2 = p->np2;
-    const size_t chunk = std::min<size_t>(kChunk, p->capacity);
-    for (int off = 0; off < n; off += (int)chunk) {
-        const int m = std::min<int>((int)chunk, n - off);
-        const size_t qd = (size_t)m * 3 * np2;
-        HIPCHK(gf::launch_unpack(p->d_stage, p->q[buf], first + off, m, (int)np2, g.stream));
-        HIPCHK(hipMemcpyAsync(p->h_stage, p->d_stage, qd * sizeof(double), hipMemcpyDeviceToHost, g.stream));
-        HIPCHK(hipStreamSynchronize(g.stream));
-        std::memcpy(q_aos + (size_t)off * 3 * np2, p->h_stage, qd * sizeof(double));
-    }
-    return GF_OK;
+    return staged_download(p, first, n, buf, [&](int k, const double* q) {
+        std::memcpy(q_aos + (size_t)k * 3 * np2, q, 3 * np2 * sizeof(double));
+    });
 }
 
 int gf_pool_download_aux(gf_pool* p, int first, int n, double* aux_aos)
@@ -1098,32 +1255,19 @@ int gf_step2_batch(int mx, int my, int mbc, int maux, int n, double* const* q, d
     }
     p->n = n;
     HIPCHK(hipMemcpyAsync(p->desc, p->h_desc.data(), sizeof(PatchDesc) * n, hipMemcpyHostToDevice, g.stream));
-    // pack: host patches -> contiguous pinned staging -> device SoA (chunked)
-    const size_t chunk = std::min<size_t>(kChunk, p->capacity);
-    for (int off = 0; off < n; off += (int)chunk) {
-        const int m = std::min<int>((int)chunk, n - off);
-        const size_t qd = (size_t)m * 3 * np2;
-        HIPCHK(hipStreamSynchronize(g.stream));
-        for (int k = 0; k < m; k++) {
-            std::memcpy(p->h_stage + (size_t)k * 3 * np2, q[off + k], 3 * np2 * sizeof(double));
-            std::memcpy(p->h_stage + qd + (size_t)k * maux * np2, aux[off + k], maux * np2 * sizeof(double));
-        }
-        HIPCHK(hipMemcpyAsync(p->d_stage, p->h_stage, (qd + (size_t)m * maux * np2) * sizeof(double),
-                              hipMemcpyHostToDevice, g.stream));
-        HIPCHK(gf::launch_pack(p->d_stage, p->d_stage + qd, p->q[0], p->aux, off, m, (int)np2, maux, g.stream));
-    }
+    // pack: host patches -> pinned staging -> device SoA, then back the same way
+    if (int r = staged_upload(p, 0, n, 0, maux, true, [&](int k, double* sq, double* sa) {
+            std::memcpy(sq, q[k], 3 * np2 * sizeof(double));
+            std::memcpy(sa, aux[k], maux * np2 * sizeof(double));
+        }))
+        return r;
     HIPCHK(gf::launch_step(mx, my, p->variant, step_args(p, 0, n, p->q[0], nullptr, nullptr, dt, 0), g.stream));
     HIPCHK(gf::launch_max(p->cfl, n, g.d_max, g.stream));
     HIPCHK(hipMemcpyAsync(g.h_max, g.d_max, sizeof(double), hipMemcpyDeviceToHost, g.stream));
-    for (int off = 0; off < n; off += (int)chunk) {
-        const int m = std::min<int>((int)chunk, n - off);
-        const size_t qd = (size_t)m * 3 * np2;
-        HIPCHK(gf::launch_unpack(p->d_stage, p->q[0], off, m, (int)np2, g.stream));
-        HIPCHK(hipMemcpyAsync(p->h_stage, p->d_stage, qd * sizeof(double), hipMemcpyDeviceToHost, g.stream));
-        HIPCHK(hipStreamSynchronize(g.stream));
-        for (int k = 0; k < m; k++)
-            std::memcpy(q[off + k], p->h_stage + (size_t)k * 3 * np2, 3 * np2 * sizeof(double));
-    }
+    if (int r = staged_download(p, 0, n, 0, [&](int k, const double* sq) {
+            std::memcpy(q[k], sq, 3 * np2 * sizeof(double));
+        }))
+        return r;
     if (maxcfl) *maxcfl = *g.h_max;
     return GF_OK;
 }

@@ -419,17 +419,45 @@ def gauge_file_append(directory: str, gid: int, level, t, rec):
           "gf_gauge_file_append")
 
 
+class HostBatch:
+    """Host patches in the reference layout (q (my+4, mx+4, 3), aux
+    (my+4, mx+4, maux) per patch) bound once for repeated gf_step2_batch
+    calls: the pointer arrays the C ABI takes are built here, as ForestClaw's
+    plugin holds its patch pointers, not on every step.  The arrays are kept
+    referenced and are updated in place by step()."""
+
+    def __init__(self, q_list, aux_list, dx, dy):
+        n = len(q_list)
+        if n == 0 or len(aux_list) != n:
+            raise ValueError("need the same, nonzero number of q and aux patches")
+        shp, ashp = q_list[0].shape, aux_list[0].shape
+        for q, a in zip(q_list, aux_list):
+            if q.shape != shp or a.shape != ashp:
+                raise ValueError("all patches must have the same shape")
+            if q.dtype != np.float64 or a.dtype != np.float64 \
+                    or not q.flags.c_contiguous or not a.flags.c_contiguous:
+                raise ValueError("expected C-contiguous float64 patches")
+        self.q_list, self.aux_list = list(q_list), list(aux_list)
+        self.n = n
+        self.my, self.mx = shp[0] - 2 * MBC, shp[1] - 2 * MBC
+        self.maux = ashp[-1]
+        self._qp = np.fromiter((q.ctypes.data for q in self.q_list), dtype=np.uintp, count=n)
+        self._ap = np.fromiter((a.ctypes.data for a in self.aux_list), dtype=np.uintp, count=n)
+        self._dx = np.ascontiguousarray(np.broadcast_to(np.asarray(dx, dtype=np.float64), (n,)))
+        self._dy = np.ascontiguousarray(np.broadcast_to(np.asarray(dy, dtype=np.float64), (n,)))
+
+    def step(self, t, dt):
+        """One gf_step2_batch call over every patch; returns maxcfl."""
+        pp = ct.POINTER(ct.POINTER(ct.c_double))
+        m = ct.c_double(0)
+        check(load().gf_step2_batch(self.mx, self.my, MBC, self.maux, self.n,
+                                    self._qp.ctypes.data_as(pp), self._ap.ctypes.data_as(pp),
+                                    dptr(self._dx), dptr(self._dy), t, dt, ct.byref(m)),
+              "gf_step2_batch")
+        return m.value
+
+
 def step2_batch(q_list, aux_list, dx, dy, t, dt):
     """Drop-in batch step over host patches in the reference layout
     (gf_step2_batch).  Arrays are updated in place; returns maxcfl."""
-    lib = load()
-    n = len(q_list)
-    my, mx = q_list[0].shape[0] - 2 * MBC, q_list[0].shape[1] - 2 * MBC
-    qp = (ct.POINTER(ct.c_double) * n)(*[dptr(a) for a in q_list])
-    ap = (ct.POINTER(ct.c_double) * n)(*[dptr(a) for a in aux_list])
-    dxa = np.ascontiguousarray(np.broadcast_to(np.asarray(dx, dtype=np.float64), (n,)))
-    dya = np.ascontiguousarray(np.broadcast_to(np.asarray(dy, dtype=np.float64), (n,)))
-    m = ct.c_double(0)
-    check(lib.gf_step2_batch(mx, my, MBC, aux_list[0].shape[-1], n, qp, ap, dptr(dxa), dptr(dya),
-                             t, dt, ct.byref(m)), "gf_step2_batch")
-    return m.value
+    return HostBatch(q_list, aux_list, dx, dy).step(t, dt)

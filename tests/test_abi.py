@@ -6,6 +6,7 @@ import glob
 import os
 import re
 
+import numpy as np
 import pytest
 
 import geoflood_amd as gf
@@ -62,3 +63,21 @@ def test_pool_bytes_accounting():
     per = b2 - b1
     # two q buffers (3 fields) + aux, 36x36 FP64, plus descriptors and control
     assert 7 * 36 * 36 * 8 * 1000 <= per <= 7 * 36 * 36 * 8 * 1000 + 300 * 1000
+
+
+def test_host_batch_binding_checks():
+    """HostBatch binds patch pointers once; mismatched or non-contiguous
+    patches are rejected before any library call."""
+    q = [np.zeros((12, 12, 3)) for _ in range(3)]
+    a = [np.zeros((12, 12, 1)) for _ in range(3)]
+    hb = gf.HostBatch(q, a, 0.1, 0.2)
+    assert (hb.n, hb.mx, hb.my, hb.maux) == (3, 8, 8, 1)
+    assert [int(x) for x in hb._qp] == [x.ctypes.data for x in q]
+    with pytest.raises(ValueError):
+        gf.HostBatch(q, a[:2], 0.1, 0.1)
+    with pytest.raises(ValueError):
+        gf.HostBatch(q[:2] + [np.zeros((12, 10, 3))], a, 0.1, 0.1)
+    with pytest.raises(ValueError):
+        gf.HostBatch([x[:, ::-1] for x in q], a, 0.1, 0.1)
+    with pytest.raises(ValueError):
+        gf.HostBatch([], [], 0.1, 0.1)
