@@ -394,12 +394,32 @@ def main():
             for _ in range(reps):
                 rp.average2coarse(cs, fs)
             ta = (time.perf_counter() - t0r) / reps
+            # coarse/fine ghost transfers: every coarse patch linked across its
+            # 8 face halves and 4 corners to the children slots (12 links each)
+            j = np.tile(np.arange(12), F)
+            lc = np.repeat(cs, 12)
+            lf = F + 4 * lc + j % 4
+            ls = np.where(j < 8, j // 2, 4 + (j - 8))
+            lg = np.where(j < 8, j % 2, 0)
+            rp.average_ghost(lc, lf, ls, lg)
+            rp.interpolate_ghost(lc, lf, ls, lg)
+            t0r = time.perf_counter()
+            for _ in range(reps):
+                rp.average_ghost(lc, lf, ls, lg)
+            tag = (time.perf_counter() - t0r) / reps
+            t0r = time.perf_counter()
+            for _ in range(reps):
+                rp.interpolate_ghost(lc, lf, ls, lg)
+            tig = (time.perf_counter() - t0r) / reps
         finally:
             rp.close()
         del q_aos, aux_aos
         regrid = dict(families=F, interpolate2fine_ms=round(1e3 * ti, 4), average2coarse_ms=round(1e3 * ta, 4),
-                      note="one call each over all families (32x32 patches), host round trip included")
-        log(f"regrid: {F} families, interpolate2fine {1e3 * ti:.3f} ms, average2coarse {1e3 * ta:.3f} ms")
+                      cf_ghost_links=int(len(lc)), average_ghost_ms=round(1e3 * tag, 4),
+                      interpolate_ghost_ms=round(1e3 * tig, 4),
+                      note="one call each over all families / links (32x32 patches), host round trip included")
+        log(f"regrid: {F} families, interpolate2fine {1e3 * ti:.3f} ms, average2coarse {1e3 * ta:.3f} ms; "
+            f"{len(lc)} coarse/fine ghost links, average {1e3 * tag:.3f} ms, interpolate {1e3 * tig:.3f} ms")
 
     # ---- CPU baseline (reference Fortran step on the host cores) ----
     cpu = None

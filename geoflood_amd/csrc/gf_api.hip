@@ -14,6 +14,7 @@
 #include <functional>
 #include <mutex>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -1686,6 +1687,94 @@ int gf_pool_average2coarse(gf_pool* p, int buf, int n, const int* coarse, const 
 int gf_pool_interpolate2fine(gf_pool* p, int buf, int n, const int* coarse, const int* fine)
 {
     return regrid_impl(p, buf, n, coarse, fine, true);
+}
+
+// Links of a coarse/fine ghost transfer: slots described and distinct per
+// link, side 0..7, igrid 0/1 on faces; each written region named once (the
+// coarse patch's for averaging, the fine patch's for interpolation); for
+// interpolation no slot may be both a coarse and a fine patch of the call,
+// since the coarse reads include the coarse patch's stored ghost cells.
+static int cf_ghost_check(gf_pool* p, int buf, int n, const int* cs, const int* fs, const int* side,
+                          const int* igrid, bool interp)
+{
+    if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
+    if (n < 0 || (n > 0 && (!cs || !fs || !side || !igrid))) return fail(GF_EINVAL, "invalid link lists");
+    if (p->mx != p->my || p->mx % 2) return fail(GF_EUNSUPPORTED, "coarse/fine ghost transfers need square patches of even size");
+    std::vector<char> role(p->capacity, 0);          // interpolation: 1 coarse, 2 fine
+    std::vector<uint16_t> seen(p->capacity, 0);      // regions written: 4 faces x 2 halves, 4 corners
+    for (int l = 0; l < n; l++) {
+        const int c = cs[l], f = fs[l], sd = side[l];
+        if (c < 0 || c >= p->capacity || !(p->h_desc[c].dx > 0.0)) return fail(GF_EINVAL, "link %d: coarse slot %d invalid", l, c);
+        if (f < 0 || f >= p->capacity || !(p->h_desc[f].dx > 0.0)) return fail(GF_EINVAL, "link %d: fine slot %d invalid", l, f);
+        if (c == f) return fail(GF_EINVAL, "link %d: coarse and fine slot are the same", l);
+        if (sd < 0 || sd > 7) return fail(GF_EINVAL, "link %d: side %d not in 0..7", l, sd);
+        if (sd < 4 && igrid[l] != 0 && igrid[l] != 1) return fail(GF_EINVAL, "link %d: igrid %d not 0 or 1", l, igrid[l]);
+        const uint16_t bit = (uint16_t)(1u << (sd < 4 ? 2 * sd + igrid[l] : 8 + (sd - 4)));
+        const int w = interp ? f : c;
+        if (seen[w] & bit) return fail(GF_EINVAL, "link %d: slot %d's region is written twice", l, w);
+        seen[w] |= bit;
+        if (interp) {
+            if (role[c] == 2 || role[f] == 1) return fail(GF_EINVAL, "link %d: a slot is both a coarse and a fine patch of the call", l);
+            role[c] = 1;
+            role[f] = 2;
+        }
+    }
+    return GF_OK;
+}
+
+static int cf_ghost_impl(gf_pool* p, int buf, int n, const int* cs, const int* fs, const int* side,
+                         const int* igrid, bool interp)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (int r = cf_ghost_check(p, buf, n, cs, fs, side, igrid, interp)) return r;
+    if (n == 0) return GF_OK;
+    // device copies of the four lists, each 256-byte aligned, in the pool's scratch
+    const size_t stride = ((sizeof(int) * (size_t)n + 255) / 256) * 256;
+    if (int r = ensure_scratch(p, 4 * stride)) return r;
+    char* d = (char*)p->tag_buf;
+    const int* lists[4] = {cs, fs, side, igrid};
+    std::vector<int> ig0;
+    for (int k = 0; k < 4; k++) {
+        const int* src = lists[k];
+        if (k == 3) {  // corners: igrid is not read, pass 0
+            ig0.assign(igrid, igrid + n);
+            for (int l = 0; l < n; l++)
+                if (side[l] >= 4) ig0[l] = 0;
+            src = ig0.data();
+        }
+        HIPCHK(hipMemcpyAsync(d + k * stride, src, sizeof(int) * n, hipMemcpyHostToDevice, g.stream));
+    }
+    gf::RegridArgs a{};
+    a.desc = p->desc;
+    a.q = p->q[buf];
+    a.aux = p->aux;
+    a.cs = (const int*)d;
+    a.fs = (const int*)(d + stride);
+    a.side = (const int*)(d + 2 * stride);
+    a.igrid = (const int*)(d + 3 * stride);
+    a.n = n;
+    a.dry_tolerance = g.P.dry_tolerance;
+    HIPCHK(interp ? gf::launch_interpolate_ghost(p->mx, p->my, a, g.stream)
+                  : gf::launch_average_ghost(p->mx, p->my, a, g.stream));
+    // the written patches' ghost cells changed: none of them may be skipped
+    // as steady-dry on the strength of its old neighbourhood (the interior
+    // depth maxima are recomputed unchanged)
+    HIPCHK(gf::launch_hmax_list(p->mx, p->my, p->q[buf], p->hmax2 + (size_t)buf * p->capacity, p->dry_age,
+                                interp ? a.fs : a.cs, n, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+}
+
+int gf_pool_average_ghost(gf_pool* p, int buf, int n, const int* coarse, const int* fine, const int* side,
+                          const int* igrid)
+{
+    return cf_ghost_impl(p, buf, n, coarse, fine, side, igrid, false);
+}
+
+int gf_pool_interpolate_ghost(gf_pool* p, int buf, int n, const int* coarse, const int* fine, const int* side,
+                              const int* igrid)
+{
+    return cf_ghost_impl(p, buf, n, coarse, fine, side, igrid, true);
 }
 
 int gf_pool_timeinterp(gf_pool* p, int n, const int* src, int buf_last, int buf_curr, double alpha,

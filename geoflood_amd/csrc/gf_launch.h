@@ -58,6 +58,8 @@ struct RegridArgs {
     const double* aux;
     const int* cs;
     const int* fs;
+    const int* side;   // coarse/fine ghost transfers: 0..3 face, 4..7 corner
+    const int* igrid;  // and the fine neighbour's half along a face
     int n;
     double dry_tolerance;
 };
@@ -178,6 +180,8 @@ hipError_t launch_all(int mx, int my, int variant, bool def, const BatchArgs* da
 hipError_t launch_batch_control(const BatchArgs& a, hipStream_t st);
 hipError_t launch_average2coarse(int mx, int my, const RegridArgs& a, hipStream_t st);
 hipError_t launch_interpolate2fine(int mx, int my, const RegridArgs& a, hipStream_t st);
+hipError_t launch_average_ghost(int mx, int my, const RegridArgs& a, hipStream_t st);
+hipError_t launch_interpolate_ghost(int mx, int my, const RegridArgs& a, hipStream_t st);
 hipError_t launch_timeinterp(int mx, int my, const TimeInterpArgs& a, hipStream_t st);
 // hmax[slot] of buffer q for the listed slots; clears their dry_age entries.
 hipError_t launch_hmax_list(int mx, int my, const double* q, double* hmax, int* dry_age, const int* slots, int n,

@@ -1417,6 +1417,53 @@ __global__ void __launch_bounds__(64) k_gauges(GaugeArgs a)
 // ghost_source, as the halo pull does.
 // ---------------------------------------------------------------------------
 
+// One coarse cell from four fine cells k[0..3] (x fastest), the per-cell rule
+// of fc2d_geoclaw_average_fort.f (average2coarse :482-603, average_face
+// :33-320, average_corner :323-480) with mcapa = 0: the wet-cell mean of eta,
+// the mean depth capped by it over the coarse bathymetry bc, momentum
+// rescaled.
+template <int MX, int MY>
+__device__ __forceinline__ void average_four(const double* __restrict__ qf, const double* __restrict__ af,
+                                             const int k[4], double bc, double drytol, double& hc, double& huc,
+                                             double& hvc)
+{
+    using G = Geo<MX, MY>;
+    double etasum = 0.0, hsum = 0.0, husum = 0.0, hvsum = 0.0;
+    int nwet = 0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const double hf = qf[k[m]] * 1.0;
+        const double bf = af[k[m]] * 1.0;
+        double huf = qf[G::NP2 + k[m]] * 1.0;
+        double hvf = qf[2 * G::NP2 + k[m]] * 1.0;
+        double etaf;
+        if (hf > drytol) {
+            etaf = hf + bf;
+            nwet++;
+        } else {
+            etaf = 0.0;
+            huf = 0.0;
+            hvf = 0.0;
+        }
+        hsum = hsum + hf;
+        husum = husum + huf;
+        hvsum = hvsum + hvf;
+        etasum = etasum + etaf;
+    }
+    hc = 0.0;
+    huc = 0.0;
+    hvc = 0.0;
+    if (nwet > 0) {
+        const double etaav = etasum / (double)nwet;
+        const double hav = hsum / (double)nwet;
+        const double e = etaav - bc * 1.0;
+        const double hmx = e > 0.0 ? e : 0.0;
+        hc = hav < hmx ? hav : hmx;
+        huc = (hc / hsum) * husum;
+        hvc = (hc / hsum) * hvsum;
+    }
+}
+
 // fc2d_geoclaw_fort_average2coarse (fc2d_geoclaw_average_fort.f:482-603),
 // mcapa = 0: quadrant g of the coarse interior from child g.
 template <int MX, int MY>
@@ -1433,40 +1480,12 @@ __global__ void __launch_bounds__(256) k_average2coarse(RegridArgs a)
     for (int c = threadIdx.x; c < (MX / 2) * (MY / 2); c += 256) {
         const int i = c % (MX / 2) + 1, j = c / (MX / 2) + 1;
         const int i1 = i + ig * MX / 2, j1 = j + jg * MY / 2;
-        double etasum = 0.0, hsum = 0.0, husum = 0.0, hvsum = 0.0;
-        int nwet = 0;
+        int k[4];
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-            const int k = G::ti((i - 1) * 2 + (m & 1) + 1, (j - 1) * 2 + (m >> 1) + 1);
-            const double hf = qf[k] * 1.0;
-            const double bf = af[k] * 1.0;
-            double huf = qf[G::NP2 + k] * 1.0;
-            double hvf = qf[2 * G::NP2 + k] * 1.0;
-            double etaf;
-            if (hf > a.dry_tolerance) {
-                etaf = hf + bf;
-                nwet++;
-            } else {
-                etaf = 0.0;
-                huf = 0.0;
-                hvf = 0.0;
-            }
-            hsum = hsum + hf;
-            husum = husum + huf;
-            hvsum = hvsum + hvf;
-            etasum = etasum + etaf;
-        }
-        double hc = 0.0, huc = 0.0, hvc = 0.0;
-        if (nwet > 0) {
-            const double etaav = etasum / (double)nwet;
-            const double hav = hsum / (double)nwet;
-            const double e = etaav - ac[G::ti(i1, j1)] * 1.0;
-            const double hmx = e > 0.0 ? e : 0.0;
-            hc = hav < hmx ? hav : hmx;
-            huc = (hc / hsum) * husum;
-            hvc = (hc / hsum) * hvsum;
-        }
+        for (int m = 0; m < 4; m++) k[m] = G::ti((i - 1) * 2 + (m & 1) + 1, (j - 1) * 2 + (m >> 1) + 1);
         const int k1 = G::ti(i1, j1);
+        double hc, huc, hvc;
+        average_four<MX, MY>(qf, af, k, ac[k1], a.dry_tolerance, hc, huc, hvc);
         qc[k1] = hc / 1.0;
         qc[G::NP2 + k1] = huc / 1.0;
         qc[2 * G::NP2 + k1] = hvc / 1.0;
@@ -1501,6 +1520,37 @@ __device__ __forceinline__ double regrid_cell(const RegridArgs& a, int patch, in
     return (m == 1 && nu) || (m == 2 && nv) ? -v : v;
 }
 
+// The limited linear reconstruction of coarse cell (ic, jc) of patch cp
+// (fc2d_geoclaw_interpolate_fort.f: interpolate2fine :448-505, interpolate_face
+// :117-171, interpolate_corner :351-396), sea_level = 0: qc, slopes gx, gy of
+// eta (dry-limited 3x3 stencil) and of hu, hv.
+template <int MX, int MY>
+__device__ __forceinline__ void reconstruct_coarse(const RegridArgs& a, int cp, const double* __restrict__ ac,
+                                                   int ic, int jc, double qc[3], double gx[3], double gy[3])
+{
+    using G = Geo<MX, MY>;
+    double eb[3][3];
+#pragma unroll
+    for (int ii = -1; ii <= 1; ii++)
+#pragma unroll
+        for (int jj = -1; jj <= 1; jj++) {
+            const double h = regrid_cell<MX, MY>(a, cp, ic + ii, jc + jj, 0);
+            const double b = ac[G::ti(ic + ii, jc + jj)];
+            eb[ii + 1][jj + 1] = h < a.dry_tolerance ? 0.0 : h + b;
+        }
+    qc[0] = eb[1][1];
+    gx[0] = amr_slope(qc[0] - eb[0][1], eb[2][1] - qc[0]);
+    gy[0] = amr_slope(qc[0] - eb[1][0], eb[1][2] - qc[0]);
+#pragma unroll
+    for (int mq = 1; mq < 3; mq++) {
+        qc[mq] = regrid_cell<MX, MY>(a, cp, ic, jc, mq);
+        gx[mq] = amr_slope(qc[mq] - regrid_cell<MX, MY>(a, cp, ic - 1, jc, mq),
+                           regrid_cell<MX, MY>(a, cp, ic + 1, jc, mq) - qc[mq]);
+        gy[mq] = amr_slope(qc[mq] - regrid_cell<MX, MY>(a, cp, ic, jc - 1, mq),
+                           regrid_cell<MX, MY>(a, cp, ic, jc + 1, mq) - qc[mq]);
+    }
+}
+
 // fc2d_geoclaw_fort_interpolate2fine (fc2d_geoclaw_interpolate_fort.f:
 // 405-579), sea_level = 0: child g's interior and the ghost layers on the
 // family's inner sides from the coarse patch.
@@ -1519,29 +1569,10 @@ __global__ void __launch_bounds__(256) k_interpolate2fine(RegridArgs a)
     for (int c = threadIdx.x; c < NI * NJ; c += 256) {
         const int i = (1 - ig) + c % NI, j = (1 - jg) + c / NI;
         const int ic = i + ig * MX / 2, jc = j + jg * MY / 2;
-        double eb[3][3];
+        double qc[3], gx[3], gy[3];
+        reconstruct_coarse<MX, MY>(a, cp, ac, ic, jc, qc, gx, gy);
 #pragma unroll
-        for (int ii = -1; ii <= 1; ii++)
-#pragma unroll
-            for (int jj = -1; jj <= 1; jj++) {
-                const double h = regrid_cell<MX, MY>(a, cp, ic + ii, jc + jj, 0);
-                const double b = ac[G::ti(ic + ii, jc + jj)];
-                eb[ii + 1][jj + 1] = h < a.dry_tolerance ? 0.0 : h + b;
-            }
-#pragma unroll
-        for (int mq = 0; mq < 3; mq++) {
-            double qc, gx, gy;
-            if (mq == 0) {
-                qc = eb[1][1];
-                gx = amr_slope(qc - eb[0][1], eb[2][1] - qc);
-                gy = amr_slope(qc - eb[1][0], eb[1][2] - qc);
-            } else {
-                qc = regrid_cell<MX, MY>(a, cp, ic, jc, mq);
-                gx = amr_slope(qc - regrid_cell<MX, MY>(a, cp, ic - 1, jc, mq),
-                               regrid_cell<MX, MY>(a, cp, ic + 1, jc, mq) - qc);
-                gy = amr_slope(qc - regrid_cell<MX, MY>(a, cp, ic, jc - 1, mq),
-                               regrid_cell<MX, MY>(a, cp, ic, jc + 1, mq) - qc);
-            }
+        for (int mq = 0; mq < 3; mq++)
 #pragma unroll
             for (int ii = 1; ii <= 2; ii++)
 #pragma unroll
@@ -1549,10 +1580,127 @@ __global__ void __launch_bounds__(256) k_interpolate2fine(RegridArgs a)
                     const double shiftx = (ii - 2 / 2.0 - 0.5) / 2;
                     const double shifty = (jj - 2 / 2.0 - 0.5) / 2;
                     const int k = G::ti((i - 1) * 2 + ii, (j - 1) * 2 + jj);
-                    const double v = qc + shiftx * gx + shifty * gy;
+                    const double v = qc[mq] + shiftx * gx[mq] + shifty * gy[mq];
                     qf[mq * G::NP2 + k] = mq == 0 ? v - af[k] : v;
                 }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Coarse/fine ghost transfers, one wave per link l between coarse slot cs[l]
+// and its half-size neighbour fs[l] of the same orientation across side[l]:
+// 0..3 = face 0..3 of the coarse patch (left, right, bottom, top), the
+// neighbour in half igrid[l] along it; 4..7 = corner 0..3 (lower-left,
+// lower-right, upper-left, upper-right).  The neighbour covers coarse cells
+// ox+1..ox+MX/2 by oy+1..oy+MY/2, so coarse cell (ic, jc) is fine cells
+// 2(ic-ox)-1 + {0,1} by 2(jc-oy)-1 + {0,1}.
+// ---------------------------------------------------------------------------
+template <int MX, int MY>
+__device__ __forceinline__ void half_origin(int side, int igrid, int& ox, int& oy)
+{
+    ox = (side == 0 || side == 4 || side == 6) ? -MX / 2 : (side == 1 || side == 5 || side == 7) ? MX : igrid * MX / 2;
+    oy = (side == 2 || side == 4 || side == 5) ? -MY / 2 : (side == 3 || side == 6 || side == 7) ? MY : igrid * MY / 2;
+}
+
+// fc2d_geoclaw_fort_average_face / _average_corner (fc2d_geoclaw_average_fort.f:
+// 33-480), mcapa = 0: the coarse ghost cells the fine neighbour covers.
+template <int MX, int MY>
+__global__ void __launch_bounds__(64) k_average_ghost(RegridArgs a)
+{
+    using G = Geo<MX, MY>;
+    const int l = blockIdx.x;
+    if (l >= a.n) return;
+    const int side = a.side[l], igrid = a.igrid[l];
+    int ox, oy;
+    half_origin<MX, MY>(side, igrid, ox, oy);
+    const double* qf = a.q + (size_t)a.fs[l] * 3 * G::NP2;
+    const double* af = a.aux + (size_t)a.fs[l] * G::NP2;
+    double* qc = a.q + (size_t)a.cs[l] * 3 * G::NP2;
+    const double* ac = a.aux + (size_t)a.cs[l] * G::NP2;
+    const int ncell = side >= 4 ? MBC * MBC : MBC * (side < 2 ? MY : MX);
+    for (int c = threadIdx.x; c < ncell; c += 64) {
+        int ic, jc;
+        if (side >= 4) {
+            const int ibc = c / MBC + 1, jbc = c % MBC + 1;
+            ic = (side - 4) & 1 ? MX + ibc : 1 - ibc;
+            jc = (side - 4) & 2 ? MY + jbc : 1 - jbc;
+        } else if (side < 2) {
+            const int ibc = c % MBC + 1;
+            jc = c / MBC + 1;
+            ic = side == 0 ? 1 - ibc : MX + ibc;
+        } else {
+            const int jbc = c / MX + 1;
+            ic = c % MX + 1;
+            jc = side == 2 ? 1 - jbc : MY + jbc;
         }
+        const int i0 = 2 * (ic - ox) - 1, j0 = 2 * (jc - oy) - 1;
+        // fclaw2d_clawpatch_is_valid_average: the four cells inside the fine interior
+        if (side < 4 && !(i0 >= 1 && i0 + 1 <= MX && j0 >= 1 && j0 + 1 <= MY)) continue;
+        int k[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) k[m] = G::ti(i0 + (m & 1), j0 + (m >> 1));
+        const int k1 = G::ti(ic, jc);
+        double hc, huc, hvc;
+        average_four<MX, MY>(qf, af, k, ac[k1], a.dry_tolerance, hc, huc, hvc);
+        qc[k1] = hc / 1.0;
+        qc[G::NP2 + k1] = huc / 1.0;
+        qc[2 * G::NP2 + k1] = hvc / 1.0;
+    }
+}
+
+// fc2d_geoclaw_fort_interpolate_face / _interpolate_corner
+// (fc2d_geoclaw_interpolate_fort.f:24-403), sea_level = 0: the fine
+// neighbour's ghost cells that the coarse boundary cells cover (a face: coarse
+// cells 1..MX along it whose four fine cells lie in the fine array, which
+// includes one hanging 2x2 corner; a corner: the corner cell), depth clipped
+// at 0.  Coarse cells outside the coarse interior are read through
+// ghost_source, as the halo pull reads them.
+template <int MX, int MY>
+__global__ void __launch_bounds__(64) k_interpolate_ghost(RegridArgs a)
+{
+    using G = Geo<MX, MY>;
+    const int l = blockIdx.x;
+    if (l >= a.n) return;
+    const int side = a.side[l], igrid = a.igrid[l];
+    int ox, oy;
+    half_origin<MX, MY>(side, igrid, ox, oy);
+    const int cp = a.cs[l];
+    const double* ac = a.aux + (size_t)cp * G::NP2;
+    double* qf = a.q + (size_t)a.fs[l] * 3 * G::NP2;
+    const double* af = a.aux + (size_t)a.fs[l] * G::NP2;
+    const int ncell = side >= 4 ? 1 : MX;
+    for (int c = threadIdx.x; c < ncell; c += 64) {
+        int ic, jc;
+        if (side >= 4) {
+            ic = (side - 4) & 1 ? MX : 1;
+            jc = (side - 4) & 2 ? MY : 1;
+        } else if (side < 2) {
+            ic = side == 0 ? 1 : MX;
+            jc = c + 1;
+        } else {
+            ic = c + 1;
+            jc = side == 2 ? 1 : MY;
+        }
+        const int i0 = 2 * (ic - ox) - 1, j0 = 2 * (jc - oy) - 1;
+        // fclaw2d_clawpatch_is_valid_interp: the four cells inside the fine array
+        if (side < 4 && !(i0 >= 1 - MBC && i0 + 1 <= MX + MBC && j0 >= 1 - MBC && j0 + 1 <= MY + MBC)) continue;
+        double qc[3], gx[3], gy[3];
+        reconstruct_coarse<MX, MY>(a, cp, ac, ic, jc, qc, gx, gy);
+#pragma unroll
+        for (int mq = 0; mq < 3; mq++)
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const int ii = m & 1, jj = m >> 1;
+                const double shiftx = (ii - 0.5) / 2.0, shifty = (jj - 0.5) / 2.0;
+                const int k = G::ti(i0 + ii, j0 + jj);
+                const double v = qc[mq] + gx[mq] * shiftx + gy[mq] * shifty;
+                if (mq == 0) {
+                    const double value = v - af[k];
+                    qf[k] = value > 0.0 ? value : 0.0;
+                } else {
+                    qf[mq * G::NP2 + k] = v;
+                }
+            }
     }
 }
 
@@ -1844,6 +1992,24 @@ hipError_t launch_interpolate2fine(int mx, int my, const RegridArgs& a, hipStrea
 #define GF_INT(M, N) k_interpolate2fine<M, N><<<4 * a.n, 256, 0, st>>>(a)
     GF_SIZES(GF_INT)
 #undef GF_INT
+    return hipGetLastError();
+}
+
+hipError_t launch_average_ghost(int mx, int my, const RegridArgs& a, hipStream_t st)
+{
+    if (a.n <= 0) return hipSuccess;
+#define GF_AG(M, N) k_average_ghost<M, N><<<a.n, 64, 0, st>>>(a)
+    GF_SIZES(GF_AG)
+#undef GF_AG
+    return hipGetLastError();
+}
+
+hipError_t launch_interpolate_ghost(int mx, int my, const RegridArgs& a, hipStream_t st)
+{
+    if (a.n <= 0) return hipSuccess;
+#define GF_IG(M, N) k_interpolate_ghost<M, N><<<a.n, 64, 0, st>>>(a)
+    GF_SIZES(GF_IG)
+#undef GF_IG
     return hipGetLastError();
 }
 

@@ -367,6 +367,25 @@ class Pool:
         assert len(f) == 4 * len(c)
         check(self.lib.gf_pool_interpolate2fine(self.h, buf, len(c), iptr(c), iptr(f)), "gf_pool_interpolate2fine")
 
+    def _cf_links(self, coarse, fine, side, igrid):
+        c, f, s = (np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(-1)) for a in (coarse, fine, side))
+        g = np.zeros_like(s) if igrid is None else np.ascontiguousarray(np.asarray(igrid, dtype=np.int32).reshape(-1))
+        assert len(c) == len(f) == len(s) == len(g)
+        return c, f, s, g
+
+    def average_ghost(self, coarse, fine, side, igrid=None, buf: int = 0):
+        """Coarse ghost cells from half-size neighbours (gf_pool_average_ghost):
+        link k = (coarse[k], fine[k], side[k] 0..3 face / 4..7 corner, igrid[k])."""
+        c, f, s, g = self._cf_links(coarse, fine, side, igrid)
+        check(self.lib.gf_pool_average_ghost(self.h, buf, len(c), iptr(c), iptr(f), iptr(s), iptr(g)),
+              "gf_pool_average_ghost")
+
+    def interpolate_ghost(self, coarse, fine, side, igrid=None, buf: int = 0):
+        """Fine ghost cells from the coarse patches (gf_pool_interpolate_ghost)."""
+        c, f, s, g = self._cf_links(coarse, fine, side, igrid)
+        check(self.lib.gf_pool_interpolate_ghost(self.h, buf, len(c), iptr(c), iptr(f), iptr(s), iptr(g)),
+              "gf_pool_interpolate_ghost")
+
     def timeinterp(self, src, buf_last, buf_curr, alpha, dst, dst_buf):
         s = np.ascontiguousarray(np.asarray(src, dtype=np.int32).reshape(-1))
         d = np.ascontiguousarray(np.asarray(dst, dtype=np.int32).reshape(-1))

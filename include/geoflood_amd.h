@@ -414,6 +414,47 @@ int gf_pool_sample_gauges(gf_pool *pool, int buf, int n, const int *slot, const 
 int gf_pool_average2coarse(gf_pool *pool, int buf, int n, const int *coarse, const int *fine);
 int gf_pool_interpolate2fine(gf_pool *pool, int buf, int n, const int *coarse, const int *fine);
 
+/* ---- Coarse/fine ghost transfers --------------------------------------------
+ * Ghost cells across a level change, between slots of one pool (buffer buf,
+ * square patches, refinement ratio 2).  Link l joins the coarse patch in slot
+ * coarse[l] and a half-size neighbour of the same orientation in slot
+ * fine[l] across side[l] of the coarse patch: 0..3 = face 0..3 (left, right,
+ * bottom, top) with the neighbour in half igrid[l] along it (0: the lower
+ * coordinate); 4..7 = corner 0..3 (lower-left, lower-right, upper-left,
+ * upper-right), igrid[l] not read.  The index map is that of neighbours
+ * within one block or across brick blocks without rotation, in place of
+ * ForestClaw's transform_face_half / transform_corner_half.  Each written
+ * region (a coarse patch's face half or corner for averaging, a fine
+ * patch's for interpolation) may appear once per call.  Both reset the
+ * written patches' steady-dry state.  A run orders them as ForestClaw's
+ * ghost update does: same-level fill (gf_pool_ghost_fill or the fused halo
+ * pull), then gf_pool_average_ghost, then gf_pool_interpolate_ghost, level by
+ * level from coarse to fine; the sides that these fill are GF_NB_STORED in
+ * the patch descriptors.
+ *
+ * gf_pool_average_ghost replaces fc2d_geoclaw_fort_average_face and
+ * fc2d_geoclaw_fort_average_corner (fc2d_geoclaw_average_fort.f:33-480,
+ * mcapa = 0), which the plugin's average_face / average_corner hooks call
+ * per (coarse, fine) pair (fc2d_cpucuda.cpp:698-745, :779-823): the coarse ghost cells
+ * that the fine neighbour covers, from its interior.
+ *
+ * gf_pool_interpolate_ghost replaces fc2d_geoclaw_fort_interpolate_face and
+ * fc2d_geoclaw_fort_interpolate_corner (fc2d_geoclaw_interpolate_fort.f:
+ * 24-403; hooks fc2d_cpucuda.cpp:746-778, :824-857): the fine neighbour's ghost cells
+ * that the coarse boundary cells cover (a face also fills the 2x2 ghost
+ * corner away from the neighbour's sibling), from limited reconstructions of
+ * eta and hu, hv as in gf_pool_interpolate2fine, the depth clipped at 0.
+ * Coarse cells beyond the coarse interior are read as the halo pull reads
+ * them (the stored ghosts on GF_NB_STORED sides); no slot may be both a
+ * coarse and a fine patch of one call.  At a fine substep between coarse
+ * steps the hooks read the coarse patch's time-interpolated copy
+ * (fclaw2d_clawpatch_timesync_data): pass the gf_pool_timeinterp
+ * destination slot, described like the coarse patch, as the coarse slot. */
+int gf_pool_average_ghost(gf_pool *pool, int buf, int n, const int *coarse, const int *fine, const int *side,
+                          const int *igrid);
+int gf_pool_interpolate_ghost(gf_pool *pool, int buf, int n, const int *coarse, const int *fine,
+                              const int *side, const int *igrid);
+
 /* Time interpolation for coarse ghost sources, replaces
  * fc2d_geoclaw_fort_timeinterp (fc2d_geoclaw_timeinterp_fort.f:1-69): for
  * k < n, the two interior layers next to each face of slot dst[k] of buffer

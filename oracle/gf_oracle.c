@@ -1576,6 +1576,57 @@ int gfo_gauge_line(char *out, int cap, int level, double t, const double qvar[3]
 #define AQ(q, m, i, j) (q)[((size_t)((j) - 1 + mbc) * nx + ((i) - 1 + mbc)) * 3 + (m)]
 #define AA(a, i, j) (a)[((size_t)((j) - 1 + mbc) * nx + ((i) - 1 + mbc)) * maux]
 
+/* One coarse cell (i1, j1) from the four fine cells (i2[m], j2[m]), the
+ * per-cell rule shared by fc2d_geoclaw_average_fort.f:482-603 (average2coarse),
+ * :33-320 (average_face) and :323-480 (average_corner), mcapa = 0 (capa =
+ * capac = 1): the wet-cell mean of eta, the mean depth capped by it over the
+ * coarse bathymetry, momentum rescaled. */
+static void average_cell(const gfo_params *p, int nx, int mbc, int maux, double *qcoarse, const double *qfine,
+                         const double *aux_coarse, const double *aux_fine, int i1, int j1, const int i2[4],
+                         const int j2[4])
+{
+    const double capac = 1.0;
+    double etasum = 0.0, hsum = 0.0, husum = 0.0, hvsum = 0.0;
+    int nwet = 0;
+    for (int m = 0; m < 4; m++) {
+        const double capa = 1.0;
+        const double hf = AQ(qfine, 0, i2[m], j2[m]) * capa;
+        const double bf = AA(aux_fine, i2[m], j2[m]) * capa;
+        double huf = AQ(qfine, 1, i2[m], j2[m]) * capa;
+        double hvf = AQ(qfine, 2, i2[m], j2[m]) * capa;
+        double etaf;
+        if (hf > p->dry_tolerance) {
+            etaf = hf + bf;
+            nwet++;
+        } else {
+            etaf = 0.0;
+            huf = 0.0;
+            hvf = 0.0;
+        }
+        hsum = hsum + hf;
+        husum = husum + huf;
+        hvsum = hvsum + hvf;
+        etasum = etasum + etaf;
+    }
+    double hc, huc, hvc;
+    if (nwet > 0) {
+        const double etaav = etasum / (double)nwet;
+        const double hav = hsum / (double)nwet;
+        const double e = etaav - AA(aux_coarse, i1, j1) * capac;
+        const double hmx = e > 0.0 ? e : 0.0;  /* Fortran MAX(a, b) = a > b ? a : b */
+        hc = hav < hmx ? hav : hmx;            /* MIN */
+        huc = (hc / hsum) * husum;
+        hvc = (hc / hsum) * hvsum;
+    } else {
+        hc = 0.0;
+        huc = 0.0;
+        hvc = 0.0;
+    }
+    AQ(qcoarse, 0, i1, j1) = hc / capac;
+    AQ(qcoarse, 1, i1, j1) = huc / capac;
+    AQ(qcoarse, 2, i1, j1) = hvc / capac;
+}
+
 /* fc2d_geoclaw_average_fort.f:482-603 with mcapa = 0 (capa = capac = 1). */
 void gfo_average2coarse(const gfo_params *p, int mx, int my, int mbc, int maux, double *qcoarse,
                         const double *qfine, const double *aux_coarse, const double *aux_fine, int igrid)
@@ -1593,47 +1644,77 @@ void gfo_average2coarse(const gfo_params *p, int mx, int my, int mbc, int maux, 
                     j2[m] = (j - 1) * 2 + jj;
                     m++;
                 }
-            const double capac = 1.0;
-            double etasum = 0.0, hsum = 0.0, husum = 0.0, hvsum = 0.0;
-            int nwet = 0;
-            for (m = 0; m < 4; m++) {
-                const double capa = 1.0;
-                const double hf = AQ(qfine, 0, i2[m], j2[m]) * capa;
-                const double bf = AA(aux_fine, i2[m], j2[m]) * capa;
-                double huf = AQ(qfine, 1, i2[m], j2[m]) * capa;
-                double hvf = AQ(qfine, 2, i2[m], j2[m]) * capa;
-                double etaf;
-                if (hf > p->dry_tolerance) {
-                    etaf = hf + bf;
-                    nwet++;
-                } else {
-                    etaf = 0.0;
-                    huf = 0.0;
-                    hvf = 0.0;
-                }
-                hsum = hsum + hf;
-                husum = husum + huf;
-                hvsum = hvsum + hvf;
-                etasum = etasum + etaf;
-            }
-            double hc, huc, hvc;
-            if (nwet > 0) {
-                const double etaav = etasum / (double)nwet;
-                const double hav = hsum / (double)nwet;
-                const double e = etaav - AA(aux_coarse, i1, j1) * capac;
-                const double hmx = e > 0.0 ? e : 0.0;  /* Fortran MAX(a, b) = a > b ? a : b */
-                hc = hav < hmx ? hav : hmx;            /* MIN */
-                huc = (hc / hsum) * husum;
-                hvc = (hc / hsum) * hvsum;
-            } else {
-                hc = 0.0;
-                huc = 0.0;
-                hvc = 0.0;
-            }
-            AQ(qcoarse, 0, i1, j1) = hc / capac;
-            AQ(qcoarse, 1, i1, j1) = huc / capac;
-            AQ(qcoarse, 2, i1, j1) = hvc / capac;
+            average_cell(p, nx, mbc, maux, qcoarse, qfine, aux_coarse, aux_fine, i1, j1, i2, j2);
         }
+}
+
+/* The index map of a half-size neighbour with the coarse patch's own
+ * orientation (same block, or a brick block without rotation), in place of
+ * ForestClaw's fclaw2d_clawpatch_transform_face_half / _corner_half, which
+ * the reference tree does not hold: the fine patch covers coarse cells
+ * ox+1 .. ox+mx/2 by oy+1 .. oy+my/2, so coarse cell (ic, jc) is fine cells
+ * (2(ic-ox)-1 + s, 2(jc-oy)-1 + t), s, t in {0, 1}, listed x fastest.
+ * side 0..3: the fine patch across face 0..3 of the coarse patch (left,
+ * right, bottom, top) in half igrid (0: lower coordinate along the face);
+ * side 4..7: the fine patch across corner 0..3 (lower-left, lower-right,
+ * upper-left, upper-right). */
+static void half_origin(int side, int igrid, int mx, int my, int *ox, int *oy)
+{
+    switch (side) {
+    case 0: *ox = -mx / 2; *oy = igrid * my / 2; break;
+    case 1: *ox = mx;      *oy = igrid * my / 2; break;
+    case 2: *ox = igrid * mx / 2; *oy = -my / 2; break;
+    case 3: *ox = igrid * mx / 2; *oy = my;      break;
+    case 4: *ox = -mx / 2; *oy = -my / 2; break;
+    case 5: *ox = mx;      *oy = -my / 2; break;
+    case 6: *ox = -mx / 2; *oy = my;      break;
+    default: *ox = mx;     *oy = my;      break;
+    }
+}
+
+static void transform_half(int ox, int oy, int ic, int jc, int i2[4], int j2[4])
+{
+    const int i0 = 2 * (ic - ox) - 1, j0 = 2 * (jc - oy) - 1;
+    i2[0] = i0;     j2[0] = j0;
+    i2[1] = i0 + 1; j2[1] = j0;
+    i2[2] = i0;     j2[2] = j0 + 1;
+    i2[3] = i0 + 1; j2[3] = j0 + 1;
+}
+
+/* fc2d_geoclaw_fort_average_face (fc2d_geoclaw_average_fort.f:33-320) and
+ * _average_corner (:323-480), mcapa = 0, for side 0..7 as in half_origin:
+ * the coarse ghost cells that the fine neighbour covers. */
+int gfo_average_ghost(const gfo_params *p, int mx, int my, int mbc, int maux, double *qcoarse,
+                      const double *qfine, const double *aux_coarse, const double *aux_fine, int side, int igrid)
+{
+    const int nx = mx + 2 * mbc;
+    int ox, oy, i2[4], j2[4];
+    if (side < 0 || side > 7 || igrid < 0 || igrid > 1) return -1;
+    half_origin(side, igrid, mx, my, &ox, &oy);
+    if (side >= 4) {
+        const int icorner = side - 4;
+        for (int ibc = 1; ibc <= mbc; ibc++)
+            for (int jbc = 1; jbc <= mbc; jbc++) {
+                const int i1 = (icorner & 1) ? mx + ibc : 1 - ibc;
+                const int j1 = (icorner & 2) ? my + jbc : 1 - jbc;
+                transform_half(ox, oy, i1, j1, i2, j2);
+                average_cell(p, nx, mbc, maux, qcoarse, qfine, aux_coarse, aux_fine, i1, j1, i2, j2);
+            }
+        return 0;
+    }
+    const int idir = side / 2;
+    for (int a = 1; a <= (idir == 0 ? my : mbc); a++)
+        for (int b = 1; b <= (idir == 0 ? mbc : mx); b++) {
+            /* idir 0: jc = a, ibc = b; idir 1: jbc = a, ic = b */
+            const int ic = idir == 0 ? (side == 0 ? 1 - b : mx + b) : b;
+            const int jc = idir == 0 ? a : (side == 2 ? 1 - a : my + a);
+            transform_half(ox, oy, ic, jc, i2, j2);
+            int skip = 0;  /* fclaw2d_clawpatch_is_valid_average: 1 <= i <= mx, 1 <= j <= my */
+            for (int m = 0; m < 4; m++)
+                if (!(i2[m] >= 1 && i2[m] <= mx && j2[m] >= 1 && j2[m] <= my)) skip = 1;
+            if (!skip) average_cell(p, nx, mbc, maux, qcoarse, qfine, aux_coarse, aux_fine, ic, jc, i2, j2);
+        }
+    return 0;
 }
 
 /* ForestClaw fclaw2d_clawpatch_compute_slopes, mth = 5: the smallest of
@@ -1695,6 +1776,82 @@ void gfo_interpolate2fine(const gfo_params *p, int mx, int my, int mbc, int maux
                             AQ(qfine, mq, iff, jf) = qc + shiftx * gradx + shifty * grady;
                     }
             }
+}
+
+/* The coarse cell (ic, jc)'s limited reconstruction written to the fine
+ * cells i2[0] + (0|1), j2[0] + (0|1) (x fastest) as the face and corner
+ * interpolations do (fc2d_geoclaw_interpolate_fort.f:117-171, :347-396):
+ * eta from the dry-limited 3x3 coarse stencil (sea_level = 0), depth =
+ * max(eta - b_fine, 0), momentum components on their own. */
+static void interpolate_cell(const gfo_params *p, int nx, int mbc, int maux, const double *qcoarse, double *qfine,
+                             const double *aux_coarse, const double *aux_fine, int ic, int jc, int i0, int j0)
+{
+    const double sea_level = 0.0;
+    const int mth = 5;
+    for (int mq = 0; mq < 3; mq++) {
+        double qc, gradx, grady;
+        if (mq == 0) {
+            double etab[3][3];
+            for (int ii = -1; ii <= 1; ii++)
+                for (int jj = -1; jj <= 1; jj++) {
+                    const double h = AQ(qcoarse, 0, ic + ii, jc + jj);
+                    const double b = AA(aux_coarse, ic + ii, jc + jj);
+                    etab[ii + 1][jj + 1] = h < p->dry_tolerance ? sea_level : h + b;
+                }
+            qc = etab[1][1];
+            gradx = gfo_compute_slopes(qc - etab[0][1], etab[2][1] - qc, mth);
+            grady = gfo_compute_slopes(qc - etab[1][0], etab[1][2] - qc, mth);
+        } else {
+            qc = AQ(qcoarse, mq, ic, jc);
+            gradx = gfo_compute_slopes(qc - AQ(qcoarse, mq, ic - 1, jc), AQ(qcoarse, mq, ic + 1, jc) - qc, mth);
+            grady = gfo_compute_slopes(qc - AQ(qcoarse, mq, ic, jc - 1), AQ(qcoarse, mq, ic, jc + 1) - qc, mth);
+        }
+        for (int m = 0; m < 4; m++) {
+            const int ii = m & 1, jj = m >> 1;
+            const double shiftx = (ii - 0.5) / 2.0, shifty = (jj - 0.5) / 2.0;
+            const int iff = i0 + ii, jff = j0 + jj;
+            if (mq == 0) {
+                const double value = qc + gradx * shiftx + grady * shifty - AA(aux_fine, iff, jff);
+                AQ(qfine, 0, iff, jff) = value > 0.0 ? value : 0.0;  /* MAX(value, 0.0) */
+            } else {
+                AQ(qfine, mq, iff, jff) = qc + gradx * shiftx + grady * shifty;
+            }
+        }
+    }
+}
+
+/* fc2d_geoclaw_fort_interpolate_face (fc2d_geoclaw_interpolate_fort.f:24-259)
+ * and _interpolate_corner (:261-403), side 0..7 as in half_origin: the fine
+ * neighbour's ghost cells that the coarse patch's boundary cells cover
+ * (faces: every coarse cell along the face whose four fine cells lie in the
+ * fine array, 1-mbc .. mx+mbc, which includes the hanging corners; the face
+ * loop runs over jc = 1..mx for idir = 0 as the reference does, so mx = my
+ * is required).  Coarse ghost cells are read as they are stored. */
+int gfo_interpolate_ghost(const gfo_params *p, int mx, int my, int mbc, int maux, const double *qcoarse,
+                          double *qfine, const double *aux_coarse, const double *aux_fine, int side, int igrid)
+{
+    const int nx = mx + 2 * mbc;
+    int ox, oy, i2[4], j2[4];
+    if (side < 0 || side > 7 || igrid < 0 || igrid > 1 || mx != my) return -1;
+    half_origin(side, igrid, mx, my, &ox, &oy);
+    if (side >= 4) {
+        const int icorner = side - 4;
+        const int ic = (icorner & 1) ? mx : 1, jc = (icorner & 2) ? my : 1;
+        transform_half(ox, oy, ic, jc, i2, j2);
+        interpolate_cell(p, nx, mbc, maux, qcoarse, qfine, aux_coarse, aux_fine, ic, jc, i2[0], j2[0]);
+        return 0;
+    }
+    const int idir = side / 2;
+    for (int t = 1; t <= mx; t++) {
+        const int ic = idir == 0 ? (side == 0 ? 1 : mx) : t;
+        const int jc = idir == 0 ? t : (side == 2 ? 1 : my);
+        transform_half(ox, oy, ic, jc, i2, j2);
+        int skip = 0;  /* fclaw2d_clawpatch_is_valid_interp: 1-mbc <= i <= mx+mbc, same in j */
+        for (int m = 0; m < 4; m++)
+            if (!(i2[m] >= 1 - mbc && i2[m] <= mx + mbc && j2[m] >= 1 - mbc && j2[m] <= my + mbc)) skip = 1;
+        if (!skip) interpolate_cell(p, nx, mbc, maux, qcoarse, qfine, aux_coarse, aux_fine, ic, jc, i2[0], j2[0]);
+    }
+    return 0;
 }
 
 #undef AQ

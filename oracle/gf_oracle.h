@@ -263,6 +263,26 @@ void gfo_average2coarse(const gfo_params *p, int mx, int my, int mbc, int maux, 
 void gfo_interpolate2fine(const gfo_params *p, int mx, int my, int mbc, int maux, const double *qcoarse,
                           double *qfine, const double *aux_coarse, const double *aux_fine, int igrid);
 
+/* Coarse/fine ghost transfers between a coarse patch and a half-size
+ * neighbour of the same orientation: side 0..3 = across face 0..3 of the
+ * coarse patch (left, right, bottom, top), igrid = the fine patch's half
+ * along that face (0: lower); side 4..7 = across corner 0..3 (lower-left,
+ * lower-right, upper-left, upper-right).  The index map is restated from the
+ * geometry (ForestClaw's transform_face_half / _corner_half are not in the
+ * reference tree).  Return -1 for a bad side / igrid (or mx != my for the
+ * interpolation).
+ *
+ * gfo_average_ghost: fc2d_geoclaw_fort_average_face / _average_corner
+ * (fc2d_geoclaw_average_fort.f:33-480), mcapa = 0: qcoarse's ghost cells
+ * from the fine interior.
+ * gfo_interpolate_ghost: fc2d_geoclaw_fort_interpolate_face / _corner
+ * (fc2d_geoclaw_interpolate_fort.f:24-403): qfine's ghost cells from the
+ * coarse patch (its interior and stored ghost cells). */
+int gfo_average_ghost(const gfo_params *p, int mx, int my, int mbc, int maux, double *qcoarse,
+                      const double *qfine, const double *aux_coarse, const double *aux_fine, int side, int igrid);
+int gfo_interpolate_ghost(const gfo_params *p, int mx, int my, int mbc, int maux, const double *qcoarse,
+                          double *qfine, const double *aux_coarse, const double *aux_fine, int side, int igrid);
+
 /* ForestClaw's slope for mth = 5 (the AMRClaw slope), as used above. */
 double gfo_compute_slopes(double sl, double sr, int mth);
 

@@ -408,6 +408,30 @@ class CLib(_Lib):
         fn(ct.byref(self._cp), nx - 4, ny - 4, 2, aux_coarse.shape[-1], ptr(qcoarse), ptr(qfine), ptr(aux_coarse),
            ptr(aux_fine), igrid)
 
+    def average_ghost(self, qcoarse, qfine, aux_coarse, aux_fine, side, igrid=0):
+        """Ghost cells of qcoarse (modified) from the half-size neighbour qfine
+        across side 0..3 (face, half igrid) or 4..7 (corner)."""
+        ny, nx, _ = qcoarse.shape
+        fn = self.lib.gfo_average_ghost
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 4 + [_dp] * 4 + [ct.c_int] * 2
+        fn.restype = ct.c_int
+        r = fn(ct.byref(self._cp), nx - 4, ny - 4, 2, aux_coarse.shape[-1], ptr(qcoarse), ptr(qfine),
+               ptr(aux_coarse), ptr(aux_fine), side, igrid)
+        if r != 0:
+            raise ValueError("bad side / igrid")
+
+    def interpolate_ghost(self, qcoarse, qfine, aux_coarse, aux_fine, side, igrid=0):
+        """Ghost cells of the half-size neighbour qfine (modified) across side
+        0..3 (face, half igrid) or 4..7 (corner) of the coarse patch."""
+        ny, nx, _ = qcoarse.shape
+        fn = self.lib.gfo_interpolate_ghost
+        fn.argtypes = [ct.c_void_p] + [ct.c_int] * 4 + [_dp] * 4 + [ct.c_int] * 2
+        fn.restype = ct.c_int
+        r = fn(ct.byref(self._cp), nx - 4, ny - 4, 2, aux_coarse.shape[-1], ptr(qcoarse), ptr(qfine),
+               ptr(aux_coarse), ptr(aux_fine), side, igrid)
+        if r != 0:
+            raise ValueError("bad side / igrid, or mx != my")
+
     def compute_slopes(self, sl, sr, mth=5):
         fn = self.lib.gfo_compute_slopes
         fn.argtypes = [ct.c_double, ct.c_double, ct.c_int]

@@ -124,3 +124,92 @@ def test_regrid_rejects_overlapping_slots(gf):
             pool.interpolate2fine([0, 1], [2, 3, 4, 5, 6, 7, 8, 2])
     finally:
         pool.close()
+
+
+CF_LINKS = [(s, g) for s in range(4) for g in range(2)] + [(s, 0) for s in range(4, 8)]
+
+
+def _cf_pool(rng, mx, n_links):
+    """Slots 0..n-1 coarse, n..2n-1 fine, random wet/dry states and bathymetry
+    (ghost cells included), every side host-supplied (GF_NB_STORED)."""
+    n = n_links
+    pool = Pool(mx, mx, 2 * n)
+    pool.set_patches(np.ones(2 * n), np.ones(2 * n), np.zeros(2 * n), np.zeros(2 * n))
+    sh = (2 * n, mx + 4, mx + 4)
+    q = rng.normal(0, 1, sh + (3,))
+    q[..., 0] = np.abs(q[..., 0]) * (rng.random(sh) < 0.7)
+    q[..., 0][rng.random(sh) < 0.05] = 1e-3  # exactly at the dry tolerance
+    aux = rng.normal(0, 0.5, sh + (1,))
+    pool.upload_aos(0, q, aux)
+    return pool, q, aux
+
+
+@pytest.mark.parametrize("mx", [8, 16, 32, 64])
+def test_average_ghost_matches_oracle(gf, oracle_c, mx):
+    """Every face half and corner in one call: the coarse patches' ghost
+    cells equal the oracle's average_face / average_corner bit for bit, and
+    nothing else of any patch changes."""
+    rng = np.random.default_rng(200 + mx)
+    n = len(CF_LINKS)
+    print(f"case average_ghost mx={mx} links={n}", flush=True)
+    pool, q, aux = _cf_pool(rng, mx, n)
+    try:
+        side = [s for s, _ in CF_LINKS]
+        igrid = [g for _, g in CF_LINKS]
+        pool.average_ghost(np.arange(n), n + np.arange(n), side, igrid)
+        got = pool.download_aos(0, 2 * n)
+        for k, (s, g) in enumerate(CF_LINKS):
+            want = q[k].copy()
+            oracle_c.average_ghost(want, q[n + k], aux[k], aux[n + k], s, g)
+            assert np.array_equal(got[k], want), (k, s, g)
+            assert np.array_equal(got[n + k], q[n + k])
+    finally:
+        pool.close()
+
+
+@pytest.mark.parametrize("mx", [8, 16, 32, 64])
+def test_interpolate_ghost_matches_oracle(gf, oracle_c, mx):
+    """Every face half and corner in one call: the fine patches' ghost cells
+    equal the oracle's interpolate_face / interpolate_corner bit for bit
+    (coarse ghost cells read as stored), and nothing else changes."""
+    rng = np.random.default_rng(300 + mx)
+    n = len(CF_LINKS)
+    print(f"case interpolate_ghost mx={mx} links={n}", flush=True)
+    pool, q, aux = _cf_pool(rng, mx, n)
+    try:
+        side = [s for s, _ in CF_LINKS]
+        igrid = [g for _, g in CF_LINKS]
+        pool.interpolate_ghost(np.arange(n), n + np.arange(n), side, igrid)
+        got = pool.download_aos(0, 2 * n)
+        for k, (s, g) in enumerate(CF_LINKS):
+            want = q[n + k].copy()
+            oracle_c.interpolate_ghost(q[k], want, aux[k], aux[n + k], s, g)
+            assert np.array_equal(got[n + k], want), (k, s, g)
+            assert np.array_equal(got[k], q[k])
+    finally:
+        pool.close()
+
+
+def test_cf_ghost_link_checks(gf):
+    """Bad links are refused before any launch."""
+    pool = Pool(8, 8, 4)
+    try:
+        pool.set_patches(np.ones(4), np.ones(4), np.zeros(4), np.zeros(4))
+        bad = [
+            ([0], [0], [1], [0]),            # coarse == fine
+            ([0], [1], [8], [0]),            # side out of range
+            ([0], [1], [1], [2]),            # igrid out of range
+            ([0], [9], [1], [0]),            # slot out of range
+            ([0, 0], [1, 2], [1, 1], [0, 0]),  # same coarse region twice
+        ]
+        for c, f, s, g in bad:
+            with pytest.raises(gf.GeoFloodError):
+                pool.average_ghost(c, f, s, g)
+        with pytest.raises(gf.GeoFloodError):   # slot 1 fine in one link, coarse in the next
+            pool.interpolate_ghost([0, 1], [1, 2], [1, 1], [0, 0])
+        with pytest.raises(gf.GeoFloodError):   # same fine region twice
+            pool.interpolate_ghost([0, 2], [1, 1], [1, 1], [0, 0])
+        pool.average_ghost([0, 0], [1, 2], [1, 1], [0, 1])  # both halves of one face: fine
+        pool.interpolate_ghost([], [], [], [])
+    finally:
+        pool.close()
