@@ -1,0 +1,103 @@
+"""The 4096^2 dam break (16,384 patches of 32x32 over a seeded fractal DEM,
+walls all round, Manning n = 0.03) at its full size: the first accepted steps
+against the oracle (dt sequence, per-cell state, total water), and over a
+longer run the properties that do not depend on size: the graph, stream and
+per-patch launch paths give one trajectory, and the water in the closed domain
+never decreases.  (It is not conserved exactly on this bed: b4step2 sets
+negative depths at the wet/dry front to 0, which adds water; the flat
+radial dam break, which has no front, conserves it to 1e-12.)"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Brick, Params
+from geoflood_amd.pool import BrickLayout, Pool
+from _parity import assert_state_close
+
+pytestmark = pytest.mark.gpu
+
+N, MX = 4096, 32
+DT0 = 0.01
+RTOL_RUN = 1e-10
+
+
+@pytest.fixture(scope="module")
+def workload(gf):
+    from geoflood_amd.scenarios import uniform_dam_break
+    print("case 4096^2 workload: building the DEM", flush=True)
+    state = uniform_dam_break(N, dx=1.0, h0=10.0, rms=3.0, regional_slope=1.0 / 500.0, seed=0)
+    gf.set_params(manning_n=0.03)
+    layout = BrickLayout(N // MX, N // MX, MX, MX, state[4], state[4], (3, 3, 3, 3))
+    pool = Pool(MX, MX, layout.npatch)
+    pool.set_layout([layout])
+    yield pool, layout, state
+    pool.close()
+    gf.set_params()
+
+
+def _run(pool, layout, state, steps, mode):
+    h, hu, hv, b, _ = state
+    pool.clear_scenarios()
+    pool.upload_global(layout, h, hu, hv, b, buf=0)
+    sid = pool.add_scenario(layout, dt0=DT0, desired_cfl=0.9, max_cfl=1.0, target_steps=steps)
+    dts, prev = [], 0
+    while True:
+        if mode == "plain":
+            active = pool.run_plain(1)
+        else:
+            active = pool.run(1, graph=(mode == "graph"))
+        info = pool.scenario(sid)
+        if info["nsteps"] > prev:
+            dts.append(info["last_dt"])
+            prev = info["nsteps"]
+        if not active:
+            break
+    return info, dts, np.stack(pool.download_global(layout, buf=info["cur"]))
+
+
+def test_fullsize_first_steps_match_oracle(workload, oracle_c):
+    """Twenty accepted steps of the whole 4096^2 brick: dt sequence to 1e-12,
+    wet/dry mask identical, every cell within the run bar, and the total
+    water equal to the oracle's to 1e-12."""
+    pool, layout, state = workload
+    steps = 20
+    h, hu, hv, b, dx = state
+    oracle_c.set_params(Params(manning_n=0.03))
+    try:
+        nthreads = min(16, len(os.sched_getaffinity(0)))
+        print(f"case 4096^2 {steps} steps: oracle on {nthreads} threads", flush=True)
+        br = Brick.from_global(h, hu, hv, b, N // MX, N // MX, 2, dx, dx)
+        rec = oracle_c.brick_run(br, steps, DT0, 0.9, 1.0, nthreads=nthreads)
+        ref = br.to_global()
+        del br
+    finally:
+        oracle_c.set_params(Params())
+    print("case 4096^2: device graph run", flush=True)
+    info, dts, got = _run(pool, layout, state, steps, "graph")
+    assert info["nsteps"] == steps and info["retakes"] == rec["retakes"]
+    np.testing.assert_allclose(dts, rec["dt"], rtol=1e-12, atol=0)
+    assert_state_close(got, ref, RTOL_RUN)
+    m_got, m_ref = float(np.sum(got[0])), float(np.sum(ref[0]))
+    assert abs(m_got - m_ref) / m_ref <= 1e-12
+
+
+def test_fullsize_mass_and_paths(workload):
+    """60 accepted steps: the graph and stream runs end in the same state and
+    the closed domain's water has not decreased; three steps of the per-patch
+    launch loop equal three batched steps."""
+    pool, layout, state = workload
+    mass0 = float(np.sum(state[0]))
+    print("case 4096^2 60 steps graph", flush=True)
+    ig, dg, sg = _run(pool, layout, state, 60, "graph")
+    print("case 4096^2 60 steps stream", flush=True)
+    is_, ds, ss = _run(pool, layout, state, 60, "stream")
+    assert ig["nsteps"] == is_["nsteps"] == 60 and ig["t"] == is_["t"]
+    assert dg == ds
+    assert np.array_equal(sg, ss)
+    assert float(np.sum(sg[0])) >= mass0
+    print("case 4096^2 3 steps plain vs graph", flush=True)
+    ip, dp, sp = _run(pool, layout, state, 3, "plain")
+    ib, db, sb = _run(pool, layout, state, 3, "graph")
+    assert ip["t"] == ib["t"] and dp == db
+    assert np.array_equal(sp, sb)
