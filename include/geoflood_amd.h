@@ -17,7 +17,12 @@
  *     1-mbc:my+mbc) and aux(maux, ...) in Fortran order, ghost cells included.
  *   - Only mbc = 2, meqn = 3, mwaves = 3, mcapa = 0 and Cartesian coordinates
  *     are supported (the configurations of the GeoFlood GPU route), with
- *     (mx, my) in {(8,8), (16,16), (32,32)}.
+ *     (mx, my) in {(8,8), (16,16), (32,32), (64,64)}.
+ *   - One device, one stream and one set of library state per process: call
+ *     the library from one host thread at a time, as the reference plugin
+ *     does (one host thread per MPI rank, fc2d_cpucuda.cpp:971-976).  The
+ *     library runs host threads of its own for staging copies and text
+ *     formatting; they are joined by gf_finalize.
  */
 #ifndef GEOFLOOD_AMD_H
 #define GEOFLOOD_AMD_H

@@ -4,8 +4,8 @@
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
  * GeoClaw Fortran step (the CPU route of the fc2d_cpu_cuda plugin).  It is
  * used by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline
- * leg, and only as the checker: the product library (geoflood_amd) never
- * links, loads or calls it.
+ * leg, and only to check or time the reference algorithm: the product
+ * library (geoflood_amd) never links, loads or calls it.
  *
  * Array layouts are the reference's host layouts (Fortran column-major):
  *   q(meqn, 1-mbc:mx+mbc, 1-mbc:my+mbc)   -> q[((j+mbc-1)*(mx+2mbc) + (i+mbc-1))*meqn + (m-1)]

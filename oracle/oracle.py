@@ -7,7 +7,7 @@ Two implementations with the same entry points:
     ``ref`` target was built).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
-this module, and only as the checker.  Arrays follow the reference's host
+this module, and only to check or time the reference algorithm.  Arrays follow the reference's host
 layout (Fortran order): q has shape (my+2mbc, mx+2mbc, 3) in C order, i.e.
 q[j, i, m], with Fortran index (m+1, i-mbc+1, j-mbc+1).
 """
