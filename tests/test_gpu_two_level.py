@@ -1,0 +1,166 @@
+"""A two-level run on the device against the same pipeline on the oracle.
+
+Layout: [0,1]^2, a 4x4 brick of 16x16 coarse patches (walls outside) whose
+centre 2x2 patches are refined into 16 fine patches (ratio 2), 28 leaves in
+all.  Every step, both sides fill the ghost cells the way ForestClaw's ghost
+update orders them -- same-level face and corner copies and the walls, then
+fine -> coarse averaging, then coarse -> fine interpolation -- and advance
+every leaf with one dt (no subcycling).  The oracle fills freshly NaN ghost
+cells, so a ghost region that no transfer covers would show.  The device
+uses gf_pool_ghost_fill, gf_pool_average_ghost, gf_pool_interpolate_ghost
+and gf_pool_step; the oracle its bc2, average_ghost, interpolate_ghost and
+patch_update.  The order of a whole ghost update is ForestClaw's
+(fclaw2d_ghost_update, not in the reference tree), restated here."""
+import numpy as np
+import pytest
+
+from geoflood_amd import GF_NB_STORED, GF_NB_WALL
+from geoflood_amd.pool import Pool
+from _parity import assert_state_close
+
+pytestmark = pytest.mark.gpu
+
+MX, MBC = 16, 2
+NC = 4                                   # coarse patches per side
+REFINED = {(1, 1), (2, 1), (1, 2), (2, 2)}
+DXC, DXF = 1.0 / (NC * MX), 0.5 / (NC * MX)
+DIRS = [(-1, 0), (1, 0), (0, -1), (0, 1), (-1, -1), (1, -1), (-1, 1), (1, 1)]
+
+
+def leaves():
+    """(level, i, j) of every leaf: coarse (i, j) on the 4x4 lattice, fine
+    (i, j) on the 8x8 lattice."""
+    out = [(0, i, j) for j in range(NC) for i in range(NC) if (i, j) not in REFINED]
+    out += [(1, 2 * pi + ci, 2 * pj + cj) for (pi, pj) in sorted(REFINED, key=lambda t: (t[1], t[0]))
+            for cj in range(2) for ci in range(2)]
+    return out
+
+
+def build_links(lv):
+    """Neighbour table (GF_NB_* or slot) and the coarse/fine links
+    (coarse slot, fine slot, side, igrid)."""
+    slot = {k: s for s, k in enumerate(lv)}
+    nbr = np.full((len(lv), 8), GF_NB_WALL, dtype=np.int32)
+    links = []
+    for s, (lev, i, j) in enumerate(lv):
+        for d, (dx, dy) in enumerate(DIRS):
+            a, b = i + dx, j + dy
+            n = NC * (1 + lev)
+            if not (0 <= a < n and 0 <= b < n):
+                continue                                  # wall
+            if (lev, a, b) in slot:
+                nbr[s, d] = slot[(lev, a, b)]
+                continue
+            nbr[s, d] = GF_NB_STORED
+            if lev == 0:                                  # refined region: fine children across d
+                kids = [(ci, cj) for cj in range(2) for ci in range(2)
+                        if (dx == 0 or ci == (0 if dx > 0 else 1)) and (dy == 0 or cj == (0 if dy > 0 else 1))]
+                for ci, cj in kids:
+                    f = slot[(1, 2 * a + ci, 2 * b + cj)]
+                    if d < 4:
+                        side, ig = d, (cj if d < 2 else ci)
+                    else:
+                        side, ig = d, 0                   # corners 4..7 in the same order
+                    links.append((s, f, side, ig))
+    return nbr, links
+
+
+def field(x, y):
+    """A hump of water near the centre and a dry island over a bump whose
+    edge crosses the refined block."""
+    b = 0.5 * np.exp(-((x - 0.7) ** 2 + (y - 0.3) ** 2) / 0.02) - 0.1 * x
+    h = np.maximum(0.3 + 0.4 * np.exp(-((x - 0.45) ** 2 + (y - 0.55) ** 2) / 0.005) - b, 0.0)
+    h[h < 0.05] = 0.0
+    return h, b
+
+
+def initial(lv):
+    """Patches of the analytic state with ghost cells (aux ghosts from a
+    level-wide padded bathymetry, mirrored at the walls)."""
+    qs, auxs = [], []
+    for lev, i, j in lv:
+        dx = DXC if lev == 0 else DXF
+        n = NC * MX * (1 + lev)
+        idx = np.arange(-MBC, n + MBC)
+        # walls: ghost k outside mirrors interior cell k-1 inside (bc2's rule)
+        mir = np.where(idx < 0, -idx - 1, np.where(idx >= n, 2 * n - idx - 1, idx))
+        xc = (mir + 0.5) * dx
+        X, Y = np.meshgrid(xc, xc)
+        h, b = field(X, Y)
+        sl = slice(i * MX, i * MX + MX + 2 * MBC), slice(j * MX, j * MX + MX + 2 * MBC)
+        q = np.zeros((MX + 2 * MBC, MX + 2 * MBC, 3))
+        q[..., 0] = h[sl[1], sl[0]]
+        qs.append(q)
+        auxs.append(np.ascontiguousarray(b[sl[1], sl[0]][..., None]))
+    return np.array(qs), np.array(auxs)
+
+
+def oracle_ghosts(oc, lv, nbr, links, qs, auxs):
+    interior = (slice(MBC, MBC + MX), slice(MBC, MBC + MX))
+    for q in qs:
+        g = np.ones(q.shape[:2], bool)
+        g[interior] = False
+        q[g] = np.nan
+    r = range(1, MBC + 1)
+    for s in range(len(lv)):                              # same-level faces, then corners
+        for d, (dx, dy) in enumerate(DIRS):
+            t = nbr[s, d]
+            if t < 0:
+                continue
+            # destination ghost rows/cols and source interior rows/cols (0-based array indices)
+            di = list(range(MBC, MBC + MX)) if dx == 0 else ([MBC - k for k in r] if dx < 0 else [MBC + MX - 1 + k for k in r])
+            dj = list(range(MBC, MBC + MX)) if dy == 0 else ([MBC - k for k in r] if dy < 0 else [MBC + MX - 1 + k for k in r])
+            si = [x - dx * MX for x in di]
+            sj = [y - dy * MX for y in dj]
+            qs[s][np.ix_(dj, di)] = qs[t][np.ix_(sj, si)]
+    for c, f, side, ig in links:
+        oc.average_ghost(qs[c], qs[f], auxs[c], auxs[f], side, ig)
+    for s, (lev, i, j) in enumerate(lv):                  # walls (coarse leaves only here)
+        if lev == 0:
+            mth = [3 if nbr[s, k] == GF_NB_WALL else -1 for k in range(4)]
+            if max(mth) > 0:
+                oc.bc2(MX, MX, MBC, qs[s], auxs[s], mth)
+    for c, f, side, ig in links:
+        oc.interpolate_ghost(qs[c], qs[f], auxs[c], auxs[f], side, ig)
+    assert all(np.isfinite(q).all() for q in qs), "a ghost region is not filled"
+
+
+def test_two_level_run_matches_oracle(gf, oracle_c):
+    lv = leaves()
+    nbr, links = build_links(lv)
+    # 8 coarse faces x 2 halves; corners: 4 diagonal coarse patches + 8 face neighbours' inner corners
+    assert len(lv) == 28 and len(links) == 8 * 2 + 12
+    n = len(lv)
+    qs, auxs = initial(lv)
+    dxs = np.array([DXC if l[0] == 0 else DXF for l in lv])
+    xlo = np.array([l[1] * MX * d for l, d in zip(lv, dxs)])
+    ylo = np.array([l[2] * MX * d for l, d in zip(lv, dxs)])
+    pool = Pool(MX, MX, n)
+    try:
+        pool.set_patches(dxs, dxs, xlo, ylo, nbr)
+        pool.upload_aos(0, qs, auxs, buf=0)
+        lc, lf, ls, lg = (np.array(x, dtype=np.int32) for x in zip(*links))
+        oq = [q.copy() for q in qs]
+        oa = [a.copy() for a in auxs]
+        dt, t, cur = 4e-4, 0.0, 0
+        for step in range(25):
+            print(f"case two-level step {step}", flush=True)
+            pool.ghost_fill(buf=cur)
+            pool.average_ghost(lc, lf, ls, lg, buf=cur)
+            pool.interpolate_ghost(lc, lf, ls, lg, buf=cur)
+            c_dev, _ = pool.step(dt, src=cur, dst=1 - cur)
+            oracle_ghosts(oracle_c, lv, nbr, links, oq, oa)
+            c_ref = 0.0
+            for s in range(n):
+                c_ref = max(c_ref, oracle_c.patch_update(MX, MX, MBC, oq[s], oa[s], dxs[s], dxs[s], t, dt))
+            assert c_dev == pytest.approx(c_ref, rel=1e-13, abs=0), step
+            assert 0.05 < c_ref < 1.0
+            cur, t = 1 - cur, t + dt
+        got = pool.download_aos(0, n, buf=cur)
+        inner = (slice(MBC, MBC + MX), slice(MBC, MBC + MX))
+        g = np.stack([np.moveaxis(got[s][inner], -1, 0) for s in range(n)], axis=1)
+        r = np.stack([np.moveaxis(oq[s][inner], -1, 0) for s in range(n)], axis=1)
+        assert_state_close(g, r, 1e-10)
+        assert (r[0] > 1e-3).any() and (r[0] <= 1e-3).any()   # wet and dry cells both present
+    finally:
+        pool.close()
