@@ -173,13 +173,17 @@ __device__ __forceinline__ void cell_transverse(const Rpt2Cell& C, const double 
 }
 
 // Source of ghost cell (fi, fj) of a patch (Fortran indices, outside the
-// interior): the pool index and interior cell whose value ForestClaw's
-// same-level face copy, corner copy and bc2 passes (left, right, bottom, top
-// over full extents; geoflood_bc2_fort.f90:118-131 and twins) leave there,
-// and whether hu / hv change sign.  Returns false when the ghost is read from
-// the patch's own ghost cells: supplied by the host (NB_STORED), written by
-// k_inflow (an NB_INFLOW side or a corner next to one), or an unresolvable
-// corner.
+// interior): the pool index and cell whose value ForestClaw's same-level face
+// copy, corner copy and bc2 passes (left, right, bottom, top over full
+// extents; geoflood_bc2_fort.f90:118-131 and twins) leave there, and whether
+// hu / hv change sign.  The cell is another patch's interior cell, or, for a
+// corner between a physical side and a stored side, the patch's own stored
+// ghost cell that bc2 mirrors (the stored strips are filled first: the
+// coarse/fine transfers run before the physical boundaries, as in
+// ForestClaw's ghost update).  Returns false when the ghost is read from the
+// patch's own ghost cells: supplied by the host or a coarse/fine transfer
+// (NB_STORED), written by k_inflow (an NB_INFLOW side or a corner next to
+// one), or a corner with no rule (two stored sides).
 template <int MX, int MY>
 __device__ __forceinline__ bool ghost_source(const int* __restrict__ nbr, int patch, int fi, int fj,
                                              int& sp, int& si, int& sj, bool& neg_u, bool& neg_v)
@@ -208,6 +212,14 @@ __device__ __forceinline__ bool ghost_source(const int* __restrict__ nbr, int pa
             neg_u = (nx == NB_WALL);
         } else if (py && nx >= 0) {
             sp = nx; si = fi - ox * MX; sj = tyv;
+            neg_v = (ny == NB_WALL);
+        } else if (px && ny == NB_STORED) {
+            // bc2's x pass runs over the full extent after the stored strip
+            // was filled (a coarse/fine transfer or the host): its mirror
+            sp = patch; si = txv; sj = fj;
+            neg_u = (nx == NB_WALL);
+        } else if (py && nx == NB_STORED) {
+            sp = patch; si = fi; sj = tyv;
             neg_v = (ny == NB_WALL);
         } else {
             return false;

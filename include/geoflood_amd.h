@@ -431,11 +431,14 @@ int gf_pool_interpolate2fine(gf_pool *pool, int buf, int n, const int *coarse, c
  * ForestClaw's transform_face_half / transform_corner_half.  Each written
  * region (a coarse patch's face half or corner for averaging, a fine
  * patch's for interpolation) may appear once per call.  Both reset the
- * written patches' steady-dry state.  A run orders them as ForestClaw's
- * ghost update does: same-level fill (gf_pool_ghost_fill or the fused halo
- * pull), then gf_pool_average_ghost, then gf_pool_interpolate_ghost, level by
- * level from coarse to fine; the sides that these fill are GF_NB_STORED in
- * the patch descriptors.
+ * written patches' steady-dry state.  The sides that these fill are
+ * GF_NB_STORED in the patch descriptors.  A run calls gf_pool_average_ghost,
+ * then gf_pool_interpolate_ghost (level by level from coarse to fine), then
+ * steps: same-level ghosts and physical sides are read from the neighbours
+ * when a step loads its tiles (or written by gf_pool_ghost_fill, called after
+ * the transfers), and a ghost corner between a physical side and a stored
+ * side takes bc2's mirror of the stored strip, as in ForestClaw's ghost
+ * update, which applies the physical boundaries after the exchanges.
  *
  * gf_pool_average_ghost replaces fc2d_geoclaw_fort_average_face and
  * fc2d_geoclaw_fort_average_corner (fc2d_geoclaw_average_fort.f:33-480,

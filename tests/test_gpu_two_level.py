@@ -1,11 +1,12 @@
 """A two-level run on the device against the same pipeline on the oracle.
 
 Layout: [0,1]^2, a 4x4 brick of 16x16 coarse patches (walls outside) whose
-centre 2x2 patches are refined into 16 fine patches (ratio 2), 28 leaves in
-all.  Every step, both sides fill the ghost cells the way ForestClaw's ghost
-update orders them -- same-level face and corner copies and the walls, then
-fine -> coarse averaging, then coarse -> fine interpolation -- and advance
-every leaf with one dt (no subcycling).  The oracle fills freshly NaN ghost
+centre (or lower-left) 2x2 patches are refined into 16 fine patches (ratio
+2), 28 leaves in all.  Every step, both sides fill the ghost cells the way
+ForestClaw's ghost update orders them -- same-level face and corner copies,
+fine -> coarse averaging, the coarse level's walls, coarse -> fine
+interpolation, the fine level's walls -- and advance every leaf with one dt
+(no subcycling).  The oracle fills freshly NaN ghost
 cells, so a ghost region that no transfer covers would show.  The device
 uses gf_pool_ghost_fill, gf_pool_average_ghost, gf_pool_interpolate_ghost
 and gf_pool_step; the oracle its bc2, average_ghost, interpolate_ghost and
@@ -22,21 +23,22 @@ pytestmark = pytest.mark.gpu
 
 MX, MBC = 16, 2
 NC = 4                                   # coarse patches per side
-REFINED = {(1, 1), (2, 1), (1, 2), (2, 2)}
+CENTRE = {(1, 1), (2, 1), (1, 2), (2, 2)}
+CORNER = {(0, 0), (1, 0), (0, 1), (1, 1)}  # fine patches on two walls
 DXC, DXF = 1.0 / (NC * MX), 0.5 / (NC * MX)
 DIRS = [(-1, 0), (1, 0), (0, -1), (0, 1), (-1, -1), (1, -1), (-1, 1), (1, 1)]
 
 
-def leaves():
+def leaves(refined):
     """(level, i, j) of every leaf: coarse (i, j) on the 4x4 lattice, fine
     (i, j) on the 8x8 lattice."""
-    out = [(0, i, j) for j in range(NC) for i in range(NC) if (i, j) not in REFINED]
-    out += [(1, 2 * pi + ci, 2 * pj + cj) for (pi, pj) in sorted(REFINED, key=lambda t: (t[1], t[0]))
+    out = [(0, i, j) for j in range(NC) for i in range(NC) if (i, j) not in refined]
+    out += [(1, 2 * pi + ci, 2 * pj + cj) for (pi, pj) in sorted(refined, key=lambda t: (t[1], t[0]))
             for cj in range(2) for ci in range(2)]
     return out
 
 
-def build_links(lv):
+def build_links(lv, refined):
     """Neighbour table (GF_NB_* or slot) and the coarse/fine links
     (coarse slot, fine slot, side, igrid)."""
     slot = {k: s for s, k in enumerate(lv)}
@@ -113,23 +115,33 @@ def oracle_ghosts(oc, lv, nbr, links, qs, auxs):
             si = [x - dx * MX for x in di]
             sj = [y - dy * MX for y in dj]
             qs[s][np.ix_(dj, di)] = qs[t][np.ix_(sj, si)]
+    def walls(level):
+        for s, (lev, i, j) in enumerate(lv):
+            if lev == level:
+                mth = [3 if nbr[s, k] == GF_NB_WALL else -1 for k in range(4)]
+                if max(mth) > 0:
+                    oc.bc2(MX, MX, MBC, qs[s], auxs[s], mth)
     for c, f, side, ig in links:
         oc.average_ghost(qs[c], qs[f], auxs[c], auxs[f], side, ig)
-    for s, (lev, i, j) in enumerate(lv):                  # walls (coarse leaves only here)
-        if lev == 0:
-            mth = [3 if nbr[s, k] == GF_NB_WALL else -1 for k in range(4)]
-            if max(mth) > 0:
-                oc.bc2(MX, MX, MBC, qs[s], auxs[s], mth)
+    walls(0)                                              # the coarse level's physical sides
     for c, f, side, ig in links:
         oc.interpolate_ghost(qs[c], qs[f], auxs[c], auxs[f], side, ig)
+    walls(1)                                              # then the fine level's
     assert all(np.isfinite(q).all() for q in qs), "a ghost region is not filled"
 
 
-def test_two_level_run_matches_oracle(gf, oracle_c):
-    lv = leaves()
-    nbr, links = build_links(lv)
-    # 8 coarse faces x 2 halves; corners: 4 diagonal coarse patches + 8 face neighbours' inner corners
-    assert len(lv) == 28 and len(links) == 8 * 2 + 12
+@pytest.mark.parametrize("where", ["centre", "corner"])
+def test_two_level_run_matches_oracle(gf, oracle_c, where):
+    """The refined block in the middle of the domain, or in its lower-left
+    corner, where fine patches meet the walls next to coarse neighbours (their
+    ghost corners between a wall and a coarse/fine side are bc2's mirror of
+    the interpolated strip)."""
+    refined = CENTRE if where == "centre" else CORNER
+    lv = leaves(refined)
+    nbr, links = build_links(lv, refined)
+    # centre: 8 coarse faces x 2 halves; corners: 4 diagonal coarse patches + 8 face
+    # neighbours' inner corners.  corner: 4 coarse faces x 2 halves, 1 + 4 corners
+    assert len(lv) == 28 and len(links) == (8 * 2 + 12 if where == "centre" else 4 * 2 + 5)
     n = len(lv)
     qs, auxs = initial(lv)
     dxs = np.array([DXC if l[0] == 0 else DXF for l in lv])
@@ -144,10 +156,10 @@ def test_two_level_run_matches_oracle(gf, oracle_c):
         oa = [a.copy() for a in auxs]
         dt, t, cur = 4e-4, 0.0, 0
         for step in range(25):
-            print(f"case two-level step {step}", flush=True)
-            pool.ghost_fill(buf=cur)
+            print(f"case two-level {where} step {step}", flush=True)
             pool.average_ghost(lc, lf, ls, lg, buf=cur)
             pool.interpolate_ghost(lc, lf, ls, lg, buf=cur)
+            pool.ghost_fill(buf=cur)
             c_dev, _ = pool.step(dt, src=cur, dst=1 - cur)
             oracle_ghosts(oracle_c, lv, nbr, links, oq, oa)
             c_ref = 0.0
