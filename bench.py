@@ -60,6 +60,17 @@ def profile_values(name):
                 if "=" in line and " " not in line.split("=", 1)[0])
 
 
+def cpu_model():
+    """The host CPU's model name (Linux /proc/cpuinfo), or None."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -78,15 +89,16 @@ def reset(pool, layout, state, target_steps):
     return sid
 
 
-def cpu_baseline(state, nsteps, nthreads):
-    """The reference Fortran step (or, where it was not built, the C
-    restatement) over the first nsteps accepted steps of the same workload."""
+def cpu_baseline(state, nsteps, nthreads, port_only=False):
+    """The reference Fortran step (or, where it was not built or port_only,
+    the C restatement) over the first nsteps accepted steps of the same
+    workload."""
     from oracle.oracle import Brick, CLib, Params, ref_lib
     h, hu, hv, b, dx = state
     p = Params(manning_n=PARAMS["manning_n"])
     c = CLib()
     c.set_params(p)
-    ref = ref_lib()
+    ref = None if port_only else ref_lib()
     kind = "port"
     if ref is not None:
         ref.set_params(p)
@@ -423,6 +435,8 @@ def main():
 
     # ---- CPU baseline (reference Fortran step on the host cores) ----
     cpu = None
+    cpu_port = None
+    cpu_host = None
     parity = None
     if not args.no_cpu:
         nthreads = min(16, len(os.sched_getaffinity(0)))
@@ -437,6 +451,13 @@ def main():
             d = np.abs(ref - gpu_first)
             parity = float(np.max(np.where(scale > 0, d / np.where(scale > 0, scale, 1), 0)))
         log(f"cpu: {cb['value'] / 1e6:.2f} M cell-updates/s ({cb['kind']}); parity max rel {parity}")
+        if cb["kind"] == "reference":
+            # the C restatement on the same sample and threads, for comparison
+            cp = cpu_baseline(state, args.cpu_steps, nthreads, port_only=True)
+            cpu_port = dict(value=cp["value"], unit="cell-updates/s", cores=nthreads, kind="port",
+                            sample=cpu["sample"].split(" (")[0] + f" ({cp['seconds']:.1f} s)")
+            log(f"cpu: {cp['value'] / 1e6:.2f} M cell-updates/s (port)")
+        cpu_host = dict(model=cpu_model(), nproc=os.cpu_count(), threads_used=nthreads)
 
     out = {
         "metric": METRIC,
@@ -475,6 +496,8 @@ def main():
         "roofline": roofline,
         "step_composite": composite,
         "cpu_baseline": cpu,
+        "cpu_port": cpu_port,
+        "cpu_host": cpu_host,
         "parity_vs_cpu_max_rel": parity,
         "ensemble": ensemble,
     }
