@@ -18,10 +18,16 @@ Runs, all from the same initial state, inputs resident in HBM:
   batched  -- K attempts, each one hipGraph launch (the reported value);
   events   -- the same K attempts launched without a graph, with HIP events
               around every fused-step launch (per-kernel time -> roofline);
-  plain    -- a few attempts with one kernel launch per patch from a host
-              loop and the dt rule on the host;
+  plain    -- the first accepted steps with one kernel launch per patch from
+              a host loop and the dt rule on the host;
   cpu      -- the reference GeoClaw Fortran step (oracle/_ref) on the host
-              cores over the same first steps, when it is present.
+              cores over the same first steps, when it is present (the
+              reported cpu_baseline), then the C restatement on the same
+              sample (cpu_port); the plain run's state is compared with the
+              CPU state cell by cell (parity_vs_cpu);
+  and, for context: an ensemble-serving sample, the host-resident drop-in
+  batch (PCIe included), ASCII frame and gauge output, and the regrid and
+  coarse/fine ghost transfers.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
