@@ -81,3 +81,15 @@ def test_host_batch_binding_checks():
         gf.HostBatch([x[:, ::-1] for x in q], a, 0.1, 0.1)
     with pytest.raises(ValueError):
         gf.HostBatch([], [], 0.1, 0.1)
+
+
+def test_cf_ghost_entry_points_reject_without_a_pool():
+    """The coarse/fine ghost transfers and the regrid calls return status
+    codes for a NULL pool instead of touching the device."""
+    lib = gf.load()
+    z = (ct.c_int * 1)(0)
+    assert lib.gf_pool_average_ghost(None, 0, 1, z, z, z, z) == -1  # GF_EINVAL
+    assert lib.gf_pool_interpolate_ghost(None, 0, 1, z, z, z, z) == -1  # GF_EINVAL
+    assert lib.gf_pool_average2coarse(None, 0, 1, z, z) == -1  # GF_EINVAL
+    assert lib.gf_pool_interpolate2fine(None, 0, 1, z, z) == -1  # GF_EINVAL
+    assert b"NULL" in lib.gf_last_error()
