@@ -101,6 +101,7 @@ PROTOTYPES = {
     "gf_pool_sample_gauges": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip, _dp, _dp, _dp]),
     "gf_pool_average2coarse": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip, _ip]),
     "gf_pool_interpolate2fine": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip, _ip]),
+    "gf_pool_copy": (ct.c_int, [ct.c_void_p, ct.c_int, _ip, ct.c_int, ct.c_int]),
     "gf_pool_average_ghost": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip, _ip, _ip, _ip]),
     "gf_pool_interpolate_ghost": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip, _ip, _ip, _ip]),
     "gf_pool_timeinterp": (ct.c_int, [ct.c_void_p, ct.c_int, _ip, ct.c_int, ct.c_int, ct.c_double, _ip,

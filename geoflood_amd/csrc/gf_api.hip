@@ -1777,6 +1777,27 @@ int gf_pool_interpolate_ghost(gf_pool* p, int buf, int n, const int* coarse, con
     return cf_ghost_impl(p, buf, n, coarse, fine, side, igrid, true);
 }
 
+int gf_pool_copy(gf_pool* p, int n, const int* slots, int src_buf, int dst_buf)
+{
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if ((src_buf != 0 && src_buf != 1) || (dst_buf != 0 && dst_buf != 1) || src_buf == dst_buf)
+        return fail(GF_EINVAL, "buffers must be 0 and 1, in either order");
+    if (n < 0 || (n > 0 && !slots)) return fail(GF_EINVAL, "invalid slot list");
+    for (int k = 0; k < n; k++)
+        if (slots[k] < 0 || slots[k] >= p->capacity) return fail(GF_EINVAL, "slot %d outside the pool", slots[k]);
+    if (n == 0) return GF_OK;
+    if (int r = ensure_scratch(p, sizeof(int) * (size_t)n)) return r;
+    int* d = (int*)p->tag_buf;
+    HIPCHK(hipMemcpyAsync(d, slots, sizeof(int) * n, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(gf::launch_copy_slots(p->q[src_buf], p->q[dst_buf], d, n, (int)p->np2, g.stream));
+    // the destination buffer's depth maxima, and no steady-dry skip on the
+    // strength of what the buffer held before
+    HIPCHK(gf::launch_hmax_list(p->mx, p->my, p->q[dst_buf], p->hmax2 + (size_t)dst_buf * p->capacity, p->dry_age,
+                                d, n, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+}
+
 int gf_pool_timeinterp(gf_pool* p, int n, const int* src, int buf_last, int buf_curr, double alpha,
                        const int* dst, int dst_buf)
 {

@@ -180,6 +180,7 @@ hipError_t launch_all(int mx, int my, int variant, bool def, const BatchArgs* da
 hipError_t launch_batch_control(const BatchArgs& a, hipStream_t st);
 hipError_t launch_average2coarse(int mx, int my, const RegridArgs& a, hipStream_t st);
 hipError_t launch_interpolate2fine(int mx, int my, const RegridArgs& a, hipStream_t st);
+hipError_t launch_copy_slots(const double* src, double* dst, const int* slots, int n, int np2, hipStream_t st);
 hipError_t launch_average_ghost(int mx, int my, const RegridArgs& a, hipStream_t st);
 hipError_t launch_interpolate_ghost(int mx, int my, const RegridArgs& a, hipStream_t st);
 hipError_t launch_timeinterp(int mx, int my, const TimeInterpArgs& a, hipStream_t st);

@@ -1780,6 +1780,17 @@ __global__ void __launch_bounds__(256) k_pack(const double* stage_q, const doubl
     }
 }
 
+// State of listed slots (q, ghost cells included) from one pool buffer to the
+// other, one workgroup per slot.
+__global__ void __launch_bounds__(256) k_copy_slots(const double* __restrict__ src, double* __restrict__ dst,
+                                                    const int* __restrict__ slots, int n, int np2)
+{
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    const size_t base = (size_t)slots[b] * 3 * np2;
+    for (int e = threadIdx.x; e < 3 * np2; e += blockDim.x) dst[base + e] = src[base + e];
+}
+
 __global__ void __launch_bounds__(256) k_unpack(double* stage_q, const double* q, int first,
                                                 int n, int np2)
 {
@@ -2004,6 +2015,13 @@ hipError_t launch_interpolate2fine(int mx, int my, const RegridArgs& a, hipStrea
 #define GF_INT(M, N) k_interpolate2fine<M, N><<<4 * a.n, 256, 0, st>>>(a)
     GF_SIZES(GF_INT)
 #undef GF_INT
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_slots(const double* src, double* dst, const int* slots, int n, int np2, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_copy_slots<<<n, 256, 0, st>>>(src, dst, slots, n, np2);
     return hipGetLastError();
 }
 

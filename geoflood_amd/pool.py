@@ -367,6 +367,11 @@ class Pool:
         assert len(f) == 4 * len(c)
         check(self.lib.gf_pool_interpolate2fine(self.h, buf, len(c), iptr(c), iptr(f)), "gf_pool_interpolate2fine")
 
+    def copy(self, slots, src_buf: int, dst_buf: int):
+        """State of slots from one buffer to the other (gf_pool_copy)."""
+        sl = np.ascontiguousarray(np.asarray(slots, dtype=np.int32).reshape(-1))
+        check(self.lib.gf_pool_copy(self.h, len(sl), iptr(sl), src_buf, dst_buf), "gf_pool_copy")
+
     def _cf_links(self, coarse, fine, side, igrid):
         c, f, s = (np.ascontiguousarray(np.asarray(a, dtype=np.int32).reshape(-1)) for a in (coarse, fine, side))
         g = np.zeros_like(s) if igrid is None else np.ascontiguousarray(np.asarray(igrid, dtype=np.int32).reshape(-1))

@@ -463,6 +463,15 @@ int gf_pool_average_ghost(gf_pool *pool, int buf, int n, const int *coarse, cons
 int gf_pool_interpolate_ghost(gf_pool *pool, int buf, int n, const int *coarse, const int *fine,
                               const int *side, const int *igrid);
 
+/* Copy the state of slots[0..n) (q, ghost cells included) from buffer src_buf
+ * to buffer dst_buf.  Replaces fclaw2d_clawpatch_save_current_step, ForestClaw's
+ * copy of a patch's state into its saved copy that the plugin's update makes
+ * before each step (fc2d_cpucuda.cpp:306, :443), for resident patches: a
+ * subcycled run steps the coarse and the fine level into different buffers
+ * and brings them into one buffer at a synchronisation time with this copy,
+ * and the saved copy is what gf_pool_timeinterp reads as q_last. */
+int gf_pool_copy(gf_pool *pool, int n, const int *slots, int src_buf, int dst_buf);
+
 /* Time interpolation for coarse ghost sources, replaces
  * fc2d_geoclaw_fort_timeinterp (fc2d_geoclaw_timeinterp_fort.f:1-69): for
  * k < n, the two interior layers next to each face of slot dst[k] of buffer

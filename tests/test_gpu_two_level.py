@@ -97,7 +97,7 @@ def initial(lv):
     return np.array(qs), np.array(auxs)
 
 
-def oracle_ghosts(oc, lv, nbr, links, qs, auxs):
+def oracle_ghosts(oc, lv, nbr, links, qs, auxs, check=None):
     interior = (slice(MBC, MBC + MX), slice(MBC, MBC + MX))
     for q in qs:
         g = np.ones(q.shape[:2], bool)
@@ -127,7 +127,8 @@ def oracle_ghosts(oc, lv, nbr, links, qs, auxs):
     for c, f, side, ig in links:
         oc.interpolate_ghost(qs[c], qs[f], auxs[c], auxs[f], side, ig)
     walls(1)                                              # then the fine level's
-    assert all(np.isfinite(q).all() for q in qs), "a ghost region is not filled"
+    for s in (range(len(qs)) if check is None else check):
+        assert np.isfinite(qs[s]).all(), f"a ghost region of slot {s} is not filled"
 
 
 @pytest.mark.parametrize("where", ["centre", "corner"])
@@ -174,5 +175,85 @@ def test_two_level_run_matches_oracle(gf, oracle_c, where):
         r = np.stack([np.moveaxis(oq[s][inner], -1, 0) for s in range(n)], axis=1)
         assert_state_close(g, r, 1e-10)
         assert (r[0] > 1e-3).any() and (r[0] <= 1e-3).any()   # wet and dry cells both present
+    finally:
+        pool.close()
+
+
+def test_two_level_subcycled_with_time_interpolation(gf, oracle_c):
+    """Subcycling: per coarse step dt the fine level takes two steps of dt/2.
+    The ghosts at t are filled as above; the coarse level steps into the other
+    buffer, the fine level takes its first substep; for the second, the coarse
+    level's border layers are interpolated in time (alpha = 1/2,
+    gf_pool_timeinterp) into 12 more slots described like the coarse patches,
+    whose ghost cells come from their same-level copies, the fine level's
+    average at t + dt/2 and the walls, and the fine ghosts are interpolated
+    from them; after the second substep gf_pool_copy brings the coarse state
+    into the fine level's buffer.  The oracle does the same over freshly NaN
+    arrays (the time-interpolated copies hold only their border layers), so a
+    read beyond what the time interpolation writes would show."""
+    lv = leaves(CENTRE)
+    nbr, links = build_links(lv, CENTRE)
+    nl = len(lv)
+    coarse = [s for s, l in enumerate(lv) if l[0] == 0]
+    fine = [s for s, l in enumerate(lv) if l[0] == 1]
+    nc, nf = len(coarse), len(fine)
+    assert coarse == list(range(nc)) and fine == list(range(nc, nl))
+    ti = [nl + s for s in coarse]                        # time-interpolated copies
+    nbr_ti = np.where(nbr[:nc] >= 0, nbr[:nc] + nl, nbr[:nc]).astype(np.int32)
+    links_ti = [(nl + c, f, side, ig) for c, f, side, ig in links]
+    qs, auxs = initial(lv)
+    dxs = np.array([DXC if l[0] == 0 else DXF for l in lv])
+    xlo = np.array([l[1] * MX * d for l, d in zip(lv, dxs)])
+    ylo = np.array([l[2] * MX * d for l, d in zip(lv, dxs)])
+    pool = Pool(MX, MX, nl + nc)
+    try:
+        pool.set_patches(np.concatenate([dxs, dxs[:nc]]), np.concatenate([dxs, dxs[:nc]]),
+                         np.concatenate([xlo, xlo[:nc]]), np.concatenate([ylo, ylo[:nc]]),
+                         np.concatenate([nbr, nbr_ti]))
+        pool.upload_aos(0, qs, auxs, buf=0)
+        pool.upload_aos(nl, qs[:nc], auxs[:nc], buf=0)
+        pool.upload_aos(nl, qs[:nc], auxs[:nc], buf=1)
+        L = [np.array(x, dtype=np.int32) for x in zip(*links)]
+        LT = [np.array(x, dtype=np.int32) for x in zip(*links_ti)]
+        oq = [q.copy() for q in qs]
+        oa = [a.copy() for a in auxs]
+        dtc, t, b = 8e-4, 0.0, 0
+        dtf = dtc / 2
+        psize = 8 * (MX - 2) * 3
+        for step in range(10):
+            print(f"case two-level subcycled coarse step {step}", flush=True)
+            # --- device ---
+            pool.average_ghost(*L, buf=b)
+            pool.interpolate_ghost(*L, buf=b)
+            pool.ghost_fill(0, nl, buf=b)
+            cc, _ = pool.step(dtc, 0, nc, src=b, dst=1 - b)
+            cf1, _ = pool.step(dtf, nc, nf, src=b, dst=1 - b)
+            pool.timeinterp(coarse, b, 1 - b, 0.5, ti, 1 - b)
+            pool.average_ghost(*LT, buf=1 - b)
+            pool.interpolate_ghost(*LT, buf=1 - b)
+            pool.ghost_fill(nc, nf, buf=1 - b)
+            cf2, _ = pool.step(dtf, nc, nf, src=1 - b, dst=b)
+            pool.copy(coarse, 1 - b, b)
+            # --- oracle ---
+            oracle_ghosts(oracle_c, lv, nbr, links, oq, oa)
+            last = [oq[s].copy() for s in coarse]
+            rc = max(oracle_c.patch_update(MX, MX, MBC, oq[s], oa[s], DXC, DXC, t, dtc) for s in coarse)
+            rf1 = max(oracle_c.patch_update(MX, MX, MBC, oq[s], oa[s], DXF, DXF, t, dtf) for s in fine)
+            mixed = []
+            for k, s in enumerate(coarse):
+                q = np.full_like(oq[s], np.nan)
+                assert oracle_c.timeinterp(oq[s], last[k], q, 0.5, psize) == 0
+                mixed.append(q)
+            mixed += [oq[s] for s in fine]
+            oracle_ghosts(oracle_c, lv, nbr, links, mixed, oa, check=fine)
+            rf2 = max(oracle_c.patch_update(MX, MX, MBC, oq[s], oa[s], DXF, DXF, t + dtf, dtf) for s in fine)
+            for got, want in ((cc, rc), (cf1, rf1), (cf2, rf2)):
+                assert got == pytest.approx(want, rel=1e-13, abs=0), step
+            t += dtc
+        got = pool.download_aos(0, nl, buf=b)
+        inner = (slice(MBC, MBC + MX), slice(MBC, MBC + MX))
+        g = np.stack([np.moveaxis(got[s][inner], -1, 0) for s in range(nl)], axis=1)
+        r = np.stack([np.moveaxis(oq[s][inner], -1, 0) for s in range(nl)], axis=1)
+        assert_state_close(g, r, 1e-10)
     finally:
         pool.close()
