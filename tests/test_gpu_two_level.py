@@ -257,3 +257,99 @@ def test_two_level_subcycled_with_time_interpolation(gf, oracle_c):
         assert_state_close(g, r, 1e-10)
     finally:
         pool.close()
+
+
+def test_regrid_cycle_matches_oracle(gf, oracle_c):
+    """A dynamic-AMR cycle on resident patches: 4 steps of a uniform 4x4
+    brick, refinement of its centre 2x2 (gf_pool_interpolate2fine from the
+    parents, whose halo is read through the one-level neighbour table), 6
+    steps on two levels with the coarse/fine ghost transfers, coarsening
+    back (gf_pool_average2coarse), 4 more steps on one level.  Slots are fixed
+    (coarse patch (i, j) in slot 4j+i, the children in slots 16..31), a regrid
+    only re-describes them; the oracle runs the same cycle with its own
+    interpolate2fine / average2coarse."""
+    parents = sorted(CENTRE, key=lambda t: (t[1], t[0]))
+
+    def slot(lev, i, j):
+        if lev == 0:
+            return j * NC + i
+        r = parents.index((i // 2, j // 2))
+        return NC * NC + 4 * r + 2 * (j % 2) + (i % 2)
+
+    nslot = NC * NC + 4 * len(parents)
+    geo = {}
+    for j in range(NC):
+        for i in range(NC):
+            geo[slot(0, i, j)] = (0, i, j)
+    for pi, pj in parents:
+        for cj in range(2):
+            for ci in range(2):
+                geo[slot(1, 2 * pi + ci, 2 * pj + cj)] = (1, 2 * pi + ci, 2 * pj + cj)
+    order = [geo[s] for s in range(nslot)]
+    qs, auxs = initial(order)                       # slot-ordered arrays; children's q is overwritten
+    dxs = np.array([DXC if l[0] == 0 else DXF for l in order])
+    xlo = np.array([l[1] * MX * d for l, d in zip(order, dxs)])
+    ylo = np.array([l[2] * MX * d for l, d in zip(order, dxs)])
+
+    def phase(refined):
+        lvp = leaves(refined)
+        nb, links = build_links(lvp, refined)
+        sl = [slot(*l) for l in lvp]
+        table = np.full((nslot, 8), GF_NB_STORED, dtype=np.int32)  # non-leaves: host-supplied, unused
+        for k, s in enumerate(sl):
+            table[s] = [sl[v] if v >= 0 else v for v in nb[k]]
+        return lvp, nb, links, sl, table
+
+    pool = Pool(MX, MX, nslot)
+    try:
+        oq = [q.copy() for q in qs]
+        oa = [a.copy() for a in auxs]
+        pool.set_patches(dxs, dxs, xlo, ylo, phase(set())[4])
+        pool.upload_aos(0, qs, auxs, buf=0)
+        dt, t, b = 4e-4, 0.0, 0
+        kids = [[slot(1, 2 * pi + ci, 2 * pj + cj) for cj in range(2) for ci in range(2)] for pi, pj in parents]
+        par = [slot(0, pi, pj) for pi, pj in parents]
+
+        def run(refined, nsteps):
+            nonlocal t, b
+            lvp, nb, links, sl, table = phase(refined)
+            L = [np.array([sl[x] for x in col], dtype=np.int32) if k < 2 else np.array(col, dtype=np.int32)
+                 for k, col in enumerate(zip(*links))] if links else None
+            for _ in range(nsteps):
+                print(f"case regrid cycle t={t:.4f} levels={1 + bool(refined)}", flush=True)
+                if L is not None:
+                    pool.average_ghost(*L, buf=b)
+                    pool.interpolate_ghost(*L, buf=b)
+                pool.ghost_fill(0, nslot, buf=b)
+                _, cfl = pool.step(dt, 0, nslot, src=b, dst=1 - b)
+                views = [oq[s] for s in sl]
+                oracle_ghosts(oracle_c, lvp, nb, links, views, [oa[s] for s in sl])
+                ref = max(oracle_c.patch_update(MX, MX, MBC, oq[s], oa[s], dxs[s], dxs[s], t, dt) for s in sl)
+                assert max(cfl[s] for s in sl) == pytest.approx(ref, rel=1e-13, abs=0)
+                b, t = 1 - b, t + dt
+            return lvp, nb, links, sl
+
+        run(set(), 4)
+        # refine: the parents' halo as the one-level step sees it
+        lvp, nb, links, sl = phase(set())[:4]
+        oracle_ghosts(oracle_c, lvp, nb, links, [oq[s] for s in sl], [oa[s] for s in sl])
+        pool.interpolate2fine(par, np.concatenate(kids), buf=b)
+        for p, ks in zip(par, kids):
+            for g, k in enumerate(ks):
+                oracle_c.interpolate2fine(oq[p], oq[k], oa[p], oa[k], g)
+        pool.set_patches(dxs, dxs, xlo, ylo, phase(CENTRE)[4])
+        run(CENTRE, 6)
+        # coarsen
+        pool.average2coarse(par, np.concatenate(kids), buf=b)
+        for p, ks in zip(par, kids):
+            for g, k in enumerate(ks):
+                oracle_c.average2coarse(oq[p], oq[k], oa[p], oa[k], g)
+        pool.set_patches(dxs, dxs, xlo, ylo, phase(set())[4])
+        run(set(), 4)
+        got = pool.download_aos(0, NC * NC, buf=b)
+        inner = (slice(MBC, MBC + MX), slice(MBC, MBC + MX))
+        g = np.stack([np.moveaxis(got[s][inner], -1, 0) for s in range(NC * NC)], axis=1)
+        r = np.stack([np.moveaxis(oq[s][inner], -1, 0) for s in range(NC * NC)], axis=1)
+        assert_state_close(g, r, 1e-10)
+    finally:
+        pool.close()
