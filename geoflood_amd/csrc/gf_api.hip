@@ -153,6 +153,7 @@ struct gf_pool {
     ScenCtrl* ctrl = nullptr;
     double* d_stage = nullptr;  // device staging (kChunk patches of q and aux)
     double* h_stage = nullptr;  // pinned host staging
+    size_t stage_doubles = 0;   // size of each staging buffer
     double* h_cfl = nullptr;    // pinned
     // batched path
     double* hmax2 = nullptr;               // [2][capacity]
@@ -448,6 +449,7 @@ gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
               hipMalloc(&p->dry_age, (size_t)capacity * sizeof(int)) == hipSuccess &&
               hipMalloc(&p->inflow_list, (size_t)capacity * sizeof(int)) == hipSuccess &&
               hipMalloc(&p->inflow_n, sizeof(int)) == hipSuccess;
+    if (ok) p->stage_doubles = chunk * 4 * p->np2;
     if (ok)
         ok = hipMemsetAsync(p->q[0], 0, qbytes, g.stream) == hipSuccess &&
              hipMemsetAsync(p->q[1], 0, qbytes, g.stream) == hipSuccess &&
@@ -611,17 +613,35 @@ static void host_parallel(int n, const std::function<void(int)>& f)
     g.workers->run(n, f);
 }
 
-// Split of a pool's staging buffers (min(kChunk, capacity) * 4 * np2 doubles
-// each) into nbuf parts of `part` patches of `per` doubles: two parts when two
-// patches fit, so that the DMA and (un)pack kernel of one part overlap the
-// host copies into or out of the other.
+// Split of a pool's staging buffers (stage_doubles each, at least
+// min(kChunk, capacity) * 4 * np2) into nbuf parts of `part` patches of `per`
+// doubles: two parts when two patches fit, so that the DMA and (un)pack kernel
+// of one part overlap the host copies into or out of the other.
 struct StageSplit {
     int nbuf, part;
 };
 static StageSplit stage_split(const gf_pool* p, size_t per)
 {
-    const size_t fit = std::min<size_t>(kChunk, p->capacity) * 4 * p->np2 / per;
+    const size_t fit = p->stage_doubles / per;
     return fit >= 2 ? StageSplit{2, (int)(fit / 2)} : StageSplit{1, (int)fit};
+}
+
+// Grow the staging buffers so that two patches of `per` doubles fit (a small
+// pool with several aux fields per cell); they never shrink.
+static int ensure_stage(gf_pool* p, size_t per)
+{
+    if (p->stage_doubles >= 2 * per) return GF_OK;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    HIPCHK(hipFree(p->d_stage));
+    HIPCHK(hipHostFree(p->h_stage));
+    p->d_stage = nullptr;
+    p->h_stage = nullptr;
+    p->stage_doubles = 0;
+    if (hipMalloc(&p->d_stage, 2 * per * sizeof(double)) != hipSuccess ||
+        hipHostMalloc(&p->h_stage, 2 * per * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return fail(GF_ENOMEM, "staging buffers of %zu bytes", 2 * per * sizeof(double));
+    p->stage_doubles = 2 * per;
+    return GF_OK;
 }
 
 // Host patches -> pool slots [first, first+n) of buffer buf through the
@@ -633,8 +653,8 @@ static int staged_upload(gf_pool* p, int first, int n, int buf, int maux, bool w
 {
     const size_t np2 = p->np2;
     const size_t per = (3 + (with_aux ? (size_t)maux : 0)) * np2;
+    if (int r = ensure_stage(p, per)) return r;
     const StageSplit sp = stage_split(p, per);
-    if (sp.part < 1) return fail(GF_EUNSUPPORTED, "maux = %d does not fit the staging buffer", maux);
     for (int off = 0, c = 0; off < n; off += sp.part, c++) {
         const int b = c % sp.nbuf, m = std::min(sp.part, n - off);
         double* hs = p->h_stage + (size_t)b * sp.part * per;

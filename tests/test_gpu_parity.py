@@ -86,6 +86,22 @@ def test_small_patch_shapes(gf, oracle_c, mx):
     assert rel_err(q[2:-2, 2:-2], qo[2:-2, 2:-2]) <= RTOL_STEP
 
 
+@pytest.mark.parametrize("maux", [3, 5])
+def test_single_patch_several_aux_fields(gf, oracle_c, maux):
+    """One patch with maux > 1 (bathymetry first, the other fields unused by
+    the step): the one-patch pool's staging grows to hold (3 + maux) fields
+    for two patches, and the step equals the oracle's."""
+    rng = np.random.default_rng(40 + maux)
+    q, a1 = random_patch(rng, mx=16, my=16, wet_frac=0.8, topo=0.5)
+    aux = np.ascontiguousarray(np.concatenate([a1, rng.random(a1.shape[:2] + (maux - 1,))], axis=-1))
+    qo, ao = q.copy(), aux.copy()
+    c_ref = oracle_c.patch_update(16, 16, 2, qo, ao, 0.05, 0.05, 0.0, 0.004, maux=maux)
+    c = gf.step2_batch([q], [aux], 0.05, 0.05, 0.0, 0.004)
+    assert c == pytest.approx(c_ref, rel=1e-15, abs=0)
+    assert rel_err(q[2:-2, 2:-2], qo[2:-2, 2:-2]) <= RTOL_STEP
+    np.testing.assert_array_equal(aux, ao)
+
+
 def test_host_batch_chunked_staging(gf, oracle_c):
     """1,500 8x8 patches with maux = 3 pass through the pinned staging in
     several chunks on both halves of the double-buffered upload and download;
