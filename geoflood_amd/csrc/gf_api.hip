@@ -56,17 +56,14 @@ class HostWorkers {
 public:
     explicit HostWorkers(int nworkers)
     {
-        for (int t = 0; t < nworkers; t++) th_.emplace_back([this] { loop(); });
-    }
-    ~HostWorkers()
-    {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            quit_ = true;
+        try {
+            for (int t = 0; t < nworkers; t++) th_.emplace_back([this] { loop(); });
+        } catch (...) {
+            stop();  // the threads already started end before the exception leaves
+            throw;
         }
-        go_.notify_all();
-        for (auto& t : th_) t.join();
     }
+    ~HostWorkers() { stop(); }
     void run(int n, const std::function<void(int)>& f)
     {
         {
@@ -86,6 +83,16 @@ public:
     }
 
 private:
+    void stop()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            quit_ = true;
+        }
+        go_.notify_all();
+        for (auto& t : th_) t.join();
+        th_.clear();
+    }
     void work()
     {
         for (int k; (k = next_.fetch_add(1)) < n_;) (*job_)(k);
@@ -609,7 +616,18 @@ static void host_parallel(int n, const std::function<void(int)>& f)
         for (int k = 0; k < n; k++) f(k);
         return;
     }
-    if (!g.workers) g.workers = new HostWorkers(nt - 1);
+    if (!g.workers) {
+        // no exception may cross the C ABI: without threads the copies run here
+        try {
+            g.workers = new HostWorkers(nt - 1);
+        } catch (...) {
+            g.workers = nullptr;
+        }
+        if (!g.workers) {
+            for (int k = 0; k < n; k++) f(k);
+            return;
+        }
+    }
     g.workers->run(n, f);
 }
 
@@ -1540,7 +1558,6 @@ int gf_pool_write_frame(gf_pool* p, int buf, int first, int n, const char* dir, 
     if (!fq) return fail(GF_EIO, "cannot write %s", qpath.c_str());
     const int cells = p->mx * p->my;
     const size_t chunk = std::min<size_t>(kChunk, (size_t)p->capacity);  // d_stage holds >= 4 * cells per patch
-    const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::string> text(chunk);
     int rc = GF_OK;
     for (int off = 0; off < n && rc == GF_OK; off += (int)chunk) {
@@ -1554,22 +1571,17 @@ int gf_pool_write_frame(gf_pool* p, int buf, int first, int n, const char* dir, 
             rc = fail(GF_EHIP, "frame pack: %s", hipGetErrorString(e));
             break;
         }
-        auto work = [&](int t) {
-            for (int k = t; k < m; k += nthreads) {
-                const int s = first + off + k;
-                const PatchDesc& d = p->h_desc[s];
-                std::string& out = text[k];
-                out.clear();
-                out.reserve(400 + (size_t)cells * 105 + (size_t)p->my * 3);
-                gf::ascii::frame_patch(out, p->h_stage + (size_t)k * cells * 4, p->mx, p->my, d.xlo, d.ylo, d.dx,
-                                       d.dy, patch_num ? patch_num[off + k] : s, level ? level[off + k] : 0,
-                                       blockno ? blockno[off + k] : 0, mpirank);
-            }
-        };
-        std::vector<std::thread> pool_threads;
-        for (int t = 1; t < std::min(nthreads, m); t++) pool_threads.emplace_back(work, t);
-        work(0);
-        for (auto& th : pool_threads) th.join();
+        // patches formatted on the library's host threads, written in order
+        host_parallel(m, [&](int k) {
+            const int s = first + off + k;
+            const PatchDesc& d = p->h_desc[s];
+            std::string& out = text[k];
+            out.clear();
+            out.reserve(400 + (size_t)cells * 105 + (size_t)p->my * 3);
+            gf::ascii::frame_patch(out, p->h_stage + (size_t)k * cells * 4, p->mx, p->my, d.xlo, d.ylo, d.dx, d.dy,
+                                   patch_num ? patch_num[off + k] : s, level ? level[off + k] : 0,
+                                   blockno ? blockno[off + k] : 0, mpirank);
+        });
         for (int k = 0; k < m; k++)
             if (std::fwrite(text[k].data(), 1, text[k].size(), fq) != text[k].size()) {
                 rc = fail(GF_EIO, "cannot write %s", qpath.c_str());
