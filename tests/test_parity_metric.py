@@ -1,5 +1,6 @@
 """The per-cell comparison used by the GPU parity tests flags what it must."""
 import numpy as np
+import pytest
 
 from _parity import state_diff
 
@@ -44,3 +45,22 @@ def test_dry_cells_absolute():
     t[0, j, i] = 5e-18
     rel, dry, mask = state_diff(s, t)
     assert mask and rel == 0.0 and 0 < dry < 1e-10
+
+
+def test_bench_state_parity_matches_the_test_metric():
+    """bench.py's parity_vs_cpu uses the floored per-cell metric of the
+    parity tests: the same numbers on the same states, a near-dry cell at
+    rounding level does not count as a relative difference of 1."""
+    import bench
+    from _parity import state_diff
+    rng = np.random.default_rng(11)
+    ref = np.stack([np.abs(rng.normal(1, 0.5, (40, 40))), rng.normal(0, 1, (40, 40)), rng.normal(0, 1, (40, 40))])
+    ref[0][rng.random((40, 40)) < 0.2] = 0.0
+    got = ref * (1 + 1e-13 * rng.normal(size=ref.shape))
+    got[0][ref[0] == 0.0] = 5e-18
+    r = bench.state_parity(got, ref)
+    rel, dry, mask = state_diff(got, ref)
+    assert r["mask_equal"] == mask is True
+    assert r["wet_max_rel"] == pytest.approx(rel, rel=1e-12)
+    assert r["dry_max_abs_over_drytol"] == pytest.approx(dry, rel=1e-12)
+    assert r["wet_max_rel"] < 1e-12 and r["dry_max_abs_over_drytol"] < 1e-12
