@@ -93,3 +93,26 @@ def test_cf_ghost_entry_points_reject_without_a_pool():
     assert lib.gf_pool_average2coarse(None, 0, 1, z, z) == -1  # GF_EINVAL
     assert lib.gf_pool_interpolate2fine(None, 0, 1, z, z) == -1  # GF_EINVAL
     assert b"NULL" in lib.gf_last_error()
+
+
+def test_header_compiles_as_c(tmp_path):
+    """The reference plugin that includes include/geoflood_amd.h is C: the
+    header must compile as C99, every declaration usable from C."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    src = tmp_path / "use.c"
+    src.write_text('#include "geoflood_amd.h"\n'
+                   "int main(void) {\n"
+                   "    gf_pool *pool = gf_pool_create(32, 32, 2, 4);\n"
+                   "    int c[1] = {0}, f[1] = {1}, s[1] = {1}, g[1] = {0};\n"
+                   "    gf_pool_average_ghost(pool, 0, 1, c, f, s, g);\n"
+                   "    gf_pool_copy(pool, 1, c, 1, 0);\n"
+                   "    return gf_pool_destroy(pool);\n"
+                   "}\n")
+    r = subprocess.run([cc, "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only",
+                        "-I", inc, str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
