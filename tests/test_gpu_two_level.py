@@ -353,3 +353,125 @@ def test_regrid_cycle_matches_oracle(gf, oracle_c):
         assert_state_close(g, r, 1e-10)
     finally:
         pool.close()
+
+
+def leaves_multi(blocks):
+    """Leaves of a nested layout: blocks[l] is the set of level-l lattice
+    patches (i, j) that are refined (each block inside the previous one's
+    children, 2:1 balanced).  Returns (level, i, j) leaves, coarse first."""
+    out = []
+    region = {(i, j) for j in range(NC) for i in range(NC)}
+    for lev in range(len(blocks) + 1):
+        ref = blocks[lev] if lev < len(blocks) else set()
+        out += [(lev, i, j) for (i, j) in sorted(region, key=lambda t: (t[1], t[0])) if (i, j) not in ref]
+        region = {(2 * i + ci, 2 * j + cj) for (i, j) in ref for cj in range(2) for ci in range(2)}
+    return out
+
+
+def build_links_multi(lv):
+    """Neighbour table and coarse/fine links for leaves of any number of
+    levels: a side whose same-level region is a leaf copies it, one covered by
+    a coarser leaf or refined further is stored; links join each leaf to the
+    finer leaves across its sides (coarse slot, fine slot, side, igrid)."""
+    slot = {k: s for s, k in enumerate(lv)}
+    nbr = np.full((len(lv), 8), GF_NB_WALL, dtype=np.int32)
+    links = []
+    for s, (lev, i, j) in enumerate(lv):
+        n = NC << lev
+        for d, (dx, dy) in enumerate(DIRS):
+            a, b = i + dx, j + dy
+            if not (0 <= a < n and 0 <= b < n):
+                continue
+            if (lev, a, b) in slot:
+                nbr[s, d] = slot[(lev, a, b)]
+                continue
+            nbr[s, d] = GF_NB_STORED
+            kids = [(ci, cj) for cj in range(2) for ci in range(2)
+                    if (dx == 0 or ci == (0 if dx > 0 else 1)) and (dy == 0 or cj == (0 if dy > 0 else 1))]
+            for ci, cj in kids:
+                f = slot.get((lev + 1, 2 * a + ci, 2 * b + cj))
+                if f is not None:
+                    links.append((s, f, d, (cj if d < 2 else ci) if d < 4 else 0))
+    return nbr, links
+
+
+def test_three_levels_match_oracle(gf, oracle_c):
+    """Three levels (the centre 2x2 of the refined block refined again):
+    averaging from both finer levels in one call (a middle-level patch is
+    coarse for one link and fine for another), interpolation level by level
+    from coarse to fine (the middle level's interpolated ghosts feed the
+    finest level's), walls per level; 15 steps with one dt."""
+    blocks = [CENTRE, {(3, 3), (4, 3), (3, 4), (4, 4)}]
+    lv = leaves_multi(blocks)
+    nbr, links = build_links_multi(lv)
+    levels = np.array([l[0] for l in lv])
+    assert sorted(set(levels)) == [0, 1, 2] and len(lv) == 12 + 12 + 16
+    by_coarse = {0: [k for k in links if lv[k[0]][0] == 0], 1: [k for k in links if lv[k[0]][0] == 1]}
+    n = len(lv)
+    dxs = np.array([DXC / (1 << l[0]) for l in lv])
+    qs, auxs = [], []
+    for lev, i, j in lv:                               # the analytic state on each level's lattice
+        dx = dxs[levels.tolist().index(lev)]
+        nn = NC * MX << lev
+        idx = np.arange(-MBC, nn + MBC)
+        mir = np.where(idx < 0, -idx - 1, np.where(idx >= nn, 2 * nn - idx - 1, idx))
+        X, Y = np.meshgrid((mir + 0.5) * dx, (mir + 0.5) * dx)
+        h, b = field(X, Y)
+        sy, sx = slice(j * MX, j * MX + MX + 2 * MBC), slice(i * MX, i * MX + MX + 2 * MBC)
+        q = np.zeros((MX + 2 * MBC, MX + 2 * MBC, 3))
+        q[..., 0] = h[sy, sx]
+        qs.append(q)
+        auxs.append(np.ascontiguousarray(b[sy, sx][..., None]))
+    xlo = np.array([l[1] * MX * d for l, d in zip(lv, dxs)])
+    ylo = np.array([l[2] * MX * d for l, d in zip(lv, dxs)])
+    pool = Pool(MX, MX, n)
+    try:
+        pool.set_patches(dxs, dxs, xlo, ylo, nbr)
+        pool.upload_aos(0, np.array(qs), np.array(auxs), buf=0)
+        LA = [np.array(x, dtype=np.int32) for x in zip(*links)]
+        LI = {c: [np.array(x, dtype=np.int32) for x in zip(*by_coarse[c])] for c in (0, 1)}
+        oq = [q.copy() for q in qs]
+        oa = [a.copy() for a in auxs]
+        dt, t, cur = 2e-4, 0.0, 0
+        interior = (slice(MBC, MBC + MX), slice(MBC, MBC + MX))
+        for step in range(15):
+            print(f"case three levels step {step}", flush=True)
+            pool.average_ghost(*LA, buf=cur)
+            pool.interpolate_ghost(*LI[0], buf=cur)
+            pool.interpolate_ghost(*LI[1], buf=cur)
+            pool.ghost_fill(buf=cur)
+            c_dev, _ = pool.step(dt, src=cur, dst=1 - cur)
+            # oracle: NaN ghosts, same-level copies, averaging, then per level walls and interpolation
+            for q in oq:
+                g = np.ones(q.shape[:2], bool)
+                g[interior] = False
+                q[g] = np.nan
+            for s in range(n):
+                for d, (dx, dy) in enumerate(DIRS):
+                    tgt = nbr[s, d]
+                    if tgt < 0:
+                        continue
+                    r = range(1, MBC + 1)
+                    di = list(range(MBC, MBC + MX)) if dx == 0 else ([MBC - k for k in r] if dx < 0 else [MBC + MX - 1 + k for k in r])
+                    dj = list(range(MBC, MBC + MX)) if dy == 0 else ([MBC - k for k in r] if dy < 0 else [MBC + MX - 1 + k for k in r])
+                    oq[s][np.ix_(dj, di)] = oq[tgt][np.ix_([y - dy * MX for y in dj], [x - dx * MX for x in di])]
+            for c, f, side, ig in links:
+                oracle_c.average_ghost(oq[c], oq[f], oa[c], oa[f], side, ig)
+            for lev in (0, 1, 2):
+                for s in np.flatnonzero(levels == lev):
+                    mth = [3 if nbr[s, k] == GF_NB_WALL else -1 for k in range(4)]
+                    if max(mth) > 0:
+                        oracle_c.bc2(MX, MX, MBC, oq[s], oa[s], mth)
+                if lev < 2:
+                    for c, f, side, ig in by_coarse[lev]:
+                        oracle_c.interpolate_ghost(oq[c], oq[f], oa[c], oa[f], side, ig)
+            assert all(np.isfinite(q).all() for q in oq), "a ghost region is not filled"
+            c_ref = max(oracle_c.patch_update(MX, MX, MBC, oq[s], oa[s], dxs[s], dxs[s], t, dt) for s in range(n))
+            assert c_dev == pytest.approx(c_ref, rel=1e-13, abs=0), step
+            cur, t = 1 - cur, t + dt
+        got = pool.download_aos(0, n, buf=cur)
+        g = np.stack([np.moveaxis(got[s][interior], -1, 0) for s in range(n)], axis=1)
+        r = np.stack([np.moveaxis(oq[s][interior], -1, 0) for s in range(n)], axis=1)
+        assert_state_close(g, r, 1e-10)
+    finally:
+        pool.close()
