@@ -11,8 +11,10 @@
 #include <algorithm>
 #include <atomic>
 #include <condition_variable>
+#include <exception>
 #include <functional>
 #include <mutex>
+#include <new>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -42,6 +44,18 @@ int fail(int code, const char* fmt, ...)
     g_err = buf;
     return code;
 }
+
+// Every entry point of the C ABI is a function-try-block: no C++ exception
+// (a failed host allocation, a thread that cannot start) crosses into a C
+// caller; it becomes a status and a message.
+#define GF_CATCH_STATUS                                                           \
+    catch (const std::bad_alloc&) { return fail(GF_ENOMEM, "host memory allocation failed"); } \
+    catch (const std::exception& e_) { return fail(GF_ESTATE, "host error: %s", e_.what()); } \
+    catch (...) { return fail(GF_ESTATE, "host error"); }
+#define GF_CATCH_NULL                                                             \
+    catch (const std::bad_alloc&) { fail(GF_ENOMEM, "host memory allocation failed"); return nullptr; } \
+    catch (const std::exception& e_) { fail(GF_ESTATE, "host error: %s", e_.what()); return nullptr; } \
+    catch (...) { fail(GF_ESTATE, "host error"); return nullptr; }
 
 #define HIPCHK(x)                                                                       \
     do {                                                                                \
@@ -260,7 +274,7 @@ static gf::StepArgs step_args(gf_pool* p, int first, int count, double* q0, doub
 extern "C" {
 
 int gf_init(int device)
-{
+try {
     int count = 0;
     HIPCHK(hipGetDeviceCount(&count));
     if (count <= 0) return fail(GF_EHIP, "no HIP device");
@@ -282,10 +296,10 @@ int gf_init(int device)
     default_params();
     g.init = true;
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_finalize(void)
-{
+try {
     if (!g.init) return GF_OK;
     hipStreamSynchronize(g.stream);
     if (g.batch_pool) gf_pool_destroy(g.batch_pool);
@@ -308,16 +322,16 @@ int gf_finalize(void)
     hipStreamDestroy(g.stream);
     g = Lib();
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_check_parameters(int mwaves)
-{
+try {
     if (mwaves != 3) return fail(GF_EUNSUPPORTED, "mwaves = %d; the shallow-water solver has 3 waves", mwaves);
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_set_params(const gf_params* p)
-{
+try {
     if (int r = need_init()) return r;
     if (!p) return fail(GF_EINVAL, "params is NULL");
     if (!(p->grav > 0.0)) return fail(GF_EINVAL, "grav must be > 0");
@@ -339,15 +353,15 @@ int gf_set_params(const gf_params* p)
     for (int k = 0; k < 3; k++) g.P.mthlim[k] = p->mthlim[k];
     g.params_gen += 1;
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_get_params(gf_params* p)
-{
+try {
     if (int r = need_init()) return r;
     if (!p) return fail(GF_EINVAL, "params is NULL");
     *p = g.user;
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 const char* gf_last_error(void) { return g_err.c_str(); }
 
@@ -412,7 +426,7 @@ static int rebuild_inflow(gf_pool* p)
 }
 
 gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
-{
+try {
     if (need_init()) return nullptr;
     if (mbc != gf::MBC) {
         fail(GF_EUNSUPPORTED, "mbc = %d; only mbc = 2 is supported", mbc);
@@ -498,10 +512,10 @@ gf_pool* gf_pool_create(int mx, int my, int mbc, int capacity)
     p->h_ctrl.assign(capacity, ScenCtrl{});
     p->hyd_mem.assign(capacity, nullptr);
     return p;
-}
+} GF_CATCH_NULL
 
 int gf_pool_destroy(gf_pool* p)
-{
+try {
     if (!p) return GF_OK;
     if (g.stream) hipStreamSynchronize(g.stream);
     if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
@@ -533,11 +547,11 @@ int gf_pool_destroy(gf_pool* p)
     if (p->h_cfl) hipHostFree(p->h_cfl);
     delete p;
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_pool_set_patches(gf_pool* p, int n, const double* dx, const double* dy, const double* xlo,
                         const double* ylo, const int* scen, const int* nbr)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (n < 0 || n > p->capacity) return fail(GF_EINVAL, "n = %d outside [0, %d]", n, p->capacity);
     if (n > 0 && (!dx || !dy)) return fail(GF_EINVAL, "dx/dy are NULL");
@@ -571,11 +585,11 @@ int gf_pool_set_patches(gf_pool* p, int n, const double* dx, const double* dy, c
     p->graph_dirty = true;
     p->args_dirty = true;
     return rebuild_inflow(p);
-}
+} GF_CATCH_STATUS
 
 int gf_pool_describe(gf_pool* p, int first, int n, const double* dx, const double* dy, const double* xlo,
                      const double* ylo, const int* nbr)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (first < 0 || n <= 0 || first + n > p->capacity) return fail(GF_EINVAL, "slots [%d,%d) outside the pool", first, first + n);
     if (!dx || !dy || !nbr) return fail(GF_EINVAL, "NULL argument");
@@ -603,7 +617,7 @@ int gf_pool_describe(gf_pool* p, int first, int n, const double* dx, const doubl
     if (int r = reset_dry_age(p, first, n)) return r;
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 }  // extern "C"
 
@@ -727,7 +741,7 @@ static int staged_download(gf_pool* p, int first, int n, int buf, Get&& get)
 extern "C" {
 
 int gf_pool_upload(gf_pool* p, int first, int n, int buf, const double* q_aos, const double* aux_aos, int maux)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (first < 0 || n < 0 || first + n > p->capacity) return fail(GF_EINVAL, "slots [%d,%d) outside the pool", first, first + n);
     if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
@@ -745,10 +759,10 @@ int gf_pool_upload(gf_pool* p, int first, int n, int buf, const double* q_aos, c
     if (int r = reset_dry_age(p, first, n)) return r;
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_pool_download(gf_pool* p, int first, int n, int buf, double* q_aos)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (first < 0 || n < 0 || first + n > p->capacity) return fail(GF_EINVAL, "slots [%d,%d) outside the pool", first, first + n);
     if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
@@ -757,10 +771,10 @@ int gf_pool_download(gf_pool* p, int first, int n, int buf, double* q_aos)
     return staged_download(p, first, n, buf, [&](int k, const double* q) {
         std::memcpy(q_aos + (size_t)k * 3 * np2, q, 3 * np2 * sizeof(double));
     });
-}
+} GF_CATCH_STATUS
 
 int gf_pool_download_aux(gf_pool* p, int first, int n, double* aux_aos)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (first < 0 || n < 0 || first + n > p->capacity) return fail(GF_EINVAL, "slots [%d,%d) outside the pool", first, first + n);
     if (n > 0 && !aux_aos) return fail(GF_EINVAL, "aux is NULL");
@@ -769,7 +783,7 @@ int gf_pool_download_aux(gf_pool* p, int first, int n, double* aux_aos)
                           hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 static gf::InflowArgs inflow_args(gf_pool* p, int first, int last, int buf, double t)
 {
@@ -793,7 +807,7 @@ static gf::InflowArgs inflow_args(gf_pool* p, int first, int last, int buf, doub
 static int inflow_grid() { return 4 * (g.num_cu > 0 ? g.num_cu : 64); }
 
 int gf_pool_ghost_fill(gf_pool* p, int first, int n, int buf, double t)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (first < 0 || n < 0 || first + n > p->n) return fail(GF_EINVAL, "patches [%d,%d) outside [0,%d)", first, first + n, p->n);
     if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
@@ -803,7 +817,7 @@ int gf_pool_ghost_fill(gf_pool* p, int first, int n, int buf, double t)
     HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, first, first + n, buf, t), inflow_grid(), g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 static int pool_step_impl(gf_pool* p, int first, int n, int src, int dst, double dt, double* cfl,
                           double* maxcfl, bool plain)
@@ -852,18 +866,18 @@ static int pool_step_impl(gf_pool* p, int first, int n, int src, int dst, double
 }
 
 int gf_pool_step(gf_pool* p, int first, int n, int src, int dst, double dt, double* cfl, double* maxcfl)
-{
+try {
     return pool_step_impl(p, first, n, src, dst, dt, cfl, maxcfl, false);
-}
+} GF_CATCH_STATUS
 
 int gf_pool_step_plain(gf_pool* p, int first, int n, int src, int dst, double dt, double* cfl, double* maxcfl)
-{
+try {
     return pool_step_impl(p, first, n, src, dst, dt, cfl, maxcfl, true);
-}
+} GF_CATCH_STATUS
 
 int gf_pool_add_scenario(gf_pool* p, int first_patch, int npatch, double t0, double dt0,
                          double desired_cfl, double max_cfl, int target_steps, double t_end)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (first_patch < 0 || npatch <= 0 || first_patch + npatch > p->n)
         return fail(GF_EINVAL, "scenario patches [%d,%d) outside the described slots [0,%d)", first_patch,
@@ -905,10 +919,10 @@ int gf_pool_add_scenario(gf_pool* p, int first_patch, int npatch, double t0, dou
     if (id + 1 > p->nscen) p->nscen = id + 1;
     if (int r = rebuild_inflow(p)) return r;
     return id;
-}
+} GF_CATCH_STATUS
 
 int gf_pool_remove_scenario(gf_pool* p, int s)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
     const int first = p->h_ctrl[s].first_patch, n = p->h_ctrl[s].npatch;
@@ -921,11 +935,11 @@ int gf_pool_remove_scenario(gf_pool* p, int s)
     if (p->hyd_mem[s]) HIPCHK(hipFree(p->hyd_mem[s]));
     p->hyd_mem[s] = nullptr;
     return rebuild_inflow(p);
-}
+} GF_CATCH_STATUS
 
 int gf_pool_set_scenario_inflow(gf_pool* p, int s, int nrows, const double* time, const double* hu,
                                 double b, double x0, double y0)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
     if (nrows < 1 || !time || !hu) return fail(GF_EINVAL, "hydrograph needs >= 1 rows of time and discharge");
@@ -950,10 +964,10 @@ int gf_pool_set_scenario_inflow(gf_pool* p, int s, int nrows, const double* time
     if (p->hyd_mem[s]) HIPCHK(hipFree(p->hyd_mem[s]));
     p->hyd_mem[s] = d;
     return rebuild_inflow(p);
-}
+} GF_CATCH_STATUS
 
 int gf_pool_set_scenario_friction(gf_pool* p, int s, double manning_n)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
     if (!(manning_n >= 0.0)) return fail(GF_EINVAL, "manning_n must be >= 0");
@@ -963,10 +977,10 @@ int gf_pool_set_scenario_friction(gf_pool* p, int s, double manning_n)
     HIPCHK(hipMemcpyAsync(p->ctrl + s, &p->h_ctrl[s], sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_pool_clear_scenarios(gf_pool* p)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     for (int s = 0; s < p->capacity; s++)
         if (p->h_ctrl[s].npatch > 0)
@@ -985,10 +999,10 @@ int gf_pool_clear_scenarios(gf_pool* p)
     p->graph_dirty = true;
     p->args_dirty = true;
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_pool_set_scenario_active(gf_pool* p, int s, int active)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
     HIPCHK(hipMemcpyAsync(&p->h_ctrl[s], p->ctrl + s, sizeof(ScenCtrl), hipMemcpyDeviceToHost, g.stream));
@@ -997,10 +1011,10 @@ int gf_pool_set_scenario_active(gf_pool* p, int s, int active)
     HIPCHK(hipMemcpyAsync(p->ctrl + s, &p->h_ctrl[s], sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_pool_get_scenario(gf_pool* p, int s, gf_scenario_info* info)
-{
+try {
     if (!p || !info) return fail(GF_EINVAL, "pool or info is NULL");
     if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
     ScenCtrl c;
@@ -1016,10 +1030,10 @@ int gf_pool_get_scenario(gf_pool* p, int s, gf_scenario_info* info)
     info->active = c.active;
     info->cur = c.cur;
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_pool_get_scenarios(gf_pool* p, int first, int n, gf_scenario_info* info)
-{
+try {
     if (!p || (n > 0 && !info)) return fail(GF_EINVAL, "pool or info is NULL");
     if (first < 0 || n < 0 || first + n > p->capacity) return fail(GF_EINVAL, "scenarios [%d,%d) outside [0,%d)", first, first + n, p->capacity);
     if (n == 0) return GF_OK;
@@ -1039,7 +1053,7 @@ int gf_pool_get_scenarios(gf_pool* p, int first, int n, gf_scenario_info* info)
         o.cur = c[k].cur;
     }
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 static int ensure_pair_events(std::vector<hipEvent_t>& v, size_t pairs)
 {
@@ -1131,7 +1145,7 @@ static int any_active_flag(gf_pool* p, int* any_active)
 }
 
 int gf_pool_run(gf_pool* p, int substeps, int use_graph, int* any_active)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (substeps < 0) return fail(GF_EINVAL, "substeps must be >= 0");
     if (p->nscen == 0 || p->n == 0) return fail(GF_ESTATE, "no scenario registered");
@@ -1180,10 +1194,10 @@ int gf_pool_run(gf_pool* p, int substeps, int use_graph, int* any_active)
         g.last_launches = substeps;
     }
     return any_active_flag(p, any_active);
-}
+} GF_CATCH_STATUS
 
 int gf_pool_run_plain(gf_pool* p, int substeps, int* any_active)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (substeps < 0) return fail(GF_EINVAL, "substeps must be >= 0");
     if (p->nscen == 0 || p->n == 0) return fail(GF_ESTATE, "no scenario registered");
@@ -1263,11 +1277,11 @@ int gf_pool_run_plain(gf_pool* p, int substeps, int* any_active)
         *any_active = a;
     }
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_step2_batch(int mx, int my, int mbc, int maux, int n, double* const* q, double* const* aux,
                    const double* dx, const double* dy, double t, double dt, double* maxcfl)
-{
+try {
     (void)t;
     if (int r = need_init()) return r;
     if (n <= 0) return fail(GF_EINVAL, "batch size must be > 0");
@@ -1309,18 +1323,18 @@ int gf_step2_batch(int mx, int my, int mbc, int maux, int n, double* const* q, d
         return r;
     if (maxcfl) *maxcfl = *g.h_max;
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_synchronize(void)
-{
+try {
     if (int r = need_init()) return r;
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_pool_stats(gf_pool* p, long long* active_patches, long long* dry_patches, long long* attempts,
                   long long* steady_patches)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     long long s[4];
     HIPCHK(hipMemcpyAsync(s, p->stats, sizeof(s), hipMemcpyDeviceToHost, g.stream));
@@ -1330,7 +1344,7 @@ int gf_pool_stats(gf_pool* p, long long* active_patches, long long* dry_patches,
     if (attempts) *attempts = s[2];
     if (steady_patches) *steady_patches = s[3];
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 static int tag_impl(gf_pool* p, bool coarsen, int buf, double t, const gf_refine* c, int level, int maxlevel,
                     int init_flag, const int* patches, int n, int* tags)
@@ -1412,18 +1426,18 @@ static int tag_impl(gf_pool* p, bool coarsen, int buf, double t, const gf_refine
 
 int gf_pool_tag_refine(gf_pool* p, int buf, double t, const gf_refine* crit, int level, int maxlevel,
                        int init_flag, const int* patches, int n, int* tags)
-{
+try {
     return tag_impl(p, false, buf, t, crit, level, maxlevel, init_flag, patches, n, tags);
-}
+} GF_CATCH_STATUS
 
 int gf_pool_tag_coarsen(gf_pool* p, int buf, double t, const gf_refine* crit, int level, int maxlevel,
                         int init_flag, const int* families, int n, int* tags)
-{
+try {
     return tag_impl(p, true, buf, t, crit, level, maxlevel, init_flag, families, n, tags);
-}
+} GF_CATCH_STATUS
 
 int gf_pool_variants(gf_pool* p, int* current, int* count, int* desc)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     const int n = gf::num_variants(p->mx, p->my);
     if (current) *current = p->variant;
@@ -1437,10 +1451,10 @@ int gf_pool_variants(gf_pool* p, int* current, int* count, int* desc)
             desc[3 * k + 2] = v.nt;
         }
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_pool_set_variant(gf_pool* p, int variant)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (variant < 0 || variant >= gf::num_variants(p->mx, p->my))
         return fail(GF_EINVAL, "variant %d outside [0,%d)", variant, gf::num_variants(p->mx, p->my));
@@ -1450,10 +1464,10 @@ int gf_pool_set_variant(gf_pool* p, int variant)
         p->graph_dirty = true;
     }
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_pool_autotune(gf_pool* p, int reps, int* best, double* ms)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (reps < 1) return fail(GF_EINVAL, "reps must be >= 1");
     if (p->nscen == 0 || p->n == 0) return fail(GF_ESTATE, "no scenario registered");
@@ -1499,15 +1513,15 @@ int gf_pool_autotune(gf_pool* p, int reps, int* best, double* ms)
     p->wet_grid = 0;
     if (best) *best = best_v;
     return gf_pool_set_variant(p, best_v);
-}
+} GF_CATCH_STATUS
 
 int gf_last_timing(double* total_ms, double* step_ms, int* step_launches)
-{
+try {
     if (total_ms) *total_ms = g.last_total_ms;
     if (step_ms) *step_ms = g.last_step_ms;
     if (step_launches) *step_launches = g.last_launches;
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 // ---- output: ASCII frames and gauges ----------------------------------------
 
@@ -1533,7 +1547,7 @@ static std::string join_path(const char* dir, const char* name)
 int gf_pool_write_frame(gf_pool* p, int buf, int first, int n, const char* dir, int frame, double time,
                         int ngrids, int write_header, const int* patch_num, const int* level, const int* blockno,
                         int mpirank)
-{
+try {
     if (!p || !dir) return fail(GF_EINVAL, "pool or dir is NULL");
     if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
     if (first < 0 || n < 0 || first + n > p->capacity) return fail(GF_EINVAL, "slots [%d,%d) outside the pool", first, first + n);
@@ -1593,11 +1607,11 @@ int gf_pool_write_frame(gf_pool* p, int buf, int first, int n, const char* dir, 
     if (int r = reset_dry_age(p, first, n)) return r;
     HIPCHK(hipStreamSynchronize(g.stream));
     return rc;
-}
+} GF_CATCH_STATUS
 
 int gf_pool_sample_gauges(gf_pool* p, int buf, int n, const int* slot, const double* xc, const double* yc,
                           double* out)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
     if (n < 0) return fail(GF_EINVAL, "n must be >= 0");
@@ -1635,7 +1649,7 @@ int gf_pool_sample_gauges(gf_pool* p, int buf, int n, const int* slot, const dou
     HIPCHK(hipMemcpyAsync(out, d + o_out, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 // ---- regrid and time interpolation between levels ----------------------------
 
@@ -1712,14 +1726,14 @@ static int regrid_impl(gf_pool* p, int buf, int n, const int* cs, const int* fs,
 }
 
 int gf_pool_average2coarse(gf_pool* p, int buf, int n, const int* coarse, const int* fine)
-{
+try {
     return regrid_impl(p, buf, n, coarse, fine, false);
-}
+} GF_CATCH_STATUS
 
 int gf_pool_interpolate2fine(gf_pool* p, int buf, int n, const int* coarse, const int* fine)
-{
+try {
     return regrid_impl(p, buf, n, coarse, fine, true);
-}
+} GF_CATCH_STATUS
 
 // Links of a coarse/fine ghost transfer: slots described and distinct per
 // link, side 0..7, igrid 0/1 on faces; each written region named once (the
@@ -1799,18 +1813,18 @@ static int cf_ghost_impl(gf_pool* p, int buf, int n, const int* cs, const int* f
 
 int gf_pool_average_ghost(gf_pool* p, int buf, int n, const int* coarse, const int* fine, const int* side,
                           const int* igrid)
-{
+try {
     return cf_ghost_impl(p, buf, n, coarse, fine, side, igrid, false);
-}
+} GF_CATCH_STATUS
 
 int gf_pool_interpolate_ghost(gf_pool* p, int buf, int n, const int* coarse, const int* fine, const int* side,
                               const int* igrid)
-{
+try {
     return cf_ghost_impl(p, buf, n, coarse, fine, side, igrid, true);
-}
+} GF_CATCH_STATUS
 
 int gf_pool_copy(gf_pool* p, int n, const int* slots, int src_buf, int dst_buf)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if ((src_buf != 0 && src_buf != 1) || (dst_buf != 0 && dst_buf != 1) || src_buf == dst_buf)
         return fail(GF_EINVAL, "buffers must be 0 and 1, in either order");
@@ -1828,11 +1842,11 @@ int gf_pool_copy(gf_pool* p, int n, const int* slots, int src_buf, int dst_buf)
                                 d, n, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_pool_timeinterp(gf_pool* p, int n, const int* src, int buf_last, int buf_curr, double alpha,
                        const int* dst, int dst_buf)
-{
+try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
     if ((buf_last != 0 && buf_last != 1) || (buf_curr != 0 && buf_curr != 1) || (dst_buf != 0 && dst_buf != 1))
         return fail(GF_EINVAL, "buffers must be 0 or 1");
@@ -1867,10 +1881,10 @@ int gf_pool_timeinterp(gf_pool* p, int n, const int* src, int buf_last, int buf_
                                 ddst, n, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_gauge_file_create(const char* dir, int id, double xc, double yc)
-{
+try {
     if (!dir) return fail(GF_EINVAL, "dir is NULL");
     char name[32];
     std::snprintf(name, sizeof name, "gauge%05d.txt", id);
@@ -1881,10 +1895,10 @@ int gf_gauge_file_create(const char* dir, int id, double xc, double yc)
     const bool ok = std::fwrite(s.data(), 1, s.size(), f) == s.size();
     if (std::fclose(f) != 0 || !ok) return fail(GF_EIO, "cannot write %s", name);
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 int gf_gauge_file_append(const char* dir, int id, int n, const int* level, const double* t, const double* rec)
-{
+try {
     if (!dir) return fail(GF_EINVAL, "dir is NULL");
     if (n < 0 || (n > 0 && (!level || !t || !rec))) return fail(GF_EINVAL, "invalid records");
     char name[32];
@@ -1897,6 +1911,6 @@ int gf_gauge_file_append(const char* dir, int id, int n, const int* level, const
     const bool ok = std::fwrite(s.data(), 1, s.size(), f) == s.size();
     if (std::fclose(f) != 0 || !ok) return fail(GF_EIO, "cannot write %s", name);
     return GF_OK;
-}
+} GF_CATCH_STATUS
 
 }  // extern "C"

@@ -36,10 +36,15 @@ gf_plugin* gf_plugin_create(int mx, int my, int mbc, int maux, int buffer_len)
     p->mbc = mbc;
     p->maux = maux;
     p->buffer_len = buffer_len;
-    p->q.reserve(buffer_len);
-    p->aux.reserve(buffer_len);
-    p->dx.reserve(buffer_len);
-    p->dy.reserve(buffer_len);
+    try {  // no exception crosses the C ABI
+        p->q.reserve(buffer_len);
+        p->aux.reserve(buffer_len);
+        p->dx.reserve(buffer_len);
+        p->dy.reserve(buffer_len);
+    } catch (...) {
+        delete p;
+        return nullptr;
+    }
     return p;
 }
 
@@ -51,7 +56,7 @@ int gf_plugin_destroy(gf_plugin* p)
 
 int gf_plugin_single_step_update(gf_plugin* p, double* q, double* aux, double dx, double dy,
                                  int iter, int total_count, double t, double dt, double* maxcfl)
-{
+try {
     if (!p || !q || !aux || !maxcfl) return GF_EINVAL;
     if (iter < 0 || total_count <= 0 || iter >= total_count) return GF_EINVAL;
     *maxcfl = 0.0;
@@ -76,6 +81,8 @@ int gf_plugin_single_step_update(gf_plugin* p, double* q, double* aux, double dx
     p->dx.clear();
     p->dy.clear();
     return rc;
+} catch (...) {  // a queue that cannot grow: no exception crosses the C ABI
+    return GF_ENOMEM;
 }
 
 }  // extern "C"

@@ -12,7 +12,8 @@
  *   - Every entry point returns an int status (GF_OK = 0, negative on error)
  *     instead of calling exit() as the reference's CHECK macro does
  *     (src/fc2d_cpu_cuda/fc2d_cudaclaw_check.h:3-11); gf_last_error() gives
- *     the message of the last failure on the calling thread.
+ *     the message of the last failure on the calling thread.  No C++
+ *     exception leaves the library.
  *   - Host patch arrays use the reference's layout: q(meqn=3, 1-mbc:mx+mbc,
  *     1-mbc:my+mbc) and aux(maux, ...) in Fortran order, ghost cells included.
  *   - Only mbc = 2, meqn = 3, mwaves = 3, mcapa = 0 and Cartesian coordinates
@@ -35,9 +36,9 @@ enum {
     GF_OK = 0,
     GF_EINVAL = -1,       /* invalid argument */
     GF_EHIP = -2,         /* HIP runtime error */
-    GF_ENOMEM = -3,       /* device or pinned host allocation failed */
+    GF_ENOMEM = -3,       /* device, pinned or host memory allocation failed */
     GF_EUNSUPPORTED = -4, /* configuration outside the supported set */
-    GF_ESTATE = -5,       /* call out of order (e.g. no gf_init) */
+    GF_ESTATE = -5,       /* call out of order (e.g. no gf_init), or another host failure */
     GF_EIO = -6           /* an output file could not be written */
 };
 
