@@ -29,42 +29,57 @@ DXC, DXF = 1.0 / (NC * MX), 0.5 / (NC * MX)
 DIRS = [(-1, 0), (1, 0), (0, -1), (0, 1), (-1, -1), (1, -1), (-1, 1), (1, 1)]
 
 
-def leaves(refined):
-    """(level, i, j) of every leaf: coarse (i, j) on the 4x4 lattice, fine
-    (i, j) on the 8x8 lattice."""
-    out = [(0, i, j) for j in range(NC) for i in range(NC) if (i, j) not in refined]
-    out += [(1, 2 * pi + ci, 2 * pj + cj) for (pi, pj) in sorted(refined, key=lambda t: (t[1], t[0]))
-            for cj in range(2) for ci in range(2)]
+def leaves_multi(blocks):
+    """Leaves of a nested layout: blocks[l] is the set of level-l lattice
+    patches (i, j) that are refined (each block inside the previous one's
+    children, 2:1 balanced).  Returns (level, i, j) leaves, coarse first."""
+    out = []
+    region = {(i, j) for j in range(NC) for i in range(NC)}
+    for lev in range(len(blocks) + 1):
+        ref = blocks[lev] if lev < len(blocks) else set()
+        out += [(lev, i, j) for (i, j) in sorted(region, key=lambda t: (t[1], t[0])) if (i, j) not in ref]
+        region = {(2 * i + ci, 2 * j + cj) for (i, j) in ref for cj in range(2) for ci in range(2)}
     return out
 
 
-def build_links(lv, refined):
-    """Neighbour table (GF_NB_* or slot) and the coarse/fine links
-    (coarse slot, fine slot, side, igrid)."""
+def build_links_multi(lv):
+    """Neighbour table and coarse/fine links for leaves of any number of
+    levels: a side whose same-level region is a leaf copies it, one covered by
+    a coarser leaf or refined further is stored; links join each leaf to the
+    finer leaves across its sides (coarse slot, fine slot, side, igrid)."""
     slot = {k: s for s, k in enumerate(lv)}
     nbr = np.full((len(lv), 8), GF_NB_WALL, dtype=np.int32)
     links = []
     for s, (lev, i, j) in enumerate(lv):
+        n = NC << lev
         for d, (dx, dy) in enumerate(DIRS):
             a, b = i + dx, j + dy
-            n = NC * (1 + lev)
             if not (0 <= a < n and 0 <= b < n):
-                continue                                  # wall
+                continue
             if (lev, a, b) in slot:
                 nbr[s, d] = slot[(lev, a, b)]
                 continue
             nbr[s, d] = GF_NB_STORED
-            if lev == 0:                                  # refined region: fine children across d
-                kids = [(ci, cj) for cj in range(2) for ci in range(2)
-                        if (dx == 0 or ci == (0 if dx > 0 else 1)) and (dy == 0 or cj == (0 if dy > 0 else 1))]
-                for ci, cj in kids:
-                    f = slot[(1, 2 * a + ci, 2 * b + cj)]
-                    if d < 4:
-                        side, ig = d, (cj if d < 2 else ci)
-                    else:
-                        side, ig = d, 0                   # corners 4..7 in the same order
-                    links.append((s, f, side, ig))
+            kids = [(ci, cj) for cj in range(2) for ci in range(2)
+                    if (dx == 0 or ci == (0 if dx > 0 else 1)) and (dy == 0 or cj == (0 if dy > 0 else 1))]
+            for ci, cj in kids:
+                f = slot.get((lev + 1, 2 * a + ci, 2 * b + cj))
+                if f is not None:
+                    links.append((s, f, d, (cj if d < 2 else ci) if d < 4 else 0))
     return nbr, links
+
+
+def leaves(refined):
+    """(level, i, j) of every leaf with the coarse patches in `refined` split
+    once: coarse (i, j) on the 4x4 lattice, fine (i, j) on the 8x8 lattice,
+    coarse first."""
+    return leaves_multi([refined] if refined else [])
+
+
+def build_links(lv, refined):
+    """Neighbour table (GF_NB_* or slot) and the coarse/fine links
+    (coarse slot, fine slot, side, igrid) of a two-level layout."""
+    return build_links_multi(lv)
 
 
 def field(x, y):
@@ -353,46 +368,6 @@ def test_regrid_cycle_matches_oracle(gf, oracle_c):
         assert_state_close(g, r, 1e-10)
     finally:
         pool.close()
-
-
-def leaves_multi(blocks):
-    """Leaves of a nested layout: blocks[l] is the set of level-l lattice
-    patches (i, j) that are refined (each block inside the previous one's
-    children, 2:1 balanced).  Returns (level, i, j) leaves, coarse first."""
-    out = []
-    region = {(i, j) for j in range(NC) for i in range(NC)}
-    for lev in range(len(blocks) + 1):
-        ref = blocks[lev] if lev < len(blocks) else set()
-        out += [(lev, i, j) for (i, j) in sorted(region, key=lambda t: (t[1], t[0])) if (i, j) not in ref]
-        region = {(2 * i + ci, 2 * j + cj) for (i, j) in ref for cj in range(2) for ci in range(2)}
-    return out
-
-
-def build_links_multi(lv):
-    """Neighbour table and coarse/fine links for leaves of any number of
-    levels: a side whose same-level region is a leaf copies it, one covered by
-    a coarser leaf or refined further is stored; links join each leaf to the
-    finer leaves across its sides (coarse slot, fine slot, side, igrid)."""
-    slot = {k: s for s, k in enumerate(lv)}
-    nbr = np.full((len(lv), 8), GF_NB_WALL, dtype=np.int32)
-    links = []
-    for s, (lev, i, j) in enumerate(lv):
-        n = NC << lev
-        for d, (dx, dy) in enumerate(DIRS):
-            a, b = i + dx, j + dy
-            if not (0 <= a < n and 0 <= b < n):
-                continue
-            if (lev, a, b) in slot:
-                nbr[s, d] = slot[(lev, a, b)]
-                continue
-            nbr[s, d] = GF_NB_STORED
-            kids = [(ci, cj) for cj in range(2) for ci in range(2)
-                    if (dx == 0 or ci == (0 if dx > 0 else 1)) and (dy == 0 or cj == (0 if dy > 0 else 1))]
-            for ci, cj in kids:
-                f = slot.get((lev + 1, 2 * a + ci, 2 * b + cj))
-                if f is not None:
-                    links.append((s, f, d, (cj if d < 2 else ci) if d < 4 else 0))
-    return nbr, links
 
 
 def test_three_levels_match_oracle(gf, oracle_c):
