@@ -213,3 +213,29 @@ def test_cf_ghost_link_checks(gf):
         pool.interpolate_ghost([], [], [], [])
     finally:
         pool.close()
+
+
+def test_pool_copy_between_buffers(gf):
+    """gf_pool_copy: the listed slots' states (ghost cells included) are
+    copied exactly from one buffer to the other; other slots and the source
+    are untouched; bad arguments are refused."""
+    rng = np.random.default_rng(21)
+    pool = Pool(16, 16, 6)
+    try:
+        pool.set_patches(np.ones(6), np.ones(6), np.zeros(6), np.zeros(6))
+        a = rng.normal(size=(6, 20, 20, 3))
+        b = rng.normal(size=(6, 20, 20, 3))
+        pool.upload_aos(0, a, None, buf=0)
+        pool.upload_aos(0, b, None, buf=1)
+        pool.copy([4, 1], 0, 1)
+        got0, got1 = pool.download_aos(0, 6, buf=0), pool.download_aos(0, 6, buf=1)
+        assert np.array_equal(got0, a)
+        want = b.copy()
+        want[[1, 4]] = a[[1, 4]]
+        assert np.array_equal(got1, want)
+        for slots, s, d in (([0], 0, 0), ([0], 0, 2), ([6], 0, 1), ([-1], 1, 0)):
+            with pytest.raises(gf.GeoFloodError):
+                pool.copy(slots, s, d)
+        pool.copy([], 1, 0)
+    finally:
+        pool.close()
