@@ -15,9 +15,13 @@ cell-update is one interior cell advanced by one accepted step (retaken
 attempts do not count).
 
 Runs, all from the same initial state, inputs resident in HBM:
-  batched  -- K attempts, each one hipGraph launch (the reported value);
-  events   -- the same K attempts launched without a graph, with HIP events
-              around every fused-step launch (per-kernel time -> roofline);
+  batched  -- W warm-up attempts (graph capture, autotuning of the step
+              variant on the population they reach), then K timed attempts
+              that continue the same trajectory, each one hipGraph launch
+              (the reported value);
+  events   -- the same W + K attempts, the K launched without a graph, with
+              HIP events around every fused-step launch (per-kernel time ->
+              roofline);
   plain    -- the first accepted steps with one kernel launch per patch from
               a host loop and the dt rule on the host;
   cpu      -- the reference GeoClaw Fortran step (oracle/_ref) on the host
@@ -25,7 +29,8 @@ Runs, all from the same initial state, inputs resident in HBM:
               reported cpu_baseline), then the C restatement on the same
               sample (cpu_port); the plain run's state is compared with the
               CPU state cell by cell (parity_vs_cpu);
-  and, for context: an ensemble-serving sample, the host-resident drop-in
+  and, for context: the ensemble-serving stream (2,000 Poisson requests at
+  32/s by default), the host-resident drop-in
   batch (PCIe included), ASCII frame and gauge output, and the regrid and
   coarse/fine ghost transfers.
 
@@ -133,6 +138,62 @@ def cpu_baseline(state, nsteps, nthreads, port_only=False):
     return dict(value=nsteps * N * N / el, seconds=el, kind=kind, rec=rec, state=br.to_global())
 
 
+def amr_overland(steps, maxlevel=4):
+    """The AMR overland flood: 2 km square, 16 x 16 patches at levels 1-4
+    (level L: 4 * 2^L patches a side, 1.95 m cells at level 4), Gaussian-sum
+    DEM (seed 0), west hydrograph Q = 200 sin(pi t / 3600) m^3/s through a
+    100 m strip, east extrapolation, north/south walls, Manning n 0.035,
+    tagged on h > dry_tolerance or |grad eta| > 0.01, regrid every 4 steps,
+    dry at t = 0 (dt bound 10 s).  Every attempt is one graph replay (ghost
+    update across levels, one persistent step of all levels, dt control);
+    the whole loop, regrids included, is timed."""
+    import geoflood_amd
+    from geoflood_amd.amr import AmrRun
+    from geoflood_amd.scenarios import gaussian_sum_dem, sine_hydrograph
+    geoflood_amd.set_params(**dict(PARAMS, manning_n=0.035))
+    t, hu = sine_hydrograph(200.0, 100.0)
+    hydro = dict(time=t, hu=hu, b=100.0, x0=0.0, y0=1000.0)
+    run = AmrRun(16, 4, 4, 2000.0, 2000.0, 1, maxlevel, lambda nx, ny, L: gaussian_sum_dem(nx, 2000.0, seed=0),
+                 4 ** maxlevel * 16, mthbc=(0, 1, 3, 3), hydro=hydro, eta_tol=0.01, regrid_interval=4, dt_max=10.0)
+    try:
+        t0 = time.perf_counter()
+        run.start(0.0, 1.0)
+        t_init = time.perf_counter() - t0
+        counts0 = run.forest.counts()
+        log(f"amr: initial forest {len(run.forest)} leaves {counts0} in {t_init:.2f} s; {steps} steps ...")
+        run.run(8)  # warm-up: graph capture, first regrids
+        c0, s0 = run.stats["cell_updates"], run.stats["steps"]
+        p0, r0 = len(run.stats["patches"]), run.stats["regrids"]
+        st0 = run.pool.stats()
+        t0 = time.perf_counter()
+        info = run.run(steps)
+        wall = time.perf_counter() - t0
+        st1 = run.pool.stats()
+        cu = run.stats["cell_updates"] - c0
+        pts = run.stats["patches"][p0:]
+        q, _ = run.leaves_state()
+        wet = int((q[:, 2:-2, 2:-2, 0] > 1e-3).sum())
+        out = dict(value=cu / wall, unit="cell-updates/s", steps=run.stats["steps"] - s0,
+                   attempts=info["attempts"], retakes=info["retakes"], wall_s=round(wall, 3),
+                   simulated_time_s=round(info["t"], 3),
+                   regrids=run.stats["regrids"] - r0, ms_per_step=round(1e3 * wall / max(run.stats["steps"] - s0, 1), 4),
+                   patches=dict(min=int(min(pts)), mean=round(float(np.mean(pts)), 1), max=int(max(pts)),
+                                final_by_level=run.forest.counts(), initial_by_level=counts0),
+                   active_patches_per_attempt=round((st1["active_patches"] - st0["active_patches"]) /
+                                                    max(st1["attempts"] - st0["attempts"], 1), 1),
+                   wet_cells_final=wet,
+                   note="whole loop timed: graph replays (multi-level ghost update, one persistent step of "
+                        "every level, device dt control) and every regrid (device tags, forest adapt on the host, "
+                        "slot moves, interpolate2fine, average2coarse, scenario re-targeting)")
+        log(f"amr: {out['steps']} steps in {wall:.2f} s ({out['ms_per_step']} ms/step), "
+            f"{cu / wall / 1e9:.3f} G cell-updates/s, patches {out['patches']['min']}-{out['patches']['max']}, "
+            f"{out['regrids']} regrids, t = {info['t']:.1f} s")
+        return out
+    finally:
+        run.close()
+        geoflood_amd.set_params(**PARAMS)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,10 +214,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=24,
                     help="accepted steps of the CPU baseline sample (about 10 s on 16 host cores)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--ensemble-requests", type=int, default=64)
+    ap.add_argument("--ensemble-requests", type=int, default=2000)
     ap.add_argument("--ensemble-rate", type=float, default=32.0)
     ap.add_argument("--ensemble-capacity", type=int, default=65536)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--amr-steps", type=int, default=2000,
+                    help="accepted steps of the AMR overland flood (levels 1-4, regrid every 4; 0: skip)")
     ap.add_argument("--variant", type=int, default=-1,
                     help="use this step variant instead of the autotuner's choice (counter runs, where "
                          "the profiler's serialisation makes the autotuner's timings unreliable)")
@@ -198,16 +261,16 @@ def main():
     cells = N * N
 
     # ---- warm-up (includes graph capture), then the timed batched run ----
-    # The step variant (tile shape, threads per workgroup) is chosen by the
-    # autotuner on the population after the warm-up attempts (the first two
-    # attempts also step every dry patch, before those become steady), and
-    # chosen again inside the timed run whenever the active population has
-    # changed by more than RETUNE_FACTOR.  The autotuner does not advance the
-    # state; the timed run starts again from the initial state.
-    reset(pool, layout, state, 0)
+    # The warm-up attempts are the first W attempts of the workload (the
+    # first two also step every dry patch, before those become steady); the
+    # step variant (tile shape, threads per workgroup) is then chosen by the
+    # autotuner on the population they reached (it does not advance the
+    # state), and chosen again inside the timed run whenever the active
+    # population has changed by more than RETUNE_FACTOR.  The K timed
+    # attempts continue the same trajectory.
+    sid = reset(pool, layout, state, 0)
     pool.run(args.warmup, graph=True)
     best, tune_ms = pool.autotune(reps=3)
-    sid = reset(pool, layout, state, 0)
     _, variants = pool.variants()
     if args.variant >= 0:
         best = args.variant
@@ -215,7 +278,9 @@ def main():
     log(f"autotune: variants {variants}, k_all ms {['%.4f' % m for m in tune_ms]}, using {best}")
     pool.run(0, graph=True)  # capture outside the timed region
     geoflood_amd.load().gf_synchronize()
-    log(f"timed batched run: {args.steps} attempts ...")
+    info_w = pool.scenario(sid)
+    log(f"timed batched run: {args.steps} attempts after {args.warmup} warm-up attempts "
+        f"(t = {info_w['t']:.4f} s) ...")
     retunes = []
     t0 = time.perf_counter()
     done, last_act, st_prev = 0, None, pool.stats()
@@ -242,7 +307,7 @@ def main():
         st_end["attempts"] - st_start["attempts"], 1)
     dry_per_attempt = (st_end["dry_patches"] - st_start["dry_patches"]) / max(
         st_end["attempts"] - st_start["attempts"], 1)
-    accepted = info_b["nsteps"]
+    accepted = info_b["nsteps"] - info_w["nsteps"]
     value = accepted * cells / wall
     ms_per_step = 1e3 * wall / args.steps
     log(f"batched: {active_per_attempt:.1f} active + {dry_per_attempt:.1f} dry patches per attempt")
@@ -251,14 +316,17 @@ def main():
 
     # ---- same attempts without a graph, HIP events around each launch ----
     # The dominant kernel is k_all, the persistent step over the attempt's
-    # active tiles and then its dry patches: its algorithmic bytes per launch
-    # are 56 B x 1024 cells x the patches it advances (all 16,384; steady dry
-    # patches, whose step leaves both buffers as they are, count as advanced
-    # but are not read or written).
+    # active tiles and then its non-steady dry patches.  Its algorithmic bytes
+    # per launch are 56 B x MX*MX cells x the patches it reads and writes
+    # (active + dry - steady): steady dry patches, whose step leaves both
+    # buffers as they are, count as advanced in the cell-update rate but are
+    # neither read nor written, so they are left out of the bandwidth figure
+    # (the all-cells figure is reported separately as "effective").
     roofline = None
     composite = None
     if not args.no_events:
         sid = reset(pool, layout, state, 0)
+        pool.run(args.warmup, graph=True)
         s0 = pool.stats()
         t0e = time.perf_counter()
         pool.run(args.steps, graph=False)
@@ -271,8 +339,11 @@ def main():
         steady = s1["steady_patches"] - s0["steady_patches"]
         launches = max(launches, 1)
         k_avg_ms = step_ms / launches
-        k_bytes = BYTES_PER_CELL_UPDATE * MX * MX * (act + dry) / launches
+        touched = (act + dry - steady) / launches
+        k_bytes = BYTES_PER_CELL_UPDATE * MX * MX * touched
         achieved = k_bytes / (k_avg_ms * 1e-3) / 1e9
+        eff_bytes = BYTES_PER_CELL_UPDATE * MX * MX * (act + dry) / launches
+        effective = eff_bytes / (k_avg_ms * 1e-3) / 1e9
         # measured HBM bytes and VALU work per launch of this same trajectory
         # (rocprofv3 --pmc over the 500 event-timed launches, profiles/)
         traffic = valu = None
@@ -280,27 +351,42 @@ def main():
         if "hbm_bytes_per_launch" in kv:
             traffic = round(float(kv["hbm_bytes_per_launch"]))
         kv = profile_values("k_all_32x32_uniform4096_valu.txt") if MX == 32 else {}
+        valu_issue_frac = fp64_frac = None
         if "fp64_flops_per_launch" in kv:
             fl = float(kv["fp64_flops_per_launch"]) / (k_avg_ms * 1e-3) / 1e12
             li = float(kv["valu_lane_instr_per_launch"]) / (k_avg_ms * 1e-3) / 1e12
+            valu_issue_frac, fp64_frac = round(li / VALU_LANE_PEAK_T, 4), round(fl / FP64_PEAK_TFLOPS, 4)
             valu = dict(fp64_tflops=round(fl, 3), fp64_peak_tflops=FP64_PEAK_TFLOPS,
-                        fp64_frac=round(fl / FP64_PEAK_TFLOPS, 4), valu_issue_frac=round(li / VALU_LANE_PEAK_T, 4),
+                        fp64_frac=fp64_frac, valu_lane_tinstr_per_s=round(li, 3),
+                        valu_lane_peak_tinstr_per_s=VALU_LANE_PEAK_T, valu_issue_frac=valu_issue_frac,
                         note="FP64 FLOPs (FMA = 2) and VALU lane-instructions per launch from the "
                              "profiles/k_all_32x32_uniform4096_valu.txt counters, over this run's k_all time")
-        roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+        # The active tiles are bound by the FP64 VALU (two augmented Riemann
+        # solves with ~17 correctly rounded divisions and ~8 square roots per
+        # face): "bound" names that; achieved / frac are the HBM figures of the
+        # bytes the launch really reads and writes.
+        roofline = dict(bound="valu", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                        valu_issue_frac=valu_issue_frac, fp64_frac=fp64_frac,
                         kernel="k_all<%d,%d,%d,%d,%d>" % ((MX, MX) + tuple(variants[pool.variants()[0]])),
                         kernel_avg_ms=round(k_avg_ms, 5), bytes_per_launch=round(k_bytes),
+                        touched_patches_per_launch=round(touched, 1),
                         active_patches_per_launch=round(act / launches, 1),
                         dry_patches_per_launch=round(dry / launches, 1),
-                        steady_patches_per_launch=round(steady / launches, 1), launches=launches, valu=valu)
+                        steady_patches_per_launch=round(steady / launches, 1), launches=launches,
+                        traffic_over_algorithmic=round(traffic / k_bytes, 3) if traffic else None,
+                        effective=dict(achieved=round(effective, 2), frac=round(effective / HBM_PEAK_GBS, 4),
+                                       bytes_per_launch=round(eff_bytes),
+                                       note="56 B for every cell the launch advances, steady dry patches "
+                                            "(not read or written) included"),
+                        valu=valu)
         composite = dict(attempt_avg_ms_without_graph=round(1e3 * ev_wall / launches, 5),
                          note="inflow (when present) -> classify -> k_all -> control per attempt; "
                               "the timed value uses one hipGraph replay per attempt")
         same = (info_e["nsteps"] == info_b["nsteps"] and info_e["t"] == info_b["t"])
         log(f"events run: k_all avg {k_avg_ms:.4f} ms ({act / launches:.0f} active + {dry / launches:.0f} dry "
-            f"patches, {steady / launches:.0f} of them steady), {achieved:.1f} GB/s algorithmic; "
-            f"same trajectory as graph run: {same}")
+            f"patches, {steady / launches:.0f} of them steady), {achieved:.1f} GB/s of the touched patches' "
+            f"algorithmic bytes ({effective:.1f} GB/s all cells); same trajectory as graph run: {same}")
 
     # ---- output: one ASCII frame of 1,024 patches and 4,096 gauge samples ----
     # of the final state (fort.qNNNN bytes in the reference's format, written
@@ -350,9 +436,10 @@ def main():
         ensemble = summarize(res, eng.wall, eng.cell_updates)
         ensemble.update(rate_per_s=args.ensemble_rate, t_end_s=1800.0, capacity_patches=args.ensemble_capacity,
                         hbm_budget_bytes=_P.bytes_for(32, 32, args.ensemble_capacity),
-                        sample=f"first {len(reqs)} requests of the Poisson stream (seed 0); domains 128^2-1024^2 "
+                        sample=f"{len(reqs)} requests of the Poisson stream (seed 0); domains 128^2-1024^2 "
                                "of 32x32 patches over 2 km, Gaussian-sum DEM (seed = request id), Manning n "
-                               "U[0.02,0.06], inflow peak log-U[20,2000] m^3/s, simulated 10 s -> 1800 s")
+                               "U[0.02,0.06], inflow peak log-U[20,2000] m^3/s, dry at t = 0 -> 1800 s "
+                               "(dt bound 10 s)")
         eng.close()
         geoflood_amd.set_params(**PARAMS)
         log(f"ensemble: p50 {ensemble['p50_latency_s']} s, p99 {ensemble['p99_latency_s']} s, "
@@ -458,6 +545,11 @@ def main():
         log(f"regrid: {F} families, interpolate2fine {1e3 * ti:.3f} ms, average2coarse {1e3 * ta:.3f} ms; "
             f"{len(lc)} coarse/fine ghost links, average {1e3 * tag:.3f} ms, interpolate {1e3 * tig:.3f} ms")
 
+    # ---- AMR overland flood on resident patches ----
+    amr = None
+    if args.amr_steps > 0:
+        amr = amr_overland(args.amr_steps)
+
     # ---- CPU baseline (reference Fortran step on the host cores) ----
     cpu = None
     cpu_port = None
@@ -472,6 +564,10 @@ def main():
                           f"({args.cpu_steps * cells} cell-updates, {cb['seconds']:.1f} s)")
         if gpu_first is not None and args.cpu_steps == args.plain_steps:
             parity = state_parity(gpu_first, cb["state"])
+            parity.update(steps=args.cpu_steps, bar=1e-10,
+                          metric="wet cells (h > drytol in both): |dh| / max|h|, and |d(hu)|, |d(hv)| relative to "
+                                 "max(|hu|, h sqrt(g h)) of the cell; dry cells: max |d(h, hu, hv)| / drytol; "
+                                 "plus identical wet/dry masks")
         log(f"cpu: {cb['value'] / 1e6:.2f} M cell-updates/s ({cb['kind']}); parity {parity}")
         if cb["kind"] == "reference":
             # the C restatement on the same sample and threads, for comparison
@@ -503,7 +599,8 @@ def main():
             "dt_control": f"initial dt {DT0} s, desired CFL {DESIRED_CFL}, max CFL {MAX_CFL}, on device",
             "mode": "batched: one hipGraph replay per attempt (classify, fused step with halo pull, dt control)",
             "accepted_steps": accepted,
-            "simulated_time_s": info_b["t"],
+            "warmup_accepted_steps": info_w["nsteps"],
+            "simulated_time_s": [info_w["t"], info_b["t"]],
             "step_variants": [dict(tile=[v[0], v[1]], threads=v[2]) for v in variants],
             "autotune_initial": dict(variant=best, k_all_ms=[round(m, 5) for m in tune_ms]),
             "autotune_retunes": retunes,
@@ -522,6 +619,7 @@ def main():
         "cpu_host": cpu_host,
         "parity_vs_cpu": parity,
         "ensemble": ensemble,
+        "amr_overland": amr,
     }
     print(json.dumps(out))
 

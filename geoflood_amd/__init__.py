@@ -32,7 +32,7 @@ class GfParams(ct.Structure):
 class GfScenarioInfo(ct.Structure):
     _fields_ = [("t", ct.c_double), ("dt", ct.c_double), ("maxcfl", ct.c_double),
                 ("last_dt", ct.c_double), ("nsteps", ct.c_int), ("retakes", ct.c_int),
-                ("attempts", ct.c_int), ("active", ct.c_int), ("cur", ct.c_int)]
+                ("attempts", ct.c_int), ("active", ct.c_int), ("cur", ct.c_int), ("status", ct.c_int)]
 
 
 class GfRefine(ct.Structure):
@@ -82,6 +82,8 @@ PROTOTYPES = {
     "gf_pool_set_scenario_friction": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_double]),
     "gf_pool_set_scenario_active": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int]),
     "gf_pool_get_scenario": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(GfScenarioInfo)]),
+    "gf_pool_set_scenario_dt_max": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_double]),
+    "gf_pool_set_scenario_stop": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double]),
     "gf_pool_get_scenarios": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.POINTER(GfScenarioInfo)]),
     "gf_pool_run": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip]),
     "gf_pool_run_plain": (ct.c_int, [ct.c_void_p, ct.c_int, _ip]),
@@ -106,6 +108,12 @@ PROTOTYPES = {
     "gf_pool_interpolate_ghost": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip, _ip, _ip, _ip]),
     "gf_pool_timeinterp": (ct.c_int, [ct.c_void_p, ct.c_int, _ip, ct.c_int, ct.c_int, ct.c_double, _ip,
                                       ct.c_int]),
+    "gf_pool_set_levels": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip]),
+    "gf_pool_set_cf_links": (ct.c_int, [ct.c_void_p, ct.c_int, _ip, _ip, _ip, _ip]),
+    "gf_pool_ghost_update": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_double]),
+    "gf_pool_move": (ct.c_int, [ct.c_void_p, ct.c_int, _ip, _ip, ct.c_int]),
+    "gf_pool_set_scenario_slots": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int]),
+    "gf_pool_tag_eta_gradient": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_double, _ip, ct.c_int, _ip]),
     "gf_gauge_file_create": (ct.c_int, [ct.c_char_p, ct.c_int, ct.c_double, ct.c_double]),
     "gf_gauge_file_append": (ct.c_int, [ct.c_char_p, ct.c_int, ct.c_int, _ip, _dp, _dp]),
     "gf_plugin_create": (ct.c_void_p, [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int]),

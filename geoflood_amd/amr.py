@@ -1,0 +1,426 @@
+"""Resident AMR on one device pool.
+
+ForestClaw owns GeoFlood's forest: p4est refines and coarsens quadtree leaves
+at every regrid (fc2d_geoclaw_run.c:569-575, fclaw2d_regrid), and the solver
+plugin's hooks tag, interpolate and average patches (fc2d_cpucuda.cpp:
+513-694).  For runs without ForestClaw -- the overland-flood and autotuner
+workloads of this package, its tests and its benchmark -- this module keeps a
+minimal stand-in for that forest and drives the library's device entry points
+with it, so every leaf stays resident in the pool from the first step to the
+last:
+
+* ``Forest``: the leaves of a quadtree over a brick of level-0 patches,
+  levels minlevel..maxlevel, 2:1 balanced across faces and corners (p4est's
+  full balance).  It is stored as the level of the covering leaf of every
+  patch of the finest lattice, so refinement, coarsening of whole families
+  and the balance closure are a few array operations per regrid.
+* ``AmrRun``: one scenario whose leaves occupy a contiguous range of pool
+  slots.  Every attempt is one replay of the pool's graph (multi-level ghost
+  update, one persistent step of every leaf with one dt -- ForestClaw's
+  subcycle = F, the setting of every GeoFlood example -- and the device dt
+  control).  Every ``regrid_interval`` accepted steps the leaves are tagged
+  on the device (gf_pool_tag_eta_gradient: wet, or |grad eta| above a
+  threshold), the forest adapts on the host, and the new forest is laid out
+  in the other half of the pool: unchanged leaves are moved, refined ones
+  interpolated from their parents, coarsened families averaged, all in the
+  scenario's current buffer, before the scenario is re-targeted to the new
+  slots.
+
+The order of a whole ghost update (fine -> coarse averaging, then per level:
+physical sides, interpolation to the next level) is ForestClaw's
+fclaw2d_ghost_update, which is not in the reference tree; it is restated here
+and in the library.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import GF_NB_STORED, GF_NB_WALL
+from .pool import _KIND, MBC, Pool, pad_bathymetry
+
+DIRS = ((-1, 0), (1, 0), (0, -1), (0, 1), (-1, -1), (1, -1), (-1, 1), (1, 1))
+
+
+def _block(a: np.ndarray, s: int) -> np.ndarray:
+    """View of a 2-D array as (ny/s, s, nx/s, s) blocks."""
+    return a.reshape(a.shape[0] // s, s, a.shape[1] // s, s)
+
+
+def _up(a: np.ndarray, s: int) -> np.ndarray:
+    return np.repeat(np.repeat(a, s, axis=0), s, axis=1)
+
+
+class Forest:
+    """Leaves of a quadtree over an nx0 x ny0 brick of level-0 patches.
+
+    ``lev[J, I]`` is the level of the leaf covering patch (I, J) of the
+    finest lattice (nx0 2^maxlevel by ny0 2^maxlevel patches).  A leaf
+    (L, i, j) covers the finest patches [i s, (i+1) s) x [j s, (j+1) s),
+    s = 2^(maxlevel - L).  Leaves are indexed in the order (level, j, i)."""
+
+    def __init__(self, nx0: int, ny0: int, minlevel: int, maxlevel: int, lev: np.ndarray | None = None):
+        assert 0 <= minlevel <= maxlevel
+        self.nx0, self.ny0 = nx0, ny0
+        self.minlevel, self.maxlevel = minlevel, maxlevel
+        self.nx, self.ny = nx0 << maxlevel, ny0 << maxlevel
+        if lev is None:
+            lev = np.full((self.ny, self.nx), minlevel, dtype=np.int64)
+        self.lev = np.asarray(lev, dtype=np.int64)
+        assert self.lev.shape == (self.ny, self.nx)
+        self._index()
+
+    def _index(self):
+        M = self.maxlevel
+        lev = self.lev
+        s = np.left_shift(1, M - lev)
+        I = np.arange(self.nx)[None, :]
+        J = np.arange(self.ny)[:, None]
+        jj, ii = np.nonzero((I % s == 0) & (J % s == 0))
+        L = lev[jj, ii]
+        i, j = ii >> (M - L), jj >> (M - L)
+        order = np.lexsort((i, j, L))
+        self.L, self.i, self.j = L[order], i[order], j[order]
+        self.oi, self.oj = ii[order], jj[order]          # origins on the finest lattice
+        ids = np.full((self.ny, self.nx), -1, dtype=np.int64)
+        ids[self.oj, self.oi] = np.arange(len(order))
+        self.leaf_at = ids[J - J % s, I - I % s]          # covering leaf of every finest patch
+
+    def __len__(self) -> int:
+        return len(self.L)
+
+    def counts(self) -> dict:
+        return {int(L): int((self.L == L).sum()) for L in np.unique(self.L)}
+
+    def same(self, other: "Forest") -> bool:
+        return np.array_equal(self.lev, other.lev)
+
+    # -- adaptation -------------------------------------------------------------------
+    def _tree(self, req: np.ndarray) -> np.ndarray:
+        """The coarsest quadtree whose every finest patch is at least at
+        level req there."""
+        M = self.maxlevel
+        lev = np.full_like(req, M)
+        done = np.zeros(req.shape, dtype=bool)
+        for L in range(M + 1):
+            s = 1 << (M - L)
+            cond = _up(_block(req, s).max(axis=(1, 3)) <= L, s)
+            lev[cond & ~done] = L
+            done |= cond
+        return lev
+
+    def _balance(self, req: np.ndarray) -> np.ndarray:
+        """The coarsest 2:1-balanced tree (faces and corners) at least as
+        fine as req: adjacent finest patches differ by at most one level."""
+        req = np.maximum(req, self.minlevel)
+        while True:
+            lev = self._tree(req)
+            p = np.pad(lev, 1, constant_values=-1)
+            nb = np.max([p[1 + dy:1 + dy + self.ny, 1 + dx:1 + dx + self.nx] for dx, dy in DIRS], axis=0)
+            need = nb - 1
+            if (lev >= need).all():
+                return lev
+            req = np.maximum(req, need)
+
+    def adapt(self, refine: np.ndarray, coarsen: np.ndarray) -> "Forest":
+        """The forest after one regrid: leaves with refine[k] split once
+        (below maxlevel); families of four sibling leaves that all have
+        coarsen[k] (and no refine) merge into their parent (above minlevel);
+        then the balance closure, which may refine further leaves or keep a
+        family from merging."""
+        M = self.maxlevel
+        refine = np.asarray(refine, dtype=bool)
+        coarsen = np.asarray(coarsen, dtype=bool) & ~refine
+        lev = self.lev
+        req = lev + (refine[self.leaf_at] & (lev < M))
+        cc = coarsen[self.leaf_at]
+        for L in range(self.minlevel + 1, M + 1):
+            sp = 1 << (M - L + 1)  # the parent's block
+            b = _block(lev, sp)
+            fam = (b.min(axis=(1, 3)) == L) & (b.max(axis=(1, 3)) == L) & _block(cc, sp).all(axis=(1, 3))
+            req[_up(fam, sp)] = L - 1
+        return Forest(self.nx0, self.ny0, self.minlevel, self.maxlevel, self._balance(req))
+
+    # -- tables -------------------------------------------------------------------------
+    def tables(self, mthbc=(3, 3, 3, 3)):
+        """Neighbour table and coarse/fine links in leaf indices.
+
+        nbr[k, d] for the directions left, right, bottom, top, left-bottom,
+        right-bottom, left-top, right-top: the same-level leaf there, the bc2
+        kind of the domain side (faces; GF_NB_WALL for corners outside), or
+        GF_NB_STORED when the region is covered by a coarser leaf or refined
+        further.  links: (coarse leaf, fine leaf, side, igrid) for every
+        finer leaf across a side or corner of a coarser one (igrid: the
+        fine leaf's half along a face, 0 the lower coordinate)."""
+        M = self.maxlevel
+        L, oi, oj = self.L, self.oi, self.oj
+        n = len(L)
+        s = np.left_shift(1, M - L)
+        ids = np.arange(n)
+        nbr = np.empty((n, 8), dtype=np.int32)
+        kinds = [_KIND[mthbc[k]] for k in range(4)]
+        lc, lf, ls, lg = [], [], [], []
+        for d, (dx, dy) in enumerate(DIRS):
+            ni, nj = oi + dx * s, oj + dy * s
+            inside = (ni >= 0) & (ni < self.nx) & (nj >= 0) & (nj < self.ny)
+            nic, njc = np.clip(ni, 0, self.nx - 1), np.clip(nj, 0, self.ny - 1)
+            ln = self.lev[njc, nic]
+            same = inside & (ln == L)
+            out = kinds[d] if d < 4 else GF_NB_WALL
+            nbr[:, d] = np.where(same, self.leaf_at[njc, nic], np.where(inside, GF_NB_STORED, out))
+            finer = inside & (ln > L)
+            if not finer.any():
+                continue
+            cis = [0] if dx > 0 else [1] if dx < 0 else [0, 1]
+            cjs = [0] if dy > 0 else [1] if dy < 0 else [0, 1]
+            h = s // 2
+            for cj in cjs:
+                for ci in cis:
+                    ki = np.where(finer, nic + ci * h, 0)
+                    kj = np.where(finer, njc + cj * h, 0)
+                    assert (self.lev[kj[finer], ki[finer]] == L[finer] + 1).all(), "forest is not 2:1 balanced"
+                    lc.append(ids[finer])
+                    lf.append(self.leaf_at[kj[finer], ki[finer]])
+                    ls.append(np.full(finer.sum(), d))
+                    lg.append(np.full(finer.sum(), (cj if d < 2 else ci) if d < 4 else 0))
+        cat = (lambda v: np.concatenate(v).astype(np.int32)) if lc else (lambda v: np.zeros(0, np.int32))
+        return nbr, (cat(lc), cat(lf), cat(ls), cat(lg))
+
+
+def regrid_plan(old: Forest, new: Forest):
+    """How the new forest's leaves arise from the old one's: a list of new
+    leaf indices whose leaf is unchanged (with the old leaf), refined
+    families (old parent, its four new children in igrid order, x fastest)
+    and coarsened families (new parent, its four old children)."""
+    M = new.maxlevel
+    ko = old.leaf_at[new.oj, new.oi]
+    lo = old.L[ko]
+    same = np.flatnonzero(lo == new.L)
+    moved = (same, ko[same])
+    child = np.flatnonzero(lo == new.L - 1)
+    fams = {}
+    for k in child:
+        g = (new.i[k] & 1) + 2 * (new.j[k] & 1)
+        fams.setdefault(int(ko[k]), [0, 0, 0, 0])[g] = int(k)
+    refined = [(p, kids) for p, kids in sorted(fams.items())]
+    par = np.flatnonzero(lo == new.L + 1)
+    coarsened = []
+    for k in par:
+        h = 1 << (M - new.L[k] - 1)
+        kids = [int(old.leaf_at[new.oj[k] + cj * h, new.oi[k] + ci * h]) for cj in range(2) for ci in range(2)]
+        coarsened.append((int(k), kids))
+    assert len(same) + len(child) + len(par) == len(new)
+    return moved, refined, coarsened
+
+
+class LevelDem:
+    """Bathymetry of each level on its own lattice, with the ghost ring of
+    every patch: ``dem_fn(nx_cells, ny_cells, level)`` gives the level's
+    interior cell values, padded by bc2's rule at the domain sides
+    (pad_bathymetry), so a patch's aux is a slice of its level's array."""
+
+    def __init__(self, forest: Forest, mx: int, dem_fn, mthbc):
+        self.mx = mx
+        self.pad = {}
+        for L in range(forest.maxlevel + 1):
+            nx, ny = (forest.nx0 << L) * mx, (forest.ny0 << L) * mx
+            self.pad[L] = pad_bathymetry(np.asarray(dem_fn(nx, ny, L), dtype=np.float64), mthbc)
+
+    def aux(self, L, i, j) -> np.ndarray:
+        """(n, mx+4, mx+4, 1) aux of leaves (L[k], i[k], j[k])."""
+        m = self.mx
+        out = np.empty((len(L), m + 2 * MBC, m + 2 * MBC, 1))
+        for k, (l, a, b) in enumerate(zip(L, i, j)):
+            out[k, ..., 0] = self.pad[int(l)][b * m:b * m + m + 2 * MBC, a * m:a * m + m + 2 * MBC]
+        return out
+
+
+class AmrRun:
+    """One scenario of resident AMR in a device pool of 2 x capacity slots
+    (two halves: the current forest in one, a regrid builds the next in the
+    other).
+
+    mx: patch size; (nx0, ny0): level-0 patches over [0, lx] x [0, ly];
+    dem_fn(nx_cells, ny_cells, level) -> (ny, nx) bathymetry of a level;
+    mthbc: bc2 kinds of the four sides (3 wall, 1 extrapolation, 0
+    hydrograph, whose table ``hydro`` = dict(time, hu, b, x0, y0) is given);
+    eta_tol: the |grad eta| refinement threshold."""
+
+    def __init__(self, mx, nx0, ny0, lx, ly, minlevel, maxlevel, dem_fn, capacity, mthbc=(3, 3, 3, 3),
+                 hydro=None, eta_tol=0.01, regrid_interval=4, manning_n=None, dt_max=0.0, flags_fn=None):
+        self.mx, self.lx, self.ly = mx, lx, ly
+        # flags_fn(forest, tags, regrid_index) -> (refine, coarsen) per leaf;
+        # default: refine the tagged leaves, coarsen families of untagged ones
+        self.flags_fn = flags_fn
+        self.mthbc, self.hydro, self.eta_tol = tuple(mthbc), hydro, eta_tol
+        self.regrid_interval, self.manning_n, self.dt_max = regrid_interval, manning_n, dt_max
+        self.capacity = capacity
+        self.forest = Forest(nx0, ny0, minlevel, maxlevel)
+        self.dem = LevelDem(self.forest, mx, dem_fn, mthbc)
+        self.pool = Pool(mx, mx, 2 * capacity)
+        self.half = 0            # the forest occupies slots [half * capacity, ...)
+        self.perm = None         # slot offset of each leaf
+        self.sid = None
+        self.stats = dict(regrids=0, attempts=0, steps=0, retakes=0, cell_updates=0, patches=[])
+        self.tag_log = []        # the tags of every regrid, in leaf order
+
+    # -- layout -----------------------------------------------------------------------
+    def _dx(self, L):
+        return self.lx / ((self.forest.nx0 << L) * self.mx), self.ly / ((self.forest.ny0 << L) * self.mx)
+
+    def _describe(self, forest: Forest, base: int, perm: np.ndarray):
+        """Describe the slots [base, base + len(forest)) of a forest whose
+        leaf k sits in slot base + perm[k]; store its levels."""
+        n = len(forest)
+        if n > self.capacity:
+            raise RuntimeError(f"forest of {n} leaves exceeds the capacity {self.capacity}")
+        nbr, _ = forest.tables(self.mthbc)
+        inv = np.empty(n, dtype=np.int64)
+        inv[perm] = np.arange(n)          # leaf of each slot offset
+        nb = nbr[inv]
+        nb = np.where(nb >= 0, base + perm[np.maximum(nb, 0)], nb).astype(np.int32)
+        L, i, j = forest.L[inv], forest.i[inv], forest.j[inv]
+        dx = np.array([self._dx(l)[0] for l in L])
+        dy = np.array([self._dx(l)[1] for l in L])
+        xlo, ylo = i * self.mx * dx, j * self.mx * dy
+        from . import check, dptr, iptr
+        d = [np.ascontiguousarray(a, dtype=np.float64) for a in (dx, dy, xlo, ylo)]
+        check(self.pool.lib.gf_pool_describe(self.pool.h, base, n, *[dptr(a) for a in d], iptr(np.ascontiguousarray(nb))),
+              "gf_pool_describe")
+        self.pool.n = max(self.pool.n, base + n)
+        self.pool.set_levels(base, L.astype(np.int32))
+        return inv
+
+    def _links(self, forest: Forest, base: int, perm: np.ndarray):
+        _, (c, f, s, g) = forest.tables(self.mthbc)
+        self.pool.set_cf_links(base + perm[c], base + perm[f], s, g)
+
+    def leaves_state(self, buf=None) -> tuple[np.ndarray, np.ndarray]:
+        """(q, aux) of every leaf in leaf order (ghost cells included)."""
+        info = self.pool.scenario(self.sid)
+        base = self.half * self.capacity
+        q = self.pool.download_aos(base, len(self.forest), buf=info["cur"] if buf is None else buf)
+        a = self.pool.download_aux(base, len(self.forest))
+        return q[self.perm], a[self.perm]
+
+    # -- set-up -----------------------------------------------------------------------
+    def start(self, t0=0.0, dt0=1.0, desired_cfl=0.9, max_cfl=1.0, state_fn=None, init_levels=True):
+        """Register the scenario on the minlevel forest, then refine it by the
+        tagging criterion on the initial state until it no longer changes
+        (ForestClaw's initialisation: new leaves take the initial state, not
+        an interpolation), alternating the pool halves as a regrid does.
+        state_fn(L, i, j, aux) -> q (n, mx+4, mx+4, 3) gives the initial
+        state (default: dry)."""
+        f = self.forest
+        self._upload_forest(f, 0, state_fn)
+        self.sid = self.pool.lib.gf_pool_add_scenario(self.pool.h, 0, len(f), t0, dt0, desired_cfl, max_cfl, 0, 0.0)
+        if self.sid < 0:
+            from . import check
+            check(self.sid, "gf_pool_add_scenario")
+        if self.manning_n is not None:
+            self.pool.set_friction(self.sid, self.manning_n)
+        if self.dt_max > 0:
+            self.pool.set_dt_max(self.sid, self.dt_max)
+        if self.hydro is not None:
+            h = self.hydro
+            self.pool.set_inflow(self.sid, h["time"], h["hu"], h["b"], h["x0"], h["y0"])
+        while init_levels:
+            tags = self._tags(f, 0, t0)
+            g = f.adapt(tags.astype(bool), np.zeros(len(f), dtype=bool))
+            if g.same(f):
+                break
+            other = 1 - self.half
+            self._upload_forest(g, other * self.capacity, state_fn)
+            self.pool.set_scenario_slots(self.sid, other * self.capacity, len(g))
+            self.half = other
+            f = g
+        self.stats["patches"].append(len(f))
+
+    def _upload_forest(self, f: Forest, base: int, state_fn):
+        self.perm = np.arange(len(f))
+        self._describe(f, base, self.perm)
+        aux = self.dem.aux(f.L, f.i, f.j)
+        q = np.zeros(aux.shape[:3] + (3,)) if state_fn is None else state_fn(f.L, f.i, f.j, aux)
+        self.pool.upload_aos(base, q, aux, buf=0)
+        self._links(f, base, self.perm)
+        self.forest = f
+
+    def _tags(self, f: Forest, buf: int, t: float) -> np.ndarray:
+        base = self.half * self.capacity
+        self.pool.ghost_update(buf, t)
+        return self.pool.tag_eta_gradient(base + self.perm, self.eta_tol, buf=buf)
+
+    # -- stepping -----------------------------------------------------------------------
+    def regrid(self):
+        """Tag the current leaves, adapt the forest and lay it out in the
+        other half of the pool (state in the scenario's current buffer)."""
+        info = self.pool.scenario(self.sid)
+        cur, t = info["cur"], info["t"]
+        old = self.forest
+        tags = self._tags(old, cur, t).astype(bool)
+        self.tag_log.append(tags)
+        refine, coarsen = (tags, ~tags) if self.flags_fn is None else self.flags_fn(old, tags, self.stats["regrids"])
+        new = old.adapt(refine, coarsen)
+        self.stats["regrids"] += 1
+        if new.same(old):
+            return False
+        ob, nb_ = self.half * self.capacity, (1 - self.half) * self.capacity
+        (keep_new, keep_old), refined, coarsened = regrid_plan(old, new)
+        # new slot order: unchanged leaves first, then the created ones
+        created = np.setdiff1d(np.arange(len(new)), keep_new, assume_unique=True)
+        perm = np.empty(len(new), dtype=np.int64)
+        perm[keep_new] = np.arange(len(keep_new))
+        perm[created] = len(keep_new) + np.arange(len(created))
+        self._describe(new, nb_, perm)
+        if len(created):
+            aux = self.dem.aux(new.L[created], new.i[created], new.j[created])
+            self.pool.upload_aos(nb_ + len(keep_new), np.zeros(aux.shape[:3] + (3,)), aux, buf=cur)
+        if len(keep_new):
+            self.pool.move(ob + self.perm[keep_old], nb_ + perm[keep_new], buf=cur)
+        if refined:
+            par = np.array([ob + self.perm[p] for p, _ in refined], dtype=np.int32)
+            kids = np.array([nb_ + perm[k] for _, ks in refined for k in ks], dtype=np.int32)
+            self.pool.interpolate2fine(par, kids, buf=cur)
+        if coarsened:
+            par = np.array([nb_ + perm[p] for p, _ in coarsened], dtype=np.int32)
+            kids = np.array([ob + self.perm[k] for _, ks in coarsened for k in ks], dtype=np.int32)
+            self.pool.average2coarse(par, kids, buf=cur)
+        self.pool.set_scenario_slots(self.sid, nb_, len(new))
+        self._links(new, nb_, perm)
+        self.forest, self.perm, self.half = new, perm, 1 - self.half
+        return True
+
+    def run(self, steps: int, graph: bool = True) -> dict:
+        """Advance by ``steps`` accepted steps, regridding every
+        regrid_interval accepted steps."""
+        info = self.pool.scenario(self.sid)
+        target = info["nsteps"] + steps
+        while info["nsteps"] < target:
+            before = info["nsteps"]
+            nxt = min(target, (before // self.regrid_interval + 1) * self.regrid_interval)
+            # the scenario stops on the device at the next regrid step: replay
+            # that many attempts, then single ones for any retakes
+            self.pool.set_stop(self.sid, nxt)
+            k = nxt - before
+            while True:
+                if graph:
+                    self.pool.run(k, graph=True)
+                else:
+                    self.pool.run_plain(k)
+                info = self.pool.scenario(self.sid)
+                if info["status"] != 0:
+                    raise RuntimeError(f"scenario stopped with status {info['status']} at t = {info['t']}")
+                if info["nsteps"] >= nxt:
+                    break
+                k = 1
+            self.stats["attempts"] = info["attempts"]
+            cells = len(self.forest) * self.mx * self.mx
+            self.stats["cell_updates"] += (info["nsteps"] - before) * cells
+            if info["nsteps"] % self.regrid_interval == 0:
+                self.regrid()
+                self.stats["patches"].append(len(self.forest))
+        self.stats["steps"] = info["nsteps"]
+        self.stats["retakes"] = info["retakes"]
+        return info
+
+    def close(self):
+        self.pool.close()

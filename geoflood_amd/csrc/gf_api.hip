@@ -13,6 +13,7 @@
 #include <condition_variable>
 #include <exception>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <cstdarg>
@@ -161,6 +162,20 @@ Lib g;
 
 constexpr size_t kChunk = 1024;  // patches per upload/download chunk
 
+// Resident AMR: refinement levels 0 .. kMaxLevels-1; device ranges
+// [lo, hi) of the stored links (all, then the interpolation links of each
+// coarse level) and of the inflow list's entries per level.
+constexpr int kMaxLevels = 16;
+constexpr int kRanges = 2 * (1 + 2 * kMaxLevels);
+inline int range_links() { return 0; }
+inline int range_interp(int l) { return 2 + 2 * l; }
+inline int range_inflow(int l) { return 2 + 2 * kMaxLevels + 2 * l; }
+constexpr int kLinkGrid = 1024;  // workgroups of a resident coarse/fine transfer (one wave per link, grid-stride)
+
+// Maximum in which a NaN operand wins, as the device reductions fold CFL
+// numbers: a patch whose new state is not finite reports a NaN CFL number.
+double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
+
 }  // namespace
 
 struct gf_pool {
@@ -205,6 +220,17 @@ struct gf_pool {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool graph_dirty = true;
+    // Resident AMR (gf_pool_set_levels, gf_pool_set_cf_links): every attempt
+    // starts with the multi-level ghost update -- fine -> coarse averaging
+    // over all links, then level by level the hydrograph sides and the
+    // coarse -> fine interpolation -- whose link and inflow ranges live in
+    // device memory, so a regrid changes them without a new graph.
+    bool amr = false;
+    std::vector<int> h_level;              // [capacity] refinement level per slot (0 when not set)
+    int nlev = 1;                          // levels the ghost update walks (max level + 1)
+    int* d_links = nullptr;                // [4][link_cap]: coarse, fine, side, igrid by the coarse level
+    int link_cap = 0, nlinks = 0;
+    int* d_ranges = nullptr;               // [kRanges]: see range_links / range_interp / range_inflow
 };
 
 static void default_params()
@@ -419,6 +445,19 @@ static int rebuild_inflow(gf_pool* p)
         p->has_inflow = n > 0;
         p->graph_dirty = true;
     }
+    if (p->amr) {
+        // level by level: the ghost update fills a level's hydrograph sides
+        // before interpolating from it to the next finer level
+        std::stable_sort(list.begin(), list.end(), [&](int a, int b) { return p->h_level[a] < p->h_level[b]; });
+        std::vector<int> rng(2 * kMaxLevels);
+        for (int l = 0, k = 0; l < kMaxLevels; l++) {
+            rng[2 * l] = k;
+            while (k < n && p->h_level[list[k]] == l) k++;
+            rng[2 * l + 1] = k;
+        }
+        HIPCHK(hipMemcpyAsync(p->d_ranges + range_inflow(0), rng.data(), sizeof(int) * rng.size(),
+                              hipMemcpyHostToDevice, g.stream));
+    }
     if (n) HIPCHK(hipMemcpyAsync(p->inflow_list, list.data(), sizeof(int) * n, hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipMemcpyAsync(p->inflow_n, &n, sizeof(int), hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
@@ -440,7 +479,9 @@ try {
         fail(GF_EINVAL, "capacity must be > 0");
         return nullptr;
     }
-    gf_pool* p = new gf_pool();
+    // freed by gf_pool_destroy if anything below fails or throws
+    std::unique_ptr<gf_pool, int (*)(gf_pool*)> hold(new gf_pool(), gf_pool_destroy);
+    gf_pool* p = hold.get();
     p->mx = mx;
     p->my = my;
     p->mbc = mbc;
@@ -491,9 +532,9 @@ try {
              hipMemsetAsync(p->inflow_n, 0, sizeof(int), g.stream) == hipSuccess &&
              hipStreamSynchronize(g.stream) == hipSuccess;
     if (!ok) {
+        (void)hipGetLastError();
         fail(GF_ENOMEM, "pool allocation of %d patches (%lld bytes) failed", capacity,
              gf_pool_bytes(mx, my, mbc, capacity));
-        gf_pool_destroy(p);
         return nullptr;
     }
     p->h_desc.resize(capacity);
@@ -506,12 +547,12 @@ try {
                        g.stream) != hipSuccess ||
         hipStreamSynchronize(g.stream) != hipSuccess) {
         fail(GF_EHIP, "descriptor upload failed");
-        gf_pool_destroy(p);
         return nullptr;
     }
     p->h_ctrl.assign(capacity, ScenCtrl{});
     p->hyd_mem.assign(capacity, nullptr);
-    return p;
+    p->h_level.assign(capacity, 0);
+    return hold.release();
 } GF_CATCH_NULL
 
 int gf_pool_destroy(gf_pool* p)
@@ -542,6 +583,8 @@ try {
     hipFree(p->tag_buf);
     hipFree(p->inflow_list);
     hipFree(p->inflow_n);
+    hipFree(p->d_links);
+    hipFree(p->d_ranges);
     for (double* m : p->hyd_mem) hipFree(m);
     if (p->h_stage) hipHostFree(p->h_stage);
     if (p->h_cfl) hipHostFree(p->h_cfl);
@@ -654,39 +697,49 @@ struct StageSplit {
 };
 static StageSplit stage_split(const gf_pool* p, size_t per)
 {
-    const size_t fit = p->stage_doubles / per;
+    const size_t fit = per ? p->stage_doubles / per : 0;
     return fit >= 2 ? StageSplit{2, (int)(fit / 2)} : StageSplit{1, (int)fit};
 }
 
 // Grow the staging buffers so that two patches of `per` doubles fit (a small
-// pool with several aux fields per cell); they never shrink.
+// pool with several aux fields per cell); they never shrink.  The new
+// buffers are allocated before the old ones are freed, so a failed grow
+// leaves the pool with the staging it had.
 static int ensure_stage(gf_pool* p, size_t per)
 {
     if (p->stage_doubles >= 2 * per) return GF_OK;
+    double* d = nullptr;
+    double* h = nullptr;
+    if (hipMalloc(&d, 2 * per * sizeof(double)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(GF_ENOMEM, "staging buffers of %zu bytes", 2 * per * sizeof(double));
+    }
+    if (hipHostMalloc(&h, 2 * per * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        hipFree(d);
+        return fail(GF_ENOMEM, "staging buffers of %zu bytes", 2 * per * sizeof(double));
+    }
     HIPCHK(hipStreamSynchronize(g.stream));
     HIPCHK(hipFree(p->d_stage));
     HIPCHK(hipHostFree(p->h_stage));
-    p->d_stage = nullptr;
-    p->h_stage = nullptr;
-    p->stage_doubles = 0;
-    if (hipMalloc(&p->d_stage, 2 * per * sizeof(double)) != hipSuccess ||
-        hipHostMalloc(&p->h_stage, 2 * per * sizeof(double), hipHostMallocDefault) != hipSuccess)
-        return fail(GF_ENOMEM, "staging buffers of %zu bytes", 2 * per * sizeof(double));
+    p->d_stage = d;
+    p->h_stage = h;
     p->stage_doubles = 2 * per;
     return GF_OK;
 }
 
-// Host patches -> pool slots [first, first+n) of buffer buf through the
-// pinned and device staging buffers. put(k, q, aux) copies host patch k's
-// 3*np2 values of q (and maux*np2 of aux when with_aux) into the pinned
-// staging.
+// Host patches -> pool slots [first, first+n) of buffer buf (and of buffer
+// buf2 as well when buf2 >= 0) through the pinned and device staging
+// buffers. put(k, q, aux) copies host patch k's 3*np2 values of q (and
+// maux*np2 of aux when with_aux) into the pinned staging.
 template <class Put>
-static int staged_upload(gf_pool* p, int first, int n, int buf, int maux, bool with_aux, Put&& put)
+static int staged_upload(gf_pool* p, int first, int n, int buf, int maux, bool with_aux, Put&& put, int buf2 = -1)
 {
     const size_t np2 = p->np2;
     const size_t per = (3 + (with_aux ? (size_t)maux : 0)) * np2;
     if (int r = ensure_stage(p, per)) return r;
     const StageSplit sp = stage_split(p, per);
+    if (sp.part <= 0) return fail(GF_ESTATE, "staging buffers hold no patch");
     for (int off = 0, c = 0; off < n; off += sp.part, c++) {
         const int b = c % sp.nbuf, m = std::min(sp.part, n - off);
         double* hs = p->h_stage + (size_t)b * sp.part * per;
@@ -698,8 +751,8 @@ static int staged_upload(gf_pool* p, int first, int n, int buf, int maux, bool w
         });
         HIPCHK(hipMemcpyAsync(ds, hs, (qd + (with_aux ? (size_t)m * maux * np2 : 0)) * sizeof(double),
                               hipMemcpyHostToDevice, g.stream));
-        HIPCHK(gf::launch_pack(ds, with_aux ? ds + qd : nullptr, p->q[buf], p->aux, first + off, m, (int)np2, maux,
-                               g.stream));
+        HIPCHK(gf::launch_pack(ds, with_aux ? ds + qd : nullptr, p->q[buf], buf2 >= 0 ? p->q[buf2] : nullptr,
+                               p->aux, first + off, m, (int)np2, maux, g.stream));
         HIPCHK(hipEventRecord(g.ev_half[b], g.stream));
     }
     return GF_OK;
@@ -713,6 +766,7 @@ static int staged_download(gf_pool* p, int first, int n, int buf, Get&& get)
     const size_t np2 = p->np2;
     const size_t per = 3 * np2;
     const StageSplit sp = stage_split(p, per);
+    if (sp.part <= 0) return fail(GF_ESTATE, "staging buffers hold no patch");
     const int nch = (n + sp.part - 1) / sp.part;
     auto enqueue = [&](int c) -> int {
         const int b = c % sp.nbuf, off = c * sp.part, m = std::min(sp.part, n - off);
@@ -826,6 +880,18 @@ static int pool_step_impl(gf_pool* p, int first, int n, int src, int dst, double
     if (first < 0 || n < 0 || first + n > p->n) return fail(GF_EINVAL, "patches [%d,%d) outside [0,%d)", first, first + n, p->n);
     if ((src != 0 && src != 1) || (dst != 0 && dst != 1)) return fail(GF_EINVAL, "buffers must be 0 or 1");
     if (!(dt > 0.0)) return fail(GF_EINVAL, "dt must be > 0");
+    if (src == dst) {
+        // A tiled variant steps a patch with several workgroups, each reading
+        // its halo from the neighbouring tiles' cells, which the others
+        // overwrite in place: only one workgroup per patch may step in place.
+        gf::StepVariant v;
+        if (!gf::get_variant(p->mx, p->my, p->variant, &v)) return fail(GF_ESTATE, "no step variant %d", p->variant);
+        if (v.tx != p->mx || v.ty != p->my)
+            return fail(GF_EUNSUPPORTED,
+                        "an in-place step (src == dst) needs a step variant with one tile per patch; variant %d "
+                        "splits %dx%d patches into %dx%d tiles (step into the other buffer instead)",
+                        p->variant, p->mx, p->my, v.tx, v.ty);
+    }
     double* q0 = (src == dst) ? p->q[src] : p->q[0];
     double* q1 = (src == dst) ? nullptr : p->q[1];
     const int cur = (src == dst) ? 0 : src;
@@ -859,7 +925,7 @@ static int pool_step_impl(gf_pool* p, int first, int n, int src, int dst, double
     double m = 0.0;
     for (int k = 0; k < n; k++) {
         if (cfl) cfl[k] = p->h_cfl[k];
-        m = p->h_cfl[k] > m ? p->h_cfl[k] : m;
+        m = max_nan(m, p->h_cfl[k]);
     }
     if (maxcfl) *maxcfl = m;
     return GF_OK;
@@ -979,6 +1045,35 @@ try {
     return GF_OK;
 } GF_CATCH_STATUS
 
+int gf_pool_set_scenario_stop(gf_pool* p, int s, int target_steps, double t_end)
+try {
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
+    HIPCHK(hipMemcpyAsync(&p->h_ctrl[s], p->ctrl + s, sizeof(ScenCtrl), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    ScenCtrl& c = p->h_ctrl[s];
+    c.target_steps = target_steps;
+    c.t_end = t_end;
+    const bool done = (target_steps > 0 && c.nsteps >= target_steps) || (t_end > 0.0 && c.t >= t_end);
+    c.active = (c.status == gf::SCEN_OK && !done) ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(p->ctrl + s, &c, sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+} GF_CATCH_STATUS
+
+int gf_pool_set_scenario_dt_max(gf_pool* p, int s, double dt_max)
+try {
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
+    if (!(dt_max >= 0.0) || dt_max - dt_max != 0.0) return fail(GF_EINVAL, "dt_max must be finite and >= 0");
+    HIPCHK(hipMemcpyAsync(&p->h_ctrl[s], p->ctrl + s, sizeof(ScenCtrl), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    p->h_ctrl[s].dt_max = dt_max;
+    HIPCHK(hipMemcpyAsync(p->ctrl + s, &p->h_ctrl[s], sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+} GF_CATCH_STATUS
+
 int gf_pool_clear_scenarios(gf_pool* p)
 try {
     if (!p) return fail(GF_EINVAL, "pool is NULL");
@@ -1029,6 +1124,7 @@ try {
     info->attempts = c.attempts;
     info->active = c.active;
     info->cur = c.cur;
+    info->status = c.status;
     return GF_OK;
 } GF_CATCH_STATUS
 
@@ -1048,6 +1144,7 @@ try {
         o.last_dt = c[k].last_dt;
         o.nsteps = c[k].nsteps;
         o.retakes = c[k].retakes;
+        o.status = c[k].status;
         o.attempts = c[k].attempts;
         o.active = p->h_ctrl[first + k].npatch > 0 ? c[k].active : 0;
         o.cur = c[k].cur;
@@ -1115,15 +1212,64 @@ static int wet_grid(gf_pool* p)
     return p->wet_grid;
 }
 
-// One attempt of every active scenario: (hydrograph ghosts) -> classify ->
-// one persistent kernel over the active tiles and then the dry patches ->
+// The multi-level ghost update of a resident AMR pool, in ForestClaw's order
+// (fclaw2d_ghost_update with time_interp off, as restated for resident
+// patches): the coarse ghost cells that finer neighbours cover, averaged from
+// them (all links at once: they read fine interiors only); then for each
+// level from coarse to fine its hydrograph sides, and the next finer level's
+// ghost cells interpolated from it.  Same-level copies and walls need no
+// pass: every reader takes them from the neighbours (ghost_source).  buf < 0:
+// each scenario's current buffer and time, inactive scenarios skipped (the
+// form captured in the attempt graph); otherwise buffer buf at time t.
+static int enqueue_ghost_update(gf_pool* p, int buf, double t)
+{
+    gf::RegridArgs a{};
+    a.desc = p->desc;
+    a.aux = p->aux;
+    a.cs = p->d_links;
+    a.fs = p->d_links + p->link_cap;
+    a.side = p->d_links + 2 * p->link_cap;
+    a.igrid = p->d_links + 3 * p->link_cap;
+    a.n = kLinkGrid;
+    a.dry_tolerance = g.P.dry_tolerance;
+    if (buf < 0) {
+        a.q = p->q[0];
+        a.q1 = p->q[1];
+        a.ctrl = p->ctrl;
+    } else {
+        a.q = p->q[buf];
+    }
+    if (p->link_cap > 0) {
+        a.range = p->d_ranges + range_links();
+        HIPCHK(gf::launch_average_ghost(p->mx, p->my, a, g.stream));
+    }
+    for (int l = 0; l < p->nlev; l++) {
+        if (p->has_inflow) {
+            gf::InflowArgs ia = inflow_args(p, 0, p->capacity, buf, t);
+            ia.range = p->d_ranges + range_inflow(l);
+            HIPCHK(gf::launch_inflow(p->mx, p->my, ia, inflow_grid(), g.stream));
+        }
+        if (l + 1 < p->nlev && p->link_cap > 0) {
+            a.range = p->d_ranges + range_interp(l);
+            HIPCHK(gf::launch_interpolate_ghost(p->mx, p->my, a, g.stream));
+        }
+    }
+    return GF_OK;
+}
+
+// One attempt of every active scenario: (hydrograph ghosts, or the
+// multi-level ghost update of a resident AMR pool) -> classify -> one
+// persistent kernel over the active tiles and then the dry patches ->
 // controller, all on one stream.  With k >= 0, timing events bracket the
 // step kernel.
 static int enqueue_attempt(gf_pool* p, int k)
 {
     const gf::BatchArgs a = host_batch_args(p);
-    if (p->has_inflow)
+    if (p->amr) {
+        if (int r = enqueue_ghost_update(p, -1, 0.0)) return r;
+    } else if (p->has_inflow) {
         HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, 0, p->capacity, -1, 0.0), inflow_grid(), g.stream));
+    }
     HIPCHK(gf::launch_classify(a, g.stream));
     if (k >= 0) HIPCHK(hipEventRecord(g.ev_step[2 * k], g.stream));
     HIPCHK(gf::launch_all(p->mx, p->my, p->variant, gf::default_method(g.P), p->d_args, wet_grid(p), g.stream));
@@ -1218,14 +1364,22 @@ try {
             if (hc[s].active) cnt += hc[s].npatch;
         if (int r = ensure_step_events(cnt)) return r;
         HIPCHK(hipEventRecord(g.ev_begin, g.stream));
+        if (p->amr) {
+            // the multi-level ghost update reads each scenario's buffer and
+            // time from the device records: bring them up to date first
+            HIPCHK(hipMemcpyAsync(p->ctrl, hc.data(), sizeof(ScenCtrl) * p->nscen, hipMemcpyHostToDevice, g.stream));
+            if (int r = enqueue_ghost_update(p, -1, 0.0)) return r;
+        }
         int e = 0;
         for (int s = 0; s < p->nscen; s++) {
             if (!hc[s].active) continue;
             gf::GhostArgs ga{p->desc, hc[s].first_patch, hc[s].npatch, p->q[0], p->q[1], nullptr, hc[s].cur};
             HIPCHK(gf::launch_ghost(p->mx, p->my, ga, g.stream));
-            HIPCHK(gf::launch_inflow(p->mx, p->my,
-                                     inflow_args(p, hc[s].first_patch, hc[s].first_patch + hc[s].npatch, hc[s].cur, hc[s].t),
-                                     inflow_grid(), g.stream));
+            if (!p->amr)
+                HIPCHK(gf::launch_inflow(p->mx, p->my,
+                                         inflow_args(p, hc[s].first_patch, hc[s].first_patch + hc[s].npatch, hc[s].cur,
+                                                     hc[s].t),
+                                         inflow_grid(), g.stream));
             for (int k = hc[s].first_patch; k < hc[s].first_patch + hc[s].npatch; k++, e++) {
                 HIPCHK(hipEventRecord(g.ev_step[2 * e], g.stream));
                 HIPCHK(gf::launch_step(p->mx, p->my, p->variant,
@@ -1249,10 +1403,13 @@ try {
             ScenCtrl& c = hc[s];
             if (!c.active) continue;
             double m = 0.0;
-            for (int k = c.first_patch; k < c.first_patch + c.npatch; k++) m = p->h_cfl[k] > m ? p->h_cfl[k] : m;
+            for (int k = c.first_patch; k < c.first_patch + c.npatch; k++) m = max_nan(m, p->h_cfl[k]);
             c.maxcfl = m;
             c.attempts += 1;
-            if (m > c.max_cfl) {
+            if (!gf::cfl_finite(m)) {
+                c.status = gf::SCEN_NONFINITE;
+                c.active = 0;
+            } else if (m > c.max_cfl) {
                 c.retakes += 1;
                 c.dt = c.dt * c.desired_cfl / m;
             } else {
@@ -1260,7 +1417,7 @@ try {
                 c.last_dt = c.dt;
                 c.cur ^= 1;
                 c.nsteps += 1;
-                c.dt = c.dt * c.desired_cfl / m;
+                c.dt = gf::next_dt(c.dt, c.desired_cfl, m, c.dt_max);
                 if ((c.target_steps > 0 && c.nsteps >= c.target_steps) || (c.t_end > 0.0 && c.t >= c.t_end))
                     c.active = 0;
             }
@@ -1308,16 +1465,19 @@ try {
     }
     p->n = n;
     HIPCHK(hipMemcpyAsync(p->desc, p->h_desc.data(), sizeof(PatchDesc) * n, hipMemcpyHostToDevice, g.stream));
-    // pack: host patches -> pinned staging -> device SoA, then back the same way
+    // pack: host patches -> pinned staging -> device SoA in both buffers (the
+    // step reads buffer 0 and writes the interior of buffer 1, whose ghost
+    // cells keep the host's values), then buffer 1 back the same way.  Never
+    // in place: the tiles of a tiled variant read each other's cells.
     if (int r = staged_upload(p, 0, n, 0, maux, true, [&](int k, double* sq, double* sa) {
             std::memcpy(sq, q[k], 3 * np2 * sizeof(double));
             std::memcpy(sa, aux[k], maux * np2 * sizeof(double));
-        }))
+        }, 1))
         return r;
-    HIPCHK(gf::launch_step(mx, my, p->variant, step_args(p, 0, n, p->q[0], nullptr, nullptr, dt, 0), g.stream));
+    HIPCHK(gf::launch_step(mx, my, p->variant, step_args(p, 0, n, p->q[0], p->q[1], nullptr, dt, 0), g.stream));
     HIPCHK(gf::launch_max(p->cfl, n, g.d_max, g.stream));
     HIPCHK(hipMemcpyAsync(g.h_max, g.d_max, sizeof(double), hipMemcpyDeviceToHost, g.stream));
-    if (int r = staged_download(p, 0, n, 0, [&](int k, const double* sq) {
+    if (int r = staged_download(p, 0, n, 1, [&](int k, const double* sq) {
             std::memcpy(q[k], sq, 3 * np2 * sizeof(double));
         }))
         return r;
@@ -1821,6 +1981,192 @@ int gf_pool_interpolate_ghost(gf_pool* p, int buf, int n, const int* coarse, con
                               const int* igrid)
 try {
     return cf_ghost_impl(p, buf, n, coarse, fine, side, igrid, true);
+} GF_CATCH_STATUS
+
+// ---- resident AMR ---------------------------------------------------------
+
+static int ensure_amr(gf_pool* p)
+{
+    if (!p->d_ranges) {
+        HIPCHK(hipMalloc(&p->d_ranges, sizeof(int) * kRanges));
+        HIPCHK(hipMemsetAsync(p->d_ranges, 0, sizeof(int) * kRanges, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+    }
+    if (!p->amr) {
+        p->amr = true;
+        p->graph_dirty = true;
+    }
+    return GF_OK;
+}
+
+int gf_pool_set_levels(gf_pool* p, int first, int n, const int* level)
+try {
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (first < 0 || n < 0 || first + n > p->capacity) return fail(GF_EINVAL, "slots [%d,%d) outside the pool", first, first + n);
+    if (n > 0 && !level) return fail(GF_EINVAL, "level is NULL");
+    for (int k = 0; k < n; k++)
+        if (level[k] < 0 || level[k] >= kMaxLevels) return fail(GF_EINVAL, "slot %d: level %d not in [0, %d)", first + k, level[k], kMaxLevels);
+    if (int r = ensure_amr(p)) return r;
+    for (int k = 0; k < n; k++) p->h_level[first + k] = level[k];
+    const int nlev = 1 + *std::max_element(p->h_level.begin(), p->h_level.end());
+    if (nlev != p->nlev) {
+        p->nlev = nlev;
+        p->graph_dirty = true;
+    }
+    return rebuild_inflow(p);
+} GF_CATCH_STATUS
+
+int gf_pool_set_cf_links(gf_pool* p, int n, const int* coarse, const int* fine, const int* side, const int* igrid)
+try {
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    // the same checks as the eager transfers (both directions): slots, sides,
+    // each written region once; and a fine patch one level below its coarse one
+    if (int r = cf_ghost_check(p, 0, n, coarse, fine, side, igrid, false)) return r;
+    {
+        std::vector<uint16_t> seen(p->capacity, 0);
+        for (int l = 0; l < n; l++) {
+            const int sd = side[l];
+            const uint16_t bit = (uint16_t)(1u << (sd < 4 ? 2 * sd + igrid[l] : 8 + (sd - 4)));
+            if (seen[fine[l]] & bit) return fail(GF_EINVAL, "link %d: slot %d's region is written twice", l, fine[l]);
+            seen[fine[l]] |= bit;
+            if (p->h_level[fine[l]] != p->h_level[coarse[l]] + 1)
+                return fail(GF_EINVAL, "link %d: fine slot %d (level %d) is not one level below coarse slot %d (level %d)", l,
+                            fine[l], p->h_level[fine[l]], coarse[l], p->h_level[coarse[l]]);
+        }
+    }
+    if (int r = ensure_amr(p)) return r;
+    if (n > p->link_cap) {
+        const int cap = std::max(n, 2 * p->link_cap);
+        int* d = nullptr;
+        HIPCHK(hipMalloc(&d, sizeof(int) * 4 * (size_t)cap));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        HIPCHK(hipFree(p->d_links));
+        p->d_links = d;
+        p->graph_dirty = true;  // the graph holds the link arrays' addresses (or had no transfers)
+        p->link_cap = cap;
+    }
+    // links by the coarse patch's level (stable), so each level's
+    // interpolation links form one range
+    std::vector<int> order(n);
+    for (int l = 0; l < n; l++) order[l] = l;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return p->h_level[coarse[a]] < p->h_level[coarse[b]]; });
+    std::vector<int> h(4 * (size_t)std::max(p->link_cap, 1), 0);
+    for (int k = 0; k < n; k++) {
+        const int l = order[k];
+        h[k] = coarse[l];
+        h[p->link_cap + k] = fine[l];
+        h[2 * p->link_cap + k] = side[l];
+        h[3 * p->link_cap + k] = side[l] >= 4 ? 0 : igrid[l];
+    }
+    std::vector<int> rng(2 * (1 + kMaxLevels), 0);
+    rng[0] = 0;
+    rng[1] = n;
+    for (int lv = 0, k = 0; lv < kMaxLevels; lv++) {
+        rng[range_interp(lv)] = k;
+        while (k < n && p->h_level[h[k]] == lv) k++;
+        rng[range_interp(lv) + 1] = k;
+    }
+    if (p->link_cap > 0)
+        HIPCHK(hipMemcpyAsync(p->d_links, h.data(), sizeof(int) * 4 * (size_t)p->link_cap, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(p->d_ranges, rng.data(), sizeof(int) * rng.size(), hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    p->nlinks = n;
+    return GF_OK;
+} GF_CATCH_STATUS
+
+int gf_pool_ghost_update(gf_pool* p, int buf, double t)
+try {
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (buf < -1 || buf > 1) return fail(GF_EINVAL, "buffer must be 0, 1 or -1 (each scenario's current one)");
+    if (!p->inflow_error.empty()) return fail(GF_EINVAL, "%s", p->inflow_error.c_str());
+    if (int r = ensure_amr(p)) return r;
+    if (int r = enqueue_ghost_update(p, buf, t)) return r;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+} GF_CATCH_STATUS
+
+int gf_pool_move(gf_pool* p, int n, const int* src, const int* dst, int buf)
+try {
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
+    if (n < 0 || (n > 0 && (!src || !dst))) return fail(GF_EINVAL, "invalid slot lists");
+    std::vector<char> role(p->capacity, 0);  // 1 source, 2 destination
+    for (int k = 0; k < n; k++) {
+        if (src[k] < 0 || src[k] >= p->capacity || dst[k] < 0 || dst[k] >= p->capacity)
+            return fail(GF_EINVAL, "entry %d: slot outside the pool", k);
+        if (role[dst[k]]) return fail(GF_EINVAL, "entry %d: slot %d is already a source or destination", k, dst[k]);
+        role[dst[k]] = 2;
+    }
+    for (int k = 0; k < n; k++) {
+        if (role[src[k]] == 2) return fail(GF_EINVAL, "entry %d: slot %d is both a source and a destination", k, src[k]);
+        role[src[k]] = 1;
+    }
+    if (n == 0) return GF_OK;
+    const size_t stride = ((sizeof(int) * (size_t)n + 255) / 256) * 256;
+    if (int r = ensure_scratch(p, 2 * stride)) return r;
+    char* d = (char*)p->tag_buf;
+    HIPCHK(hipMemcpyAsync(d, src, sizeof(int) * n, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(d + stride, dst, sizeof(int) * n, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(gf::launch_move_slots(p->q[buf], p->aux, (const int*)d, (const int*)(d + stride), n, (int)p->np2, g.stream));
+    HIPCHK(gf::launch_hmax_list(p->mx, p->my, p->q[buf], p->hmax2 + (size_t)buf * p->capacity, p->dry_age,
+                                (const int*)(d + stride), n, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
+} GF_CATCH_STATUS
+
+int gf_pool_set_scenario_slots(gf_pool* p, int s, int first, int n)
+try {
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
+    if (first < 0 || n <= 0 || first + n > p->n)
+        return fail(GF_EINVAL, "slots [%d,%d) outside the described slots [0,%d)", first, first + n, p->n);
+    ScenCtrl c;
+    HIPCHK(hipMemcpyAsync(&c, p->ctrl + s, sizeof(ScenCtrl), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    const int of = c.first_patch, on = c.npatch;
+    for (int k = first; k < first + n; k++)
+        if (p->h_desc[k].scen >= 0 && p->h_desc[k].scen != s)
+            return fail(GF_EINVAL, "slot %d belongs to scenario %d", k, p->h_desc[k].scen);
+    for (int k = of; k < of + on; k++) p->h_desc[k].scen = -1;
+    for (int k = first; k < first + n; k++) p->h_desc[k].scen = s;
+    c.first_patch = first;
+    c.npatch = n;
+    p->h_ctrl[s] = c;
+    HIPCHK(hipMemcpyAsync(p->desc + of, p->h_desc.data() + of, sizeof(PatchDesc) * on, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(p->desc + first, p->h_desc.data() + first, sizeof(PatchDesc) * n, hipMemcpyHostToDevice,
+                          g.stream));
+    HIPCHK(hipMemcpyAsync(p->ctrl + s, &c, sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
+    if (int r = reset_dry_age(p, first, n)) return r;
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return rebuild_inflow(p);
+} GF_CATCH_STATUS
+
+int gf_pool_tag_eta_gradient(gf_pool* p, int buf, double eta_tol, const int* patches, int n, int* tags)
+try {
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (buf != 0 && buf != 1) return fail(GF_EINVAL, "buffer must be 0 or 1");
+    if (n < 0 || (n > 0 && (!patches || !tags))) return fail(GF_EINVAL, "invalid patch or tag list");
+    if (!(eta_tol >= 0.0)) return fail(GF_EINVAL, "eta_tol must be >= 0");
+    for (int k = 0; k < n; k++)
+        if (patches[k] < 0 || patches[k] >= p->capacity || !(p->h_desc[patches[k]].dx > 0.0))
+            return fail(GF_EINVAL, "entry %d: slot %d is not described", k, patches[k]);
+    if (n == 0) return GF_OK;
+    const size_t stride = ((sizeof(int) * (size_t)n + 255) / 256) * 256;
+    if (int r = ensure_scratch(p, 2 * stride)) return r;
+    char* d = (char*)p->tag_buf;
+    HIPCHK(hipMemcpyAsync(d, patches, sizeof(int) * n, hipMemcpyHostToDevice, g.stream));
+    gf::TagArgs a{};
+    a.desc = p->desc;
+    a.q = p->q[buf];
+    a.aux = p->aux;
+    a.patches = (const int*)d;
+    a.n = n;
+    a.out = (int*)(d + stride);
+    a.drytol = g.P.dry_tolerance;
+    HIPCHK(gf::launch_tag_eta_gradient(p->mx, p->my, a, eta_tol, g.stream));
+    HIPCHK(hipMemcpyAsync(tags, a.out, sizeof(int) * n, hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    return GF_OK;
 } GF_CATCH_STATUS
 
 int gf_pool_copy(gf_pool* p, int n, const int* slots, int src_buf, int dst_buf)

@@ -45,8 +45,9 @@ struct ScenCtrl {
     double manning_n;    // Manning coefficient of this scenario's src2
     const double* hyd;   // hydrograph table: hyd[0..rows) times, hyd[rows..2 rows) discharge per unit width
     double hyd_b, hyd_x0, hyd_y0;  // channel width and centre of the inflow strip
+    double dt_max;       // upper bound of the next dt (<= 0: none)
     int hyd_rows;        // 0: no hydrograph
-    int hyd_pad;
+    int status;          // SCEN_OK, or why the scenario stopped (SCEN_NONFINITE)
     int cur;             // index of the q buffer holding the accepted state
     int nsteps;          // accepted steps
     int target_steps;    // stop after this many accepted steps (<= 0: no limit)
@@ -56,5 +57,24 @@ struct ScenCtrl {
     int npatch;
     int attempts;
 };
+
+// ScenCtrl::status (the values of gf_scenario_info::status)
+constexpr int SCEN_OK = 0;
+constexpr int SCEN_NONFINITE = 1;  // a NaN/Inf state or CFL number: stopped, no step accepted
+
+// dt of the attempt after an accepted step with CFL number m (the outstyle_3
+// rule dt * desired_cfl / m, fc2d_geoclaw_run.c:577-580), bounded by dt_max
+// when that is > 0.  A zero CFL number (every face dry on both sides, so no
+// wave anywhere) keeps dt, or takes dt_max when one is set, where the
+// reference's rule would give an infinite dt.
+__host__ __device__ inline double next_dt(double dt, double desired_cfl, double m, double dt_max)
+{
+    double d = m > 0.0 ? dt * desired_cfl / m : dt;
+    if (dt_max > 0.0 && (m == 0.0 || d > dt_max)) d = dt_max;
+    return d;
+}
+
+// The CFL number is finite (not NaN, not +-Inf).
+__host__ __device__ inline bool cfl_finite(double m) { return m - m == 0.0; }
 
 }  // namespace gf

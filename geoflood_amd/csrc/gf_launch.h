@@ -52,6 +52,12 @@ struct GaugeArgs {
 // Regrid between levels on resident patches (refinement ratio 2): family f
 // has the coarse patch in slot cs[f] and its children in slots fs[4f + g],
 // g = igrid = 0..3 (x fastest), all in buffer q.
+//
+// Coarse/fine ghost transfers also run in a resident mode (ctrl != nullptr),
+// as nodes of the attempt graph: link l's buffer is the current one of the
+// coarse patch's scenario (q, or q1 when its ``cur`` is 1; links of inactive
+// scenarios are skipped), and the links are [range[0], range[1]) of the
+// arrays, read from device memory, taken grid-stride by n workgroups.
 struct RegridArgs {
     const PatchDesc* desc;
     double* q;
@@ -62,6 +68,9 @@ struct RegridArgs {
     const int* igrid;  // and the fine neighbour's half along a face
     int n;
     double dry_tolerance;
+    double* q1;             // resident mode: the other q buffer
+    const ScenCtrl* ctrl;   // resident mode: per-scenario current buffer
+    const int* range;       // nullable: links [range[0], range[1]) instead of [0, n)
 };
 
 // Time interpolation of the two interior border layers: slot dst[k] of qdst
@@ -93,6 +102,7 @@ struct InflowArgs {
     int buf;
     double t;
     Params P;
+    const int* range;  // nullable: entries [range[0], range[1]) of list (one level) instead of [0, *list_n)
 };
 
 // Dry-list entries carry the patch's current buffer: patch | cur << 30.
@@ -159,6 +169,10 @@ struct TagArgs {
     int level, maxlevel, init_flag;
 };
 hipError_t launch_tag(int mx, int my, bool coarsen, const TagArgs& a, hipStream_t st);
+// Surface-gradient tagging: out[k] = 1 when an interior cell of patch
+// patches[k] (buffer q) has h > drytol or |grad eta| > eta_tol (centred
+// differences of eta = h + b, neighbours read as the halo pull reads them).
+hipError_t launch_tag_eta_gradient(int mx, int my, const TagArgs& a, double eta_tol, hipStream_t st);
 
 // A step variant: patch shape, tile shape, threads per workgroup.
 struct StepVariant {
@@ -195,9 +209,12 @@ hipError_t launch_step(int mx, int my, int variant, const StepArgs& a, hipStream
 hipError_t launch_ghost(int mx, int my, const GhostArgs& a, hipStream_t st);
 hipError_t launch_control(ScenCtrl* ctrl, int nscen, const double* cfl, hipStream_t st);
 hipError_t launch_inflow(int mx, int my, const InflowArgs& a, int grid, hipStream_t st);
-hipError_t launch_pack(const double* stage_q, const double* stage_aux, double* q, double* aux,
+// q2 (nullable): a second q buffer that receives the same state
+hipError_t launch_pack(const double* stage_q, const double* stage_aux, double* q, double* q2, double* aux,
                        int first, int n, int np2, int maux, hipStream_t st);
 hipError_t launch_unpack(double* stage_q, const double* q, int first, int n, int np2, hipStream_t st);
+// q (one buffer) and aux of slot src[k] to slot dst[k]
+hipError_t launch_move_slots(double* q, double* aux, const int* src, const int* dst, int n, int np2, hipStream_t st);
 hipError_t launch_max(const double* v, int n, double* out, hipStream_t st);
 
 }  // namespace gf

@@ -25,33 +25,31 @@
 
 #include <cstddef>
 
+#include "gf_ghost.h"
 #include "gf_inflow.h"
 #include "gf_launch.h"
 
 namespace gf {
 
-template <int MX, int MY>
-struct Geo {
-    static constexpr int NXP = MX + 2 * MBC;
-    static constexpr int NYP = MY + 2 * MBC;
-    static constexpr int NP2 = NXP * NYP;
-    static constexpr int PX = MX + 3;
-    static constexpr int PY = MY + 3;
-    static constexpr int NPOS = PX * PY;
-    // x-face flux increments kept in LDS between the x and y phases:
-    // faces I in [1, MX+1] of rows j in [1, MY]
-    static constexpr int NF = (MX + 1) * MY;
-    __device__ static constexpr int fi_(int I, int j) { return (j - 1) * (MX + 1) + (I - 1); }
-    // LDS index of Fortran cell (fi, fj), fi in [-1, MX+2]
-    __device__ static constexpr int ti(int fi, int fj) { return (fj + 1) * NXP + (fi + 1); }
-};
+
+// Maximum in which a NaN operand wins (fmax would drop it), so that a
+// non-finite state reaches the scenario's controller.  +0 wins over -0.
+__device__ __forceinline__ double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
+
+__device__ __forceinline__ double quiet_nan() { return __longlong_as_double(0x7ff8000000000000ll); }
+
+// The three values are finite (no NaN, no Inf).
+__device__ __forceinline__ bool finite3(double a, double b, double c)
+{
+    return (a - a == 0.0) && (b - b == 0.0) && (c - c == 0.0);
+}
 
 __device__ __forceinline__ double wave_max(double v)
 {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {
         const double o = __shfl_xor(v, off, 64);
-        v = fmaxf64(v, o);
+        v = max_nan(v, o);
     }
     return v;
 }
@@ -172,70 +170,6 @@ __device__ __forceinline__ void cell_transverse(const Rpt2Cell& C, const double 
     }
 }
 
-// Source of ghost cell (fi, fj) of a patch (Fortran indices, outside the
-// interior): the pool index and cell whose value ForestClaw's same-level face
-// copy, corner copy and bc2 passes (left, right, bottom, top over full
-// extents; geoflood_bc2_fort.f90:118-131 and twins) leave there, and whether
-// hu / hv change sign.  The cell is another patch's interior cell, or, for a
-// corner between a physical side and a stored side, the patch's own stored
-// ghost cell that bc2 mirrors (the stored strips are filled first: the
-// coarse/fine transfers run before the physical boundaries, as in
-// ForestClaw's ghost update).  Returns false when the ghost is read from the
-// patch's own ghost cells: supplied by the host or a coarse/fine transfer
-// (NB_STORED), written by k_inflow (an NB_INFLOW side or a corner next to
-// one), or a corner with no rule (two stored sides).
-template <int MX, int MY>
-__device__ __forceinline__ bool ghost_source(const int* __restrict__ nbr, int patch, int fi, int fj,
-                                             int& sp, int& si, int& sj, bool& neg_u, bool& neg_v)
-{
-    const int ox = fi < 1 ? -1 : (fi > MX ? 1 : 0);
-    const int oy = fj < 1 ? -1 : (fj > MY ? 1 : 0);
-    const int nx = ox < 0 ? nbr[0] : (ox > 0 ? nbr[1] : 0);
-    const int ny = oy < 0 ? nbr[2] : (oy > 0 ? nbr[3] : 0);
-    const bool px = (nx == NB_WALL || nx == NB_EXTRAP);
-    const bool py = (ny == NB_WALL || ny == NB_EXTRAP);
-    const int txv = (nx == NB_WALL) ? (ox < 0 ? 1 - fi : 2 * MX + 1 - fi) : (ox < 0 ? 1 : MX);
-    const int tyv = (ny == NB_WALL) ? (oy < 0 ? 1 - fj : 2 * MY + 1 - fj) : (oy < 0 ? 1 : MY);
-    neg_u = false;
-    neg_v = false;
-    if (ox != 0 && oy != 0) {
-        const int nc = oy < 0 ? (ox < 0 ? nbr[4] : nbr[5]) : (ox < 0 ? nbr[6] : nbr[7]);
-        if (nx == NB_INFLOW || ny == NB_INFLOW) {
-            return false;  // bc2's inflow side pass overwrites the corner; k_inflow stores it
-        } else if (nc >= 0) {
-            sp = nc; si = fi - ox * MX; sj = fj - oy * MY;
-        } else if (px && py) {
-            sp = patch; si = txv; sj = tyv;
-            neg_u = (nx == NB_WALL); neg_v = (ny == NB_WALL);
-        } else if (px && ny >= 0) {
-            sp = ny; si = txv; sj = fj - oy * MY;
-            neg_u = (nx == NB_WALL);
-        } else if (py && nx >= 0) {
-            sp = nx; si = fi - ox * MX; sj = tyv;
-            neg_v = (ny == NB_WALL);
-        } else if (px && ny == NB_STORED) {
-            // bc2's x pass runs over the full extent after the stored strip
-            // was filled (a coarse/fine transfer or the host): its mirror
-            sp = patch; si = txv; sj = fj;
-            neg_u = (nx == NB_WALL);
-        } else if (py && nx == NB_STORED) {
-            sp = patch; si = fi; sj = tyv;
-            neg_v = (ny == NB_WALL);
-        } else {
-            return false;
-        }
-    } else if (ox != 0) {
-        if (nx >= 0) { sp = nx; si = fi - ox * MX; sj = fj; }
-        else if (px) { sp = patch; si = txv; sj = fj; neg_u = (nx == NB_WALL); }
-        else return false;
-    } else {
-        if (ny >= 0) { sp = ny; si = fi; sj = fj - oy * MY; }
-        else if (py) { sp = patch; si = fi; sj = tyv; neg_v = (ny == NB_WALL); }
-        else return false;
-    }
-    return true;
-}
-
 // Block-wide maximum (result valid in thread 0).
 template <int NT>
 __device__ __forceinline__ double block_max(double v, double* red)
@@ -247,7 +181,7 @@ __device__ __forceinline__ double block_max(double v, double* red)
     __syncthreads();
     double r = red[0];
 #pragma unroll
-    for (int k = 1; k < NWAVE; k++) r = fmaxf64(r, red[k]);
+    for (int k = 1; k < NWAVE; k++) r = max_nan(r, red[k]);
     return r;
 }
 
@@ -269,8 +203,8 @@ __device__ __forceinline__ void block_max2(double& a, double& b, double* red)
     b = red[NWAVE];
 #pragma unroll
     for (int k = 1; k < NWAVE; k++) {
-        a = fmaxf64(a, red[k]);
-        b = fmaxf64(b, red[NWAVE + k]);
+        a = max_nan(a, red[k]);
+        b = max_nan(b, red[NWAVE + k]);
     }
 }
 
@@ -386,6 +320,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         // and transverse splits are zero and the update leaves q unchanged, so
         // the step reduces to b4step2 + src2 on the interior and CFL = 0.
         double hm = 0.0;
+        bool bad = false;
         for (int c = tid; c < TX * TY; c += NT) {
             const int i = c % TX + 1, j = c / TX + 1;
             const int k = G::ti(i, j);
@@ -396,10 +331,14 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
             dst[PNP2 + pk] = hu;
             dst[2 * PNP2 + pk] = hv;
             hm = fmaxf64(hm, h);
+            bad |= !finite3(h, hu, hv);
         }
+        // a non-finite cell (a NaN depth is neither wet nor dry) makes the
+        // tile's CFL number NaN, which stops the scenario
+        if (bad) hm = quiet_nan();
         hm = block_max<NT>(hm, red);
         if (tid == 0) {
-            cfl_res = 0.0;
+            cfl_res = hm != hm ? quiet_nan() : 0.0;
             hmax_res = hm;
         }
         return;
@@ -683,6 +622,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
 
     // ================= conservative update + src2 =================
     double hm = 0.0;
+    bool bad = false;
 #pragma unroll
     for (int s = 0; s < SLOTS; s++) {
         const int i = pi[s], j = pj[s];
@@ -702,12 +642,14 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
             dst[PNP2 + pk] = qn[1];
             dst[2 * PNP2 + pk] = qn[2];
             hm = fmaxf64(hm, qn[0]);
+            bad |= !finite3(qn[0], qn[1], qn[2]);
         }
     }
 
     // ================= CFL and depth reductions =================
-    // (the per-patch MAXCFL_CAP is applied by the caller, once per patch)
-    double v = cfl, hmx = hm;
+    // (the per-patch MAXCFL_CAP is applied by the caller, once per patch; a
+    // non-finite new state makes the CFL number NaN, which the cap keeps)
+    double v = bad ? quiet_nan() : cfl, hmx = hm;
     block_max2<NT>(v, hmx, red);
     if (tid == 0) {
         cfl_res = v;
@@ -943,6 +885,10 @@ __device__ __forceinline__ void dry_item(const BatchArgs* pa, int k, int next, i
             hu = src[NP2 + kk[r]];
             hv = src[2 * NP2 + kk[r]];
             if (P.src_term > 0) src2_cell(h, hu, hv, a.aux[(size_t)pt * NP2 + kk[r]], c->dt, P);
+            // a NaN depth lands here too: it sets the scenario's CFL maximum
+            // to NaN, which stops the scenario at the controller
+            if (!finite3(h, hu, hv))
+                atomicMax(a.cflbits + a.desc[pt].scen, (unsigned long long)__double_as_longlong(quiet_nan()));
         }
         dst[kk[r]] = h;
         dst[NP2 + kk[r]] = hu;
@@ -1123,8 +1069,8 @@ __global__ void __launch_bounds__(256) k_inflow(InflowArgs a)
     using G = Geo<MX, MY>;
     constexpr int NXP = G::NXP, NP2 = G::NP2;
     __shared__ double L[4 * NP2];
-    const int n = *a.list_n;
-    for (int k = blockIdx.x; k < n; k += gridDim.x) {
+    const int lo = a.range ? a.range[0] : 0, n = a.range ? a.range[1] : *a.list_n;
+    for (int k = lo + (int)blockIdx.x; k < n; k += gridDim.x) {
         const int patch = a.list[k];
         if (patch < a.first || patch >= a.last) continue;
         const PatchDesc* dp = a.desc + patch;
@@ -1186,6 +1132,33 @@ __global__ void __launch_bounds__(256) k_inflow(InflowArgs a)
 // step (flip the q buffer, t += dt) or rejects it (retake), and sets the next
 // dt = dt * desired_cfl / maxcfl.
 // ---------------------------------------------------------------------------
+// One decision of the outstyle_3 loop (fc2d_geoclaw_run.c:529-580) for a
+// scenario whose attempt gave CFL number m: retake with a smaller dt when m
+// exceeds max_cfl, else accept (t += dt, flip the buffer) and choose the next
+// dt.  A non-finite m (a NaN state or wave speed) stops the scenario with
+// status SCEN_NONFINITE and leaves t, dt and the accepted buffer as they
+// were; the reference has no such check.
+__device__ __forceinline__ void control_step(ScenCtrl* c, double m)
+{
+    c->maxcfl = m;
+    c->attempts += 1;
+    if (!cfl_finite(m)) {
+        c->status = SCEN_NONFINITE;
+        c->active = 0;
+    } else if (m > c->max_cfl) {
+        c->retakes += 1;
+        c->dt = c->dt * c->desired_cfl / m;
+    } else {
+        c->t = c->t + c->dt;
+        c->last_dt = c->dt;
+        c->cur ^= 1;
+        c->nsteps += 1;
+        c->dt = next_dt(c->dt, c->desired_cfl, m, c->dt_max);
+        if ((c->target_steps > 0 && c->nsteps >= c->target_steps) || (c->t_end > 0.0 && c->t >= c->t_end))
+            c->active = 0;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_control(ScenCtrl* ctrl, int nscen, const double* cfl)
 {
     const int s = blockIdx.x;
@@ -1194,28 +1167,14 @@ __global__ void __launch_bounds__(256) k_control(ScenCtrl* ctrl, int nscen, cons
     if (!c->active) return;
     __shared__ double red[4];
     double v = 0.0;
-    for (int k = threadIdx.x; k < c->npatch; k += blockDim.x) v = fmaxf64(v, cfl[c->first_patch + k]);
+    for (int k = threadIdx.x; k < c->npatch; k += blockDim.x) v = max_nan(v, cfl[c->first_patch + k]);
     v = wave_max(v);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = red[0];
-        for (int k = 1; k < (int)(blockDim.x >> 6); k++) m = fmaxf64(m, red[k]);
-        c->maxcfl = m;
-        c->attempts += 1;
-        if (m > c->max_cfl) {
-            c->retakes += 1;
-            c->dt = c->dt * c->desired_cfl / m;
-        } else {
-            c->t = c->t + c->dt;
-            c->last_dt = c->dt;
-            c->cur ^= 1;
-            c->nsteps += 1;
-            c->dt = c->dt * c->desired_cfl / m;
-            if ((c->target_steps > 0 && c->nsteps >= c->target_steps) ||
-                (c->t_end > 0.0 && c->t >= c->t_end))
-                c->active = 0;
-        }
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) m = max_nan(m, red[k]);
+        control_step(c, m);
     }
 }
 
@@ -1277,7 +1236,7 @@ __global__ void __launch_bounds__(256) k_classify(BatchArgs a)
 
 // Controller of the batched path: the per-scenario CFL maximum was folded by
 // atomicMax in the step kernels; apply the dt rule (fc2d_geoclaw_run.c:
-// 529-565), clear the maxima and the list counters for the next attempt.
+// 529-580), clear the maxima and the list counters for the next attempt.
 __global__ void __launch_bounds__(64) k_batch_control(BatchArgs a)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1293,21 +1252,7 @@ __global__ void __launch_bounds__(64) k_batch_control(BatchArgs a)
     if (!c->active) return;
     const double m = __longlong_as_double((long long)a.cflbits[s]);
     a.cflbits[s] = 0ull;
-    c->maxcfl = m;
-    c->attempts += 1;
-    if (m > c->max_cfl) {
-        c->retakes += 1;
-        c->dt = c->dt * c->desired_cfl / m;
-    } else {
-        c->t = c->t + c->dt;
-        c->last_dt = c->dt;
-        c->cur ^= 1;
-        c->nsteps += 1;
-        c->dt = c->dt * c->desired_cfl / m;
-        if ((c->target_steps > 0 && c->nsteps >= c->target_steps) ||
-            (c->t_end > 0.0 && c->t >= c->t_end))
-            c->active = 0;
-    }
+    control_step(c, m);
 }
 
 // Maximum interior depth of patches [first, first+n) of a q buffer.
@@ -1616,18 +1561,43 @@ __device__ __forceinline__ void half_origin(int side, int igrid, int& ox, int& o
 
 // fc2d_geoclaw_fort_average_face / _average_corner (fc2d_geoclaw_average_fort.f:
 // 33-480), mcapa = 0: the coarse ghost cells the fine neighbour covers.
+// The buffer of link l: a.q, or in the resident mode the current buffer of
+// the coarse patch's scenario (nullptr: an inactive scenario, skipped).
+__device__ __forceinline__ double* link_buffer(const RegridArgs& a, int l)
+{
+    if (!a.ctrl) return a.q;
+    const int sc = a.desc[a.cs[l]].scen;
+    if (sc < 0) return nullptr;
+    const ScenCtrl& c = a.ctrl[sc];
+    if (!c.active) return nullptr;
+    return c.cur ? a.q1 : a.q;
+}
+
+template <int MX, int MY>
+__device__ __forceinline__ void average_link(const RegridArgs& a, int l, double* q);
+template <int MX, int MY>
+__device__ __forceinline__ void interpolate_link(const RegridArgs& a, int l);
+
 template <int MX, int MY>
 __global__ void __launch_bounds__(64) k_average_ghost(RegridArgs a)
 {
+    const int lo = a.range ? a.range[0] : 0, hi = a.range ? a.range[1] : a.n;
+    for (int l = lo + (int)blockIdx.x; l < hi; l += gridDim.x) {
+        double* q = link_buffer(a, l);
+        if (q) average_link<MX, MY>(a, l, q);
+    }
+}
+
+template <int MX, int MY>
+__device__ __forceinline__ void average_link(const RegridArgs& a, int l, double* q)
+{
     using G = Geo<MX, MY>;
-    const int l = blockIdx.x;
-    if (l >= a.n) return;
     const int side = a.side[l], igrid = a.igrid[l];
     int ox, oy;
     half_origin<MX, MY>(side, igrid, ox, oy);
-    const double* qf = a.q + (size_t)a.fs[l] * 3 * G::NP2;
+    const double* qf = q + (size_t)a.fs[l] * 3 * G::NP2;
     const double* af = a.aux + (size_t)a.fs[l] * G::NP2;
-    double* qc = a.q + (size_t)a.cs[l] * 3 * G::NP2;
+    double* qc = q + (size_t)a.cs[l] * 3 * G::NP2;
     const double* ac = a.aux + (size_t)a.cs[l] * G::NP2;
     const int ncell = side >= 4 ? MBC * MBC : MBC * (side < 2 ? MY : MX);
     for (int c = threadIdx.x; c < ncell; c += 64) {
@@ -1670,9 +1640,20 @@ __global__ void __launch_bounds__(64) k_average_ghost(RegridArgs a)
 template <int MX, int MY>
 __global__ void __launch_bounds__(64) k_interpolate_ghost(RegridArgs a)
 {
+    const int lo = a.range ? a.range[0] : 0, hi = a.range ? a.range[1] : a.n;
+    for (int l = lo + (int)blockIdx.x; l < hi; l += gridDim.x) {
+        double* q = link_buffer(a, l);
+        if (!q) continue;
+        RegridArgs b = a;  // the per-cell helpers read the state through b.q
+        b.q = q;
+        interpolate_link<MX, MY>(b, l);
+    }
+}
+
+template <int MX, int MY>
+__device__ __forceinline__ void interpolate_link(const RegridArgs& a, int l)
+{
     using G = Geo<MX, MY>;
-    const int l = blockIdx.x;
-    if (l >= a.n) return;
     const int side = a.side[l], igrid = a.igrid[l];
     int ox, oy;
     half_origin<MX, MY>(side, igrid, ox, oy);
@@ -1762,16 +1743,19 @@ __global__ void __launch_bounds__(256) k_hmax_list(const double* q, double* hmax
 // patch.  ``stage`` holds n consecutive AoS patches.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_pack(const double* stage_q, const double* stage_aux,
-                                              double* q, double* aux, int first, int n,
+                                              double* q, double* q2, double* aux, int first, int n,
                                               int np2, int maux)
 {
     const int b = blockIdx.x;
     if (b >= n) return;
     const double* sq = stage_q + (size_t)b * 3 * np2;
     double* dq = q + (size_t)(first + b) * 3 * np2;
+    double* dq2 = q2 ? q2 + (size_t)(first + b) * 3 * np2 : nullptr;
     for (int e = threadIdx.x; e < 3 * np2; e += blockDim.x) {
         const int c = e / 3, m = e - 3 * c;
-        dq[(size_t)m * np2 + c] = sq[e];
+        const double v = sq[e];
+        dq[(size_t)m * np2 + c] = v;
+        if (dq2) dq2[(size_t)m * np2 + c] = v;
     }
     if (stage_aux) {
         const double* sa = stage_aux + (size_t)b * maux * np2;
@@ -1789,6 +1773,20 @@ __global__ void __launch_bounds__(256) k_copy_slots(const double* __restrict__ s
     if (b >= n) return;
     const size_t base = (size_t)slots[b] * 3 * np2;
     for (int e = threadIdx.x; e < 3 * np2; e += blockDim.x) dst[base + e] = src[base + e];
+}
+
+// State (q of one buffer, ghost cells included) and aux of slot src[b] to
+// slot dst[b], one workgroup per slot (the destinations are distinct and no
+// destination is a source, so the order does not matter).
+__global__ void __launch_bounds__(256) k_move_slots(double* __restrict__ q, double* __restrict__ aux,
+                                                    const int* __restrict__ src, const int* __restrict__ dst,
+                                                    int n, int np2)
+{
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    const size_t s = (size_t)src[b], d = (size_t)dst[b];
+    for (int e = threadIdx.x; e < 3 * np2; e += blockDim.x) q[d * 3 * np2 + e] = q[s * 3 * np2 + e];
+    for (int e = threadIdx.x; e < np2; e += blockDim.x) aux[d * np2 + e] = aux[s * np2 + e];
 }
 
 __global__ void __launch_bounds__(256) k_unpack(double* stage_q, const double* q, int first,
@@ -1810,13 +1808,13 @@ __global__ void __launch_bounds__(256) k_max(const double* v, int n, double* out
 {
     __shared__ double red[4];
     double m = 0.0;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) m = fmaxf64(m, v[k]);
+    for (int k = threadIdx.x; k < n; k += blockDim.x) m = max_nan(m, v[k]);
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double r = red[0];
-        for (int k = 1; k < 4; k++) r = fmaxf64(r, red[k]);
+        for (int k = 1; k < 4; k++) r = max_nan(r, red[k]);
         out[0] = r;
     }
 }
@@ -2111,11 +2109,18 @@ hipError_t launch_control(ScenCtrl* ctrl, int nscen, const double* cfl, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_pack(const double* stage_q, const double* stage_aux, double* q, double* aux,
+hipError_t launch_pack(const double* stage_q, const double* stage_aux, double* q, double* q2, double* aux,
                        int first, int n, int np2, int maux, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
-    k_pack<<<n, 256, 0, st>>>(stage_q, stage_aux, q, aux, first, n, np2, maux);
+    k_pack<<<n, 256, 0, st>>>(stage_q, stage_aux, q, q2, aux, first, n, np2, maux);
+    return hipGetLastError();
+}
+
+hipError_t launch_move_slots(double* q, double* aux, const int* src, const int* dst, int n, int np2, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_move_slots<<<n, 256, 0, st>>>(q, aux, src, dst, n, np2);
     return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@
 // HBM-bound pass of 32 B per cell.
 #include <hip/hip_runtime.h>
 
+#include "gf_ghost.h"
 #include "gf_launch.h"
 
 namespace gf {
@@ -98,7 +99,68 @@ __global__ void __launch_bounds__(256) k_tag_coarsen(TagArgs a)
     if (threadIdx.x == 0) a.out[k] = f ? 0 : 1;
 }
 
+// Surface-gradient criterion of an overland-flood run (no counterpart in the
+// reference's flag2refine): interior cell (i, j) flags when it is wet
+// (h > drytol) or when the centred differences of eta = h + b give
+//   ((eta(i+1,j) - eta(i-1,j)) / (2 dx))^2 + ((eta(i,j+1) - eta(i,j-1)) / (2 dy))^2 > tol2,
+// tol2 = |grad eta| threshold squared.  A neighbour beyond the interior has
+// its h where the halo pull reads it (a neighbour's interior, a wall mirror,
+// or the patch's own stored ghost on stored and inflow sides) and its b from
+// the patch's own aux ghost cells, as the step reads them.
+template <int MX, int MY>
+__device__ __forceinline__ double eta_at(const TagArgs& a, const PatchDesc& d, int patch, int fi, int fj)
+{
+    using G = Geo<MX, MY>;
+    const double* q = a.q + (size_t)patch * 3 * G::NP2;
+    const double b = a.aux[(size_t)patch * G::NP2 + G::ti(fi, fj)];
+    if (fi >= 1 && fi <= MX && fj >= 1 && fj <= MY) return q[G::ti(fi, fj)] + b;
+    int sp, si, sj;
+    bool nu, nv;
+    if (ghost_source<MX, MY>(d.nbr, patch, fi, fj, sp, si, sj, nu, nv))
+        return a.q[(size_t)sp * 3 * G::NP2 + G::ti(si, sj)] + b;
+    return q[G::ti(fi, fj)] + b;
+}
+
+template <int MX, int MY>
+__global__ void __launch_bounds__(256) k_tag_eta_grad(TagArgs a, double tol2)
+{
+    using G = Geo<MX, MY>;
+    const int k = blockIdx.x;
+    if (k >= a.n) return;
+    const int patch = a.patches[k];
+    const PatchDesc d = a.desc[patch];
+    const double* q = a.q + (size_t)patch * 3 * G::NP2;
+    int any = 0;
+    for (int c = threadIdx.x; c < MX * MY; c += blockDim.x) {
+        const int i = c % MX + 1, j = c / MX + 1;
+        if (q[G::ti(i, j)] > a.drytol) {
+            any = 1;
+            continue;
+        }
+        const double gx = (eta_at<MX, MY>(a, d, patch, i + 1, j) - eta_at<MX, MY>(a, d, patch, i - 1, j)) / (2.0 * d.dx);
+        const double gy = (eta_at<MX, MY>(a, d, patch, i, j + 1) - eta_at<MX, MY>(a, d, patch, i, j - 1)) / (2.0 * d.dy);
+        if (gx * gx + gy * gy > tol2) any = 1;
+    }
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) a.out[k] = any ? 1 : 0;
+}
+
 }  // namespace
+
+hipError_t launch_tag_eta_gradient(int mx, int my, const TagArgs& a, double eta_tol, hipStream_t st)
+{
+    if (a.n <= 0) return hipSuccess;
+    if (mx != my) return hipErrorInvalidValue;
+    const double tol2 = eta_tol * eta_tol;
+    switch (mx) {
+    case 8: k_tag_eta_grad<8, 8><<<a.n, 256, 0, st>>>(a, tol2); break;
+    case 16: k_tag_eta_grad<16, 16><<<a.n, 256, 0, st>>>(a, tol2); break;
+    case 32: k_tag_eta_grad<32, 32><<<a.n, 256, 0, st>>>(a, tol2); break;
+    case 64: k_tag_eta_grad<64, 64><<<a.n, 256, 0, st>>>(a, tol2); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_tag(int mx, int my, bool coarsen, const TagArgs& a, hipStream_t st)
 {

@@ -200,7 +200,9 @@ class Pool:
         q = np.ascontiguousarray(q, dtype=np.float64)
         n = q.shape[0]
         a = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
-        check(self.lib.gf_pool_upload(self.h, first, n, buf, dptr(q), None if a is None else dptr(a), 1),
+        # aux fields per cell: the last axis of a (n, ny, nx, maux) array (b is field 0)
+        maux = 1 if a is None or a.ndim < 4 else int(a.shape[-1])
+        check(self.lib.gf_pool_upload(self.h, first, n, buf, dptr(q), None if a is None else dptr(a), maux),
               "gf_pool_upload")
 
     def download_aos(self, first: int, n: int, buf: int = 0) -> np.ndarray:
@@ -249,6 +251,16 @@ class Pool:
     def set_friction(self, sid: int, manning_n: float):
         check(self.lib.gf_pool_set_scenario_friction(self.h, sid, manning_n),
               "gf_pool_set_scenario_friction")
+
+    def set_stop(self, sid: int, target_steps: int = 0, t_end: float = 0.0):
+        """New stop condition; the scenario runs again when it has not reached
+        it (gf_pool_set_scenario_stop)."""
+        check(self.lib.gf_pool_set_scenario_stop(self.h, sid, target_steps, t_end), "gf_pool_set_scenario_stop")
+
+    def set_dt_max(self, sid: int, dt_max: float):
+        """Upper bound of the scenario's dt (0: none); with a zero CFL number
+        the next dt is dt_max (gf_pool_set_scenario_dt_max)."""
+        check(self.lib.gf_pool_set_scenario_dt_max(self.h, sid, dt_max), "gf_pool_set_scenario_dt_max")
 
     def set_inflow(self, sid: int, time, hu, b: float, x0: float, y0: float):
         """Hydrograph of the scenario's inflow sides (gf_pool_set_scenario_inflow)."""
@@ -396,6 +408,39 @@ class Pool:
         d = np.ascontiguousarray(np.asarray(dst, dtype=np.int32).reshape(-1))
         check(self.lib.gf_pool_timeinterp(self.h, len(s), iptr(s), buf_last, buf_curr, alpha, iptr(d), dst_buf),
               "gf_pool_timeinterp")
+
+    # -- resident AMR ---------------------------------------------------------------
+    def set_levels(self, first: int, levels):
+        lv = np.ascontiguousarray(levels, dtype=np.int32)
+        check(self.lib.gf_pool_set_levels(self.h, first, len(lv), iptr(lv)), "gf_pool_set_levels")
+
+    def set_cf_links(self, coarse, fine, side, igrid):
+        """Store the forest's coarse/fine links: every attempt of run() and
+        ghost_update() transfers over them (gf_pool_set_cf_links)."""
+        c, f, sd, ig = self._cf_links(coarse, fine, side, igrid)
+        n = len(c)
+        check(self.lib.gf_pool_set_cf_links(self.h, n, iptr(c), iptr(f), iptr(sd), iptr(ig)),
+              "gf_pool_set_cf_links")
+
+    def ghost_update(self, buf: int = -1, t: float = 0.0):
+        check(self.lib.gf_pool_ghost_update(self.h, buf, t), "gf_pool_ghost_update")
+
+    def move(self, src, dst, buf: int):
+        s_ = np.ascontiguousarray(src, dtype=np.int32)
+        d_ = np.ascontiguousarray(dst, dtype=np.int32)
+        assert s_.shape == d_.shape
+        check(self.lib.gf_pool_move(self.h, len(s_), iptr(s_), iptr(d_), buf), "gf_pool_move")
+
+    def set_scenario_slots(self, sid: int, first: int, n: int):
+        check(self.lib.gf_pool_set_scenario_slots(self.h, sid, first, n), "gf_pool_set_scenario_slots")
+        self.n = max(self.n, first + n)
+
+    def tag_eta_gradient(self, patches, eta_tol: float, buf: int = 0) -> np.ndarray:
+        pt = np.ascontiguousarray(patches, dtype=np.int32)
+        out = np.zeros(len(pt), dtype=np.int32)
+        check(self.lib.gf_pool_tag_eta_gradient(self.h, buf, eta_tol, iptr(pt), len(pt), iptr(out)),
+              "gf_pool_tag_eta_gradient")
+        return out
 
     # -- step variants ------------------------------------------------------------
     def variants(self):

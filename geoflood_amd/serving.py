@@ -34,8 +34,9 @@ from .scenarios import gaussian_sum_dem, sine_hydrograph
 MX = 32
 LENGTH = 2000.0      # every scenario covers the same 2 km x 2 km domain
 T_END = 1800.0       # simulated seconds per scenario
-T_START = 10.0       # start after the hydrograph has left zero (Q(0) = 0 gives CFL 0)
+T_START = 0.0        # every scenario starts dry at t = 0, where the hydrograph's Q(0) = 0
 DT0 = 1.0
+DT_MAX = 10.0        # dt bound: a step with no wave anywhere (CFL 0) is followed by dt = DT_MAX
 STRIP = 100.0        # inflow channel width (m), centred on the west edge
 RETUNE_FACTOR = 2.0  # re-run the autotuner when the live patches change by this factor
 
@@ -68,6 +69,8 @@ class Result:
     max_depth: float
     wet_cells: int
     volume: float
+    status: int = 0   # gf_scenario_info status: 0, or GF_SCEN_NONFINITE when it stopped on a NaN state
+
 
     @property
     def latency(self) -> float:
@@ -182,6 +185,7 @@ class Engine:
         self.pool.upload_aos(first, q, aux, buf=0)
         sid = self.pool.add_scenario(layout, dt0=DT0, desired_cfl=0.9, max_cfl=1.0, t_end=req.t_end,
                                      t0=T_START)
+        self.pool.set_dt_max(sid, DT_MAX)
         self.pool.set_friction(sid, req.manning)
         t, hu = sine_hydrograph(req.peak_q, STRIP)
         self.pool.set_inflow(sid, t, hu, STRIP, 0.0, 0.5 * LENGTH)
@@ -198,8 +202,8 @@ class Engine:
         self.alloc.release(lv.first, lv.req.npatch)
         self.cell_updates += int(info["nsteps"]) * lv.req.n * lv.req.n
         return Result(lv.req.rid, lv.req.n, lv.req.arrival, lv.admitted, now, int(info["nsteps"]),
-                      int(info["retakes"]), float(info["t"]), float(h.max()), int((h > 1e-3).sum()),
-                      float(h.sum() * dx * dx))
+                      int(info["retakes"]), float(info["t"]), float(np.nanmax(h)), int((h > 1e-3).sum()),
+                      float(np.nansum(h) * dx * dx), int(info["status"]))
 
     def poll(self, now: float) -> list[Result]:
         if not self.live:
@@ -261,7 +265,7 @@ class Engine:
 
 def summarize(results: list[Result], wall: float, cell_updates: int) -> dict:
     lat = np.array([r.latency for r in results])
-    out = dict(requests=len(results), wall_s=round(wall, 3),
+    out = dict(requests=len(results), failed=sum(1 for r in results if r.status != 0), wall_s=round(wall, 3),
                p50_latency_s=round(float(np.percentile(lat, 50)), 4) if len(lat) else None,
                p99_latency_s=round(float(np.percentile(lat, 99)), 4) if len(lat) else None,
                mean_latency_s=round(float(lat.mean()), 4) if len(lat) else None,

@@ -75,7 +75,15 @@ typedef struct gf_scenario_info {
     int attempts;    /* all attempts */
     int active;      /* 1 while stepping */
     int cur;         /* q buffer holding the accepted state */
+    int status;      /* GF_SCEN_OK, or why the scenario stopped */
 } gf_scenario_info;
+
+/* gf_scenario_info::status.  GF_SCEN_NONFINITE: an attempt produced a NaN or
+ * infinite state or CFL number; the scenario stopped without accepting it
+ * (t, dt and the accepted buffer are those before the attempt).  The
+ * reference has no such check (a NaN CFL number fails "maxcfl > max_cfl" in
+ * fc2d_geoclaw_run.c:529 and the step is accepted). */
+enum { GF_SCEN_OK = 0, GF_SCEN_NONFINITE = 1 };
 
 /* Refinement criteria of the tagging entry points (refinement_data_module,
  * geoclaw_module::sea_level and the regions common block of the reference:
@@ -241,9 +249,13 @@ int gf_pool_download_aux(gf_pool *pool, int first, int n, double *aux_aos);
 int gf_pool_ghost_fill(gf_pool *pool, int first, int n, int buf, double t);
 
 /* One step attempt of patches [first, first+n) from buffer `src` to buffer
- * `dst` (== src: in place) with time step dt, all patches in one launch.
- * cfl (optional, host, length n) receives the per-patch capped CFL numbers;
- * *maxcfl (optional) their maximum.  Ghost cells must already be filled. */
+ * `dst` with time step dt, all patches in one launch.  dst == src steps in
+ * place, which only a step variant with one tile per patch can do (its one
+ * workgroup holds the whole patch before writing); with a tiled variant
+ * (gf_pool_variants) it returns GF_EUNSUPPORTED.  cfl (optional, host,
+ * length n) receives the per-patch capped CFL numbers (NaN for a patch whose
+ * new state is not finite); *maxcfl (optional) their maximum, NaN if any is.
+ * Ghost cells must already be filled. */
 int gf_pool_step(gf_pool *pool, int first, int n, int src, int dst, double dt,
                  double *cfl, double *maxcfl);
 
@@ -286,6 +298,22 @@ int gf_pool_set_scenario_inflow(gf_pool *pool, int scenario, int nrows, const do
  * manning_n).  Replaces manning_coefficient(1) of friction_module
  * (fortran_source/geoclaw_src2_fort.f90:53) for one member of an ensemble. */
 int gf_pool_set_scenario_friction(gf_pool *pool, int scenario, double manning_n);
+
+/* New stop condition of a scenario: target_steps accepted steps and/or
+ * simulated time t_end (<= 0 disables each), as in gf_pool_add_scenario; the
+ * scenario is active again when it has not reached them (and has not
+ * stopped on a non-finite state).  A driver that regrids every few steps
+ * (fc2d_geoclaw_run.c:569-575) moves the target to the next regrid and
+ * replays attempts until the scenario stops there. */
+int gf_pool_set_scenario_stop(gf_pool *pool, int scenario, int target_steps, double t_end);
+
+/* Upper bound of a scenario's dt (0, the default: none).  After an accepted
+ * attempt with CFL number m the next dt is dt * desired_cfl / m
+ * (fc2d_geoclaw_run.c:577-580), capped at dt_max; with m = 0 (no wave: every
+ * cell dry, e.g. a dry start with a hydrograph at zero discharge) it is
+ * dt_max, or the same dt when no bound is set, where the reference's rule
+ * gives an infinite dt.  The reference's outstyle_3 has no bound. */
+int gf_pool_set_scenario_dt_max(gf_pool *pool, int scenario, double dt_max);
 
 /* Stop or restart a scenario without removing it. */
 int gf_pool_set_scenario_active(gf_pool *pool, int scenario, int active);
@@ -480,6 +508,64 @@ int gf_pool_copy(gf_pool *pool, int n, const int *slots, int src_buf, int dst_bu
  * buf_last, buf_curr). */
 int gf_pool_timeinterp(gf_pool *pool, int n, const int *src, int buf_last, int buf_curr, double alpha,
                        const int *dst, int dst_buf);
+
+/* ---- Resident AMR: levels in one graph per attempt --------------------------
+ * ForestClaw keeps a forest of leaves of several levels and, with subcycling
+ * off (every GeoFlood example: subcycle = F in fclaw_options.ini), advances
+ * all of them with one dt per step after a ghost update across levels
+ * (fc2d_geoclaw_run.c:505, fclaw2d_advance_all_levels); its regrid hooks
+ * refine and coarsen leaves between steps (fc2d_cpucuda.cpp:513-694,
+ * fc2d_geoclaw_run.c:569-575).  For a pool whose slots hold the leaves of
+ * such a forest, the entry points below keep the level structure on the
+ * device, so that each gf_pool_run attempt -- one hipGraph launch -- is the
+ * whole ghost update (fine -> coarse averaging over all links, then level by
+ * level from coarse to fine: the level's hydrograph sides and the
+ * interpolation to the next finer level) followed by one persistent step of
+ * every leaf of every level and the dt control.  The links and level ranges
+ * live in device memory, so a regrid changes them without a new graph. */
+
+/* Refinement level (0 .. 15) of slots [first, first+n); slots never set are
+ * level 0.  Orders the hydrograph sides of the ghost update by level.
+ * Replaces the level a ForestClaw patch carries (fclaw2d_patch_t::level). */
+int gf_pool_set_levels(gf_pool *pool, int first, int n, const int *level);
+
+/* The coarse/fine links of the forest (as for gf_pool_average_ghost: coarse
+ * slot, fine slot one level finer, side 0..7, igrid), replacing any stored
+ * before; n = 0 clears them.  Every gf_pool_run attempt and
+ * gf_pool_ghost_update then performs the transfers of fc2d_cpucuda.cpp:
+ * 698-857 (average_face/corner, interpolate_face/corner) over them. */
+int gf_pool_set_cf_links(gf_pool *pool, int n, const int *coarse, const int *fine, const int *side,
+                         const int *igrid);
+
+/* The multi-level ghost update of the stored links on buffer buf at time t
+ * (buf = -1: each scenario's current buffer and time), as each gf_pool_run
+ * attempt begins with it; for a regrid or an output between steps.  Replaces
+ * fclaw2d_ghost_update with time_interp off for resident patches. */
+int gf_pool_ghost_update(gf_pool *pool, int buf, double t);
+
+/* Copy the state (buffer buf, ghost cells included) and aux of slot src[k]
+ * to slot dst[k]: destinations distinct and no destination a source.  A
+ * regrid keeps unchanged leaves this way when it lays the new forest out in
+ * other slots.  Replaces ForestClaw's copy of a patch that survives a regrid
+ * (fclaw2d_patch_copy via the plugin's copy hook). */
+int gf_pool_move(gf_pool *pool, int n, const int *src, const int *dst, int buf);
+
+/* Re-target scenario s to slots [first, first+n) (described; free or its
+ * own), keeping its time, dt, buffer index, counters, friction and
+ * hydrograph: the slots it held before return to no scenario.  The new
+ * forest's state must already be in the scenario's current buffer there.
+ * Replaces ForestClaw's switch to the new domain after a regrid
+ * (fclaw2d_regrid_new_domain_setup) for a resident scenario. */
+int gf_pool_set_scenario_slots(gf_pool *pool, int scenario, int first, int n);
+
+/* tags[k] = 1 when an interior cell of slot patches[k] (buffer buf) is wet
+ * (h > dry_tolerance) or has |grad eta| > eta_tol, eta = h + b, by centred
+ * differences over the cell's four neighbours (read as the step reads its
+ * halo: run gf_pool_ghost_update first on stored and hydrograph sides).  The
+ * overland-flood runs' refinement criterion; it has no counterpart in the
+ * reference's flag2refine (fc2d_geoflood_flag2refine.f90:7-130), whose
+ * criteria gf_pool_tag_refine restates. */
+int gf_pool_tag_eta_gradient(gf_pool *pool, int buf, double eta_tol, const int *patches, int n, int *tags);
 
 /* Gauge files in the reference's format (geoclaw_create_gauge_files_default
  * and geoclaw_print_gauges_default, fc2d_geoclaw_gauges_default.c:172-200,

@@ -1319,6 +1319,19 @@ int gfo_brick_run_ex(const gfo_params *p, int mi, int mj, int mx, int my, int mb
                      gfo_update_fn fn, const void *ctx, int nthreads,
                      double *dt_out, double *cfl_out, double *mass_out)
 {
+    return gfo_brick_run_dtmax(p, mi, mj, mx, my, mbc, maux, q, aux, mthbc, xlo, ylo, dx, dy, h, t0,
+                               initial_dt, desired_cfl, max_cfl, 0.0, nsteps, fn, ctx, nthreads, dt_out,
+                               cfl_out, mass_out);
+}
+
+int gfo_brick_run_dtmax(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
+                        int maux, double **q, double **aux, const int mthbc[4],
+                        double xlo, double ylo, double dx, double dy,
+                        const gfo_hydrograph *h, double t0, double initial_dt,
+                        double desired_cfl, double max_cfl, double dt_max, int nsteps,
+                        gfo_update_fn fn, const void *ctx, int nthreads,
+                        double *dt_out, double *cfl_out, double *mass_out)
+{
     const int np = mi * mj;
     const size_t nq = (size_t)MEQN * (mx + 2 * mbc) * (my + 2 * mbc);
     double *save = (double *)malloc(sizeof(double) * nq * np);
@@ -1354,7 +1367,14 @@ int gfo_brick_run_ex(const gfo_params *p, int mi, int mj, int mx, int my, int mb
             mass_out[n] = mass;
         }
         t = t + dt;
-        dt = dt * desired_cfl / maxcfl;
+        if (dt_max > 0.0) {
+            /* the library's bound (no counterpart in the reference): a zero
+             * CFL number takes dt_max, any other is capped at it */
+            const double d = maxcfl > 0.0 ? dt * desired_cfl / maxcfl : dt;
+            dt = (maxcfl == 0.0 || d > dt_max) ? dt_max : d;
+        } else {
+            dt = dt * desired_cfl / maxcfl;
+        }
         n++;
     }
     free(save);

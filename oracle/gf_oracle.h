@@ -216,6 +216,18 @@ int gfo_brick_run_ex(const gfo_params *p, int mi, int mj, int mx, int my, int mb
                      gfo_update_fn fn, const void *ctx, int nthreads,
                      double *dt_out, double *cfl_out, double *mass_out);
 
+/* gfo_brick_run_ex with the library's dt bound (gf_pool_set_scenario_dt_max;
+ * no counterpart in the reference): with dt_max > 0 the next dt after an
+ * accepted step is dt * desired_cfl / maxcfl capped at dt_max, and dt_max
+ * itself when maxcfl = 0; dt_max = 0 is the reference's rule unchanged. */
+int gfo_brick_run_dtmax(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
+                        int maux, double **q, double **aux, const int mthbc[4],
+                        double xlo, double ylo, double dx, double dy,
+                        const gfo_hydrograph *h, double t0, double initial_dt,
+                        double desired_cfl, double max_cfl, double dt_max, int nsteps,
+                        gfo_update_fn fn, const void *ctx, int nthreads,
+                        double *dt_out, double *cfl_out, double *mass_out);
+
 /* ---- ASCII frame output and gauges ---------------------------------------
  * fc2d_geoclaw_output_ascii_fort.f and fc2d_geoclaw_gauges_default.c.  The
  * writers append bytes to out (capacity cap) and return the number written,

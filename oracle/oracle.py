@@ -447,9 +447,10 @@ class CLib(_Lib):
 
     # --- brick driver -------------------------------------------------------
     def brick_run(self, brick: "Brick", nsteps, initial_dt, desired_cfl=0.9, max_cfl=1.0,
-                  t0=0.0, update=None, nthreads=1):
+                  t0=0.0, update=None, nthreads=1, dt_max=0.0):
         """Run the outstyle_3 loop on a Brick in place.  ``update`` is None
-        (C restatement) or a RefLib (reference Fortran patch update)."""
+        (C restatement) or a RefLib (reference Fortran patch update).
+        dt_max > 0 adds the library's dt bound (gfo_brick_run_dtmax)."""
         np_ = brick.mi * brick.mj
         qp = (_dp * np_)(*[ptr(a) for a in brick.q])
         ap = (_dp * np_)(*[ptr(a) for a in brick.aux])
@@ -459,15 +460,15 @@ class CLib(_Lib):
         if update is not None:
             fnp = ct.cast(update.lib.gfref_patch_update, ct.c_void_p)
         h = None if brick.hydro is None else brick.hydro.to_c()
-        fn = self.lib.gfo_brick_run_ex
+        fn = self.lib.gfo_brick_run_dtmax
         fn.argtypes = [ct.c_void_p] + [ct.c_int] * 6 + [ct.POINTER(_dp), ct.POINTER(_dp), _ip] + \
-            [ct.c_double] * 4 + [ct.c_void_p] + [ct.c_double] * 4 + \
+            [ct.c_double] * 4 + [ct.c_void_p] + [ct.c_double] * 5 + \
             [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_int, _dp, _dp, _dp]
         fn.restype = ct.c_int
         retakes = fn(ct.byref(self._cp), brick.mi, brick.mj, brick.mx, brick.my, brick.mbc, 1,
                      qp, ap, iptr(mb), brick.xlo, brick.ylo, brick.dx, brick.dy,
                      None if h is None else ct.cast(ct.byref(h), ct.c_void_p),
-                     t0, initial_dt, desired_cfl, max_cfl,
+                     t0, initial_dt, desired_cfl, max_cfl, dt_max,
                      nsteps, fnp, None, nthreads, ptr(dt_out), ptr(cfl_out), ptr(mass_out))
         return dict(dt=dt_out, cfl=cfl_out, mass=mass_out, retakes=retakes)
 

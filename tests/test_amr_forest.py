@@ -1,0 +1,144 @@
+"""The resident-AMR forest (geoflood_amd.amr.Forest) and the oracle's AMR
+cycle on the CPU: a quadtree stand-in for ForestClaw's p4est forest.
+
+Checked against what a forest must satisfy by construction: the leaves
+partition the domain into aligned blocks; levels stay in [minlevel,
+maxlevel]; adjacent leaves (faces and corners) differ by at most one level
+(p4est's full 2:1 balance); a tagged leaf is refined, a family merges only
+when all four siblings agree and the balance allows; the neighbour table is
+symmetric; every finer leaf across a coarser leaf's side has a link; and in
+the oracle's cycle every ghost cell of every leaf is filled by the ghost
+update (a ghost left NaN fails), across refinement and coarsening."""
+import numpy as np
+import pytest
+
+from geoflood_amd import GF_NB_STORED
+from geoflood_amd.amr import DIRS, Forest, regrid_plan
+
+
+def _check(f: Forest):
+    M = f.maxlevel
+    assert f.lev.min() >= f.minlevel and f.lev.max() <= M
+    # partition: each finest patch in exactly one leaf, leaves aligned
+    cover = np.zeros_like(f.lev)
+    for L, oi, oj in zip(f.L, f.oi, f.oj):
+        s = 1 << (M - L)
+        assert oi % s == 0 and oj % s == 0
+        cover[oj:oj + s, oi:oi + s] += 1
+        assert (f.lev[oj:oj + s, oi:oi + s] == L).all()
+    assert (cover == 1).all()
+    # full balance
+    p = np.pad(f.lev, 1, constant_values=-100)
+    for dx, dy in DIRS:
+        nb = p[1 + dy:1 + dy + f.ny, 1 + dx:1 + dx + f.nx]
+        ok = nb >= 0
+        assert (np.abs(nb - f.lev)[ok] <= 1).all()
+
+
+def _random_forest(rng, steps=6):
+    f = Forest(3, 2, 1, 4)
+    for _ in range(steps):
+        ref = rng.random(len(f)) < 0.3
+        coa = rng.random(len(f)) < 0.6
+        f = f.adapt(ref, coa)
+        _check(f)
+    return f
+
+
+def test_forest_balance_and_partition():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        _random_forest(rng)
+
+
+def test_adapt_refines_tags_and_coarsens_whole_families():
+    f = Forest(2, 2, 0, 3)
+    g = f.adapt(np.array([True, False, False, False]), np.zeros(4, bool))
+    assert g.counts() == {0: 3, 1: 4}
+    # refining one level-1 leaf of the corner family forces nothing at level 0 (2:1 holds)
+    h = g.adapt(np.array([False] * 3 + [True, False, False, False]), np.zeros(7, bool))
+    _check(h)
+    assert h.counts() == {0: 3, 1: 3, 2: 4}
+    # three siblings agreeing are not enough to merge
+    k = np.flatnonzero(g.L == 1)
+    coa = np.zeros(len(g), bool)
+    coa[k[:3]] = True
+    assert g.adapt(np.zeros(len(g), bool), coa).same(g)
+    coa[k] = True
+    assert g.adapt(np.zeros(len(g), bool), coa).same(f)
+    # balance keeps a family from merging next to a finer leaf
+    coa = np.zeros(len(h), bool)
+    coa[h.L == 1] = True
+    assert h.adapt(np.zeros(len(h), bool), coa).counts() == h.counts()
+
+
+def test_tables_symmetric_and_links_complete():
+    rng = np.random.default_rng(1)
+    f = _random_forest(rng)
+    nbr, (lc, lf, ls, lg) = f.tables((3, 1, 3, 3))
+    opp = [1, 0, 3, 2, 7, 6, 5, 4]
+    for k in range(len(f)):
+        for d in range(8):
+            t = nbr[k, d]
+            if t >= 0:
+                assert nbr[t, opp[d]] == k and f.L[t] == f.L[k]
+    # a stored side of a leaf that has a coarser leaf across it is the fine end of a link
+    fine_sides = {(int(b), opp[int(s)]) for b, s in zip(lf, ls)}
+    for k in range(len(f)):
+        for d, (dx, dy) in enumerate(DIRS[:4]):
+            if nbr[k, d] != GF_NB_STORED:
+                continue
+            s = 1 << (f.maxlevel - f.L[k])
+            ni, nj = f.oi[k] + dx * s, f.oj[k] + dy * s
+            if f.lev[nj, ni] < f.L[k]:
+                assert (k, d) in fine_sides
+    assert (f.L[lf] == f.L[lc] + 1).all()
+
+
+def test_regrid_plan_accounts_for_every_leaf():
+    rng = np.random.default_rng(2)
+    f = _random_forest(rng, 3)
+    for _ in range(5):
+        g = f.adapt(rng.random(len(f)) < 0.3, rng.random(len(f)) < 0.6)
+        (kn, ko), refined, coarsened = regrid_plan(f, g)
+        covered = set(kn.tolist()) | {k for _, ks in refined for k in ks} | {p for p, _ in coarsened}
+        assert covered == set(range(len(g)))
+        assert all(g.L[k] == f.L[p] + 1 for p, ks in refined for k in ks)
+        assert all(f.L[k] == g.L[p] + 1 for p, ks in coarsened for k in ks)
+        f = g
+
+
+def band_flags(width=0.25, speed=0.15):
+    """Refine the leaves whose centre lies in a vertical band moving east one
+    step per regrid, coarsen the others: every regrid refines and coarsens."""
+    def flags(forest, tags, k):
+        x = (forest.i + 0.5) / (forest.nx0 << forest.L)
+        lo = (0.05 + speed * k) % 1.0
+        inb = (x >= lo) & (x < lo + width)
+        return inb, ~inb
+    return flags
+
+
+def test_oracle_amr_cycle_fills_every_ghost(oracle_c):
+    """A walled basin with a hump of water, levels 0-2 over a 2 x 2 brick of
+    8 x 8 patches, a refinement band moving across it: 24 steps, a regrid
+    every 2 (each refines and coarsens), every ghost update fully covered,
+    depths non-negative throughout."""
+    from oracle.amr_driver import OracleAmr
+
+    def dem(nx, ny, L):
+        x = (np.arange(nx) + 0.5) / nx
+        X, Y = np.meshgrid(x, x)
+        return 0.2 * np.exp(-((X - 0.6) ** 2 + (Y - 0.4) ** 2) / 0.02)
+
+    def state(L, i, j, aux):
+        q = np.zeros(aux.shape[:3] + (3,))
+        q[..., 0] = np.maximum(0.5 - aux[..., 0], 0.0)
+        return q
+
+    r = OracleAmr(oracle_c, 8, 2, 2, 1.0, 1.0, 0, 2, dem, eta_tol=1e9, regrid_interval=2,
+                  flags_fn=band_flags())
+    r.start(0.0, 1e-3, state_fn=state, init_levels=False)
+    r.run(24)
+    assert r.nregrids == 12 and len(set(r.patches)) > 2
+    assert (r.state()[:, 0] >= 0).all()

@@ -1,0 +1,132 @@
+"""Parity of the non-default methods on the device: first order, transverse
+order 0 and 1, and the other limiters (minmod, superbee, van Leer,
+Beam-Warming), which run the kernels instantiated with the method read from
+the parameters instead of fixed at compile time.  The oracle is pinned to
+the reference Fortran build for the same method sets
+(tests/test_oracle.py::test_vs_reference_method_variants); here the device
+is compared with it:
+
+  * the drop-in batch step (k_step) on random wet/dry patches, one step,
+    to the single-step bar (1e-13; src2's pow() may differ by an ulp);
+  * 20 accepted steps of a wet case (no front) and of a dry-bed dam break
+    through the batched graph path (k_all, halo pull, device dt control) and
+    the per-patch launch loop, to the run bar (dt sequence 1e-12, per cell
+    1e-10, identical mask).
+
+Method options: geoclaw_flux2fw_fort.f:186-224 (order, transverse),
+philim.f (limiters 1..5)."""
+import numpy as np
+import pytest
+
+from oracle.oracle import Brick, Params
+from geoflood_amd.pool import BrickLayout, Pool
+from _parity import assert_state_close
+
+pytestmark = pytest.mark.gpu
+
+METHODS = [(1, 0, (4, 4, 4)), (2, 1, (1, 1, 1)), (2, 2, (2, 3, 5)), (2, 0, (4, 0, 4))]
+
+
+def rel_err(a, b):
+    scale = np.maximum(np.abs(a), np.abs(b))
+    d = np.abs(a - b)
+    return np.max(np.where(scale > 0, d / np.where(scale > 0, scale, 1), 0.0))
+
+
+@pytest.fixture
+def method(request, gf, oracle_c):
+    order, transverse, mthlim = request.param
+    p = Params(order=order, transverse=transverse, mthlim=mthlim, manning_n=0.03)
+    oracle_c.set_params(p)
+    gf.set_params(order=order, transverse=transverse, mthlim=mthlim, manning_n=0.03)
+    yield p
+    oracle_c.set_params(Params())
+    gf.set_params()
+
+
+@pytest.mark.parametrize("method", METHODS, indirect=True, ids=lambda m: f"o{m[0]}t{m[1]}l{''.join(map(str, m[2]))}")
+@pytest.mark.parametrize("mx", [16, 32])
+def test_batch_step_methods(gf, oracle_c, method, mx):
+    rng = np.random.default_rng(100 * method.order + 10 * method.transverse + mx)
+    print(f"case methods step2_batch {method.order}/{method.transverse}/{method.mthlim} {mx}x{mx}", flush=True)
+    qs, auxs, ref = [], [], []
+    c_ref = 0.0
+    for k in range(6):
+        n = mx + 4
+        h = rng.random((n, n)) * [0.05, 1.0, 3.0][k % 3]
+        h[rng.random(h.shape) < [0.0, 0.3, 0.6][k % 3]] = 0.0
+        q = np.zeros((n, n, 3))
+        q[..., 0] = h
+        q[..., 1] = rng.normal(0, 1, h.shape) * h
+        q[..., 2] = rng.normal(0, 1, h.shape) * h
+        aux = np.ascontiguousarray(rng.normal(0, 0.5, (n, n, 1)))
+        qo = q.copy()
+        c_ref = max(c_ref, oracle_c.patch_update(mx, mx, 2, qo, aux, 0.1, 0.1, 0.0, 0.008))
+        qs.append(q); auxs.append(aux); ref.append(qo)
+    c = gf.step2_batch(qs, auxs, 0.1, 0.1, 0.0, 0.008)
+    assert c == pytest.approx(c_ref, rel=1e-15, abs=0)
+    assert c_ref > 0.05
+    for q, qo in zip(qs, ref):
+        assert rel_err(q[2:-2, 2:-2], qo[2:-2, 2:-2]) <= 1e-13
+        assert np.array_equal(q[2:-2, 2:-2, 0] > 1e-3, qo[2:-2, 2:-2, 0] > 1e-3)
+
+
+def _wet_bump(n=256):
+    """Wet everywhere (h >= 0.82): a hump of water beside a submerged bump on
+    a bed sloping in +x, walls, u = v = 0 -- no wet/dry front."""
+    dx = 1.0 / n
+    x = (np.arange(n) + 0.5) * dx
+    X, Y = np.meshgrid(x, x)
+    b = 0.3 * np.exp(-((X - 0.6) ** 2 + (Y - 0.4) ** 2) / 0.01) - 0.2 * X
+    h = np.maximum(1.0 + 0.5 * np.exp(-((X - 0.35) ** 2 + (Y - 0.6) ** 2) / 0.005) - b, 0.0)
+    z = np.zeros_like(h)
+    return h, z, z.copy(), b, dx, 1e-4
+
+
+def _dry_bed(n=256):
+    from geoflood_amd.scenarios import uniform_dam_break
+    h, hu, hv, b, dx = uniform_dam_break(n, dx=1.0, h0=10.0, rms=3.0, seed=5)
+    return h, hu, hv, b, dx, 0.01
+
+
+# The dry-bed dam break with the Beam-Warming limiter (2, 2, (2, 3, 5)) is
+# left out: that run is chaotic in the last bit -- raising every wet depth of
+# the oracle's own initial state by one ulp changes its dt by 2.4e-3 at step 3
+# -- so no two implementations whose pow() rounds differently can meet a
+# 1e-10 bar on it.  The TVD sets stay within 3.5e-15 under the same test, and
+# every set within 1.3e-12 on the wet case.
+CASES = [(c, m) for c in ("wet", "dry-bed") for m in METHODS if not (c == "dry-bed" and m[2] == (2, 3, 5))]
+
+
+@pytest.mark.parametrize("case,method", CASES, indirect=["method"],
+                         ids=[f"{c}-o{m[0]}t{m[1]}l{''.join(map(str, m[2]))}" for c, m in CASES])
+@pytest.mark.parametrize("mode", ["graph", "plain"])
+def test_run_20_steps_methods(gf, oracle_c, case, method, mode):
+    n, steps = 256, 20
+    print(f"case methods {case} {method.order}/{method.transverse}/{method.mthlim} {mode}", flush=True)
+    h, hu, hv, b, dx, dt0 = _wet_bump(n) if case == "wet" else _dry_bed(n)
+    br = Brick.from_global(h, hu, hv, b, n // 32, n // 32, 2, dx, dx)
+    rec = oracle_c.brick_run(br, steps, dt0, 0.9, 1.0, nthreads=8)
+    layout = BrickLayout(n // 32, n // 32, 32, 32, dx, dx)
+    pool = Pool(32, 32, layout.npatch)
+    try:
+        pool.set_layout([layout])
+        pool.upload_global(layout, h, hu, hv, b)
+        sid = pool.add_scenario(layout, dt0=dt0, target_steps=steps)
+        dts, prev = [], 0
+        while True:
+            active = pool.run(1, graph=True) if mode == "graph" else pool.run_plain(1)
+            info = pool.scenario(sid)
+            if info["nsteps"] > prev:
+                dts.append(info["last_dt"]); prev = info["nsteps"]
+            if not active:
+                break
+        assert info["retakes"] == rec["retakes"]
+        np.testing.assert_allclose(dts, rec["dt"], rtol=1e-12, atol=0)
+        got = np.stack(pool.download_global(layout, buf=info["cur"]))
+        ref = br.to_global()
+        assert_state_close(got, ref, 1e-10)
+        if case == "dry-bed":
+            assert (ref[0] > 1e-3).any() and (ref[0] <= 1e-3).any()
+    finally:
+        pool.close()

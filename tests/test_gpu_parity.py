@@ -347,3 +347,75 @@ def test_plugin_single_step_update_protocol(gf, oracle_c):
     assert flushes == 3
     assert m == pytest.approx(cref, rel=1e-15)
     assert rel_err(br.to_global(), ref.to_global()) <= RTOL_STEP
+
+
+def test_64x64_batch_step_in_tiles(gf, oracle_c):
+    """64x64 patches: the default variant steps each patch as four 32x32
+    tiles, each reading its 2-cell halo from the neighbouring tiles' cells.
+    The drop-in batch step reads one buffer and writes the other (never in
+    place), so every tile sees the state before the step: the result equals
+    the oracle's geoclaw_update."""
+    rng = np.random.default_rng(64)
+    print("case 64x64 step2_batch, 4 tiles per patch", flush=True)
+    qs, auxs, ref = [], [], []
+    c_ref = 0.0
+    for k in range(4):
+        q, aux = random_patch(rng, mx=64, my=64, wet_frac=[1.0, 0.7, 0.4, 0.9][k], topo=0.5,
+                              hscale=[1.0, 2.0, 5.0, 0.5][k])
+        qo = q.copy()
+        c_ref = max(c_ref, oracle_c.patch_update(64, 64, 2, qo, aux, 0.1, 0.1, 0.0, 0.006))
+        qs.append(q); auxs.append(aux); ref.append(qo)
+    given = [q.copy() for q in qs]
+    c = gf.step2_batch(qs, auxs, 0.1, 0.1, 0.0, 0.006)
+    assert c == pytest.approx(c_ref, rel=1e-15, abs=0)
+    ring = np.ones(qs[0].shape[:2], dtype=bool)
+    ring[2:-2, 2:-2] = False
+    for q, qo, q0 in zip(qs, ref, given):
+        assert rel_err(q[2:-2, 2:-2], qo[2:-2, 2:-2]) <= RTOL_STEP
+        # the ghost ring comes back as it was given
+        np.testing.assert_array_equal(q[ring], q0[ring])
+
+
+def test_in_place_step_only_untiled(gf, oracle_c):
+    """gf_pool_step with src == dst: a variant with one workgroup per patch
+    steps in place and equals the oracle; a variant that splits the patch
+    into tiles is refused (GF_EUNSUPPORTED), since its tiles would read cells
+    that other tiles have already overwritten; stepping into the other
+    buffer with that variant equals the oracle."""
+    from geoflood_amd import GeoFloodError
+    rng = np.random.default_rng(11)
+    n = 4
+    qs, auxs = [], []
+    for k in range(n):
+        q, aux = random_patch(rng, wet_frac=0.8, topo=0.5, hscale=2.0)
+        qs.append(q); auxs.append(aux)
+    qs, auxs = np.array(qs), np.array(auxs)
+    ref = [q.copy() for q in qs]
+    c_ref = max(oracle_c.patch_update(32, 32, 2, r, a, 0.1, 0.1, 0.0, 0.01) for r, a in zip(ref, auxs))
+    pool = Pool(32, 32, n)
+    try:
+        pool.set_patches(np.full(n, 0.1), np.full(n, 0.1), np.zeros(n), np.zeros(n))
+        _, vs = pool.variants()
+        whole = [k for k, v in enumerate(vs) if v[:2] == (32, 32)]
+        tiled = [k for k, v in enumerate(vs) if v[:2] != (32, 32)]
+        assert whole and tiled
+        pool.set_variant(whole[0])
+        pool.upload_aos(0, qs, auxs, buf=0)
+        c, _ = pool.step(0.01, src=0, dst=0)
+        got = pool.download_aos(0, n, buf=0)
+        assert c == pytest.approx(c_ref, rel=1e-15, abs=0)
+        for g, r in zip(got, ref):
+            assert rel_err(g[2:-2, 2:-2], r[2:-2, 2:-2]) <= RTOL_STEP
+        pool.set_variant(tiled[0])
+        pool.upload_aos(0, qs, auxs, buf=0)
+        with pytest.raises(GeoFloodError, match=r"\(-4\)"):
+            pool.step(0.01, src=0, dst=0)
+        with pytest.raises(GeoFloodError, match=r"\(-4\)"):
+            pool.step(0.01, src=1, dst=1, plain=True)
+        c, _ = pool.step(0.01, src=0, dst=1)
+        got = pool.download_aos(0, n, buf=1)
+        assert c == pytest.approx(c_ref, rel=1e-15, abs=0)
+        for g, r in zip(got, ref):
+            assert rel_err(g[2:-2, 2:-2], r[2:-2, 2:-2]) <= RTOL_STEP
+    finally:
+        pool.close()

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import Brick, Hydrograph, Params
-from geoflood_amd.serving import DT0, LENGTH, MX, STRIP, T_START, Engine, Request
+from geoflood_amd.serving import DT0, DT_MAX, LENGTH, MX, STRIP, T_START, Engine, Request
 from geoflood_amd.scenarios import gaussian_sum_dem, sine_hydrograph
 from _parity import assert_state_close
 
@@ -80,7 +80,7 @@ def test_served_scenario_matches_oracle(gf, oracle_c):
     br = Brick.from_global(z, z, z, b, r.n // MX, r.n // MX, 2, dx, dx, mthbc=(0, 1, 3, 3), hydro=H)
     oracle_c.set_params(Params(manning_n=r.manning))
     try:
-        rec = oracle_c.brick_run(br, res.steps, DT0, 0.9, 1.0, t0=T_START, nthreads=8)
+        rec = oracle_c.brick_run(br, res.steps, DT0, 0.9, 1.0, t0=T_START, nthreads=8, dt_max=DT_MAX)
     finally:
         oracle_c.set_params(Params())
     assert rec["retakes"] == res.retakes

@@ -123,7 +123,8 @@ def test_vs_reference_random_patches(oracle_c, oracle_ref):
         assert same(qa, qb) and same(ca, cb)
 
 
-@pytest.mark.parametrize("order,transverse,mthlim", [(1, 0, (4, 4, 4)), (2, 1, (1, 1, 1)), (2, 2, (2, 3, 5))])
+@pytest.mark.parametrize("order,transverse,mthlim", [(1, 0, (4, 4, 4)), (2, 1, (1, 1, 1)), (2, 2, (2, 3, 5)),
+                                                      (2, 0, (4, 0, 4))])
 def test_vs_reference_method_variants(oracle_c, oracle_ref, order, transverse, mthlim):
     p = Params(order=order, transverse=transverse, mthlim=mthlim, friction_forcing=0)
     oracle_c.set_params(p)
@@ -345,3 +346,30 @@ def test_ghost_fill_equals_global_reflection(oracle_c):
         for pi in range(2):
             tile = H[:, pj * 16:pj * 16 + 20, pi * 16:pi * 16 + 20]
             assert np.array_equal(np.moveaxis(br.q[pj * 2 + pi], 2, 0), tile)
+
+
+def test_brick_run_dt_bound(oracle_c):
+    """The dt bound of the run loop (the library's gf_pool_set_scenario_dt_max;
+    no counterpart in the reference).  dt_max = 0 is the reference's rule
+    dt * desired_cfl / maxcfl, which test_golden_radial_dam_break pins to the
+    reference build's dt sequence through the same loop.  With a bound, an
+    all-dry brick (no face has a wave, CFL 0) steps dt0 then dt_max every
+    step, and a dam break follows dt_k = min(dt_{k-1} * 0.9 / cfl_{k-1},
+    dt_max) exactly, capped at first and below the cap later."""
+    n = 64
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    b = 0.2 * np.exp(-((X - 0.5) ** 2 + (Y - 0.5) ** 2) / 0.02)
+    z = np.zeros_like(b)
+    br = Brick.from_global(z, z, z, b, 2, 2, 2, 1.0 / n, 1.0 / n)
+    rec = oracle_c.brick_run(br, 5, 1e-3, dt_max=0.25)
+    assert np.all(rec["cfl"] == 0.0)
+    assert rec["dt"].tolist() == [1e-3, 0.25, 0.25, 0.25, 0.25]
+    h2 = np.where(X < 0.5, 2.0, 1.0)
+    dt_max = 3e-3
+    br = Brick.from_global(h2, z, z, z, 2, 2, 2, 1.0 / n, 1.0 / n)
+    rec = oracle_c.brick_run(br, 12, 1e-3, dt_max=dt_max)
+    assert rec["retakes"] == 0
+    for k in range(1, 12):
+        assert rec["dt"][k] == min(rec["dt"][k - 1] * 0.9 / rec["cfl"][k - 1], dt_max)
+    assert (rec["dt"][1:] == dt_max).any() and (rec["dt"][1:] < dt_max).any()
