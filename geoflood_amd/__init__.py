@@ -60,6 +60,9 @@ PROTOTYPES = {
     "gf_step2_batch": (ct.c_int, [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
                                   ct.POINTER(_dp), ct.POINTER(_dp), _dp, _dp,
                                   ct.c_double, ct.c_double, _dp]),
+    "gf_step2_batch_noalloc": (ct.c_int, [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int,
+                                  ct.POINTER(_dp), ct.POINTER(_dp), _dp, _dp,
+                                  ct.c_double, ct.c_double, _dp]),
     "gf_pool_create": (ct.c_void_p, [ct.c_int, ct.c_int, ct.c_int, ct.c_int]),
     "gf_pool_destroy": (ct.c_int, [ct.c_void_p]),
     "gf_pool_set_patches": (ct.c_int, [ct.c_void_p, ct.c_int, _dp, _dp, _dp, _dp, _ip, _ip]),

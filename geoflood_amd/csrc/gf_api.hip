@@ -31,6 +31,13 @@
 using gf::PatchDesc;
 using gf::ScenCtrl;
 
+// Device memory of gf_step2_batch_noalloc: a fixed array per GPU, loaded
+// with the library's code object and held until the process ends.  Host code
+// reaches it only through the address hipGetSymbolAddress returns on each
+// device (Lib::scratch_dev); the name itself is not a device address there.
+constexpr size_t kNoAllocBytes = size_t(32) << 20;
+static __device__ __align__(256) double scratch[kNoAllocBytes / sizeof(double)];
+
 namespace {
 
 thread_local std::string g_err;
@@ -157,6 +164,9 @@ struct Lib {
     size_t h_stage_bytes = 0;
     double* d_max = nullptr;
     double* h_max = nullptr;  // pinned
+    // per device: the address of `scratch` there (set by gf_init)
+    static constexpr int kMaxDevices = 64;
+    double* scratch_dev[kMaxDevices] = {};
 };
 Lib g;
 
@@ -318,6 +328,11 @@ try {
     HIPCHK(hipDeviceGetAttribute(&g.num_cu, hipDeviceAttributeMultiprocessorCount, device));
     HIPCHK(hipMalloc(&g.d_max, sizeof(double)));
     HIPCHK(hipHostMalloc(&g.h_max, sizeof(double), hipHostMallocDefault));
+    if (device < Lib::kMaxDevices && !g.scratch_dev[device]) {
+        void* sp = nullptr;
+        HIPCHK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+        g.scratch_dev[device] = static_cast<double*>(sp);
+    }
     g.device = device;
     default_params();
     g.init = true;
@@ -1482,6 +1497,95 @@ try {
         }))
         return r;
     if (maxcfl) *maxcfl = *g.h_max;
+    return GF_OK;
+} GF_CATCH_STATUS
+
+// Layout of the no-allocation batch in the device scratch array, in doubles,
+// each region 256-byte aligned.
+struct NoAllocLayout {
+    size_t q0, q1, aux, stage, dxy, desc, cfl, tiles, dmax, total;
+};
+static NoAllocLayout noalloc_layout(int n, int maux, size_t np2)
+{
+    auto al = [](size_t d) { return (d + 31) / 32 * 32; };
+    NoAllocLayout L{};
+    size_t off = 0;
+    L.q0 = off;    off += al((size_t)n * 3 * np2);
+    L.q1 = off;    off += al((size_t)n * 3 * np2);
+    L.aux = off;   off += al((size_t)n * np2);
+    L.stage = off; off += al((size_t)n * (3 + (size_t)maux) * np2);
+    L.dxy = off;   off += al(2 * (size_t)n);
+    L.desc = off;  off += al((size_t)n * sizeof(PatchDesc) / sizeof(double));
+    L.cfl = off;   off += al((size_t)n);
+    L.tiles = off; off += al(2 * (size_t)n + ((size_t)n + 1) / 2);  // tile CFL, depth (8 B), count (4 B)
+    L.dmax = off;  off += 32;
+    L.total = off;
+    return L;
+}
+
+int gf_step2_batch_noalloc(int mx, int my, int mbc, int maux, int n, double* const* q, double* const* aux,
+                           const double* dx, const double* dy, double t, double dt, double* maxcfl)
+try {
+    (void)t;
+    if (int r = need_init()) return r;
+    if (n <= 0) return fail(GF_EINVAL, "batch size must be > 0");
+    if (!q || !aux || !dx || !dy) return fail(GF_EINVAL, "NULL argument");
+    if (maux < 1) return fail(GF_EINVAL, "maux must be >= 1");
+    if (!(dt > 0.0)) return fail(GF_EINVAL, "dt must be > 0");
+    if (mbc != gf::MBC) return fail(GF_EUNSUPPORTED, "mbc = %d; only mbc = 2 is supported", mbc);
+    if (!gf::supported_shape(mx, my)) return fail(GF_EUNSUPPORTED, "patch shape %dx%d not supported", mx, my);
+    for (int k = 0; k < n; k++)
+        if (!(dx[k] > 0.0) || !(dy[k] > 0.0)) return fail(GF_EINVAL, "patch %d: dx, dy must be > 0", k);
+    double* base = g.device < Lib::kMaxDevices ? g.scratch_dev[g.device] : nullptr;
+    if (!base) return fail(GF_ESTATE, "no scratch address on device %d", g.device);
+    const size_t np2 = (size_t)(mx + 2 * mbc) * (my + 2 * mbc);
+    const NoAllocLayout L = noalloc_layout(n, maux, np2);
+    if (L.total * sizeof(double) > kNoAllocBytes)
+        return fail(GF_EINVAL, "a batch of %d patches of %dx%d with maux = %d needs %zu bytes of device scratch; "
+                    "gf_step2_batch_noalloc holds %zu (use gf_step2_batch, or smaller batches)",
+                    n, mx, my, maux, L.total * sizeof(double), kNoAllocBytes);
+    double* q0 = base + L.q0;
+    double* q1 = base + L.q1;
+    double* st = base + L.stage;
+    double* dxy = base + L.dxy;
+    PatchDesc* desc = reinterpret_cast<PatchDesc*>(base + L.desc);
+    unsigned long long* tcfl = reinterpret_cast<unsigned long long*>(base + L.tiles);
+    // host patches straight into the staging area (no host staging either)
+    const size_t qd = (size_t)n * 3 * np2;
+    for (int k = 0; k < n; k++) {
+        HIPCHK(hipMemcpyAsync(st + (size_t)k * 3 * np2, q[k], sizeof(double) * 3 * np2, hipMemcpyHostToDevice, g.stream));
+        HIPCHK(hipMemcpyAsync(st + qd + (size_t)k * maux * np2, aux[k], sizeof(double) * maux * np2,
+                              hipMemcpyHostToDevice, g.stream));
+    }
+    HIPCHK(hipMemcpyAsync(dxy, dx, sizeof(double) * n, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(hipMemcpyAsync(dxy + n, dy, sizeof(double) * n, hipMemcpyHostToDevice, g.stream));
+    HIPCHK(gf::launch_desc_fill(desc, dxy, dxy + n, n, g.stream));
+    // the tile fold slots start at zero (an earlier call may have stopped half way)
+    HIPCHK(gf::launch_zero_words(reinterpret_cast<int*>(tcfl), 2 * (2 * (size_t)n) + (size_t)n, g.stream));
+    HIPCHK(gf::launch_pack(st, st + qd, q0, q1, base + L.aux, 0, n, (int)np2, maux, g.stream));
+    gf::StepArgs a{};
+    a.desc = desc;
+    a.first = 0;
+    a.count = n;
+    a.q0 = q0;
+    a.q1 = q1;
+    a.aux = base + L.aux;
+    a.cfl_out = base + L.cfl;
+    a.dt = dt;
+    a.cur = 0;
+    a.P = g.P;
+    a.tile_cfl = tcfl;
+    a.tile_h = tcfl + n;
+    a.tile_cnt = reinterpret_cast<int*>(tcfl + 2 * (size_t)n);
+    HIPCHK(gf::launch_step(mx, my, 0, a, g.stream));
+    HIPCHK(gf::launch_max(base + L.cfl, n, base + L.dmax, g.stream));
+    HIPCHK(gf::launch_unpack(st, q1, 0, n, (int)np2, g.stream));
+    for (int k = 0; k < n; k++)
+        HIPCHK(hipMemcpyAsync(q[k], st + (size_t)k * 3 * np2, sizeof(double) * 3 * np2, hipMemcpyDeviceToHost, g.stream));
+    double m = 0.0;
+    HIPCHK(hipMemcpyAsync(&m, base + L.dmax, sizeof(double), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (maxcfl) *maxcfl = m;
     return GF_OK;
 } GF_CATCH_STATUS
 

@@ -213,6 +213,10 @@ hipError_t launch_inflow(int mx, int my, const InflowArgs& a, int grid, hipStrea
 hipError_t launch_pack(const double* stage_q, const double* stage_aux, double* q, double* q2, double* aux,
                        int first, int n, int np2, int maux, hipStream_t st);
 hipError_t launch_unpack(double* stage_q, const double* q, int first, int n, int np2, hipStream_t st);
+// descriptors of n host patches: dx[k], dy[k], every side stored
+hipError_t launch_desc_fill(PatchDesc* d, const double* dx, const double* dy, int n, hipStream_t st);
+// n 4-byte words set to zero (a kernel, so that a captured stream holds no memset node)
+hipError_t launch_zero_words(int* p, size_t n, hipStream_t st);
 // q (one buffer) and aux of slot src[k] to slot dst[k]
 hipError_t launch_move_slots(double* q, double* aux, const int* src, const int* dst, int n, int np2, hipStream_t st);
 hipError_t launch_max(const double* v, int n, double* out, hipStream_t st);

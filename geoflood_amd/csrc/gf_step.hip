@@ -23,6 +23,8 @@
 // bit of pow() in src2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cstddef>
 
 #include "gf_ghost.h"
@@ -2114,6 +2116,40 @@ hipError_t launch_pack(const double* stage_q, const double* stage_aux, double* q
 {
     if (n <= 0) return hipSuccess;
     k_pack<<<n, 256, 0, st>>>(stage_q, stage_aux, q, q2, aux, first, n, np2, maux);
+    return hipGetLastError();
+}
+
+// Descriptors of host patches whose ghost cells come with them: cell sizes
+// only, every side stored.
+__global__ void __launch_bounds__(256) k_desc_fill(PatchDesc* d, const double* dx, const double* dy, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    PatchDesc p{};
+    p.dx = dx[k];
+    p.dy = dy[k];
+    p.scen = 0;
+    for (int s = 0; s < 8; s++) p.nbr[s] = NB_STORED;
+    d[k] = p;
+}
+
+__global__ void __launch_bounds__(256) k_zero_words(int* p, size_t n)
+{
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = 0;
+}
+
+hipError_t launch_desc_fill(PatchDesc* d, const double* dx, const double* dy, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_desc_fill<<<(n + 255) / 256, 256, 0, st>>>(d, dx, dy, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_words(int* p, size_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const size_t grid = std::min<size_t>((n + 255) / 256, 1024);
+    k_zero_words<<<(unsigned)grid, 256, 0, st>>>(p, n);
     return hipGetLastError();
 }
 

@@ -153,6 +153,23 @@ int gf_step2_batch(int mx, int my, int mbc, int maux, int n,
                    const double *dx, const double *dy,
                    double t, double dt, double *maxcfl);
 
+/* gf_step2_batch for callers that cannot allocate memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * has no counterpart in the reference.  Same arguments and results as
+ * gf_step2_batch, but it calls no hipMalloc, hipFree or other allocating
+ * routine, on the device or the host: its device memory is a fixed 32 MiB
+ * array per GPU (loaded with the library and held until the process ends),
+ * and the host patches are copied straight between the caller's arrays and
+ * that array.  A batch needs about n * (10 + maux) * (mx+4) * (my+4) * 8
+ * bytes of it (up to about 290 patches of 32x32 with maux = 1); a larger
+ * batch returns GF_EINVAL.  Every call on one GPU uses the same array and leaves its last
+ * contents there: calls that overlap on one GPU (from several host threads)
+ * race. */
+int gf_step2_batch_noalloc(int mx, int my, int mbc, int maux, int n,
+                           double *const *q, double *const *aux,
+                           const double *dx, const double *dy,
+                           double t, double dt, double *maxcfl);
+
 /* ---- plugin slot: per-patch callback with batching ---------------------- */
 
 typedef struct gf_plugin gf_plugin;
