@@ -368,7 +368,7 @@ def main():
         roofline = dict(bound="valu", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                         valu_issue_frac=valu_issue_frac, fp64_frac=fp64_frac,
-                        kernel="k_all<%d,%d,%d,%d,%d>" % ((MX, MX) + tuple(variants[pool.variants()[0]])),
+                        kernel="k_all<%d,%d,%d,%d,%d,%d,%d>" % ((MX, MX) + tuple(variants[pool.variants()[0]])),
                         kernel_avg_ms=round(k_avg_ms, 5), bytes_per_launch=round(k_bytes),
                         touched_patches_per_launch=round(touched, 1),
                         active_patches_per_launch=round(act / launches, 1),
@@ -601,7 +601,8 @@ def main():
             "accepted_steps": accepted,
             "warmup_accepted_steps": info_w["nsteps"],
             "simulated_time_s": [info_w["t"], info_b["t"]],
-            "step_variants": [dict(tile=[v[0], v[1]], threads=v[2]) for v in variants],
+            "step_variants": [dict(tile=[v[0], v[1]], threads=v[2], patches_per_workgroup=v[3],
+                                   xface_increments="registers" if v[4] else "lds") for v in variants],
             "autotune_initial": dict(variant=best, k_all_ms=[round(m, 5) for m in tune_ms]),
             "autotune_retunes": retunes,
             "active_patches_per_attempt": round(active_per_attempt, 1),

@@ -1710,9 +1710,11 @@ try {
         for (int k = 0; k < n; k++) {
             gf::StepVariant v;
             gf::get_variant(p->mx, p->my, k, &v);
-            desc[3 * k] = v.tx;
-            desc[3 * k + 1] = v.ty;
-            desc[3 * k + 2] = v.nt;
+            desc[5 * k] = v.tx;
+            desc[5 * k + 1] = v.ty;
+            desc[5 * k + 2] = v.nt;
+            desc[5 * k + 3] = v.npp;
+            desc[5 * k + 4] = v.freg;
         }
     return GF_OK;
 } GF_CATCH_STATUS
@@ -1742,7 +1744,11 @@ try {
     // The active list of the current states (what the next attempt steps).
     // Each candidate steps it from the accepted buffer into the other one,
     // which the next attempt overwrites: no scenario advances.
-    HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, 0, p->capacity, -1, 0.0), inflow_grid(), g.stream));
+    if (p->amr) {
+        if (int r = enqueue_ghost_update(p, -1, 0.0)) return r;
+    } else {
+        HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, 0, p->capacity, -1, 0.0), inflow_grid(), g.stream));
+    }
     HIPCHK(gf::launch_classify(a, g.stream));
     if (int r = ensure_pair_events(g.ev_tune, 1)) return r;
     const int saved = p->variant;

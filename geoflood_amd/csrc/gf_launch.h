@@ -174,9 +174,11 @@ hipError_t launch_tag(int mx, int my, bool coarsen, const TagArgs& a, hipStream_
 // differences of eta = h + b, neighbours read as the halo pull reads them).
 hipError_t launch_tag_eta_gradient(int mx, int my, const TagArgs& a, double eta_tol, hipStream_t st);
 
-// A step variant: patch shape, tile shape, threads per workgroup.
+// A step variant: patch shape, tile shape, threads per patch (tile),
+// patches per workgroup of the persistent kernel, and where the x-face flux
+// increments wait for the y phase (1 registers, 0 LDS).
 struct StepVariant {
-    int mx, my, tx, ty, nt;
+    int mx, my, tx, ty, nt, npp, freg;
 };
 int num_variants(int mx, int my);
 bool get_variant(int mx, int my, int k, StepVariant* out);

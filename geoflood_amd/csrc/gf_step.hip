@@ -172,14 +172,17 @@ __device__ __forceinline__ void cell_transverse(const Rpt2Cell& C, const double 
     }
 }
 
-// Block-wide maximum (result valid in thread 0).
+// Maximum over a group of NT threads (the workgroup, or one of the groups
+// of a workgroup that steps several patches; ltid is the thread's index in
+// its group, NT a multiple of 64) with one barrier of the whole workgroup;
+// the result is valid in the group's thread 0.
 template <int NT>
-__device__ __forceinline__ double block_max(double v, double* red)
+__device__ __forceinline__ double block_max(double v, double* red, int ltid)
 {
     constexpr int NWAVE = NT / 64;
     v = wave_max(v);
     __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    if ((ltid & 63) == 0) red[ltid >> 6] = v;
     __syncthreads();
     double r = red[0];
 #pragma unroll
@@ -187,18 +190,19 @@ __device__ __forceinline__ double block_max(double v, double* red)
     return r;
 }
 
-// Block-wide maxima of two values with one barrier (results valid in every
-// thread).  red holds 2 * NT / 64 doubles; the caller guarantees that no
-// thread still reads red from an earlier use (a barrier separates them).
+// Group maxima of two values with one barrier (results valid in every
+// thread of the group).  red holds 2 * NT / 64 doubles per group; the caller
+// guarantees that no thread still reads red from an earlier use (a barrier
+// separates them).
 template <int NT>
-__device__ __forceinline__ void block_max2(double& a, double& b, double* red)
+__device__ __forceinline__ void block_max2(double& a, double& b, double* red, int ltid)
 {
     constexpr int NWAVE = NT / 64;
     a = wave_max(a);
     b = wave_max(b);
-    if ((threadIdx.x & 63) == 0) {
-        red[threadIdx.x >> 6] = a;
-        red[NWAVE + (threadIdx.x >> 6)] = b;
+    if ((ltid & 63) == 0) {
+        red[ltid >> 6] = a;
+        red[NWAVE + (ltid >> 6)] = b;
     }
     __syncthreads();
     a = red[0];
@@ -222,18 +226,30 @@ __device__ __forceinline__ void block_max2(double& a, double& b, double* red)
 // load); otherwise the stored ghost cells are used; halo cells inside the
 // patch are its own interior.  Thread 0 receives the tile's maximum CFL
 // number (uncapped) and the maximum interior depth after the step.
-// Called by step_patch's last thread right after the tile is in LDS: the
-// workgroup's highest thread has no position in the last slot, so its wave
-// has slack in the phases that follow.
+//
+// A workgroup may step several patches at once, one per group of NT
+// threads (ltid: the thread's index in its group; T, E, F, red: the group's
+// slices of LDS).  The groups pass every barrier together, so the all-dry
+// fast path is taken only when every group's tile is dry (the full path
+// gives the same values on a dry tile).  A group without a patch (valid
+// false: the last item of the work list) computes on a zero tile and writes
+// nothing.  FREG: the x-face flux increments stay in the owning thread's
+// registers between the x and the y phase (the thread that forms a face's
+// increments is the one that reads them back) instead of in LDS -- the "LDS
+// split" of a step variant.
+//
+// Hook: called by thread NT-1 (of group 0) right after the tile is in LDS:
+// that thread has no position in the last slot, so its wave has slack in
+// the phases that follow.
 struct NoHook {
     __device__ void operator()() const {}
 };
 
-template <int MX, int MY, int TX, int TY, int NT, bool PULL, bool DEF, typename Hook = NoHook>
+template <int MX, int MY, int TX, int TY, int NT, bool PULL, bool DEF, bool FREG = false, typename Hook = NoHook>
 __device__ __forceinline__ void step_patch(double* T, double* E, double* F, double* red, const PatchDesc* dptr, int patch,
                            int ox, int oy, const double* q_src, double* q_dst, const double* aux,
                            double dt, const Params& P, double& cfl_res, double& hmax_res,
-                           const Hook& hook = Hook())
+                           int ltid, bool valid, const Hook& hook = Hook())
 {
     using G = Geo<TX, TY>;   // tile geometry (LDS)
     using GP = Geo<MX, MY>;  // patch geometry (HBM)
@@ -243,7 +259,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
     // Opaque copy of the thread index: inside the persistent loop this stops
     // the compiler from hoisting every index computation of the body out of
     // the loop, which would keep them live in registers across iterations.
-    int tid = threadIdx.x;
+    int tid = ltid;
     asm volatile("" : "+v"(tid));
     const double* src = q_src + (size_t)patch * 3 * PNP2;
     double* dst = q_dst + (size_t)patch * 3 * PNP2;
@@ -263,7 +279,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
 #pragma unroll
     for (int r = 0; r < LS; r++) {
         const int c = tid + r * NT;
-        lok[r] = c < NP2;
+        lok[r] = valid && c < NP2;
         const int cc = lok[r] ? c : 0;
         const int fi = ox + cc % NXP - 1, fj = oy + cc / NXP - 1;  // patch Fortran indices
         const int pc = GP::ti(fi, fj);
@@ -310,6 +326,12 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
             T[2 * NP2 + c] = hv;
             T[3 * NP2 + c] = lb[r];
             wet |= (h > drytol);
+        } else if (!valid && tid + r * NT < NP2) {
+            const int c = tid + r * NT;  // a group without a patch: a dry zero tile
+            T[c] = 0.0;
+            T[NP2 + c] = 0.0;
+            T[2 * NP2 + c] = 0.0;
+            T[3 * NP2 + c] = 0.0;
         }
     }
     const int anywet = __syncthreads_or(wet);
@@ -329,16 +351,18 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
             const int pk = GP::ti(ox + i, oy + j);
             double h = T[k], hu = T[NP2 + k], hv = T[2 * NP2 + k];
             if (P.src_term > 0) src2_cell(h, hu, hv, T[3 * NP2 + k], dt, P);
-            dst[pk] = h;
-            dst[PNP2 + pk] = hu;
-            dst[2 * PNP2 + pk] = hv;
+            if (valid) {
+                dst[pk] = h;
+                dst[PNP2 + pk] = hu;
+                dst[2 * PNP2 + pk] = hv;
+            }
             hm = fmaxf64(hm, h);
             bad |= !finite3(h, hu, hv);
         }
         // a non-finite cell (a NaN depth is neither wet nor dry) makes the
         // tile's CFL number NaN, which stops the scenario
         if (bad) hm = quiet_nan();
-        hm = block_max<NT>(hm, red);
+        hm = block_max<NT>(hm, red, tid);
         if (tid == 0) {
             cfl_res = hm != hm ? quiet_nan() : 0.0;
             hmax_res = hm;
@@ -383,6 +407,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         }
     }
     double AP[SLOTS][3], AM[SLOTS][3], GA[SLOTS][3], GB[SLOTS][3];
+    double FX[FREG ? SLOTS : 1][6];  // FREG: the position's x-face increments (faddm, faddp)
 
     // ================= x-faces: normal Riemann problems =================
 #pragma unroll
@@ -428,7 +453,13 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
             }
             double fxm[3], fxp[3];
             face_second_order<0, DEF>(wc, Sr[s], wl, wr, dtdx, P, fxm, fxp, AM[s], AP[s]);
-            if (j >= 1 && j <= TY) {
+            if constexpr (FREG) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    FX[s][k] = fxm[k];
+                    FX[s][3 + k] = fxp[k];
+                }
+            } else if (j >= 1 && j <= TY) {
                 const int f = G::fi_(i, j);
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
@@ -601,11 +632,16 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         if (in[s] && i >= 1 && i <= TX + 1 && j >= 1 && j <= TY) {
             const double g2[3] = {E[p - 1], E[2 * NPOS + p - 1], E[NPOS + p - 1]};  // local -> physical
             const double g1[3] = {GA[s][0], GA[s][2], GA[s][1]};
-            const int f = G::fi_(i, j);
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                FMr[s][k] = F[k * G::NF + f] + g2[k] + g1[k];
-                FPr[s][k] = F[(3 + k) * G::NF + f] + g2[k] + g1[k];
+                if constexpr (FREG) {
+                    FMr[s][k] = FX[s][k] + g2[k] + g1[k];
+                    FPr[s][k] = FX[s][3 + k] + g2[k] + g1[k];
+                } else {
+                    const int f = G::fi_(i, j);
+                    FMr[s][k] = F[k * G::NF + f] + g2[k] + g1[k];
+                    FPr[s][k] = F[(3 + k) * G::NF + f] + g2[k] + g1[k];
+                }
             }
         }
     }
@@ -629,7 +665,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
     for (int s = 0; s < SLOTS; s++) {
         const int i = pi[s], j = pj[s];
         const int p = tid + s * NT;
-        if (in[s] && i >= 1 && i <= TX && j >= 1 && j <= TY) {
+        if (valid && in[s] && i >= 1 && i <= TX && j >= 1 && j <= TY) {
             const int k = G::ti(i, j);
             const int pk = GP::ti(ox + i, oy + j);
             double qn[3];
@@ -652,7 +688,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
     // (the per-patch MAXCFL_CAP is applied by the caller, once per patch; a
     // non-finite new state makes the CFL number NaN, which the cap keeps)
     double v = bad ? quiet_nan() : cfl, hmx = hm;
-    block_max2<NT>(v, hmx, red);
+    block_max2<NT>(v, hmx, red, tid);
     if (tid == 0) {
         cfl_res = v;
         hmax_res = hmx;
@@ -694,14 +730,14 @@ __device__ __forceinline__ bool tile_fold(unsigned long long* tcfl, unsigned lon
 }
 
 // Plain step kernel: one workgroup per tile, stored ghost cells.
-template <int MX, int MY, int TX, int TY, int NT, bool DEF>
+template <int MX, int MY, int TX, int TY, int NT, bool FREG, bool DEF>
 __global__ void __launch_bounds__(NT) k_step(StepArgs a)
 {
     using G = Geo<TX, TY>;
     using TL = Tiling<MX, MY, TX, TY>;
     __shared__ double T[4 * G::NP2];
     __shared__ double E[7 * G::NPOS];
-    __shared__ double F[6 * G::NF];
+    __shared__ double F[FREG ? 1 : 6 * G::NF];
     __shared__ double red[2 * (NT / 64)];
     const int bid = blockIdx.x;
     if (bid >= a.count * TL::NTILES) return;
@@ -711,8 +747,8 @@ __global__ void __launch_bounds__(NT) k_step(StepArgs a)
     const double* src = a.cur ? a.q1 : a.q0;
     double* dst = a.q1 ? (a.cur ? a.q0 : a.q1) : a.q0;
     double cfl = 0.0, hm = 0.0;
-    step_patch<MX, MY, TX, TY, NT, false, DEF>(T, E, F, red, a.desc + patch, patch, ox, oy, src, dst, a.aux, a.dt,
-                                          a.P, cfl, hm);
+    step_patch<MX, MY, TX, TY, NT, false, DEF, FREG>(T, E, F, red, a.desc + patch, patch, ox, oy, src, dst, a.aux,
+                                                     a.dt, a.P, cfl, hm, threadIdx.x, true);
     if (threadIdx.x == 0) {
         bool last = true;
         if (TL::NTILES > 1) last = tile_fold(a.tile_cfl, a.tile_h, a.tile_cnt, patch, TL::NTILES, cfl, hm);
@@ -757,8 +793,9 @@ __device__ __forceinline__ void load_head(const BatchArgs& a, int patch, ItemHea
 }
 
 // Run by one thread of the current active item: takes the next item from
-// the cursor, publishes it, and when it is active fills its ItemHead.
-template <int NTILES>
+// the cursor, publishes it, and when it is active fills its ItemHeads (one
+// per patch of the item: entries next*NPP .. of the active tile list).
+template <int NTILES, int NPP>
 struct ItemPrefetch {
     const BatchArgs* pa;
     int* next_slot;
@@ -768,7 +805,11 @@ struct ItemPrefetch {
     {
         const int next = atomicAdd(&pa->counters[NEXT_WET], 1);
         *next_slot = next;
-        if (next < nwet) load_head(*pa, pa->list_wet[next / NTILES], next_head);
+#pragma unroll
+        for (int g = 0; g < NPP; g++) {
+            const int e = next * NPP + g;
+            if (e < nwet) load_head(*pa, pa->list_wet[e / NTILES], next_head + g);
+        }
     }
 };
 
@@ -781,33 +822,45 @@ struct ItemPrefetch {
 // motion off: hoisting the body's constant and index materialisations out
 // of the persistent loop would keep them live across the whole step and
 // spill.)
-template <int MX, int MY, int TX, int TY, int NT, bool DEF>
+//
+// NPP patches per workgroup (untiled variants): the item is entries
+// idx*NPP .. idx*NPP+NPP-1 of the active list, one per group of NT threads.
+template <int MX, int MY, int TX, int TY, int NT, int NPP, bool FREG, bool DEF>
 __device__ __forceinline__ void wet_item(const BatchArgs* pa, int idx, const ItemHead* head, ItemHead* next_head,
                                          int* next_slot, int nwet)
 {
     using G = Geo<TX, TY>;
     using TL = Tiling<MX, MY, TX, TY>;
-    __shared__ double T[4 * G::NP2];
-    __shared__ double E[7 * G::NPOS];
-    __shared__ double F[6 * G::NF];
-    __shared__ double red[2 * (NT / 64)];
+    static_assert(NPP == 1 || TL::NTILES == 1, "several patches per workgroup only for variants with one tile per patch");
+    __shared__ double T[NPP * 4 * G::NP2];
+    __shared__ double E[NPP * 7 * G::NPOS];
+    __shared__ double F[FREG ? 1 : NPP * 6 * G::NF];
+    __shared__ double red[NPP * 2 * (NT / 64)];
     const BatchArgs& a = *pa;
-    const int patch = head->patch;
+    const int grp = NPP > 1 ? (int)threadIdx.x / NT : 0;
+    const int ltid = NPP > 1 ? (int)threadIdx.x - grp * NT : (int)threadIdx.x;
+    const int entry = idx * NPP + grp;
+    const bool valid = NPP == 1 || entry < nwet;
+    // a group without a patch computes on a zero tile with group 0's header
+    const ItemHead* hd = valid ? head + grp : head;
+    const int patch = hd->patch;
     int ox, oy;
-    TL::origin(idx % TL::NTILES, ox, oy);
-    const int scen = head->d.scen;
-    const int cur = head->cur;
-    const double dt = head->dt;
+    TL::origin(entry % TL::NTILES, ox, oy);
+    const int scen = hd->d.scen;
+    const int cur = hd->cur;
+    const double dt = hd->dt;
     const double* src = cur ? a.q1 : a.q0;
     double* dst = cur ? a.q0 : a.q1;
     Params P = a.P;
-    P.manning_n = head->manning;
-    if (threadIdx.x == 0) a.dry_age[patch] = 0;
+    P.manning_n = hd->manning;
+    if (ltid == 0 && valid) a.dry_age[patch] = 0;
     double cfl = 0.0, hm = 0.0;
-    const ItemPrefetch<TL::NTILES> pf{pa, next_slot, next_head, nwet};
-    step_patch<MX, MY, TX, TY, NT, true, DEF>(T, E, F, red, &head->d, patch, ox, oy, src, dst, a.aux, dt, P,
-                                         cfl, hm, pf);
-    if (threadIdx.x == 0) {
+    const ItemPrefetch<TL::NTILES, NPP> pf{pa, next_slot, next_head, nwet};
+    step_patch<MX, MY, TX, TY, NT, true, DEF, FREG>(T + grp * 4 * G::NP2, E + grp * 7 * G::NPOS,
+                                                    F + (FREG ? 0 : grp * 6 * G::NF), red + grp * 2 * (NT / 64),
+                                                    &hd->d, patch, ox, oy, src, dst, a.aux, dt, P, cfl, hm, ltid,
+                                                    valid, pf);
+    if (ltid == 0 && valid) {
         bool last = true;
         if (TL::NTILES > 1) last = tile_fold(a.tile_cfl, a.tile_h, a.tile_cnt, patch, TL::NTILES, cfl, hm);
         if (last) {
@@ -916,33 +969,37 @@ __device__ __forceinline__ void dry_item(const BatchArgs* pa, int k, int next, i
 // (wet_item), then the dry patches in items of dry_per_item (dry_item), from
 // one atomic cursor.  The streaming dry items fill the tail that the
 // compute-bound active items leave when the list runs out.
-template <int MX, int MY, int TX, int TY, int NT, bool DEF>
-__global__ void __launch_bounds__(NT) k_all(const BatchArgs* pa)
+template <int MX, int MY, int TX, int TY, int NT, int NPP, bool FREG, bool DEF>
+__global__ void __launch_bounds__(NT * NPP) k_all(const BatchArgs* pa)
 {
     using TL = Tiling<MX, MY, TX, TY>;
-    constexpr int DPI = dry_per_item<MX, MY, NT>();
+    constexpr int NB = NT * NPP;  // threads of the workgroup
+    constexpr int DPI = dry_per_item<MX, MY, NB>();
     __shared__ int item[2];
-    __shared__ ItemHead head[2];
+    __shared__ ItemHead head[2][NPP];
     __shared__ int unsteady[DPI];  // per dry patch of an item: a cell at dry_tolerance
-    const int nwet = pa->counters[CNT_WET] * TL::NTILES;
+    const int nwet = pa->counters[CNT_WET] * TL::NTILES;  // active tiles
+    const int witems = (nwet + NPP - 1) / NPP;             // active items
     if (threadIdx.x == 0) {
         item[0] = atomicAdd(&pa->counters[NEXT_WET], 1);
-        if (item[0] < nwet) load_head(*pa, pa->list_wet[item[0] / TL::NTILES], &head[0]);
+        for (int g = 0; g < NPP; g++)
+            if (item[0] * NPP + g < nwet) load_head(*pa, pa->list_wet[(item[0] * NPP + g) / TL::NTILES], &head[0][g]);
     }
     if (threadIdx.x < DPI) unsteady[threadIdx.x] = 0;
     __syncthreads();
-    const int limit = nwet + (pa->counters[CNT_DRY] + DPI - 1) / DPI;
+    const int limit = witems + (pa->counters[CNT_DRY] + DPI - 1) / DPI;
     for (int it = 0;; it++) {
         const int idx = item[it & 1];
         if (idx >= limit) break;
-        if (idx < nwet) {
+        if (idx < witems) {
             // active items come first in the cursor's order, so an active
-            // item's successor's head is always prefetched by an active item
-            wet_item<MX, MY, TX, TY, NT, DEF>(pa, idx, &head[it & 1], &head[(it + 1) & 1], &item[(it + 1) & 1], nwet);
+            // item's successor's heads are always prefetched by an active item
+            wet_item<MX, MY, TX, TY, NT, NPP, FREG, DEF>(pa, idx, head[it & 1], head[(it + 1) & 1],
+                                                         &item[(it + 1) & 1], nwet);
         } else {
             int next = 0;
             if (threadIdx.x == 0) next = atomicAdd(&pa->counters[NEXT_WET], 1);
-            dry_item<MX, MY, NT>(pa, idx - nwet, next, &item[(it + 1) & 1], unsteady);
+            dry_item<MX, MY, NB>(pa, idx - witems, next, &item[(it + 1) & 1], unsteady);
         }
     }
 }
@@ -1268,7 +1325,7 @@ __global__ void __launch_bounds__(256) k_hmax(const double* q, double* hmax, int
     const double* src = q + (size_t)(first + b) * 3 * G::NP2;
     double m = 0.0;
     for (int e = threadIdx.x; e < MX * MY; e += 256) m = fmaxf64(m, src[G::ti(e % MX + 1, e / MX + 1)]);
-    m = block_max<256>(m, red);
+    m = block_max<256>(m, red, threadIdx.x);
     if (threadIdx.x == 0) hmax[first + b] = m;
 }
 
@@ -1732,7 +1789,7 @@ __global__ void __launch_bounds__(256) k_hmax_list(const double* q, double* hmax
     const double* src = q + (size_t)s * 3 * G::NP2;
     double m = 0.0;
     for (int e = threadIdx.x; e < MX * MY; e += 256) m = fmaxf64(m, src[G::ti(e % MX + 1, e / MX + 1)]);
-    m = block_max<256>(m, red);
+    m = block_max<256>(m, red, threadIdx.x);
     if (threadIdx.x == 0) {
         hmax[s] = m;
         dry_age[s] = 0;
@@ -1831,20 +1888,35 @@ bool default_method(const Params& P)
     return P.order == 2 && P.transverse == 2 && P.mthlim[0] == 4 && P.mthlim[1] == 4 && P.mthlim[2] == 4;
 }
 
-// Step variants per patch shape: tile shape and threads per workgroup.
-// Variant 0 is the default; the autotuner picks among them by timing.
+// Step variants per patch shape: X(MX, TX, NT, NPP, FREG) = tile TX x TX,
+// NT threads per patch (tile), NPP patches per workgroup, x-face increments
+// in registers (FREG 1) or LDS (0).  The first variant of a shape is its
+// default; the autotuner picks among them by timing.  Every variant gives
+// the same results bit for bit.
+#define GF_VARIANTS(X)       \
+    X(8, 8, 128, 1, 0)       \
+    X(8, 8, 128, 1, 1)       \
+    X(8, 8, 128, 2, 1)       \
+    X(8, 8, 128, 3, 1)       \
+    X(16, 16, 384, 1, 0)     \
+    X(16, 16, 192, 1, 0)     \
+    X(16, 16, 384, 1, 1)     \
+    X(16, 16, 192, 1, 1)     \
+    X(16, 16, 384, 2, 1)     \
+    X(32, 32, 640, 1, 0)     \
+    X(32, 16, 384, 1, 0)     \
+    X(32, 32, 768, 1, 0)     \
+    X(32, 32, 704, 1, 0)     \
+    X(32, 32, 1024, 1, 0)    \
+    X(32, 16, 384, 1, 1)     \
+    X(64, 32, 640, 1, 0)     \
+    X(64, 16, 384, 1, 0)     \
+    X(64, 32, 768, 1, 0)
+
 static const StepVariant kVariants[] = {
-    {8, 8, 8, 8, 128},
-    {16, 16, 16, 16, 384},
-    {16, 16, 16, 16, 192},
-    {32, 32, 32, 32, 640},
-    {32, 32, 16, 16, 384},
-    {32, 32, 32, 32, 768},
-    {32, 32, 32, 32, 704},
-    {32, 32, 32, 32, 1024},
-    {64, 64, 32, 32, 640},
-    {64, 64, 16, 16, 384},
-    {64, 64, 32, 32, 768},
+#define GF_ROW(MX_, TX_, NT_, NPP_, FREG_) {MX_, MX_, TX_, TX_, NT_, NPP_, FREG_},
+    GF_VARIANTS(GF_ROW)
+#undef GF_ROW
 };
 
 int num_variants(int mx, int my)
@@ -1864,11 +1936,9 @@ bool get_variant(int mx, int my, int k, StepVariant* out)
     return false;
 }
 
-static int variant_key(int mx, int variant)
+static bool same_variant(const StepVariant& v, int mx, int tx, int nt, int npp, int freg)
 {
-    StepVariant v;
-    if (!get_variant(mx, mx, variant, &v)) return -1;
-    return (v.mx * 100 + v.tx) * 10000 + v.nt;
+    return v.mx == mx && v.tx == tx && v.nt == nt && v.npp == npp && v.freg == freg;
 }
 
 template <typename K>
@@ -1881,21 +1951,14 @@ static int occupancy_of(K kernel, int nt)
 
 int wet_occupancy(int mx, int my, int variant)
 {
-    if (mx != my) return 1;
-    switch (variant_key(mx, variant)) {
-    case 8080128: return occupancy_of(k_all<8, 8, 8, 8, 128, true>, 128);
-    case 16160384: return occupancy_of(k_all<16, 16, 16, 16, 384, true>, 384);
-    case 16160192: return occupancy_of(k_all<16, 16, 16, 16, 192, true>, 192);
-    case 32320640: return occupancy_of(k_all<32, 32, 32, 32, 640, true>, 640);
-    case 32160384: return occupancy_of(k_all<32, 32, 16, 16, 384, true>, 384);
-    case 32320768: return occupancy_of(k_all<32, 32, 32, 32, 768, true>, 768);
-    case 32320704: return occupancy_of(k_all<32, 32, 32, 32, 704, true>, 704);
-    case 32321024: return occupancy_of(k_all<32, 32, 32, 32, 1024, true>, 1024);
-    case 64320768: return occupancy_of(k_all<64, 64, 32, 32, 768, true>, 768);
-    case 64320640: return occupancy_of(k_all<64, 64, 32, 32, 640, true>, 640);
-    case 64160384: return occupancy_of(k_all<64, 64, 16, 16, 384, true>, 384);
-    default: return 1;
-    }
+    StepVariant v;
+    if (mx != my || !get_variant(mx, my, variant, &v)) return 1;
+#define GF_OCC(MX_, TX_, NT_, NPP_, FREG_)                                                  \
+    if (same_variant(v, MX_, TX_, NT_, NPP_, FREG_))                                       \
+        return occupancy_of(k_all<MX_, MX_, TX_, TX_, NT_, NPP_, (bool)FREG_, true>, NT_ * NPP_);
+    GF_VARIANTS(GF_OCC)
+#undef GF_OCC
+    return 1;
 }
 
 hipError_t launch_step(int mx, int my, int variant, const StepArgs& a, hipStream_t st)
@@ -1906,25 +1969,16 @@ hipError_t launch_step(int mx, int my, int variant, const StepArgs& a, hipStream
     if (!get_variant(mx, my, variant, &v)) return hipErrorInvalidValue;
     const int grid = a.count * (mx / v.tx) * (my / v.ty);
     const bool def = default_method(a.P);
-#define GF_STEP(MX_, TX_, NT_)                                                                  \
-    (def ? (k_step<MX_, MX_, TX_, TX_, NT_, true><<<grid, NT_, 0, st>>>(a), 0)                 \
-         : (k_step<MX_, MX_, TX_, TX_, NT_, false><<<grid, NT_, 0, st>>>(a), 0))
-    switch (variant_key(mx, variant)) {
-    case 8080128: GF_STEP(8, 8, 128); break;
-    case 16160384: GF_STEP(16, 16, 384); break;
-    case 16160192: GF_STEP(16, 16, 192); break;
-    case 32320640: GF_STEP(32, 32, 640); break;
-    case 32160384: GF_STEP(32, 16, 384); break;
-    case 32320768: GF_STEP(32, 32, 768); break;
-    case 32320704: GF_STEP(32, 32, 704); break;
-    case 32321024: GF_STEP(32, 32, 1024); break;
-    case 64320768: GF_STEP(64, 32, 768); break;
-    case 64320640: GF_STEP(64, 32, 640); break;
-    case 64160384: GF_STEP(64, 16, 384); break;
-    default: return hipErrorInvalidValue;
+    // the plain step runs one tile per workgroup: a variant's NPP does not apply
+#define GF_STEP(MX_, TX_, NT_, NPP_, FREG_)                                                 \
+    if (same_variant(v, MX_, TX_, NT_, NPP_, FREG_)) {                                     \
+        if (def) k_step<MX_, MX_, TX_, TX_, NT_, (bool)FREG_, true><<<grid, NT_, 0, st>>>(a);  \
+        else k_step<MX_, MX_, TX_, TX_, NT_, (bool)FREG_, false><<<grid, NT_, 0, st>>>(a);     \
+        return hipGetLastError();                                                           \
     }
+    GF_VARIANTS(GF_STEP)
 #undef GF_STEP
-    return hipGetLastError();
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_ghost(int mx, int my, const GhostArgs& a, hipStream_t st)
@@ -1962,26 +2016,17 @@ hipError_t launch_classify(const BatchArgs& a, hipStream_t st)
 
 hipError_t launch_all(int mx, int my, int variant, bool def, const BatchArgs* da, int grid, hipStream_t st)
 {
-    if (mx != my) return hipErrorInvalidValue;
-#define GF_ALL(MX_, TX_, NT_)                                                                   \
-    (def ? (k_all<MX_, MX_, TX_, TX_, NT_, true><<<grid, NT_, 0, st>>>(da), 0)                 \
-         : (k_all<MX_, MX_, TX_, TX_, NT_, false><<<grid, NT_, 0, st>>>(da), 0))
-    switch (variant_key(mx, variant)) {
-    case 8080128: GF_ALL(8, 8, 128); break;
-    case 16160384: GF_ALL(16, 16, 384); break;
-    case 16160192: GF_ALL(16, 16, 192); break;
-    case 32320640: GF_ALL(32, 32, 640); break;
-    case 32160384: GF_ALL(32, 16, 384); break;
-    case 32320768: GF_ALL(32, 32, 768); break;
-    case 32320704: GF_ALL(32, 32, 704); break;
-    case 32321024: GF_ALL(32, 32, 1024); break;
-    case 64320640: GF_ALL(64, 32, 640); break;
-    case 64160384: GF_ALL(64, 16, 384); break;
-    case 64320768: GF_ALL(64, 32, 768); break;
-    default: return hipErrorInvalidValue;
+    StepVariant v;
+    if (mx != my || !get_variant(mx, my, variant, &v)) return hipErrorInvalidValue;
+#define GF_ALL(MX_, TX_, NT_, NPP_, FREG_)                                                                      \
+    if (same_variant(v, MX_, TX_, NT_, NPP_, FREG_)) {                                                         \
+        if (def) k_all<MX_, MX_, TX_, TX_, NT_, NPP_, (bool)FREG_, true><<<grid, NT_ * NPP_, 0, st>>>(da);     \
+        else k_all<MX_, MX_, TX_, TX_, NT_, NPP_, (bool)FREG_, false><<<grid, NT_ * NPP_, 0, st>>>(da);        \
+        return hipGetLastError();                                                                               \
     }
+    GF_VARIANTS(GF_ALL)
 #undef GF_ALL
-    return hipGetLastError();
+    return hipErrorInvalidValue;
 }
 
 

@@ -444,12 +444,13 @@ class Pool:
 
     # -- step variants ------------------------------------------------------------
     def variants(self):
-        """(current variant, [(tile_x, tile_y, threads), ...])."""
+        """(current variant, [(tile_x, tile_y, threads, patches_per_workgroup,
+        freg), ...]); freg 1: x-face increments in registers, 0: in LDS."""
         cur, n = ct.c_int(0), ct.c_int(0)
         check(self.lib.gf_pool_variants(self.h, ct.byref(cur), ct.byref(n), None), "gf_pool_variants")
-        d = np.zeros(3 * n.value, dtype=np.int32)
+        d = np.zeros(5 * n.value, dtype=np.int32)
         check(self.lib.gf_pool_variants(self.h, None, None, iptr(d)), "gf_pool_variants")
-        return cur.value, [tuple(int(x) for x in d[3 * k:3 * k + 3]) for k in range(n.value)]
+        return cur.value, [tuple(int(x) for x in d[5 * k:5 * k + 5]) for k in range(n.value)]
 
     def set_variant(self, v: int):
         check(self.lib.gf_pool_set_variant(self.h, v), "gf_pool_set_variant")

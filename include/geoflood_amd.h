@@ -355,12 +355,16 @@ int gf_pool_tag_coarsen(gf_pool *pool, int buf, double t, const gf_refine *crit,
 
 /* ---- step variants and the autotuner ------------------------------------ */
 
-/* Step variants of the pool's patch shape: *count variants, desc[3k..3k+2] =
- * tile width, tile height, threads per workgroup of variant k (a tile
- * smaller than the patch splits each patch into tiles with their own
- * 2-cell halo; the results do not depend on the variant).  *current is the
- * variant in use.  Any pointer may be NULL.  The reference fixes one shape
- * (cudaclaw_flux2.cu:68-120: one block of 128 threads per patch). */
+/* Step variants of the pool's patch shape: *count variants, desc[5k..5k+4]
+ * = tile width, tile height, threads per tile, patches per workgroup of the
+ * batched step kernel, and where a face's flux increments wait between the
+ * x and y sweeps (1 the owning thread's registers, 0 LDS: the LDS split) of
+ * variant k.  A tile smaller than the patch splits each patch into tiles with
+ * their own 2-cell halo; several patches per workgroup step side by side,
+ * one per group of threads.  The results do not depend on the variant.
+ * *current is the variant in use.  Any pointer may be NULL.  The reference
+ * fixes one shape (cudaclaw_flux2.cu:68-120: one block of 128 threads per
+ * patch). */
 int gf_pool_variants(gf_pool *pool, int *current, int *count, int *desc);
 
 /* Select a step variant (the next gf_pool_run recaptures its graph). */

@@ -1,7 +1,10 @@
-"""Step variants (tile shape, threads per workgroup) and the autotuner: every
-variant produces bit-identical states and dt sequences (a tile with its own
-2-cell halo computes exactly what the whole-patch step computes), 64x64
-patches (tiled) match the oracle, and autotuning advances no scenario."""
+"""Step variants (tile shape, threads per tile, patches per workgroup, LDS
+split) and the autotuner: every variant produces bit-identical states and dt
+sequences (a tile with its own 2-cell halo computes exactly what the
+whole-patch step computes; patches stepped side by side in one workgroup,
+or with their x-face increments in registers, compute what they compute
+alone), 64x64 patches (tiled) match the oracle, and autotuning advances no
+scenario."""
 import numpy as np
 import pytest
 
@@ -45,7 +48,7 @@ def _run(pool, layout, steps, graph=True):
     return info, np.array(dts), np.stack(pool.download_global(layout, buf=info["cur"]))
 
 
-@pytest.mark.parametrize("mx", [16, 32, 64])
+@pytest.mark.parametrize("mx", [8, 16, 32, 64])
 def test_variants_bit_identical(gf, dam_params, mx):
     n, steps = 256, 25
     pool, layout, _ = _pool(n, mx)
