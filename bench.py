@@ -162,7 +162,7 @@ def amr_overland(steps, maxlevel=4):
         counts0 = run.forest.counts()
         log(f"amr: initial forest {len(run.forest)} leaves {counts0} in {t_init:.2f} s; {steps} steps ...")
         run.run(8)  # warm-up: graph capture, first regrids
-        c0, s0 = run.stats["cell_updates"], run.stats["steps"]
+        c0, s0, d0 = run.stats["cell_updates"], run.stats["steps"], run.stats["device_ms"]
         p0, r0 = len(run.stats["patches"]), run.stats["regrids"]
         st0 = run.pool.stats()
         t0 = time.perf_counter()
@@ -173,7 +173,9 @@ def amr_overland(steps, maxlevel=4):
         pts = run.stats["patches"][p0:]
         q, _ = run.leaves_state()
         wet = int((q[:, 2:-2, 2:-2, 0] > 1e-3).sum())
+        dev_s = (run.stats["device_ms"] - d0) * 1e-3
         out = dict(value=cu / wall, unit="cell-updates/s", steps=run.stats["steps"] - s0,
+                   device_s=round(dev_s, 4), device_cell_updates_per_s=cu / dev_s if dev_s > 0 else None,
                    attempts=info["attempts"], retakes=info["retakes"], wall_s=round(wall, 3),
                    simulated_time_s=round(info["t"], 3),
                    regrids=run.stats["regrids"] - r0, ms_per_step=round(1e3 * wall / max(run.stats["steps"] - s0, 1), 4),
@@ -192,6 +194,84 @@ def amr_overland(steps, maxlevel=4):
     finally:
         run.close()
         geoflood_amd.set_params(**PARAMS)
+
+
+_DEM_CACHE = {}
+
+
+def _dem2km(nx, ny, L):
+    """The overland-flood DEM on an nx x nx lattice over 2 km (cached: every
+    patch size of the sweep uses the same level lattices)."""
+    from geoflood_amd.scenarios import gaussian_sum_dem
+    if nx not in _DEM_CACHE:
+        _DEM_CACHE[nx] = gaussian_sum_dem(nx, 2000.0, seed=0)
+    return _DEM_CACHE[nx]
+
+
+def autotune_sweep(steps, sizes=(8, 16, 32, 64), maxlevel=5):
+    """Autotuner sweep under dynamic AMR: for each patch size mx, a 2 km
+    square of level-0 bricks of 64/mx patches (levels 1-5, finest cells
+    0.98 m at every mx), the overland-flood DEM, walls, a 30 m deep
+    reservoir over x < 250 m, y >= 1750 m released at t = 0 (Manning n
+    0.035); leaves tagged where wet, regrid every 2 steps; the step variant
+    (tile shape, threads per tile, patches per workgroup, LDS split) is
+    re-selected whenever the active population per attempt moves to another
+    power-of-two bucket, from a cache of earlier choices (persisted to a JSON
+    file for the run) or by timing every variant."""
+    import tempfile
+    import geoflood_amd
+    from geoflood_amd.amr import AmrRun
+    from geoflood_amd.autotune import TuneCache
+    from geoflood_amd.scenarios import reservoir_state
+    geoflood_amd.set_params(**dict(PARAMS, manning_n=0.035))
+    out = {}
+    cache_path = os.path.join(tempfile.mkdtemp(prefix="gf_tune_"), "variants.json")
+    cache = TuneCache(cache_path)
+    try:
+        for mx in sizes:
+            nx0 = 64 // mx
+            cap = (nx0 << maxlevel) ** 2
+            run = AmrRun(mx, nx0, nx0, 2000.0, 2000.0, 1, maxlevel, _dem2km, cap, eta_tol=1e30,
+                         regrid_interval=2, tune_cache=cache)
+            try:
+                run.start(0.0, 0.01, state_fn=reservoir_state(run))
+                n0 = len(run.forest)
+                _, variants = run.pool.variants()
+                t0 = time.perf_counter()
+                info = run.run(steps)
+                wall = time.perf_counter() - t0
+                pts = run.stats["patches"]
+                tuned = [dict(e, variant_desc=list(variants[e["variant"]])) for e in cache.log if e["key"].startswith(f"{mx}:")]
+                timings = {k: v for k, v in cache.table.items() if k.startswith(f"{mx}:")}
+                gain = []
+                for k, v in sorted(timings.items(), key=lambda kv: int(kv[0].split(":")[-1])):
+                    ms = v["ms"]
+                    gain.append(dict(bucket=int(k.split(":")[-1]), active=v["active"], chosen=v["variant"],
+                                     chosen_ms=ms[v["variant"]], variant0_ms=ms[0],
+                                     speedup_vs_variant0=round(ms[0] / ms[v["variant"]], 3)))
+                dev_s = run.stats["device_ms"] * 1e-3
+                out[str(mx)] = dict(steps=info["nsteps"], attempts=info["attempts"], retakes=info["retakes"],
+                                    wall_s=round(wall, 3), device_s=round(dev_s, 4),
+                                    simulated_time_s=round(info["t"], 3),
+                                    cell_updates_per_s=run.stats["cell_updates"] / wall,
+                                    device_cell_updates_per_s=run.stats["cell_updates"] / dev_s if dev_s > 0 else None,
+                                    patches=dict(initial=n0, final=len(run.forest), max=int(max(pts)),
+                                                 final_by_level=run.forest.counts()),
+                                    selections=tuned, tuning_by_bucket=gain,
+                                    variants=[list(v) for v in variants])
+                log(f"sweep mx={mx}: {info['nsteps']} steps in {wall:.1f} s, patches {n0} -> {len(run.forest)} "
+                    f"(max {max(pts)}), {run.stats['cell_updates'] / wall / 1e9:.3f} G cell-updates/s, "
+                    f"selections {[(e['key'].split(':')[-1], e['variant'], e['cached']) for e in tuned]}")
+            finally:
+                run.close()
+    finally:
+        geoflood_amd.set_params(**PARAMS)
+    out["cache_entries"] = len(cache.table)
+    out["note"] = ("variant = (tile x, tile y, threads per tile, patches per workgroup, x-face increments in "
+                   "registers 1 / LDS 0); speedup_vs_variant0: the persistent step kernel's time with the chosen "
+                   "variant against the fixed default one-workgroup-per-patch shape, timed by the autotuner on the "
+                   "same population")
+    return out
 
 
 def main():
@@ -218,6 +298,9 @@ def main():
     ap.add_argument("--ensemble-rate", type=float, default=32.0)
     ap.add_argument("--ensemble-capacity", type=int, default=65536)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--sweep-steps", type=int, default=600,
+                    help="accepted steps per patch size of the autotuner sweep under dynamic AMR (0: skip; "
+                         "the workload's full length is 3000)")
     ap.add_argument("--amr-steps", type=int, default=2000,
                     help="accepted steps of the AMR overland flood (levels 1-4, regrid every 4; 0: skip)")
     ap.add_argument("--variant", type=int, default=-1,
@@ -550,6 +633,11 @@ def main():
     if args.amr_steps > 0:
         amr = amr_overland(args.amr_steps)
 
+    # ---- autotuner sweep under dynamic AMR ----
+    sweep = None
+    if args.sweep_steps > 0:
+        sweep = autotune_sweep(args.sweep_steps)
+
     # ---- CPU baseline (reference Fortran step on the host cores) ----
     cpu = None
     cpu_port = None
@@ -621,6 +709,7 @@ def main():
         "parity_vs_cpu": parity,
         "ensemble": ensemble,
         "amr_overland": amr,
+        "autotune_sweep": sweep,
     }
     print(json.dumps(out))
 

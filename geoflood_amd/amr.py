@@ -35,7 +35,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import GF_NB_STORED, GF_NB_WALL
+from . import GF_NB_STORED, GF_NB_WALL, last_timing
 from .pool import _KIND, MBC, Pool, pad_bathymetry
 
 DIRS = ((-1, 0), (1, 0), (0, -1), (0, 1), (-1, -1), (1, -1), (-1, 1), (1, 1))
@@ -67,6 +67,7 @@ class Forest:
             lev = np.full((self.ny, self.nx), minlevel, dtype=np.int64)
         self.lev = np.asarray(lev, dtype=np.int64)
         assert self.lev.shape == (self.ny, self.nx)
+        self._tables = {}
         self._index()
 
     def _index(self):
@@ -151,6 +152,12 @@ class Forest:
         further.  links: (coarse leaf, fine leaf, side, igrid) for every
         finer leaf across a side or corner of a coarser one (igrid: the
         fine leaf's half along a face, 0 the lower coordinate)."""
+        key = tuple(mthbc)
+        if key not in self._tables:
+            self._tables[key] = self._build_tables(key)
+        return self._tables[key]
+
+    def _build_tables(self, mthbc):
         M = self.maxlevel
         L, oi, oj = self.L, self.oi, self.oj
         n = len(L)
@@ -246,11 +253,18 @@ class AmrRun:
     eta_tol: the |grad eta| refinement threshold."""
 
     def __init__(self, mx, nx0, ny0, lx, ly, minlevel, maxlevel, dem_fn, capacity, mthbc=(3, 3, 3, 3),
-                 hydro=None, eta_tol=0.01, regrid_interval=4, manning_n=None, dt_max=0.0, flags_fn=None):
+                 hydro=None, eta_tol=0.01, regrid_interval=4, manning_n=None, dt_max=0.0, flags_fn=None,
+                 tune_cache=None):
         self.mx, self.lx, self.ly = mx, lx, ly
         # flags_fn(forest, tags, regrid_index) -> (refine, coarsen) per leaf;
         # default: refine the tagged leaves, coarsen families of untagged ones
         self.flags_fn = flags_fn
+        # tune_cache (geoflood_amd.autotune.TuneCache): after a regrid, when the
+        # active population has moved to another bucket, re-select the step
+        # variant (cached choice, or an autotune)
+        self.tune = tune_cache
+        self._tune_bucket = None
+        self._tune_stats = None
         self.mthbc, self.hydro, self.eta_tol = tuple(mthbc), hydro, eta_tol
         self.regrid_interval, self.manning_n, self.dt_max = regrid_interval, manning_n, dt_max
         self.capacity = capacity
@@ -260,12 +274,39 @@ class AmrRun:
         self.half = 0            # the forest occupies slots [half * capacity, ...)
         self.perm = None         # slot offset of each leaf
         self.sid = None
-        self.stats = dict(regrids=0, attempts=0, steps=0, retakes=0, cell_updates=0, patches=[])
+        self.stats = dict(regrids=0, attempts=0, steps=0, retakes=0, cell_updates=0, patches=[], device_ms=0.0)
         self.tag_log = []        # the tags of every regrid, in leaf order
 
     # -- layout -----------------------------------------------------------------------
     def _dx(self, L):
         return self.lx / ((self.forest.nx0 << L) * self.mx), self.ly / ((self.forest.ny0 << L) * self.mx)
+
+    def cell_centres(self, L, i, j):
+        """Cell centres (x, y) of leaves (L[k], i[k], j[k]), ghost cells
+        included: two arrays of shape (n, mx+4, mx+4)."""
+        m = self.mx
+        c = np.arange(-MBC, m + MBC) + 0.5
+        dx = np.array([self._dx(l)[0] for l in L])[:, None]
+        dy = np.array([self._dx(l)[1] for l in L])[:, None]
+        xs = (np.asarray(i)[:, None] * m + c[None, :]) * dx
+        ys = (np.asarray(j)[:, None] * m + c[None, :]) * dy
+        return np.broadcast_to(xs[:, None, :], (len(L), m + 2 * MBC, m + 2 * MBC)), \
+            np.broadcast_to(ys[:, :, None], (len(L), m + 2 * MBC, m + 2 * MBC))
+
+    def _retune(self):
+        """Re-select the step variant when the active population per attempt
+        since the last check has moved to another bucket."""
+        from .autotune import TuneCache
+        st = self.pool.stats()
+        prev, self._tune_stats = self._tune_stats, st
+        if prev is None or st["attempts"] <= prev["attempts"]:
+            return
+        active = (st["active_patches"] - prev["active_patches"]) / (st["attempts"] - prev["attempts"])
+        b = TuneCache.bucket(active)
+        if b != self._tune_bucket:
+            self.tune.choose(self.pool, active, levels=self.forest.counts())
+            self._tune_bucket = b
+            self._tune_stats = self.pool.stats()  # the candidates' launches are not attempts of the run
 
     def _describe(self, forest: Forest, base: int, perm: np.ndarray):
         """Describe the slots [base, base + len(forest)) of a forest whose
@@ -406,6 +447,7 @@ class AmrRun:
                     self.pool.run(k, graph=True)
                 else:
                     self.pool.run_plain(k)
+                self.stats["device_ms"] += last_timing()[0]
                 info = self.pool.scenario(self.sid)
                 if info["status"] != 0:
                     raise RuntimeError(f"scenario stopped with status {info['status']} at t = {info['t']}")
@@ -418,6 +460,8 @@ class AmrRun:
             if info["nsteps"] % self.regrid_interval == 0:
                 self.regrid()
                 self.stats["patches"].append(len(self.forest))
+                if self.tune is not None:
+                    self._retune()
         self.stats["steps"] = info["nsteps"]
         self.stats["retakes"] = info["retakes"]
         return info

@@ -93,3 +93,16 @@ def overland_flood(n: int, length: float = 2000.0, peak_q: float = 200.0, strip:
     t, hu = sine_hydrograph(peak_q, strip)
     return dict(h=z.copy(), hu=z.copy(), hv=z.copy(), b=b, dx=length / n, mthbc=(0, 1, 3, 3),
                 hydro=dict(time=t, hu=hu, b=strip, x0=0.0, y0=0.5 * length))
+
+
+def reservoir_state(run, x1: float = 250.0, y0: float = 1750.0, h0: float = 30.0):
+    """Initial state of a moving dam-break front for a resident-AMR run
+    (geoflood_amd.amr.AmrRun): depth h0 over the reservoir x < x1, y >= y0
+    (cell centres, ghost cells included), dry elsewhere, at rest.  Returns
+    the state_fn that AmrRun.start / OracleAmr.start take."""
+    def state(L, i, j, aux):
+        X, Y = run.cell_centres(L, i, j)
+        q = np.zeros(aux.shape[:3] + (3,))
+        q[..., 0] = np.where((X < x1) & (Y >= y0), h0, 0.0)
+        return q
+    return state
