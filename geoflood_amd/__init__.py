@@ -87,6 +87,7 @@ PROTOTYPES = {
     "gf_pool_get_scenario": (ct.c_int, [ct.c_void_p, ct.c_int, ct.POINTER(GfScenarioInfo)]),
     "gf_pool_set_scenario_dt_max": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_double]),
     "gf_pool_set_scenario_stop": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_double]),
+    "gf_eval_powers": (ct.c_int, [ct.c_int, ct.c_int, _dp, _dp]),
     "gf_pool_get_scenarios": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.POINTER(GfScenarioInfo)]),
     "gf_pool_run": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip]),
     "gf_pool_run_plain": (ct.c_int, [ct.c_void_p, ct.c_int, _ip]),

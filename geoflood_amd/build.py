@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgeoflood_amd.so")
 SOURCES = ["gf_step.hip", "gf_tag.hip", "gf_api.hip", "gf_plugin.cpp", "gf_output.cpp"]
-HEADERS = ["gf_physics.h", "gf_kernels.h", "gf_launch.h", "gf_inflow.h", "gf_output.h", "gf_ghost.h"]
+HEADERS = ["gf_physics.h", "gf_kernels.h", "gf_launch.h", "gf_inflow.h", "gf_output.h", "gf_ghost.h", "gf_pow.h"]
 
 # -ffp-contract=off: no fused multiply-add anywhere, so every FP64 operation
 # rounds as the reference Fortran's does.  Packed FP32 is disabled as a rule

@@ -1589,6 +1589,23 @@ try {
     return GF_OK;
 } GF_CATCH_STATUS
 
+int gf_eval_powers(int which, int n, const double* x, double* out)
+try {
+    if (int r = need_init()) return r;
+    if (which != 0 && which != 1) return fail(GF_EINVAL, "which must be 0 (7/3) or 1 (2/3)");
+    if (n < 0 || (n > 0 && (!x || !out))) return fail(GF_EINVAL, "invalid arrays");
+    if (n == 0) return GF_OK;
+    double* d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(double) * 2 * (size_t)n));
+    hipError_t e = hipMemcpyAsync(d, x, sizeof(double) * n, hipMemcpyHostToDevice, g.stream);
+    if (e == hipSuccess) e = gf::launch_eval_powers(which, d, d + n, n, g.stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + n, sizeof(double) * n, hipMemcpyDeviceToHost, g.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g.stream);
+    hipFree(d);
+    if (e != hipSuccess) return fail(GF_EHIP, "gf_eval_powers: %s", hipGetErrorString(e));
+    return GF_OK;
+} GF_CATCH_STATUS
+
 int gf_synchronize(void)
 try {
     if (int r = need_init()) return r;

@@ -12,6 +12,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "gf_pow.h"
+
 namespace gf {
 
 // hydrograh_module.f90:186-196: clamp outside the table, else the first
@@ -35,7 +37,7 @@ __device__ inline double riemann_invariant_depth(double hu0, double h1, double h
 {
     const double u1 = hu1 / h1;
     const double Fr = 0.5;
-    double h0 = pow(hu0 / sqrt(g) * Fr, 2.0 / 3.0);
+    double h0 = pow_2_3(hu0 / sqrt(g) * Fr);
     for (int it = 0; it < HYDRO_MAXITER; it++) {
         const double fxn = hu0 / h0 - 2.0 * sqrt(g * h0) - u1 + 2.0 * sqrt(g * h1);
         if (fabs(fxn) < HYDRO_TOL) break;
@@ -50,7 +52,7 @@ __device__ inline double two_shock_depth(double hu0, double h1, double hu1, doub
 {
     const double u1 = hu1 / h1;
     const double Fr = 1.0;
-    double h0 = pow(hu0 / sqrt(g) * Fr, 2.0 / 3.0);
+    double h0 = pow_2_3(hu0 / sqrt(g) * Fr);
     for (int it = 0; it < HYDRO_MAXITER; it++) {
         const double fxn = hu0 / h0 - u1 - (h0 - h1) * sqrt((g / 2.0) * (1.0 / h0 + 1.0 / h1));
         if (fabs(fxn) < HYDRO_TOL) break;

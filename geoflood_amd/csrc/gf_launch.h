@@ -217,6 +217,8 @@ hipError_t launch_pack(const double* stage_q, const double* stage_aux, double* q
 hipError_t launch_unpack(double* stage_q, const double* q, int first, int n, int np2, hipStream_t st);
 // descriptors of n host patches: dx[k], dy[k], every side stored
 hipError_t launch_desc_fill(PatchDesc* d, const double* dx, const double* dy, int n, hipStream_t st);
+// the step's powers (gf_pow.h) of n device values: which 0 -> x^(7/3), 1 -> x^(2/3)
+hipError_t launch_eval_powers(int which, const double* x, double* out, int n, hipStream_t st);
 // n 4-byte words set to zero (a kernel, so that a captured stream holds no memset node)
 hipError_t launch_zero_words(int* p, size_t n, hipStream_t st);
 // q (one buffer) and aux of slot src[k] to slot dst[k]

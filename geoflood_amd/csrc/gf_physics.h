@@ -12,6 +12,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "gf_pow.h"
+
 namespace gf {
 
 struct Params {
@@ -456,7 +458,7 @@ __device__ __forceinline__ void src2_cell(double h, double& hu, double& hv, doub
         hv = 0.0;
     } else if (h <= P.friction_depth) {
         const double coeff = (b < (double)10.e32f) ? P.manning_n : 0.0;
-        const double gamma = sqrt(hu * hu + hv * hv) * P.grav * (coeff * coeff) / pow(h, 7.0 / 3.0);
+        const double gamma = sqrt(hu * hu + hv * hv) * P.grav * (coeff * coeff) / pow_7_3(h);
         const double dgamma = 1.0 + dt * gamma;
         hu = hu / dgamma;
         hv = hv / dgamma;

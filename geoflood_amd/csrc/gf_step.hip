@@ -19,8 +19,9 @@
 // Every floating-point value is formed with the same operations in the same
 // order as the reference Fortran (geoclaw_step2_fort.f90, geoclaw_flux2fw_
 // fort.f, rpn2_geoclaw.f, rpt2_geoclaw.f, geoclaw_src2_fort.f90), so the
-// results equal the reference's up to the sign of exact zeros and the last
-// bit of pow() in src2.
+// results equal the reference's up to the sign of exact zeros and the rare
+// calls in which the host libm's pow misrounds (src2's power is correctly
+// rounded here, gf_pow.h).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1097,7 +1098,7 @@ __device__ void bc2_side(double* L, int side, int code, const PatchDesc* __restr
                 mirror = false;
                 double h0;
                 if (h1 < P.dry_tolerance) {
-                    h0 = fmaxf64(pow(hu0 / sqrt(P.grav), 2.0 / 3.0), 0.001);
+                    h0 = fmaxf64(pow_2_3(hu0 / sqrt(P.grav)), 0.001);
                 } else if (hu0 != 0.0) {
                     h0 = riemann_invariant_depth(hu0, h1, hu1, P.grav);
                     if (h0 > h1) h0 = two_shock_depth(hu0, h1, hu1, P.grav);
@@ -2176,6 +2177,20 @@ __global__ void __launch_bounds__(256) k_desc_fill(PatchDesc* d, const double* d
     p.scen = 0;
     for (int s = 0; s < 8; s++) p.nbr[s] = NB_STORED;
     d[k] = p;
+}
+
+// The step's two powers (gf_pow.h) for n values: which 0 -> x^(7/3), 1 -> x^(2/3).
+__global__ void __launch_bounds__(256) k_eval_powers(int which, const double* x, double* out, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = which == 0 ? pow_7_3(x[k]) : pow_2_3(x[k]);
+}
+
+hipError_t launch_eval_powers(int which, const double* x, double* out, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_eval_powers<<<(n + 255) / 256, 256, 0, st>>>(which, x, out, n);
+    return hipGetLastError();
 }
 
 __global__ void __launch_bounds__(256) k_zero_words(int* p, size_t n)

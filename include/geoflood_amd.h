@@ -398,6 +398,15 @@ int gf_pool_run(gf_pool *pool, int substeps, int use_graph, int *any_active);
  * Semantics equal gf_pool_run. */
 int gf_pool_run_plain(gf_pool *pool, int substeps, int *any_active);
 
+/* The step's two powers evaluated on the device, for n host values x:
+ * which = 0 gives x^y with y = 7.0/3.0 (src2's h**(7.d0/3.d0),
+ * geoclaw_src2_fort.f90:62), which = 1 with y = 2.0/3.0 (the hydrograph's
+ * (q0(2)/sqrt(grav))**(2.0d0/3.0d0), geoflood_bc2_fort.f90:63).  Both are
+ * correctly rounded on the device; this entry point has no counterpart in
+ * the reference and exists to check the device arithmetic against the host
+ * libm's pow. */
+int gf_eval_powers(int which, int n, const double *x, double *out);
+
 /* Wait for all work queued on the library stream. */
 int gf_synchronize(void);
 

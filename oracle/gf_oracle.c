@@ -9,6 +9,7 @@
  *   MAX(a,b)  -> a > b ? a : b        MIN(a,b) -> a < b ? a : b
  *   DMAX1/DMIN1 with three arguments fold left to right
  *   SIGN(a,b) -> copysign(|a|, b)     x**2 -> x*x      x**(7/3) -> pow()
+ *   (gfo_pow: the libm's pow, or with gfo_set_pow_mode(1) correctly rounded)
  * Unary minus in Fortran applies to a whole product (-a*b = -(a*b)); IEEE
  * negation commutes with rounding, so C's (-a)*b is bitwise identical.
  */
@@ -24,6 +25,46 @@
 
 static inline double FMAX(double a, double b) { return a > b ? a : b; }
 static inline double FMIN(double a, double b) { return a < b ? a : b; }
+
+/* The two powers of the step, x**(7.d0/3.d0) (src2) and x**(2.0d0/3.0d0)
+ * (hydrograph inflow depth).  By default the host libm's pow, as the
+ * reference build calls it.  With gfo_set_pow_mode(1): the correctly rounded
+ * value of x^y (the exact real power of the double y), computed as
+ * x^(k/3) * x^d with y = k/3 + d exactly (d = 2^-51/3 for 7/3, -2^-53/3 for
+ * 2/3): cbrt(x) refined by one Newton step in double-double from exact FMA
+ * products (J.-M. Muller et al., Handbook of Floating-Point Arithmetic, 2nd
+ * ed., 2018, secs. 4.4, 5.3), x^d = 1 + d ln x (|d ln x| < 1e-13), one final
+ * rounding -- the reference's algorithm evaluated without the libm's rare
+ * misroundings, which is how the device library evaluates these powers. */
+static int g_pow_cr = 0;
+
+void gfo_set_pow_mode(int correctly_rounded) { g_pow_cr = correctly_rounded != 0; }
+
+static double cbrt_dd(double x, double *lo)
+{
+    const double c0 = cbrt(x), a = c0 * c0, a_lo = fma(c0, c0, -a);
+    const double b = a * c0, b_lo = fma(a, c0, -b);
+    const double r = (b - x) + (b_lo + a_lo * c0);
+    *lo = -r / (3.0 * a);
+    return c0;
+}
+
+double gfo_pow(double x, double y)
+{
+    if (!g_pow_cr || !(x > 0x1p-300 && x < 0x1p300) || (y != 7.0 / 3.0 && y != 2.0 / 3.0))
+        return pow(x, y);
+    double c_lo;
+    const double c = cbrt_dd(x, &c_lo);
+    if (y == 7.0 / 3.0) {
+        const double p = x * x, e = fma(x, x, -p), hi = p * c;
+        const double lo = fma(p, c, -hi) + (p * c_lo + e * c);
+        const double t = log(x) / 6755399441055744.0;
+        return hi + (lo + hi * t);
+    }
+    const double hi = c * c, lo = fma(c, c, -hi) + 2.0 * (c * c_lo);
+    const double t = -log(x) / 27021597764222976.0;
+    return hi + (lo + hi * t);
+}
 
 /* 1-D slice helpers: Fortran index i in [1-mbc, n+mbc] -> 0-based k = i+mbc-1 */
 #define K(i) ((i) + mbc - 1)
@@ -849,7 +890,7 @@ void gfo_src2(const gfo_params *p, int mx, int my, int mbc, int maux,
                 double hu = P2(q, MEQN, 2, i, j), hv = P2(q, MEQN, 3, i, j);
                 if (P2(aux, maux, 1, i, j) < manning_break)
                     coeff = p->manning_n;
-                gamma = sqrt(hu * hu + hv * hv) * g * (coeff * coeff) / pow(h, 7.0 / 3.0);
+                gamma = sqrt(hu * hu + hv * hv) * g * (coeff * coeff) / gfo_pow(h, 7.0 / 3.0);
                 dgamma = 1.0 + dt * gamma;
                 P2(q, MEQN, 2, i, j) = hu / dgamma;
                 P2(q, MEQN, 3, i, j) = P2(q, MEQN, 3, i, j) / dgamma;
@@ -967,7 +1008,7 @@ void gfo_riemann_invariants(double q0[2], const double q1[2], double grav)
     /* hydrograh_module.f90:213-242 (discharge branch), Fr = 0.5 */
     const double u1 = q1[1] / q1[0];
     const double Fr = 0.5;
-    q0[0] = pow(q0[1] / sqrt(grav) * Fr, 2.0 / 3.0);
+    q0[0] = gfo_pow(q0[1] / sqrt(grav) * Fr, 2.0 / 3.0);
     for (int it = 0; it < HYDRO_MAXITER; it++) {
         const double fxn = q0[1] / q0[0] - 2.0 * sqrt(grav * q0[0]) - u1 + 2.0 * sqrt(grav * q1[0]);
         if (fabs(fxn) < HYDRO_TOL)
@@ -982,7 +1023,7 @@ void gfo_two_shock(double q0[2], const double q1[2], double grav)
     /* hydrograh_module.f90:274-303 (discharge branch), Fr = 1 */
     const double u1 = q1[1] / q1[0];
     const double Fr = 1.0;
-    q0[0] = pow(q0[1] / sqrt(grav) * Fr, 2.0 / 3.0);
+    q0[0] = gfo_pow(q0[1] / sqrt(grav) * Fr, 2.0 / 3.0);
     for (int it = 0; it < HYDRO_MAXITER; it++) {
         const double fxn = q0[1] / q0[0] - u1 -
                            (q0[0] - q1[0]) * sqrt((grav / 2.0) * (1.0 / q0[0] + 1.0 / q1[0]));
@@ -1031,7 +1072,7 @@ static void bc2_inflow_side(const gfo_params *p, int side, int mx, int my, int m
             int mirror = !in_strip;
             if (in_strip) {
                 if (q1[0] < p->dry_tolerance) {
-                    P2(q, MEQN, 1, gi, gj) = FMAX(pow(q0[1] / sqrt(g), 2.0 / 3.0), 0.001);
+                    P2(q, MEQN, 1, gi, gj) = FMAX(gfo_pow(q0[1] / sqrt(g), 2.0 / 3.0), 0.001);
                     P2(q, MEQN, 2, gi, gj) = q0[1];
                     P2(q, MEQN, 3, gi, gj) = 0.0;
                 } else if (q0[1] != 0.0) {

@@ -208,6 +208,13 @@ int gfo_brick_ghost_fill_ex(const gfo_params *p, int mi, int mj, int mx, int my,
                             double xlo, double ylo, double dx, double dy, double t,
                             const gfo_hydrograph *h);
 
+/* How the step's two powers x**(7/3) and x**(2/3) are evaluated: 0 (the
+ * default) the host libm's pow, as the reference build does; 1 the
+ * correctly rounded value (see gf_oracle.c).  gfo_pow(x, y) is that
+ * evaluation (other exponents: pow). */
+void gfo_set_pow_mode(int correctly_rounded);
+double gfo_pow(double x, double y);
+
 int gfo_brick_run_ex(const gfo_params *p, int mi, int mj, int mx, int my, int mbc,
                      int maux, double **q, double **aux, const int mthbc[4],
                      double xlo, double ylo, double dx, double dy,

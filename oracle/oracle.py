@@ -200,6 +200,17 @@ class CLib(_Lib):
         self.params = params
         self._cp = params.to_c()
 
+    def set_pow_mode(self, correctly_rounded: bool):
+        """x**(7/3) and x**(2/3) by the libm's pow (False, the reference
+        build's arithmetic) or correctly rounded (True, the device's)."""
+        self.lib.gfo_set_pow_mode(1 if correctly_rounded else 0)
+
+    def pow(self, x, y):
+        fn = self.lib.gfo_pow
+        fn.argtypes = [ct.c_double, ct.c_double]
+        fn.restype = ct.c_double
+        return np.array([fn(float(v), y) for v in np.ravel(x)])
+
     def rpn2(self, ixy, mx, mbc, q1d, aux2, maux=1):
         n = mx + 2 * mbc
         fw = np.zeros((n, 3, 3)); s = np.zeros((n, 3))

@@ -142,3 +142,30 @@ def test_oracle_amr_cycle_fills_every_ghost(oracle_c):
     r.run(24)
     assert r.nregrids == 12 and len(set(r.patches)) > 2
     assert (r.state()[:, 0] >= 0).all()
+
+
+def test_eta_gradient_criterion_closed_forms():
+    """The surface-gradient tags of the oracle (no counterpart in the
+    reference): a linear surface eta = a x + c y flags exactly when
+    sqrt(a^2 + c^2) exceeds the threshold (away from equality); any wet
+    cell flags; a bump in one ghost cell reaches only the interior cell next
+    to it; a flat dry bed never flags."""
+    from oracle.amr_driver import eta_gradient_tags
+    mx, dx = 8, 0.5
+    c = (np.arange(-2, mx + 2) + 0.5) * dx
+    X, Y = np.meshgrid(c, c)
+    q = np.zeros((mx + 4, mx + 4, 3))
+    for a, cc, tol in [(0.03, 0.04, 0.049), (0.03, 0.04, 0.051), (0.0, 0.02, 0.0199), (0.0, 0.02, 0.0201)]:
+        aux = (a * X + cc * Y)[..., None]
+        assert eta_gradient_tags(q, aux, mx, dx, dx, 1e-3, tol) == (np.hypot(a, cc) > tol)
+    flat = np.zeros((mx + 4, mx + 4, 1))
+    assert not eta_gradient_tags(q, flat, mx, dx, dx, 1e-3, 1e-6)
+    w = q.copy()
+    w[5, 6, 0] = 2e-3                       # one wet interior cell
+    assert eta_gradient_tags(w, flat, mx, dx, dx, 1e-3, 1e9)
+    g = q.copy()
+    g[0, 5, 0] = 1.0                        # outer ghost layer: no interior difference reaches it
+    assert not eta_gradient_tags(g, flat, mx, dx, dx, 1e-3, 0.1)
+    g[1, 5, 0] = 1.0                        # inner ghost layer: (eta(i,1) - eta(i,-1)) / (2 dy) = -1
+    assert eta_gradient_tags(g, flat, mx, dx, dx, 1e-3, 0.99)
+    assert not eta_gradient_tags(g, flat, mx, dx, dx, 1e-3, 1.01)

@@ -7,7 +7,7 @@ the reference Fortran build for the same method sets
 is compared with it:
 
   * the drop-in batch step (k_step) on random wet/dry patches, one step,
-    to the single-step bar (1e-13; src2's pow() may differ by an ulp);
+    to the single-step bar (1e-13);
   * 20 accepted steps of a wet case (no front) and of a dry-bed dam break
     through the batched graph path (k_all, halo pull, device dt control) and
     the per-patch launch loop, to the run bar (dt sequence 1e-12, per cell

@@ -373,3 +373,37 @@ def test_brick_run_dt_bound(oracle_c):
     for k in range(1, 12):
         assert rec["dt"][k] == min(rec["dt"][k - 1] * 0.9 / rec["cfl"][k - 1], dt_max)
     assert (rec["dt"][1:] == dt_max).any() and (rec["dt"][1:] < dt_max).any()
+
+
+def test_correctly_rounded_pow_mode(oracle_c):
+    """The oracle's correctly rounded powers (gfo_set_pow_mode(1), the
+    device's arithmetic): equal to 60-digit decimal arithmetic, and a patch
+    update in that mode is bit for bit the default (libm) update whenever the
+    libm's pow rounds correctly on the patch's depths."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 60
+    rng = np.random.default_rng(11)
+    x = np.exp(rng.uniform(np.log(1e-3), np.log(20.0), 2000))
+    try:
+        oracle_c.set_pow_mode(True)
+        for y in (7.0 / 3.0, 2.0 / 3.0):
+            exact = np.array([float(Decimal(float(v)) ** Decimal(y)) for v in x])
+            assert np.array_equal(oracle_c.pow(x, y), exact)
+        q = np.zeros((36, 36, 3))
+        h = 0.1 + rng.random((36, 36))
+        q[..., 0] = h
+        q[..., 1:] = rng.normal(0, 1, (36, 36, 2)) * h[..., None]
+        aux = np.ascontiguousarray(rng.normal(0, 0.2, (36, 36, 1)))
+        qa = q.copy()
+        oracle_c.patch_update(32, 32, 2, qa, aux, 0.05, 0.05, 0.0, 0.002)
+        hc = qa[..., 0].copy()
+        cr = oracle_c.pow(hc, 7.0 / 3.0)
+        oracle_c.set_pow_mode(False)
+        lm = oracle_c.pow(hc, 7.0 / 3.0)
+        qb = q.copy()
+        oracle_c.patch_update(32, 32, 2, qb, aux, 0.05, 0.05, 0.0, 0.002)
+        differ = (cr != lm).reshape(36, 36)[2:-2, 2:-2]
+        same_cells = ~differ
+        assert same(qa[2:-2, 2:-2][same_cells], qb[2:-2, 2:-2][same_cells])
+    finally:
+        oracle_c.set_pow_mode(False)
