@@ -427,12 +427,17 @@ def main():
         achieved = k_bytes / (k_avg_ms * 1e-3) / 1e9
         eff_bytes = BYTES_PER_CELL_UPDATE * MX * MX * (act + dry) / launches
         effective = eff_bytes / (k_avg_ms * 1e-3) / 1e9
-        # measured HBM bytes and VALU work per launch of this same trajectory
-        # (rocprofv3 --pmc over the 500 event-timed launches, profiles/)
+        # measured HBM bytes and VALU work per launch (rocprofv3 --pmc over
+        # the 20 event-timed launches of the --steps 20 --warmup 5 trajectory,
+        # profiles/); another trajectory scales the bytes by its touched
+        # patches per launch
         traffic = valu = None
         kv = profile_values("k_all_32x32_uniform4096_traffic.txt") if MX == 32 else {}
         if "hbm_bytes_per_launch" in kv:
-            traffic = round(float(kv["hbm_bytes_per_launch"]))
+            traffic = float(kv["hbm_bytes_per_launch"])
+            if "touched_patches_per_launch" in kv and touched > 0:
+                traffic *= touched / float(kv["touched_patches_per_launch"])
+            traffic = round(traffic)
         kv = profile_values("k_all_32x32_uniform4096_valu.txt") if MX == 32 else {}
         valu_issue_frac = fp64_frac = None
         if "fp64_flops_per_launch" in kv:

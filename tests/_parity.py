@@ -1,9 +1,14 @@
-"""Per-cell comparison of device and oracle states (test helper).
+"""Comparison of device and oracle results (test helper).
 
-The device and the oracle evaluate every FP64 value with the same operations
-in the same order, except pow() in src2 and in the hydrograph depth, where
-the device library and the host libm may round differently by an ulp.  Such
-differences are carried by the flow, so long runs are compared per cell:
+assert_same: the device and the oracle evaluate every FP64 value with the
+same operations in the same order (no FMA contraction on either side, the
+two powers correctly rounded on both when the oracle's pow mode is set), so
+device runs are held to identical values: every dt, CFL and cell.
+
+state_diff / assert_state_close: the per-cell measure used where two sides
+may round differently -- the reference build's libm pow against correctly
+rounded powers.  Such differences are carried by the flow, so long runs are
+compared per cell:
 
 * the wet/dry mask (h > dry_tolerance) must be identical;
 * wet cells: |dh| / max(|h|) <= rtol, and for each momentum component
@@ -41,6 +46,18 @@ def state_diff(got, ref, drytol=1e-3, grav=9.81):
     if dry.any():
         dry_abs = float(max(np.max(np.abs(got[m][dry] - ref[m][dry])) for m in range(3))) / drytol
     return rel, dry_abs, mask_equal
+
+
+def assert_same(got, ref, what=""):
+    """Identical values, element by element (NaN equal to NaN, +0 to -0)."""
+    g = np.asarray(got, dtype=np.float64)
+    r = np.asarray(ref, dtype=np.float64)
+    assert g.shape == r.shape, f"{what} shapes differ: {g.shape} vs {r.shape}"
+    if not np.array_equal(g, r, equal_nan=True):
+        bad = ~((g == r) | (np.isnan(g) & np.isnan(r)))
+        i = np.unravel_index(np.argmax(bad), g.shape)
+        raise AssertionError(f"{what} {int(bad.sum())} of {g.size} values differ; first at {tuple(map(int, i))}: "
+                             f"{g[i]!r} vs {r[i]!r}")
 
 
 def assert_state_close(got, ref, rtol, drytol=1e-3, grav=9.81):

@@ -7,15 +7,15 @@ device dt control); every few steps the leaves are tagged on the device and
 the forest is rebuilt in the other half of the pool by slot moves,
 interpolate2fine and average2coarse.  oracle.amr_driver.OracleAmr runs the
 same cycle with the oracle's routines over NaN-filled ghost cells.  The run
-bar: identical forests and tags at every regrid, dt sequence to 1e-12, per
-cell 1e-10 (tests/_parity.py), identical wet/dry mask."""
+bar: identical forests and tags at every regrid, identical dt sequence and
+identical values in every cell."""
 import numpy as np
 import pytest
 
 from geoflood_amd.amr import AmrRun
 from oracle.amr_driver import OracleAmr, eta_gradient_tags
 from oracle.oracle import Params
-from _parity import assert_state_close
+from _parity import assert_same
 from test_amr_forest import band_flags
 
 pytestmark = pytest.mark.gpu
@@ -44,11 +44,11 @@ def _compare(dev: AmrRun, ora: OracleAmr, dts):
     assert dev.forest.same(ora.forest)
     info = dev.pool.scenario(dev.sid)
     assert info["retakes"] == ora.retakes and info["nsteps"] == ora.nsteps
-    np.testing.assert_allclose(dts, ora.dts, rtol=1e-12, atol=0)
+    assert_same(dts, ora.dts)
     q, _ = dev.leaves_state()
     got = np.moveaxis(q[:, 2:-2, 2:-2], -1, 1)
     ref = ora.state()
-    assert_state_close(np.moveaxis(got, 1, 0), np.moveaxis(ref, 1, 0), 1e-10)
+    assert_same(np.moveaxis(got, 1, 0), np.moveaxis(ref, 1, 0))
 
 
 def _run_both(dev, ora, steps):

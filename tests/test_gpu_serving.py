@@ -7,7 +7,7 @@ import pytest
 from oracle.oracle import Brick, Hydrograph, Params
 from geoflood_amd.serving import DT0, DT_MAX, LENGTH, MX, STRIP, T_START, Engine, Request
 from geoflood_amd.scenarios import gaussian_sum_dem, sine_hydrograph
-from _parity import assert_state_close
+from _parity import assert_same
 
 pytestmark = pytest.mark.gpu
 
@@ -85,4 +85,4 @@ def test_served_scenario_matches_oracle(gf, oracle_c):
         oracle_c.set_params(Params())
     assert rec["retakes"] == res.retakes
     assert abs((T_START + rec["dt"].sum()) - res.t) <= 1e-9 * res.t
-    assert_state_close(got, br.to_global(), 1e-10)
+    assert_same(got, br.to_global())

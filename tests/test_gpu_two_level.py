@@ -17,7 +17,7 @@ import pytest
 
 from geoflood_amd import GF_NB_STORED, GF_NB_WALL
 from geoflood_amd.pool import Pool
-from _parity import assert_state_close
+from _parity import assert_same
 
 pytestmark = pytest.mark.gpu
 
@@ -181,14 +181,14 @@ def test_two_level_run_matches_oracle(gf, oracle_c, where):
             c_ref = 0.0
             for s in range(n):
                 c_ref = max(c_ref, oracle_c.patch_update(MX, MX, MBC, oq[s], oa[s], dxs[s], dxs[s], t, dt))
-            assert c_dev == pytest.approx(c_ref, rel=1e-13, abs=0), step
+            assert c_dev == c_ref, step
             assert 0.05 < c_ref < 1.0
             cur, t = 1 - cur, t + dt
         got = pool.download_aos(0, n, buf=cur)
         inner = (slice(MBC, MBC + MX), slice(MBC, MBC + MX))
         g = np.stack([np.moveaxis(got[s][inner], -1, 0) for s in range(n)], axis=1)
         r = np.stack([np.moveaxis(oq[s][inner], -1, 0) for s in range(n)], axis=1)
-        assert_state_close(g, r, 1e-10)
+        assert_same(g, r)
         assert (r[0] > 1e-3).any() and (r[0] <= 1e-3).any()   # wet and dry cells both present
     finally:
         pool.close()
@@ -263,13 +263,13 @@ def test_two_level_subcycled_with_time_interpolation(gf, oracle_c):
             oracle_ghosts(oracle_c, lv, nbr, links, mixed, oa, check=fine)
             rf2 = max(oracle_c.patch_update(MX, MX, MBC, oq[s], oa[s], DXF, DXF, t + dtf, dtf) for s in fine)
             for got, want in ((cc, rc), (cf1, rf1), (cf2, rf2)):
-                assert got == pytest.approx(want, rel=1e-13, abs=0), step
+                assert got == want, step
             t += dtc
         got = pool.download_aos(0, nl, buf=b)
         inner = (slice(MBC, MBC + MX), slice(MBC, MBC + MX))
         g = np.stack([np.moveaxis(got[s][inner], -1, 0) for s in range(nl)], axis=1)
         r = np.stack([np.moveaxis(oq[s][inner], -1, 0) for s in range(nl)], axis=1)
-        assert_state_close(g, r, 1e-10)
+        assert_same(g, r)
     finally:
         pool.close()
 
@@ -340,7 +340,7 @@ def test_regrid_cycle_matches_oracle(gf, oracle_c):
                 views = [oq[s] for s in sl]
                 oracle_ghosts(oracle_c, lvp, nb, links, views, [oa[s] for s in sl])
                 ref = max(oracle_c.patch_update(MX, MX, MBC, oq[s], oa[s], dxs[s], dxs[s], t, dt) for s in sl)
-                assert max(cfl[s] for s in sl) == pytest.approx(ref, rel=1e-13, abs=0)
+                assert max(cfl[s] for s in sl) == ref
                 b, t = 1 - b, t + dt
             return lvp, nb, links, sl
 
@@ -365,7 +365,7 @@ def test_regrid_cycle_matches_oracle(gf, oracle_c):
         inner = (slice(MBC, MBC + MX), slice(MBC, MBC + MX))
         g = np.stack([np.moveaxis(got[s][inner], -1, 0) for s in range(NC * NC)], axis=1)
         r = np.stack([np.moveaxis(oq[s][inner], -1, 0) for s in range(NC * NC)], axis=1)
-        assert_state_close(g, r, 1e-10)
+        assert_same(g, r)
     finally:
         pool.close()
 
@@ -442,11 +442,11 @@ def test_three_levels_match_oracle(gf, oracle_c):
                         oracle_c.interpolate_ghost(oq[c], oq[f], oa[c], oa[f], side, ig)
             assert all(np.isfinite(q).all() for q in oq), "a ghost region is not filled"
             c_ref = max(oracle_c.patch_update(MX, MX, MBC, oq[s], oa[s], dxs[s], dxs[s], t, dt) for s in range(n))
-            assert c_dev == pytest.approx(c_ref, rel=1e-13, abs=0), step
+            assert c_dev == c_ref, step
             cur, t = 1 - cur, t + dt
         got = pool.download_aos(0, n, buf=cur)
         g = np.stack([np.moveaxis(got[s][interior], -1, 0) for s in range(n)], axis=1)
         r = np.stack([np.moveaxis(oq[s][interior], -1, 0) for s in range(n)], axis=1)
-        assert_state_close(g, r, 1e-10)
+        assert_same(g, r)
     finally:
         pool.close()
