@@ -56,6 +56,7 @@ BYTES_PER_CELL_UPDATE = 56  # read h, hu, hv, b; write h, hu, hv (FP64)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 (FMA = 2 FLOPs)
 VALU_LANE_PEAK_T = 39.3          # 256 CUs x 64 lanes x 2.4 GHz, lane-instructions per second (T)
+SIMDS = 1024                     # 256 CUs x 4 SIMDs
 PARAMS = dict(grav=9.81, dry_tolerance=1e-3, manning_n=0.03, friction_depth=1.0e6,
               friction_forcing=1, src_term=1, order=2, transverse=2, mthlim=(4, 4, 4))
 DT0, DESIRED_CFL, MAX_CFL = 0.01, 0.9, 1.0
@@ -449,6 +450,12 @@ def main():
                         valu_lane_peak_tinstr_per_s=VALU_LANE_PEAK_T, valu_issue_frac=valu_issue_frac,
                         note="FP64 FLOPs (FMA = 2) and VALU lane-instructions per launch from the "
                              "profiles/k_all_32x32_uniform4096_valu.txt counters, over this run's k_all time")
+            if "valu_active_quad_cycles_per_launch" in kv and "clock_ghz" in kv:
+                # SQ_ACTIVE_INST_VALU over the SIMD quad-cycles of this run's
+                # k_all time at the clock measured in the counter runs
+                simd_quads = SIMDS * k_avg_ms * 1e-3 * float(kv["clock_ghz"]) * 1e9 / 4.0
+                valu["valu_busy_frac"] = round(float(kv["valu_active_quad_cycles_per_launch"]) / simd_quads, 4)
+                valu["clock_ghz"] = float(kv["clock_ghz"])
         # The active tiles are bound by the FP64 VALU (two augmented Riemann
         # solves with ~17 correctly rounded divisions and ~8 square roots per
         # face): "bound" names that; achieved / frac are the HBM figures of the
@@ -456,6 +463,7 @@ def main():
         roofline = dict(bound="valu", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                         frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                         valu_issue_frac=valu_issue_frac, fp64_frac=fp64_frac,
+                        valu_busy_frac=(valu or {}).get("valu_busy_frac"),
                         kernel="k_all<%d,%d,%d,%d,%d,%d,%d>" % ((MX, MX) + tuple(variants[pool.variants()[0]])),
                         kernel_avg_ms=round(k_avg_ms, 5), bytes_per_launch=round(k_bytes),
                         touched_patches_per_launch=round(touched, 1),

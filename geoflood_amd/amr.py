@@ -279,6 +279,12 @@ class AmrRun:
 
     # -- layout -----------------------------------------------------------------------
     def _dx(self, L):
+        """Cell sizes (dx, dy) of level L, or of every level of an integer
+        array L (the same IEEE division per element)."""
+        if np.ndim(L):
+            L = np.asarray(L, dtype=np.int64)
+            return (self.lx / ((np.left_shift(self.forest.nx0, L) * self.mx).astype(np.float64)),
+                    self.ly / ((np.left_shift(self.forest.ny0, L) * self.mx).astype(np.float64)))
         return self.lx / ((self.forest.nx0 << L) * self.mx), self.ly / ((self.forest.ny0 << L) * self.mx)
 
     def cell_centres(self, L, i, j):
@@ -286,8 +292,8 @@ class AmrRun:
         included: two arrays of shape (n, mx+4, mx+4)."""
         m = self.mx
         c = np.arange(-MBC, m + MBC) + 0.5
-        dx = np.array([self._dx(l)[0] for l in L])[:, None]
-        dy = np.array([self._dx(l)[1] for l in L])[:, None]
+        dx, dy = self._dx(np.asarray(L))
+        dx, dy = dx[:, None], dy[:, None]
         xs = (np.asarray(i)[:, None] * m + c[None, :]) * dx
         ys = (np.asarray(j)[:, None] * m + c[None, :]) * dy
         return np.broadcast_to(xs[:, None, :], (len(L), m + 2 * MBC, m + 2 * MBC)), \
@@ -320,8 +326,7 @@ class AmrRun:
         nb = nbr[inv]
         nb = np.where(nb >= 0, base + perm[np.maximum(nb, 0)], nb).astype(np.int32)
         L, i, j = forest.L[inv], forest.i[inv], forest.j[inv]
-        dx = np.array([self._dx(l)[0] for l in L])
-        dy = np.array([self._dx(l)[1] for l in L])
+        dx, dy = self._dx(L)
         xlo, ylo = i * self.mx * dx, j * self.mx * dy
         from . import check, dptr, iptr
         d = [np.ascontiguousarray(a, dtype=np.float64) for a in (dx, dy, xlo, ylo)]

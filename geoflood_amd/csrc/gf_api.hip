@@ -143,6 +143,18 @@ private:
     bool quit_ = false;
 };
 
+// The solver's constants of grav and dry_tolerance (gf::Params::halfg ...),
+// each formed exactly as the per-face expression it replaces.
+static void derive_params(gf::Params& P)
+{
+    P.halfg = 0.5 * P.grav;
+    const double c = P.dry_tolerance * P.grav;
+    P.crit = c > 1e-6 ? c : 1e-6;
+    P.crit2 = std::sqrt(P.crit);
+    P.critsq = P.crit * P.crit;
+    P.inv16g = 1.0 / (16.0 * P.grav);
+}
+
 struct Lib {
     bool init = false;
     int device = 0;
@@ -266,6 +278,7 @@ static void default_params()
     g.P.order = p.order;
     g.P.transverse = p.transverse;
     for (int k = 0; k < 3; k++) g.P.mthlim[k] = p.mthlim[k];
+    derive_params(g.P);
 }
 
 static int need_init()
@@ -392,6 +405,7 @@ try {
     g.P.order = p->order;
     g.P.transverse = p->transverse;
     for (int k = 0; k < 3; k++) g.P.mthlim[k] = p->mthlim[k];
+    derive_params(g.P);
     g.params_gen += 1;
     return GF_OK;
 } GF_CATCH_STATUS

@@ -26,6 +26,13 @@ struct Params {
     int order;       // method(2)
     int transverse;  // method(3)
     int mthlim[3];
+    // Formed once on the host from grav and dry_tolerance with the operations
+    // the solver would otherwise repeat per face: halfg = 0.5*grav (the .5*g
+    // of rpn2 and riemanntype), crit = max(drytol*g, 1d-6), crit2 = sqrt(crit)
+    // and critsq = crit*crit (riemann_aug_JCP's criticaltol, criticaltol_2
+    // and criticaltol**2, geoclaw_riemann_utils.f:135-136,177), inv16g =
+    // 1/(16*grav) (riemanntype's two-rarefaction depth).
+    double halfg, crit, crit2, critsq, inv16g;
 };
 
 __device__ __forceinline__ double fmaxf64(double a, double b) { return a > b ? a : b; }
@@ -36,7 +43,7 @@ __device__ __forceinline__ double fminf64(double a, double b) { return a < b ? a
 // sgL, sgR are sqrt(g*hL), sqrt(g*hR), which the caller has already formed:
 // every sqrt(g*h) of hL, hR, h_min or h_max below is one of them, bit for bit.
 __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, double uR,
-                                            double sgL, double sgR, double drytol, double g,
+                                            double sgL, double sgR, double drytol, double g, double halfg, double inv16g,
                                             double& hm, double& s1m, double& s2m)
 {
     const double h_min = fminf64(hR, hL);
@@ -51,7 +58,7 @@ __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, dou
         return;
     }
     const double F_min = delu + 2.0 * (sg_min - sg_max);
-    const double F_max = delu + (h_max - h_min) * (sqrt(.5 * g * (h_max + h_min) / (h_max * h_min)));
+    const double F_max = delu + (h_max - h_min) * (sqrt(halfg * (h_max + h_min) / (h_max * h_min)));
     // The three branches leave hm (and, for two shocks, u1m, u2m); sqrt(g*hm)
     // and the speeds are formed once after them, so that a wave whose lanes
     // take different branches pays for the shared part only once.
@@ -59,12 +66,12 @@ __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, dou
     int kind;  // 0: two rarefactions, 1: two shocks, 2: one shock, one rarefaction
     if (F_min > 0.0) {
         const double t = fmaxf64(0.0, -delu + 2.0 * (sgL + sgR));
-        hm = (1.0 / (16.0 * g)) * (t * t);
+        hm = inv16g * (t * t);
         kind = 0;
     } else if (F_max <= 0.0) {  // one Newton step on sqrt(h)
         double h0 = h_max;
-        const double gL = sqrt(.5 * g * (1 / h0 + 1 / hL));
-        const double gR = sqrt(.5 * g * (1 / h0 + 1 / hR));
+        const double gL = sqrt(halfg * (1 / h0 + 1 / hL));
+        const double gR = sqrt(halfg * (1 / h0 + 1 / hR));
         const double F0 = delu + (h0 - hL) * gL + (h0 - hR) * gR;
         const double dfdh = gL - g * (h0 - hL) / (4.0 * (h0 * h0) * gL) + gR -
                             g * (h0 - hR) / (4.0 * (h0 * h0) * gR);
@@ -72,8 +79,8 @@ __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, dou
         const double t = sqrt(h0) - F0 / slope;
         h0 = t * t;
         hm = h0;
-        u1m = uL - (hm - hL) * sqrt((.5 * g) * (1 / hm + 1 / hL));
-        u2m = uR + (hm - hR) * sqrt((.5 * g) * (1 / hm + 1 / hR));
+        u1m = uL - (hm - hL) * sqrt(halfg * (1 / hm + 1 / hL));
+        u2m = uR + (hm - hR) * sqrt(halfg * (1 / hm + 1 / hR));
         kind = 1;
     } else {
         // F0 = delu + 2(sqrt(g h0) - sqrt(g h_max)) + (h0 - h_min) sqrt(g/2 (1/h0 + 1/h_min))
@@ -82,7 +89,7 @@ __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, dou
         double h0 = h_min;
         const double F0 = (h_min >= 1e-150 && g <= 1e150)
                               ? F_min + 0.0
-                              : delu + 2.0 * (sg_min - sg_max) + (h0 - h_min) * sqrt(.5 * g * (1 / h0 + 1 / h_min));
+                              : delu + 2.0 * (sg_min - sg_max) + (h0 - h_min) * sqrt(halfg * (1 / h0 + 1 / h_min));
         const double slope = (F_max - F0) / (h_max - h_min);
         h0 = h0 - F0 / slope;
         hm = h0;
@@ -112,20 +119,20 @@ __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, dou
 // positive F takes the two-rarefaction branch and any other the two-shock
 // branch with gL = gR and dfdh = gL + gL.  Outside the range where those
 // zeros are exact (g/h overflowing) the general routine is used.
-__device__ __forceinline__ double wall_hstar(double h, double u, double sg, double drytol, double g)
+__device__ __forceinline__ double wall_hstar(double h, double u, double sg, double drytol, double g, double halfg, double inv16g)
 {
     if (!(h >= 1e-150 && g <= 1e150) || h <= drytol) {
         double hm, s1m, s2m;
-        riemanntype(h, h, u, -u, sg, sg, drytol, g, hm, s1m, s2m);
+        riemanntype(h, h, u, -u, sg, sg, drytol, g, halfg, inv16g, hm, s1m, s2m);
         return hm;
     }
     const double delu = -u - u;
     const double F = delu + 0.0;
     if (F > 0.0) {
         const double t = fmaxf64(0.0, -delu + 2.0 * (sg + sg));
-        return (1.0 / (16.0 * g)) * (t * t);
+        return inv16g * (t * t);
     }
-    const double gL = sqrt(.5 * g * (1 / h + 1 / h));
+    const double gL = sqrt(halfg * (1 / h + 1 / h));
     const double F0 = (delu + 0.0) + 0.0;
     const double dfdh = gL + gL;
     const double slope = 2.0 * sqrt(h) * dfdh;
@@ -148,9 +155,10 @@ struct RightCell {
 template <bool RC = false>
 __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, double bL,
                                           double hR, double huR, double hvR, double bR,
-                                          double g, double drytol,
+                                          const Params& P,
                                           double s[3], double w[3][3], const RightCell& rc = RightCell{})
 {
+    const double g = P.grav, drytol = P.dry_tolerance, halfg = P.halfg;
     if (hL <= drytol && hR <= drytol) {
 #pragma unroll
         for (int mw = 0; mw < 3; mw++) {
@@ -164,14 +172,14 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
     if (hR > drytol) {
         uR = RC ? rc.u : huR / hR;
         vR = RC ? rc.v : hvR / hR;
-        phiR = 0.5 * g * (hR * hR) + (huR * huR) / hR;
+        phiR = halfg * (hR * hR) + (huR * huR) / hR;
     } else {
         hR = 0.0; huR = 0.0; hvR = 0.0; uR = 0.0; vR = 0.0; phiR = 0.0;
     }
     if (hL > drytol) {
         uL = huL / hL;
         vL = hvL / hL;
-        phiL = 0.5 * g * (hL * hL) + (huL * huL) / hL;
+        phiL = halfg * (hL * hL) + (huL * huL) / hL;
     } else {
         hL = 0.0; huL = 0.0; hvL = 0.0; uL = 0.0; vL = 0.0; phiL = 0.0;
     }
@@ -183,7 +191,7 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
     if (hR <= drytol || hL <= drytol) {
         const double hw = hR <= drytol ? hL : hR;
         const double uw = hR <= drytol ? uL : -uR;
-        hstar = wall_hstar(hw, uw, sqrt(g * hw), drytol, g);
+        hstar = wall_hstar(hw, uw, sqrt(g * hw), drytol, g, halfg, P.inv16g);
     }
     if (hR <= drytol) {
         const double hstartest = fmaxf64(hL, hstar);
@@ -208,7 +216,7 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
     const double sL = uL - sghL;
     const double sR = uR + sghR;
     const double uhat = (sghL * uL + sghR * uR) / (sghR + sghL);
-    const double chat = sqrt(g * 0.5 * (hR + hL));
+    const double chat = sqrt(halfg * (hR + hL));
     const double sRoe1 = uhat - chat;
     const double sRoe2 = uhat + chat;
     double sE1 = fminf64(sL, sRoe1);
@@ -221,7 +229,7 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
     const double delb = bR - bL;
 
     double hm, s1m, s2m;
-    riemanntype(hL, hR, uL, uR, sghL, sghR, drytol, g, hm, s1m, s2m);
+    riemanntype(hL, hR, uL, uR, sghL, sghR, drytol, g, halfg, P.inv16g, hm, s1m, s2m);
 
     const double lam0 = fminf64(sE1, s2m);
     const double lam2 = fmaxf64(sE2, s1m);
@@ -234,8 +242,8 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
     const double r10 = lam0, r20 = lam0 * lam0;
     const double r12 = lam2, r22 = lam2 * lam2;
 
-    const double criticaltol = fmaxf64(drytol * g, 1e-6);
-    const double criticaltol_2 = sqrt(criticaltol);
+    const double criticaltol = P.crit;
+    const double criticaltol_2 = P.crit2;
 
     const double hbar = fmaxf64(0.5 * (hL + hR), 0.0);
     const double tu = uL + uR;
@@ -244,8 +252,8 @@ __device__ __forceinline__ bool rpn2_face(double hL, double huL, double hvL, dou
 
     bool sonic;
     if (fabs(s1s2bar) <= criticaltol) sonic = true;
-    else if (s1s2bar * s1s2tilde <= criticaltol * criticaltol) sonic = true;
-    else if (s1s2bar * sE1 * sE2 <= criticaltol * criticaltol) sonic = true;
+    else if (s1s2bar * s1s2tilde <= P.critsq) sonic = true;
+    else if (s1s2bar * sE1 * sE2 <= P.critsq) sonic = true;
     else if (fminf64(fabs(sE1), fabs(sE2)) < criticaltol_2) sonic = true;
     else if (sE1 < criticaltol_2 && s1m > -criticaltol_2) sonic = true;
     else if (sE2 > -criticaltol_2 && s2m < criticaltol_2) sonic = true;

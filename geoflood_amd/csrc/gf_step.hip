@@ -419,7 +419,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
         if (in[s] && j <= TY + 1) {
             const int kL = G::ti(i - 1, j), kR = G::ti(i, j);
             rpn2_face<OWN>(T[kL], T[NP2 + kL], T[2 * NP2 + kL], T[3 * NP2 + kL],
-                           T[kR], T[NP2 + kR], T[2 * NP2 + kR], T[3 * NP2 + kR], g, drytol, sw, w,
+                           T[kR], T[NP2 + kR], T[2 * NP2 + kR], T[3 * NP2 + kR], P, sw, w,
                            RightCell{CU[s], CV[s], CS[s]});
 #pragma unroll
             for (int mw = 0; mw < 3; mw++) {
@@ -538,7 +538,7 @@ __device__ __forceinline__ void step_patch(double* T, double* E, double* F, doub
             const int kL = G::ti(i, j - 1), kR = G::ti(i, j);
             // direction-local order for y: (h, hv, hu)
             rpn2_face<OWN>(T[kL], T[2 * NP2 + kL], T[NP2 + kL], T[3 * NP2 + kL],
-                           T[kR], T[2 * NP2 + kR], T[NP2 + kR], T[3 * NP2 + kR], g, drytol, sw, w,
+                           T[kR], T[2 * NP2 + kR], T[NP2 + kR], T[3 * NP2 + kR], P, sw, w,
                            RightCell{CV[s], CU[s], CS[s]});
 #pragma unroll
             for (int mw = 0; mw < 3; mw++) {
