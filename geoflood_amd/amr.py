@@ -128,9 +128,45 @@ class Forest:
         coarsen[k] (and no refine) merge into their parent (above minlevel);
         then the balance closure, which may refine further leaves or keep a
         family from merging."""
-        M = self.maxlevel
         refine = np.asarray(refine, dtype=bool)
         coarsen = np.asarray(coarsen, dtype=bool) & ~refine
+        if not self._changes(refine, coarsen):
+            return self
+        return self._adapt(refine, coarsen)
+
+    def _changes(self, refine: np.ndarray, coarsen: np.ndarray) -> bool:
+        """Whether adapt can change the forest.  It cannot when no leaf below
+        maxlevel is to refine and every family of four sibling leaves (level
+        L > minlevel) that is to coarsen has a face or corner neighbour of
+        its parent's block finer than L: merging it would put level L-1 next
+        to level L+1, which the balance closure undoes, and such a neighbour
+        cannot get coarser in this regrid unless its own family merges --
+        which, being subject to the same test, would have made this return
+        True.  The requested levels are then the forest's own, a balanced
+        tree, which the closure returns as it is."""
+        M = self.maxlevel
+        if (refine & (self.L < M)).any():
+            return True
+        c = coarsen & (self.L > self.minlevel)
+        if not c.any():
+            return False
+        key = (self.L[c] << 48) | ((self.j[c] >> 1) << 24) | (self.i[c] >> 1)
+        keys, counts = np.unique(key, return_counts=True)
+        fams = keys[counts == 4]
+        if len(fams) == 0:
+            return False
+        p = np.pad(self.lev, 1, constant_values=-1)
+        nb = np.max([p[1 + dy:1 + dy + self.ny, 1 + dx:1 + dx + self.nx] for dx, dy in DIRS], axis=0)
+        FL, fj, fi = fams >> 48, (fams >> 24) & 0xFFFFFF, fams & 0xFFFFFF
+        for L in np.unique(FL):
+            sel = FL == L
+            finest_near = _block(nb, 1 << (M - int(L) + 1)).max(axis=(1, 3))
+            if (finest_near[fj[sel], fi[sel]] <= L).any():
+                return True
+        return False
+
+    def _adapt(self, refine: np.ndarray, coarsen: np.ndarray) -> "Forest":
+        M = self.maxlevel
         lev = self.lev
         req = lev + (refine[self.leaf_at] & (lev < M))
         cc = coarsen[self.leaf_at]

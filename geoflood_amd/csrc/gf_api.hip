@@ -501,7 +501,7 @@ try {
         return nullptr;
     }
     if (!gf::supported_shape(mx, my)) {
-        fail(GF_EUNSUPPORTED, "patch shape %dx%d not supported (8x8, 16x16, 32x32)", mx, my);
+        fail(GF_EUNSUPPORTED, "patch shape %dx%d not supported (8x8, 16x16, 32x32, 64x64)", mx, my);
         return nullptr;
     }
     if (capacity <= 0) {

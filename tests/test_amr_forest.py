@@ -169,3 +169,35 @@ def test_eta_gradient_criterion_closed_forms():
     g[1, 5, 0] = 1.0                        # inner ghost layer: (eta(i,1) - eta(i,-1)) / (2 dy) = -1
     assert eta_gradient_tags(g, flat, mx, dx, dx, 1e-3, 0.99)
     assert not eta_gradient_tags(g, flat, mx, dx, dx, 1e-3, 1.01)
+
+
+def test_adapt_shortcut_equals_full_adapt():
+    """Forest.adapt returns the forest unchanged, without the balance
+    closure, when no leaf below maxlevel is to refine and every family of
+    four siblings to coarsen is kept by a finer neighbour of its parent's
+    block.  On random forests and flags the result always equals the full
+    computation; the shortcut is taken with and without such blocked
+    families, and whenever it is not taken the forest does change."""
+    rng = np.random.default_rng(11)
+    short = blocked = full_changed = 0
+    for trial in range(300):
+        f = Forest(int(rng.integers(1, 4)), int(rng.integers(1, 4)), int(rng.integers(0, 2)), int(rng.integers(2, 5)))
+        for _ in range(int(rng.integers(1, 5))):
+            t = rng.random(len(f)) < rng.uniform(0.05, 0.6)
+            f = f.adapt(t, ~t & (rng.random(len(f)) < 0.5))
+        for _ in range(5):
+            refine = rng.random(len(f)) < rng.choice([0.0, 0.01, 0.3])
+            if rng.random() < 0.7:
+                refine &= f.L == f.maxlevel          # only leaves that cannot refine
+            coarsen = ~refine & (rng.random(len(f)) < rng.choice([0.0, 0.5, 0.9, 1.0]))
+            full = f._adapt(refine, coarsen)
+            got = f.adapt(refine, coarsen)
+            assert got.same(full), trial
+            if got is f:
+                short += 1
+                c = coarsen & (f.L > f.minlevel)
+                key = (f.L[c] << 48) | ((f.j[c] >> 1) << 24) | (f.i[c] >> 1)
+                blocked += bool(len(key)) and bool((np.unique(key, return_counts=True)[1] == 4).any())
+            else:
+                full_changed += not full.same(f)
+    assert short > 0 and blocked > 0 and full_changed > 0
