@@ -299,7 +299,7 @@ def main():
     ap.add_argument("--ensemble-rate", type=float, default=32.0)
     ap.add_argument("--ensemble-capacity", type=int, default=65536)
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--sweep-steps", type=int, default=600,
+    ap.add_argument("--sweep-steps", type=int, default=3000,
                     help="accepted steps per patch size of the autotuner sweep under dynamic AMR (0: skip; "
                          "the workload's full length is 3000)")
     ap.add_argument("--amr-steps", type=int, default=2000,
