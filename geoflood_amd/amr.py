@@ -50,6 +50,16 @@ def _up(a: np.ndarray, s: int) -> np.ndarray:
     return np.repeat(np.repeat(a, s, axis=0), s, axis=1)
 
 
+def _pyramid(a: np.ndarray, levels: int, op) -> list:
+    """[a, op over 2x2 blocks of a, op over 2x2 blocks of that, ...]:
+    levels + 1 arrays, each half the size of the last in both directions."""
+    out = [a]
+    for _ in range(levels):
+        b = out[-1]
+        out.append(op(op(b[0::2, 0::2], b[1::2, 0::2]), op(b[0::2, 1::2], b[1::2, 1::2])))
+    return out
+
+
 class Forest:
     """Leaves of a quadtree over an nx0 x ny0 brick of level-0 patches.
 
@@ -98,15 +108,15 @@ class Forest:
     # -- adaptation -------------------------------------------------------------------
     def _tree(self, req: np.ndarray) -> np.ndarray:
         """The coarsest quadtree whose every finest patch is at least at
-        level req there."""
+        level req there: each finest patch takes the smallest level L whose
+        block around it has max(req) <= L (true at L = maxlevel, and true at
+        every finer level once true at a coarser one)."""
         M = self.maxlevel
-        lev = np.full_like(req, M)
-        done = np.zeros(req.shape, dtype=bool)
-        for L in range(M + 1):
-            s = 1 << (M - L)
-            cond = _up(_block(req, s).max(axis=(1, 3)) <= L, s)
-            lev[cond & ~done] = L
-            done |= cond
+        mx = _pyramid(req, M, np.maximum)      # mx[M - L]: block maxima at level L
+        lev = np.where(mx[M] <= 0, 0, -1)
+        for L in range(1, M + 1):
+            lev = np.repeat(np.repeat(lev, 2, axis=0), 2, axis=1)
+            lev = np.where((lev < 0) & (mx[M - L] <= L), L, lev)
         return lev
 
     def _balance(self, req: np.ndarray) -> np.ndarray:
@@ -116,7 +126,9 @@ class Forest:
         while True:
             lev = self._tree(req)
             p = np.pad(lev, 1, constant_values=-1)
-            nb = np.max([p[1 + dy:1 + dy + self.ny, 1 + dx:1 + dx + self.nx] for dx, dy in DIRS], axis=0)
+            nb = p[1:1 + self.ny, 0:self.nx]
+            for dx, dy in DIRS[1:]:
+                nb = np.maximum(nb, p[1 + dy:1 + dy + self.ny, 1 + dx:1 + dx + self.nx])
             need = nb - 1
             if (lev >= need).all():
                 return lev
@@ -170,11 +182,22 @@ class Forest:
         lev = self.lev
         req = lev + (refine[self.leaf_at] & (lev < M))
         cc = coarsen[self.leaf_at]
-        for L in range(self.minlevel + 1, M + 1):
-            sp = 1 << (M - L + 1)  # the parent's block
-            b = _block(lev, sp)
-            fam = (b.min(axis=(1, 3)) == L) & (b.max(axis=(1, 3)) == L) & _block(cc, sp).all(axis=(1, 3))
-            req[_up(fam, sp)] = L - 1
+        # a family of level L merges where its parent's block (level L-1) is
+        # all level L and all to coarsen; such blocks of different L are
+        # disjoint, so one map of target levels, filled from coarse to fine
+        mn = _pyramid(lev, M, np.minimum)
+        mxp = _pyramid(lev, M, np.maximum)
+        al = _pyramid(cc, M, np.logical_and)
+        target = np.full(mn[M].shape, -1, dtype=np.int64)   # at level 0 resolution
+        for P in range(0, M):                                # parent level P = L - 1
+            if P > 0:
+                target = np.repeat(np.repeat(target, 2, axis=0), 2, axis=1)
+            L = P + 1
+            if L > self.minlevel:
+                k = M - P
+                target = np.where((mn[k] == L) & (mxp[k] == L) & al[k], P, target)
+        target = np.repeat(np.repeat(target, 2, axis=0), 2, axis=1)
+        req = np.where(target >= 0, target, req)
         return Forest(self.nx0, self.ny0, self.minlevel, self.maxlevel, self._balance(req))
 
     # -- tables -------------------------------------------------------------------------

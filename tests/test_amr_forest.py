@@ -201,3 +201,58 @@ def test_adapt_shortcut_equals_full_adapt():
             else:
                 full_changed += not full.same(f)
     assert short > 0 and blocked > 0 and full_changed > 0
+
+
+def _ref_tree(f, req):
+    """The block-by-block form of Forest._tree (each level's block maxima
+    taken over the finest lattice): the reference for the pyramid form."""
+    M = f.maxlevel
+    lev = np.full_like(req, M)
+    done = np.zeros(req.shape, dtype=bool)
+    for L in range(M + 1):
+        s = 1 << (M - L)
+        b = req.reshape(req.shape[0] // s, s, req.shape[1] // s, s).max(axis=(1, 3))
+        cond = np.repeat(np.repeat(b <= L, s, axis=0), s, axis=1)
+        lev[cond & ~done] = L
+        done |= cond
+    return lev
+
+
+def _ref_adapt_levels(f, refine, coarsen):
+    """Forest._adapt's levels written block by block, with a balance closure
+    over _ref_tree."""
+    M = f.maxlevel
+    lev = f.lev
+    req = lev + (refine[f.leaf_at] & (lev < M))
+    cc = coarsen[f.leaf_at]
+    for L in range(f.minlevel + 1, M + 1):
+        sp = 1 << (M - L + 1)
+        b = lev.reshape(lev.shape[0] // sp, sp, lev.shape[1] // sp, sp)
+        c = cc.reshape(cc.shape[0] // sp, sp, cc.shape[1] // sp, sp)
+        fam = (b.min(axis=(1, 3)) == L) & (b.max(axis=(1, 3)) == L) & c.all(axis=(1, 3))
+        req[np.repeat(np.repeat(fam, sp, axis=0), sp, axis=1)] = L - 1
+    req = np.maximum(req, f.minlevel)
+    while True:
+        t = _ref_tree(f, req)
+        p = np.pad(t, 1, constant_values=-1)
+        nb = np.max([p[1 + dy:1 + dy + f.ny, 1 + dx:1 + dx + f.nx] for dx, dy in DIRS], axis=0)
+        if (t >= nb - 1).all():
+            return t
+        req = np.maximum(req, nb - 1)
+
+
+def test_pyramid_forest_equals_blockwise_reference():
+    """The pyramid forms of the tree, the family merge and the balance
+    closure give the same levels as the block-by-block reference on random
+    forests, requests and flags."""
+    rng = np.random.default_rng(21)
+    for trial in range(120):
+        f = Forest(int(rng.integers(1, 4)), int(rng.integers(1, 4)), int(rng.integers(0, 2)), int(rng.integers(1, 5)))
+        for _ in range(int(rng.integers(0, 4))):
+            t = rng.random(len(f)) < rng.uniform(0.05, 0.6)
+            f = f.adapt(t, ~t & (rng.random(len(f)) < 0.5))
+        req = rng.integers(0, f.maxlevel + 1, size=f.lev.shape)
+        assert np.array_equal(f._tree(req), _ref_tree(f, req)), trial
+        refine = rng.random(len(f)) < rng.choice([0.0, 0.1, 0.4])
+        coarsen = ~refine & (rng.random(len(f)) < rng.choice([0.3, 0.8, 1.0]))
+        assert np.array_equal(f._adapt(refine, coarsen).lev, _ref_adapt_levels(f, refine, coarsen)), trial
