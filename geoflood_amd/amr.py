@@ -83,18 +83,18 @@ class Forest:
     def _index(self):
         M = self.maxlevel
         lev = self.lev
-        s = np.left_shift(1, M - lev)
+        m = np.left_shift(1, M - lev) - 1                  # block size - 1 (a power of two - 1)
         I = np.arange(self.nx)[None, :]
         J = np.arange(self.ny)[:, None]
-        jj, ii = np.nonzero((I % s == 0) & (J % s == 0))
+        jj, ii = np.nonzero(((I & m) == 0) & ((J & m) == 0))
         L = lev[jj, ii]
         i, j = ii >> (M - L), jj >> (M - L)
-        order = np.lexsort((i, j, L))
+        order = np.argsort((L << 42) | (j << 21) | i, kind="stable")   # by level, then j, then i
         self.L, self.i, self.j = L[order], i[order], j[order]
         self.oi, self.oj = ii[order], jj[order]          # origins on the finest lattice
         ids = np.full((self.ny, self.nx), -1, dtype=np.int64)
         ids[self.oj, self.oi] = np.arange(len(order))
-        self.leaf_at = ids[J - J % s, I - I % s]          # covering leaf of every finest patch
+        self.leaf_at = ids[J & ~m, I & ~m]                # covering leaf of every finest patch
 
     def __len__(self) -> int:
         return len(self.L)
