@@ -67,10 +67,13 @@ def gaussian_sum_dem(n: int, length: float = 2000.0, valley_slope: float = 0.002
     dx = length / n
     xc = (np.arange(n) + 0.5) * dx
     b = np.broadcast_to(valley_slope * (length - xc)[None, :], (n, n)).copy()
+    tmp = np.empty((n, n))  # a[k] * outer(gy, gx), formed in place
     for k in range(ngauss):
         gx = np.exp(-((xc - cx[k]) ** 2) / (2.0 * w[k] ** 2))
         gy = np.exp(-((xc - cy[k]) ** 2) / (2.0 * w[k] ** 2))
-        b += a[k] * np.outer(gy, gx)
+        np.outer(gy, gx, out=tmp)
+        np.multiply(tmp, a[k], out=tmp)
+        b += tmp
     return b
 
 

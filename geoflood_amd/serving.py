@@ -22,7 +22,8 @@ from __future__ import annotations
 import argparse
 import json
 import time
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor
+from concurrent.futures import wait as futures_wait
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -147,7 +148,9 @@ class Engine:
         self.live: dict[int, _Live] = {}
         self.attempts = 0
         self.cell_updates = 0
-        self.prep = ThreadPoolExecutor(max_workers=2)  # host-side DEM/layout of queued requests
+        # host-side DEM/layout of queued requests (numpy releases the GIL in
+        # the array work, so the threads run in parallel with the stepping)
+        self.prep = ThreadPoolExecutor(max_workers=6)
         self.ready: dict[int, object] = {}
         self.tuned_at = 0  # live patches when the step variant was last chosen
 
@@ -174,9 +177,14 @@ class Engine:
     def footprint_bytes(req: Request) -> int:
         return Pool.bytes_for(MX, MX, req.npatch)
 
-    def admit(self, req: Request, now: float) -> bool:
-        """Place the request in free slots if its footprint fits."""
+    def admit(self, req: Request, now: float, wait: bool = True) -> bool:
+        """Place the request in free slots if its footprint fits (and, with
+        wait False, only once its host-side preparation has finished, so that
+        the engine keeps stepping the live scenarios meanwhile)."""
         if req.npatch > self.alloc.largest():
+            return False
+        pending = self.ready.get(req.rid)
+        if not wait and pending is not None and not pending.done():
             return False
         fut = self.ready.pop(req.rid, None)
         layout, q, aux = fut.result() if fut is not None else self.prepare(req)
@@ -235,17 +243,25 @@ class Engine:
                 queue.append(pending.pop(0))
                 self.enqueue(queue[-1])
             # admit in arrival order; a request that does not fit waits, later
-            # smaller ones may pass it (backfill)
+            # smaller ones may pass it (backfill).  On the wall clock a request
+            # whose DEM is still being prepared waits the same way; on a
+            # deterministic clock admission waits for the preparation.
             still = []
             for r in queue:
                 if r.npatch > self.capacity:
                     raise ValueError(f"request {r.rid} needs {r.npatch} slots > capacity {self.capacity}")
-                if not self.admit(r, tnow):
+                if not self.admit(r, tnow, wait=clock is not None):
                     still.append(r)
             queue = still
             if not self.live:
-                if pending and clock is None:
-                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.01)))
+                if clock is None:
+                    # nothing to step: wait for a queued request's preparation
+                    # or for the next arrival
+                    waiting = [self.ready[r.rid] for r in queue if r.rid in self.ready]
+                    if waiting:
+                        futures_wait(waiting, timeout=0.01, return_when=FIRST_COMPLETED)
+                    elif pending:
+                        time.sleep(max(0.0, min(pending[0].arrival - now(), 0.01)))
                 elif pending:
                     self.attempts += 1  # idle tick of the deterministic clock
                 continue
