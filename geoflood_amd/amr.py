@@ -313,7 +313,7 @@ class AmrRun:
 
     def __init__(self, mx, nx0, ny0, lx, ly, minlevel, maxlevel, dem_fn, capacity, mthbc=(3, 3, 3, 3),
                  hydro=None, eta_tol=0.01, regrid_interval=4, manning_n=None, dt_max=0.0, flags_fn=None,
-                 tune_cache=None):
+                 tune_cache=None, log=None):
         self.mx, self.lx, self.ly = mx, lx, ly
         # flags_fn(forest, tags, regrid_index) -> (refine, coarsen) per leaf;
         # default: refine the tagged leaves, coarsen families of untagged ones
@@ -335,6 +335,11 @@ class AmrRun:
         self.sid = None
         self.stats = dict(regrids=0, attempts=0, steps=0, retakes=0, cell_updates=0, patches=[], device_ms=0.0)
         self.tag_log = []        # the tags of every regrid, in leaf order
+        # log(record): called once per regrid with a JSON-serialisable dict
+        # (the time, dt, CFL and step counters at the regrid, the leaves per
+        # level after it, whether the forest changed) -- the run's line of
+        # ForestClaw's per-step production output (fc2d_geoclaw_run.c:522-527)
+        self.log = log
 
     # -- layout -----------------------------------------------------------------------
     def _dx(self, L):
@@ -466,6 +471,12 @@ class AmrRun:
         refine, coarsen = (tags, ~tags) if self.flags_fn is None else self.flags_fn(old, tags, self.stats["regrids"])
         new = old.adapt(refine, coarsen)
         self.stats["regrids"] += 1
+        if self.log is not None:
+            self.log(dict(regrid=self.stats["regrids"], t=float(t), dt=float(info["dt"]),
+                          last_dt=float(info["last_dt"]), maxcfl=float(info["maxcfl"]), nsteps=int(info["nsteps"]),
+                          retakes=int(info["retakes"]), attempts=int(info["attempts"]), tagged=int(tags.sum()),
+                          leaves=len(new), levels={str(k): v for k, v in new.counts().items()},
+                          changed=not new.same(old)))
         if new.same(old):
             return False
         ob, nb_ = self.half * self.capacity, (1 - self.half) * self.capacity

@@ -73,6 +73,15 @@ class Result:
     status: int = 0   # gf_scenario_info status: 0, or GF_SCEN_NONFINITE when it stopped on a NaN state
 
 
+    def record(self) -> dict:
+        """One JSON-lines record of the request: arrival, admission and finish
+        times (stream seconds), latency, steps, retakes, the final state's
+        summary and status."""
+        return dict(rid=self.rid, n=self.n, arrival=round(self.arrival, 6), admitted=round(self.admitted, 6),
+                    finished=round(self.finished, 6), latency=round(self.latency, 6), steps=self.steps,
+                    retakes=self.retakes, t=self.t, cell_updates=self.steps * self.n * self.n,
+                    max_depth=self.max_depth, wet_cells=self.wet_cells, volume=self.volume, status=self.status)
+
     @property
     def latency(self) -> float:
         return self.finished - self.arrival
@@ -303,12 +312,17 @@ def main():
     ap.add_argument("--substeps", type=int, default=8)
     ap.add_argument("--t-end", type=float, default=T_END)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--jsonl", default=None, help="write one JSON line per finished request to this file")
     args = ap.parse_args()
     init(0)
     set_params(manning_n=0.035)
     reqs = make_requests(args.requests, args.rate, seed=args.seed, t_end=args.t_end)
     eng = Engine(args.capacity, substeps=args.substeps)
     res = eng.serve(reqs, log=lambda m: print(m, flush=True))
+    if args.jsonl:
+        with open(args.jsonl, "w") as f:
+            for r in sorted(res, key=lambda r: r.rid):
+                f.write(json.dumps(r.record()) + "\n")
     s = summarize(res, eng.wall, eng.cell_updates)
     s.update(rate=args.rate, capacity_patches=args.capacity,
              hbm_budget_bytes=Pool.bytes_for(MX, MX, args.capacity), t_end=args.t_end)

@@ -73,7 +73,9 @@ def test_moving_band_regrid_matches_oracle(gf, oracle_c, graph):
     print(f"case amr moving band graph={graph}", flush=True)
     gf.set_params(manning_n=0.03)
     oracle_c.set_params(Params(manning_n=0.03))
-    dev = AmrRun(16, 2, 2, 1.0, 1.0, 0, 2, _basin_dem, 64, eta_tol=1e9, regrid_interval=2, flags_fn=band_flags())
+    log = []
+    dev = AmrRun(16, 2, 2, 1.0, 1.0, 0, 2, _basin_dem, 64, eta_tol=1e9, regrid_interval=2, flags_fn=band_flags(),
+                 log=log.append)
     ora = OracleAmr(oracle_c, 16, 2, 2, 1.0, 1.0, 0, 2, _basin_dem, eta_tol=1e9, regrid_interval=2,
                     flags_fn=band_flags())
     try:
@@ -89,6 +91,10 @@ def test_moving_band_regrid_matches_oracle(gf, oracle_c, graph):
         ora.run(20)
         assert len(set(dev.stats["patches"])) > 2 and dev.stats["regrids"] == 10
         _compare(dev, ora, dts)
+        # one log record per regrid, at the regrid's step, with the new forest
+        assert [r["nsteps"] for r in log] == list(range(2, 21, 2))
+        assert [r["leaves"] for r in log] == dev.stats["patches"][1:]
+        assert any(r["changed"] for r in log) and all(r["maxcfl"] > 0 for r in log)
     finally:
         dev.close()
         gf.set_params()

@@ -31,3 +31,16 @@ def test_slot_allocator_first_fit_and_merge():
     a.release(0, 30)
     a.release(s2, 40)
     assert a.free == [(0, 100)]
+
+
+def test_result_record_is_json_line():
+    """A finished request's record (the --jsonl output): arrival, admission,
+    finish, latency and the step counters, serialisable as one JSON line."""
+    import json
+    from geoflood_amd.serving import Result
+    r = Result(7, 256, 1.5, 1.75, 3.0, 120, 2, 1800.0, 2.5, 900, 1.0e5)
+    line = json.dumps(r.record())
+    back = json.loads(line)
+    assert "\n" not in line
+    assert back["rid"] == 7 and back["latency"] == 1.5 and back["cell_updates"] == 120 * 256 * 256
+    assert back["admitted"] - back["arrival"] == 0.25 and back["status"] == 0
