@@ -118,15 +118,17 @@ def reset(pool, layout, state, target_steps):
     return sid
 
 
-def cpu_baseline(state, nsteps, nthreads, port_only=False):
+def cpu_baseline(state, nsteps, nthreads, port_only=False, cr_pow=False):
     """The reference Fortran step (or, where it was not built or port_only,
     the C restatement) over the first nsteps accepted steps of the same
-    workload."""
+    workload.  cr_pow: the C restatement evaluates its two powers correctly
+    rounded, as the device does."""
     from oracle.oracle import Brick, CLib, Params, ref_lib
     h, hu, hv, b, dx = state
     p = Params(manning_n=PARAMS["manning_n"])
     c = CLib()
     c.set_params(p)
+    c.set_pow_mode(cr_pow)
     ref = None if port_only else ref_lib()
     kind = "port"
     if ref is not None:
@@ -136,6 +138,7 @@ def cpu_baseline(state, nsteps, nthreads, port_only=False):
     t0 = time.perf_counter()
     rec = c.brick_run(br, nsteps, DT0, DESIRED_CFL, MAX_CFL, update=ref, nthreads=nthreads)
     el = time.perf_counter() - t0
+    c.set_pow_mode(False)
     return dict(value=nsteps * N * N / el, seconds=el, kind=kind, rec=rec, state=br.to_global())
 
 
@@ -672,10 +675,15 @@ def main():
         log(f"cpu: {cb['value'] / 1e6:.2f} M cell-updates/s ({cb['kind']}); parity {parity}")
         if cb["kind"] == "reference":
             # the C restatement on the same sample and threads, for comparison
-            cp = cpu_baseline(state, args.cpu_steps, nthreads, port_only=True)
+            # (with the device's correctly rounded powers: its state must equal
+            # the device's bit for bit)
+            cp = cpu_baseline(state, args.cpu_steps, nthreads, port_only=True, cr_pow=True)
             cpu_port = dict(value=cp["value"], unit="cell-updates/s", cores=nthreads, kind="port",
                             sample=cpu["sample"].split(" (")[0] + f" ({cp['seconds']:.1f} s)")
-            log(f"cpu: {cp['value'] / 1e6:.2f} M cell-updates/s (port)")
+            if gpu_first is not None and args.cpu_steps == args.plain_steps:
+                cpu_port["device_state_identical"] = bool(np.array_equal(gpu_first, cp["state"]))
+            log(f"cpu: {cp['value'] / 1e6:.2f} M cell-updates/s (port); device state identical to it: "
+                f"{cpu_port.get('device_state_identical')}")
         cpu_host = dict(model=cpu_model(), nproc=os.cpu_count(), threads_used=nthreads)
 
     out = {
