@@ -460,10 +460,12 @@ class AmrRun:
         return self.pool.tag_eta_gradient(base + self.perm, self.eta_tol, buf=buf)
 
     # -- stepping -----------------------------------------------------------------------
-    def regrid(self):
+    def regrid(self, info=None):
         """Tag the current leaves, adapt the forest and lay it out in the
-        other half of the pool (state in the scenario's current buffer)."""
-        info = self.pool.scenario(self.sid)
+        other half of the pool (state in the scenario's current buffer).
+        info: the scenario's current record, when the caller has just read it."""
+        if info is None:
+            info = self.pool.scenario(self.sid)
         cur, t = info["cur"], info["t"]
         old = self.forest
         tags = self._tags(old, cur, t).astype(bool)
@@ -533,7 +535,7 @@ class AmrRun:
             cells = len(self.forest) * self.mx * self.mx
             self.stats["cell_updates"] += (info["nsteps"] - before) * cells
             if info["nsteps"] % self.regrid_interval == 0:
-                self.regrid()
+                self.regrid(info)
                 self.stats["patches"].append(len(self.forest))
                 if self.tune is not None:
                     self._retune()
