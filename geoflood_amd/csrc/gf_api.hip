@@ -1474,6 +1474,8 @@ try {
     if (!q || !aux || !dx || !dy) return fail(GF_EINVAL, "NULL argument");
     if (maux < 1) return fail(GF_EINVAL, "maux must be >= 1");
     if (!(dt > 0.0)) return fail(GF_EINVAL, "dt must be > 0");
+    if (mbc != gf::MBC) return fail(GF_EUNSUPPORTED, "mbc = %d; only mbc = 2 is supported", mbc);
+    if (!gf::supported_shape(mx, my)) return fail(GF_EUNSUPPORTED, "patch shape %dx%d not supported", mx, my);
     gf_pool* p = g.batch_pool;
     if (!p || p->mx != mx || p->my != my || p->capacity < n) {
         if (p) gf_pool_destroy(p);
