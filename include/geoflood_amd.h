@@ -146,8 +146,11 @@ long long gf_pool_bytes(int mx, int my, int mbc, int capacity);
  * point to patch k's arrays in the reference layout with ghost cells filled
  * by the caller; dx[k], dy[k] are its cell sizes.  *maxcfl receives the
  * maximum over the patches of the per-patch CFL numbers, each capped to 0
- * above 1000 as geoclaw_update does (fc2d_cpucuda.cpp:392).  Replaces
- * cudaclaw_step2_batch (cuda_source/cudaclaw_step2.cu:57-324). */
+ * above 1000 as geoclaw_update does (fc2d_cpucuda.cpp:392).  Returns
+ * GF_EINVAL for n <= 0, NULL arrays, maux < 1, dt <= 0 or a cell size <= 0,
+ * and GF_EUNSUPPORTED for mbc != 2 or a patch shape other than 8x8, 16x16,
+ * 32x32 or 64x64, before any device work.  Replaces cudaclaw_step2_batch
+ * (cuda_source/cudaclaw_step2.cu:57-324). */
 int gf_step2_batch(int mx, int my, int mbc, int maux, int n,
                    double *const *q, double *const *aux,
                    const double *dx, const double *dy,
