@@ -236,6 +236,7 @@ struct gf_pool {
     size_t tag_bytes = 0;
     bool args_dirty = true;
     unsigned args_gen = 0;                 // params generation of d_args and of the graph
+    int args_n = -1, args_nscen = -1;      // the slot and scenario counts in d_args
     std::vector<PatchDesc> h_desc;
     std::vector<ScenCtrl> h_ctrl;
     int nscen = 0;
@@ -1195,7 +1196,7 @@ static gf::BatchArgs host_batch_args(gf_pool* p)
 {
     gf::BatchArgs a{};
     a.desc = p->desc;
-    a.n = p->capacity;  // every slot; undescribed and idle slots carry scenario -1
+    a.n = p->n;         // the described slots; undescribed and idle ones carry scenario -1
     a.capacity = p->capacity;
     a.q0 = p->q[0];
     a.q1 = p->q[1];
@@ -1203,7 +1204,7 @@ static gf::BatchArgs host_batch_args(gf_pool* p)
     a.hmax = p->hmax2;
     a.cflbits = p->cflbits;
     a.ctrl = p->ctrl;
-    a.nscen = p->capacity;  // every scenario slot; idle ones are inactive
+    a.nscen = p->nscen;  // the scenario ids in use so far; removed ones are inactive
     a.list_wet = p->list_wet;
     a.list_dry = p->list_dry;
     a.counters = p->counters;
@@ -1224,12 +1225,18 @@ static int sync_batch_args(gf_pool* p)
         p->args_dirty = true;
         p->graph_dirty = true;
     }
+    // the classify and control kernels read the slot and scenario counts
+    // from the device arguments, so a captured graph stays valid when they
+    // grow or shrink; only the arguments are uploaded again
+    if (p->args_n != p->n || p->args_nscen != p->nscen) p->args_dirty = true;
     if (!p->args_dirty) return GF_OK;
     const gf::BatchArgs a = host_batch_args(p);
     HIPCHK(hipMemcpyAsync(p->d_args, &a, sizeof(a), hipMemcpyHostToDevice, g.stream));
     HIPCHK(hipStreamSynchronize(g.stream));
     p->args_dirty = false;
     p->args_gen = g.params_gen;
+    p->args_n = p->n;
+    p->args_nscen = p->nscen;
     return GF_OK;
 }
 
@@ -1293,17 +1300,16 @@ static int enqueue_ghost_update(gf_pool* p, int buf, double t)
 // step kernel.
 static int enqueue_attempt(gf_pool* p, int k)
 {
-    const gf::BatchArgs a = host_batch_args(p);
     if (p->amr) {
         if (int r = enqueue_ghost_update(p, -1, 0.0)) return r;
     } else if (p->has_inflow) {
         HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, 0, p->capacity, -1, 0.0), inflow_grid(), g.stream));
     }
-    HIPCHK(gf::launch_classify(a, g.stream));
+    HIPCHK(gf::launch_classify(p->d_args, p->capacity, g.stream));
     if (k >= 0) HIPCHK(hipEventRecord(g.ev_step[2 * k], g.stream));
     HIPCHK(gf::launch_all(p->mx, p->my, p->variant, gf::default_method(g.P), p->d_args, wet_grid(p), g.stream));
     if (k >= 0) HIPCHK(hipEventRecord(g.ev_step[2 * k + 1], g.stream));
-    HIPCHK(gf::launch_batch_control(a, g.stream));
+    HIPCHK(gf::launch_batch_control(p->d_args, p->capacity, g.stream));
     return GF_OK;
 }
 
@@ -1773,7 +1779,6 @@ try {
     if (!p->inflow_error.empty()) return fail(GF_EINVAL, "%s", p->inflow_error.c_str());
     if (int r = sync_batch_args(p)) return r;
     const int nv = gf::num_variants(p->mx, p->my);
-    const gf::BatchArgs a = host_batch_args(p);
     // The active list of the current states (what the next attempt steps).
     // Each candidate steps it from the accepted buffer into the other one,
     // which the next attempt overwrites: no scenario advances.
@@ -1782,7 +1787,7 @@ try {
     } else {
         HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, 0, p->capacity, -1, 0.0), inflow_grid(), g.stream));
     }
-    HIPCHK(gf::launch_classify(a, g.stream));
+    HIPCHK(gf::launch_classify(p->d_args, p->capacity, g.stream));
     if (int r = ensure_pair_events(g.ev_tune, 1)) return r;
     const int saved = p->variant;
     double best_ms = 0.0;

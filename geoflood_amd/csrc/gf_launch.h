@@ -186,14 +186,18 @@ bool get_variant(int mx, int my, int k, StepVariant* out);
 int wet_occupancy(int mx, int my, int variant);
 
 bool supported_shape(int mx, int my);
-hipError_t launch_classify(const BatchArgs& a, hipStream_t st);
+// Work lists of an attempt over slots [0, da->n), read from the device
+// arguments (a graph replay sees the current count); grid for `slots` slots.
+hipError_t launch_classify(const BatchArgs* da, int slots, hipStream_t st);
 // da: the BatchArgs in device memory (read by the persistent kernel on every iteration)
 // def: the method is the default one (order 2, transverse 2, MC limiter);
 // its kernels have the method fixed at compile time.
 bool default_method(const Params& P);
 // Active tiles then dry patches from one persistent kernel (see k_all).
 hipError_t launch_all(int mx, int my, int variant, bool def, const BatchArgs* da, int grid, hipStream_t st);
-hipError_t launch_batch_control(const BatchArgs& a, hipStream_t st);
+// The dt controller of scenarios [0, da->nscen) (device arguments); grid for
+// `scenarios` scenario slots.
+hipError_t launch_batch_control(const BatchArgs* da, int scenarios, hipStream_t st);
 hipError_t launch_average2coarse(int mx, int my, const RegridArgs& a, hipStream_t st);
 hipError_t launch_interpolate2fine(int mx, int my, const RegridArgs& a, hipStream_t st);
 hipError_t launch_copy_slots(const double* src, double* dst, const int* slots, int n, int np2, hipStream_t st);

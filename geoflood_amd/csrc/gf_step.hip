@@ -1247,8 +1247,9 @@ __global__ void __launch_bounds__(256) k_control(ScenCtrl* ctrl, int nscen, cons
 // or a neighbour's maximum interior depth exceeds the tolerance, or a side's
 // ghost cells come from the host.  Active patches go to the wet list (full
 // step), the others to the dry list.  Appends are aggregated per wavefront.
-__global__ void __launch_bounds__(256) k_classify(BatchArgs a)
+__global__ void __launch_bounds__(256) k_classify(const BatchArgs* pa)
 {
+    const BatchArgs& a = *pa;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     int kind = -1;  // -1: not stepping, 0: dry, 1: active
     int cur = 0;
@@ -1297,8 +1298,9 @@ __global__ void __launch_bounds__(256) k_classify(BatchArgs a)
 // Controller of the batched path: the per-scenario CFL maximum was folded by
 // atomicMax in the step kernels; apply the dt rule (fc2d_geoclaw_run.c:
 // 529-580), clear the maxima and the list counters for the next attempt.
-__global__ void __launch_bounds__(64) k_batch_control(BatchArgs a)
+__global__ void __launch_bounds__(64) k_batch_control(const BatchArgs* pa)
 {
+    const BatchArgs& a = *pa;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s == 0) {
         a.stats[0] += a.counters[CNT_WET];
@@ -2008,10 +2010,10 @@ hipError_t launch_inflow(int mx, int my, const InflowArgs& a, int grid, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_classify(const BatchArgs& a, hipStream_t st)
+hipError_t launch_classify(const BatchArgs* da, int slots, hipStream_t st)
 {
-    if (a.n <= 0) return hipSuccess;
-    k_classify<<<(a.n + 255) / 256, 256, 0, st>>>(a);
+    if (slots <= 0) return hipSuccess;
+    k_classify<<<(slots + 255) / 256, 256, 0, st>>>(da);
     return hipGetLastError();
 }
 
@@ -2031,9 +2033,9 @@ hipError_t launch_all(int mx, int my, int variant, bool def, const BatchArgs* da
 }
 
 
-hipError_t launch_batch_control(const BatchArgs& a, hipStream_t st)
+hipError_t launch_batch_control(const BatchArgs* da, int scenarios, hipStream_t st)
 {
-    k_batch_control<<<(a.nscen + 63) / 64 > 0 ? (a.nscen + 63) / 64 : 1, 64, 0, st>>>(a);
+    k_batch_control<<<(scenarios + 63) / 64 > 0 ? (scenarios + 63) / 64 : 1, 64, 0, st>>>(da);
     return hipGetLastError();
 }
 
