@@ -142,7 +142,7 @@ def cpu_baseline(state, nsteps, nthreads, port_only=False, cr_pow=False):
     return dict(value=nsteps * N * N / el, seconds=el, kind=kind, rec=rec, state=br.to_global())
 
 
-def amr_overland(steps, maxlevel=4):
+def amr_overland(steps, maxlevel=4, tune=True):
     """The AMR overland flood: 2 km square, 16 x 16 patches at levels 1-4
     (level L: 4 * 2^L patches a side, 1.95 m cells at level 4), Gaussian-sum
     DEM (seed 0), west hydrograph Q = 200 sin(pi t / 3600) m^3/s through a
@@ -150,15 +150,20 @@ def amr_overland(steps, maxlevel=4):
     tagged on h > dry_tolerance or |grad eta| > 0.01, regrid every 4 steps,
     dry at t = 0 (dt bound 10 s).  Every attempt is one graph replay (ghost
     update across levels, one persistent step of all levels, dt control);
-    the whole loop, regrids included, is timed."""
+    the whole loop, regrids included, is timed.  With tune, the step variant
+    is selected by timing every variant on the population the warm-up
+    reaches (and again if the population moves to another bucket)."""
     import geoflood_amd
     from geoflood_amd.amr import AmrRun
+    from geoflood_amd.autotune import TuneCache
     from geoflood_amd.scenarios import gaussian_sum_dem, sine_hydrograph
     geoflood_amd.set_params(**dict(PARAMS, manning_n=0.035))
     t, hu = sine_hydrograph(200.0, 100.0)
     hydro = dict(time=t, hu=hu, b=100.0, x0=0.0, y0=1000.0)
+    cache = TuneCache() if tune else None
     run = AmrRun(16, 4, 4, 2000.0, 2000.0, 1, maxlevel, lambda nx, ny, L: gaussian_sum_dem(nx, 2000.0, seed=0),
-                 4 ** maxlevel * 16, mthbc=(0, 1, 3, 3), hydro=hydro, eta_tol=0.01, regrid_interval=4, dt_max=10.0)
+                 4 ** maxlevel * 16, mthbc=(0, 1, 3, 3), hydro=hydro, eta_tol=0.01, regrid_interval=4, dt_max=10.0,
+                 tune_cache=cache)
     try:
         t0 = time.perf_counter()
         run.start(0.0, 1.0)
@@ -188,6 +193,8 @@ def amr_overland(steps, maxlevel=4):
                    active_patches_per_attempt=round((st1["active_patches"] - st0["active_patches"]) /
                                                     max(st1["attempts"] - st0["attempts"], 1), 1),
                    wet_cells_final=wet,
+                   variant=list(run.pool.variants()[1][run.pool.variants()[0]]),
+                   tuning=[dict(e) for e in cache.log] if cache else None,
                    note="whole loop timed: graph replays (multi-level ghost update, one persistent step of "
                         "every level, device dt control) and every regrid (device tags, forest adapt on the host, "
                         "slot moves, interpolate2fine, average2coarse, scenario re-targeting)")
@@ -307,6 +314,8 @@ def main():
                          "the workload's full length is 3000)")
     ap.add_argument("--amr-steps", type=int, default=2000,
                     help="accepted steps of the AMR overland flood (levels 1-4, regrid every 4; 0: skip)")
+    ap.add_argument("--amr-fixed-variant", action="store_true",
+                    help="AMR overland flood with the default step variant (no autotune)")
     ap.add_argument("--variant", type=int, default=-1,
                     help="use this step variant instead of the autotuner's choice (counter runs, where "
                          "the profiler's serialisation makes the autotuner's timings unreliable)")
@@ -647,7 +656,7 @@ def main():
     # ---- AMR overland flood on resident patches ----
     amr = None
     if args.amr_steps > 0:
-        amr = amr_overland(args.amr_steps)
+        amr = amr_overland(args.amr_steps, tune=not args.amr_fixed_variant)
 
     # ---- autotuner sweep under dynamic AMR ----
     sweep = None

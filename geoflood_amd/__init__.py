@@ -91,6 +91,7 @@ PROTOTYPES = {
     "gf_pool_get_scenarios": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.POINTER(GfScenarioInfo)]),
     "gf_pool_run": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, _ip]),
     "gf_pool_run_plain": (ct.c_int, [ct.c_void_p, ct.c_int, _ip]),
+    "gf_pool_advance": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.POINTER(GfScenarioInfo)]),
     "gf_pool_tag_refine": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_double, ct.POINTER(GfRefine), ct.c_int,
                                       ct.c_int, ct.c_int, _ip, ct.c_int, _ip]),
     "gf_pool_tag_coarsen": (ct.c_int, [ct.c_void_p, ct.c_int, ct.c_double, ct.POINTER(GfRefine), ct.c_int,

@@ -78,6 +78,7 @@ class Forest:
         self.lev = np.asarray(lev, dtype=np.int64)
         assert self.lev.shape == (self.ny, self.nx)
         self._tables = {}
+        self._fam = None
         self._index()
 
     def _index(self):
@@ -156,26 +157,41 @@ class Forest:
         which, being subject to the same test, would have made this return
         True.  The requested levels are then the forest's own, a balanced
         tree, which the closure returns as it is."""
-        M = self.maxlevel
-        if (refine & (self.L < M)).any():
+        if (refine & (self.L < self.maxlevel)).any():
             return True
-        c = coarsen & (self.L > self.minlevel)
-        if not c.any():
+        fam, nf = self._mergeable()
+        if nf == 0:
             return False
-        key = (self.L[c] << 48) | ((self.j[c] >> 1) << 24) | (self.i[c] >> 1)
-        keys, counts = np.unique(key, return_counts=True)
-        fams = keys[counts == 4]
-        if len(fams) == 0:
-            return False
-        p = np.pad(self.lev, 1, constant_values=-1)
-        nb = np.max([p[1 + dy:1 + dy + self.ny, 1 + dx:1 + dx + self.nx] for dx, dy in DIRS], axis=0)
-        FL, fj, fi = fams >> 48, (fams >> 24) & 0xFFFFFF, fams & 0xFFFFFF
-        for L in np.unique(FL):
-            sel = FL == L
-            finest_near = _block(nb, 1 << (M - int(L) + 1)).max(axis=(1, 3))
-            if (finest_near[fj[sel], fi[sel]] <= L).any():
-                return True
-        return False
+        sel = coarsen & (fam >= 0)
+        return bool(sel.any() and (np.bincount(fam[sel], minlength=nf) == 4).any())
+
+    def _mergeable(self):
+        """(fam, nf): per leaf the index of its family among the nf families
+        that a regrid can merge -- four sibling leaves of one level L >
+        minlevel whose parent's block has no face or corner neighbour finer
+        than L -- or -1.  A property of the forest, computed once."""
+        if self._fam is None:
+            M = self.maxlevel
+            fam = np.full(len(self.L), -1, dtype=np.int64)
+            nf = 0
+            idx = np.nonzero(self.L > self.minlevel)[0]
+            if len(idx):
+                key = (self.L[idx] << 48) | ((self.j[idx] >> 1) << 24) | (self.i[idx] >> 1)
+                keys, inv, counts = np.unique(key, return_inverse=True, return_counts=True)
+                full = counts == 4
+                ok = np.zeros(len(keys), dtype=bool)
+                if full.any():
+                    p = np.pad(self.lev, 1, constant_values=-1)
+                    nb = np.max([p[1 + dy:1 + dy + self.ny, 1 + dx:1 + dx + self.nx] for dx, dy in DIRS], axis=0)
+                    FL, fj, fi = keys >> 48, (keys >> 24) & 0xFFFFFF, keys & 0xFFFFFF
+                    for L in np.unique(FL[full]):
+                        sel = full & (FL == L)
+                        finest_near = _block(nb, 1 << (M - int(L) + 1)).max(axis=(1, 3))
+                        ok[sel] = finest_near[fj[sel], fi[sel]] <= L
+                nf = int(ok.sum())
+                fam[idx] = np.where(ok[inv], (np.cumsum(ok) - 1)[inv], -1)
+            self._fam = (fam, nf)
+        return self._fam
 
     def _adapt(self, refine: np.ndarray, coarsen: np.ndarray) -> "Forest":
         M = self.maxlevel
@@ -318,9 +334,9 @@ class AmrRun:
         # flags_fn(forest, tags, regrid_index) -> (refine, coarsen) per leaf;
         # default: refine the tagged leaves, coarsen families of untagged ones
         self.flags_fn = flags_fn
-        # tune_cache (geoflood_amd.autotune.TuneCache): after a regrid, when the
-        # active population has moved to another bucket, re-select the step
-        # variant (cached choice, or an autotune)
+        # tune_cache (geoflood_amd.autotune.TuneCache): every fourth regrid,
+        # when the active population per attempt has moved to another bucket,
+        # re-select the step variant (cached choice, or an autotune)
         self.tune = tune_cache
         self._tune_bucket = None
         self._tune_stats = None
@@ -517,27 +533,28 @@ class AmrRun:
             nxt = min(target, (before // self.regrid_interval + 1) * self.regrid_interval)
             # the scenario stops on the device at the next regrid step: replay
             # that many attempts, then single ones for any retakes
-            self.pool.set_stop(self.sid, nxt)
-            k = nxt - before
-            while True:
-                if graph:
-                    self.pool.run(k, graph=True)
-                else:
-                    self.pool.run_plain(k)
+            if graph:
+                info = self.pool.advance(self.sid, nxt)
                 self.stats["device_ms"] += last_timing()[0]
-                info = self.pool.scenario(self.sid)
-                if info["status"] != 0:
-                    raise RuntimeError(f"scenario stopped with status {info['status']} at t = {info['t']}")
-                if info["nsteps"] >= nxt:
-                    break
-                k = 1
+            else:
+                self.pool.set_stop(self.sid, nxt)
+                k = nxt - before
+                while True:
+                    self.pool.run_plain(k)
+                    self.stats["device_ms"] += last_timing()[0]
+                    info = self.pool.scenario(self.sid)
+                    if info["status"] != 0 or info["nsteps"] >= nxt:
+                        break
+                    k = 1
+            if info["status"] != 0:
+                raise RuntimeError(f"scenario stopped with status {info['status']} at t = {info['t']}")
             self.stats["attempts"] = info["attempts"]
             cells = len(self.forest) * self.mx * self.mx
             self.stats["cell_updates"] += (info["nsteps"] - before) * cells
             if info["nsteps"] % self.regrid_interval == 0:
                 self.regrid(info)
                 self.stats["patches"].append(len(self.forest))
-                if self.tune is not None:
+                if self.tune is not None and self.stats["regrids"] % 4 == 1:
                     self._retune()
         self.stats["steps"] = info["nsteps"]
         self.stats["retakes"] = info["retakes"]

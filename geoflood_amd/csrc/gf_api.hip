@@ -247,12 +247,18 @@ struct gf_pool {
     // starts with the multi-level ghost update -- fine -> coarse averaging
     // over all links, then level by level the hydrograph sides and the
     // coarse -> fine interpolation -- whose link and inflow ranges live in
-    // device memory, so a regrid changes them without a new graph.
+    // device memory, so a regrid changes them without a new graph.  The
+    // graph holds only the passes whose ranges are not empty (a forest that
+    // occupies levels 3-4 launches no pass for levels 0-2); it is captured
+    // again when that set changes, which a regrid that keeps the occupied
+    // levels does not.
     bool amr = false;
     std::vector<int> h_level;              // [capacity] refinement level per slot (0 when not set)
     int nlev = 1;                          // levels the ghost update walks (max level + 1)
     int* d_links = nullptr;                // [4][link_cap]: coarse, fine, side, igrid by the coarse level
     int link_cap = 0, nlinks = 0;
+    unsigned inflow_levels = 0;            // bit l: level l has hydrograph patches (its k_inflow pass runs)
+    unsigned interp_levels = 0;            // bit l: links with a level-l coarse patch (its interpolation runs)
     int* d_ranges = nullptr;               // [kRanges]: see range_links / range_interp / range_inflow
 };
 
@@ -480,10 +486,16 @@ static int rebuild_inflow(gf_pool* p)
         // before interpolating from it to the next finer level
         std::stable_sort(list.begin(), list.end(), [&](int a, int b) { return p->h_level[a] < p->h_level[b]; });
         std::vector<int> rng(2 * kMaxLevels);
+        unsigned levels = 0;
         for (int l = 0, k = 0; l < kMaxLevels; l++) {
             rng[2 * l] = k;
             while (k < n && p->h_level[list[k]] == l) k++;
             rng[2 * l + 1] = k;
+            if (k > rng[2 * l]) levels |= 1u << l;
+        }
+        if (levels != p->inflow_levels) {
+            p->inflow_levels = levels;
+            p->graph_dirty = true;
         }
         HIPCHK(hipMemcpyAsync(p->d_ranges + range_inflow(0), rng.data(), sizeof(int) * rng.size(),
                               hipMemcpyHostToDevice, g.stream));
@@ -1275,17 +1287,17 @@ static int enqueue_ghost_update(gf_pool* p, int buf, double t)
     } else {
         a.q = p->q[buf];
     }
-    if (p->link_cap > 0) {
+    if (p->nlinks > 0) {
         a.range = p->d_ranges + range_links();
         HIPCHK(gf::launch_average_ghost(p->mx, p->my, a, g.stream));
     }
     for (int l = 0; l < p->nlev; l++) {
-        if (p->has_inflow) {
+        if (p->has_inflow && (p->inflow_levels >> l & 1u)) {
             gf::InflowArgs ia = inflow_args(p, 0, p->capacity, buf, t);
             ia.range = p->d_ranges + range_inflow(l);
             HIPCHK(gf::launch_inflow(p->mx, p->my, ia, inflow_grid(), g.stream));
         }
-        if (l + 1 < p->nlev && p->link_cap > 0) {
+        if (l + 1 < p->nlev && p->nlinks > 0 && (p->interp_levels >> l & 1u)) {
             a.range = p->d_ranges + range_interp(l);
             HIPCHK(gf::launch_interpolate_ghost(p->mx, p->my, a, g.stream));
         }
@@ -1325,12 +1337,11 @@ static int any_active_flag(gf_pool* p, int* any_active)
     return GF_OK;
 }
 
-int gf_pool_run(gf_pool* p, int substeps, int use_graph, int* any_active)
-try {
-    if (!p) return fail(GF_EINVAL, "pool is NULL");
-    if (substeps < 0) return fail(GF_EINVAL, "substeps must be >= 0");
-    if (p->nscen == 0 || p->n == 0) return fail(GF_ESTATE, "no scenario registered");
-    if (!p->inflow_error.empty()) return fail(GF_EINVAL, "%s", p->inflow_error.c_str());
+// `substeps` attempts of every active scenario (graph replays or stream
+// launches), then a stream synchronisation; the device time between the
+// first and the last attempt goes to g.last_total_ms.
+static int run_attempts(gf_pool* p, int substeps, int use_graph)
+{
     if (int r = sync_batch_args(p)) return r;
     if (use_graph) {
         if (p->graph_dirty) {
@@ -1374,7 +1385,66 @@ try {
         }
         g.last_launches = substeps;
     }
+    return GF_OK;
+}
+
+int gf_pool_run(gf_pool* p, int substeps, int use_graph, int* any_active)
+try {
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (substeps < 0) return fail(GF_EINVAL, "substeps must be >= 0");
+    if (p->nscen == 0 || p->n == 0) return fail(GF_ESTATE, "no scenario registered");
+    if (!p->inflow_error.empty()) return fail(GF_EINVAL, "%s", p->inflow_error.c_str());
+    if (int r = run_attempts(p, substeps, use_graph)) return r;
     return any_active_flag(p, any_active);
+} GF_CATCH_STATUS
+
+int gf_pool_advance(gf_pool* p, int s, int target_steps, int use_graph, int max_attempts, gf_scenario_info* info)
+try {
+    if (!p) return fail(GF_EINVAL, "pool is NULL");
+    if (s < 0 || s >= p->capacity || p->h_ctrl[s].npatch == 0) return fail(GF_EINVAL, "scenario %d unknown", s);
+    if (target_steps <= 0) return fail(GF_EINVAL, "target_steps must be > 0");
+    if (!p->inflow_error.empty()) return fail(GF_EINVAL, "%s", p->inflow_error.c_str());
+    // the stop condition, as gf_pool_set_scenario_stop sets it (t_end off)
+    ScenCtrl& c = p->h_ctrl[s];
+    HIPCHK(hipMemcpyAsync(&c, p->ctrl + s, sizeof(ScenCtrl), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (max_attempts > 0 && target_steps - c.nsteps > max_attempts)
+        return fail(GF_ESTATE, "scenario %d: %d steps short of %d exceed max_attempts %d", s, target_steps - c.nsteps,
+                    target_steps, max_attempts);
+    c.target_steps = target_steps;
+    c.t_end = 0.0;
+    c.active = (c.status == gf::SCEN_OK && c.nsteps < target_steps) ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(p->ctrl + s, &c, sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
+    // one attempt per missing step, then single attempts for the retakes
+    double total_ms = 0.0;
+    int attempts = 0;
+    ScenCtrl now = c;
+    while (now.active) {
+        const int k = attempts == 0 ? target_steps - now.nsteps : 1;
+        if (max_attempts > 0 && attempts + k > max_attempts)
+            return fail(GF_ESTATE, "scenario %d: %d steps short of %d after %d attempts", s, target_steps - now.nsteps,
+                        target_steps, attempts);
+        if (int r = run_attempts(p, k, use_graph)) return r;
+        total_ms += g.last_total_ms;
+        attempts += k;
+        HIPCHK(hipMemcpyAsync(&now, p->ctrl + s, sizeof(ScenCtrl), hipMemcpyDeviceToHost, g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+    }
+    g.last_total_ms = total_ms;
+    p->h_ctrl[s] = now;
+    if (info) {
+        info->t = now.t;
+        info->dt = now.dt;
+        info->maxcfl = now.maxcfl;
+        info->last_dt = now.last_dt;
+        info->nsteps = now.nsteps;
+        info->retakes = now.retakes;
+        info->attempts = now.attempts;
+        info->active = now.active;
+        info->cur = now.cur;
+        info->status = now.status;
+    }
+    return GF_OK;
 } GF_CATCH_STATUS
 
 int gf_pool_run_plain(gf_pool* p, int substeps, int* any_active)
@@ -2209,10 +2279,16 @@ try {
     std::vector<int> rng(2 * (1 + kMaxLevels), 0);
     rng[0] = 0;
     rng[1] = n;
+    unsigned levels = 0;
     for (int lv = 0, k = 0; lv < kMaxLevels; lv++) {
         rng[range_interp(lv)] = k;
         while (k < n && p->h_level[h[k]] == lv) k++;
         rng[range_interp(lv) + 1] = k;
+        if (k > rng[range_interp(lv)]) levels |= 1u << lv;
+    }
+    if (levels != p->interp_levels || (n > 0) != (p->nlinks > 0)) {
+        p->interp_levels = levels;
+        p->graph_dirty = true;
     }
     if (p->link_cap > 0)
         HIPCHK(hipMemcpyAsync(p->d_links, h.data(), sizeof(int) * 4 * (size_t)p->link_cap, hipMemcpyHostToDevice, g.stream));
@@ -2228,9 +2304,8 @@ try {
     if (buf < -1 || buf > 1) return fail(GF_EINVAL, "buffer must be 0, 1 or -1 (each scenario's current one)");
     if (!p->inflow_error.empty()) return fail(GF_EINVAL, "%s", p->inflow_error.c_str());
     if (int r = ensure_amr(p)) return r;
-    if (int r = enqueue_ghost_update(p, buf, t)) return r;
-    HIPCHK(hipStreamSynchronize(g.stream));
-    return GF_OK;
+    // stream-ordered: every call that reads the pool afterwards waits for it
+    return enqueue_ghost_update(p, buf, t);
 } GF_CATCH_STATUS
 
 int gf_pool_move(gf_pool* p, int n, const int* src, const int* dst, int buf)

@@ -290,6 +290,14 @@ class Pool:
                       align=True)
         return np.frombuffer(bytes(arr), dtype=dt, count=n).copy()
 
+    def advance(self, sid: int, target_steps: int, graph: bool = True, max_attempts: int = 0) -> dict:
+        """Run until scenario sid has target_steps accepted steps (or stops);
+        its control record afterwards (gf_pool_advance)."""
+        info = GfScenarioInfo()
+        check(self.lib.gf_pool_advance(self.h, sid, target_steps, int(graph), max_attempts, ct.byref(info)),
+              "gf_pool_advance")
+        return {k: getattr(info, k) for k, _ in GfScenarioInfo._fields_}
+
     def run(self, substeps: int, graph: bool = True) -> bool:
         act = ct.c_int(0)
         check(self.lib.gf_pool_run(self.h, substeps, int(graph), ct.byref(act)), "gf_pool_run")

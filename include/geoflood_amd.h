@@ -396,6 +396,19 @@ int gf_pool_get_scenarios(gf_pool *pool, int first, int n, gf_scenario_info *inf
  * finished; *any_active (optional) tells whether a scenario is still active. */
 int gf_pool_run(gf_pool *pool, int substeps, int use_graph, int *any_active);
 
+/* Advance scenario s to target_steps accepted steps: set its stop there (as
+ * gf_pool_set_scenario_stop with t_end off), run one attempt per missing
+ * step, then single attempts until it stops (each retake costs one), and
+ * copy its control record to *info (optional).  Every other active scenario
+ * of the pool advances with it.  max_attempts > 0 bounds the attempts
+ * (GF_ESTATE beyond it); the scenario stops early on a non-finite state
+ * (info->status).  One call per regrid interval of the outstyle_3 loop
+ * (fc2d_geoclaw_run.c:505-575: steps and retakes up to the next regrid),
+ * in place of gf_pool_set_scenario_stop, gf_pool_run and
+ * gf_pool_get_scenario. */
+int gf_pool_advance(gf_pool *pool, int scenario, int target_steps, int use_graph, int max_attempts,
+                    gf_scenario_info *info);
+
 /* The plain loop for the same scenarios: per attempt, a host-issued launch
  * per patch, a host read of the CFL numbers and the dt rule on the host.
  * Semantics equal gf_pool_run. */
@@ -572,7 +585,9 @@ int gf_pool_set_cf_links(gf_pool *pool, int n, const int *coarse, const int *fin
 
 /* The multi-level ghost update of the stored links on buffer buf at time t
  * (buf = -1: each scenario's current buffer and time), as each gf_pool_run
- * attempt begins with it; for a regrid or an output between steps.  Replaces
+ * attempt begins with it; for a regrid or an output between steps.  Queued
+ * on the library's stream: it returns before the device has finished, and
+ * every later call that reads or writes the pool runs after it.  Replaces
  * fclaw2d_ghost_update with time_interp off for resident patches. */
 int gf_pool_ghost_update(gf_pool *pool, int buf, double t);
 

@@ -147,3 +147,48 @@ def test_dry_start_zero_discharge_advances(gf, oracle_c, mode):
         gf.set_params()
         oracle_c.set_params(Params())
         pool.close()
+
+
+def test_advance_equals_stop_and_replay(gf):
+    """gf_pool_advance to a step count equals setting the stop there and
+    replaying attempts until the scenario stops (retakes included: the first
+    dt is five times too large), in every counter, the dt and the state; it
+    refuses a bound that the missing steps exceed and does nothing at a
+    target already reached."""
+    from geoflood_amd import GeoFloodError
+    from geoflood_amd.scenarios import uniform_dam_break
+    print("case advance vs stop-and-replay", flush=True)
+    n = 256
+    sc = uniform_dam_break(n, dx=1.0, h0=10.0, rms=3.0, seed=5)
+    out = []
+    for mode in ("loop", "advance"):
+        lay = BrickLayout(n // 32, n // 32, 32, 32, 1.0, 1.0)
+        pool = Pool(32, 32, lay.npatch)
+        try:
+            pool.set_layout([lay])
+            pool.upload_global(lay, *sc[:4])
+            sid = pool.add_scenario(lay, dt0=0.5, target_steps=0)
+            recs = []
+            for target in (3, 7, 12):
+                if mode == "loop":
+                    pool.set_stop(sid, target)
+                    k = target - pool.scenario(sid)["nsteps"]
+                    while pool.run(k):
+                        k = 1
+                    recs.append(pool.scenario(sid))
+                else:
+                    recs.append(pool.advance(sid, target))
+            if mode == "advance":
+                again = pool.advance(sid, 12)
+                assert again == recs[-1]                      # target reached: no attempt
+                with pytest.raises(GeoFloodError, match="max_attempts"):
+                    pool.advance(sid, 20, max_attempts=3)
+                assert pool.scenario(sid)["nsteps"] == 12     # refused before any attempt
+            out.append((recs, np.stack(pool.download_global(lay, buf=recs[-1]["cur"]))))
+        finally:
+            pool.close()
+    (ra, sa), (rb, sb) = out
+    assert ra[0]["retakes"] > 0
+    for a, b in zip(ra, rb):
+        assert a == b
+    assert_same(sb, sa)
