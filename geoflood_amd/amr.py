@@ -12,8 +12,10 @@ last:
 * ``Forest``: the leaves of a quadtree over a brick of level-0 patches,
   levels minlevel..maxlevel, 2:1 balanced across faces and corners (p4est's
   full balance).  It is stored as the level of the covering leaf of every
-  patch of the finest lattice, so refinement, coarsening of whole families
-  and the balance closure are a few array operations per regrid.
+  patch of the finest lattice; the leaf index, the neighbour table and
+  coarse/fine links, and refinement, coarsening of whole families and the
+  balance closure are loops over that lattice in a small host library
+  (gf_forest.cpp, libgfforest.so).
 * ``AmrRun``: one scenario whose leaves occupy a contiguous range of pool
   slots.  Every attempt is one replay of the pool's graph (multi-level ghost
   update, one persistent step of every leaf with one dt -- ForestClaw's
@@ -33,6 +35,9 @@ and in the library.
 """
 from __future__ import annotations
 
+import ctypes as ct
+import os
+
 import numpy as np
 
 from . import GF_NB_STORED, GF_NB_WALL, last_timing
@@ -41,23 +46,27 @@ from .pool import _KIND, MBC, Pool, pad_bathymetry
 DIRS = ((-1, 0), (1, 0), (0, -1), (0, 1), (-1, -1), (1, -1), (-1, 1), (1, 1))
 
 
-def _block(a: np.ndarray, s: int) -> np.ndarray:
-    """View of a 2-D array as (ny/s, s, nx/s, s) blocks."""
-    return a.reshape(a.shape[0] // s, s, a.shape[1] // s, s)
+_FOREST = None
 
 
-def _up(a: np.ndarray, s: int) -> np.ndarray:
-    return np.repeat(np.repeat(a, s, axis=0), s, axis=1)
-
-
-def _pyramid(a: np.ndarray, levels: int, op) -> list:
-    """[a, op over 2x2 blocks of a, op over 2x2 blocks of that, ...]:
-    levels + 1 arrays, each half the size of the last in both directions."""
-    out = [a]
-    for _ in range(levels):
-        b = out[-1]
-        out.append(op(op(b[0::2, 0::2], b[1::2, 0::2]), op(b[0::2, 1::2], b[1::2, 1::2])))
-    return out
+def _forest_lib():
+    """libgfforest.so (gf_forest.cpp): the forest's index, tables and adapt."""
+    global _FOREST
+    if _FOREST is None:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgfforest.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built (python -m geoflood_amd.build)")
+        lib = ct.CDLL(path)
+        ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        bp = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+        ci = ct.c_int
+        lib.gff_index.argtypes = [ci, ci, ci, ip, ip, ip, ip, ip, ip, ip]
+        lib.gff_tables.argtypes = [ci, ci, ci, ip, ip, ci, ip, ip, ip, ip, ci, ci, ip, ip, ip, ip, ip]
+        lib.gff_adapt.argtypes = [ci, ci, ci, ci, ip, ip, ci, ip, ip, ip, ip, ip, bp, bp, ip]
+        for f in (lib.gff_index, lib.gff_tables, lib.gff_adapt):
+            f.restype = ci
+        _FOREST = lib
+    return _FOREST
 
 
 class Forest:
@@ -66,36 +75,32 @@ class Forest:
     ``lev[J, I]`` is the level of the leaf covering patch (I, J) of the
     finest lattice (nx0 2^maxlevel by ny0 2^maxlevel patches).  A leaf
     (L, i, j) covers the finest patches [i s, (i+1) s) x [j s, (j+1) s),
-    s = 2^(maxlevel - L).  Leaves are indexed in the order (level, j, i)."""
+    s = 2^(maxlevel - L).  Leaves are indexed in the order (level, j, i).
+    The index, the tables and adapt are native (gf_forest.cpp)."""
 
     def __init__(self, nx0: int, ny0: int, minlevel: int, maxlevel: int, lev: np.ndarray | None = None):
-        assert 0 <= minlevel <= maxlevel
+        assert 0 <= minlevel <= maxlevel <= 15
         self.nx0, self.ny0 = nx0, ny0
         self.minlevel, self.maxlevel = minlevel, maxlevel
         self.nx, self.ny = nx0 << maxlevel, ny0 << maxlevel
         if lev is None:
-            lev = np.full((self.ny, self.nx), minlevel, dtype=np.int64)
-        self.lev = np.asarray(lev, dtype=np.int64)
+            lev = np.full((self.ny, self.nx), minlevel, dtype=np.int32)
+        self.lev = np.ascontiguousarray(lev, dtype=np.int32)
         assert self.lev.shape == (self.ny, self.nx)
         self._tables = {}
-        self._fam = None
         self._index()
 
     def _index(self):
-        M = self.maxlevel
-        lev = self.lev
-        m = np.left_shift(1, M - lev) - 1                  # block size - 1 (a power of two - 1)
-        I = np.arange(self.nx)[None, :]
-        J = np.arange(self.ny)[:, None]
-        jj, ii = np.nonzero(((I & m) == 0) & ((J & m) == 0))
-        L = lev[jj, ii]
-        i, j = ii >> (M - L), jj >> (M - L)
-        order = np.argsort((L << 42) | (j << 21) | i, kind="stable")   # by level, then j, then i
-        self.L, self.i, self.j = L[order], i[order], j[order]
-        self.oi, self.oj = ii[order], jj[order]          # origins on the finest lattice
-        ids = np.full((self.ny, self.nx), -1, dtype=np.int64)
-        ids[self.oj, self.oi] = np.arange(len(order))
-        self.leaf_at = ids[J & ~m, I & ~m]                # covering leaf of every finest patch
+        cells = self.nx * self.ny
+        L, i, j, oi, oj = (np.empty(cells, dtype=np.int32) for _ in range(5))
+        at = np.empty((self.ny, self.nx), dtype=np.int32)
+        n = _forest_lib().gff_index(self.nx, self.ny, self.maxlevel, self.lev, L, i, j, oi, oj, at)
+        if n < 0:
+            raise ValueError("the levels do not form a forest of aligned blocks")
+        self._c = (at, L[:n].copy(), oi[:n].copy(), oj[:n].copy(), i[:n].copy(), j[:n].copy())  # int32, native calls
+        self.L, self.i, self.j = L[:n].astype(np.int64), i[:n].astype(np.int64), j[:n].astype(np.int64)
+        self.oi, self.oj = oi[:n].astype(np.int64), oj[:n].astype(np.int64)   # origins on the finest lattice
+        self.leaf_at = at.astype(np.int64)                # covering leaf of every finest patch
 
     def __len__(self) -> int:
         return len(self.L)
@@ -107,114 +112,25 @@ class Forest:
         return np.array_equal(self.lev, other.lev)
 
     # -- adaptation -------------------------------------------------------------------
-    def _tree(self, req: np.ndarray) -> np.ndarray:
-        """The coarsest quadtree whose every finest patch is at least at
-        level req there: each finest patch takes the smallest level L whose
-        block around it has max(req) <= L (true at L = maxlevel, and true at
-        every finer level once true at a coarser one)."""
-        M = self.maxlevel
-        mx = _pyramid(req, M, np.maximum)      # mx[M - L]: block maxima at level L
-        lev = np.where(mx[M] <= 0, 0, -1)
-        for L in range(1, M + 1):
-            lev = np.repeat(np.repeat(lev, 2, axis=0), 2, axis=1)
-            lev = np.where((lev < 0) & (mx[M - L] <= L), L, lev)
-        return lev
-
-    def _balance(self, req: np.ndarray) -> np.ndarray:
-        """The coarsest 2:1-balanced tree (faces and corners) at least as
-        fine as req: adjacent finest patches differ by at most one level."""
-        req = np.maximum(req, self.minlevel)
-        while True:
-            lev = self._tree(req)
-            p = np.pad(lev, 1, constant_values=-1)
-            nb = p[1:1 + self.ny, 0:self.nx]
-            for dx, dy in DIRS[1:]:
-                nb = np.maximum(nb, p[1 + dy:1 + dy + self.ny, 1 + dx:1 + dx + self.nx])
-            need = nb - 1
-            if (lev >= need).all():
-                return lev
-            req = np.maximum(req, need)
-
     def adapt(self, refine: np.ndarray, coarsen: np.ndarray) -> "Forest":
         """The forest after one regrid: leaves with refine[k] split once
         (below maxlevel); families of four sibling leaves that all have
         coarsen[k] (and no refine) merge into their parent (above minlevel);
-        then the balance closure, which may refine further leaves or keep a
-        family from merging."""
-        refine = np.asarray(refine, dtype=bool)
-        coarsen = np.asarray(coarsen, dtype=bool) & ~refine
-        if not self._changes(refine, coarsen):
+        then the coarsest 2:1-balanced closure (faces and corners), which may
+        refine further leaves or keep a family from merging.  Returns this
+        forest when nothing changes."""
+        n = len(self.L)
+        r = np.asarray(refine, dtype=bool)
+        c = np.asarray(coarsen, dtype=bool) & ~r
+        if r.shape != (n,) or c.shape != (n,):
+            raise ValueError(f"refine and coarsen need one flag per leaf ({n})")
+        out = np.empty_like(self.lev)
+        at, L, oi, oj, i, j = self._c
+        if not _forest_lib().gff_adapt(self.nx, self.ny, self.minlevel, self.maxlevel, self.lev, at, n, L, i, j, oi, oj,
+                                       np.ascontiguousarray(r, dtype=np.uint8), np.ascontiguousarray(c, dtype=np.uint8),
+                                       out):
             return self
-        return self._adapt(refine, coarsen)
-
-    def _changes(self, refine: np.ndarray, coarsen: np.ndarray) -> bool:
-        """Whether adapt can change the forest.  It cannot when no leaf below
-        maxlevel is to refine and every family of four sibling leaves (level
-        L > minlevel) that is to coarsen has a face or corner neighbour of
-        its parent's block finer than L: merging it would put level L-1 next
-        to level L+1, which the balance closure undoes, and such a neighbour
-        cannot get coarser in this regrid unless its own family merges --
-        which, being subject to the same test, would have made this return
-        True.  The requested levels are then the forest's own, a balanced
-        tree, which the closure returns as it is."""
-        if (refine & (self.L < self.maxlevel)).any():
-            return True
-        fam, nf = self._mergeable()
-        if nf == 0:
-            return False
-        sel = coarsen & (fam >= 0)
-        return bool(sel.any() and (np.bincount(fam[sel], minlength=nf) == 4).any())
-
-    def _mergeable(self):
-        """(fam, nf): per leaf the index of its family among the nf families
-        that a regrid can merge -- four sibling leaves of one level L >
-        minlevel whose parent's block has no face or corner neighbour finer
-        than L -- or -1.  A property of the forest, computed once."""
-        if self._fam is None:
-            M = self.maxlevel
-            fam = np.full(len(self.L), -1, dtype=np.int64)
-            nf = 0
-            idx = np.nonzero(self.L > self.minlevel)[0]
-            if len(idx):
-                key = (self.L[idx] << 48) | ((self.j[idx] >> 1) << 24) | (self.i[idx] >> 1)
-                keys, inv, counts = np.unique(key, return_inverse=True, return_counts=True)
-                full = counts == 4
-                ok = np.zeros(len(keys), dtype=bool)
-                if full.any():
-                    p = np.pad(self.lev, 1, constant_values=-1)
-                    nb = np.max([p[1 + dy:1 + dy + self.ny, 1 + dx:1 + dx + self.nx] for dx, dy in DIRS], axis=0)
-                    FL, fj, fi = keys >> 48, (keys >> 24) & 0xFFFFFF, keys & 0xFFFFFF
-                    for L in np.unique(FL[full]):
-                        sel = full & (FL == L)
-                        finest_near = _block(nb, 1 << (M - int(L) + 1)).max(axis=(1, 3))
-                        ok[sel] = finest_near[fj[sel], fi[sel]] <= L
-                nf = int(ok.sum())
-                fam[idx] = np.where(ok[inv], (np.cumsum(ok) - 1)[inv], -1)
-            self._fam = (fam, nf)
-        return self._fam
-
-    def _adapt(self, refine: np.ndarray, coarsen: np.ndarray) -> "Forest":
-        M = self.maxlevel
-        lev = self.lev
-        req = lev + (refine[self.leaf_at] & (lev < M))
-        cc = coarsen[self.leaf_at]
-        # a family of level L merges where its parent's block (level L-1) is
-        # all level L and all to coarsen; such blocks of different L are
-        # disjoint, so one map of target levels, filled from coarse to fine
-        mn = _pyramid(lev, M, np.minimum)
-        mxp = _pyramid(lev, M, np.maximum)
-        al = _pyramid(cc, M, np.logical_and)
-        target = np.full(mn[M].shape, -1, dtype=np.int64)   # at level 0 resolution
-        for P in range(0, M):                                # parent level P = L - 1
-            if P > 0:
-                target = np.repeat(np.repeat(target, 2, axis=0), 2, axis=1)
-            L = P + 1
-            if L > self.minlevel:
-                k = M - P
-                target = np.where((mn[k] == L) & (mxp[k] == L) & al[k], P, target)
-        target = np.repeat(np.repeat(target, 2, axis=0), 2, axis=1)
-        req = np.where(target >= 0, target, req)
-        return Forest(self.nx0, self.ny0, self.minlevel, self.maxlevel, self._balance(req))
+        return Forest(self.nx0, self.ny0, self.minlevel, self.maxlevel, out)
 
     # -- tables -------------------------------------------------------------------------
     def tables(self, mthbc=(3, 3, 3, 3)):
@@ -233,39 +149,16 @@ class Forest:
         return self._tables[key]
 
     def _build_tables(self, mthbc):
-        M = self.maxlevel
-        L, oi, oj = self.L, self.oi, self.oj
+        at, L, oi, oj = self._c[:4]
         n = len(L)
-        s = np.left_shift(1, M - L)
-        ids = np.arange(n)
         nbr = np.empty((n, 8), dtype=np.int32)
-        kinds = [_KIND[mthbc[k]] for k in range(4)]
-        lc, lf, ls, lg = [], [], [], []
-        for d, (dx, dy) in enumerate(DIRS):
-            ni, nj = oi + dx * s, oj + dy * s
-            inside = (ni >= 0) & (ni < self.nx) & (nj >= 0) & (nj < self.ny)
-            nic, njc = np.clip(ni, 0, self.nx - 1), np.clip(nj, 0, self.ny - 1)
-            ln = self.lev[njc, nic]
-            same = inside & (ln == L)
-            out = kinds[d] if d < 4 else GF_NB_WALL
-            nbr[:, d] = np.where(same, self.leaf_at[njc, nic], np.where(inside, GF_NB_STORED, out))
-            finer = inside & (ln > L)
-            if not finer.any():
-                continue
-            cis = [0] if dx > 0 else [1] if dx < 0 else [0, 1]
-            cjs = [0] if dy > 0 else [1] if dy < 0 else [0, 1]
-            h = s // 2
-            for cj in cjs:
-                for ci in cis:
-                    ki = np.where(finer, nic + ci * h, 0)
-                    kj = np.where(finer, njc + cj * h, 0)
-                    assert (self.lev[kj[finer], ki[finer]] == L[finer] + 1).all(), "forest is not 2:1 balanced"
-                    lc.append(ids[finer])
-                    lf.append(self.leaf_at[kj[finer], ki[finer]])
-                    ls.append(np.full(finer.sum(), d))
-                    lg.append(np.full(finer.sum(), (cj if d < 2 else ci) if d < 4 else 0))
-        cat = (lambda v: np.concatenate(v).astype(np.int32)) if lc else (lambda v: np.zeros(0, np.int32))
-        return nbr, (cat(lc), cat(lf), cat(ls), cat(lg))
+        lc, lf, ls, lg = (np.empty(12 * max(n, 1), dtype=np.int32) for _ in range(4))
+        kinds = np.array([_KIND[mthbc[k]] for k in range(4)], dtype=np.int32)
+        nl = _forest_lib().gff_tables(self.nx, self.ny, self.maxlevel, self.lev, at, n, L, oi, oj, kinds,
+                                      GF_NB_STORED, GF_NB_WALL, nbr, lc, lf, ls, lg)
+        if nl < 0:
+            raise ValueError("forest is not 2:1 balanced")
+        return nbr, (lc[:nl].copy(), lf[:nl].copy(), ls[:nl].copy(), lg[:nl].copy())
 
 
 def regrid_plan(old: Forest, new: Forest):

@@ -1,4 +1,5 @@
-"""Build libgeoflood_amd.so (gfx950) in place with hipcc.
+"""Build libgeoflood_amd.so (gfx950) in place with hipcc, and the host
+library of the resident-AMR driver's quadtree (libgfforest.so) with g++.
 
     python -m geoflood_amd.build
 """
@@ -37,7 +38,25 @@ def hipcc() -> str:
     return os.path.join(rocm, "bin", "hipcc")
 
 
+FOREST_LIB = os.path.join(HERE, "libgfforest.so")
+FOREST_SOURCE = "gf_forest.cpp"   # host only: the resident-AMR driver's quadtree
+
+
+def build_forest(verbose: bool = False) -> str:
+    tmp = FOREST_LIB + ".tmp"
+    cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", tmp, os.path.join(CSRC, FOREST_SOURCE)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building libgfforest.so")
+    os.replace(tmp, FOREST_LIB)
+    return FOREST_LIB
+
+
 def build(verbose: bool = False) -> str:
+    build_forest(verbose)
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
     cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + srcs

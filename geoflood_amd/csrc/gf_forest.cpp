@@ -1,0 +1,242 @@
+// gf_forest.cpp -- the quadtree of the resident-AMR driver, on the host.
+//
+// In GeoFlood, ForestClaw's p4est forest owns the leaves: fclaw2d_regrid
+// (fc2d_geoclaw_run.c:569-575) refines tagged leaves, merges families of
+// four siblings that all ask to coarsen, and closes the result under 2:1
+// balance across faces and corners; the solver plugin then copies,
+// interpolates and averages patches (fc2d_cpucuda.cpp:513-694).  Runs
+// without ForestClaw (geoflood_amd.amr) keep this stand-in instead.
+//
+// A forest over an nx0 x ny0 brick of level-0 patches with levels
+// 0..M is stored as the level of the covering leaf of every patch of the
+// finest lattice: lev[J * nx + I], nx = nx0 2^M, ny = ny0 2^M.  A leaf
+// (L, i, j) covers the finest patches [i s, (i+1) s) x [j s, (j+1) s),
+// s = 2^(M - L).  Leaves are numbered in the order (L, j, i).
+//
+// Every function is a plain loop over the lattice (a few passes per
+// regrid); arrays are int32, flags one byte per leaf.
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+// left, right, bottom, top, left-bottom, right-bottom, left-top, right-top
+constexpr int kDx[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+constexpr int kDy[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+
+// The coarsest tree whose every finest patch has a level >= req there: each
+// finest patch takes the smallest L whose level-L block around it has
+// max(req) <= L (true at L = M when req <= M).  mx[k] holds the block maxima
+// at the resolution of level M - k (mx[0] = req); out is the padded lattice
+// (a border of -1 around nx x ny).
+struct Tree {
+    int nx, ny, M;
+    std::vector<std::vector<int>> mx;
+    std::vector<int> out;   // (ny + 2) x (nx + 2)
+
+    Tree(int nx_, int ny_, int M_) : nx(nx_), ny(ny_), M(M_), mx(M_ + 1), out((size_t)(nx_ + 2) * (ny_ + 2), -1)
+    {
+        for (int k = 0; k <= M; k++) mx[k].resize((size_t)(nx >> k) * (ny >> k));
+    }
+
+    void fill(int l, int by, int bx)
+    {
+        const int k = M - l;
+        if (l == M || mx[k][(size_t)by * (nx >> k) + bx] <= l) {
+            const int s = 1 << k, W = nx + 2;
+            for (int y = by * s; y < (by + 1) * s; y++) {
+                int* row = &out[(size_t)(y + 1) * W + 1];
+                std::fill(row + bx * s, row + (bx + 1) * s, l);
+            }
+            return;
+        }
+        for (int cy = 0; cy < 2; cy++)
+            for (int cx = 0; cx < 2; cx++) fill(l + 1, 2 * by + cy, 2 * bx + cx);
+    }
+
+    void build(const std::vector<int>& req)
+    {
+        std::copy(req.begin(), req.end(), mx[0].begin());
+        for (int k = 1; k <= M; k++) {
+            const int w = nx >> k, h = ny >> k, wp = nx >> (k - 1);
+            const int* a = mx[k - 1].data();
+            int* b = mx[k].data();
+            for (int y = 0; y < h; y++)
+                for (int x = 0; x < w; x++) {
+                    const int* r0 = a + (size_t)(2 * y) * wp + 2 * x;
+                    const int* r1 = r0 + wp;
+                    b[(size_t)y * w + x] = std::max(std::max(r0[0], r0[1]), std::max(r1[0], r1[1]));
+                }
+        }
+        for (int by = 0; by < (ny >> M); by++)
+            for (int bx = 0; bx < (nx >> M); bx++) fill(0, by, bx);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+// The leaves of lev in the order (L, j, i): their level, block indices and
+// origins on the finest lattice, and leaf_at[J * nx + I], the leaf covering
+// each finest patch.  Returns the number of leaves, or -1 when lev is not a
+// forest (a level outside [0, M], or a block not of one level).
+int gff_index(int nx, int ny, int M, const int* lev, int* L, int* I, int* J, int* OI, int* OJ, int* leaf_at)
+{
+    const size_t cells = (size_t)nx * ny;
+    for (size_t c = 0; c < cells; c++) {
+        if (lev[c] < 0 || lev[c] > M) return -1;
+        leaf_at[c] = -1;
+    }
+    int n = 0;
+    for (int l = 0; l <= M; l++) {
+        const int s = 1 << (M - l);
+        for (int bj = 0; bj < ny / s; bj++)
+            for (int bi = 0; bi < nx / s; bi++) {
+                const int oi = bi * s, oj = bj * s;
+                if (lev[(size_t)oj * nx + oi] != l) continue;
+                for (int y = oj; y < oj + s; y++)
+                    for (int x = oi; x < oi + s; x++) {
+                        const size_t c = (size_t)y * nx + x;
+                        if (lev[c] != l || leaf_at[c] != -1) return -1;
+                        leaf_at[c] = n;
+                    }
+                L[n] = l;
+                I[n] = bi;
+                J[n] = bj;
+                OI[n] = oi;
+                OJ[n] = oj;
+                n++;
+            }
+    }
+    for (size_t c = 0; c < cells; c++)
+        if (leaf_at[c] < 0) return -1;
+    return n;
+}
+
+// Neighbour table and coarse/fine links of the n leaves of a forest (as
+// gff_index returns them).  nbr[8 k + d] for the directions left, right,
+// bottom, top, left-bottom, right-bottom, left-top, right-top: the
+// same-level leaf there; else `stored` when the region lies inside the
+// domain (covered by a coarser leaf or refined further); else the side's
+// kind kinds[d] for a face and `wall` for a corner.  Links (coarse leaf,
+// fine leaf, direction from the coarse one, igrid): every finer leaf across
+// a side or corner of a coarser one, by direction, then by the fine leaf's
+// half along the side (igrid, 0 the lower coordinate; 0 at corners), then
+// by coarse leaf.  The link arrays hold 12 n entries.  Returns the number of
+// links, or -1 when a finer neighbour is not one level finer (not 2:1
+// balanced).
+int gff_tables(int nx, int ny, int M, const int* lev, const int* leaf_at, int n, const int* L, const int* OI,
+               const int* OJ, const int* kinds, int stored, int wall, int* nbr, int* lc, int* lf, int* ls, int* lg)
+{
+    int nl = 0;
+    for (int d = 0; d < 8; d++) {
+        const int dx = kDx[d], dy = kDy[d];
+        bool any_finer = false;
+        for (int k = 0; k < n; k++) {
+            const int s = 1 << (M - L[k]);
+            const int ni = OI[k] + dx * s, nj = OJ[k] + dy * s;
+            const bool inside = ni >= 0 && ni < nx && nj >= 0 && nj < ny;
+            int v;
+            if (!inside) {
+                v = d < 4 ? kinds[d] : wall;
+            } else {
+                const int ln = lev[(size_t)nj * nx + ni];
+                v = ln == L[k] ? leaf_at[(size_t)nj * nx + ni] : stored;
+                any_finer |= ln > L[k];
+            }
+            nbr[8 * k + d] = v;
+        }
+        if (!any_finer) continue;
+        const int ci_lo = dx > 0 ? 0 : 1, ci_hi = dx < 0 ? 1 : (dx > 0 ? 0 : 1);
+        const int cj_lo = dy > 0 ? 0 : 1, cj_hi = dy < 0 ? 1 : (dy > 0 ? 0 : 1);
+        for (int cj = (dy == 0 ? 0 : cj_lo); cj <= cj_hi; cj++)
+            for (int ci = (dx == 0 ? 0 : ci_lo); ci <= ci_hi; ci++)
+                for (int k = 0; k < n; k++) {
+                    const int s = 1 << (M - L[k]);
+                    const int ni = OI[k] + dx * s, nj = OJ[k] + dy * s;
+                    if (!(ni >= 0 && ni < nx && nj >= 0 && nj < ny)) continue;
+                    if (lev[(size_t)nj * nx + ni] <= L[k]) continue;
+                    const int h = s / 2;
+                    const int ki = ni + ci * h, kj = nj + cj * h;
+                    const size_t c = (size_t)kj * nx + ki;
+                    if (lev[c] != L[k] + 1) return -1;
+                    lc[nl] = k;
+                    lf[nl] = leaf_at[c];
+                    ls[nl] = d;
+                    lg[nl] = d < 4 ? (d < 2 ? cj : ci) : 0;
+                    nl++;
+                }
+    }
+    return nl;
+}
+
+// The levels after one regrid of the forest lev (its n leaves L, I, J, OI,
+// OJ and leaf_at as gff_index returns them): leaves with refine[k] split
+// once (below M); a family of four sibling leaves of level L > minlevel that
+// all have coarsen[k] and none refine[k] merges into its parent; then the
+// coarsest 2:1-balanced tree (faces and corners) at least as fine as that
+// request, which may refine further leaves or keep a family from merging.
+// Writes out[ny * nx]; returns 1 when it differs from lev, else 0.
+int gff_adapt(int nx, int ny, int minlevel, int M, const int* lev, const int* leaf_at, int n, const int* L,
+              const int* I, const int* J, const int* OI, const int* OJ, const unsigned char* refine,
+              const unsigned char* coarsen, int* out)
+{
+    const size_t cells = (size_t)nx * ny;
+    std::vector<int> req(lev, lev + cells);
+    bool request = false;
+    for (int k = 0; k < n; k++) {
+        if (!refine[k] || L[k] >= M) continue;
+        const int s = 1 << (M - L[k]);
+        for (int y = OJ[k]; y < OJ[k] + s; y++) std::fill(&req[(size_t)y * nx + OI[k]], &req[(size_t)y * nx + OI[k] + s], L[k] + 1);
+        request = true;
+    }
+    // families: the leaf at the lower-left of its parent, and its three
+    // siblings (leaves of the same level at the parent's other quarters)
+    for (int k = 0; k < n; k++) {
+        if (L[k] <= minlevel || (I[k] & 1) || (J[k] & 1)) continue;
+        const int s = 1 << (M - L[k]);
+        bool all = true;
+        for (int q = 0; q < 4 && all; q++) {
+            const size_t c = (size_t)(OJ[k] + (q >> 1) * s) * nx + OI[k] + (q & 1) * s;
+            const int m = leaf_at[c];
+            all = lev[c] == L[k] && coarsen[m] && !refine[m];
+        }
+        if (!all) continue;
+        for (int y = OJ[k]; y < OJ[k] + 2 * s; y++)
+            std::fill(&req[(size_t)y * nx + OI[k]], &req[(size_t)y * nx + OI[k] + 2 * s], L[k] - 1);
+        request = true;
+    }
+    if (!request) {
+        // the request is the forest itself, a balanced tree: the closure keeps it
+        std::memcpy(out, lev, sizeof(int) * cells);
+        return 0;
+    }
+    // balance closure
+    for (size_t c = 0; c < cells; c++) req[c] = std::max(req[c], minlevel);
+    Tree t(nx, ny, M);
+    const int W = nx + 2;
+    for (;;) {
+        t.build(req);
+        bool ok = true;
+        for (int y = 0; y < ny; y++) {
+            const int* up = &t.out[(size_t)y * W + 1];
+            const int* mid = up + W;
+            const int* dn = mid + W;
+            int* rq = &req[(size_t)y * nx];
+            for (int x = 0; x < nx; x++) {
+                const int nb = std::max(std::max(std::max(up[x - 1], up[x]), std::max(up[x + 1], mid[x - 1])),
+                                        std::max(std::max(mid[x + 1], dn[x - 1]), std::max(dn[x], dn[x + 1])));
+                ok &= mid[x] >= nb - 1;
+                rq[x] = std::max(rq[x], nb - 1);
+            }
+        }
+        if (ok) break;
+    }
+    for (int y = 0; y < ny; y++) std::memcpy(out + (size_t)y * nx, &t.out[(size_t)(y + 1) * W + 1], sizeof(int) * nx);
+    return std::memcmp(out, lev, sizeof(int) * cells) != 0 ? 1 : 0;
+}
+
+}  // extern "C"

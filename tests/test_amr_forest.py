@@ -12,6 +12,7 @@ update (a ghost left NaN fails), across refinement and coarsening."""
 import numpy as np
 import pytest
 
+import _forest_ref as R
 from geoflood_amd import GF_NB_STORED
 from geoflood_amd.amr import DIRS, Forest, regrid_plan
 
@@ -171,41 +172,57 @@ def test_eta_gradient_criterion_closed_forms():
     assert not eta_gradient_tags(g, flat, mx, dx, dx, 1e-3, 1.01)
 
 
-def test_adapt_shortcut_equals_full_adapt():
-    """Forest.adapt returns the forest unchanged, without the balance
-    closure, when no leaf below maxlevel is to refine and every family of
-    four siblings to coarsen is kept by a finer neighbour of its parent's
-    block.  On random forests and flags the result always equals the full
-    computation; the shortcut is taken with and without such blocked
-    families, and whenever it is not taken the forest does change."""
-    rng = np.random.default_rng(11)
-    short = blocked = full_changed = 0
-    for trial in range(300):
-        f = Forest(int(rng.integers(1, 4)), int(rng.integers(1, 4)), int(rng.integers(0, 2)), int(rng.integers(2, 5)))
-        for _ in range(int(rng.integers(1, 5))):
+def _random_forests(rng, trials):
+    for trial in range(trials):
+        f = Forest(int(rng.integers(1, 4)), int(rng.integers(1, 4)), int(rng.integers(0, 2)), int(rng.integers(1, 5)))
+        for _ in range(int(rng.integers(0, 5))):
             t = rng.random(len(f)) < rng.uniform(0.05, 0.6)
             f = f.adapt(t, ~t & (rng.random(len(f)) < 0.5))
+        yield trial, f
+
+
+def test_native_forest_equals_numpy_reference():
+    """The native forest (gf_forest.cpp) against the numpy reference
+    (tests/_forest_ref.py) on random forests and flags: the same leaves in
+    the same order, the same neighbour table and links in the same order
+    (two sets of domain sides), and the same levels after adapt -- through
+    the merge map and the closure, and the reference's shortcut where no
+    leaf can change -- with the forest itself returned exactly when its
+    levels stay.  Flags that refine only leaves at maxlevel and coarsen
+    families kept by a finer neighbour exercise the unchanged case."""
+    rng = np.random.default_rng(11)
+    same = changed = 0
+    for trial, f in _random_forests(rng, 300):
+        ix = R.ref_index(f.lev, f.maxlevel)
+        for k in ("L", "i", "j", "oi", "oj", "leaf_at"):
+            assert np.array_equal(getattr(ix, k), getattr(f, k)), (trial, k)
+        for mthbc in ((3, 3, 3, 3), (0, 1, 3, 1)):
+            nb, links = f.tables(mthbc)
+            rnb, rlinks = R.ref_tables(f, mthbc)
+            assert np.array_equal(nb, rnb), trial
+            for a, b in zip(links, rlinks):
+                assert np.array_equal(a, b), trial
         for _ in range(5):
             refine = rng.random(len(f)) < rng.choice([0.0, 0.01, 0.3])
             if rng.random() < 0.7:
                 refine &= f.L == f.maxlevel          # only leaves that cannot refine
             coarsen = ~refine & (rng.random(len(f)) < rng.choice([0.0, 0.5, 0.9, 1.0]))
-            full = f._adapt(refine, coarsen)
             got = f.adapt(refine, coarsen)
-            assert got.same(full), trial
-            if got is f:
-                short += 1
-                c = coarsen & (f.L > f.minlevel)
-                key = (f.L[c] << 48) | ((f.j[c] >> 1) << 24) | (f.i[c] >> 1)
-                blocked += bool(len(key)) and bool((np.unique(key, return_counts=True)[1] == 4).any())
+            ref = R.ref_adapt(f, refine, coarsen)
+            assert np.array_equal(got.lev, ref), trial
+            assert np.array_equal(ref, R.ref_adapt_full(f, refine, ~refine & coarsen)), trial
+            if np.array_equal(ref, f.lev):
+                assert got is f
+                same += 1
             else:
-                full_changed += not full.same(f)
-    assert short > 0 and blocked > 0 and full_changed > 0
+                assert got is not f
+                changed += 1
+    assert same > 0 and changed > 0
 
 
 def _ref_tree(f, req):
-    """The block-by-block form of Forest._tree (each level's block maxima
-    taken over the finest lattice): the reference for the pyramid form."""
+    """The block-by-block form of the tree (each level's block maxima taken
+    over the finest lattice): the reference for the pyramid form."""
     M = f.maxlevel
     lev = np.full_like(req, M)
     done = np.zeros(req.shape, dtype=bool)
@@ -219,10 +236,10 @@ def _ref_tree(f, req):
 
 
 def _ref_adapt_levels(f, refine, coarsen):
-    """Forest._adapt's levels written block by block, with a balance closure
+    """One regrid's levels written block by block, with a balance closure
     over _ref_tree."""
     M = f.maxlevel
-    lev = f.lev
+    lev = f.lev.astype(np.int64)
     req = lev + (refine[f.leaf_at] & (lev < M))
     cc = coarsen[f.leaf_at]
     for L in range(f.minlevel + 1, M + 1):
@@ -243,16 +260,32 @@ def _ref_adapt_levels(f, refine, coarsen):
 
 def test_pyramid_forest_equals_blockwise_reference():
     """The pyramid forms of the tree, the family merge and the balance
-    closure give the same levels as the block-by-block reference on random
-    forests, requests and flags."""
+    closure (the numpy reference) give the same levels as the block-by-block
+    forms on random forests, requests and flags; the native forest equals
+    both."""
     rng = np.random.default_rng(21)
-    for trial in range(120):
-        f = Forest(int(rng.integers(1, 4)), int(rng.integers(1, 4)), int(rng.integers(0, 2)), int(rng.integers(1, 5)))
-        for _ in range(int(rng.integers(0, 4))):
-            t = rng.random(len(f)) < rng.uniform(0.05, 0.6)
-            f = f.adapt(t, ~t & (rng.random(len(f)) < 0.5))
+    for trial, f in _random_forests(rng, 120):
         req = rng.integers(0, f.maxlevel + 1, size=f.lev.shape)
-        assert np.array_equal(f._tree(req), _ref_tree(f, req)), trial
+        assert np.array_equal(R.ref_tree(f, req), _ref_tree(f, req)), trial
         refine = rng.random(len(f)) < rng.choice([0.0, 0.1, 0.4])
         coarsen = ~refine & (rng.random(len(f)) < rng.choice([0.3, 0.8, 1.0]))
-        assert np.array_equal(f._adapt(refine, coarsen).lev, _ref_adapt_levels(f, refine, coarsen)), trial
+        blockwise = _ref_adapt_levels(f, refine, coarsen)
+        assert np.array_equal(R.ref_adapt_full(f, refine, coarsen), blockwise), trial
+        assert np.array_equal(f.adapt(refine, coarsen).lev, blockwise), trial
+
+
+def test_native_forest_refuses_non_forests():
+    """Levels that are not aligned blocks, flags of the wrong length, and a
+    2:1 violation in the neighbour tables are errors, not results."""
+    lev = np.zeros((4, 4), dtype=np.int32)
+    lev[0, 1] = 1                                   # a level-1 value inside a level-0 block
+    with pytest.raises(ValueError):
+        Forest(1, 1, 0, 2, lev)
+    f = Forest(2, 2, 0, 2)
+    with pytest.raises(ValueError):
+        f.adapt(np.zeros(3, bool), np.zeros(4, bool))
+    lev = np.zeros((8, 8), dtype=np.int32)
+    lev[:4, :4] = 2                                 # level 2 next to level 0: not balanced
+    g = Forest(2, 2, 0, 2, lev)
+    with pytest.raises(ValueError, match="balanced"):
+        g.tables()
