@@ -458,7 +458,7 @@ static int rebuild_inflow(gf_pool* p)
 {
     std::vector<int> list;
     p->inflow_error.clear();
-    for (int s = 0; s < p->capacity; s++) {
+    for (int s = 0; s < p->nscen; s++) {   // scenario ids in use: [0, nscen)
         const ScenCtrl& c = p->h_ctrl[s];
         if (c.npatch == 0) continue;
         int sides = 0;
