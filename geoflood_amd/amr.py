@@ -126,9 +126,11 @@ class Forest:
             raise ValueError(f"refine and coarsen need one flag per leaf ({n})")
         out = np.empty_like(self.lev)
         at, L, oi, oj, i, j = self._c
-        if not _forest_lib().gff_adapt(self.nx, self.ny, self.minlevel, self.maxlevel, self.lev, at, n, L, i, j, oi, oj,
-                                       np.ascontiguousarray(r, dtype=np.uint8), np.ascontiguousarray(c, dtype=np.uint8),
-                                       out):
+        rc = _forest_lib().gff_adapt(self.nx, self.ny, self.minlevel, self.maxlevel, self.lev, at, n, L, i, j, oi, oj,
+                                     np.ascontiguousarray(r, dtype=np.uint8), np.ascontiguousarray(c, dtype=np.uint8), out)
+        if rc < 0:
+            raise MemoryError("gff_adapt: work arrays")
+        if rc == 0:
             return self
         return Forest(self.nx0, self.ny0, self.minlevel, self.maxlevel, out)
 

@@ -85,6 +85,7 @@ extern "C" {
 // forest (a level outside [0, M], or a block not of one level).
 int gff_index(int nx, int ny, int M, const int* lev, int* L, int* I, int* J, int* OI, int* OJ, int* leaf_at)
 {
+    if (M < 0 || M > 15 || nx <= 0 || ny <= 0 || nx % (1 << M) || ny % (1 << M)) return -1;
     const size_t cells = (size_t)nx * ny;
     for (size_t c = 0; c < cells; c++) {
         if (lev[c] < 0 || lev[c] > M) return -1;
@@ -179,11 +180,12 @@ int gff_tables(int nx, int ny, int M, const int* lev, const int* leaf_at, int n,
 // all have coarsen[k] and none refine[k] merges into its parent; then the
 // coarsest 2:1-balanced tree (faces and corners) at least as fine as that
 // request, which may refine further leaves or keep a family from merging.
-// Writes out[ny * nx]; returns 1 when it differs from lev, else 0.
+// Writes out[ny * nx]; returns 1 when it differs from lev, else 0 (-2 when
+// its work arrays cannot be allocated).
 int gff_adapt(int nx, int ny, int minlevel, int M, const int* lev, const int* leaf_at, int n, const int* L,
               const int* I, const int* J, const int* OI, const int* OJ, const unsigned char* refine,
               const unsigned char* coarsen, int* out)
-{
+try {
     const size_t cells = (size_t)nx * ny;
     std::vector<int> req(lev, lev + cells);
     bool request = false;
@@ -237,6 +239,8 @@ int gff_adapt(int nx, int ny, int minlevel, int M, const int* lev, const int* le
     }
     for (int y = 0; y < ny; y++) std::memcpy(out + (size_t)y * nx, &t.out[(size_t)(y + 1) * W + 1], sizeof(int) * nx);
     return std::memcmp(out, lev, sizeof(int) * cells) != 0 ? 1 : 0;
+} catch (...) {
+    return -2;  // no exception crosses the C ABI (the work vectors' allocation)
 }
 
 }  // extern "C"
