@@ -1151,8 +1151,12 @@ __global__ void __launch_bounds__(256) k_inflow(InflowArgs a)
         double* q = base + (size_t)patch * 3 * NP2;
         double* ax = a.aux + (size_t)patch * NP2;
         __syncthreads();  // the previous patch's write-back has read L
+        // bc2's side passes read the ghost ring and the MBC interior rows and
+        // columns next to it (mirror and boundary cells): the deeper interior
+        // is never read, and not loaded
         for (int e = threadIdx.x; e < NP2; e += blockDim.x) {
             const int fi = e % NXP - 1, fj = e / NXP - 1;
+            if (fi > MBC && fi <= MX - MBC && fj > MBC && fj <= MY - MBC) continue;
             const double* src = q;
             int kk = e;
             bool nu = false, nv = false;
