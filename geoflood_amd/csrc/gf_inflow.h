@@ -17,14 +17,26 @@
 namespace gf {
 
 // hydrograh_module.f90:186-196: clamp outside the table, else the first
-// interval [time(i), time(i+1)] holding t.
+// interval [time(i), time(i+1)] holding t.  Evaluated by a whole wavefront
+// (every lane must call it, with the same arguments; every lane returns the
+// value): lane l tests interval base + l for 64 intervals at a time, and the
+// lowest lane that holds t is the serial search's first interval, so the
+// value is the serial formula's on the same interval.
 __device__ inline double hydro_interpolate(const double* __restrict__ tt, const double* __restrict__ hu,
                                            int n, double t)
 {
     if (t < tt[0]) return hu[0];
     if (t > tt[n - 1]) return hu[n - 1];
-    for (int i = 0; i < n - 1; i++)
-        if (t >= tt[i] && t <= tt[i + 1]) return hu[i] + (hu[i + 1] - hu[i]) / (tt[i + 1] - tt[i]) * (t - tt[i]);
+    const int lane = (int)(threadIdx.x & 63u);
+    for (int base = 0; base < n - 1; base += 64) {
+        const int i = base + lane;
+        const bool hit = i < n - 1 && t >= tt[i] && t <= tt[i + 1];
+        const unsigned long long m = __ballot(hit);
+        if (m) {
+            const int f = base + __builtin_ctzll(m);
+            return hu[f] + (hu[f + 1] - hu[f]) / (tt[f + 1] - tt[f]) * (t - tt[f]);
+        }
+    }
     return 0.0;
 }
 

@@ -1062,7 +1062,7 @@ __global__ void __launch_bounds__(256) k_ghost(GhostArgs a)
 // writes only this side's ghost cells.
 template <int MX, int MY>
 __device__ void bc2_side(double* L, int side, int code, const PatchDesc* __restrict__ dp,
-                         const ScenCtrl* __restrict__ c, double t, const Params& P)
+                         const ScenCtrl* __restrict__ c, double hu0, const Params& P)
 {
     using G = Geo<MX, MY>;
     constexpr int NP2 = G::NP2;
@@ -1094,7 +1094,6 @@ __device__ void bc2_side(double* L, int side, int code, const PatchDesc* __restr
             const double coord = lo + ((double)r - 0.5) * del;
             if (fabs(coord - c0) <= c->hyd_b / 2.0) {
                 const double h1 = L[bk], hu1 = L[NP2 + bk];
-                const double hu0 = hydro_interpolate(c->hyd, c->hyd + c->hyd_rows, c->hyd_rows, t);
                 mirror = false;
                 double h0;
                 if (h1 < P.dry_tolerance) {
@@ -1174,9 +1173,14 @@ __global__ void __launch_bounds__(256) k_inflow(InflowArgs a)
             L[3 * NP2 + e] = ax[e];
         }
         __syncthreads();
+        // the discharge per unit width at time t, once for the patch (all
+        // four waves take the same branch and value)
+        bool inflow = false;
+        for (int side = 0; side < 4; side++) inflow |= dp->nbr[side] == NB_INFLOW;
+        const double hu0 = inflow ? hydro_interpolate(c->hyd, c->hyd + c->hyd_rows, c->hyd_rows, t) : 0.0;
         for (int side = 0; side < 4; side++) {
             const int code = dp->nbr[side];
-            if (code == NB_WALL || code == NB_EXTRAP || code == NB_INFLOW) bc2_side<MX, MY>(L, side, code, dp, c, t, a.P);
+            if (code == NB_WALL || code == NB_EXTRAP || code == NB_INFLOW) bc2_side<MX, MY>(L, side, code, dp, c, hu0, a.P);
             __syncthreads();
         }
         for (int e = threadIdx.x; e < NP2; e += blockDim.x) {
