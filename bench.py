@@ -259,7 +259,7 @@ def autotune_sweep(steps, sizes=(8, 16, 32, 64), maxlevel=5):
                     ms = v["ms"]
                     gain.append(dict(bucket=int(k.split(":")[-1]), active=v["active"], chosen=v["variant"],
                                      chosen_ms=ms[v["variant"]], variant0_ms=ms[0],
-                                     speedup_vs_variant0=round(ms[0] / ms[v["variant"]], 3)))
+                                     speedup_vs_variant0=round(ms[0] / ms[v["variant"]], 3), ms_by_variant=ms))
                 dev_s = run.stats["device_ms"] * 1e-3
                 out[str(mx)] = dict(steps=info["nsteps"], attempts=info["attempts"], retakes=info["retakes"],
                                     wall_s=round(wall, 3), device_s=round(dev_s, 4),
