@@ -28,11 +28,14 @@ def flood_params(gf, oracle_c):
     gf.set_params()
 
 
-def _setup(n, mx, peak, y0=None, mthbc=(0, 1, 3, 3), seed=0):
+def _setup(n, mx, peak, y0=None, mthbc=(0, 1, 3, 3), seed=0, rows=None):
     sc = overland_flood(n, peak_q=peak, seed=seed)
     hy = sc["hydro"]
     if y0 is not None:
         hy["y0"] = y0
+    if rows is not None:
+        from geoflood_amd.scenarios import sine_hydrograph
+        hy["time"], hy["hu"] = sine_hydrograph(peak, hy["b"], rows=rows)
     H = Hydrograph(hy["time"], hy["hu"], hy["b"], hy["x0"], hy["y0"], (1, 0, 0, 0))
     m = n // mx
     br = Brick.from_global(sc["h"], sc["hu"], sc["hv"], sc["b"], m, m, 2, sc["dx"], sc["dx"],
@@ -44,15 +47,22 @@ def _setup(n, mx, peak, y0=None, mthbc=(0, 1, 3, 3), seed=0):
     return sc, hy, br, layout, pool
 
 
-@pytest.mark.parametrize("t,y0", [(900.0, None), (0.0, None), (1800.0, 0.0)])
-def test_inflow_ghost_fill_forestclaw_order(gf, oracle_c, flood_params, t, y0):
+@pytest.mark.parametrize("t,y0,rows", [(900.0, None, None), (0.0, None, None), (1800.0, 0.0, None),
+                                       (64, None, 200), (130, None, 200), (2999.7, None, 200)])
+def test_inflow_ghost_fill_forestclaw_order(gf, oracle_c, flood_params, t, y0, rows):
     """Random wet/dry states on a 4x4 brick of 16x16 patches: every cell of
     every patch (ghost ring, corners, aux) after the device ghost fill equals
-    the oracle's ForestClaw-order fill with bc2's hydrograph side."""
-    print(f"case inflow ghost fill t={t} y0={y0}", flush=True)
+    the oracle's ForestClaw-order fill with bc2's hydrograph side.  The
+    61-row table's cases fall on knots (t = 0, 900, 1800 s); a 200-row table
+    (the device searches 64 intervals at a time) is taken at its row 64 (a
+    knot on the boundary of the first 64 intervals: the earlier interval
+    holds it), at row 130 and between knots in the third 64."""
     rng = np.random.default_rng(int(t) + 7)
     n, mx = 64, 16
-    sc, hy, br, layout, pool = _setup(n, mx, 200.0, y0=y0)
+    sc, hy, br, layout, pool = _setup(n, mx, 200.0, y0=y0, rows=rows)
+    if rows is not None and isinstance(t, int):
+        t = float(hy["time"][t])                 # exactly on the table's knot
+    print(f"case inflow ghost fill t={t!r} y0={y0} rows={rows}", flush=True)
     h = np.where(rng.random((n, n)) < 0.5, 0.0, rng.uniform(0, 2, (n, n)))
     h[rng.random((n, n)) < 0.05] = 1e-3
     hu = rng.normal(0, 0.5, (n, n)) * h
