@@ -1899,34 +1899,37 @@ bool default_method(const Params& P)
     return P.order == 2 && P.transverse == 2 && P.mthlim[0] == 4 && P.mthlim[1] == 4 && P.mthlim[2] == 4;
 }
 
-// Step variants per patch shape: X(MX, TX, NT, NPP, FREG) = tile TX x TX,
-// NT threads per patch (tile), NPP patches per workgroup, x-face increments
-// in registers (FREG 1) or LDS (0).  The first variant of a shape is its
-// default; the autotuner picks among them by timing.  Every variant gives
-// the same results bit for bit.
+// Step variants per patch shape: X(MX, TX, TY, NT, NPP, FREG) = tiles of
+// TX x TY cells, NT threads per tile, NPP patches per workgroup (one-tile
+// variants), x-face increments in registers (FREG 1) or LDS (0).  The first
+// variant of a shape is its default; the autotuner picks among them by
+// timing.  Every variant gives the same results bit for bit.
 #define GF_VARIANTS(X)       \
-    X(8, 8, 128, 1, 0)       \
-    X(8, 8, 128, 1, 1)       \
-    X(8, 8, 128, 2, 1)       \
-    X(8, 8, 128, 3, 1)       \
-    X(16, 16, 384, 1, 0)     \
-    X(16, 16, 192, 1, 0)     \
-    X(16, 16, 384, 1, 1)     \
-    X(16, 16, 192, 1, 1)     \
-    X(16, 16, 384, 2, 1)     \
-    X(16, 16, 192, 2, 1)     \
-    X(32, 32, 640, 1, 0)     \
-    X(32, 16, 384, 1, 0)     \
-    X(32, 32, 768, 1, 0)     \
-    X(32, 32, 704, 1, 0)     \
-    X(32, 32, 1024, 1, 0)    \
-    X(32, 16, 384, 1, 1)     \
-    X(64, 32, 640, 1, 0)     \
-    X(64, 16, 384, 1, 0)     \
-    X(64, 32, 768, 1, 0)
+    X(8, 8, 8, 128, 1, 0)       \
+    X(8, 8, 8, 128, 1, 1)       \
+    X(8, 8, 8, 128, 2, 1)       \
+    X(8, 8, 8, 128, 3, 1)       \
+    X(16, 16, 16, 384, 1, 0)    \
+    X(16, 16, 16, 192, 1, 0)    \
+    X(16, 16, 16, 384, 1, 1)    \
+    X(16, 16, 16, 192, 1, 1)    \
+    X(16, 16, 16, 384, 2, 1)    \
+    X(16, 16, 16, 192, 2, 1)    \
+    X(32, 32, 32, 640, 1, 0)    \
+    X(32, 16, 16, 384, 1, 0)    \
+    X(32, 32, 32, 768, 1, 0)    \
+    X(32, 32, 32, 704, 1, 0)    \
+    X(32, 32, 32, 1024, 1, 0)   \
+    X(32, 16, 16, 384, 1, 1)    \
+    X(32, 32, 16, 384, 1, 1)    \
+    X(32, 32, 16, 704, 1, 1)    \
+    X(64, 32, 32, 640, 1, 0)    \
+    X(64, 16, 16, 384, 1, 0)    \
+    X(64, 32, 32, 768, 1, 0)    \
+    X(64, 32, 16, 704, 1, 1)
 
 static const StepVariant kVariants[] = {
-#define GF_ROW(MX_, TX_, NT_, NPP_, FREG_) {MX_, MX_, TX_, TX_, NT_, NPP_, FREG_},
+#define GF_ROW(MX_, TX_, TY_, NT_, NPP_, FREG_) {MX_, MX_, TX_, TY_, NT_, NPP_, FREG_},
     GF_VARIANTS(GF_ROW)
 #undef GF_ROW
 };
@@ -1948,9 +1951,9 @@ bool get_variant(int mx, int my, int k, StepVariant* out)
     return false;
 }
 
-static bool same_variant(const StepVariant& v, int mx, int tx, int nt, int npp, int freg)
+static bool same_variant(const StepVariant& v, int mx, int tx, int ty, int nt, int npp, int freg)
 {
-    return v.mx == mx && v.tx == tx && v.nt == nt && v.npp == npp && v.freg == freg;
+    return v.mx == mx && v.tx == tx && v.ty == ty && v.nt == nt && v.npp == npp && v.freg == freg;
 }
 
 template <typename K>
@@ -1965,9 +1968,9 @@ int wet_occupancy(int mx, int my, int variant)
 {
     StepVariant v;
     if (mx != my || !get_variant(mx, my, variant, &v)) return 1;
-#define GF_OCC(MX_, TX_, NT_, NPP_, FREG_)                                                  \
-    if (same_variant(v, MX_, TX_, NT_, NPP_, FREG_))                                       \
-        return occupancy_of(k_all<MX_, MX_, TX_, TX_, NT_, NPP_, (bool)FREG_, true>, NT_ * NPP_);
+#define GF_OCC(MX_, TX_, TY_, NT_, NPP_, FREG_)                                             \
+    if (same_variant(v, MX_, TX_, TY_, NT_, NPP_, FREG_))                                  \
+        return occupancy_of(k_all<MX_, MX_, TX_, TY_, NT_, NPP_, (bool)FREG_, true>, NT_ * NPP_);
     GF_VARIANTS(GF_OCC)
 #undef GF_OCC
     return 1;
@@ -1982,10 +1985,10 @@ hipError_t launch_step(int mx, int my, int variant, const StepArgs& a, hipStream
     const int grid = a.count * (mx / v.tx) * (my / v.ty);
     const bool def = default_method(a.P);
     // the plain step runs one tile per workgroup: a variant's NPP does not apply
-#define GF_STEP(MX_, TX_, NT_, NPP_, FREG_)                                                 \
-    if (same_variant(v, MX_, TX_, NT_, NPP_, FREG_)) {                                     \
-        if (def) k_step<MX_, MX_, TX_, TX_, NT_, (bool)FREG_, true><<<grid, NT_, 0, st>>>(a);  \
-        else k_step<MX_, MX_, TX_, TX_, NT_, (bool)FREG_, false><<<grid, NT_, 0, st>>>(a);     \
+#define GF_STEP(MX_, TX_, TY_, NT_, NPP_, FREG_)                                             \
+    if (same_variant(v, MX_, TX_, TY_, NT_, NPP_, FREG_)) {                                \
+        if (def) k_step<MX_, MX_, TX_, TY_, NT_, (bool)FREG_, true><<<grid, NT_, 0, st>>>(a);  \
+        else k_step<MX_, MX_, TX_, TY_, NT_, (bool)FREG_, false><<<grid, NT_, 0, st>>>(a);     \
         return hipGetLastError();                                                           \
     }
     GF_VARIANTS(GF_STEP)
@@ -2030,10 +2033,10 @@ hipError_t launch_all(int mx, int my, int variant, bool def, const BatchArgs* da
 {
     StepVariant v;
     if (mx != my || !get_variant(mx, my, variant, &v)) return hipErrorInvalidValue;
-#define GF_ALL(MX_, TX_, NT_, NPP_, FREG_)                                                                      \
-    if (same_variant(v, MX_, TX_, NT_, NPP_, FREG_)) {                                                         \
-        if (def) k_all<MX_, MX_, TX_, TX_, NT_, NPP_, (bool)FREG_, true><<<grid, NT_ * NPP_, 0, st>>>(da);     \
-        else k_all<MX_, MX_, TX_, TX_, NT_, NPP_, (bool)FREG_, false><<<grid, NT_ * NPP_, 0, st>>>(da);        \
+#define GF_ALL(MX_, TX_, TY_, NT_, NPP_, FREG_)                                                                 \
+    if (same_variant(v, MX_, TX_, TY_, NT_, NPP_, FREG_)) {                                                    \
+        if (def) k_all<MX_, MX_, TX_, TY_, NT_, NPP_, (bool)FREG_, true><<<grid, NT_ * NPP_, 0, st>>>(da);     \
+        else k_all<MX_, MX_, TX_, TY_, NT_, NPP_, (bool)FREG_, false><<<grid, NT_ * NPP_, 0, st>>>(da);        \
         return hipGetLastError();                                                                               \
     }
     GF_VARIANTS(GF_ALL)
