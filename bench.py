@@ -538,7 +538,7 @@ def main():
         from geoflood_amd.serving import Engine, make_requests, summarize
         geoflood_amd.set_params(**dict(PARAMS, manning_n=0.035))
         reqs = make_requests(args.ensemble_requests, args.ensemble_rate, seed=0)
-        eng = Engine(args.ensemble_capacity, substeps=8)
+        eng = Engine(args.ensemble_capacity, substeps=16)
         log(f"ensemble: {len(reqs)} requests at {args.ensemble_rate}/s ...")
         res = eng.serve(reqs)
         ensemble = summarize(res, eng.wall, eng.cell_updates)

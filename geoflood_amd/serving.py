@@ -146,7 +146,7 @@ class Engine:
     """Serving loop over one pool of ``capacity`` 32x32 patch slots (the HBM
     budget: ``Pool.bytes_for(32, 32, capacity)`` bytes)."""
     capacity: int
-    substeps: int = 8           # graph replays between host polls
+    substeps: int = 16          # graph replays between host polls (each poll leaves the GPU idle)
     keep_states: bool = False   # keep each result's final (h, hu, hv)
     pool: Pool = field(init=False)
     states: dict = field(default_factory=dict)
@@ -309,7 +309,7 @@ def main():
     ap.add_argument("--rate", type=float, default=32.0)
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--capacity", type=int, default=65536, help="pool slots (32x32 patches)")
-    ap.add_argument("--substeps", type=int, default=8)
+    ap.add_argument("--substeps", type=int, default=16)
     ap.add_argument("--t-end", type=float, default=T_END)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--jsonl", default=None, help="write one JSON line per finished request to this file")
