@@ -44,7 +44,9 @@ FOREST_SOURCE = "gf_forest.cpp"   # host only: the resident-AMR driver's quadtre
 
 def build_forest(verbose: bool = False) -> str:
     tmp = FOREST_LIB + ".tmp"
-    cmd = ["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", tmp, os.path.join(CSRC, FOREST_SOURCE)]
+    # x86-64-v2 (SSE4.2): byte max/compare vectors for the lattice passes
+    cmd = ["g++", "-O3", "-march=x86-64-v2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", tmp,
+           os.path.join(CSRC, FOREST_SOURCE)]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -14,7 +14,8 @@
 // s = 2^(M - L).  Leaves are numbered in the order (L, j, i).
 //
 // Every function is a plain loop over the lattice (a few passes per
-// regrid); arrays are int32, flags one byte per leaf.
+// regrid); arrays are int32, flags one byte per leaf, and adapt's work
+// lattices one byte per patch.
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -30,48 +31,63 @@ constexpr int kDy[8] = {0, 0, -1, 1, -1, -1, 1, 1};
 // finest patch takes the smallest L whose level-L block around it has
 // max(req) <= L (true at L = M when req <= M).  mx[k] holds the block maxima
 // at the resolution of level M - k (mx[0] = req); out is the padded lattice
-// (a border of -1 around nx x ny).
+// (a border of -1 around nx x ny).  Levels are bytes (0..15).
 struct Tree {
     int nx, ny, M;
-    std::vector<std::vector<int>> mx;
-    std::vector<int> out;   // (ny + 2) x (nx + 2)
+    std::vector<std::vector<int8_t>> mx;
+    std::vector<int8_t> out;   // (ny + 2) x (nx + 2)
 
     Tree(int nx_, int ny_, int M_) : nx(nx_), ny(ny_), M(M_), mx(M_ + 1), out((size_t)(nx_ + 2) * (ny_ + 2), -1)
     {
         for (int k = 0; k <= M; k++) mx[k].resize((size_t)(nx >> k) * (ny >> k));
     }
 
-    void fill(int l, int by, int bx)
-    {
-        const int k = M - l;
-        if (l == M || mx[k][(size_t)by * (nx >> k) + bx] <= l) {
-            const int s = 1 << k, W = nx + 2;
-            for (int y = by * s; y < (by + 1) * s; y++) {
-                int* row = &out[(size_t)(y + 1) * W + 1];
-                std::fill(row + bx * s, row + (bx + 1) * s, l);
-            }
-            return;
-        }
-        for (int cy = 0; cy < 2; cy++)
-            for (int cx = 0; cx < 2; cx++) fill(l + 1, 2 * by + cy, 2 * bx + cx);
-    }
+    // lv[k]: the level assigned to each block at the resolution of level
+    // M - k, or -1 while none is (a block takes its parent's level, else L
+    // when its maxima allow it)
+    std::vector<std::vector<int8_t>> lv;
 
-    void build(const std::vector<int>& req)
+    void build(const std::vector<int8_t>& req)
     {
         std::copy(req.begin(), req.end(), mx[0].begin());
         for (int k = 1; k <= M; k++) {
             const int w = nx >> k, h = ny >> k, wp = nx >> (k - 1);
-            const int* a = mx[k - 1].data();
-            int* b = mx[k].data();
-            for (int y = 0; y < h; y++)
-                for (int x = 0; x < w; x++) {
-                    const int* r0 = a + (size_t)(2 * y) * wp + 2 * x;
-                    const int* r1 = r0 + wp;
-                    b[(size_t)y * w + x] = std::max(std::max(r0[0], r0[1]), std::max(r1[0], r1[1]));
-                }
+            const int8_t* a = mx[k - 1].data();
+            int8_t* b = mx[k].data();
+            for (int y = 0; y < h; y++) {
+                const int8_t* __restrict__ r0 = a + (size_t)(2 * y) * wp;
+                const int8_t* __restrict__ r1 = r0 + wp;
+                int8_t* __restrict__ o = b + (size_t)y * w;
+                for (int x = 0; x < w; x++)
+                    o[x] = std::max(std::max(r0[2 * x], r0[2 * x + 1]), std::max(r1[2 * x], r1[2 * x + 1]));
+            }
         }
-        for (int by = 0; by < (ny >> M); by++)
-            for (int bx = 0; bx < (nx >> M); bx++) fill(0, by, bx);
+        if (lv.empty()) {
+            lv.resize(M + 1);
+            for (int k = 0; k <= M; k++) lv[k].resize((size_t)(nx >> k) * (ny >> k));
+        }
+        {
+            const int8_t* m0 = mx[M].data();
+            int8_t* l0 = lv[M].data();
+            for (size_t b = 0; b < lv[M].size(); b++) l0[b] = m0[b] <= 0 ? 0 : -1;
+        }
+        for (int L = 1; L <= M; L++) {
+            const int k = M - L, w = nx >> k, h = ny >> k, wp = nx >> (k + 1);
+            const int8_t* par = lv[k + 1].data();
+            const int8_t* m = mx[k].data();
+            int8_t* o = lv[k].data();
+            for (int y = 0; y < h; y++) {
+                const int8_t* __restrict__ pr = par + (size_t)(y >> 1) * wp;
+                const int8_t* __restrict__ mr = m + (size_t)y * w;
+                int8_t* __restrict__ orow = o + (size_t)y * w;
+                for (int x = 0; x < w; x++) {
+                    const int8_t pv = pr[x >> 1];
+                    orow[x] = pv >= 0 ? pv : (mr[x] <= L ? (int8_t)L : (int8_t)-1);
+                }
+            }
+        }
+        const int W = nx + 2;
+        for (int y = 0; y < ny; y++) std::memcpy(&out[(size_t)(y + 1) * W + 1], &lv[0][(size_t)y * nx], (size_t)nx);
     }
 };
 
@@ -187,12 +203,13 @@ int gff_adapt(int nx, int ny, int minlevel, int M, const int* lev, const int* le
               const unsigned char* coarsen, int* out)
 try {
     const size_t cells = (size_t)nx * ny;
-    std::vector<int> req(lev, lev + cells);
+    std::vector<int8_t> req(cells);
+    for (size_t c = 0; c < cells; c++) req[c] = (int8_t)lev[c];
     bool request = false;
     for (int k = 0; k < n; k++) {
         if (!refine[k] || L[k] >= M) continue;
         const int s = 1 << (M - L[k]);
-        for (int y = OJ[k]; y < OJ[k] + s; y++) std::fill(&req[(size_t)y * nx + OI[k]], &req[(size_t)y * nx + OI[k] + s], L[k] + 1);
+        for (int y = OJ[k]; y < OJ[k] + s; y++) std::memset(&req[(size_t)y * nx + OI[k]], L[k] + 1, (size_t)s);
         request = true;
     }
     // families: the leaf at the lower-left of its parent, and its three
@@ -207,8 +224,7 @@ try {
             all = lev[c] == L[k] && coarsen[m] && !refine[m];
         }
         if (!all) continue;
-        for (int y = OJ[k]; y < OJ[k] + 2 * s; y++)
-            std::fill(&req[(size_t)y * nx + OI[k]], &req[(size_t)y * nx + OI[k] + 2 * s], L[k] - 1);
+        for (int y = OJ[k]; y < OJ[k] + 2 * s; y++) std::memset(&req[(size_t)y * nx + OI[k]], L[k] - 1, (size_t)(2 * s));
         request = true;
     }
     if (!request) {
@@ -217,28 +233,39 @@ try {
         return 0;
     }
     // balance closure
-    for (size_t c = 0; c < cells; c++) req[c] = std::max(req[c], minlevel);
+    const int8_t lo = (int8_t)minlevel;
+    for (size_t c = 0; c < cells; c++) req[c] = std::max(req[c], lo);
     Tree t(nx, ny, M);
     const int W = nx + 2;
     for (;;) {
         t.build(req);
-        bool ok = true;
+        int ok = 1;
         for (int y = 0; y < ny; y++) {
-            const int* up = &t.out[(size_t)y * W + 1];
-            const int* mid = up + W;
-            const int* dn = mid + W;
-            int* rq = &req[(size_t)y * nx];
+            const int8_t* __restrict__ up = &t.out[(size_t)y * W + 1];
+            const int8_t* __restrict__ mid = up + W;
+            const int8_t* __restrict__ dn = mid + W;
+            int8_t* __restrict__ rq = &req[(size_t)y * nx];
             for (int x = 0; x < nx; x++) {
-                const int nb = std::max(std::max(std::max(up[x - 1], up[x]), std::max(up[x + 1], mid[x - 1])),
-                                        std::max(std::max(mid[x + 1], dn[x - 1]), std::max(dn[x], dn[x + 1])));
-                ok &= mid[x] >= nb - 1;
-                rq[x] = std::max(rq[x], nb - 1);
+                const int8_t nb = std::max(std::max(std::max(up[x - 1], up[x]), std::max(up[x + 1], mid[x - 1])),
+                                           std::max(std::max(mid[x + 1], dn[x - 1]), std::max(dn[x], dn[x + 1])));
+                const int8_t need = (int8_t)(nb - 1);
+                ok &= mid[x] >= need;
+                rq[x] = std::max(rq[x], need);
             }
         }
         if (ok) break;
     }
-    for (int y = 0; y < ny; y++) std::memcpy(out + (size_t)y * nx, &t.out[(size_t)(y + 1) * W + 1], sizeof(int) * nx);
-    return std::memcmp(out, lev, sizeof(int) * cells) != 0 ? 1 : 0;
+    int changed = 0;
+    for (int y = 0; y < ny; y++) {
+        const int8_t* row = &t.out[(size_t)(y + 1) * W + 1];
+        int* o = out + (size_t)y * nx;
+        const int* l0 = lev + (size_t)y * nx;
+        for (int x = 0; x < nx; x++) {
+            o[x] = row[x];
+            changed |= o[x] != l0[x];
+        }
+    }
+    return changed;
 } catch (...) {
     return -2;  // no exception crosses the C ABI (the work vectors' allocation)
 }
