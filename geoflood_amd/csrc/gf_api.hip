@@ -198,6 +198,33 @@ constexpr int kLinkGrid = 1024;  // workgroups of a resident coarse/fine transfe
 // numbers: a patch whose new state is not finite reports a NaN CFL number.
 double max_nan(double a, double b) { return (b > a || b != b) ? b : a; }
 
+// Host memory that the copy engines read directly (hipHostMalloc), for the
+// pool's descriptor table: a regrid re-describes every leaf (72 bytes each,
+// 1.7 MB at 23k leaves), and an upload from pageable memory goes through
+// the runtime's staging copy first.  Every upload from it is followed by a
+// stream synchronisation before the table is written again.
+template <typename T>
+struct PinnedAlloc {
+    using value_type = T;
+    PinnedAlloc() = default;
+    template <typename U>
+    PinnedAlloc(const PinnedAlloc<U>&) {}
+    T* allocate(size_t n)
+    {
+        void* m = nullptr;
+        if (hipHostMalloc(&m, n * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            throw std::bad_alloc();
+        }
+        return static_cast<T*>(m);
+    }
+    void deallocate(T* m, size_t) { (void)hipHostFree(m); }
+};
+template <typename T, typename U>
+bool operator==(const PinnedAlloc<T>&, const PinnedAlloc<U>&) { return true; }
+template <typename T, typename U>
+bool operator!=(const PinnedAlloc<T>&, const PinnedAlloc<U>&) { return false; }
+
 }  // namespace
 
 struct gf_pool {
@@ -237,7 +264,7 @@ struct gf_pool {
     bool args_dirty = true;
     unsigned args_gen = 0;                 // params generation of d_args and of the graph
     int args_n = -1, args_nscen = -1;      // the slot and scenario counts in d_args
-    std::vector<PatchDesc> h_desc;
+    std::vector<PatchDesc, PinnedAlloc<PatchDesc>> h_desc;
     std::vector<ScenCtrl> h_ctrl;
     int nscen = 0;
     hipGraph_t graph = nullptr;
