@@ -235,6 +235,7 @@ class AmrRun:
         self.tune = tune_cache
         self._tune_bucket = None
         self._tune_stats = None
+        self._retune_due = False
         self.mthbc, self.hydro, self.eta_tol = tuple(mthbc), hydro, eta_tol
         self.regrid_interval, self.manning_n, self.dt_max = regrid_interval, manning_n, dt_max
         self.capacity = capacity
@@ -426,6 +427,12 @@ class AmrRun:
         while info["nsteps"] < target:
             before = info["nsteps"]
             nxt = min(target, (before // self.regrid_interval + 1) * self.regrid_interval)
+            if self._retune_due:
+                # the autotuner times the patches of active scenarios only: set
+                # the next stop first (the scenario stopped at the regrid step)
+                self.pool.set_stop(self.sid, nxt)
+                self._retune()
+                self._retune_due = False
             # the scenario stops on the device at the next regrid step: replay
             # that many attempts, then single ones for any retakes
             if graph:
@@ -450,7 +457,7 @@ class AmrRun:
                 self.regrid(info)
                 self.stats["patches"].append(len(self.forest))
                 if self.tune is not None and self.stats["regrids"] % 4 == 1:
-                    self._retune()
+                    self._retune_due = True
         self.stats["steps"] = info["nsteps"]
         self.stats["retakes"] = info["retakes"]
         return info

@@ -1885,6 +1885,16 @@ try {
         HIPCHK(gf::launch_inflow(p->mx, p->my, inflow_args(p, 0, p->capacity, -1, 0.0), inflow_grid(), g.stream));
     }
     HIPCHK(gf::launch_classify(p->d_args, p->capacity, g.stream));
+    // an inactive scenario (stopped at its target, or between regrids) lists
+    // nothing: with every list empty there is no work to time
+    int cnt[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(cnt, p->counters + gf::CNT_WET, sizeof(cnt), hipMemcpyDeviceToHost, g.stream));
+    HIPCHK(hipStreamSynchronize(g.stream));
+    if (cnt[0] == 0 && cnt[1] == 0) {
+        HIPCHK(hipMemsetAsync(p->counters, 0, gf::NCOUNTERS * sizeof(int), g.stream));
+        HIPCHK(hipStreamSynchronize(g.stream));
+        return fail(GF_ESTATE, "no active or dry patch to time: every scenario is inactive or steady");
+    }
     if (int r = ensure_pair_events(g.ev_tune, 1)) return r;
     const int saved = p->variant;
     double best_ms = 0.0;

@@ -28,7 +28,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import init, set_params
+from . import GeoFloodError, init, set_params
 from .pool import MBC, BrickLayout, Pool, global_to_aos
 from .scenarios import gaussian_sum_dem, sine_hydrograph
 
@@ -276,7 +276,10 @@ class Engine:
                 continue
             live = sum(lv.req.npatch for lv in self.live.values())
             if live > RETUNE_FACTOR * self.tuned_at or RETUNE_FACTOR * live < self.tuned_at:
-                self.pool.autotune(reps=1)  # choose the step variant for this population
+                try:
+                    self.pool.autotune(reps=1)  # choose the step variant for this population
+                except GeoFloodError:
+                    pass                        # nothing listed to time (all steady): keep the variant
                 self.tuned_at = live
             self.pool.run(self.substeps, graph=True)
             self.attempts += self.substeps

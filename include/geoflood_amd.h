@@ -377,7 +377,10 @@ int gf_pool_set_variant(gf_pool *pool, int variant);
  * patches of the pool's current states (no scenario advances) and select
  * the fastest; *best (nullable) receives it, ms[k] (nullable, one entry per
  * variant) the average time of variant k in milliseconds.  Re-run when the
- * active population changes (after regrid, or as a flood front grows). */
+ * active population changes (after regrid, or as a flood front grows).
+ * Only active scenarios' patches are stepped: with none listed (every
+ * scenario stopped, or all its patches steady and dry) it returns GF_ESTATE
+ * and keeps the variant. */
 int gf_pool_autotune(gf_pool *pool, int reps, int *best, double *ms);
 
 /* Copy a scenario's control record to the host. */
