@@ -1914,15 +1914,12 @@ bool default_method(const Params& P)
     X(16, 16, 16, 384, 1, 1)    \
     X(16, 16, 16, 192, 1, 1)    \
     X(16, 16, 16, 384, 2, 1)    \
-    X(16, 16, 16, 192, 2, 1)    \
     X(32, 32, 32, 640, 1, 0)    \
     X(32, 16, 16, 384, 1, 0)    \
     X(32, 32, 32, 768, 1, 0)    \
     X(32, 32, 32, 704, 1, 0)    \
     X(32, 32, 32, 1024, 1, 0)   \
     X(32, 16, 16, 384, 1, 1)    \
-    X(32, 32, 16, 384, 1, 1)    \
-    X(32, 32, 16, 704, 1, 1)    \
     X(64, 32, 32, 640, 1, 0)    \
     X(64, 16, 16, 384, 1, 0)    \
     X(64, 32, 32, 768, 1, 0)    \
