@@ -5,7 +5,8 @@ Refusals are status codes with a message, raised before any launch:
 - pools of unsupported shape or size;
 - parameters outside the supported set, which leave the active ones unchanged;
 - runs without a scenario;
-- slots out of range.
+- slots out of range;
+- an autotune with nothing to time (every scenario stopped).
 
 The smallest pool, one 16x16 patch with walls on every side, and a batch of
 ragged cell sizes both step exactly as the oracle does (fc2d_cpucuda.cpp:409-495,
@@ -164,3 +165,34 @@ def test_one_patch_pool_and_ragged_batch_match_oracle(gf, oracle_c):
     finally:
         gf.set_params()
         oracle_c.set_params(Params())
+
+
+def test_autotune_refuses_an_empty_population(gf):
+    """With its only scenario stopped (at its step target) a pool lists no
+    patch: the autotuner refuses instead of timing empty launches, and keeps
+    the variant; once the scenario runs again it times and selects."""
+    print("case edges: autotune on a stopped scenario", flush=True)
+    from geoflood_amd import GeoFloodError
+    from geoflood_amd.scenarios import uniform_dam_break
+    n = 128
+    sc = uniform_dam_break(n, dx=1.0, h0=10.0, rms=3.0, seed=9)
+    layout = BrickLayout(n // 32, n // 32, 32, 32, 1.0, 1.0)
+    pool = Pool(32, 32, layout.npatch)
+    try:
+        pool.set_layout([layout])
+        pool.upload_global(layout, *sc[:4])
+        sid = pool.add_scenario(layout, dt0=0.01, target_steps=2)
+        while pool.run(2, graph=True):
+            pass
+        assert not pool.scenario(sid)["active"]
+        pool.set_variant(1)
+        with pytest.raises(GeoFloodError, match="no active or dry patch"):
+            pool.autotune(reps=1)
+        assert pool.variants()[0] == 1
+        info = pool.scenario(sid)
+        pool.set_stop(sid, info["nsteps"] + 2)
+        best, ms = pool.autotune(reps=1)
+        assert pool.variants()[0] == best and min(ms) > 0
+        assert pool.scenario(sid)["nsteps"] == info["nsteps"]      # timing advances nothing
+    finally:
+        pool.close()
