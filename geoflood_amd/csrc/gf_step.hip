@@ -1173,11 +1173,24 @@ __global__ void __launch_bounds__(256) k_inflow(InflowArgs a)
             L[3 * NP2 + e] = ax[e];
         }
         __syncthreads();
-        // the discharge per unit width at time t, once for the patch (all
-        // four waves take the same branch and value)
-        bool inflow = false;
-        for (int side = 0; side < 4; side++) inflow |= dp->nbr[side] == NB_INFLOW;
-        const double hu0 = inflow ? hydro_interpolate(c->hyd, c->hyd + c->hyd_rows, c->hyd_rows, t) : 0.0;
+        // the discharge per unit width at time t, once for the patch, when a
+        // ghost cell of its hydrograph side lies in the channel (bc2_side's
+        // own test; every wave takes the same branch and value)
+        int in_channel = 0;
+        for (int side = 0; side < 4; side++) {
+            if (dp->nbr[side] != NB_INFLOW) continue;
+            const bool xs = side < 2;
+            const int r = (int)threadIdx.x + 1 - MBC;  // one row (x sides) or column (y sides) per thread
+            if (r <= (xs ? MY : MX) + MBC) {
+                const double lo = xs ? dp->ylo : dp->xlo;
+                const double del = xs ? dp->dy : dp->dx;
+                const double c0 = xs ? c->hyd_y0 : c->hyd_x0;
+                in_channel |= fabs(lo + ((double)r - 0.5) * del - c0) <= c->hyd_b / 2.0;
+            }
+        }
+        const double hu0 = __syncthreads_or(in_channel)
+                               ? hydro_interpolate(c->hyd, c->hyd + c->hyd_rows, c->hyd_rows, t)
+                               : 0.0;
         for (int side = 0; side < 4; side++) {
             const int code = dp->nbr[side];
             if (code == NB_WALL || code == NB_EXTRAP || code == NB_INFLOW) bc2_side<MX, MY>(L, side, code, dp, c, hu0, a.P);
