@@ -50,7 +50,8 @@ _FOREST = None
 
 
 def _forest_lib():
-    """libgfforest.so (gf_forest.cpp): the forest's index, tables and adapt."""
+    """libgfforest.so (gf_forest.cpp): the forest's index, tables and adapt,
+    and the neighbour table in pool slots."""
     global _FOREST
     if _FOREST is None:
         path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgfforest.so")
@@ -63,7 +64,9 @@ def _forest_lib():
         lib.gff_index.argtypes = [ci, ci, ci, ip, ip, ip, ip, ip, ip, ip]
         lib.gff_tables.argtypes = [ci, ci, ci, ip, ip, ci, ip, ip, ip, ip, ci, ci, ip, ip, ip, ip, ip]
         lib.gff_adapt.argtypes = [ci, ci, ci, ci, ip, ip, ci, ip, ip, ip, ip, ip, bp, bp, ip]
-        for f in (lib.gff_index, lib.gff_tables, lib.gff_adapt):
+        lp = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        lib.gff_slot_table.argtypes = [ci, ip, lp, ci, lp, ip]
+        for f in (lib.gff_index, lib.gff_tables, lib.gff_adapt, lib.gff_slot_table):
             f.restype = ci
         _FOREST = lib
     return _FOREST
@@ -158,34 +161,47 @@ class Forest:
         kinds = np.array([_KIND[mthbc[k]] for k in range(4)], dtype=np.int32)
         nl = _forest_lib().gff_tables(self.nx, self.ny, self.maxlevel, self.lev, at, n, L, oi, oj, kinds,
                                       GF_NB_STORED, GF_NB_WALL, nbr, lc, lf, ls, lg)
+        if nl == -2:
+            raise MemoryError("gff_tables: work lists")
         if nl < 0:
             raise ValueError("forest is not 2:1 balanced")
         return nbr, (lc[:nl].copy(), lf[:nl].copy(), ls[:nl].copy(), lg[:nl].copy())
 
 
-def regrid_plan(old: Forest, new: Forest):
-    """How the new forest's leaves arise from the old one's: a list of new
-    leaf indices whose leaf is unchanged (with the old leaf), refined
-    families (old parent, its four new children in igrid order, x fastest)
-    and coarsened families (new parent, its four old children)."""
+def regrid_arrays(old: Forest, new: Forest):
+    """How the new forest's leaves arise from the old one's, as arrays:
+    (new leaves whose leaf is unchanged, their old leaves); refined families
+    (old parents in ascending order, (n, 4) their new children in igrid
+    order, x fastest); coarsened families (new parents in ascending order,
+    (n, 4) their old children in igrid order).  A regrid changes a leaf's
+    level by at most one."""
     M = new.maxlevel
     ko = old.leaf_at[new.oj, new.oi]
     lo = old.L[ko]
     same = np.flatnonzero(lo == new.L)
-    moved = (same, ko[same])
     child = np.flatnonzero(lo == new.L - 1)
-    fams = {}
-    for k in child:
-        g = (new.i[k] & 1) + 2 * (new.j[k] & 1)
-        fams.setdefault(int(ko[k]), [0, 0, 0, 0])[g] = int(k)
-    refined = [(p, kids) for p, kids in sorted(fams.items())]
     par = np.flatnonzero(lo == new.L + 1)
-    coarsened = []
-    for k in par:
-        h = 1 << (M - new.L[k] - 1)
-        kids = [int(old.leaf_at[new.oj[k] + cj * h, new.oi[k] + ci * h]) for cj in range(2) for ci in range(2)]
-        coarsened.append((int(k), kids))
-    assert len(same) + len(child) + len(par) == len(new)
+    if len(same) + len(child) + len(par) != len(new):
+        raise ValueError("a leaf changed level by more than one")
+    g = (new.i[child] & 1) + 2 * (new.j[child] & 1)
+    order = np.lexsort((g, ko[child]))          # by old parent, then igrid
+    kids = child[order].reshape(-1, 4)
+    parents = ko[child[order[::4]]]
+    if not ((g[order].reshape(-1, 4) == np.arange(4)).all() and (ko[kids] == parents[:, None]).all()):
+        raise ValueError("a refined leaf does not have four children")
+    h = np.left_shift(1, M - new.L[par] - 1)[:, None]
+    ci, cj = np.array([0, 1, 0, 1]), np.array([0, 0, 1, 1])
+    ckids = old.leaf_at[new.oj[par][:, None] + cj * h, new.oi[par][:, None] + ci * h]
+    return (same, ko[same]), (parents, kids), (par, ckids)
+
+
+def regrid_plan(old: Forest, new: Forest):
+    """regrid_arrays as lists: unchanged leaves (new, old), refined families
+    [(old parent, [four new children])] and coarsened families [(new parent,
+    [four old children])]."""
+    moved, (rp, rk), (cp, ck) = regrid_arrays(old, new)
+    refined = [(int(p), [int(k) for k in ks]) for p, ks in zip(rp, rk)]
+    coarsened = [(int(p), [int(k) for k in ks]) for p, ks in zip(cp, ck)]
     return moved, refined, coarsened
 
 
@@ -297,10 +313,10 @@ class AmrRun:
         if n > self.capacity:
             raise RuntimeError(f"forest of {n} leaves exceeds the capacity {self.capacity}")
         nbr, _ = forest.tables(self.mthbc)
-        inv = np.empty(n, dtype=np.int64)
-        inv[perm] = np.arange(n)          # leaf of each slot offset
-        nb = nbr[inv]
-        nb = np.where(nb >= 0, base + perm[np.maximum(nb, 0)], nb).astype(np.int32)
+        inv = np.empty(n, dtype=np.int64)  # leaf of each slot offset
+        nb = np.empty((n, 8), dtype=np.int32)
+        if _forest_lib().gff_slot_table(n, nbr, np.ascontiguousarray(perm, dtype=np.int64), base, inv, nb) < 0:
+            raise ValueError("perm is not a permutation of the leaves")
         L, i, j = forest.L[inv], forest.i[inv], forest.j[inv]
         dx, dy = self._dx(L)
         xlo, ylo = i * self.mx * dx, j * self.mx * dy
@@ -394,9 +410,11 @@ class AmrRun:
         if new.same(old):
             return False
         ob, nb_ = self.half * self.capacity, (1 - self.half) * self.capacity
-        (keep_new, keep_old), refined, coarsened = regrid_plan(old, new)
+        (keep_new, keep_old), (rpar, rkids), (cpar, ckids) = regrid_arrays(old, new)
         # new slot order: unchanged leaves first, then the created ones
-        created = np.setdiff1d(np.arange(len(new)), keep_new, assume_unique=True)
+        fresh = np.ones(len(new), dtype=bool)
+        fresh[keep_new] = False
+        created = np.flatnonzero(fresh)
         perm = np.empty(len(new), dtype=np.int64)
         perm[keep_new] = np.arange(len(keep_new))
         perm[created] = len(keep_new) + np.arange(len(created))
@@ -406,14 +424,10 @@ class AmrRun:
             self.pool.upload_aos(nb_ + len(keep_new), np.zeros(aux.shape[:3] + (3,)), aux, buf=cur)
         if len(keep_new):
             self.pool.move(ob + self.perm[keep_old], nb_ + perm[keep_new], buf=cur)
-        if refined:
-            par = np.array([ob + self.perm[p] for p, _ in refined], dtype=np.int32)
-            kids = np.array([nb_ + perm[k] for _, ks in refined for k in ks], dtype=np.int32)
-            self.pool.interpolate2fine(par, kids, buf=cur)
-        if coarsened:
-            par = np.array([nb_ + perm[p] for p, _ in coarsened], dtype=np.int32)
-            kids = np.array([ob + self.perm[k] for _, ks in coarsened for k in ks], dtype=np.int32)
-            self.pool.average2coarse(par, kids, buf=cur)
+        if len(rpar):
+            self.pool.interpolate2fine(ob + self.perm[rpar], (nb_ + perm[rkids]).reshape(-1), buf=cur)
+        if len(cpar):
+            self.pool.average2coarse(nb_ + perm[cpar], (ob + self.perm[ckids]).reshape(-1), buf=cur)
         self.pool.set_scenario_slots(self.sid, nb_, len(new))
         self._links(new, nb_, perm)
         self.forest, self.perm, self.half = new, perm, 1 - self.half

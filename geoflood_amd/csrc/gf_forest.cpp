@@ -144,40 +144,44 @@ int gff_index(int nx, int ny, int M, const int* lev, int* L, int* I, int* J, int
 // half along the side (igrid, 0 the lower coordinate; 0 at corners), then
 // by coarse leaf.  The link arrays hold 12 n entries.  Returns the number of
 // links, or -1 when a finer neighbour is not one level finer (not 2:1
-// balanced).
+// balanced), -2 when a work list cannot be allocated.
+//
+// One pass over the leaves fills nbr and lists, per direction, the leaves
+// with a finer neighbour there (in leaf order); the links are then written
+// from those lists alone.
 int gff_tables(int nx, int ny, int M, const int* lev, const int* leaf_at, int n, const int* L, const int* OI,
                const int* OJ, const int* kinds, int stored, int wall, int* nbr, int* lc, int* lf, int* ls, int* lg)
-{
-    int nl = 0;
-    for (int d = 0; d < 8; d++) {
-        const int dx = kDx[d], dy = kDy[d];
-        bool any_finer = false;
-        for (int k = 0; k < n; k++) {
-            const int s = 1 << (M - L[k]);
-            const int ni = OI[k] + dx * s, nj = OJ[k] + dy * s;
-            const bool inside = ni >= 0 && ni < nx && nj >= 0 && nj < ny;
+try {
+    std::vector<int> finer[8];
+    for (int k = 0; k < n; k++) {
+        const int l = L[k], s = 1 << (M - l);
+        for (int d = 0; d < 8; d++) {
+            const int ni = OI[k] + kDx[d] * s, nj = OJ[k] + kDy[d] * s;
             int v;
-            if (!inside) {
+            if (!(ni >= 0 && ni < nx && nj >= 0 && nj < ny)) {
                 v = d < 4 ? kinds[d] : wall;
             } else {
-                const int ln = lev[(size_t)nj * nx + ni];
-                v = ln == L[k] ? leaf_at[(size_t)nj * nx + ni] : stored;
-                any_finer |= ln > L[k];
+                const size_t c = (size_t)nj * nx + ni;
+                const int ln = lev[c];
+                v = ln == l ? leaf_at[c] : stored;
+                if (ln > l) finer[d].push_back(k);
             }
             nbr[8 * k + d] = v;
         }
-        if (!any_finer) continue;
-        const int ci_lo = dx > 0 ? 0 : 1, ci_hi = dx < 0 ? 1 : (dx > 0 ? 0 : 1);
-        const int cj_lo = dy > 0 ? 0 : 1, cj_hi = dy < 0 ? 1 : (dy > 0 ? 0 : 1);
-        for (int cj = (dy == 0 ? 0 : cj_lo); cj <= cj_hi; cj++)
-            for (int ci = (dx == 0 ? 0 : ci_lo); ci <= ci_hi; ci++)
-                for (int k = 0; k < n; k++) {
-                    const int s = 1 << (M - L[k]);
-                    const int ni = OI[k] + dx * s, nj = OJ[k] + dy * s;
-                    if (!(ni >= 0 && ni < nx && nj >= 0 && nj < ny)) continue;
-                    if (lev[(size_t)nj * nx + ni] <= L[k]) continue;
-                    const int h = s / 2;
-                    const int ki = ni + ci * h, kj = nj + cj * h;
+    }
+    int nl = 0;
+    for (int d = 0; d < 8; d++) {
+        if (finer[d].empty()) continue;
+        const int dx = kDx[d], dy = kDy[d];
+        // the fine leaves' halves next to the coarse leaf: both along the
+        // side, the near one across it
+        const int ci0 = dx < 0 ? 1 : 0, ci1 = dx > 0 ? 0 : 1;
+        const int cj0 = dy < 0 ? 1 : 0, cj1 = dy > 0 ? 0 : 1;
+        for (int cj = cj0; cj <= cj1; cj++)
+            for (int ci = ci0; ci <= ci1; ci++)
+                for (const int k : finer[d]) {
+                    const int s = 1 << (M - L[k]), h = s / 2;
+                    const int ki = OI[k] + dx * s + ci * h, kj = OJ[k] + dy * s + cj * h;
                     const size_t c = (size_t)kj * nx + ki;
                     if (lev[c] != L[k] + 1) return -1;
                     lc[nl] = k;
@@ -188,6 +192,33 @@ int gff_tables(int nx, int ny, int M, const int* lev, const int* leaf_at, int n,
                 }
     }
     return nl;
+} catch (...) {
+    return -2;  // no exception crosses the C ABI (the work lists' allocation)
+}
+
+// A forest's neighbour table laid out in pool slots: leaf k sits in slot
+// base + perm[k].  inv[r] receives the leaf in slot base + r, and row r of
+// out that leaf's row of nbr with every leaf entry v >= 0 replaced by its
+// slot base + perm[v] (negative codes kept).  Returns 0, or -1 when perm is
+// not a permutation of [0, n) or an entry of nbr is not below n.
+int gff_slot_table(int n, const int* nbr, const long long* perm, int base, long long* inv, int* out)
+{
+    for (int r = 0; r < n; r++) inv[r] = -1;
+    for (int k = 0; k < n; k++) {
+        const long long r = perm[k];
+        if (r < 0 || r >= n || inv[r] >= 0) return -1;
+        inv[r] = k;
+    }
+    for (int r = 0; r < n; r++) {
+        const int* row = nbr + 8 * inv[r];
+        int* o = out + 8 * (size_t)r;
+        for (int d = 0; d < 8; d++) {
+            const int v = row[d];
+            if (v >= n) return -1;
+            o[d] = v >= 0 ? base + (int)perm[v] : v;
+        }
+    }
+    return 0;
 }
 
 // The levels after one regrid of the forest lev (its n leaves L, I, J, OI,

@@ -175,3 +175,23 @@ def ref_adapt(f, refine, coarsen) -> np.ndarray:
     if not ref_changes(f, refine, coarsen):
         return np.asarray(f.lev, dtype=np.int64).copy()
     return ref_adapt_full(f, refine, coarsen)
+
+
+def ref_regrid_plan(old, new):
+    """Unchanged, refined and coarsened leaves of a regrid, leaf by leaf:
+    (new leaves kept, their old leaves), {old parent: new children by
+    igrid}, {new parent: old children by igrid}."""
+    M = new.maxlevel
+    kept_new, kept_old, refined, coarsened = [], [], {}, {}
+    for k in range(len(new)):
+        ko = int(old.leaf_at[new.oj[k], new.oi[k]])
+        if old.L[ko] == new.L[k]:
+            kept_new.append(k)
+            kept_old.append(ko)
+        elif old.L[ko] == new.L[k] - 1:
+            refined.setdefault(ko, [None] * 4)[int(new.i[k] & 1) + 2 * int(new.j[k] & 1)] = k
+        else:
+            assert old.L[ko] == new.L[k] + 1
+            h = 1 << (M - int(new.L[k]) - 1)
+            coarsened[k] = [int(old.leaf_at[new.oj[k] + cj * h, new.oi[k] + ci * h]) for cj in range(2) for ci in range(2)]
+    return (kept_new, kept_old), refined, coarsened

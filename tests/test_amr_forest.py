@@ -14,7 +14,7 @@ import pytest
 
 import _forest_ref as R
 from geoflood_amd import GF_NB_STORED
-from geoflood_amd.amr import DIRS, Forest, regrid_plan
+from geoflood_amd.amr import DIRS, Forest, regrid_arrays, regrid_plan
 
 
 def _check(f: Forest):
@@ -107,6 +107,27 @@ def test_regrid_plan_accounts_for_every_leaf():
         assert all(g.L[k] == f.L[p] + 1 for p, ks in refined for k in ks)
         assert all(f.L[k] == g.L[p] + 1 for p, ks in coarsened for k in ks)
         f = g
+
+
+def test_regrid_arrays_equal_the_leaf_by_leaf_plan():
+    """The vectorised regrid plan (AmrRun.regrid's) equals a leaf-by-leaf
+    restatement over regrids that refine and coarsen, with families in
+    ascending parent order and children in igrid order."""
+    rng = np.random.default_rng(5)
+    f = _random_forest(rng, 4)
+    seen = [0, 0]
+    for _ in range(8):
+        g = f.adapt(rng.random(len(f)) < 0.25, rng.random(len(f)) < 0.7)
+        (kn, ko), (rp, rk), (cp, ck) = regrid_arrays(f, g)
+        (rkn, rko), rref, cref = R.ref_regrid_plan(f, g)
+        np.testing.assert_array_equal(kn, rkn)
+        np.testing.assert_array_equal(ko, rko)
+        assert rp.tolist() == sorted(rref) and rk.tolist() == [rref[p] for p in sorted(rref)]
+        assert cp.tolist() == sorted(cref) and ck.tolist() == [cref[p] for p in sorted(cref)]
+        seen[0] += len(rp)
+        seen[1] += len(cp)
+        f = g
+    assert seen[0] > 0 and seen[1] > 0
 
 
 def band_flags(width=0.25, speed=0.15):
@@ -289,3 +310,35 @@ def test_native_forest_refuses_non_forests():
     g = Forest(2, 2, 0, 2, lev)
     with pytest.raises(ValueError, match="balanced"):
         g.tables()
+
+
+def test_slot_table_equals_the_numpy_layout():
+    """gff_slot_table (the neighbour table in pool slots, AmrRun._describe)
+    equals the numpy gather and remap it replaces, for random leaf-to-slot
+    permutations of adapted forests; a non-permutation is refused."""
+    from geoflood_amd.amr import _forest_lib
+    rng = np.random.default_rng(11)
+    f = Forest(3, 2, 1, 4)
+    for it in range(6):
+        r = rng.random(len(f)) < 0.3
+        f = f.adapt(r, ~r & (rng.random(len(f)) < 0.5))
+        nbr, _ = f.tables((0, 1, 3, 3))
+        n = len(f)
+        perm = rng.permutation(n).astype(np.int64)
+        base = int(rng.integers(0, 1000))
+        inv_ref = np.empty(n, dtype=np.int64)
+        inv_ref[perm] = np.arange(n)
+        nb_ref = nbr[inv_ref]
+        nb_ref = np.where(nb_ref >= 0, base + perm[np.maximum(nb_ref, 0)], nb_ref).astype(np.int32)
+        inv = np.empty(n, dtype=np.int64)
+        nb = np.empty((n, 8), dtype=np.int32)
+        assert _forest_lib().gff_slot_table(n, nbr, perm, base, inv, nb) == 0
+        np.testing.assert_array_equal(inv, inv_ref)
+        np.testing.assert_array_equal(nb, nb_ref)
+        assert (nb_ref < 0).any() and (nb_ref >= base).any()
+    bad = perm.copy()
+    bad[0] = bad[1]
+    assert _forest_lib().gff_slot_table(n, nbr, bad, 0, inv, nb) == -1
+    big = nbr.copy()
+    big[0, 0] = n
+    assert _forest_lib().gff_slot_table(n, big, perm, 0, inv, nb) == -1
