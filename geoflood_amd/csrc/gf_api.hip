@@ -2392,9 +2392,10 @@ try {
     c.first_patch = first;
     c.npatch = n;
     p->h_ctrl[s] = c;
-    HIPCHK(hipMemcpyAsync(p->desc + of, p->h_desc.data() + of, sizeof(PatchDesc) * on, hipMemcpyHostToDevice, g.stream));
-    HIPCHK(hipMemcpyAsync(p->desc + first, p->h_desc.data() + first, sizeof(PatchDesc) * n, hipMemcpyHostToDevice,
-                          g.stream));
+    // only the scen field changes: set it on the device (the old range
+    // first, so slots in both ranges end with s, as on the host)
+    HIPCHK(gf::launch_set_scen(p->desc, of, on, -1, g.stream));
+    HIPCHK(gf::launch_set_scen(p->desc, first, n, s, g.stream));
     HIPCHK(hipMemcpyAsync(p->ctrl + s, &c, sizeof(ScenCtrl), hipMemcpyHostToDevice, g.stream));
     if (int r = reset_dry_age(p, first, n)) return r;
     HIPCHK(hipStreamSynchronize(g.stream));

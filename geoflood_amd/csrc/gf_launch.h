@@ -227,6 +227,8 @@ hipError_t launch_eval_powers(int which, const double* x, double* out, int n, hi
 hipError_t launch_zero_words(int* p, size_t n, hipStream_t st);
 // q (one buffer) and aux of slot src[k] to slot dst[k]
 hipError_t launch_move_slots(double* q, double* aux, const int* src, const int* dst, int n, int np2, hipStream_t st);
+// desc[first .. first + n).scen = scen
+hipError_t launch_set_scen(PatchDesc* desc, int first, int n, int scen, hipStream_t st);
 hipError_t launch_max(const double* v, int n, double* out, hipStream_t st);
 
 }  // namespace gf

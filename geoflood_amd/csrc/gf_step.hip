@@ -1875,6 +1875,14 @@ __global__ void __launch_bounds__(256) k_move_slots(double* __restrict__ q, doub
     for (int e = threadIdx.x; e < np2; e += blockDim.x) aux[d * np2 + e] = aux[s * np2 + e];
 }
 
+// The scenario of slots [first, first + n): the one field of their
+// descriptors that re-targeting a scenario changes.
+__global__ void __launch_bounds__(256) k_set_scen(PatchDesc* __restrict__ desc, int first, int n, int scen)
+{
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k < n) desc[first + k].scen = scen;
+}
+
 __global__ void __launch_bounds__(256) k_unpack(double* stage_q, const double* q, int first,
                                                 int n, int np2)
 {
@@ -2244,6 +2252,13 @@ hipError_t launch_move_slots(double* q, double* aux, const int* src, const int* 
 {
     if (n <= 0) return hipSuccess;
     k_move_slots<<<n, 256, 0, st>>>(q, aux, src, dst, n, np2);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_scen(PatchDesc* desc, int first, int n, int scen, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    k_set_scen<<<(n + 255) / 256, 256, 0, st>>>(desc, first, n, scen);
     return hipGetLastError();
 }
 
