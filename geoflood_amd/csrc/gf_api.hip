@@ -2261,14 +2261,22 @@ try {
     if (n > 0 && !level) return fail(GF_EINVAL, "level is NULL");
     for (int k = 0; k < n; k++)
         if (level[k] < 0 || level[k] >= kMaxLevels) return fail(GF_EINVAL, "slot %d: level %d not in [0, %d)", first + k, level[k], kMaxLevels);
+    const bool was_amr = p->amr;
     if (int r = ensure_amr(p)) return r;
-    for (int k = 0; k < n; k++) p->h_level[first + k] = level[k];
+    bool in_scenario = false;
+    for (int k = 0; k < n; k++) {
+        in_scenario |= p->h_desc[first + k].scen >= 0;
+        p->h_level[first + k] = level[k];
+    }
     const int nlev = 1 + *std::max_element(p->h_level.begin(), p->h_level.end());
     if (nlev != p->nlev) {
         p->nlev = nlev;
         p->graph_dirty = true;
     }
-    return rebuild_inflow(p);
+    // the hydrograph list holds the slots of scenarios, by level: levels of
+    // slots outside every scenario (a regrid's next half) leave it as it is,
+    // once the pool keeps it by level
+    return in_scenario || !was_amr ? rebuild_inflow(p) : GF_OK;
 } GF_CATCH_STATUS
 
 int gf_pool_set_cf_links(gf_pool* p, int n, const int* coarse, const int* fine, const int* side, const int* igrid)
